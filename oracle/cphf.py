@@ -1,0 +1,111 @@
+"""Oracle: dimensionless CPHF driver pieces (RHF).  Test infrastructure only.
+
+Restates src/hessian/rscf.rs (ajz34/rstsr-showcase-hessian):
+  compute_dimless_cphf_rhs  104-182 (final rhs assembly 173-178)
+  response_mo               210-220
+  response_dimless_cphf     237-263
+  solve_dimless_cphf        278-300
+  finalize_cphf             328-353
+  get_cphf_hess             371-399
+and the configuration defaults of HessSCFConfig (rscf.rs:14-18).
+"""
+import numpy as np
+
+from . import krylov
+
+DEFAULTS = dict(level_shift=0.0, cphf_tol=1e-8, cphf_max_cycle=42, cphf_max_space=14, cphf_lindep=1e-14)
+
+
+def split(mo_occ, mo_energy):
+    occ = np.asarray(mo_occ) > 0
+    nocc = int(occ.sum())
+    eocc = np.asarray(mo_energy)[occ]
+    evir = np.asarray(mo_energy)[~occ]
+    return nocc, eocc, evir
+
+
+def dimless_rhs(f1mo, s1mo, mo_occ, mo_energy, level_shift=0.0):
+    """rscf.rs:170-178: rhs[vir] = -(f1mo - s1mo e_occ)/(e_a - e_i + shift); rhs[occ] = -s1mo/2."""
+    nocc, eocc, evir = split(mo_occ, mo_energy)
+    e_ai = evir[:, None] - eocc[None, :] + level_shift
+    tail = (1,) * (f1mo.ndim - 2)
+    b1 = f1mo - s1mo * eocc.reshape((1, -1) + tail)
+    rhs = np.zeros_like(f1mo)
+    rhs[nocc:] = -b1[nocc:] / e_ai.reshape(e_ai.shape + tail)
+    rhs[:nocc] = -0.5 * s1mo[:nocc]
+    return rhs
+
+
+def response_mo(mo_coeff, response_bras, mo1):
+    """rscf.rs:210-220: resp = sum_el C^T el.get_response_bra(C mo1)."""
+    shape = mo1.shape
+    m = mo1.reshape(shape[0], shape[1], -1)
+    ubra = np.einsum("um,mia->uia", mo_coeff, m)
+    resp = np.zeros_like(m)
+    for rb in response_bras:
+        resp += np.einsum("um,uia->mia", mo_coeff, rb(ubra).reshape(ubra.shape))
+    return resp.reshape(shape)
+
+
+def response_dimless(mo_coeff, mo_occ, mo_energy, response_bras, mo1, level_shift=0.0):
+    """rscf.rs:237-263: resp[vir] = (resp - shift mo1)[vir] / (e_ai + shift); resp[occ] = 0."""
+    nocc, eocc, evir = split(mo_occ, mo_energy)
+    e_ai = (evir[:, None] - eocc[None, :] + level_shift)
+    resp = response_mo(mo_coeff, response_bras, mo1)
+    if level_shift != 0.0:
+        resp = resp - level_shift * mo1
+    tail = (1,) * (mo1.ndim - 2)
+    resp[nocc:] /= e_ai.reshape(e_ai.shape + tail)
+    resp[:nocc] = 0.0
+    return resp
+
+
+def solve_dimless(mo_coeff, mo_occ, mo_energy, response_bras, rhs, cfg=None, stats=None):
+    """rscf.rs:278-300: Krylov on the flattened [nmo*nocc, nprop] system."""
+    cfg = dict(DEFAULTS, **(cfg or {}))
+    shape = rhs.shape
+    nmo, nocc = shape[:2]
+
+    def aop(x):
+        y = response_dimless(mo_coeff, mo_occ, mo_energy, response_bras,
+                             x.reshape(nmo, nocc, -1), cfg["level_shift"])
+        return y.reshape(nmo * nocc, -1)
+
+    x = krylov.krylov_block(aop, rhs.reshape(nmo * nocc, -1), None, cfg["cphf_tol"],
+                            cfg["cphf_max_cycle"], cfg["cphf_max_space"], cfg["cphf_lindep"], stats=stats)
+    return x.reshape(shape)
+
+
+def finalize(mo_coeff, mo_occ, mo_energy, response_bras, f1mo, s1mo, mo1):
+    """rscf.rs:328-353: one more response, mo1[vir] = -b1mo[vir]/e_ai, mo_e1 = b1mo[occ] + mo1[occ] e_ij."""
+    nocc, eocc, evir = split(mo_occ, mo_energy)
+    tail = (1,) * (mo1.ndim - 2)
+    e_ai = (evir[:, None] - eocc[None, :]).reshape((len(evir), nocc) + tail)
+    e_ij = (eocc[:, None] - eocc[None, :]).reshape((nocc, nocc) + tail)
+    b1 = f1mo - s1mo * eocc.reshape((1, -1) + tail) + response_mo(mo_coeff, response_bras, mo1)
+    mo1 = mo1.copy()
+    mo1[nocc:] = -b1[nocc:] / e_ai
+    mo_e1 = b1[:nocc] + mo1[:nocc] * e_ij
+    return mo1, mo_e1
+
+
+def cphf_hess(f1mo, s1mo, mo1, mo_e1, mo_occ, mo_energy):
+    """rscf.rs:371-399: de[s,t,B,A] (B <= A, then mirrored with the 3x3 transpose).
+
+    f1mo, s1mo, mo1: [nmo, nocc, 3, natm]; mo_e1: [nocc, nocc, 3, natm].
+    """
+    nocc, eocc, _ = split(mo_occ, mo_energy)
+    natm = f1mo.shape[3]
+    de = np.zeros((3, 3, natm, natm))
+    s1oo = s1mo[:nocc]
+    for A in range(natm):
+        for B in range(A + 1):
+            # f1mo[.., .., None, .., A] * mo1[.., .., .., None, B] -> [nmo, nocc, 3(s from mo1), 3(t from f1mo)]
+            # summed over axes 0,1 gives de_BA[s, t] with s <-> mo1's xyz, t <-> f1mo's xyz.
+            blk = 4.0 * np.einsum("mit,mis->st", f1mo[:, :, :, A], mo1[:, :, :, B])
+            blk -= 4.0 * np.einsum("mit,mis,i->st", s1mo[:, :, :, A], mo1[:, :, :, B], eocc)
+            blk -= 2.0 * np.einsum("jit,jis->st", s1oo[:, :, :, A], mo_e1[:, :, :, B])
+            de[:, :, B, A] += blk
+        for B in range(A):
+            de[:, :, A, B] += de[:, :, B, A].T
+    return de

@@ -1,0 +1,84 @@
+// RI-JK response component (RHF nspin = 1, UHF nspin = 2).
+//
+// Mirrors RHessRIJK / UHessRIJK's response side (src/ri_jk/hess_r.rs:1737-1954,
+// src/ri_jk/hess_u.rs:38-316) and the separated core get_rijk_response_bra_separated
+// (hess_r.rs:82-186).
+#pragma once
+#include <memory>
+
+#include "gemm.hpp"
+
+namespace hh {
+
+class Rijk {
+ public:
+  ~Rijk();
+  Rijk(const double* cderi, int64_t nao, int64_t naux, double scale_j, double scale_k, int nspin, hipStream_t s);
+
+  void prepare_response(const double* const* mo_coeff, const double* const* mo_occ, const int64_t* nmo);
+  // resp[s] = scale_j * f_s * J_ao mocc_s + scale_k * K_s  (f = 1 RHF, 0.5 UHF)
+  void get_response_bra(const double* const* bra, int64_t nprop, double* const* resp, Prec prec);
+  // j_ao (unscaled) and k_bra[s] (unscaled, sign included); either may be null.
+  void response_bra_separated(const double* const* bra, int64_t nprop, double* j_ao, double* const* k_bra, Prec prec);
+
+  int64_t nao() const { return nao_; }
+  int64_t naux() const { return naux_; }
+  int nspin() const { return nspin_; }
+  int64_t nocc(int s) const { return nocc_[s]; }
+  bool prepared() const { return prepared_; }
+  hipStream_t stream() const { return stream_; }
+  double scale_j() const { return scale_j_; }
+  double scale_k() const { return scale_k_; }
+
+  // Aux-chunk size for the K contraction (0 = choose from the size of the intermediate).
+  int64_t k_chunk = 0;
+
+  // Stage timing with HIP events on the handle's stream (the reference keeps wall-clock
+  // `timing` vectors per component, hess_r.rs:1746).  Indices of the accumulated times (ms):
+  enum { T_K_HOT = 0, T_K_TOTAL = 1, T_J_TOTAL = 2, T_RESPONSE = 3, T_COUNT = 4 };
+  void set_profiling(bool on);
+  // times[T_*] in ms and, at index T_COUNT, the number of hot-kernel launches
+  void get_timing(double* out) const;
+  void reset_timing();
+
+ private:
+  // J: j_ao [nao, nao, nprop] from total dm1 (sum over spins); written unscaled.
+  void build_j_ao(const double* const* bra, int64_t nprop, Prec prec, double* j_ao);
+  // K: out_s[nao, nocc_s, nprop] = alpha * (K_s without its minus sign) + beta * out_s
+  void build_k(int s, const double* bra, int64_t nprop, Prec prec, double alpha, double beta, double* out);
+  void ensure_mixed();
+  hipEvent_t ev(int i);
+  int mark();                     // record a fresh event now, return its index
+  void span(int key, int begin);  // accumulate (begin event .. now) into times_[key]
+  void flush_timing();
+
+  bool profiling_ = false;
+  std::vector<hipEvent_t> events_;
+  std::vector<std::pair<int, std::pair<int, int>>> pending_;  // (key, (begin event, end event))
+  int next_event_ = 0;
+  double times_[T_COUNT] = {0, 0, 0, 0};
+  int64_t hot_launches_ = 0;
+
+  int64_t nao_, naux_, nao_tp_;
+  double scale_j_, scale_k_;
+  int nspin_;
+  hipStream_t stream_;
+  bool prepared_ = false;
+  bool mixed_ready_ = false;
+  int64_t nocc_[2] = {0, 0};
+
+  DevBuf cderi_;      // [nao_tp, naux] f64
+  DevBuf bfull_;      // [nao, nao, naux] f64 (unpacked)
+  DevBuf cderi32_;    // f32 copies for the mixed-precision modes
+  DevBuf bfull32_;
+  DevBuf mocc_[2];    // [nao, nocc] f64
+  DevBuf mocc32_[2];
+  DevBuf x_[2];       // X[mu, i, P] = (B_P C_o)[mu, i]
+  DevBuf x32_[2];
+  DevBuf o_[2];       // O[j, i, P] = (C_o^T B_P C_o)[j, i]
+  DevBuf o32_[2];
+  // per-call scratch
+  DevBuf w_, t_, bra32_, dm_, dmtp_, itm_, jtp_, jao_, ws_;
+};
+
+}  // namespace hh

@@ -1,0 +1,11 @@
+"""hesship: host-side binding of libhesship (MI355X / gfx950) for the analytic-Hessian hot path.
+
+Mirrors the reference's (ajz34/rstsr-showcase-hessian) Rust API names:
+  RHessRIJK, UHessRIJK, get_rijk_response_bra_separated     (src/ri_jk)
+  krylov_block                                               (src/util/krylov_block.rs)
+Device tensors are torch tensors on a ROCm device, in the reference's logical shapes; the
+binding hands column-major device buffers to the C ABI.
+"""
+from ._lib import PREC, HesshipError, KrylovNotConverged, lib  # noqa: F401
+from .tensors import fortran, empty_f  # noqa: F401
+from .rijk import RHessRIJK, UHessRIJK, get_rijk_response_bra_separated, pack_triu_tilde, unpack_triu  # noqa: F401

@@ -1,0 +1,89 @@
+"""ctypes binding of libhesship.so (the C ABI declared in include/hesship.h).
+
+The library is loaded from this directory.  There is no fallback: if the library is missing
+or fails to load, every entry point raises.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhesship.so")
+
+HH_OK = 0
+HH_ERR_INVALID = 1
+HH_ERR_HIP = 2
+HH_ERR_NOT_CONVERGED = 3
+HH_ERR_NOMEM = 4
+HH_ERR_STATE = 5
+HH_ERR_INTERNAL = 6
+
+PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+
+_p = C.c_void_p
+_i64 = C.c_int64
+_i32 = C.c_int32
+_d = C.c_double
+_pp = C.POINTER(C.c_void_p)
+_pi64 = C.POINTER(C.c_int64)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "hh_last_error": (C.c_char_p, []),
+    "hh_version": (C.c_char_p, []),
+    "hh_pack_triu_tilde": (_i32, [_p, _i64, _i64, _p, _p]),
+    "hh_unpack_triu": (_i32, [_p, _i64, _i64, _p, _p]),
+    "hh_rijk_create": (_i32, [_p, _i64, _i64, _d, _d, _i32, _p, C.POINTER(C.c_void_p)]),
+    "hh_rijk_destroy": (_i32, [_p]),
+    "hh_rijk_prepare_response": (_i32, [_p, _pp, _pp, _pi64]),
+    "hh_rijk_nocc": (_i32, [_p, _i32, _pi64]),
+    "hh_rijk_get_response_bra": (_i32, [_p, _pp, _i64, _pp, _i32]),
+    "hh_rijk_response_bra_separated": (_i32, [_p, _pp, _i64, _p, _pp, _i32]),
+    "hh_rijk_set_aux_chunk": (_i32, [_p, _i64]),
+    "hh_rijk_set_profiling": (_i32, [_p, _i32]),
+    "hh_rijk_get_timing": (_i32, [_p, C.POINTER(C.c_double)]),
+    "hh_rijk_reset_timing": (_i32, [_p]),
+    "hh_util_gemm": (_i32, [_i64, _i64, _i64, _d, _p, _i64, _i32, _p, _i64, _i32, _d, _p, _i64, _i64, _i64,
+                            _pi64, _i32, _p]),
+}
+
+
+class HesshipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[hh status {code}] {msg}")
+        self.code = code
+
+
+class KrylovNotConverged(HesshipError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"libhesship.so not built at {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(code):
+    if code != HH_OK:
+        msg = lib().hh_last_error().decode(errors="replace")
+        if code == HH_ERR_NOT_CONVERGED:
+            raise KrylovNotConverged(code, msg)
+        raise HesshipError(code, msg)
+
+
+def ptr_array(ptrs):
+    arr = (C.c_void_p * len(ptrs))()
+    for i, p in enumerate(ptrs):
+        arr[i] = p
+    return arr

@@ -1,0 +1,155 @@
+"""RI-JK response components (reference: src/ri_jk/hess_r.rs, src/ri_jk/hess_u.rs).
+
+Shapes (reference, column-major):  cderi [nao_tp, naux];  mo_coeff [nao, nmo];  mo_occ [nmo];
+bra [nao, nocc, ...] (trailing dims are the perturbations, collectively nprop).
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from ._lib import PREC, check, lib, ptr_array
+from .tensors import empty_f, fortran, stream_of
+
+
+def _host_occ(mo_occ):
+    if isinstance(mo_occ, torch.Tensor):
+        mo_occ = mo_occ.detach().cpu().numpy()
+    return np.ascontiguousarray(np.asarray(mo_occ, dtype=np.float64))
+
+
+def pack_triu_tilde(dm):
+    """density_matrices.rs:61-77: [nao, nao, ...] -> [nao_tp, ...], off-diagonals x2."""
+    nao = dm.shape[0]
+    tail = tuple(dm.shape[2:])
+    nmat = int(np.prod(tail)) if tail else 1
+    d = fortran(dm.reshape(nao, nao, nmat))
+    out = empty_f((nao * (nao + 1) // 2, nmat), dm.device)
+    check(lib().hh_pack_triu_tilde(d.data_ptr(), nao, nmat, out.data_ptr(), stream_of(dm.device)))
+    return out.reshape((nao * (nao + 1) // 2,) + tail)
+
+
+def unpack_triu(tp, nao):
+    """rstsr unpack_tri(Upper, Sy): [nao_tp, ...] -> [nao, nao, ...]."""
+    tail = tuple(tp.shape[1:])
+    nmat = int(np.prod(tail)) if tail else 1
+    t = fortran(tp.reshape(tp.shape[0], nmat))
+    out = empty_f((nao, nao, nmat), tp.device)
+    check(lib().hh_unpack_triu(t.data_ptr(), nao, nmat, out.data_ptr(), stream_of(tp.device)))
+    return out.reshape((nao, nao) + tail)
+
+
+class _RIJKBase:
+    nspin = 1
+
+    def __init__(self, cderi, scale_j, scale_k, nao=None):
+        nao_tp, naux = cderi.shape
+        if nao is None:
+            nao = int(round((np.sqrt(8 * nao_tp + 1) - 1) / 2))
+        assert nao * (nao + 1) // 2 == nao_tp, "cderi first dim must be nao*(nao+1)/2"
+        self.nao, self.naux = nao, naux
+        self.scale_j, self.scale_k = float(scale_j), float(scale_k)
+        self.device = cderi.device
+        cd = fortran(cderi)
+        h = C.c_void_p()
+        check(lib().hh_rijk_create(cd.data_ptr(), nao, naux, self.scale_j, self.scale_k, self.nspin,
+                                   stream_of(self.device), C.byref(h)))
+        self._h = h
+        self.nocc = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                lib().hh_rijk_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def set_aux_chunk(self, n):
+        check(lib().hh_rijk_set_aux_chunk(self._h, int(n)))
+
+    def set_profiling(self, on=True):
+        check(lib().hh_rijk_set_profiling(self._h, int(bool(on))))
+
+    def timing(self):
+        """dict of accumulated stage times (ms) and hot-kernel launch count."""
+        out = (C.c_double * 5)()
+        check(lib().hh_rijk_get_timing(self._h, out))
+        return dict(k_hot_ms=out[0], k_total_ms=out[1], j_total_ms=out[2], response_ms=out[3],
+                    k_hot_launches=int(out[4]))
+
+    def reset_timing(self):
+        check(lib().hh_rijk_reset_timing(self._h))
+
+    def _prepare(self, mo_coeff, mo_occ):
+        cs = [fortran(c) for c in mo_coeff]
+        occ = [_host_occ(o) for o in mo_occ]
+        nmo = (C.c_int64 * self.nspin)(*[c.shape[1] for c in cs])
+        check(lib().hh_rijk_prepare_response(self._h, ptr_array([c.data_ptr() for c in cs]),
+                                             ptr_array([o.ctypes.data for o in occ]), nmo))
+        self.nocc = [int((o > 0).sum()) for o in occ]
+        self._keep = (cs, occ)
+
+    def _bras(self, bra):
+        out, shapes = [], []
+        for s, b in enumerate(bra):
+            shapes.append(tuple(b.shape))
+            out.append(fortran(b.reshape(self.nao, self.nocc[s], -1)))
+        nprop = out[0].shape[2]
+        assert all(o.shape[2] == nprop for o in out), "bra trailing dim (nprop) must agree across spins"
+        return out, shapes, nprop
+
+    def _response(self, bra, precision):
+        bras, shapes, nprop = self._bras(bra)
+        outs = [empty_f((self.nao, self.nocc[s], nprop), self.device) for s in range(self.nspin)]
+        check(lib().hh_rijk_get_response_bra(self._h, ptr_array([b.data_ptr() for b in bras]), nprop,
+                                             ptr_array([o.data_ptr() for o in outs]), PREC[precision]))
+        return [o.reshape(sh) for o, sh in zip(outs, shapes)]
+
+    def response_bra_separated(self, bra, do_j=True, do_k=True, precision="fp64"):
+        bras, shapes, nprop = self._bras(bra)
+        j_ao = empty_f((self.nao, self.nao, nprop), self.device) if do_j else None
+        ks = [empty_f((self.nao, self.nocc[s], nprop), self.device) for s in range(self.nspin)] if do_k else []
+        check(lib().hh_rijk_response_bra_separated(
+            self._h, ptr_array([b.data_ptr() for b in bras]), nprop,
+            j_ao.data_ptr() if do_j else None,
+            ptr_array([k.data_ptr() for k in ks]) if do_k else None, PREC[precision]))
+        return j_ao, [k.reshape((self.nao,) + sh[1:]) for k, sh in zip(ks, shapes)]
+
+
+class RHessRIJK(_RIJKBase):
+    """RHessRIJK (hess_r.rs:1737-1954), response side, built from an existing cderi
+    (new_with_cderi, hess_r.rs:1775-1794)."""
+    nspin = 1
+
+    def make_response_preparation(self, mo_coeff, mo_occ):
+        self._prepare([mo_coeff], [mo_occ])
+
+    def get_response_bra(self, bra, precision="fp64"):
+        return self._response([bra], precision)[0]
+
+
+class UHessRIJK(_RIJKBase):
+    """UHessRIJK (hess_u.rs:38-316), response side."""
+    nspin = 2
+
+    def make_response_preparation(self, mo_coeff, mo_occ):
+        self._prepare(list(mo_coeff), list(mo_occ))
+
+    def get_response_bra(self, bra, precision="fp64"):
+        return self._response(list(bra), precision)
+
+
+def get_rijk_response_bra_separated(cderi, mo_coeff, mo_occ, bra, do_j, do_k, nbatch_aux=72, precision="fp64"):
+    """hess_r.rs:82-186.  Returns (j_ao [nao,nao,nprop] or None, [k_bra_s]).
+
+    `nbatch_aux` is accepted for signature parity; the GPU path chooses its own aux chunking."""
+    nset = len(mo_coeff)
+    cls = RHessRIJK if nset == 1 else UHessRIJK
+    obj = cls(cderi, 1.0, 1.0)
+    obj._prepare(list(mo_coeff), list(mo_occ))
+    j_ao, ks = obj.response_bra_separated(list(bra), do_j, do_k, precision)
+    if j_ao is not None:
+        j_ao = j_ao.reshape((obj.nao, obj.nao) + tuple(bra[0].shape[2:]))
+    return j_ao, ks
