@@ -94,6 +94,122 @@ hh_status hh_rijk_get_timing(const hh_rijk* h, double* out5);
 hh_status hh_rijk_reset_timing(hh_rijk* h);
 
 /* ---------------------------------------------------------------------------------------- */
+/* Numerical-integration grid contractions (src/numint_matmul)                               */
+
+typedef struct hh_numint hh_numint;
+/* XCDenType (src/xceff/flags.rs:19-24) */
+#define HH_DEN_RHO 0
+#define HH_DEN_SIGMA 1
+#define HH_DEN_TAU 2
+
+/* Grid handle from device AO values ao [ng, nao, ncomp] (column-major; component 0 = value,
+ * 1..3 = gradient, as NIMatmul's cached AO, nimatmul.rs:105-126) and weights [ng]; both are
+ * copied.  (The reference's NIMatmul::new evaluates AOs itself with libcint, nimatmul.rs:41-99;
+ * AO evaluation is outside this library.) */
+hh_status hh_numint_create(const double* ao, int64_t ng, int64_t nao, int64_t ncomp, const double* weights,
+                           void* stream, hh_numint** out);
+hh_status hh_numint_destroy(hh_numint* h);
+/* Grid points per batch (the reference's nbatch, nimatmul.rs:18-36); 0 = size from memory. */
+hh_status hh_numint_set_batch(hh_numint* h, int64_t nbatch);
+/* rho1 [ng, nvar, nprop] from bra [nao, nocc] and kets [nao, nocc, nprop]
+ * (make_rho_from_one_bra_mult_ket, nimatmul.rs:196-229 / pure_eval_rho.rs:216-334). */
+hh_status hh_numint_rho1_one_bra_mult_ket(hh_numint* h, int32_t den_type, const double* bra, int64_t nocc,
+                                          const double* kets, int64_t nprop, double* rho1, hh_precision precision);
+/* out [nao, nocc, nprop] = make_rks_fxc_pot_with_eff_bra_trans(fxc_eff [ng, nvar, nvar], rho1, bra)
+ * (nimatmul.rs:361-389 / pure_xcpot.rs:370-465). */
+hh_status hh_numint_fxc_pot_bra_trans(hh_numint* h, int32_t den_type, const double* fxc_eff, const double* rho1,
+                                      int64_t nprop, const double* bra, int64_t nocc, double* out,
+                                      hh_precision precision);
+/* out [nao, nocc, nprop] = get_rks_response_bra (hess_rks.rs:570-605, x4 closed-shell factor),
+ * batched over the grid (get_rks_response_bra_batched, hess_rks.rs:752-797). */
+hh_status hh_numint_rks_response_bra(hh_numint* h, int32_t den_type, const double* fxc_eff, const double* mo1_bra,
+                                     int64_t nprop, const double* mocc, int64_t nocc, double* out,
+                                     hh_precision precision);
+
+/* XC effective kernels from LibXC-variable derivatives, unpolarised (transform_xc_inner,
+ * xc_deriv.rs:305-428): rho [ng, >= nvar] (ld_rho), xc_val [ng, ncol] (ld_xc) with columns
+ * [exc | 1st derivatives | unique 2nd derivatives]; order 1 -> out [ng, nvar] (vxc_eff),
+ * order 2 -> out [ng, nvar, nvar] (fxc_eff).  RHO: out = the order's raw column. */
+hh_status hh_xceff_transform(int32_t den_type, int32_t order, const double* rho, int64_t ld_rho, const double* xc_val,
+                             int64_t ld_xc, int64_t ng, double* out, void* stream);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Block Krylov solver for (I + A) x = b (src/util/krylov_block.rs)                         */
+
+typedef struct {
+  int32_t cycles;       /* trial-block aop calls (the reference's total_cycles) */
+  int32_t restarts;     /* hard restarts */
+  int32_t aop_calls;    /* all aop calls, including residual evaluations */
+  int32_t refinements;  /* FP64 refinement sweeps (CPHF mixed precision; 0 otherwise) */
+  double max_innerprod; /* last max ||v||^2 of the new trial block */
+  double residual;      /* final max column 2-norm of the FP64 residual (CPHF mixed precision) */
+} hh_krylov_stats;
+
+/* ax[n, nblock] = A x[n, nblock]; device pointers, column-major; must enqueue its work on the
+ * stream passed to hh_krylov_block (or finish before returning).  Non-zero return aborts. */
+typedef int32_t (*hh_aop_fn)(void* ctx, const double* x, int64_t n, int64_t nblock, double* ax);
+
+/* x[n, nset] solving (I + A) x = b (replaces krylov_block, krylov_block.rs:39-197; same
+ * convergence test, restart rule and defaults).  x0 may be NULL.  Returns HH_ERR_NOT_CONVERGED
+ * where the reference panics (x then holds the last approximation).  Synchronous. */
+hh_status hh_krylov_block(hh_aop_fn aop, void* ctx, const double* b, const double* x0, int64_t n, int64_t nset,
+                          double tol, int32_t max_cycle, int32_t max_space, double lindep, double* x,
+                          hh_krylov_stats* stats, void* stream);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Dimensionless CPHF driver, restricted SCF (src/hessian/rscf.rs)                           */
+
+typedef struct hh_cphf hh_cphf;
+
+/* mo_coeff: device [nao, nmo]; mo_occ, mo_energy: HOST [nmo] (occupied orbitals first).
+ * Config as HessSCFConfig (rscf.rs:5-18): level_shift, cphf_tol, cphf_max_cycle,
+ * cphf_max_space, cphf_lindep. */
+hh_status hh_cphf_create(const double* mo_coeff, int64_t nao, int64_t nmo, const double* mo_occ,
+                         const double* mo_energy, double level_shift, double tol, int32_t max_cycle,
+                         int32_t max_space, double lindep, void* stream, hh_cphf** out);
+hh_status hh_cphf_destroy(hh_cphf* h);
+/* Add an RI-JK component (must be RHF, prepared with the same MOs) to the response list
+ * (the reference's el_list, rscf.rs:28).  The component must outlive the driver. */
+hh_status hh_cphf_add_rijk(hh_cphf* h, hh_rijk* rijk);
+/* Add the DFT grid response (RHessKSNIMatmul::get_response_bra, hess_rks.rs:886-909) with
+ * fxc_eff [ng, nvar, nvar] (device; must outlive the driver) to the response list. */
+hh_status hh_cphf_add_numint(hh_cphf* h, hh_numint* ni, int32_t den_type, const double* fxc_eff);
+/* rhs [nmo, nocc, nprop] from f1mo, s1mo [nmo, nocc, nprop] (compute_dimless_cphf_rhs,
+ * rscf.rs:170-178). */
+hh_status hh_cphf_dimless_rhs(hh_cphf* h, const double* f1mo, const double* s1mo, int64_t nprop, double* rhs);
+/* resp = C^T sum_el get_response_bra(C mo1) (response_mo, rscf.rs:210-220). */
+hh_status hh_cphf_response_mo(hh_cphf* h, const double* mo1, int64_t nprop, double* resp, hh_precision precision);
+/* response_dimless_cphf, rscf.rs:237-263. */
+hh_status hh_cphf_response_dimless(hh_cphf* h, const double* mo1, int64_t nprop, double* resp,
+                                   hh_precision precision);
+/* solve_dimless_cphf, rscf.rs:278-300.  precision FP64: the reference algorithm.  FP32 /
+ * BF16X3: Krylov on the reduced-precision response inside FP64 iterative refinement, stopping
+ * when every column's FP64 residual 2-norm is below cphf_tol.  diag (device [nmo, nocc], may be
+ * NULL): right diagonal preconditioner on the virtual rows (x = diag * y). */
+hh_status hh_cphf_solve_dimless(hh_cphf* h, const double* rhs, int64_t nprop, double* mo1, hh_precision precision,
+                                const double* diag, hh_krylov_stats* stats);
+/* finalize_cphf, rscf.rs:328-353: mo1_out [nmo, nocc, nprop], mo_e1 [nocc, nocc, nprop]. */
+hh_status hh_cphf_finalize(hh_cphf* h, const double* f1mo, const double* s1mo, const double* mo1, int64_t nprop,
+                           double* mo1_out, double* mo_e1);
+/* get_cphf_hess, rscf.rs:371-399: de [3, 3, natm, natm] from [.., .., 3, natm] inputs. */
+hh_status hh_cphf_hess(hh_cphf* h, const double* f1mo, const double* s1mo, const double* mo1, const double* mo_e1,
+                       int64_t natm, double* de);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Dense FP64 linear algebra on the device                                                    */
+
+/* Solve H X = G in place (H overwritten by LU factors, G by X); replaces solve_general as
+ * used by projected_solve (krylov_block.rs:214).  HH_ERR_INVALID if singular.  Synchronous. */
+hh_status hh_linalg_solve(double* H, int64_t nd, double* G, int64_t nrhs, void* stream);
+/* Lower Cholesky in place (decomp_j2c_cd, pure_decompose.rs:107-133).  HH_ERR_INVALID if not
+ * positive definite.  Synchronous. */
+hh_status hh_linalg_cholesky(double* H, int64_t nd, void* stream);
+/* Triangular solve with lower L [nd, nd]: side_right = 0: X[nd, n2] <- L^-1 X;
+ * side_right = 1: X[n2, nd] <- X L^-T (solve_by_j2c side Right, pure_decompose.rs:179-380). */
+hh_status hh_linalg_trsm_lower(const double* L, int64_t nd, double* X, int64_t n2, int64_t ldx, int32_t side_right,
+                               void* stream);
+
+/* ---------------------------------------------------------------------------------------- */
 /* Utility: the library's strided batched GEMM (no reference counterpart; exposed so callers  */
 /* and tests can drive the matrix-core kernel directly).                                      */
 /* C[m,n] = alpha sum_k A[m,k] B[k,n] + beta C[m,n]; A[m,k] at a_kmajor ? m*lda+k : m+k*lda;   */
@@ -102,6 +218,10 @@ hh_status hh_util_gemm(int64_t M, int64_t N, int64_t K, double alpha, const doub
                        int32_t a_kmajor, const double* B, int64_t ldb, int32_t b_kmajor, double beta, double* C,
                        int64_t ldc, int64_t nb0, int64_t nb1, const int64_t* strides6, hh_precision precision,
                        void* stream);
+
+/* Device-to-device copy of `bytes` on `stream` (no reference counterpart; lets foreign
+ * callers of hh_krylov_block move data in and out of the solver's buffers). */
+hh_status hh_util_copy(void* dst, const void* src, int64_t bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,6 +1,9 @@
 // extern "C" boundary of libhesship: argument checks, exception -> status translation.
 #include "elementwise.hpp"
 #include "gemm.hpp"
+#include "cphf.hpp"
+#include "linalg.hpp"
+#include "numint.hpp"
 #include "rijk.hpp"
 
 namespace hh {
@@ -12,6 +15,52 @@ void set_last_error(const std::string& msg) { g_last_error = msg; }
 
 struct hh_rijk {
   std::unique_ptr<hh::Rijk> impl;
+};
+
+namespace {
+// RHF RI-JK as a CPHF electron-interaction component
+struct RijkResponse : hh::ElecResponse {
+  hh::Rijk* r;
+  explicit RijkResponse(hh::Rijk* x) : r(x) {}
+  void response_bra(const double* bra, int64_t nprop, double* out, bool accumulate, hh::Prec prec) override {
+    const double* b[1] = {bra};
+    double* o[1] = {out};
+    r->get_response_bra(b, nprop, o, prec, accumulate);
+  }
+};
+
+// RKS grid response as a CPHF component: mocc = the driver's first nocc MO columns
+struct NumintResponse : hh::ElecResponse {
+  hh::Numint* ni;
+  int den;
+  const double* fxc;
+  const double* mocc;
+  int64_t nocc;
+  NumintResponse(hh::Numint* n, int d, const double* f, const double* m, int64_t no)
+      : ni(n), den(d), fxc(f), mocc(m), nocc(no) {}
+  void response_bra(const double* bra, int64_t nprop, double* out, bool accumulate, hh::Prec prec) override {
+    ni->rks_response_bra(den, fxc, bra, nprop, mocc, nocc, out, prec, accumulate);
+  }
+};
+
+void fill_stats(hh_krylov_stats* out, const hh::KrylovStats& k, int refinements, double residual) {
+  if (!out) return;
+  out->cycles = k.cycles;
+  out->restarts = k.restarts;
+  out->aop_calls = k.aop_calls;
+  out->refinements = refinements;
+  out->max_innerprod = k.max_innerprod;
+  out->residual = residual;
+}
+}  // namespace
+
+struct hh_numint {
+  std::unique_ptr<hh::Numint> impl;
+};
+
+struct hh_cphf {
+  std::unique_ptr<hh::Cphf> impl;
+  std::vector<std::unique_ptr<hh::ElecResponse>> comps;
 };
 
 using namespace hh;
@@ -120,6 +169,231 @@ hh_status hh_rijk_reset_timing(hh_rijk* h) {
   HH_ABI_END
 }
 
+hh_status hh_krylov_block(hh_aop_fn aop, void* ctx, const double* b, const double* x0, int64_t n, int64_t nset,
+                          double tol, int32_t max_cycle, int32_t max_space, double lindep, double* x,
+                          hh_krylov_stats* stats, void* stream) {
+  hh::KrylovStats ks;
+  try {
+    HH_REQUIRE(aop && b && x, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    hh::Aop f = [&](const double* xv, int64_t nb, double* axv) {
+      const int32_t rc = aop(ctx, xv, n, nb, axv);
+      if (rc != 0) throw hh::Error(HH_ERR_INVALID, "aop callback returned " + std::to_string(rc));
+    };
+    hh::krylov_block(f, b, x0, n, nset, tol, max_cycle, max_space, lindep, x, &ks, s);
+    fill_stats(stats, ks, 0, 0.0);
+    return HH_OK;
+  } catch (const hh::Error& e) {
+    fill_stats(stats, ks, 0, 0.0);
+    hh::set_last_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    hh::set_last_error(e.what());
+    return HH_ERR_INTERNAL;
+  }
+}
+
+hh_status hh_cphf_create(const double* mo_coeff, int64_t nao, int64_t nmo, const double* mo_occ,
+                         const double* mo_energy, double level_shift, double tol, int32_t max_cycle,
+                         int32_t max_space, double lindep, void* stream, hh_cphf** out) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(out != nullptr, "out is null");
+  *out = nullptr;
+  CphfConfig cfg;
+  cfg.level_shift = level_shift;
+  cfg.tol = tol;
+  cfg.max_cycle = max_cycle;
+  cfg.max_space = max_space;
+  cfg.lindep = lindep;
+  auto h = std::make_unique<hh_cphf>();
+  h->impl = std::make_unique<Cphf>(mo_coeff, nao, nmo, mo_occ, mo_energy, cfg, (hipStream_t)stream);
+  *out = h.release();
+  HH_ABI_END
+}
+
+hh_status hh_cphf_destroy(hh_cphf* h) {
+  HH_ABI_BEGIN
+  if (h) {
+    HH_HIP(hipStreamSynchronize(h->impl->stream()));
+    delete h;
+  }
+  HH_ABI_END
+}
+
+hh_status hh_cphf_add_rijk(hh_cphf* h, hh_rijk* rijk) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && rijk, "null handle");
+  HH_REQUIRE(rijk->impl->nspin() == 1, "RHF CPHF needs an RHF (nspin = 1) RI-JK component");
+  if (!rijk->impl->prepared()) throw Error(HH_ERR_STATE, "RI-JK component not prepared (make_response_preparation)");
+  HH_REQUIRE(rijk->impl->nocc(0) == h->impl->nocc(), "RI-JK component occupation differs from the driver's");
+  h->comps.push_back(std::make_unique<RijkResponse>(rijk->impl.get()));
+  h->impl->add_component(h->comps.back().get());
+  HH_ABI_END
+}
+
+hh_status hh_cphf_add_numint(hh_cphf* h, hh_numint* ni, int32_t den_type, const double* fxc_eff) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && ni && fxc_eff, "null argument");
+  HH_REQUIRE(den_type >= HH_DEN_RHO && den_type <= HH_DEN_TAU, "den_type must be RHO, SIGMA or TAU");
+  HH_REQUIRE(ni->impl->nao() == h->impl->nao(), "numint nao differs from the driver's");
+  h->comps.push_back(std::make_unique<NumintResponse>(ni->impl.get(), den_type, fxc_eff, h->impl->mo_coeff(),
+                                                      h->impl->nocc()));
+  h->impl->add_component(h->comps.back().get());
+  HH_ABI_END
+}
+
+hh_status hh_numint_create(const double* ao, int64_t ng, int64_t nao, int64_t ncomp, const double* weights,
+                           void* stream, hh_numint** out) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(out != nullptr, "out is null");
+  *out = nullptr;
+  auto h = std::make_unique<hh_numint>();
+  h->impl = std::make_unique<Numint>(ao, ng, nao, ncomp, weights, (hipStream_t)stream);
+  *out = h.release();
+  HH_ABI_END
+}
+
+hh_status hh_numint_destroy(hh_numint* h) {
+  HH_ABI_BEGIN
+  if (h) {
+    HH_HIP(hipStreamSynchronize(h->impl->stream()));
+    delete h;
+  }
+  HH_ABI_END
+}
+
+hh_status hh_numint_set_batch(hh_numint* h, int64_t nbatch) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && nbatch >= 0, "null handle or negative batch");
+  h->impl->batch = nbatch;
+  HH_ABI_END
+}
+
+hh_status hh_numint_rho1_one_bra_mult_ket(hh_numint* h, int32_t den_type, const double* bra, int64_t nocc,
+                                          const double* kets, int64_t nprop, double* rho1, hh_precision precision) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && bra && kets && rho1 && nocc > 0 && nprop > 0, "null argument or empty size");
+  HH_REQUIRE(precision >= HH_PREC_FP64 && precision <= HH_PREC_BF16X3, "unknown precision");
+  h->impl->rho1_one_bra_mult_ket(den_type, bra, nocc, kets, nprop, rho1, (Prec)precision);
+  HH_ABI_END
+}
+
+hh_status hh_numint_fxc_pot_bra_trans(hh_numint* h, int32_t den_type, const double* fxc_eff, const double* rho1,
+                                      int64_t nprop, const double* bra, int64_t nocc, double* out,
+                                      hh_precision precision) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && fxc_eff && rho1 && bra && out && nocc > 0 && nprop > 0, "null argument or empty size");
+  HH_REQUIRE(precision >= HH_PREC_FP64 && precision <= HH_PREC_BF16X3, "unknown precision");
+  h->impl->fxc_pot_bra_trans(den_type, fxc_eff, rho1, nprop, bra, nocc, out, (Prec)precision);
+  HH_ABI_END
+}
+
+hh_status hh_numint_rks_response_bra(hh_numint* h, int32_t den_type, const double* fxc_eff, const double* mo1_bra,
+                                     int64_t nprop, const double* mocc, int64_t nocc, double* out,
+                                     hh_precision precision) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && fxc_eff && mo1_bra && mocc && out && nocc > 0 && nprop > 0, "null argument or empty size");
+  HH_REQUIRE(precision >= HH_PREC_FP64 && precision <= HH_PREC_BF16X3, "unknown precision");
+  h->impl->rks_response_bra(den_type, fxc_eff, mo1_bra, nprop, mocc, nocc, out, (Prec)precision);
+  HH_ABI_END
+}
+
+hh_status hh_xceff_transform(int32_t den_type, int32_t order, const double* rho, int64_t ld_rho, const double* xc_val,
+                             int64_t ld_xc, int64_t ng, double* out, void* stream) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(ng >= 0 && xc_val && out && (den_type == HH_DEN_RHO || rho), "null argument");
+  xceff_transform(den_type, order, rho, ld_rho, xc_val, ld_xc, ng, out, (hipStream_t)stream);
+  HH_ABI_END
+}
+
+hh_status hh_cphf_dimless_rhs(hh_cphf* h, const double* f1mo, const double* s1mo, int64_t nprop, double* rhs) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && f1mo && s1mo && rhs && nprop > 0, "null argument or nprop <= 0");
+  h->impl->dimless_rhs(f1mo, s1mo, nprop, rhs);
+  HH_ABI_END
+}
+
+hh_status hh_cphf_response_mo(hh_cphf* h, const double* mo1, int64_t nprop, double* resp, hh_precision precision) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && mo1 && resp && nprop > 0, "null argument or nprop <= 0");
+  HH_REQUIRE(precision >= HH_PREC_FP64 && precision <= HH_PREC_BF16X3, "unknown precision");
+  h->impl->response_mo(mo1, nprop, resp, (Prec)precision);
+  HH_ABI_END
+}
+
+hh_status hh_cphf_response_dimless(hh_cphf* h, const double* mo1, int64_t nprop, double* resp,
+                                   hh_precision precision) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && mo1 && resp && nprop > 0, "null argument or nprop <= 0");
+  HH_REQUIRE(precision >= HH_PREC_FP64 && precision <= HH_PREC_BF16X3, "unknown precision");
+  h->impl->response_dimless(mo1, nprop, resp, (Prec)precision);
+  HH_ABI_END
+}
+
+hh_status hh_cphf_solve_dimless(hh_cphf* h, const double* rhs, int64_t nprop, double* mo1, hh_precision precision,
+                                const double* diag, hh_krylov_stats* stats) {
+  CphfStats st;
+  try {
+    HH_REQUIRE(h && rhs && mo1 && nprop > 0, "null argument or nprop <= 0");
+    HH_REQUIRE(precision >= HH_PREC_FP64 && precision <= HH_PREC_BF16X3, "unknown precision");
+    h->impl->solve_dimless(rhs, nprop, mo1, (Prec)precision, &st, diag);
+    fill_stats(stats, st.krylov, st.refinements, st.residual);
+    return HH_OK;
+  } catch (const hh::Error& e) {
+    fill_stats(stats, st.krylov, st.refinements, st.residual);
+    hh::set_last_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    hh::set_last_error(e.what());
+    return HH_ERR_INTERNAL;
+  }
+}
+
+hh_status hh_cphf_finalize(hh_cphf* h, const double* f1mo, const double* s1mo, const double* mo1, int64_t nprop,
+                           double* mo1_out, double* mo_e1) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && f1mo && s1mo && mo1 && mo1_out && mo_e1 && nprop > 0, "null argument or nprop <= 0");
+  h->impl->finalize(f1mo, s1mo, mo1, nprop, mo1_out, mo_e1);
+  HH_ABI_END
+}
+
+hh_status hh_cphf_hess(hh_cphf* h, const double* f1mo, const double* s1mo, const double* mo1, const double* mo_e1,
+                       int64_t natm, double* de) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && f1mo && s1mo && mo1 && mo_e1 && de && natm > 0, "null argument or natm <= 0");
+  h->impl->cphf_hess(f1mo, s1mo, mo1, mo_e1, natm, de);
+  HH_ABI_END
+}
+
+hh_status hh_linalg_solve(double* H, int64_t nd, double* G, int64_t nrhs, void* stream) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(nd >= 0 && nrhs >= 0 && (nd == 0 || (H && G)), "invalid argument");
+  DevBuf ws;
+  lu_solve(H, nd, G, nrhs, (hipStream_t)stream, ws);
+  HH_HIP(hipStreamSynchronize((hipStream_t)stream));
+  HH_ABI_END
+}
+
+hh_status hh_linalg_cholesky(double* H, int64_t nd, void* stream) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(nd >= 0 && (nd == 0 || H), "invalid argument");
+  DevBuf ws;
+  const bool ok = cholesky(H, nd, (hipStream_t)stream, ws);
+  HH_HIP(hipStreamSynchronize((hipStream_t)stream));
+  HH_REQUIRE(ok, "cholesky: matrix is not positive definite");
+  HH_ABI_END
+}
+
+hh_status hh_linalg_trsm_lower(const double* L, int64_t nd, double* X, int64_t n2, int64_t ldx, int32_t side_right,
+                               void* stream) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(nd >= 0 && n2 >= 0 && ldx >= (side_right ? n2 : nd), "invalid sizes");
+  DevBuf ws;
+  trsm_lower(L, nd, X, n2, ldx, side_right != 0, (hipStream_t)stream, ws);
+  HH_HIP(hipStreamSynchronize((hipStream_t)stream));
+  HH_ABI_END
+}
+
 hh_status hh_util_gemm(int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda, int32_t a_kmajor,
                        const double* B, int64_t ldb, int32_t b_kmajor, double beta, double* C, int64_t ldc,
                        int64_t nb0, int64_t nb1, const int64_t* strides6, hh_precision precision, void* stream) {
@@ -140,6 +414,13 @@ hh_status hh_util_gemm(int64_t M, int64_t N, int64_t K, double alpha, const doub
     g.sB1 = strides6[3]; g.sC0 = strides6[4]; g.sC1 = strides6[5];
   }
   gemm<double, double>(g, (Prec)precision, (hipStream_t)stream, *ws);
+  HH_ABI_END
+}
+
+hh_status hh_util_copy(void* dst, const void* src, int64_t bytes, void* stream) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(bytes >= 0 && (bytes == 0 || (dst && src)), "invalid argument");
+  if (bytes > 0) HH_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
   HH_ABI_END
 }
 

@@ -1,0 +1,22 @@
+// Dense FP64 linear algebra on the GPU: blocked LU with partial pivoting and triangular solves.
+#pragma once
+#include "gemm.hpp"
+
+namespace hh {
+
+// Solve H X = G in place: H [nd, nd] (column-major, ld = nd) is overwritten by its LU factors,
+// G [nd, nrhs] (ld = nd) by X.  Throws Error(HH_ERR_INVALID) if H is singular (zero pivot).
+// Replaces rstsr linalg solve_general (LAPACK getrf/getrs) as used by krylov_block.rs:214.
+void lu_solve(double* H, int64_t nd, double* G, int64_t nrhs, hipStream_t s, DevBuf& ws);
+
+// Cholesky H = L L^T in place (lower, column-major, ld = nd); returns false if not positive
+// definite (a pivot <= 0 was met).
+bool cholesky(double* H, int64_t nd, hipStream_t s, DevBuf& ws);
+
+// In-place triangular solves with the lower Cholesky factor L [nd, nd]:
+//   side_right = false:  X <- L^-1 X     (X [nd, ncol], ld = ldx)
+//   side_right = true :  X <- X L^-T     (X [nrow, nd], ld = ldx)
+void trsm_lower(const double* L, int64_t nd, double* X, int64_t nrow_or_ncol, int64_t ldx, bool side_right,
+                hipStream_t s, DevBuf& ws);
+
+}  // namespace hh

@@ -1,0 +1,49 @@
+// Numerical-integration grid contractions for the CPKS response (reference:
+// src/numint_matmul/{nimatmul.rs, pure_eval_rho.rs, pure_xcpot.rs, hess_rks.rs}).
+#pragma once
+#include "gemm.hpp"
+
+namespace hh {
+
+// XCDenType (src/xceff/flags.rs:19-24): RHO 0, SIGMA 1, TAU 2.
+enum DenType : int { DEN_RHO = 0, DEN_SIGMA = 1, DEN_TAU = 2 };
+inline int den_nvar(int d) { return d == DEN_RHO ? 1 : (d == DEN_SIGMA ? 4 : 5); }
+inline int den_ncomp(int d) { return d == DEN_RHO ? 1 : 4; }
+
+class Numint {
+ public:
+  // ao [ng, nao, ncomp] (column-major, AO values and first derivatives, as NIMatmul's cached
+  // AO, nimatmul.rs:105-126) and weights [ng] are copied to the handle.
+  Numint(const double* ao, int64_t ng, int64_t nao, int64_t ncomp, const double* weights, hipStream_t s);
+
+  // rho1[ng, nvar, nprop] = make_rho_from_one_bra_mult_ket(bra, kets) (pure_eval_rho.rs:216-334)
+  void rho1_one_bra_mult_ket(int den, const double* bra, int64_t nocc, const double* kets, int64_t nprop, double* rho1,
+                             Prec prec);
+  // out[nao, nocc, nprop] = make_rks_fxc_pot_with_eff_bra_trans (pure_xcpot.rs:370-465)
+  void fxc_pot_bra_trans(int den, const double* fxc_eff, const double* rho1, int64_t nprop, const double* bra,
+                         int64_t nocc, double* out, Prec prec, double alpha = 1.0, bool accumulate = false);
+  // resp = 4 fxc_pot_bra_trans(rho1(mocc, mo1_bra)) (get_rks_response_bra, hess_rks.rs:570-605),
+  // fused per grid batch so rho1 never leaves the batch.
+  void rks_response_bra(int den, const double* fxc_eff, const double* mo1_bra, int64_t nprop, const double* mocc,
+                        int64_t nocc, double* out, Prec prec, bool accumulate = false);
+
+  int64_t ng() const { return ng_; }
+  int64_t nao() const { return nao_; }
+  hipStream_t stream() const { return stream_; }
+  int64_t batch = 0;  // grid points per batch (0 = sized from memory)
+
+ private:
+  int64_t pick_batch(int64_t ncol) const;
+  int64_t ng_, nao_, ncomp_;
+  hipStream_t stream_;
+  DevBuf ao_, w_, aob_, akb_, rho_, ws_;
+};
+
+// LibXC-variable derivatives -> rho_t-based effective kernels, unpolarised (reference:
+// src/xceff/xc_deriv.rs:305-428, transform_xc_inner).  rho [ng, >= nvar], xc_val [ng, ncol]
+// with columns [exc | 1st (xlen) | 2nd (xlen(xlen+1)/2)], xlen = 2 (SIGMA) / 3 (TAU).
+// order 1: out [ng, nvar];  order 2: out [ng, nvar, nvar].  RHO: out = the raw derivative block.
+void xceff_transform(int den, int order, const double* rho, int64_t ld_rho, const double* xc_val, int64_t ld_xc,
+                     int64_t ng, double* out, hipStream_t s);
+
+}  // namespace hh

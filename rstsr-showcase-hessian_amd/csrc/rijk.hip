@@ -293,7 +293,8 @@ void Rijk::response_bra_separated(const double* const* bra, int64_t nprop, doubl
   }
 }
 
-void Rijk::get_response_bra(const double* const* bra, int64_t nprop, double* const* resp, Prec prec) {
+void Rijk::get_response_bra(const double* const* bra, int64_t nprop, double* const* resp, Prec prec,
+                            bool accumulate) {
   if (!prepared_) throw Error(HH_ERR_STATE, "response requested before prepare_response");
   HH_REQUIRE(bra != nullptr && resp != nullptr && nprop > 0, "null argument or nprop <= 0");
   for (int s = 0; s < nspin_; ++s) HH_REQUIRE(bra[s] && resp[s], "bra[s] / resp[s] is null");
@@ -302,7 +303,7 @@ void Rijk::get_response_bra(const double* const* bra, int64_t nprop, double* con
   if (do_k)
     for (int s = 0; s < nspin_; ++s) {
       const int a = profiling_ ? mark() : -1;
-      build_k(s, bra[s], nprop, prec, -scale_k_, 0.0, resp[s]);
+      build_k(s, bra[s], nprop, prec, -scale_k_, accumulate ? 1.0 : 0.0, resp[s]);
       if (profiling_) span(T_K_TOTAL, a);
     }
   const int aj = (profiling_ && do_j) ? mark() : -1;
@@ -318,12 +319,12 @@ void Rijk::get_response_bra(const double* const* bra, int64_t nprop, double* con
       g.C = resp[s]; g.ldc = nao_; g.sC0 = nao_ * nocc_[s];
       g.nb0 = nprop;
       g.alpha = scale_j_ * f;
-      g.beta = do_k ? 1.0 : 0.0;
+      g.beta = (do_k || accumulate) ? 1.0 : 0.0;
       gemm<double, double>(g, PREC_F64, stream_, ws_);
     }
   }
   if (profiling_ && do_j) span(T_J_TOTAL, aj);
-  if (!do_j && !do_k)
+  if (!do_j && !do_k && !accumulate)
     for (int s = 0; s < nspin_; ++s) fill<double>(resp[s], nao_ * nocc_[s] * nprop, 0.0, stream_);
   if (profiling_) {
     span(T_RESPONSE, r0);

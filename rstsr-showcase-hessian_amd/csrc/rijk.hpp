@@ -17,7 +17,8 @@ class Rijk {
 
   void prepare_response(const double* const* mo_coeff, const double* const* mo_occ, const int64_t* nmo);
   // resp[s] = scale_j * f_s * J_ao mocc_s + scale_k * K_s  (f = 1 RHF, 0.5 UHF)
-  void get_response_bra(const double* const* bra, int64_t nprop, double* const* resp, Prec prec);
+  void get_response_bra(const double* const* bra, int64_t nprop, double* const* resp, Prec prec,
+                        bool accumulate = false);
   // j_ao (unscaled) and k_bra[s] (unscaled, sign included); either may be null.
   void response_bra_separated(const double* const* bra, int64_t nprop, double* j_ao, double* const* k_bra, Prec prec);
 

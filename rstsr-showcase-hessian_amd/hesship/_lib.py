@@ -47,6 +47,41 @@ SIGNATURES = {
 }
 
 
+class KrylovStats(C.Structure):
+    _fields_ = [("cycles", C.c_int32), ("restarts", C.c_int32), ("aop_calls", C.c_int32),
+                ("refinements", C.c_int32), ("max_innerprod", C.c_double), ("residual", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+AOP_FN = C.CFUNCTYPE(C.c_int32, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p)
+SIGNATURES.update({
+    "hh_krylov_block": (_i32, [AOP_FN, _p, _p, _p, _i64, _i64, _d, _i32, _i32, _d, _p, C.POINTER(KrylovStats), _p]),
+    "hh_cphf_create": (_i32, [_p, _i64, _i64, _p, _p, _d, _d, _i32, _i32, _d, _p, C.POINTER(C.c_void_p)]),
+    "hh_cphf_destroy": (_i32, [_p]),
+    "hh_cphf_add_rijk": (_i32, [_p, _p]),
+    "hh_cphf_dimless_rhs": (_i32, [_p, _p, _p, _i64, _p]),
+    "hh_cphf_response_mo": (_i32, [_p, _p, _i64, _p, _i32]),
+    "hh_cphf_response_dimless": (_i32, [_p, _p, _i64, _p, _i32]),
+    "hh_cphf_solve_dimless": (_i32, [_p, _p, _i64, _p, _i32, _p, C.POINTER(KrylovStats)]),
+    "hh_cphf_finalize": (_i32, [_p, _p, _p, _p, _i64, _p, _p]),
+    "hh_cphf_hess": (_i32, [_p, _p, _p, _p, _p, _i64, _p]),
+    "hh_linalg_solve": (_i32, [_p, _i64, _p, _i64, _p]),
+    "hh_linalg_cholesky": (_i32, [_p, _i64, _p]),
+    "hh_linalg_trsm_lower": (_i32, [_p, _i64, _p, _i64, _i64, _i32, _p]),
+    "hh_util_copy": (_i32, [_p, _p, _i64, _p]),
+    "hh_cphf_add_numint": (_i32, [_p, _p, _i32, _p]),
+    "hh_numint_create": (_i32, [_p, _i64, _i64, _i64, _p, _p, C.POINTER(C.c_void_p)]),
+    "hh_numint_destroy": (_i32, [_p]),
+    "hh_numint_set_batch": (_i32, [_p, _i64]),
+    "hh_numint_rho1_one_bra_mult_ket": (_i32, [_p, _i32, _p, _i64, _p, _i64, _p, _i32]),
+    "hh_numint_fxc_pot_bra_trans": (_i32, [_p, _i32, _p, _p, _i64, _p, _i64, _p, _i32]),
+    "hh_numint_rks_response_bra": (_i32, [_p, _i32, _p, _p, _i64, _p, _i64, _p, _i32]),
+    "hh_xceff_transform": (_i32, [_i32, _i32, _p, _i64, _p, _i64, _i64, _p, _p]),
+})
+
+
 class HesshipError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"[hh status {code}] {msg}")
