@@ -58,6 +58,14 @@ typedef struct hh_rijk hh_rijk;
  * hess_r.rs:1775-1794) or 2 (UHF, UHessRIJK, hess_u.rs:38-146). */
 hh_status hh_rijk_create(const double* cderi, int64_t nao, int64_t naux, double scale_j, double scale_k,
                          int32_t nspin, void* stream, hh_rijk** out);
+/* The same from the 3-centre integrals: j3c [nao_tp, naux] (packed upper, column-major) and the
+ * metric j2c [naux, naux] (device).  j2c = L L^T by Cholesky on the device in FP64, then
+ * cderi = j3c L^-T (replaces generate_cderi_with_decomp, hess_r.rs:1725-1735, with
+ * J2CDecompOption {Cd, threshold, ..}, decomp_j2c_cd pure_decompose.rs:28-55).  A factor with a
+ * diagonal element <= threshold gives HH_ERR_INVALID: the reference's eigenvalue-floor fallback
+ * is not provided. */
+hh_status hh_rijk_create_from_j3c(const double* j3c, const double* j2c, int64_t nao, int64_t naux, double threshold,
+                                  double scale_j, double scale_k, int32_t nspin, void* stream, hh_rijk** out);
 hh_status hh_rijk_destroy(hh_rijk* h);
 
 /* Store MO coefficients and occupations for the response (replaces make_response_preparation,
@@ -81,9 +89,11 @@ hh_status hh_rijk_get_response_bra(hh_rijk* h, const double* const* bra, int64_t
 hh_status hh_rijk_response_bra_separated(hh_rijk* h, const double* const* bra, int64_t nprop, double* j_ao,
                                          double* const* k_bra, hh_precision precision);
 
-/* Aux-index chunk of the K contraction (the reference's nbatch_aux, hard-coded 72 at
- * hess_r.rs:1941 "TODO tunable"); 0 = size it from the intermediate's memory. */
-hh_status hh_rijk_set_aux_chunk(hh_rijk* h, int64_t naux_chunk);
+/* Perturbations per K-contraction batch (the GPU counterpart of the reference's memory knob
+ * nbatch_aux, hard-coded 72 at hess_r.rs:1941 "TODO tunable": here the whole aux range is one
+ * contraction and the intermediate is bounded by batching perturbations); 0 = size it from
+ * memory. */
+hh_status hh_rijk_set_batch(hh_rijk* h, int64_t nprop_batch);
 
 /* Stage timing with HIP events on the handle's stream (the reference keeps per-stage wall
  * times in `timing`, hess_r.rs:1746).  When on, each response call waits for its own events.

@@ -97,6 +97,18 @@ hh_status hh_rijk_create(const double* cderi, int64_t nao, int64_t naux, double 
   HH_ABI_END
 }
 
+hh_status hh_rijk_create_from_j3c(const double* j3c, const double* j2c, int64_t nao, int64_t naux, double threshold,
+                                  double scale_j, double scale_k, int32_t nspin, void* stream, hh_rijk** out) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(out != nullptr, "out is null");
+  *out = nullptr;
+  HH_REQUIRE(nspin == 1 || nspin == 2, "nspin must be 1 (RHF) or 2 (UHF)");
+  auto h = std::make_unique<hh_rijk>();
+  h->impl = Rijk::from_j3c(j3c, j2c, nao, naux, threshold, scale_j, scale_k, nspin, (hipStream_t)stream);
+  *out = h.release();
+  HH_ABI_END
+}
+
 hh_status hh_rijk_destroy(hh_rijk* h) {
   HH_ABI_BEGIN
   if (h) {
@@ -141,10 +153,10 @@ hh_status hh_rijk_response_bra_separated(hh_rijk* h, const double* const* bra, i
   HH_ABI_END
 }
 
-hh_status hh_rijk_set_aux_chunk(hh_rijk* h, int64_t naux_chunk) {
+hh_status hh_rijk_set_batch(hh_rijk* h, int64_t nprop_batch) {
   HH_ABI_BEGIN
-  HH_REQUIRE(h != nullptr && naux_chunk >= 0, "null handle or negative chunk");
-  h->impl->k_chunk = naux_chunk;
+  HH_REQUIRE(h != nullptr && nprop_batch >= 0, "null handle or negative batch");
+  h->impl->k_chunk = nprop_batch;
   HH_ABI_END
 }
 

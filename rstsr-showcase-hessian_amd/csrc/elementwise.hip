@@ -94,5 +94,6 @@ template void cast<double, float>(const double*, int64_t, float*, hipStream_t);
 template void cast<float, double>(const float*, int64_t, double*, hipStream_t);
 template void fill<double>(double*, int64_t, double, hipStream_t);
 template void fill<float>(float*, int64_t, float, hipStream_t);
+template void fill<unsigned short>(unsigned short*, int64_t, unsigned short, hipStream_t);
 
 }  // namespace hh

@@ -10,6 +10,9 @@
 //   32-deep (bf16)   : A frag = A[l%16][8*(l/16) .. +7],  B frag = B[8*(l/16) .. +7][l%16]
 //   result f32/bf16  : acc[r] = D[4*(l/16) + r][l%16],  r = 0..3
 //   result f64       : acc[r] = D[(l/16) + 4*r][l%16]   (rows interleaved; measured on gfx950)
+// A row fragment and a B column fragment have the same lane layout, so the kernel issues
+// mfma(B-frag, A-frag) and accumulates C^T blocks: lane l then holds C[m = l%16][n = ...], and
+// the column-major C (m contiguous) is stored by 16 consecutive lanes at consecutive addresses.
 #include "gemm.hpp"
 
 namespace hh {
@@ -33,6 +36,11 @@ struct GemmParams {
   int64_t kchunk;  // K elements per split (multiple of BK)
   int ksplit;
   void* partial;   // [ksplit][nbatch][N][M] accumulator-typed partial sums (ksplit > 1)
+  int64_t K1;      // second operand pair for k >= K1 (K1 >= K: none)
+  const void* A2;
+  const void* B2;
+  int64_t lda2, ldb2, sA2_0, sA2_1, sB2_0, sB2_1;
+  void* C_lo;      // non-null: write C as a bf16 hi (C) / lo (C_lo) pair of u16 arrays
 };
 
 template <int PREC> struct PrecTraits;
@@ -91,6 +99,9 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmParams p) {
 
   const TIn* __restrict__ A = static_cast<const TIn*>(p.A) + b0 * p.sA0 + b1 * p.sA1;
   const TIn* __restrict__ B = static_cast<const TIn*>(p.B) + b0 * p.sB0 + b1 * p.sB1;
+  const bool dual = p.K1 < p.K;
+  const TIn* __restrict__ A2 = dual ? static_cast<const TIn*>(p.A2) + b0 * p.sA2_0 + b1 * p.sA2_1 : nullptr;
+  const TIn* __restrict__ B2 = dual ? static_cast<const TIn*>(p.B2) + b0 * p.sB2_0 + b1 * p.sB2_1 : nullptr;
 
   vacc_t acc[FM][FN];
 #pragma unroll
@@ -108,7 +119,13 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmParams p) {
       const int kk = AK ? idx % BK : idx / BM;
       const int64_t gm = m0 + mm, gk = k0 + kk;
       ct v = 0;
-      if (gm < p.M && gk < kend) v = (ct)(AK ? A[gm * p.lda + gk] : A[gm + gk * p.lda]);
+      if (gm < p.M && gk < kend) {
+        if (!dual || gk < p.K1) v = (ct)(AK ? A[gm * p.lda + gk] : A[gm + gk * p.lda]);
+        else {
+          const int64_t k2 = gk - p.K1;
+          v = (ct)(AK ? A2[gm * p.lda2 + k2] : A2[gm + k2 * p.lda2]);
+        }
+      }
       ra[e] = v;
     }
 #pragma unroll
@@ -118,7 +135,13 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmParams p) {
       const int kk = BKM ? idx % BK : idx / BN;
       const int64_t gn = n0 + nn, gk = k0 + kk;
       ct v = 0;
-      if (gn < p.N && gk < kend) v = (ct)(BKM ? B[gk + gn * p.ldb] : B[gk * p.ldb + gn]);
+      if (gn < p.N && gk < kend) {
+        if (!dual || gk < p.K1) v = (ct)(BKM ? B[gk + gn * p.ldb] : B[gk * p.ldb + gn]);
+        else {
+          const int64_t k2 = gk - p.K1;
+          v = (ct)(BKM ? B2[k2 + gn * p.ldb2] : B2[k2 * p.ldb2 + gn]);
+        }
+      }
       rb[e] = v;
     }
   };
@@ -186,9 +209,9 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmParams p) {
 #pragma unroll
           for (int j = 0; j < FN; ++j) {
             if constexpr (PREC == PREC_F64)
-              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bfr[j], acc[i][j], 0, 0, 0);
+              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bfr[j], af[i], acc[i][j], 0, 0, 0);
             else
-              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i], bfr[j], acc[i][j], 0, 0, 0);
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(bfr[j], af[i], acc[i][j], 0, 0, 0);
           }
       }
     } else {
@@ -216,9 +239,9 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmParams p) {
         for (int i = 0; i < FM; ++i)
 #pragma unroll
           for (int j = 0; j < FN; ++j) {
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al[i], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
           }
       }
     }
@@ -235,10 +258,10 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmParams p) {
     }
   }
 
-  // epilogue
-  const int64_t ncol0 = n0 + wn * WN + (lane & 15);
-  constexpr int RSTRIDE = (PREC == PREC_F64) ? 4 : 1;  // row step between acc[r] entries
-  const int64_t mrow0 = m0 + wm * WM + ((PREC == PREC_F64) ? (lane >> 4) : 4 * (lane >> 4));
+  // epilogue: acc[i][j][r] = C[mrow0 + 16 i][ncol0 + 16 j + RSTRIDE r]  (transposed blocks)
+  const int64_t mrow0 = m0 + wm * WM + (lane & 15);
+  constexpr int RSTRIDE = (PREC == PREC_F64) ? 4 : 1;  // column step between acc[r] entries
+  const int64_t ncol0 = n0 + wn * WN + ((PREC == PREC_F64) ? (lane >> 4) : 4 * (lane >> 4));
   if (p.ksplit > 1) {
     const int64_t nbatch = (int64_t)gridDim.y;
     acc_t* P = static_cast<acc_t*>(p.partial) + ((int64_t)kz * nbatch + bidx) * p.M * p.N;
@@ -248,7 +271,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmParams p) {
       for (int j = 0; j < FN; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int64_t gm = mrow0 + i * 16 + r * RSTRIDE, gn = ncol0 + j * 16;
+          const int64_t gm = mrow0 + i * 16, gn = ncol0 + j * 16 + r * RSTRIDE;
           if (gm < p.M && gn < p.N) P[gm + gn * p.M] = acc[i][j][r];
         }
   } else {
@@ -259,12 +282,22 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmParams p) {
       for (int j = 0; j < FN; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int64_t gm = mrow0 + i * 16 + r * RSTRIDE, gn = ncol0 + j * 16;
+          const int64_t gm = mrow0 + i * 16, gn = ncol0 + j * 16 + r * RSTRIDE;
           if (gm < p.M && gn < p.N) {
             double v = p.alpha * (double)acc[i][j][r];
-            TOut* c = C + gm + gn * p.ldc;
-            if (p.beta != 0.0) v += p.beta * (double)(*c);
-            *c = (TOut)v;
+            const int64_t off = gm + gn * p.ldc;
+            if (p.C_lo) {
+              // split output: hi = bf16(v), lo = bf16(v - hi) into two u16 arrays (beta ignored)
+              const float f = (float)v;
+              const __bf16 h = (__bf16)f;
+              const __bf16 l = (__bf16)(f - (float)h);
+              static_cast<unsigned short*>(p.C)[off + (b0 * p.sC0 + b1 * p.sC1)] = __builtin_bit_cast(unsigned short, h);
+              static_cast<unsigned short*>(p.C_lo)[off + (b0 * p.sC0 + b1 * p.sC1)] = __builtin_bit_cast(unsigned short, l);
+            } else {
+              TOut* c = C + off;
+              if (p.beta != 0.0) v += p.beta * (double)(*c);
+              *c = (TOut)v;
+            }
           }
         }
   }
@@ -312,6 +345,12 @@ void launch_cfg(const GemmArgs& g, hipStream_t stream, DevBuf& ws) {
   p.alpha = g.alpha; p.beta = g.beta;
   p.nb0 = g.nb0;
   p.sA0 = g.sA0; p.sA1 = g.sA1; p.sB0 = g.sB0; p.sB1 = g.sB1; p.sC0 = g.sC0; p.sC1 = g.sC1;
+  p.K1 = (g.K1 >= 0 && g.K1 < g.K) ? g.K1 : g.K;
+  p.A2 = g.A2; p.B2 = g.B2; p.lda2 = g.lda2; p.ldb2 = g.ldb2;
+  p.sA2_0 = g.sA2_0; p.sA2_1 = g.sA2_1; p.sB2_0 = g.sB2_0; p.sB2_1 = g.sB2_1;
+  if (p.K1 < p.K) HH_REQUIRE(g.A2 && g.B2, "gemm: second operand pair missing");
+  p.C_lo = g.C_lo;
+  if (g.C_lo) HH_REQUIRE(g.ksplit == 1, "gemm: split bf16 output needs ksplit = 1");
   p.tiles_m = ceil_div(g.M, BM);
   p.tiles_n = ceil_div(g.N, BN);
   const int64_t nbatch = g.nb0 * g.nb1;

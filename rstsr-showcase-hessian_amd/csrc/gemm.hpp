@@ -34,6 +34,16 @@ struct GemmArgs {
   int64_t nb0 = 1, nb1 = 1;
   int64_t sA0 = 0, sA1 = 0, sB0 = 0, sB1 = 0, sC0 = 0, sC1 = 0;
   int ksplit = 0;  // 0: choose automatically
+  // Optional second operand pair concatenated along K: for k >= K1 the product uses
+  // A2[m, k - K1] B2[k - K1, n] (same majorness as A / B; K is the total length).  K1 = -1: off.
+  int64_t K1 = -1;
+  const void* A2 = nullptr;
+  const void* B2 = nullptr;
+  int64_t lda2 = 0, ldb2 = 0;
+  int64_t sA2_0 = 0, sA2_1 = 0, sB2_0 = 0, sB2_1 = 0;
+  // Non-null: store C as two u16 arrays, C (bf16 hi) and C_lo (bf16 of the remainder), same
+  // indexing; beta must be 0 and ksplit 1.
+  void* C_lo = nullptr;
   // Optional timing events recorded on the launch stream: before the main kernel, after it
   // (before any split-K reduction) and after the reduction.
   hipEvent_t ev_begin = nullptr, ev_main_end = nullptr, ev_end = nullptr;

@@ -1,0 +1,436 @@
+// NT GEMM on packed K-contiguous operands for gfx950 (see gemm_nt.hpp).
+//
+// Workgroup: 512 lanes = 8 wave64 arranged WGM x WGN; each wave owns FM x FN MFMA blocks of
+// 16 x 16.  Per K step (BK = 32 for bf16, 16 for f64) the workgroup stages A[BM x BK] and
+// B[BN x BK] in LDS with 16-byte global loads into registers, double-buffered in LDS so one
+// barrier per step suffices.  LDS rows are padded by 16 bytes, which makes the 16-lane
+// fragment reads conflict-free.
+//
+// Block order is XCD-aware: workgroups are dealt round-robin to the 8 XCDs, so the linear id
+// is remapped to keep every XCD on one K split with all its (M, N) tiles; the A and B slices
+// of that split are then fetched from HBM about once per XCD and reused from its L2.
+#include "gemm_nt.hpp"
+
+namespace hh {
+
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef unsigned short u16;
+
+struct NtParams {
+  int64_t M, N, K;
+  const void* A;
+  const void* Alo;
+  int64_t lda;
+  const void* B;
+  const void* Blo;
+  int64_t ldb;
+  int64_t tiles_m, tiles_n;
+  int64_t kchunk;
+  int ksplit;
+  int64_t nwg;
+  double* partial;  // [ksplit][N][M]
+  double* C;
+  int64_t ldc;
+  double alpha, beta;
+};
+
+__device__ __forceinline__ int64_t remap_xcd(int64_t w, int64_t nwg) {
+  if (nwg % 8 != 0) return w;
+  return (w % 8) * (nwg / 8) + w / 8;
+}
+
+// ---------------------------------------------------------------- bf16 x3 ----
+template <int WGM, int WGN, int FM, int FN>
+__global__ __launch_bounds__(512) void nt_bf16x3_kernel(NtParams p) {
+  constexpr int BM = WGM * FM * 16, BN = WGN * FN * 16, BK = 32;
+  constexpr int LD = BK + 8;                 // bf16 elements per LDS row (80 B)
+  constexpr int A_ELEMS = BM * LD, B_ELEMS = BN * LD;
+  constexpr int STAGE = 2 * A_ELEMS + 2 * B_ELEMS;  // hi, lo for A and B
+  constexpr int A_CH = BM * (BK / 8), B_CH = BN * (BK / 8);  // 16-byte chunks per array
+  constexpr int A_PT = (A_CH + 511) / 512, B_PT = (B_CH + 511) / 512;
+  __shared__ __attribute__((aligned(16))) u16 smem[2 * STAGE];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WGM, wn = wave / WGM;
+  const int64_t lin = remap_xcd(blockIdx.x, p.nwg);
+  const int64_t tm = lin % p.tiles_m;
+  const int64_t tn = (lin / p.tiles_m) % p.tiles_n;
+  const int kz = (int)(lin / (p.tiles_m * p.tiles_n));
+  const int64_t m0 = tm * BM, n0 = tn * BN;
+  const int64_t kbeg = (int64_t)kz * p.kchunk;
+  const int64_t kend = min(p.K, kbeg + p.kchunk);
+  const u16* Ah = static_cast<const u16*>(p.A);
+  const u16* Al = static_cast<const u16*>(p.Alo);
+  const u16* Bh = static_cast<const u16*>(p.B);
+  const u16* Bl = static_cast<const u16*>(p.Blo);
+
+  v4f acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = v4f{0, 0, 0, 0};
+
+  uint4 ra_h[A_PT], ra_l[A_PT], rb_h[B_PT], rb_l[B_PT];
+  auto gload = [&](int64_t k0) {
+#pragma unroll
+    for (int e = 0; e < A_PT; ++e) {
+      const int c = tid + 512 * e;
+      uint4 vh = {0, 0, 0, 0}, vl = {0, 0, 0, 0};
+      if (c < A_CH) {
+        const int r = c / (BK / 8), kk = (c % (BK / 8)) * 8;
+        const int64_t gm = m0 + r;
+        if (gm < p.M) {
+          const int64_t off = gm * p.lda + k0 + kk;
+          vh = *reinterpret_cast<const uint4*>(Ah + off);
+          vl = *reinterpret_cast<const uint4*>(Al + off);
+        }
+      }
+      ra_h[e] = vh;
+      ra_l[e] = vl;
+    }
+#pragma unroll
+    for (int e = 0; e < B_PT; ++e) {
+      const int c = tid + 512 * e;
+      uint4 vh = {0, 0, 0, 0}, vl = {0, 0, 0, 0};
+      if (c < B_CH) {
+        const int r = c / (BK / 8), kk = (c % (BK / 8)) * 8;
+        const int64_t gn = n0 + r;
+        if (gn < p.N) {
+          const int64_t off = gn * p.ldb + k0 + kk;
+          vh = *reinterpret_cast<const uint4*>(Bh + off);
+          vl = *reinterpret_cast<const uint4*>(Bl + off);
+        }
+      }
+      rb_h[e] = vh;
+      rb_l[e] = vl;
+    }
+  };
+  auto sstore = [&](int buf) {
+    u16* s = smem + buf * STAGE;
+#pragma unroll
+    for (int e = 0; e < A_PT; ++e) {
+      const int c = tid + 512 * e;
+      if (c < A_CH) {
+        const int r = c / (BK / 8), kk = (c % (BK / 8)) * 8;
+        *reinterpret_cast<uint4*>(s + r * LD + kk) = ra_h[e];
+        *reinterpret_cast<uint4*>(s + A_ELEMS + r * LD + kk) = ra_l[e];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < B_PT; ++e) {
+      const int c = tid + 512 * e;
+      if (c < B_CH) {
+        const int r = c / (BK / 8), kk = (c % (BK / 8)) * 8;
+        *reinterpret_cast<uint4*>(s + 2 * A_ELEMS + r * LD + kk) = rb_h[e];
+        *reinterpret_cast<uint4*>(s + 2 * A_ELEMS + B_ELEMS + r * LD + kk) = rb_l[e];
+      }
+    }
+  };
+  auto compute = [&](int buf) {
+    const u16* s = smem + buf * STAGE;
+    const int kc = 8 * (lane >> 4);
+    v8bf bh[FN], bl[FN];
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int r = (wn * FN + j) * 16 + (lane & 15);
+      bh[j] = *reinterpret_cast<const v8bf*>(s + 2 * A_ELEMS + r * LD + kc);
+      bl[j] = *reinterpret_cast<const v8bf*>(s + 2 * A_ELEMS + B_ELEMS + r * LD + kc);
+    }
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+      const int r = (wm * FM + i) * 16 + (lane & 15);
+      const v8bf ah = *reinterpret_cast<const v8bf*>(s + r * LD + kc);
+      const v8bf al = *reinterpret_cast<const v8bf*>(s + A_ELEMS + r * LD + kc);
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
+      }
+    }
+  };
+
+  if (kbeg < kend) {
+    gload(kbeg);
+    sstore(0);
+    __syncthreads();
+    int buf = 0;
+    for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
+      const bool more = k0 + BK < kend;
+      if (more) gload(k0 + BK);
+      compute(buf);
+      if (more) sstore(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+    }
+  }
+  // epilogue: D[4*(l/16) + r][l%16]
+  const int64_t col0 = n0 + wn * FN * 16 + (lane & 15);
+  const int64_t row0 = m0 + wm * FM * 16 + 4 * (lane >> 4);
+  double* P = p.partial + (int64_t)kz * p.M * p.N;
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gm = row0 + i * 16 + r, gn = col0 + j * 16;
+        if (gm < p.M && gn < p.N) {
+          if (p.ksplit > 1) {
+            P[gm + gn * p.M] = (double)acc[i][j][r];
+          } else {
+            double* c = p.C + gm + gn * p.ldc;
+            double v = p.alpha * (double)acc[i][j][r];
+            if (p.beta != 0.0) v += p.beta * (*c);
+            *c = v;
+          }
+        }
+      }
+}
+
+// ------------------------------------------------------------------- f64 ----
+template <int WGM, int WGN, int FM, int FN>
+__global__ __launch_bounds__(512) void nt_f64_kernel(NtParams p) {
+  constexpr int BM = WGM * FM * 16, BN = WGN * FN * 16, BK = 16;
+  constexpr int LD = BK + 2;  // doubles per LDS row (144 B)
+  constexpr int A_ELEMS = BM * LD, B_ELEMS = BN * LD;
+  constexpr int STAGE = A_ELEMS + B_ELEMS;
+  constexpr int A_CH = BM * (BK / 2), B_CH = BN * (BK / 2);  // 16-byte chunks (2 doubles)
+  constexpr int A_PT = (A_CH + 511) / 512, B_PT = (B_CH + 511) / 512;
+  __shared__ __attribute__((aligned(16))) double smem[2 * STAGE];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WGM, wn = wave / WGM;
+  const int64_t lin = remap_xcd(blockIdx.x, p.nwg);
+  const int64_t tm = lin % p.tiles_m;
+  const int64_t tn = (lin / p.tiles_m) % p.tiles_n;
+  const int kz = (int)(lin / (p.tiles_m * p.tiles_n));
+  const int64_t m0 = tm * BM, n0 = tn * BN;
+  const int64_t kbeg = (int64_t)kz * p.kchunk;
+  const int64_t kend = min(p.K, kbeg + p.kchunk);
+  const double* A = static_cast<const double*>(p.A);
+  const double* B = static_cast<const double*>(p.B);
+
+  v4d acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = v4d{0, 0, 0, 0};
+
+  double2 ra[A_PT], rb[B_PT];
+  auto gload = [&](int64_t k0) {
+#pragma unroll
+    for (int e = 0; e < A_PT; ++e) {
+      const int c = tid + 512 * e;
+      double2 v = {0.0, 0.0};
+      if (c < A_CH) {
+        const int r = c / (BK / 2), kk = (c % (BK / 2)) * 2;
+        const int64_t gm = m0 + r;
+        if (gm < p.M) v = *reinterpret_cast<const double2*>(A + gm * p.lda + k0 + kk);
+      }
+      ra[e] = v;
+    }
+#pragma unroll
+    for (int e = 0; e < B_PT; ++e) {
+      const int c = tid + 512 * e;
+      double2 v = {0.0, 0.0};
+      if (c < B_CH) {
+        const int r = c / (BK / 2), kk = (c % (BK / 2)) * 2;
+        const int64_t gn = n0 + r;
+        if (gn < p.N) v = *reinterpret_cast<const double2*>(B + gn * p.ldb + k0 + kk);
+      }
+      rb[e] = v;
+    }
+  };
+  auto sstore = [&](int buf) {
+    double* s = smem + buf * STAGE;
+#pragma unroll
+    for (int e = 0; e < A_PT; ++e) {
+      const int c = tid + 512 * e;
+      if (c < A_CH) {
+        const int r = c / (BK / 2), kk = (c % (BK / 2)) * 2;
+        *reinterpret_cast<double2*>(s + r * LD + kk) = ra[e];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < B_PT; ++e) {
+      const int c = tid + 512 * e;
+      if (c < B_CH) {
+        const int r = c / (BK / 2), kk = (c % (BK / 2)) * 2;
+        *reinterpret_cast<double2*>(s + A_ELEMS + r * LD + kk) = rb[e];
+      }
+    }
+  };
+  auto compute = [&](int buf) {
+    const double* s = smem + buf * STAGE;
+#pragma unroll
+    for (int ks = 0; ks < BK / 4; ++ks) {
+      const int kc = ks * 4 + (lane >> 4);
+      double bf[FN];
+#pragma unroll
+      for (int j = 0; j < FN; ++j) bf[j] = s[A_ELEMS + ((wn * FN + j) * 16 + (lane & 15)) * LD + kc];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        const double af = s[((wm * FM + i) * 16 + (lane & 15)) * LD + kc];
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af, bf[j], acc[i][j], 0, 0, 0);
+      }
+    }
+  };
+
+  if (kbeg < kend) {
+    gload(kbeg);
+    sstore(0);
+    __syncthreads();
+    int buf = 0;
+    for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
+      const bool more = k0 + BK < kend;
+      if (more) gload(k0 + BK);
+      compute(buf);
+      if (more) sstore(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+    }
+  }
+  // epilogue: f64 result layout D[(l/16) + 4r][l%16]
+  const int64_t col0 = n0 + wn * FN * 16 + (lane & 15);
+  const int64_t row0 = m0 + wm * FM * 16 + (lane >> 4);
+  double* P = p.partial + (int64_t)kz * p.M * p.N;
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gm = row0 + i * 16 + 4 * r, gn = col0 + j * 16;
+        if (gm < p.M && gn < p.N) {
+          if (p.ksplit > 1) {
+            P[gm + gn * p.M] = acc[i][j][r];
+          } else {
+            double* c = p.C + gm + gn * p.ldc;
+            double v = p.alpha * acc[i][j][r];
+            if (p.beta != 0.0) v += p.beta * (*c);
+            *c = v;
+          }
+        }
+      }
+}
+
+__global__ __launch_bounds__(256) void nt_reduce_kernel(const double* __restrict__ partial, int ksplit, int64_t M,
+                                                        int64_t N, double* C, int64_t ldc, double alpha, double beta) {
+  const int64_t MN = M * N;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < MN; e += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int z = 0; z < ksplit; ++z) s += partial[(int64_t)z * MN + e];
+    const int64_t m = e % M, n = e / M;
+    double* c = C + m + n * ldc;
+    double v = alpha * s;
+    if (beta != 0.0) v += beta * (*c);
+    *c = v;
+  }
+}
+
+template <typename T>
+__global__ void split_bf16_rows_kernel(const T* __restrict__ in, int64_t rows, int64_t cols, int64_t ld_in,
+                                       u16* __restrict__ hi, u16* __restrict__ lo, int64_t ld_out) {
+  const int64_t total = rows * ld_out;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = e / ld_out, c = e % ld_out;
+    float x = 0.f;
+    if (c < cols) x = (float)in[r * ld_in + c];
+    const __bf16 h = (__bf16)x;
+    const __bf16 l = (__bf16)(x - (float)h);
+    hi[e] = __builtin_bit_cast(u16, h);
+    lo[e] = __builtin_bit_cast(u16, l);
+  }
+}
+
+int cu_count() {
+  static int n = 0;
+  if (n == 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+      n = 256;
+  }
+  return n;
+}
+
+template <int WGM, int WGN, int FM, int FN, bool BF>
+void launch_nt(const NtArgs& a, hipStream_t s, DevBuf& ws) {
+  constexpr int BM = WGM * FM * 16, BN = WGN * FN * 16, BK = BF ? 32 : 16;
+  NtParams p{};
+  p.M = a.M; p.N = a.N; p.K = a.K;
+  p.A = a.A; p.Alo = a.A_lo; p.lda = a.lda;
+  p.B = a.B; p.Blo = a.B_lo; p.ldb = a.ldb;
+  p.C = a.C; p.ldc = a.ldc; p.alpha = a.alpha; p.beta = a.beta;
+  p.tiles_m = ceil_div(a.M, BM);
+  p.tiles_n = ceil_div(a.N, BN);
+  const int64_t tiles = p.tiles_m * p.tiles_n;
+  int ks = a.ksplit;
+  if (ks <= 0) {
+    // about one workgroup per CU, whole multiple of the XCD count, >= 16 K steps per split
+    ks = (int)std::max<int64_t>(1, (int64_t)cu_count() / std::max<int64_t>(tiles, 1));
+    while (ks > 1 && a.K / ((int64_t)BK * ks) < 16) --ks;
+    while (ks > 1 && (tiles * ks) % 8 != 0) --ks;
+  }
+  int64_t kchunk = ceil_div(ceil_div(a.K, ks), BK) * BK;
+  ks = (int)ceil_div(std::max<int64_t>(a.K, 1), kchunk);
+  p.kchunk = kchunk;
+  p.ksplit = ks;
+  p.nwg = tiles * ks;
+  if (ks > 1) p.partial = static_cast<double*>(ws.reserve(sizeof(double) * (size_t)ks * a.M * a.N));
+  HH_REQUIRE(p.nwg < (1LL << 31), "gemm_nt: grid too large");
+  if (a.ev_begin) HH_HIP(hipEventRecord(a.ev_begin, s));
+  if (BF)
+    hipLaunchKernelGGL((nt_bf16x3_kernel<WGM, WGN, FM, FN>), dim3((unsigned)p.nwg), dim3(512), 0, s, p);
+  else
+    hipLaunchKernelGGL((nt_f64_kernel<WGM, WGN, FM, FN>), dim3((unsigned)p.nwg), dim3(512), 0, s, p);
+  HH_HIP(hipGetLastError());
+  if (a.ev_main_end) HH_HIP(hipEventRecord(a.ev_main_end, s));
+  if (ks > 1) {
+    const unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(a.M * a.N, 256), 8192);
+    hipLaunchKernelGGL(nt_reduce_kernel, dim3(blocks), dim3(256), 0, s, p.partial, ks, a.M, a.N, a.C, a.ldc, a.alpha,
+                       a.beta);
+    HH_HIP(hipGetLastError());
+  }
+}
+
+}  // namespace
+
+void gemm_nt(const NtArgs& a, NtFormat fmt, hipStream_t s, DevBuf& ws) {
+  HH_REQUIRE(a.M >= 0 && a.N >= 0 && a.K >= 0, "gemm_nt: negative size");
+  if (a.M == 0 || a.N == 0) return;
+  HH_REQUIRE(a.C && (a.K == 0 || (a.A && a.B)), "gemm_nt: null operand");
+  HH_REQUIRE(a.K % 32 == 0 && a.lda % 8 == 0 && a.ldb % 8 == 0, "gemm_nt: K must be a multiple of 32 and lda, ldb "
+                                                                  "multiples of 8");
+  if (fmt == NT_BF16X3) {
+    HH_REQUIRE(a.K == 0 || (a.A_lo && a.B_lo), "gemm_nt: bf16x3 needs lo arrays");
+    // 208 x 256 tiles: M = 400 is two tiles with 4 % padding
+    launch_nt<1, 8, 13, 2, true>(a, s, ws);
+  } else {
+    launch_nt<1, 8, 13, 1, false>(a, s, ws);
+  }
+}
+
+template <typename T>
+void split_bf16_rows(const T* in, int64_t rows, int64_t cols, int64_t ld_in, unsigned short* hi, unsigned short* lo,
+                     int64_t ld_out, hipStream_t s) {
+  const int64_t total = rows * ld_out;
+  if (total == 0) return;
+  const unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(total, 256), 65536);
+  hipLaunchKernelGGL((split_bf16_rows_kernel<T>), dim3(blocks), dim3(256), 0, s, in, rows, cols, ld_in, hi, lo,
+                     ld_out);
+  HH_HIP(hipGetLastError());
+}
+
+template void split_bf16_rows<double>(const double*, int64_t, int64_t, int64_t, unsigned short*, unsigned short*,
+                                      int64_t, hipStream_t);
+template void split_bf16_rows<float>(const float*, int64_t, int64_t, int64_t, unsigned short*, unsigned short*, int64_t,
+                                     hipStream_t);
+
+}  // namespace hh

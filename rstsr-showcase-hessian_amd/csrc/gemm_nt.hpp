@@ -1,0 +1,41 @@
+// "NT" GEMM on packed, K-contiguous operands (the RI-K hot contraction).
+//
+//   C[m, n] = alpha * sum_k A[m*lda + k] * B[n*ldb + k] + beta * C[m + n*ldc]     (C column-major)
+//
+// K must be a multiple of 32 and lda, ldb multiples of 8 (16-byte rows); operands are padded
+// by their producers.  Element formats:
+//   NT_F64    : A, B double; v_mfma_f64_16x16x4f64, FP64 accumulate.
+//   NT_BF16X3 : A, B given as two bf16 arrays each (hi and lo = x - hi); hi*hi + hi*lo + lo*hi
+//               on v_mfma_f32_16x16x32_bf16, FP32 accumulate; split-K partials summed in FP64.
+#pragma once
+#include "gemm.hpp"
+
+namespace hh {
+
+enum NtFormat : int { NT_F64 = 0, NT_BF16X3 = 1 };
+
+struct NtArgs {
+  int64_t M = 0, N = 0, K = 0;
+  const void* A = nullptr;     // F64: double*;   BF16X3: hi array (unsigned short*)
+  const void* A_lo = nullptr;  // BF16X3 only
+  int64_t lda = 0;
+  const void* B = nullptr;
+  const void* B_lo = nullptr;
+  int64_t ldb = 0;
+  double* C = nullptr;
+  int64_t ldc = 0;
+  double alpha = 1.0, beta = 0.0;
+  int ksplit = 0;  // 0: automatic
+  hipEvent_t ev_begin = nullptr, ev_main_end = nullptr;
+};
+
+void gemm_nt(const NtArgs& a, NtFormat fmt, hipStream_t s, DevBuf& ws);
+
+// Split x (double or float) into packed bf16 hi / lo arrays: hi = bf16(x), lo = bf16(x - hi).
+// Rows of `cols` elements are read with stride ld_in and written with stride ld_out; the columns
+// [cols, ld_out) of every output row are zeroed (K padding).
+template <typename T>
+void split_bf16_rows(const T* in, int64_t rows, int64_t cols, int64_t ld_in, unsigned short* hi, unsigned short* lo,
+                     int64_t ld_out, hipStream_t s);
+
+}  // namespace hh

@@ -7,15 +7,19 @@
 //          = sum_P B_P [ bra_a O_P + C_o (bra_a^T X_P) ],   X_P = B_P C_o,  O_P = C_o^T X_P
 // and the reference's k_bra is -K_a.  The reference loops over a and P with small GEMMs; here
 // all perturbations are batched into the GEMM N dimension:
-//   W[(j,a),(i,P)] = sum_mu bra[mu,(j,a)] X[mu,(i,P)]                 (M=nocc*nprop, K=nao)
-//   T[mu,P,i,a]    = sum_j bra[mu,j,a] O[j,i,P] + sum_k C_o[mu,k] W[(k,a),(i,P)]   (K=nocc each)
-//   K[u,(i,a)]     = sum_{(mu,P)} B[u,(mu,P)] T[(mu,P),(i,a)]          (K=nao*naux, the hot GEMM)
-// with the aux index processed in chunks so T stays a bounded buffer.  X and O depend only on
-// the MO coefficients and are built once per prepare_response.  Flop count equals the
-// reference's: 2 nao^2 naux nocc nprop + 6 nao naux nocc^2 nprop (+ X/O setup).
+//   W[(k,a),(P,i)] = sum_mu bra[mu,(k,a)] X[mu,(P,i)]                 (one GEMM, K = nao)
+//   T[(mu,P),(i,a)] = sum_j bra[mu,j,a] O[j,(P,i)] + sum_k C_o[mu,k] W[(k,a),(P,i)]
+//                                                      (one GEMM per a, K = 2 nocc concatenated)
+//   K[u,(i,a)]     = sum_{(mu,P)} B[u,(mu,P)] T[(mu,P),(i,a)]         (K = nao naux, the hot GEMM)
+// X[mu,(P,i)] = (B_P C_o)[mu,i] and O[j,(P,i)] = (C_o^T B_P C_o)[j,i] depend only on the MO
+// coefficients and are built once per prepare_response.  The hot GEMM reads B and T in a packed
+// layout with mu padded to kpad (a multiple of 32) and contiguous: B as [u][P][mu], T as
+// [(i,a)][P][mu].  Perturbations are processed in chunks so T stays a bounded buffer.  Flop count
+// equals the reference's: 2 nao^2 naux nocc nprop + 6 nao naux nocc^2 nprop (+ X/O setup).
 #include "rijk.hpp"
 
 #include "elementwise.hpp"
+#include "linalg.hpp"
 
 namespace hh {
 
@@ -29,6 +33,49 @@ Rijk::Rijk(const double* cderi, int64_t nao, int64_t naux, double scale_j, doubl
   HH_HIP(hipMemcpyAsync(cderi_.ptr, cderi, sizeof(double) * nao_tp_ * naux_, hipMemcpyDeviceToDevice, stream_));
   bfull_.reserve(sizeof(double) * nao_ * nao_ * naux_);
   unpack_triu<double, double>(cderi_.as<double>(), nao_, nao_tp_, naux_, bfull_.as<double>(), stream_);
+}
+
+namespace {
+__global__ void diag_min_kernel(const double* L, int64_t n, double* out) {
+  __shared__ double red[256];
+  double m = 1e300;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) m = fmin(m, L[i + i * n]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+}  // namespace
+
+std::unique_ptr<Rijk> Rijk::from_j3c(const double* j3c, const double* j2c, int64_t nao, int64_t naux,
+                                     double threshold, double scale_j, double scale_k, int nspin, hipStream_t s) {
+  HH_REQUIRE(j3c && j2c && nao > 0 && naux > 0, "from_j3c: null or empty input");
+  const int64_t ntp = nao * (nao + 1) / 2;
+  DevBuf L, cd, ws, dmin;
+  double* Lp = static_cast<double*>(L.reserve(sizeof(double) * naux * naux));
+  HH_HIP(hipMemcpyAsync(Lp, j2c, sizeof(double) * naux * naux, hipMemcpyDeviceToDevice, s));
+  const bool ok = cholesky(Lp, naux, s, ws);
+  if (!ok) throw Error(HH_ERR_INVALID, "j2c is not positive definite (Cholesky failed); eigenvalue-floor fallback "
+                                       "of decomp_j2c_cd is not provided by this library");
+  double* dm = static_cast<double*>(dmin.reserve(sizeof(double)));
+  hipLaunchKernelGGL(diag_min_kernel, dim3(1), dim3(256), 0, s, Lp, naux, dm);
+  double hmin = 0.0;
+  HH_HIP(hipMemcpyAsync(&hmin, dm, sizeof(double), hipMemcpyDeviceToHost, s));
+  HH_HIP(hipStreamSynchronize(s));
+  if (!(hmin > threshold))
+    throw Error(HH_ERR_INVALID, "j2c Cholesky factor has a diagonal element <= threshold (" + std::to_string(hmin) +
+                                    "); eigenvalue-floor fallback of decomp_j2c_cd is not provided by this library");
+  double* cdp = static_cast<double*>(cd.reserve(sizeof(double) * ntp * naux));
+  HH_HIP(hipMemcpyAsync(cdp, j3c, sizeof(double) * ntp * naux, hipMemcpyDeviceToDevice, s));
+  trsm_lower(Lp, naux, cdp, ntp, ntp, true, s, ws);
+  auto r = std::make_unique<Rijk>(cdp, nao, naux, scale_j, scale_k, nspin, s);
+  HH_HIP(hipStreamSynchronize(s));
+  std::swap(r->j2c_l_.ptr, L.ptr);
+  std::swap(r->j2c_l_.bytes, L.bytes);
+  return r;
 }
 
 Rijk::~Rijk() {
@@ -90,18 +137,18 @@ void Rijk::prepare_response(const double* const* mo_coeff, const double* const* 
     for (int64_t c = 0; c < nocc; ++c)
       HH_HIP(hipMemcpyAsync(mocc + c * nao_, mo_coeff[s] + occ_idx[c] * nao_, sizeof(double) * nao_,
                             hipMemcpyDeviceToDevice, stream_));
-    // X_P = B_P C_o  (batched over P): X[mu + nao*(i + nocc*P)]
+    // X_P = B_P C_o  (batched over P): X[mu + nao*(P + naux*i)]
     double* X = static_cast<double*>(x_[s].reserve(sizeof(double) * nao_ * nocc * naux_));
     {
       GemmArgs g;
       g.M = nao_; g.N = nocc; g.K = nao_;
       g.A = bfull_.ptr; g.lda = nao_; g.a_kmajor = false; g.sA0 = nao_ * nao_;
       g.B = mocc; g.ldb = nao_; g.b_kmajor = true; g.sB0 = 0;
-      g.C = X; g.ldc = nao_; g.sC0 = nao_ * nocc;
+      g.C = X; g.ldc = nao_ * naux_; g.sC0 = nao_;
       g.nb0 = naux_;
       gemm<double, double>(g, PREC_F64, stream_, ws_);
     }
-    // O[j, (i,P)] = sum_nu C_o[nu, j] X[nu, (i,P)]
+    // O[j, (P,i)] = sum_nu C_o[nu, j] X[nu, (P,i)]
     double* O = static_cast<double*>(o_[s].reserve(sizeof(double) * nocc * nocc * naux_));
     {
       GemmArgs g;
@@ -188,83 +235,180 @@ void Rijk::build_j_ao(const double* const* bra, int64_t nprop, Prec prec, double
   unpack_triu<double, double>(jtp, nao_, nao_tp_, nprop, j_ao, stream_);
 }
 
+namespace {
+// Bp[u][P][mu] = B_P[u, mu] for mu < nao, 0 for nao <= mu < kpad  (from packed cderi)
+template <typename T>
+__global__ void pack_hot_a_kernel(const double* __restrict__ cderi, int64_t nao, int64_t naux, int64_t kpad,
+                                  T* __restrict__ out) {
+  const int64_t total = nao * naux * kpad;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t mu = e % kpad, rest = e / kpad;
+    const int64_t P = rest % naux, u = rest / naux;
+    double v = 0.0;
+    if (mu < nao) {
+      const int64_t lo = mu < u ? mu : u, hi = mu < u ? u : mu;
+      v = cderi[hi * (hi + 1) / 2 + lo + P * (nao * (nao + 1) / 2)];
+    }
+    out[e] = (T)v;
+  }
+}
+
+// zero rows [nao, kpad) of every (n, Pl) segment of T[n][Pl][kpad]
+__global__ void zero_pad_rows_kernel(double* T, int64_t nseg, int64_t nao, int64_t kpad) {
+  const int64_t w = kpad - nao;
+  const int64_t total = nseg * w;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x)
+    T[(e / w) * kpad + nao + e % w] = 0.0;
+}
+
+unsigned ew_blocks(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 65536)); }
+}  // namespace
+
+void Rijk::ensure_packed(Prec prec) {
+  kpad_ = ceil_div(nao_, 32) * 32;
+  const int64_t n = nao_ * naux_ * kpad_;
+  const bool bf16 = (prec == PREC_BF16X3);
+  if (prec == PREC_F32 && !bp32_.ptr) {
+    bp32_.reserve(sizeof(float) * n);
+    hipLaunchKernelGGL((pack_hot_a_kernel<float>), dim3(ew_blocks(n)), dim3(256), 0, stream_, cderi_.as<double>(),
+                       nao_, naux_, kpad_, bp32_.as<float>());
+    HH_HIP(hipGetLastError());
+  }
+  if (prec == PREC_F64 && !bp64_.ptr) {
+    bp64_.reserve(sizeof(double) * n);
+    hipLaunchKernelGGL((pack_hot_a_kernel<double>), dim3(ew_blocks(n)), dim3(256), 0, stream_, cderi_.as<double>(),
+                       nao_, naux_, kpad_, bp64_.as<double>());
+    HH_HIP(hipGetLastError());
+  }
+  if (bf16 && !bp_hi_.ptr) {
+    // pack to an FP32 staging copy, then split every row into hi / lo
+    bp_hi_.reserve(sizeof(unsigned short) * n);
+    bp_lo_.reserve(sizeof(unsigned short) * n);
+    DevBuf full;
+    float* f = static_cast<float*>(full.reserve(sizeof(float) * n));
+    hipLaunchKernelGGL((pack_hot_a_kernel<float>), dim3(ew_blocks(n)), dim3(256), 0, stream_, cderi_.as<double>(), nao_,
+                       naux_, kpad_, f);
+    HH_HIP(hipGetLastError());
+    split_bf16_rows<float>(f, nao_ * naux_, kpad_, kpad_, bp_hi_.as<unsigned short>(), bp_lo_.as<unsigned short>(),
+                           kpad_, stream_);
+    HH_HIP(hipStreamSynchronize(stream_));
+  }
+}
+
 void Rijk::build_k(int s, const double* bra, int64_t nprop, Prec prec, double alpha, double beta, double* out) {
   const int64_t nocc = nocc_[s];
-  const int64_t ncol = nocc * nprop;  // (i, a) columns
-  const bool f64 = (prec == PREC_F64);
-  const size_t esz = f64 ? sizeof(double) : sizeof(float);
+  const int64_t nP = naux_ * nocc;  // (P, i) columns
+  const bool f64 = (prec == PREC_F64), bf = (prec == PREC_BF16X3);
+  ensure_packed(prec);
   if (!f64) ensure_mixed();
+  const size_t esz = f64 ? sizeof(double) : sizeof(float);
   const void* braX = bra;
   if (!f64) {
-    float* b32 = static_cast<float*>(bra32_.reserve(sizeof(float) * std::max(nao_ * ncol, std::max(nao_tp_, naux_) * nprop)));
-    cast<double, float>(bra, nao_ * ncol, b32, stream_);
+    float* b32 = static_cast<float*>(bra32_.reserve(sizeof(float) * std::max(nao_ * nocc * nprop, std::max(nao_tp_, naux_) * nprop)));
+    cast<double, float>(bra, nao_ * nocc * nprop, b32, stream_);
     braX = b32;
   }
   const void* X = f64 ? x_[s].ptr : x32_[s].ptr;
   const void* O = f64 ? o_[s].ptr : o32_[s].ptr;
-  const void* C = f64 ? mocc_[s].ptr : mocc32_[s].ptr;
-  const void* Bf = f64 ? bfull_.ptr : bfull32_.ptr;
-
-  // W[(j,a),(i,P)] = sum_mu bra[mu,(j,a)] X[mu,(i,P)]
-  void* W = w_.reserve(esz * ncol * nocc * naux_);
+  const void* Co = f64 ? mocc_[s].ptr : mocc32_[s].ptr;
+  // W[(k + nocc*a) + nocc*nprop*(P + naux*i)] = sum_mu bra[mu,(k,a)] X[mu,(P,i)]
+  const int64_t ldw = nocc * nprop;
+  void* W = w_.reserve(esz * ldw * nP);
   {
     GemmArgs g;
-    g.M = ncol; g.N = nocc * naux_; g.K = nao_;
+    g.M = ldw; g.N = nP; g.K = nao_;
     g.A = braX; g.lda = nao_; g.a_kmajor = true;
     g.B = X; g.ldb = nao_; g.b_kmajor = true;
-    g.C = W; g.ldc = ncol;
+    g.C = W; g.ldc = ldw;
     if (f64) gemm<double, double>(g, prec, stream_, ws_);
     else gemm<float, float>(g, prec, stream_, ws_);
   }
-  // aux chunk so that T[mu, P, i, a] stays within ~2 GiB
-  int64_t pc = k_chunk;
-  if (pc <= 0) pc = std::max<int64_t>(1, (int64_t)((2LL << 30) / (esz * nao_ * ncol)));
-  pc = std::min(pc, naux_);
-  void* T = t_.reserve(esz * nao_ * pc * ncol);
-  for (int64_t p0 = 0; p0 < naux_; p0 += pc) {
-    const int64_t np = std::min(pc, naux_ - p0);
-    // T[mu + nao*(Pl + np*(i + nocc*a))] = sum_j bra[mu, j, a] O[j, i, p0+Pl]   (batch Pl, a)
-    {
-      GemmArgs g;
-      g.M = nao_; g.N = nocc; g.K = nocc;
-      g.A = braX; g.lda = nao_; g.a_kmajor = false; g.sA0 = 0; g.sA1 = nao_ * nocc;
-      g.B = static_cast<const char*>(O) + esz * (p0 * nocc * nocc); g.ldb = nocc; g.b_kmajor = true;
-      g.sB0 = nocc * nocc; g.sB1 = 0;
-      g.C = T; g.ldc = nao_ * np; g.sC0 = nao_; g.sC1 = nao_ * np * nocc;
-      g.nb0 = np; g.nb1 = nprop;
-      if (f64) gemm<double, double>(g, prec, stream_, ws_);
-      else gemm<float, float>(g, prec, stream_, ws_);
+  // perturbation chunk: T[(i,a)][P][kpad] within ~12 GiB
+  const int64_t per_a = nocc * naux_ * kpad_;
+  int64_t ac = k_chunk;
+  if (ac <= 0) ac = std::max<int64_t>(1, (int64_t)((12LL << 30) / (8 * per_a)));
+  ac = std::min(ac, nprop);
+  // rows nao..kpad of every (n, P) segment stay zero: the generation GEMM writes mu < nao only,
+  // so the buffers are cleared once per layout
+  void* T = nullptr;
+  unsigned short* th = nullptr;
+  unsigned short* tl = nullptr;
+  const int64_t tlen = per_a * ac;
+  if (bf) {
+    th = static_cast<unsigned short*>(tp_hi_.reserve(sizeof(unsigned short) * tlen));
+    tl = static_cast<unsigned short*>(tp_lo_.reserve(sizeof(unsigned short) * tlen));
+    if (tp_key_[0] != 2 || tp_key_[1] != tlen) {
+      fill<unsigned short>(th, tlen, 0, stream_);
+      fill<unsigned short>(tl, tlen, 0, stream_);
+      tp_key_[0] = 2;
+      tp_key_[1] = tlen;
     }
-    //   += sum_k C_o[mu, k] W[k + nocc*a + ncol*(i + nocc*(p0+Pl))]
-    {
-      GemmArgs g;
-      g.M = nao_; g.N = nocc; g.K = nocc;
-      g.A = C; g.lda = nao_; g.a_kmajor = false;
-      g.B = static_cast<const char*>(W) + esz * (p0 * ncol * nocc); g.ldb = ncol; g.b_kmajor = true;
-      g.sB0 = ncol * nocc; g.sB1 = nocc;
-      g.C = T; g.ldc = nao_ * np; g.sC0 = nao_; g.sC1 = nao_ * np * nocc;
-      g.nb0 = np; g.nb1 = nprop;
-      g.beta = 1.0;
-      if (f64) gemm<double, double>(g, prec, stream_, ws_);
-      else gemm<float, float>(g, prec, stream_, ws_);
+  } else {
+    T = tp_.reserve(esz * tlen);
+    if (tp_key_[0] != (int64_t)esz || tp_key_[1] != tlen) {
+      if (f64) fill<double>(static_cast<double*>(T), tlen, 0.0, stream_);
+      else fill<float>(static_cast<float*>(T), tlen, 0.f, stream_);
+      tp_key_[0] = (int64_t)esz;
+      tp_key_[1] = tlen;
     }
-    // out[u, (i,a)] = alpha * sum_{(mu,Pl)} B[u, mu, p0+Pl] T[(mu,Pl), (i,a)] + beta' * out
+  }
+  for (int64_t a0 = 0; a0 < nprop; a0 += ac) {
+    const int64_t na = std::min(ac, nprop - a0);
     {
+      // T[mu + kpad*(P + naux*(i + nocc*al))] = bra_a O + C_o W_a   (one GEMM per a, K = 2 nocc)
       GemmArgs g;
-      g.M = nao_; g.N = ncol; g.K = nao_ * np;
-      g.A = static_cast<const char*>(Bf) + esz * (p0 * nao_ * nao_); g.lda = nao_; g.a_kmajor = false;
-      g.B = T; g.ldb = nao_ * np; g.b_kmajor = true;
-      g.C = out; g.ldc = nao_;
-      g.alpha = alpha;
-      g.beta = (p0 == 0) ? beta : 1.0;
-      if (profiling_) {
-        const int e0 = next_event_++, e1 = next_event_++;
-        g.ev_begin = ev(e0);
-        g.ev_main_end = ev(e1);
-        pending_.push_back({T_K_HOT, {e0, e1}});
+      g.M = nao_; g.N = nP; g.K = 2 * nocc; g.K1 = nocc;
+      g.A = static_cast<const char*>(braX) + esz * (nao_ * nocc * a0); g.lda = nao_; g.a_kmajor = false;
+      g.sA0 = nao_ * nocc;
+      g.B = O; g.ldb = nocc; g.b_kmajor = true; g.sB0 = 0;
+      g.A2 = Co; g.lda2 = nao_; g.sA2_0 = 0;
+      g.B2 = static_cast<const char*>(W) + esz * (nocc * a0); g.ldb2 = ldw; g.sB2_0 = nocc;
+      g.ldc = kpad_; g.sC0 = per_a;
+      g.nb0 = na;
+      g.ksplit = 1;
+      if (bf) {
+        g.C = th;  // hi / lo written directly by the epilogue
+        g.C_lo = tl;
+      } else {
+        g.C = T;
       }
       if (f64) gemm<double, double>(g, prec, stream_, ws_);
-      else gemm<float, double>(g, prec, stream_, ws_);
+      else gemm<float, float>(g, prec, stream_, ws_);
+    }
+    double* outc = out + nao_ * nocc * a0;
+    const double bet = beta;
+    hipEvent_t e_begin = nullptr, e_end = nullptr;
+    if (profiling_) {
+      const int e0 = next_event_++, e1 = next_event_++;
+      e_begin = ev(e0);
+      e_end = ev(e1);
+      pending_.push_back({T_K_HOT, {e0, e1}});
+    }
+    if (prec == PREC_F32) {
+      GemmArgs g;
+      g.M = nao_; g.N = nocc * na; g.K = naux_ * kpad_;
+      g.A = bp32_.ptr; g.lda = naux_ * kpad_; g.a_kmajor = true;
+      g.B = T; g.ldb = naux_ * kpad_; g.b_kmajor = true;
+      g.C = outc; g.ldc = nao_;
+      g.alpha = alpha; g.beta = bet;
+      g.ev_begin = e_begin; g.ev_main_end = e_end;
+      gemm<float, double>(g, PREC_F32, stream_, ws_);
+      continue;
+    }
+    NtArgs a;
+    a.M = nao_; a.N = nocc * na; a.K = naux_ * kpad_;
+    a.lda = naux_ * kpad_; a.ldb = naux_ * kpad_;
+    a.C = outc; a.ldc = nao_;
+    a.alpha = alpha; a.beta = bet;
+    a.ev_begin = e_begin; a.ev_main_end = e_end;
+    if (bf) {
+      a.A = bp_hi_.as<unsigned short>(); a.A_lo = bp_lo_.as<unsigned short>();
+      a.B = th; a.B_lo = tl;
+      gemm_nt(a, NT_BF16X3, stream_, ws_);
+    } else {
+      a.A = bp64_.as<double>();
+      a.B = T;
+      gemm_nt(a, NT_F64, stream_, ws_);
     }
   }
 }

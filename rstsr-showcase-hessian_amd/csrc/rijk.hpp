@@ -7,6 +7,7 @@
 #include <memory>
 
 #include "gemm.hpp"
+#include "gemm_nt.hpp"
 
 namespace hh {
 
@@ -14,6 +15,13 @@ class Rijk {
  public:
   ~Rijk();
   Rijk(const double* cderi, int64_t nao, int64_t naux, double scale_j, double scale_k, int nspin, hipStream_t s);
+  // generate_cderi_with_decomp (hess_r.rs:1725-1735) from device j3c [nao_tp, naux] (packed
+  // upper, column-major) and j2c [naux, naux]: j2c = L L^T (decomp_j2c_cd,
+  // pure_decompose.rs:28-55), cderi = j3c L^-T (solve_by_j2c, side Right).  Throws if the
+  // Cholesky factor has a diagonal element <= threshold.
+  static std::unique_ptr<Rijk> from_j3c(const double* j3c, const double* j2c, int64_t nao, int64_t naux,
+                                        double threshold, double scale_j, double scale_k, int nspin, hipStream_t s);
+  const double* j2c_l() const { return j2c_l_.as<double>(); }
 
   void prepare_response(const double* const* mo_coeff, const double* const* mo_occ, const int64_t* nmo);
   // resp[s] = scale_j * f_s * J_ao mocc_s + scale_k * K_s  (f = 1 RHF, 0.5 UHF)
@@ -31,7 +39,7 @@ class Rijk {
   double scale_j() const { return scale_j_; }
   double scale_k() const { return scale_k_; }
 
-  // Aux-chunk size for the K contraction (0 = choose from the size of the intermediate).
+  // Perturbations per K-contraction batch (0 = choose from the size of the intermediate).
   int64_t k_chunk = 0;
 
   // Stage timing with HIP events on the handle's stream (the reference keeps wall-clock
@@ -48,6 +56,9 @@ class Rijk {
   // K: out_s[nao, nocc_s, nprop] = alpha * (K_s without its minus sign) + beta * out_s
   void build_k(int s, const double* bra, int64_t nprop, Prec prec, double alpha, double beta, double* out);
   void ensure_mixed();
+  // Packed hot-GEMM operand Bp[u][P][mu_pad] (mu_pad = kpad_ = nao rounded up to 32, zeros
+  // beyond nao): FP64 copy and/or bf16 hi/lo split, built once per handle.
+  void ensure_packed(Prec prec);
   hipEvent_t ev(int i);
   int mark();                     // record a fresh event now, return its index
   void span(int key, int begin);  // accumulate (begin event .. now) into times_[key]
@@ -68,15 +79,20 @@ class Rijk {
   bool mixed_ready_ = false;
   int64_t nocc_[2] = {0, 0};
 
+  DevBuf j2c_l_;      // [naux, naux] lower Cholesky factor of the metric (from_j3c only)
   DevBuf cderi_;      // [nao_tp, naux] f64
   DevBuf bfull_;      // [nao, nao, naux] f64 (unpacked)
+  int64_t kpad_ = 0;
+  DevBuf bp64_, bp32_, bp_hi_, bp_lo_;  // packed hot-GEMM A operand
+  DevBuf tp_, tp_hi_, tp_lo_;           // packed hot-GEMM B operand (one perturbation chunk)
+  int64_t tp_key_[2] = {0, 0};          // (element size, length) of tp_ when its padding was zeroed
   DevBuf cderi32_;    // f32 copies for the mixed-precision modes
   DevBuf bfull32_;
   DevBuf mocc_[2];    // [nao, nocc] f64
   DevBuf mocc32_[2];
-  DevBuf x_[2];       // X[mu, i, P] = (B_P C_o)[mu, i]
+  DevBuf x_[2];       // X[mu, P, i] = (B_P C_o)[mu, i]
   DevBuf x32_[2];
-  DevBuf o_[2];       // O[j, i, P] = (C_o^T B_P C_o)[j, i]
+  DevBuf o_[2];       // O[j, P, i] = (C_o^T B_P C_o)[j, i]
   DevBuf o32_[2];
   // per-call scratch
   DevBuf w_, t_, bra32_, dm_, dmtp_, itm_, jtp_, jao_, ws_;

@@ -62,8 +62,9 @@ def cholesky(a):
 def trsm_lower(L, x, side_right=False):
     """side_right False: L^-1 x (x [nd, n2]);  True: x L^-T (x [n2, nd])."""
     X = fortran(x).clone()
+    Lf = fortran(L)  # held across the call
     nd = L.shape[0]
     n2 = X.shape[0] if side_right else X.shape[1]
-    check(lib().hh_linalg_trsm_lower(fortran(L).data_ptr(), nd, X.data_ptr(), n2, X.shape[0], int(side_right),
+    check(lib().hh_linalg_trsm_lower(Lf.data_ptr(), nd, X.data_ptr(), n2, X.shape[0], int(side_right),
                                      stream_of(x.device)))
     return X

@@ -39,22 +39,26 @@ class NIMatmul:
                 pass
             self._h = None
 
+    # Every device operand is bound to a local name for the duration of the C call: a temporary
+    # copy made inline in the argument list would be freed (and its block reused) before the
+    # asynchronous kernels read it.
     def make_rho_from_one_bra_mult_ket(self, bra, kets, den_type, precision="fp64"):
         nao, nocc = bra.shape
         k = fortran(kets.reshape(nao, nocc, -1))
+        b = fortran(bra)
         nprop = k.shape[2]
         out = empty_f((self.ng, NVAR[den_type], nprop), self.device)
-        check(lib().hh_numint_rho1_one_bra_mult_ket(self._h, DEN[den_type], fortran(bra).data_ptr(), nocc,
+        check(lib().hh_numint_rho1_one_bra_mult_ket(self._h, DEN[den_type], b.data_ptr(), nocc,
                                                     k.data_ptr(), nprop, out.data_ptr(), PREC[precision]))
         return out
 
     def make_rks_fxc_pot_with_eff_bra_trans(self, fxc_eff, rho1, bra, den_type, precision="fp64"):
         nao, nocc = bra.shape
         nprop = rho1.shape[2]
+        f, r, b = fortran(fxc_eff), fortran(rho1), fortran(bra)
         out = empty_f((nao, nocc, nprop), self.device)
-        check(lib().hh_numint_fxc_pot_bra_trans(self._h, DEN[den_type], fortran(fxc_eff).data_ptr(),
-                                                fortran(rho1).data_ptr(), nprop, fortran(bra).data_ptr(), nocc,
-                                                out.data_ptr(), PREC[precision]))
+        check(lib().hh_numint_fxc_pot_bra_trans(self._h, DEN[den_type], f.data_ptr(), r.data_ptr(), nprop,
+                                                b.data_ptr(), nocc, out.data_ptr(), PREC[precision]))
         return out
 
     def get_rks_response_bra(self, den_type, fxc_eff, mo1_bra, mocc, precision="fp64"):
@@ -62,11 +66,11 @@ class NIMatmul:
         shape = tuple(mo1_bra.shape)
         nao, nocc = mocc.shape
         k = fortran(mo1_bra.reshape(nao, nocc, -1))
+        f, m = fortran(fxc_eff), fortran(mocc)
         nprop = k.shape[2]
         out = empty_f((nao, nocc, nprop), self.device)
-        check(lib().hh_numint_rks_response_bra(self._h, DEN[den_type], fortran(fxc_eff).data_ptr(), k.data_ptr(),
-                                               nprop, fortran(mocc).data_ptr(), nocc, out.data_ptr(),
-                                               PREC[precision]))
+        check(lib().hh_numint_rks_response_bra(self._h, DEN[den_type], f.data_ptr(), k.data_ptr(), nprop,
+                                               m.data_ptr(), nocc, out.data_ptr(), PREC[precision]))
         return out.reshape(shape)
 
 

@@ -42,7 +42,7 @@ def unpack_triu(tp, nao):
 class _RIJKBase:
     nspin = 1
 
-    def __init__(self, cderi, scale_j, scale_k, nao=None):
+    def __init__(self, cderi, scale_j, scale_k, nao=None, _j2c=None, _threshold=1e-14):
         nao_tp, naux = cderi.shape
         if nao is None:
             nao = int(round((np.sqrt(8 * nao_tp + 1) - 1) / 2))
@@ -52,8 +52,14 @@ class _RIJKBase:
         self.device = cderi.device
         cd = fortran(cderi)
         h = C.c_void_p()
-        check(lib().hh_rijk_create(cd.data_ptr(), nao, naux, self.scale_j, self.scale_k, self.nspin,
-                                   stream_of(self.device), C.byref(h)))
+        if _j2c is None:
+            check(lib().hh_rijk_create(cd.data_ptr(), nao, naux, self.scale_j, self.scale_k, self.nspin,
+                                       stream_of(self.device), C.byref(h)))
+        else:
+            j2 = fortran(_j2c)  # held across the call
+            check(lib().hh_rijk_create_from_j3c(cd.data_ptr(), j2.data_ptr(), nao, naux, float(_threshold),
+                                                self.scale_j, self.scale_k, self.nspin, stream_of(self.device),
+                                                C.byref(h)))
         self._h = h
         self.nocc = None
 
@@ -66,8 +72,9 @@ class _RIJKBase:
                 pass
             self._h = None
 
-    def set_aux_chunk(self, n):
-        check(lib().hh_rijk_set_aux_chunk(self._h, int(n)))
+    def set_batch(self, nprop_batch):
+        """Perturbations per K-contraction batch (0 = sized from memory)."""
+        check(lib().hh_rijk_set_batch(self._h, int(nprop_batch)))
 
     def set_profiling(self, on=True):
         check(lib().hh_rijk_set_profiling(self._h, int(bool(on))))
@@ -122,6 +129,11 @@ class RHessRIJK(_RIJKBase):
     """RHessRIJK (hess_r.rs:1737-1954), response side, built from an existing cderi
     (new_with_cderi, hess_r.rs:1775-1794)."""
     nspin = 1
+
+    @classmethod
+    def from_j3c(cls, j3c_tp, j2c, scale_j, scale_k, threshold=1e-14):
+        """cderi generated on the device from j3c [nao_tp, naux] and j2c (hess_r.rs:1725-1735)."""
+        return cls(j3c_tp, scale_j, scale_k, _j2c=j2c, _threshold=threshold)
 
     def make_response_preparation(self, mo_coeff, mo_occ):
         self._prepare([mo_coeff], [mo_occ])

@@ -44,13 +44,13 @@ def test_rhf_response_fp64(dev, nao, naux, nocc, nprop):
 
 
 def test_rhf_response_trailing_shape_and_chunking(dev):
-    """bra [nao, nocc, 3, natm]; force several aux chunks for the K contraction."""
+    """bra [nao, nocc, 3, natm]; force several perturbation batches for the K contraction."""
     from hesship import RHessRIJK
     nao, naux, nocc, natm = 20, 41, 4, 3
     rng, cderi, C, occ = system(nao, naux, nocc, seed=1)
     bra = rng.standard_normal((nao, nocc, 3, natm))
     obj = RHessRIJK(T(cderi, dev), 1.0, 1.0)
-    obj.set_aux_chunk(7)   # 41 aux -> chunks of 7, last one ragged
+    obj.set_batch(4)   # 9 perturbations -> batches of 4, last one ragged
     obj.set_profiling(True)
     obj.make_response_preparation(T(C, dev), occ)
     ref = orc.rhf_response_bra(cderi, C, occ, bra, 1.0, 1.0)
@@ -58,7 +58,7 @@ def test_rhf_response_trailing_shape_and_chunking(dev):
     assert got.shape == bra.shape
     assert rel(got, ref) < 1e-10
     t = obj.timing()
-    assert t["k_hot_launches"] == 6 and 0 < t["k_hot_ms"] <= t["k_total_ms"] <= t["response_ms"]
+    assert t["k_hot_launches"] == 3 and 0 < t["k_hot_ms"] <= t["k_total_ms"] <= t["response_ms"]
 
 
 def test_separated_core_matches_oracle(dev):
@@ -129,3 +129,22 @@ def test_pack_unpack_device(dev):
     sym = dm + dm.transpose(1, 0, 2)
     full = unpack_triu(T(orc.pack_triu(sym), dev), 11).cpu().numpy()
     np.testing.assert_array_equal(full, sym)
+
+
+def test_cderi_generated_from_j3c_and_metric(dev):
+    """RI metric on the device (Cholesky + TRSM, FP64) + response == naive einsum with j2c^-1."""
+    from hesship import RHessRIJK, HesshipError
+    rng = np.random.default_rng(11)
+    nao, naux, nocc = 14, 130, 3
+    j3c = rng.standard_normal((nao, nao, naux)); j3c = 0.5 * (j3c + j3c.transpose(1, 0, 2))
+    a = rng.standard_normal((naux, naux)); j2c = a @ a.T + naux * np.eye(naux)
+    C, _ = np.linalg.qr(rng.standard_normal((nao, nao)))
+    occ = np.zeros(nao); occ[:nocc] = 2.0
+    bra = rng.standard_normal((nao, nocc, 4))
+    obj = RHessRIJK.from_j3c(T(orc.pack_triu(j3c), dev), T(j2c, dev), 1.0, 1.0)
+    obj.make_response_preparation(T(C, dev), occ)
+    got = obj.get_response_bra(T(bra, dev)).cpu().numpy()
+    ref = orc.response_bra_naive(j3c, j2c, C, occ, bra, 1.0, 1.0)
+    assert rel(got, ref) < 1e-9
+    with pytest.raises(HesshipError):
+        RHessRIJK.from_j3c(T(orc.pack_triu(j3c), dev), T(-j2c, dev), 1.0, 1.0)
