@@ -75,6 +75,38 @@ def cpu_baseline(cderi, C, occ, bra, budget_s=12.0, nbatch=72):
                        f"({-(-done // nbatch)} batches of {nbatch}), all {nprop} perturbations, {dt:.1f} s")
 
 
+def cphf_case(obj, d_C, occ, nao, nocc, nprop, dev):
+    """Synthetic CPHF system on the benchmark's RI-JK tensors: orbital energies and first-order
+    Fock/overlap derivatives drawn from the same seeded generator (seed 0x5EED0004); the RI-JK
+    scale is set so the dimensionless response has norm ~0.3 of its input, as in a typical
+    closed-shell CPHF."""
+    import numpy as np
+    import torch
+    import hesship
+    from hesship import synth
+    z = synth.normal(0x5EED0004, 0, nao + 2 * nao * nocc * nprop)
+    e = np.sort(np.concatenate([-1.4 + 1.1 * np.abs(np.tanh(z[:nocc])), 0.2 + 2.5 * np.abs(np.tanh(z[nocc:nao]))]))
+    f1 = z[nao:nao + nao * nocc * nprop].reshape(nprop, nocc, nao).T.copy()
+    s1 = 0.1 * z[nao + nao * nocc * nprop:].reshape(nprop, nocc, nao).T.copy()
+    drv = hesship.RHessCPHF(d_C, occ, e, dict(cphf_tol=1e-8))
+    drv.add_el(obj)
+    v = torch.from_numpy(np.random.default_rng(1).standard_normal((nao, nocc, 4))).to(dev)
+    ratio = (drv.response_dimless_cphf(v).norm() / v.norm()).item()
+    return drv, e, torch.from_numpy(f1).to(dev), torch.from_numpy(s1).to(dev), ratio
+
+
+def run_cphf(drv, f1, s1, prec):
+    import torch
+    st = {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rhs = drv.compute_dimless_rhs(f1, s1)
+    mo1 = drv.solve_dimless_cphf(rhs, precision=prec, stats=st)
+    mo1f, moe1 = drv.finalize_cphf(f1, s1, mo1)
+    torch.cuda.synchronize()
+    return mo1f, moe1, time.perf_counter() - t0, st
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +117,7 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-cphf", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the RI-JK response of one molecule runs on "
@@ -184,6 +217,27 @@ def main():
                      "kernel": "gemm_kernel (hot K contraction, one aux chunk per launch)",
                      "launches_per_step": nl, "avg_launch_ms": round(t_launch * 1e3, 4)},
     }
+    if not args.no_cphf:
+        # CPHF wall time: same tensors, RI-JK scaled so the dimensionless response is ~0.3 of its
+        # input (the benchmark's raw N(0,1) cderi would make I + A far from a physical CPHF)
+        probe, _, _, _, ratio = cphf_case(obj, d_C, occ, nao, nocc, nprop, dev)
+        del probe
+        scale = (0.3 / ratio) ** 0.5
+        obj2 = hesship.RHessRIJK(d_cderi * scale, 1.0, 1.0)
+        obj2.make_response_preparation(d_C, occ)
+        drv, e, f1, s1, ratio2 = cphf_case(obj2, d_C, occ, nao, nocc, nprop, dev)
+        res = {"natm_equiv": nprop // 3, "cphf_tol": 1e-8, "response_ratio": round(ratio2, 4)}
+        ref = None
+        for prec in ("fp64", args.precision):
+            mo1, moe1, wall, st = run_cphf(drv, f1, s1, prec)
+            if ref is None:
+                ref = mo1
+            res[prec] = {"wall_s": round(wall, 4), "cycles": st.get("cycles"), "aop_calls": st.get("aop_calls"),
+                         "refinements": st.get("refinements"), "residual": st.get("residual"),
+                         "max_abs_mo1_err_vs_fp64": (mo1 - ref).abs().max().item()}
+            log(f"[bench] cphf {prec}: {wall:.3f} s, cycles {st.get('cycles')}, aop calls {st.get('aop_calls')}")
+        res["speedup_vs_fp64"] = round(res["fp64"]["wall_s"] / res[args.precision]["wall_s"], 3)
+        line["cphf"] = res
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(cderi, C, occ, bra, args.cpu_budget)
     print(json.dumps(line), flush=True)

@@ -206,6 +206,41 @@ hh_status hh_cphf_hess(hh_cphf* h, const double* f1mo, const double* s1mo, const
                        int64_t natm, double* de);
 
 /* ---------------------------------------------------------------------------------------- */
+/* Learned diagonal CPHF preconditioner (no reference counterpart: the reference preconditions */
+/* CPHF only by the orbital-energy scaling of the dimensionless form, rscf.rs:237-263).       */
+
+typedef struct hh_mlp hh_mlp;
+#define HH_MLP_NPARAM 8641  /* 3 -> 64 -> 64 -> 64 -> 1, ReLU, FP32 */
+
+/* Network with seeded He-uniform init (an untrained network predicts ~0, i.e. diag ~1). */
+hh_status hh_mlp_create(uint64_t seed, void* stream, hh_mlp** out);
+hh_status hh_mlp_destroy(hh_mlp* h);
+/* y [n] = f(x [n, 3]) (float, device, column-major). */
+hh_status hh_mlp_forward(hh_mlp* h, const float* x, int64_t n, float* y);
+/* Weighted MSE sum w (f(x) - t)^2 / sum w (host *loss) and its gradient grad [HH_MLP_NPARAM]
+ * (device). */
+hh_status hh_mlp_loss_and_grad(hh_mlp* h, const float* x, const float* t, const float* w, int64_t n, float* grad,
+                               double* loss);
+/* `epochs` full-batch Adam steps; *loss = loss before the last step.  Synchronous. */
+hh_status hh_mlp_train(hh_mlp* h, const float* x, const float* t, const float* w, int64_t n, int32_t epochs, float lr,
+                       double* loss);
+hh_status hh_mlp_get_params(const hh_mlp* h, float* host_out);
+hh_status hh_mlp_set_params(hh_mlp* h, const float* host_in);
+
+/* RI pair norms pair[m, i] = sum_P ((C^T B_P C_o)[m, i])^2 for all MOs m of mo_coeff [nao, nmo]
+ * (device) and the handle's occupied orbitals of spin s.  Synchronous. */
+hh_status hh_rijk_pair_norms(hh_rijk* h, int32_t s, const double* mo_coeff, int64_t nmo, double* pair);
+/* Features x [nvir*nocc, 3] (float) of the virtual-occupied pairs (sample a + nvir*i):
+ * standardised ln(e_a - e_i), ln pair[a,i], ln rms_p |r[a,i,p]| for r [nmo, nocc, nprop]. */
+hh_status hh_cphf_precond_features(hh_cphf* h, const double* pair, const double* r, int64_t nprop, float* x);
+/* Training target from a (residual r, solution xs) pair: t = ln clip(sum_p r xs / sum_p r^2,
+ * 0.05, 20), weight w = sum_p r^2 (both float [nvir*nocc]). */
+hh_status hh_cphf_precond_target(hh_cphf* h, const double* r, const double* xs, int64_t nprop, float* t, float* w);
+/* diag [nmo, nocc] = exp(clip(y, -3, 3)) on virtual rows, 1 on occupied rows; pass it as the
+ * `diag` argument of hh_cphf_solve_dimless. */
+hh_status hh_cphf_precond_diag(hh_cphf* h, const float* y, double* diag);
+
+/* ---------------------------------------------------------------------------------------- */
 /* Dense FP64 linear algebra on the device                                                    */
 
 /* Solve H X = G in place (H overwritten by LU factors, G by X); replaces solve_general as

@@ -3,6 +3,7 @@
 #include "gemm.hpp"
 #include "cphf.hpp"
 #include "linalg.hpp"
+#include "mlp.hpp"
 #include "numint.hpp"
 #include "rijk.hpp"
 
@@ -56,6 +57,10 @@ void fill_stats(hh_krylov_stats* out, const hh::KrylovStats& k, int refinements,
 
 struct hh_numint {
   std::unique_ptr<hh::Numint> impl;
+};
+
+struct hh_mlp {
+  std::unique_ptr<hh::DiagMlp> impl;
 };
 
 struct hh_cphf {
@@ -374,6 +379,92 @@ hh_status hh_cphf_hess(hh_cphf* h, const double* f1mo, const double* s1mo, const
   HH_ABI_BEGIN
   HH_REQUIRE(h && f1mo && s1mo && mo1 && mo_e1 && de && natm > 0, "null argument or natm <= 0");
   h->impl->cphf_hess(f1mo, s1mo, mo1, mo_e1, natm, de);
+  HH_ABI_END
+}
+
+hh_status hh_mlp_create(uint64_t seed, void* stream, hh_mlp** out) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(out != nullptr, "out is null");
+  *out = nullptr;
+  auto h = std::make_unique<hh_mlp>();
+  h->impl = std::make_unique<DiagMlp>(seed, (hipStream_t)stream);
+  *out = h.release();
+  HH_ABI_END
+}
+
+hh_status hh_mlp_destroy(hh_mlp* h) {
+  HH_ABI_BEGIN
+  if (h) {
+    HH_HIP(hipStreamSynchronize(h->impl->stream()));
+    delete h;
+  }
+  HH_ABI_END
+}
+
+hh_status hh_mlp_forward(hh_mlp* h, const float* x, int64_t n, float* y) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && n >= 0 && (n == 0 || (x && y)), "invalid argument");
+  h->impl->forward(x, n, y);
+  HH_ABI_END
+}
+
+hh_status hh_mlp_loss_and_grad(hh_mlp* h, const float* x, const float* t, const float* w, int64_t n, float* grad,
+                               double* loss) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && x && t && w && grad && loss && n > 0, "invalid argument");
+  *loss = h->impl->loss_and_grad(x, t, w, n, grad);
+  HH_HIP(hipStreamSynchronize(h->impl->stream()));
+  HH_ABI_END
+}
+
+hh_status hh_mlp_train(hh_mlp* h, const float* x, const float* t, const float* w, int64_t n, int32_t epochs, float lr,
+                       double* loss) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && x && t && w && n > 0 && epochs >= 0, "invalid argument");
+  const double l = h->impl->train(x, t, w, n, epochs, lr);
+  if (loss) *loss = l;
+  HH_ABI_END
+}
+
+hh_status hh_mlp_get_params(const hh_mlp* h, float* host_out) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && host_out, "invalid argument");
+  h->impl->get_params(host_out);
+  HH_ABI_END
+}
+
+hh_status hh_mlp_set_params(hh_mlp* h, const float* host_in) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && host_in, "invalid argument");
+  h->impl->set_params(host_in);
+  HH_ABI_END
+}
+
+hh_status hh_rijk_pair_norms(hh_rijk* h, int32_t s, const double* mo_coeff, int64_t nmo, double* pair) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h != nullptr, "handle is null");
+  h->impl->pair_norms(s, mo_coeff, nmo, pair);
+  HH_ABI_END
+}
+
+hh_status hh_cphf_precond_features(hh_cphf* h, const double* pair, const double* r, int64_t nprop, float* x) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && pair && r && x && nprop > 0, "invalid argument");
+  h->impl->precond_features(pair, r, nprop, x);
+  HH_ABI_END
+}
+
+hh_status hh_cphf_precond_target(hh_cphf* h, const double* r, const double* xs, int64_t nprop, float* t, float* w) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && r && xs && t && w && nprop > 0, "invalid argument");
+  h->impl->precond_target(r, xs, nprop, t, w);
+  HH_ABI_END
+}
+
+hh_status hh_cphf_precond_diag(hh_cphf* h, const float* y, double* diag) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && y && diag, "invalid argument");
+  h->impl->precond_diag(y, diag);
   HH_ABI_END
 }
 

@@ -134,6 +134,73 @@ __global__ void axpy_kernel(double* y, const double* x, double alpha, int64_t n)
     y[e] += alpha * x[e];
 }
 
+// raw features (before standardisation) for sample (a, i)
+__global__ void raw_features_kernel(const double* pair, const double* r, const double* eocc, const double* evir,
+                                    int64_t nmo, int64_t nocc, int64_t nprop, float* x) {
+  const int64_t nvir = nmo - nocc, ns = nvir * nocc;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < ns; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t a = e % nvir, i = e / nvir, m = nocc + a;
+    double rr = 0.0;
+    for (int64_t p = 0; p < nprop; ++p) {
+      const double v = r[m + nmo * (i + nocc * p)];
+      rr += v * v;
+    }
+    x[e] = (float)log(fmax(evir[a] - eocc[i], 1e-8));
+    x[e + ns] = (float)log(pair[m + nmo * i] + 1e-30);
+    x[e + 2 * ns] = (float)log(sqrt(rr / (double)nprop) + 1e-30);
+  }
+}
+
+// standardise each of the 3 feature columns in place (one workgroup per column, FP64 sums)
+__global__ __launch_bounds__(256) void standardise_kernel(float* x, int64_t ns) {
+  __shared__ double a[256], b[256];
+  float* col = x + ns * blockIdx.x;
+  double sa = 0.0, sb = 0.0;
+  for (int64_t i = threadIdx.x; i < ns; i += blockDim.x) {
+    sa += col[i];
+    sb += (double)col[i] * col[i];
+  }
+  a[threadIdx.x] = sa;
+  b[threadIdx.x] = sb;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      a[threadIdx.x] += a[threadIdx.x + st];
+      b[threadIdx.x] += b[threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  const double mean = a[0] / (double)ns;
+  const double var = fmax(b[0] / (double)ns - mean * mean, 0.0);
+  const double inv = var > 1e-20 ? 1.0 / sqrt(var) : 1.0;
+  for (int64_t i = threadIdx.x; i < ns; i += blockDim.x) col[i] = (float)((col[i] - mean) * inv);
+}
+
+__global__ void lsq_target_kernel(const double* r, const double* xs, int64_t nmo, int64_t nocc, int64_t nprop,
+                                  float* t, float* w) {
+  const int64_t nvir = nmo - nocc, ns = nvir * nocc;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < ns; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t a = e % nvir, i = e / nvir, m = nocc + a;
+    double rx = 0.0, rr = 0.0;
+    for (int64_t p = 0; p < nprop; ++p) {
+      const int64_t o = m + nmo * (i + nocc * p);
+      rx += r[o] * xs[o];
+      rr += r[o] * r[o];
+    }
+    const double d = rr > 0.0 ? rx / rr : 1.0;
+    t[e] = (float)log(fmin(fmax(d, 0.05), 20.0));
+    w[e] = (float)rr;
+  }
+}
+
+__global__ void precond_diag_kernel(const float* y, int64_t nmo, int64_t nocc, double* diag) {
+  const int64_t nvir = nmo - nocc, total = nmo * nocc;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t m = e % nmo, i = e / nmo;
+    diag[e] = (m >= nocc) ? exp(fmin(fmax((double)y[(m - nocc) + nvir * i], -3.0), 3.0)) : 1.0;
+  }
+}
+
 }  // namespace
 
 Cphf::Cphf(const double* mo_coeff, int64_t nao, int64_t nmo, const double* mo_occ, const double* mo_energy,
@@ -303,6 +370,25 @@ void Cphf::finalize(const double* f1mo, const double* s1mo, const double* mo1, i
   response_mo(mo1, nprop, b1, PREC_F64);
   hipLaunchKernelGGL(finalize_kernel, dim3(nblk(blk)), dim3(256), 0, stream_, f1mo, s1mo, mo1, b1, eocc_.as<double>(),
                      evir_.as<double>(), nmo_, nocc_, nprop, mo1_out, mo_e1);
+  HH_HIP(hipGetLastError());
+}
+
+void Cphf::precond_features(const double* pair, const double* r, int64_t nprop, float* x) {
+  const int64_t ns = (nmo_ - nocc_) * nocc_;
+  hipLaunchKernelGGL(raw_features_kernel, dim3(nblk(ns)), dim3(256), 0, stream_, pair, r, eocc_.as<double>(),
+                     evir_.as<double>(), nmo_, nocc_, nprop, x);
+  hipLaunchKernelGGL(standardise_kernel, dim3(3), dim3(256), 0, stream_, x, ns);
+  HH_HIP(hipGetLastError());
+}
+
+void Cphf::precond_target(const double* r, const double* xs, int64_t nprop, float* t, float* w) {
+  const int64_t ns = (nmo_ - nocc_) * nocc_;
+  hipLaunchKernelGGL(lsq_target_kernel, dim3(nblk(ns)), dim3(256), 0, stream_, r, xs, nmo_, nocc_, nprop, t, w);
+  HH_HIP(hipGetLastError());
+}
+
+void Cphf::precond_diag(const float* y, double* diag) {
+  hipLaunchKernelGGL(precond_diag_kernel, dim3(nblk(nmo_ * nocc_)), dim3(256), 0, stream_, y, nmo_, nocc_, diag);
   HH_HIP(hipGetLastError());
 }
 

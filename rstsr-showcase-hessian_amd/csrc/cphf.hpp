@@ -60,6 +60,16 @@ class Cphf {
 
   hipStream_t stream() const { return stream_; }
 
+  // Learned-preconditioner support (virtual rows only; N = nvir * nocc samples, sample index
+  // a + nvir * i).  Features x [N, 3] (float, column-major), standardised over the samples:
+  // ln(e_a - e_i), ln(pair[a,i]) (RI pair norm (ai|ai)), ln(rms_p |r[a,i,p]|).
+  void precond_features(const double* pair, const double* r, int64_t nprop, float* x);
+  // Least-squares diagonal mapping residual r to solution x over the perturbations:
+  // D*[a,i] = sum_p r x / sum_p r^2; target t = ln(clip(D*, 0.05, 20)), weight w = sum_p r^2 (normalised).
+  void precond_target(const double* r, const double* xs, int64_t nprop, float* t, float* w);
+  // diag [nmo, nocc] (double) = exp(clip(y, -3, 3)) on virtual rows, 1 on occupied rows.
+  void precond_diag(const float* y, double* diag);
+
  private:
   int64_t nao_, nmo_, nocc_;
   CphfConfig cfg_;

@@ -163,6 +163,41 @@ void Rijk::prepare_response(const double* const* mo_coeff, const double* const* 
   mixed_ready_ = false;
 }
 
+namespace {
+// pair[m, i] = sum_P V[m, (P, i)]^2, V [nmo, naux * nocc]
+__global__ void pair_sumsq_kernel(const double* V, int64_t nmo, int64_t naux, int64_t nocc, double* pair) {
+  const int64_t total = nmo * nocc;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t m = e % nmo, i = e / nmo;
+    double s = 0.0;
+    for (int64_t P = 0; P < naux; ++P) {
+      const double v = V[m + nmo * (P + naux * i)];
+      s += v * v;
+    }
+    pair[e] = s;
+  }
+}
+}  // namespace
+
+void Rijk::pair_norms(int s, const double* mo_coeff, int64_t nmo, double* pair) {
+  if (!prepared_) throw Error(HH_ERR_STATE, "pair_norms requested before prepare_response");
+  HH_REQUIRE(s >= 0 && s < nspin_ && mo_coeff && pair && nmo > 0, "pair_norms: bad argument");
+  const int64_t nocc = nocc_[s];
+  DevBuf vb;
+  double* V = static_cast<double*>(vb.reserve(sizeof(double) * nmo * naux_ * nocc));
+  GemmArgs g;  // V[m, (P,i)] = sum_mu C[mu, m] X[mu, (P,i)]
+  g.M = nmo; g.N = naux_ * nocc; g.K = nao_;
+  g.A = mo_coeff; g.lda = nao_; g.a_kmajor = true;
+  g.B = x_[s].ptr; g.ldb = nao_; g.b_kmajor = true;
+  g.C = V; g.ldc = nmo;
+  gemm<double, double>(g, PREC_F64, stream_, ws_);
+  const int64_t total = nmo * nocc;
+  hipLaunchKernelGGL(pair_sumsq_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(total, 256), 65535)), dim3(256), 0,
+                     stream_, V, nmo, naux_, nocc, pair);
+  HH_HIP(hipGetLastError());
+  HH_HIP(hipStreamSynchronize(stream_));
+}
+
 void Rijk::ensure_mixed() {
   if (mixed_ready_) return;
   if (!cderi32_.ptr) {

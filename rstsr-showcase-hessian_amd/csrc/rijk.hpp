@@ -30,6 +30,10 @@ class Rijk {
   // j_ao (unscaled) and k_bra[s] (unscaled, sign included); either may be null.
   void response_bra_separated(const double* const* bra, int64_t nprop, double* j_ao, double* const* k_bra, Prec prec);
 
+  // pair[m, i] = sum_P ((C^T B_P C_o)[m, i])^2 = (mi|mi) in the RI approximation, for all nmo
+  // MOs m of the device coefficients C [nao, nmo] (spin s; needs prepare_response).
+  void pair_norms(int s, const double* mo_coeff, int64_t nmo, double* pair);
+
   int64_t nao() const { return nao_; }
   int64_t naux() const { return naux_; }
   int nspin() const { return nspin_; }

@@ -72,6 +72,17 @@ SIGNATURES.update({
     "hh_linalg_cholesky": (_i32, [_p, _i64, _p]),
     "hh_linalg_trsm_lower": (_i32, [_p, _i64, _p, _i64, _i64, _i32, _p]),
     "hh_util_copy": (_i32, [_p, _p, _i64, _p]),
+    "hh_mlp_create": (_i32, [C.c_uint64, _p, C.POINTER(C.c_void_p)]),
+    "hh_mlp_destroy": (_i32, [_p]),
+    "hh_mlp_forward": (_i32, [_p, _p, _i64, _p]),
+    "hh_mlp_loss_and_grad": (_i32, [_p, _p, _p, _p, _i64, _p, C.POINTER(C.c_double)]),
+    "hh_mlp_train": (_i32, [_p, _p, _p, _p, _i64, _i32, C.c_float, C.POINTER(C.c_double)]),
+    "hh_mlp_get_params": (_i32, [_p, C.POINTER(C.c_float)]),
+    "hh_mlp_set_params": (_i32, [_p, C.POINTER(C.c_float)]),
+    "hh_rijk_pair_norms": (_i32, [_p, _i32, _p, _i64, _p]),
+    "hh_cphf_precond_features": (_i32, [_p, _p, _p, _i64, _p]),
+    "hh_cphf_precond_target": (_i32, [_p, _p, _p, _i64, _p, _p]),
+    "hh_cphf_precond_diag": (_i32, [_p, _p, _p]),
     "hh_cphf_add_numint": (_i32, [_p, _p, _i32, _p]),
     "hh_numint_create": (_i32, [_p, _i64, _i64, _i64, _p, _p, C.POINTER(C.c_void_p)]),
     "hh_numint_destroy": (_i32, [_p]),
