@@ -226,6 +226,7 @@ def main():
         obj2 = hesship.RHessRIJK(d_cderi * scale, 1.0, 1.0)
         obj2.make_response_preparation(d_C, occ)
         drv, e, f1, s1, ratio2 = cphf_case(obj2, d_C, occ, nao, nocc, nprop, dev)
+        drv.set_verbose(True)
         res = {"natm_equiv": nprop // 3, "cphf_tol": 1e-8, "response_ratio": round(ratio2, 4)}
         ref = None
         for prec in ("fp64", args.precision):

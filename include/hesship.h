@@ -178,6 +178,9 @@ hh_status hh_cphf_create(const double* mo_coeff, int64_t nao, int64_t nmo, const
                          const double* mo_energy, double level_shift, double tol, int32_t max_cycle,
                          int32_t max_space, double lindep, void* stream, hh_cphf** out);
 hh_status hh_cphf_destroy(hh_cphf* h);
+/* Per-sweep / per-cycle diagnostics on stderr (the reference prints per Krylov cycle,
+ * krylov_block.rs:148-151). */
+hh_status hh_cphf_set_verbose(hh_cphf* h, int32_t on);
 /* Add an RI-JK component (must be RHF, prepared with the same MOs) to the response list
  * (the reference's el_list, rscf.rs:28).  The component must outlive the driver. */
 hh_status hh_cphf_add_rijk(hh_cphf* h, hh_rijk* rijk);

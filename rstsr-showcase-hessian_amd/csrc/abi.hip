@@ -237,6 +237,13 @@ hh_status hh_cphf_destroy(hh_cphf* h) {
   HH_ABI_END
 }
 
+hh_status hh_cphf_set_verbose(hh_cphf* h, int32_t on) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h != nullptr, "handle is null");
+  h->impl->verbose = on != 0;
+  HH_ABI_END
+}
+
 hh_status hh_cphf_add_rijk(hh_cphf* h, hh_rijk* rijk) {
   HH_ABI_BEGIN
   HH_REQUIRE(h && rijk, "null handle");

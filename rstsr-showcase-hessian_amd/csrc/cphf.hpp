@@ -59,6 +59,7 @@ class Cphf {
                  double* de);
 
   hipStream_t stream() const { return stream_; }
+  bool verbose = false;  // per-sweep diagnostics on stderr (the reference prints per cycle)
 
   // Learned-preconditioner support (virtual rows only; N = nvir * nocc samples, sample index
   // a + nvir * i).  Features x [N, 3] (float, column-major), standardised over the samples:

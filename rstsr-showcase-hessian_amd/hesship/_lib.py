@@ -84,6 +84,7 @@ SIGNATURES.update({
     "hh_cphf_precond_target": (_i32, [_p, _p, _p, _i64, _p, _p]),
     "hh_cphf_precond_diag": (_i32, [_p, _p, _p]),
     "hh_cphf_add_numint": (_i32, [_p, _p, _i32, _p]),
+    "hh_cphf_set_verbose": (_i32, [_p, _i32]),
     "hh_numint_create": (_i32, [_p, _i64, _i64, _i64, _p, _p, C.POINTER(C.c_void_p)]),
     "hh_numint_destroy": (_i32, [_p]),
     "hh_numint_set_batch": (_i32, [_p, _i64]),

@@ -40,6 +40,9 @@ class RHessCPHF:
                 pass
             self._h = None
 
+    def set_verbose(self, on=True):
+        check(lib().hh_cphf_set_verbose(self._h, int(bool(on))))
+
     def add_el(self, rijk):
         """Append an RHF RI-JK component (prepared) to the response list."""
         check(lib().hh_cphf_add_rijk(self._h, rijk._h))
