@@ -25,6 +25,12 @@ for _p in (ROOT, PKG):
 
 METRIC = "CPHF+Hessian wall s & RI-K TFLOP/s: mixed+learned vs plain FP64, 1 MI355X"
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}  # MI355X dense matrix peaks
+# HBM bytes per launch of the hot kernel, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per
+# dispatch) and scaled by the same counters on a 1 GiB copy16_kernel copy (factors 1.99994 and 1.00000);
+# profiles/nt_bf16x3_kernel_ri_k_nao400_naux2400_nocc40_nprop90_hbm_traffic.txt.  Keyed by
+# (precision, nao, naux, nocc, nprop); other workloads report null.
+MEASURED_TRAFFIC = {("bf16x3", 400, 2400, 40, 90): 20371236.8 * 1024 * 1.99994 + 91540.0 * 1024 * 1.00000}
+HOT_KERNEL = {"fp64": "nt_f64_kernel", "fp32": "gemm_kernel<float,double>", "bf16x3": "nt_bf16x3_kernel"}
 SIZES = {
     "full": dict(nao=400, naux=2400, nocc=40, nprop=90),
     "small": dict(nao=96, naux=480, nocc=12, nprop=18),
@@ -118,6 +124,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cphf", action="store_true")
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
+                    help="only run the library's 16-byte copy kernel over GIB GiB three times (the reference "
+                         "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the RI-JK response of one molecule runs on "
@@ -129,6 +138,19 @@ def main():
 
     import hesship
     from hesship import synth
+
+    if args.copy_calibration > 0:
+        nbytes = int(args.copy_calibration * 2**30) // 16 * 16
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        stream = torch.cuda.current_stream().cuda_stream
+        for _ in range(3):
+            hesship._lib.check(hesship._lib.lib().hh_util_copy(dst.data_ptr(), src.data_ptr(), nbytes, stream))
+        torch.cuda.synchronize()
+        assert bool((dst == 1).all())
+        print(json.dumps({"copy_calibration_bytes_read": nbytes, "copy_calibration_bytes_written": nbytes,
+                          "launches": 3, "kernel": "copy16_kernel"}))
+        return
 
     sz = SIZES[args.size]
     nao, naux, nocc, nprop = sz["nao"], sz["naux"], sz["nocc"], sz["nprop"]
@@ -213,8 +235,9 @@ def main():
         "max_abs_err_vs_fp64": diff,
         "rel_err_vs_fp64": diff / scale,
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-                     "frac": round(achieved / peak, 4), "traffic": None,
-                     "kernel": "gemm_kernel (hot K contraction, one aux chunk per launch)",
+                     "frac": round(achieved / peak, 4),
+                     "traffic": MEASURED_TRAFFIC.get((args.precision, nao, naux, nocc, nprop)),
+                     "kernel": HOT_KERNEL[args.precision] + " (hot K contraction, one perturbation chunk per launch)",
                      "launches_per_step": nl, "avg_launch_ms": round(t_launch * 1e3, 4)},
     }
     if not args.no_cphf:

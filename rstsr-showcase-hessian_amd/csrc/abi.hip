@@ -530,7 +530,7 @@ hh_status hh_util_gemm(int64_t M, int64_t N, int64_t K, double alpha, const doub
 hh_status hh_util_copy(void* dst, const void* src, int64_t bytes, void* stream) {
   HH_ABI_BEGIN
   HH_REQUIRE(bytes >= 0 && (bytes == 0 || (dst && src)), "invalid argument");
-  if (bytes > 0) HH_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  copy_bytes(dst, src, bytes, (hipStream_t)stream);
   HH_ABI_END
 }
 

@@ -58,6 +58,12 @@ __global__ void fill_kernel(T* __restrict__ out, int64_t n, T v) {
     out[t] = v;
 }
 
+// 16-byte vector copy, grid-stride; every byte is read once and written once.
+__global__ void copy16_kernel(const uint4* __restrict__ src, int64_t n16, uint4* __restrict__ dst) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n16; e += (int64_t)gridDim.x * blockDim.x)
+    dst[e] = src[e];
+}
+
 template <typename TI, typename TO>
 void unpack_triu(const TI* tp, int64_t nao, int64_t ld_tp, int64_t nmat, TO* out, hipStream_t s) {
   const int64_t total = nao * nao * nmat;
@@ -85,6 +91,21 @@ template <typename T>
 void fill(T* out, int64_t n, T v, hipStream_t s) {
   if (n == 0) return;
   hipLaunchKernelGGL((fill_kernel<T>), dim3(grid_for(n)), dim3(TPB), 0, s, out, n, v);
+  HH_HIP(hipGetLastError());
+}
+
+void copy_bytes(void* dst, const void* src, int64_t bytes, hipStream_t s) {
+  if (bytes <= 0) return;
+  const bool vec = (reinterpret_cast<uintptr_t>(dst) % 16 == 0) && (reinterpret_cast<uintptr_t>(src) % 16 == 0) &&
+                   (bytes % 16 == 0);
+  if (!vec) {
+    HH_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice, s));
+    return;
+  }
+  const int64_t n16 = bytes / 16;
+  const int64_t nb = std::min<int64_t>((n16 + 255) / 256, 8192);
+  hipLaunchKernelGGL(copy16_kernel, dim3((unsigned)nb), dim3(256), 0, s, static_cast<const uint4*>(src), n16,
+                     static_cast<uint4*>(dst));
   HH_HIP(hipGetLastError());
 }
 

@@ -14,6 +14,9 @@ void pack_triu_tilde(const double* dm, int64_t nao, int64_t nmat, bool symmetriz
 template <typename TI, typename TO>
 void cast(const TI* in, int64_t n, TO* out, hipStream_t s);
 
+// Device-to-device copy; a 16-byte vector kernel when both pointers and the size are 16-byte aligned.
+void copy_bytes(void* dst, const void* src, int64_t bytes, hipStream_t s);
+
 // Fill with a kernel (never hipMemsetAsync, so the call is safe inside captured graphs).
 template <typename T>
 void fill(T* out, int64_t n, T v, hipStream_t s);

@@ -358,10 +358,15 @@ void Rijk::build_k(int s, const double* bra, int64_t nprop, Prec prec, double al
     if (f64) gemm<double, double>(g, prec, stream_, ws_);
     else gemm<float, float>(g, prec, stream_, ws_);
   }
-  // perturbation chunk: T[(i,a)][P][kpad] within ~12 GiB
+  // perturbation chunk: the caller's batch, else equal chunks with T[(i,a)][P][kpad] within ~16 GiB
+  // (bf16 hi + lo, FP32 and FP64 elements: 4, 4 and 8 bytes), so every hot launch has one shape
   const int64_t per_a = nocc * naux_ * kpad_;
   int64_t ac = k_chunk;
-  if (ac <= 0) ac = std::max<int64_t>(1, (int64_t)((12LL << 30) / (8 * per_a)));
+  if (ac <= 0) {
+    ac = std::min(nprop, std::max<int64_t>(1, (int64_t)((16LL << 30) / ((f64 ? 8 : 4) * per_a))));
+    const int64_t nchunk = (nprop + ac - 1) / ac;
+    ac = (nprop + nchunk - 1) / nchunk;
+  }
   ac = std::min(ac, nprop);
   // rows nao..kpad of every (n, P) segment stay zero: the generation GEMM writes mu < nao only,
   // so the buffers are cleared once per layout
