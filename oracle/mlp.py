@@ -48,3 +48,19 @@ def loss_and_grad(p, x, t, w):
     loss = (w * (y - t) ** 2).sum() / w.sum()
     loss.backward()
     return loss.item(), pp.grad.numpy()
+
+
+def adam_train(p, x, t, w, epochs, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+    """Full-batch Adam (Kingma & Ba 2015, bias-corrected moments) from zero moments, as the
+    library's hh_mlp_train: m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2,
+    p -= lr (m / (1-b1^k)) / (sqrt(v / (1-b2^k)) + eps).  Returns (params, loss of the last step)."""
+    p = np.asarray(p, dtype=np.float64).copy()
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    loss = 0.0
+    for k in range(1, epochs + 1):
+        loss, g = loss_and_grad(p, x, t, w)
+        m = beta1 * m + (1 - beta1) * g
+        v = beta2 * v + (1 - beta2) * g * g
+        p -= lr * (m / (1 - beta1 ** k)) / (np.sqrt(v / (1 - beta2 ** k)) + eps)
+    return p, loss

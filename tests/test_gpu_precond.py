@@ -31,6 +31,35 @@ def test_forward_and_gradient_match_oracle(dev):
     assert np.abs(g - g_ref).max() <= 1e-4 * np.abs(g_ref).max()
 
 
+def test_adam_steps_match_oracle(dev):
+    """Ten library Adam steps (hh_mlp_train) against the oracle's Adam from the same parameters."""
+    from hesship.precond import DiagMLP
+    rng = np.random.default_rng(4)
+    n = 1500
+    x = rng.standard_normal((n, 3)).astype(np.float32)
+    t = rng.standard_normal(n).astype(np.float32)
+    w = rng.uniform(0.1, 1.0, n).astype(np.float32)
+    net = DiagMLP(seed=9)
+    p0 = net.params()
+    net.set_params(p0)  # zero moments, step counter 0
+    lr, steps = 3e-4, 10
+    loss = net.train(T(x, dev), T(t, dev), T(w, dev), epochs=steps, lr=lr)
+    p1 = net.params().astype(np.float64)
+    p_ref, loss_ref = omlp.adam_train(p0, x, t, w, epochs=steps, lr=lr)
+    assert abs(loss - loss_ref) <= 1e-4 * abs(loss_ref)
+    # Adam moves every parameter by ~lr per step whatever its gradient's size, so parameters whose
+    # gradient is at the FP32 rounding level wander: bound all moves, and require the firmly
+    # determined ones (|g| above 1e-2 of the largest) to agree.  The tolerance sits between
+    # FP32-level gradient noise (~4e-5 at 3e-5 of max|g|) and formula mistakes of the same test
+    # (no bias correction: 1.3e-2; beta1 0.8: 3.8e-4), both measured with the oracle.
+    _, g_ref = omlp.loss_and_grad(p0, x, t, w)
+    d = np.abs(p1 - p_ref)
+    assert d.max() <= 4 * lr * steps
+    firm = np.abs(g_ref) > 1e-2 * np.abs(g_ref).max()
+    assert firm.sum() >= 20
+    assert d[firm].max() <= 2e-4, d[firm].max()
+
+
 def test_training_reduces_loss(dev):
     from hesship.precond import DiagMLP
     rng = np.random.default_rng(2)
