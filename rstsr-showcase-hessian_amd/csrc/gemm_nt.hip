@@ -44,7 +44,7 @@ __device__ __forceinline__ int64_t remap_xcd(int64_t w, int64_t nwg) {
 }
 
 // ---------------------------------------------------------------- bf16 x3 ----
-template <int WGM, int WGN, int FM, int FN>
+template <int WGM, int WGN, int FM, int FN, bool TILED>
 __global__ __launch_bounds__(512) void nt_bf16x3_kernel(NtParams p) {
   constexpr int BM = WGM * FM * 16, BN = WGN * FN * 16, BK = 32;
   constexpr int LD = BK + 8;                 // bf16 elements per LDS row (80 B)
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(512) void nt_bf16x3_kernel(NtParams p) {
         const int r = c / (BK / 8), kk = (c % (BK / 8)) * 8;
         const int64_t gm = m0 + r;
         if (gm < p.M) {
-          const int64_t off = gm * p.lda + k0 + kk;
+          const int64_t off = TILED ? ((k0 >> 5) * p.M + gm) * 32 + kk : gm * p.lda + k0 + kk;
           vh = *reinterpret_cast<const uint4*>(Ah + off);
           vl = *reinterpret_cast<const uint4*>(Al + off);
         }
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(512) void nt_bf16x3_kernel(NtParams p) {
         const int r = c / (BK / 8), kk = (c % (BK / 8)) * 8;
         const int64_t gn = n0 + r;
         if (gn < p.N) {
-          const int64_t off = gn * p.ldb + k0 + kk;
+          const int64_t off = TILED ? ((k0 >> 5) * p.N + gn) * 32 + kk : gn * p.ldb + k0 + kk;
           vh = *reinterpret_cast<const uint4*>(Bh + off);
           vl = *reinterpret_cast<const uint4*>(Bl + off);
         }
@@ -360,7 +360,7 @@ int cu_count() {
   return n;
 }
 
-template <int WGM, int WGN, int FM, int FN, bool BF>
+template <int WGM, int WGN, int FM, int FN, bool BF, bool TILED = false>
 void launch_nt(const NtArgs& a, hipStream_t s, DevBuf& ws) {
   constexpr int BM = WGM * FM * 16, BN = WGN * FN * 16, BK = BF ? 32 : 16;
   NtParams p{};
@@ -387,7 +387,7 @@ void launch_nt(const NtArgs& a, hipStream_t s, DevBuf& ws) {
   HH_REQUIRE(p.nwg < (1LL << 31), "gemm_nt: grid too large");
   if (a.ev_begin) HH_HIP(hipEventRecord(a.ev_begin, s));
   if (BF)
-    hipLaunchKernelGGL((nt_bf16x3_kernel<WGM, WGN, FM, FN>), dim3((unsigned)p.nwg), dim3(512), 0, s, p);
+    hipLaunchKernelGGL((nt_bf16x3_kernel<WGM, WGN, FM, FN, TILED>), dim3((unsigned)p.nwg), dim3(512), 0, s, p);
   else
     hipLaunchKernelGGL((nt_f64_kernel<WGM, WGN, FM, FN>), dim3((unsigned)p.nwg), dim3(512), 0, s, p);
   HH_HIP(hipGetLastError());
@@ -411,8 +411,10 @@ void gemm_nt(const NtArgs& a, NtFormat fmt, hipStream_t s, DevBuf& ws) {
   if (fmt == NT_BF16X3) {
     HH_REQUIRE(a.K == 0 || (a.A_lo && a.B_lo), "gemm_nt: bf16x3 needs lo arrays");
     // 208 x 256 tiles: M = 400 is two tiles with 4 % padding
-    launch_nt<1, 8, 13, 2, true>(a, s, ws);
+    if (a.tiled) launch_nt<1, 8, 13, 2, true, true>(a, s, ws);
+    else launch_nt<1, 8, 13, 2, true, false>(a, s, ws);
   } else {
+    HH_REQUIRE(!a.tiled, "gemm_nt: tiled operands are BF16X3 only");
     launch_nt<1, 8, 13, 1, false>(a, s, ws);
   }
 }

@@ -3,7 +3,10 @@
 //   C[m, n] = alpha * sum_k A[m*lda + k] * B[n*ldb + k] + beta * C[m + n*ldc]     (C column-major)
 //
 // K must be a multiple of 32 and lda, ldb multiples of 8 (16-byte rows); operands are padded
-// by their producers.  Element formats:
+// by their producers.  With `tiled` set (BF16X3 only) both operands are stored in K blocks of 32:
+// A[((k / 32) * M + m) * 32 + k % 32], B[((k / 32) * N + n) * 32 + k % 32] (lda, ldb unused), so
+// each K step of a workgroup reads one contiguous run of 64 bytes per row for all its rows.
+// Element formats:
 //   NT_F64    : A, B double; v_mfma_f64_16x16x4f64, FP64 accumulate.
 //   NT_BF16X3 : A, B given as two bf16 arrays each (hi and lo = x - hi); hi*hi + hi*lo + lo*hi
 //               on v_mfma_f32_16x16x32_bf16, FP32 accumulate; split-K partials summed in FP64.
@@ -25,6 +28,7 @@ struct NtArgs {
   double* C = nullptr;
   int64_t ldc = 0;
   double alpha = 1.0, beta = 0.0;
+  bool tiled = false;  // K-block tiled operands (BF16X3)
   int ksplit = 0;  // 0: automatic
   hipEvent_t ev_begin = nullptr, ev_main_end = nullptr;
 };

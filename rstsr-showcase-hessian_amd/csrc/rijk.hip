@@ -12,14 +12,17 @@
 //                                                      (one GEMM per a, K = 2 nocc concatenated)
 //   K[u,(i,a)]     = sum_{(mu,P)} B[u,(mu,P)] T[(mu,P),(i,a)]         (K = nao naux, the hot GEMM)
 // X[mu,(P,i)] = (B_P C_o)[mu,i] and O[j,(P,i)] = (C_o^T B_P C_o)[j,i] depend only on the MO
-// coefficients and are built once per prepare_response.  The hot GEMM reads B and T in a packed
-// layout with mu padded to kpad (a multiple of 32) and contiguous: B as [u][P][mu], T as
-// [(i,a)][P][mu].  Perturbations are processed in chunks so T stays a bounded buffer.  Flop count
-// equals the reference's: 2 nao^2 naux nocc nprop + 6 nao naux nocc^2 nprop (+ X/O setup).
+// coefficients and are built once per prepare_response.  The hot GEMM reads B and T with the
+// K index k = mu + kpad P (mu padded to kpad, a multiple of 32): FP64 / FP32 as rows [u][P][mu]
+// and [(i,a)][P][mu]; BF16x3 as bf16 hi / lo pairs in K blocks of 32, [k/32][u][k%32] and
+// [k/32][(i,a)][k%32], so every K step of the hot GEMM and every row group of the T generation
+// touches contiguous memory.  Perturbations are processed in chunks so T stays a bounded buffer.
+// Flop count equals the reference's: 2 nao^2 naux nocc nprop + 6 nao naux nocc^2 nprop (+ X/O).
 #include "rijk.hpp"
 
 #include "elementwise.hpp"
 #include "linalg.hpp"
+#include "rijk_tgen.hpp"
 
 namespace hh {
 
@@ -288,6 +291,26 @@ __global__ void pack_hot_a_kernel(const double* __restrict__ cderi, int64_t nao,
   }
 }
 
+// BF16x3 hot operand in K blocks of 32 (k = mu + kpad P, kbp = kpad / 32):
+// hi/lo[((P kbp + mu / 32) nao + u) 32 + mu % 32] = split(float(B_P[u, mu])), 0 for mu >= nao
+__global__ void pack_hot_a_bf16_kernel(const double* __restrict__ cderi, int64_t nao, int64_t naux, int64_t kbp,
+                                       unsigned short* __restrict__ hi, unsigned short* __restrict__ lo) {
+  const int64_t total = naux * kbp * nao * 32;
+  const int64_t ntp = nao * (nao + 1) / 2;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t kl = e % 32, u = (e / 32) % nao, kb = e / (32 * nao);
+    const int64_t P = kb / kbp, mu = (kb % kbp) * 32 + kl;
+    float x = 0.f;
+    if (mu < nao) {
+      const int64_t l = mu < u ? mu : u, h = mu < u ? u : mu;
+      x = (float)cderi[h * (h + 1) / 2 + l + P * ntp];
+    }
+    const __bf16 xh = (__bf16)x;
+    hi[e] = __builtin_bit_cast(unsigned short, xh);
+    lo[e] = __builtin_bit_cast(unsigned short, (__bf16)(x - (float)xh));
+  }
+}
+
 // zero rows [nao, kpad) of every (n, Pl) segment of T[n][Pl][kpad]
 __global__ void zero_pad_rows_kernel(double* T, int64_t nseg, int64_t nao, int64_t kpad) {
   const int64_t w = kpad - nao;
@@ -316,17 +339,11 @@ void Rijk::ensure_packed(Prec prec) {
     HH_HIP(hipGetLastError());
   }
   if (bf16 && !bp_hi_.ptr) {
-    // pack to an FP32 staging copy, then split every row into hi / lo
     bp_hi_.reserve(sizeof(unsigned short) * n);
     bp_lo_.reserve(sizeof(unsigned short) * n);
-    DevBuf full;
-    float* f = static_cast<float*>(full.reserve(sizeof(float) * n));
-    hipLaunchKernelGGL((pack_hot_a_kernel<float>), dim3(ew_blocks(n)), dim3(256), 0, stream_, cderi_.as<double>(), nao_,
-                       naux_, kpad_, f);
+    hipLaunchKernelGGL(pack_hot_a_bf16_kernel, dim3(ew_blocks(n)), dim3(256), 0, stream_, cderi_.as<double>(), nao_,
+                       naux_, kpad_ / 32, bp_hi_.as<unsigned short>(), bp_lo_.as<unsigned short>());
     HH_HIP(hipGetLastError());
-    split_bf16_rows<float>(f, nao_ * naux_, kpad_, kpad_, bp_hi_.as<unsigned short>(), bp_lo_.as<unsigned short>(),
-                           kpad_, stream_);
-    HH_HIP(hipStreamSynchronize(stream_));
   }
 }
 
@@ -334,6 +351,7 @@ void Rijk::build_k(int s, const double* bra, int64_t nprop, Prec prec, double al
   const int64_t nocc = nocc_[s];
   const int64_t nP = naux_ * nocc;  // (P, i) columns
   const bool f64 = (prec == PREC_F64), bf = (prec == PREC_BF16X3);
+  HH_REQUIRE(!bf || tgen_supported(nocc), "BF16x3 RI-K needs nocc <= 128 (use FP32 or FP64)");
   ensure_packed(prec);
   if (!f64) ensure_mixed();
   const size_t esz = f64 ? sizeof(double) : sizeof(float);
@@ -368,8 +386,9 @@ void Rijk::build_k(int s, const double* bra, int64_t nprop, Prec prec, double al
     ac = (nprop + nchunk - 1) / nchunk;
   }
   ac = std::min(ac, nprop);
-  // rows nao..kpad of every (n, P) segment stay zero: the generation GEMM writes mu < nao only,
-  // so the buffers are cleared once per layout
+  // FP32 / FP64: rows nao..kpad of every (n, P) segment stay zero (the generation GEMM writes
+  // mu < nao only), so the buffer is cleared once per layout.  BF16x3: the generation kernel
+  // writes every row of the K-block tiled layout, padding included.
   void* T = nullptr;
   unsigned short* th = nullptr;
   unsigned short* tl = nullptr;
@@ -377,12 +396,6 @@ void Rijk::build_k(int s, const double* bra, int64_t nprop, Prec prec, double al
   if (bf) {
     th = static_cast<unsigned short*>(tp_hi_.reserve(sizeof(unsigned short) * tlen));
     tl = static_cast<unsigned short*>(tp_lo_.reserve(sizeof(unsigned short) * tlen));
-    if (tp_key_[0] != 2 || tp_key_[1] != tlen) {
-      fill<unsigned short>(th, tlen, 0, stream_);
-      fill<unsigned short>(tl, tlen, 0, stream_);
-      tp_key_[0] = 2;
-      tp_key_[1] = tlen;
-    }
   } else {
     T = tp_.reserve(esz * tlen);
     if (tp_key_[0] != (int64_t)esz || tp_key_[1] != tlen) {
@@ -394,7 +407,11 @@ void Rijk::build_k(int s, const double* bra, int64_t nprop, Prec prec, double al
   }
   for (int64_t a0 = 0; a0 < nprop; a0 += ac) {
     const int64_t na = std::min(ac, nprop - a0);
-    {
+    if (bf) {
+      // T[k = mu + kpad P][n = i + nocc al] = bra_a O + C_o W_a as the bf16 pair, K-block tiled
+      tgen_bf16x3(static_cast<const float*>(braX), static_cast<const float*>(Co), static_cast<const float*>(O),
+                  static_cast<const float*>(W), nao_, nocc, nprop, nP, a0, na, kpad_, th, tl, gp_, stream_);
+    } else {
       // T[mu + kpad*(P + naux*(i + nocc*al))] = bra_a O + C_o W_a   (one GEMM per a, K = 2 nocc)
       GemmArgs g;
       g.M = nao_; g.N = nP; g.K = 2 * nocc; g.K1 = nocc;
@@ -406,12 +423,7 @@ void Rijk::build_k(int s, const double* bra, int64_t nprop, Prec prec, double al
       g.ldc = kpad_; g.sC0 = per_a;
       g.nb0 = na;
       g.ksplit = 1;
-      if (bf) {
-        g.C = th;  // hi / lo written directly by the epilogue
-        g.C_lo = tl;
-      } else {
-        g.C = T;
-      }
+      g.C = T;
       if (f64) gemm<double, double>(g, prec, stream_, ws_);
       else gemm<float, float>(g, prec, stream_, ws_);
     }
@@ -444,6 +456,7 @@ void Rijk::build_k(int s, const double* bra, int64_t nprop, Prec prec, double al
     if (bf) {
       a.A = bp_hi_.as<unsigned short>(); a.A_lo = bp_lo_.as<unsigned short>();
       a.B = th; a.B_lo = tl;
+      a.tiled = true;
       gemm_nt(a, NT_BF16X3, stream_, ws_);
     } else {
       a.A = bp64_.as<double>();

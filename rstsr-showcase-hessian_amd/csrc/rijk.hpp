@@ -89,6 +89,7 @@ class Rijk {
   int64_t kpad_ = 0;
   DevBuf bp64_, bp32_, bp_hi_, bp_lo_;  // packed hot-GEMM A operand
   DevBuf tp_, tp_hi_, tp_lo_;           // packed hot-GEMM B operand (one perturbation chunk)
+  DevBuf gp_;                           // [bra_a | C_o] packed for the BF16x3 T generation
   int64_t tp_key_[2] = {0, 0};          // (element size, length) of tp_ when its padding was zeroed
   DevBuf cderi32_;    // f32 copies for the mixed-precision modes
   DevBuf bfull32_;

@@ -102,6 +102,25 @@ def test_rhf_response_mixed_precision(dev, prec, tol):
     assert 1e-13 < e < tol, e
 
 
+# BF16x3 operand generation: nocc % 8 != 0 (element loads) and == 0 (vector loads), several
+# 64-row groups with a 16-row tail, 2 nocc > 128 (two-column-tile variant), column counts
+# naux*nocc that are not multiples of the workgroup's columns, and batches starting at a0 > 0
+@pytest.mark.parametrize("nao,naux,nocc,nprop,batch", [
+    (37, 90, 5, 7, 0), (70, 64, 24, 5, 0), (90, 40, 70, 3, 0), (45, 33, 16, 9, 4), (130, 50, 13, 4, 3)])
+def test_rhf_response_bf16x3_shapes(dev, nao, naux, nocc, nprop, batch):
+    from hesship import RHessRIJK
+    rng, cderi, C, occ = system(nao, naux, nocc, seed=nao + nocc)
+    bra = rng.standard_normal((nao, nocc, nprop))
+    obj = RHessRIJK(T(cderi, dev), 1.0, 1.0)
+    if batch:
+        obj.set_batch(batch)
+    obj.make_response_preparation(T(C, dev), occ)
+    got = obj.get_response_bra(T(bra, dev), precision="bf16x3").cpu().numpy()
+    ref = orc.rhf_response_bra(cderi, C, occ, bra, 1.0, 1.0)
+    e = rel(got, ref)
+    assert 1e-13 < e < 1e-4, e
+
+
 def test_zero_bra_gives_zero(dev):
     from hesship import RHessRIJK
     rng, cderi, C, occ = system(12, 20, 3, seed=5)
