@@ -34,6 +34,7 @@ struct NtParams {
   int64_t nwg;
   double* partial;  // [ksplit][N][M]
   double* C;
+  float* Cf;  // non-null: FP32 output (C unused)
   int64_t ldc;
   double alpha, beta;
 };
@@ -171,6 +172,33 @@ __global__ __launch_bounds__(512) void nt_bf16x3_kernel(NtParams p) {
   // epilogue: D[4*(l/16) + r][l%16]
   const int64_t col0 = n0 + wn * FN * 16 + (lane & 15);
   const int64_t row0 = m0 + wm * FM * 16 + 4 * (lane >> 4);
+  if (p.Cf && p.ksplit == 1) {
+    // FP32 output: the lane's 4 consecutive rows are one 16-byte store where they fit
+    const bool vec = (p.ldc % 4) == 0 && p.beta == 0.0;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int64_t gm = row0 + i * 16, gn = col0 + j * 16;
+        if (gn >= p.N) continue;
+        float* c = p.Cf + gm + gn * p.ldc;
+        if (vec && gm + 3 < p.M) {
+          *reinterpret_cast<float4*>(c) =
+              make_float4((float)(p.alpha * acc[i][j][0]), (float)(p.alpha * acc[i][j][1]),
+                          (float)(p.alpha * acc[i][j][2]), (float)(p.alpha * acc[i][j][3]));
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            if (gm + r < p.M) {
+              double v = p.alpha * (double)acc[i][j][r];
+              if (p.beta != 0.0) v += p.beta * (double)c[r];
+              c[r] = (float)v;
+            }
+          }
+        }
+      }
+    return;
+  }
   double* P = p.partial + (int64_t)kz * p.M * p.N;
 #pragma unroll
   for (int i = 0; i < FM; ++i)
@@ -321,31 +349,38 @@ __global__ __launch_bounds__(512) void nt_f64_kernel(NtParams p) {
 }
 
 __global__ __launch_bounds__(256) void nt_reduce_kernel(const double* __restrict__ partial, int ksplit, int64_t M,
-                                                        int64_t N, double* C, int64_t ldc, double alpha, double beta) {
+                                                        int64_t N, double* C, float* Cf, int64_t ldc, double alpha,
+                                                        double beta) {
   const int64_t MN = M * N;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < MN; e += (int64_t)gridDim.x * blockDim.x) {
     double s = 0.0;
     for (int z = 0; z < ksplit; ++z) s += partial[(int64_t)z * MN + e];
     const int64_t m = e % M, n = e / M;
-    double* c = C + m + n * ldc;
     double v = alpha * s;
-    if (beta != 0.0) v += beta * (*c);
-    *c = v;
+    if (Cf) {
+      float* c = Cf + m + n * ldc;
+      if (beta != 0.0) v += beta * (double)(*c);
+      *c = (float)v;
+    } else {
+      double* c = C + m + n * ldc;
+      if (beta != 0.0) v += beta * (*c);
+      *c = v;
+    }
   }
 }
 
+// tiled[((r / 32) * ncol + n) * 32 + r % 32] = split(float(src[r + ld n])), 0 for r >= nrow
 template <typename T>
-__global__ void split_bf16_rows_kernel(const T* __restrict__ in, int64_t rows, int64_t cols, int64_t ld_in,
-                                       u16* __restrict__ hi, u16* __restrict__ lo, int64_t ld_out) {
-  const int64_t total = rows * ld_out;
+__global__ void split_bf16_tiled_kernel(const T* __restrict__ src, int64_t ld, int64_t nrow, int64_t ncol,
+                                        int64_t nkb, u16* __restrict__ hi, u16* __restrict__ lo) {
+  const int64_t total = nkb * ncol * 32;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t r = e / ld_out, c = e % ld_out;
-    float x = 0.f;
-    if (c < cols) x = (float)in[r * ld_in + c];
+    const int64_t kl = e % 32, n = (e / 32) % ncol, kb = e / (32 * ncol);
+    const int64_t r = kb * 32 + kl;
+    const float x = r < nrow ? (float)src[r + ld * n] : 0.f;
     const __bf16 h = (__bf16)x;
-    const __bf16 l = (__bf16)(x - (float)h);
     hi[e] = __builtin_bit_cast(u16, h);
-    lo[e] = __builtin_bit_cast(u16, l);
+    lo[e] = __builtin_bit_cast(u16, (__bf16)(x - (float)h));
   }
 }
 
@@ -367,7 +402,7 @@ void launch_nt(const NtArgs& a, hipStream_t s, DevBuf& ws) {
   p.M = a.M; p.N = a.N; p.K = a.K;
   p.A = a.A; p.Alo = a.A_lo; p.lda = a.lda;
   p.B = a.B; p.Blo = a.B_lo; p.ldb = a.ldb;
-  p.C = a.C; p.ldc = a.ldc; p.alpha = a.alpha; p.beta = a.beta;
+  p.C = a.C; p.Cf = a.C_f32; p.ldc = a.ldc; p.alpha = a.alpha; p.beta = a.beta;
   p.tiles_m = ceil_div(a.M, BM);
   p.tiles_n = ceil_div(a.N, BN);
   const int64_t tiles = p.tiles_m * p.tiles_n;
@@ -386,7 +421,7 @@ void launch_nt(const NtArgs& a, hipStream_t s, DevBuf& ws) {
   if (ks > 1) p.partial = static_cast<double*>(ws.reserve(sizeof(double) * (size_t)ks * a.M * a.N));
   HH_REQUIRE(p.nwg < (1LL << 31), "gemm_nt: grid too large");
   if (a.ev_begin) HH_HIP(hipEventRecord(a.ev_begin, s));
-  if (BF)
+  if constexpr (BF)
     hipLaunchKernelGGL((nt_bf16x3_kernel<WGM, WGN, FM, FN, TILED>), dim3((unsigned)p.nwg), dim3(512), 0, s, p);
   else
     hipLaunchKernelGGL((nt_f64_kernel<WGM, WGN, FM, FN>), dim3((unsigned)p.nwg), dim3(512), 0, s, p);
@@ -394,7 +429,7 @@ void launch_nt(const NtArgs& a, hipStream_t s, DevBuf& ws) {
   if (a.ev_main_end) HH_HIP(hipEventRecord(a.ev_main_end, s));
   if (ks > 1) {
     const unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(a.M * a.N, 256), 8192);
-    hipLaunchKernelGGL(nt_reduce_kernel, dim3(blocks), dim3(256), 0, s, p.partial, ks, a.M, a.N, a.C, a.ldc, a.alpha,
+    hipLaunchKernelGGL(nt_reduce_kernel, dim3(blocks), dim3(256), 0, s, p.partial, ks, a.M, a.N, a.C, a.C_f32, a.ldc, a.alpha,
                        a.beta);
     HH_HIP(hipGetLastError());
   }
@@ -405,7 +440,8 @@ void launch_nt(const NtArgs& a, hipStream_t s, DevBuf& ws) {
 void gemm_nt(const NtArgs& a, NtFormat fmt, hipStream_t s, DevBuf& ws) {
   HH_REQUIRE(a.M >= 0 && a.N >= 0 && a.K >= 0, "gemm_nt: negative size");
   if (a.M == 0 || a.N == 0) return;
-  HH_REQUIRE(a.C && (a.K == 0 || (a.A && a.B)), "gemm_nt: null operand");
+  HH_REQUIRE((a.C || a.C_f32) && (a.K == 0 || (a.A && a.B)), "gemm_nt: null operand");
+  HH_REQUIRE(!a.C_f32 || fmt == NT_BF16X3, "gemm_nt: FP32 output is BF16X3 only");
   HH_REQUIRE(a.K % 32 == 0 && a.lda % 8 == 0 && a.ldb % 8 == 0, "gemm_nt: K must be a multiple of 32 and lda, ldb "
                                                                   "multiples of 8");
   if (fmt == NT_BF16X3) {
@@ -420,19 +456,20 @@ void gemm_nt(const NtArgs& a, NtFormat fmt, hipStream_t s, DevBuf& ws) {
 }
 
 template <typename T>
-void split_bf16_rows(const T* in, int64_t rows, int64_t cols, int64_t ld_in, unsigned short* hi, unsigned short* lo,
-                     int64_t ld_out, hipStream_t s) {
-  const int64_t total = rows * ld_out;
+void split_bf16_tiled(const T* src, int64_t ld, int64_t nrow, int64_t ncol, int64_t kpad, unsigned short* hi,
+                      unsigned short* lo, hipStream_t s) {
+  HH_REQUIRE(kpad % 32 == 0 && kpad >= nrow && ld >= nrow, "split_bf16_tiled: kpad must be a multiple of 32 >= nrow");
+  const int64_t total = kpad * ncol;
   if (total == 0) return;
   const unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(total, 256), 65536);
-  hipLaunchKernelGGL((split_bf16_rows_kernel<T>), dim3(blocks), dim3(256), 0, s, in, rows, cols, ld_in, hi, lo,
-                     ld_out);
+  hipLaunchKernelGGL((split_bf16_tiled_kernel<T>), dim3(blocks), dim3(256), 0, s, src, ld, nrow, ncol, kpad / 32, hi,
+                     lo);
   HH_HIP(hipGetLastError());
 }
 
-template void split_bf16_rows<double>(const double*, int64_t, int64_t, int64_t, unsigned short*, unsigned short*,
-                                      int64_t, hipStream_t);
-template void split_bf16_rows<float>(const float*, int64_t, int64_t, int64_t, unsigned short*, unsigned short*, int64_t,
-                                     hipStream_t);
+template void split_bf16_tiled<double>(const double*, int64_t, int64_t, int64_t, int64_t, unsigned short*,
+                                       unsigned short*, hipStream_t);
+template void split_bf16_tiled<float>(const float*, int64_t, int64_t, int64_t, int64_t, unsigned short*,
+                                      unsigned short*, hipStream_t);
 
 }  // namespace hh

@@ -26,6 +26,7 @@ struct NtArgs {
   const void* B_lo = nullptr;
   int64_t ldb = 0;
   double* C = nullptr;
+  float* C_f32 = nullptr;  // BF16X3: FP32 output instead of C
   int64_t ldc = 0;
   double alpha = 1.0, beta = 0.0;
   bool tiled = false;  // K-block tiled operands (BF16X3)
@@ -35,11 +36,11 @@ struct NtArgs {
 
 void gemm_nt(const NtArgs& a, NtFormat fmt, hipStream_t s, DevBuf& ws);
 
-// Split x (double or float) into packed bf16 hi / lo arrays: hi = bf16(x), lo = bf16(x - hi).
-// Rows of `cols` elements are read with stride ld_in and written with stride ld_out; the columns
-// [cols, ld_out) of every output row are zeroed (K padding).
+// Split the column-major [nrow, ncol] matrix src (leading dimension ld) into bf16 hi / lo arrays
+// (hi = bf16(x), lo = bf16(x - hi)) in the K-block tiled operand layout with K = rows padded to
+// kpad (a multiple of 32): hi/lo[((r / 32) * ncol + n) * 32 + r % 32], zero for r >= nrow.
 template <typename T>
-void split_bf16_rows(const T* in, int64_t rows, int64_t cols, int64_t ld_in, unsigned short* hi, unsigned short* lo,
-                     int64_t ld_out, hipStream_t s);
+void split_bf16_tiled(const T* src, int64_t ld, int64_t nrow, int64_t ncol, int64_t kpad, unsigned short* hi,
+                      unsigned short* lo, hipStream_t s);
 
 }  // namespace hh

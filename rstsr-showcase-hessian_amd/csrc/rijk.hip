@@ -164,6 +164,7 @@ void Rijk::prepare_response(const double* const* mo_coeff, const double* const* 
   }
   prepared_ = true;
   mixed_ready_ = false;
+  xt_ready_ = false;
 }
 
 namespace {
@@ -367,7 +368,29 @@ void Rijk::build_k(int s, const double* bra, int64_t nprop, Prec prec, double al
   // W[(k + nocc*a) + nocc*nprop*(P + naux*i)] = sum_mu bra[mu,(k,a)] X[mu,(P,i)]
   const int64_t ldw = nocc * nprop;
   void* W = w_.reserve(esz * ldw * nP);
-  {
+  if (bf) {
+    // NT GEMM on K-block tiled bf16 pairs (K = mu padded to kpad), FP32 output
+    if (!xt_ready_) {
+      for (int t = 0; t < nspin_; ++t) {
+        const int64_t nc = naux_ * nocc_[t];
+        xt_hi_[t].reserve(sizeof(unsigned short) * kpad_ * nc);
+        xt_lo_[t].reserve(sizeof(unsigned short) * kpad_ * nc);
+        split_bf16_tiled<float>(x32_[t].as<float>(), nao_, nao_, nc, kpad_, xt_hi_[t].as<unsigned short>(),
+                                xt_lo_[t].as<unsigned short>(), stream_);
+      }
+      xt_ready_ = true;
+    }
+    unsigned short* bth = static_cast<unsigned short*>(brat_.reserve(2 * sizeof(unsigned short) * kpad_ * ldw));
+    unsigned short* btl = bth + kpad_ * ldw;
+    split_bf16_tiled<double>(bra, nao_, nao_, ldw, kpad_, bth, btl, stream_);
+    NtArgs a;
+    a.M = ldw; a.N = nP; a.K = kpad_;
+    a.A = bth; a.A_lo = btl; a.B = xt_hi_[s].ptr; a.B_lo = xt_lo_[s].ptr;
+    a.lda = a.ldb = kpad_;
+    a.tiled = true;
+    a.C_f32 = static_cast<float*>(W); a.ldc = ldw;
+    gemm_nt(a, NT_BF16X3, stream_, ws_);
+  } else {
     GemmArgs g;
     g.M = ldw; g.N = nP; g.K = nao_;
     g.A = braX; g.lda = nao_; g.a_kmajor = true;

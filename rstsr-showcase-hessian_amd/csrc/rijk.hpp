@@ -81,6 +81,7 @@ class Rijk {
   hipStream_t stream_;
   bool prepared_ = false;
   bool mixed_ready_ = false;
+  bool xt_ready_ = false;
   int64_t nocc_[2] = {0, 0};
 
   DevBuf j2c_l_;      // [naux, naux] lower Cholesky factor of the metric (from_j3c only)
@@ -99,7 +100,9 @@ class Rijk {
   DevBuf x32_[2];
   DevBuf o_[2];       // O[j, P, i] = (C_o^T B_P C_o)[j, i]
   DevBuf o32_[2];
+  DevBuf xt_hi_[2], xt_lo_[2];  // X as BF16x3 NT operand: K = mu (padded to kpad), K-block tiled
   // per-call scratch
+  DevBuf brat_;                 // bra as BF16x3 NT operand (hi then lo), K-block tiled
   DevBuf w_, t_, bra32_, dm_, dmtp_, itm_, jtp_, jao_, ws_;
 };
 
