@@ -29,8 +29,9 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}  # MI355X dense ma
 # dispatch) and scaled by the same counters on a 1 GiB copy16_kernel copy (factors 1.99994 and 1.00000);
 # profiles/nt_bf16x3_kernel_ri_k_nao400_naux2400_nocc40_nprop90_hbm_traffic.txt.  Keyed by
 # (precision, nao, naux, nocc, nprop); other workloads report null.
-MEASURED_TRAFFIC = {("bf16x3", 400, 2400, 40, 90): 20371236.8 * 1024 * 1.99994 + 91540.0 * 1024 * 1.00000}
-HOT_KERNEL = {"fp64": "nt_f64_kernel", "fp32": "gemm_kernel<float,double>", "bf16x3": "nt_bf16x3_kernel"}
+MEASURED_TRAFFIC = {("bf16x3", 400, 2400, 40, 90): 13881227.0 * 1024 * 1.99994 + 91504.8 * 1024 * 1.00000}
+HOT_KERNEL = {"fp64": "nt_f64_kernel<1,8,13,1>", "fp32": "gemm_kernel<1,float,double>",
+              "bf16x3": "nt_bf16x3_kernel<1,8,13,2,true,13,2>"}
 SIZES = {
     "full": dict(nao=400, naux=2400, nocc=40, nprop=90),
     "small": dict(nao=96, naux=480, nocc=12, nprop=18),

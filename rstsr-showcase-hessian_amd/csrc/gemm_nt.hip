@@ -44,10 +44,20 @@ __device__ __forceinline__ int64_t remap_xcd(int64_t w, int64_t nwg) {
   return (w % 8) * (nwg / 8) + w / 8;
 }
 
+// Workgroup barrier ordering LDS only: prefetched global loads stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // ---------------------------------------------------------------- bf16 x3 ----
-template <int WGM, int WGN, int FM, int FN, bool TILED>
+// BMT 16-row tiles per workgroup (wave wm owns tiles wm*FM .. min(BMT, (wm+1)*FM) - 1, so the
+// waves of a column may own unequal row counts); PD: K steps of global loads in flight (1 or 2).
+template <int WGM, int WGN, int FM, int FN, bool TILED, int BMT = WGM * FM, int PD = 2>
 __global__ __launch_bounds__(512) void nt_bf16x3_kernel(NtParams p) {
-  constexpr int BM = WGM * FM * 16, BN = WGN * FN * 16, BK = 32;
+  static_assert(BMT <= WGM * FM && BMT > (WGM - 1) * FM && (PD == 1 || PD == 2), "tile shape");
+  constexpr int BM = BMT * 16, BN = WGN * FN * 16, BK = 32;
   constexpr int LD = BK + 8;                 // bf16 elements per LDS row (80 B)
   constexpr int A_ELEMS = BM * LD, B_ELEMS = BN * LD;
   constexpr int STAGE = 2 * A_ELEMS + 2 * B_ELEMS;  // hi, lo for A and B
@@ -75,8 +85,13 @@ __global__ __launch_bounds__(512) void nt_bf16x3_kernel(NtParams p) {
 #pragma unroll
     for (int j = 0; j < FN; ++j) acc[i][j] = v4f{0, 0, 0, 0};
 
-  uint4 ra_h[A_PT], ra_l[A_PT], rb_h[B_PT], rb_l[B_PT];
-  auto gload = [&](int64_t k0) {
+  // K-step operands in registers: two sets, so the loads of step k + 2 are in flight while step
+  // k computes (the LDS-only barriers do not wait for them)
+  struct Regs {
+    uint4 ah[A_PT], al[A_PT], bh[B_PT], bl[B_PT];
+  };
+  Regs r0, r1;
+  auto gload = [&](int64_t k0, Regs& R) {
 #pragma unroll
     for (int e = 0; e < A_PT; ++e) {
       const int c = tid + 512 * e;
@@ -90,8 +105,8 @@ __global__ __launch_bounds__(512) void nt_bf16x3_kernel(NtParams p) {
           vl = *reinterpret_cast<const uint4*>(Al + off);
         }
       }
-      ra_h[e] = vh;
-      ra_l[e] = vl;
+      R.ah[e] = vh;
+      R.al[e] = vl;
     }
 #pragma unroll
     for (int e = 0; e < B_PT; ++e) {
@@ -106,19 +121,19 @@ __global__ __launch_bounds__(512) void nt_bf16x3_kernel(NtParams p) {
           vl = *reinterpret_cast<const uint4*>(Bl + off);
         }
       }
-      rb_h[e] = vh;
-      rb_l[e] = vl;
+      R.bh[e] = vh;
+      R.bl[e] = vl;
     }
   };
-  auto sstore = [&](int buf) {
+  auto sstore = [&](int buf, const Regs& R) {
     u16* s = smem + buf * STAGE;
 #pragma unroll
     for (int e = 0; e < A_PT; ++e) {
       const int c = tid + 512 * e;
       if (c < A_CH) {
         const int r = c / (BK / 8), kk = (c % (BK / 8)) * 8;
-        *reinterpret_cast<uint4*>(s + r * LD + kk) = ra_h[e];
-        *reinterpret_cast<uint4*>(s + A_ELEMS + r * LD + kk) = ra_l[e];
+        *reinterpret_cast<uint4*>(s + r * LD + kk) = R.ah[e];
+        *reinterpret_cast<uint4*>(s + A_ELEMS + r * LD + kk) = R.al[e];
       }
     }
 #pragma unroll
@@ -126,8 +141,8 @@ __global__ __launch_bounds__(512) void nt_bf16x3_kernel(NtParams p) {
       const int c = tid + 512 * e;
       if (c < B_CH) {
         const int r = c / (BK / 8), kk = (c % (BK / 8)) * 8;
-        *reinterpret_cast<uint4*>(s + 2 * A_ELEMS + r * LD + kk) = rb_h[e];
-        *reinterpret_cast<uint4*>(s + 2 * A_ELEMS + B_ELEMS + r * LD + kk) = rb_l[e];
+        *reinterpret_cast<uint4*>(s + 2 * A_ELEMS + r * LD + kk) = R.bh[e];
+        *reinterpret_cast<uint4*>(s + 2 * A_ELEMS + B_ELEMS + r * LD + kk) = R.bl[e];
       }
     }
   };
@@ -141,32 +156,62 @@ __global__ __launch_bounds__(512) void nt_bf16x3_kernel(NtParams p) {
       bh[j] = *reinterpret_cast<const v8bf*>(s + 2 * A_ELEMS + r * LD + kc);
       bl[j] = *reinterpret_cast<const v8bf*>(s + 2 * A_ELEMS + B_ELEMS + r * LD + kc);
     }
+    // A fragments software-pipelined: row tile i + 1 is read while tile i's MFMAs run
+    auto aload = [&](int i, v8bf& h, v8bf& l) {
+      const int r = (wm * FM + i) * 16 + (lane & 15);
+      h = *reinterpret_cast<const v8bf*>(s + r * LD + kc);
+      l = *reinterpret_cast<const v8bf*>(s + A_ELEMS + r * LD + kc);
+    };
+    v8bf ah, al, nh, nl;
+    aload(0, ah, al);
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
-      const int r = (wm * FM + i) * 16 + (lane & 15);
-      const v8bf ah = *reinterpret_cast<const v8bf*>(s + r * LD + kc);
-      const v8bf al = *reinterpret_cast<const v8bf*>(s + A_ELEMS + r * LD + kc);
+      if (wm * FM + i >= BMT) continue;  // wave-uniform
+      if (i + 1 < FM && wm * FM + i + 1 < BMT) aload(i + 1, nh, nl);
+      // product-major order: consecutive MFMAs update different accumulators
 #pragma unroll
-      for (int j = 0; j < FN; ++j) {
+      for (int j = 0; j < FN; ++j)
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
-      }
+      ah = nh;
+      al = nl;
     }
   };
 
-  if (kbeg < kend) {
-    gload(kbeg);
-    sstore(0);
-    __syncthreads();
+  if (PD == 1 && kbeg < kend) {
+    gload(kbeg, r0);
+    sstore(0, r0);
+    lds_barrier();
     int buf = 0;
     for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
       const bool more = k0 + BK < kend;
-      if (more) gload(k0 + BK);
+      if (more) gload(k0 + BK, r0);
       compute(buf);
-      if (more) sstore(buf ^ 1);
-      __syncthreads();
+      if (more) sstore(buf ^ 1, r0);
+      lds_barrier();
       buf ^= 1;
+    }
+  } else if (kbeg < kend) {
+    gload(kbeg, r0);
+    sstore(0, r0);
+    if (kbeg + BK < kend) gload(kbeg + BK, r1);
+    lds_barrier();
+    // step k0 computes LDS buffer `buf`, requests step k0 + 2 BK into `ahead` (the set whose
+    // data is already in LDS) and moves step k0 + BK from `next` into the other buffer
+    auto step = [&](int64_t k0, int buf, Regs& ahead, const Regs& next) {
+      if (k0 + 2 * BK < kend) gload(k0 + 2 * BK, ahead);
+      compute(buf);
+      if (k0 + BK < kend) sstore(buf ^ 1, next);
+      lds_barrier();
+    };
+    for (int64_t k0 = kbeg; k0 < kend; k0 += 2 * BK) {
+      step(k0, 0, r0, r1);
+      if (k0 + BK < kend) step(k0 + BK, 1, r1, r0);
     }
   }
   // epilogue: D[4*(l/16) + r][l%16]
@@ -180,7 +225,7 @@ __global__ __launch_bounds__(512) void nt_bf16x3_kernel(NtParams p) {
 #pragma unroll
       for (int j = 0; j < FN; ++j) {
         const int64_t gm = row0 + i * 16, gn = col0 + j * 16;
-        if (gn >= p.N) continue;
+        if (wm * FM + i >= BMT || gn >= p.N) continue;
         float* c = p.Cf + gm + gn * p.ldc;
         if (vec && gm + 3 < p.M) {
           *reinterpret_cast<float4*>(c) =
@@ -207,7 +252,7 @@ __global__ __launch_bounds__(512) void nt_bf16x3_kernel(NtParams p) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int64_t gm = row0 + i * 16 + r, gn = col0 + j * 16;
-        if (gm < p.M && gn < p.N) {
+        if (wm * FM + i < BMT && gm < p.M && gn < p.N) {
           if (p.ksplit > 1) {
             P[gm + gn * p.M] = (double)acc[i][j][r];
           } else {
@@ -395,9 +440,9 @@ int cu_count() {
   return n;
 }
 
-template <int WGM, int WGN, int FM, int FN, bool BF, bool TILED = false>
+template <int WGM, int WGN, int FM, int FN, bool BF, bool TILED = false, int BMT = WGM * FM, int PD = 2>
 void launch_nt(const NtArgs& a, hipStream_t s, DevBuf& ws) {
-  constexpr int BM = WGM * FM * 16, BN = WGN * FN * 16, BK = BF ? 32 : 16;
+  constexpr int BM = BMT * 16, BN = WGN * FN * 16, BK = BF ? 32 : 16;
   NtParams p{};
   p.M = a.M; p.N = a.N; p.K = a.K;
   p.A = a.A; p.Alo = a.A_lo; p.lda = a.lda;
@@ -422,7 +467,8 @@ void launch_nt(const NtArgs& a, hipStream_t s, DevBuf& ws) {
   HH_REQUIRE(p.nwg < (1LL << 31), "gemm_nt: grid too large");
   if (a.ev_begin) HH_HIP(hipEventRecord(a.ev_begin, s));
   if constexpr (BF)
-    hipLaunchKernelGGL((nt_bf16x3_kernel<WGM, WGN, FM, FN, TILED>), dim3((unsigned)p.nwg), dim3(512), 0, s, p);
+    hipLaunchKernelGGL((nt_bf16x3_kernel<WGM, WGN, FM, FN, TILED, BMT, PD>), dim3((unsigned)p.nwg), dim3(512), 0, s,
+                       p);
   else
     hipLaunchKernelGGL((nt_f64_kernel<WGM, WGN, FM, FN>), dim3((unsigned)p.nwg), dim3(512), 0, s, p);
   HH_HIP(hipGetLastError());
