@@ -258,6 +258,53 @@ hh_status hh_linalg_trsm_lower(const double* L, int64_t nd, double* X, int64_t n
                                void* stream);
 
 /* ---------------------------------------------------------------------------------------- */
+/* Harmonic analysis (src/hessian/vib.rs).  HOST arrays: the readout of a [3 natm, 3 natm]    */
+/* Hessian is FP64 CPU work (the caller copies the Hessian from the device).  Synchronous.     */
+
+#define HH_ROTOR_ATOM 0
+#define HH_ROTOR_LINEAR 1
+#define HH_ROTOR_REGULAR 2
+#define HH_TRV_V 0                /* vibration ("V") */
+#define HH_TRV_TR 1               /* translation / rotation ("TR") */
+#define HH_TRV_NONE 2             /* near-zero force constant, not TR ("-") */
+
+/* Output arrays of hh_vib_harmonic_analysis (fields of VibInfo, vib.rs:221-256); ndof = 3 natm.
+ * Any pointer may be NULL (not written).  Per-mode arrays [ndof]; q, w, x [ndof, ndof] column
+ * major with modes as columns.  omega is the magnitude of the frequency in cm^-1, imag = 1 for
+ * an imaginary mode. */
+typedef struct hh_vib_out {
+  double* omega;
+  int32_t* imag;
+  double* q;              /* mass-weighted normal modes */
+  double* w;              /* un-mass-weighted */
+  double* x;              /* un-mass-weighted, normalised */
+  int64_t* degeneracy;
+  int32_t* trv;           /* HH_TRV_* */
+  double* mu;             /* reduced mass [u] */
+  double* k;              /* force constant [mDyne/A] */
+  double* dq0;            /* RMS deviation v=0 [a0 u^1/2] */
+  double* qtp0;           /* turning point v=0, mass-weighted [a0 u^1/2] */
+  double* xtp0;           /* turning point v=0, Cartesian [a0] */
+  double* theta_vib;      /* characteristic temperature [K] */
+} hh_vib_out;
+
+/* Frequencies and normal modes of hess [3 natm, 3 natm] (E_h/a0^2, not mass-weighted) at geom
+ * [3, natm] (a0) with masses [natm] (u), translations / rotations projected out on request
+ * (replaces harmonic_analysis, vib.rs:336-552). */
+hh_status hh_vib_harmonic_analysis(const double* hess, const double* geom, const double* mass, int64_t natm,
+                                   int32_t project_trans, int32_t project_rot, const hh_vib_out* out);
+/* Rotational constants out3[3] of a mass-centred geometry; unit 0 GHz, 1 cm^-1
+ * (replaces rotation_const, vib.rs:152-197). */
+hh_status hh_vib_rotation_const(const double* mass, const double* geom_c, int64_t natm, int32_t unit,
+                                double* out3);
+/* HH_ROTOR_* from rotational constants in GHz (replaces get_rotor_type, vib.rs:203-211). */
+hh_status hh_vib_rotor_type(const double* rc_ghz, int32_t* rotor);
+/* Orthonormal translation (space bit 1) / rotation (bit 2) basis tr [3 natm, ncol] (buffer for
+ * 6 columns), rank by the default tolerance (replaces get_tr_space, vib.rs:52-54). */
+hh_status hh_vib_tr_space(const double* mass, const double* geom, int64_t natm, int32_t space, double* tr,
+                          int64_t* ncol);
+
+/* ---------------------------------------------------------------------------------------- */
 /* Utility: the library's strided batched GEMM (no reference counterpart; exposed so callers  */
 /* and tests can drive the matrix-core kernel directly).                                      */
 /* C[m,n] = alpha sum_k A[m,k] B[k,n] + beta C[m,n]; A[m,k] at a_kmajor ? m*lda+k : m+k*lda;   */

@@ -6,6 +6,7 @@
 #include "mlp.hpp"
 #include "numint.hpp"
 #include "rijk.hpp"
+#include "vib.hpp"
 
 namespace hh {
 namespace {
@@ -524,6 +525,58 @@ hh_status hh_util_gemm(int64_t M, int64_t N, int64_t K, double alpha, const doub
     g.sB1 = strides6[3]; g.sC0 = strides6[4]; g.sC1 = strides6[5];
   }
   gemm<double, double>(g, (Prec)precision, (hipStream_t)stream, *ws);
+  HH_ABI_END
+}
+
+hh_status hh_vib_harmonic_analysis(const double* hess, const double* geom, const double* mass, int64_t natm,
+                                   int32_t project_trans, int32_t project_rot, const hh_vib_out* out) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(hess && geom && mass && out && natm >= 1 && natm <= 100000, "invalid argument");
+  for (int64_t a = 0; a < natm; ++a) HH_REQUIRE(mass[a] > 0.0, "masses must be positive");
+  const vib::Result r = vib::harmonic_analysis(hess, geom, mass, (int)natm, project_trans != 0, project_rot != 0);
+  auto put = [&](auto* dst, const auto& src) {
+    if (dst)
+      for (size_t i = 0; i < src.size(); ++i) dst[i] = src[i];
+  };
+  put(out->omega, r.omega);
+  put(out->imag, r.imag);
+  put(out->q, r.q);
+  put(out->w, r.w);
+  put(out->x, r.x);
+  put(out->degeneracy, r.degeneracy);
+  put(out->trv, r.trv);
+  put(out->mu, r.mu);
+  put(out->k, r.k);
+  put(out->dq0, r.dq0);
+  put(out->qtp0, r.qtp0);
+  put(out->xtp0, r.xtp0);
+  put(out->theta_vib, r.theta_vib);
+  HH_ABI_END
+}
+
+hh_status hh_vib_rotation_const(const double* mass, const double* geom_c, int64_t natm, int32_t unit,
+                                double* out3) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(mass && geom_c && out3 && natm >= 1 && (unit == 0 || unit == 1), "invalid argument");
+  vib::rotation_const(mass, geom_c, (int)natm, unit, out3);
+  HH_ABI_END
+}
+
+hh_status hh_vib_rotor_type(const double* rc_ghz, int32_t* rotor) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(rc_ghz && rotor, "invalid argument");
+  *rotor = vib::rotor_type(rc_ghz);
+  HH_ABI_END
+}
+
+hh_status hh_vib_tr_space(const double* mass, const double* geom, int64_t natm, int32_t space, double* tr,
+                          int64_t* ncol) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(mass && geom && tr && ncol && natm >= 1 && space >= 0 && space <= 3, "invalid argument");
+  int nc = 0;
+  const std::vector<double> u = vib::tr_space(mass, geom, (int)natm, space, -1, &nc);
+  std::copy(u.begin(), u.end(), tr);
+  *ncol = nc;
   HH_ABI_END
 }
 

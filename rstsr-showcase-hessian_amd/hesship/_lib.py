@@ -72,6 +72,10 @@ SIGNATURES.update({
     "hh_linalg_cholesky": (_i32, [_p, _i64, _p]),
     "hh_linalg_trsm_lower": (_i32, [_p, _i64, _p, _i64, _i64, _i32, _p]),
     "hh_util_copy": (_i32, [_p, _p, _i64, _p]),
+    "hh_vib_harmonic_analysis": (_i32, [_p, _p, _p, _i64, _i32, _i32, _p]),
+    "hh_vib_rotation_const": (_i32, [_p, _p, _i64, _i32, _p]),
+    "hh_vib_rotor_type": (_i32, [_p, C.POINTER(C.c_int32)]),
+    "hh_vib_tr_space": (_i32, [_p, _p, _i64, _i32, _p, _pi64]),
     "hh_mlp_create": (_i32, [C.c_uint64, _p, C.POINTER(C.c_void_p)]),
     "hh_mlp_destroy": (_i32, [_p]),
     "hh_mlp_forward": (_i32, [_p, _p, _i64, _p]),
