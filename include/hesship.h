@@ -167,7 +167,8 @@ hh_status hh_krylov_block(hh_aop_fn aop, void* ctx, const double* b, const doubl
                           hh_krylov_stats* stats, void* stream);
 
 /* ---------------------------------------------------------------------------------------- */
-/* Dimensionless CPHF driver, restricted SCF (src/hessian/rscf.rs)                           */
+/* Dimensionless CPHF driver, restricted (src/hessian/rscf.rs) and unrestricted              */
+/* (src/hessian/uscf.rs) SCF                                                                  */
 
 typedef struct hh_cphf hh_cphf;
 
@@ -181,8 +182,9 @@ hh_status hh_cphf_destroy(hh_cphf* h);
 /* Per-sweep / per-cycle diagnostics on stderr (the reference prints per Krylov cycle,
  * krylov_block.rs:148-151). */
 hh_status hh_cphf_set_verbose(hh_cphf* h, int32_t on);
-/* Add an RI-JK component (must be RHF, prepared with the same MOs) to the response list
- * (the reference's el_list, rscf.rs:28).  The component must outlive the driver. */
+/* Add an RI-JK component (nspin 1 for the restricted driver, 2 for the unrestricted one,
+ * prepared with the same MOs) to the response list (the reference's el_list, rscf.rs:28,
+ * uscf.rs:24).  The component must outlive the driver. */
 hh_status hh_cphf_add_rijk(hh_cphf* h, hh_rijk* rijk);
 /* Add the DFT grid response (RHessKSNIMatmul::get_response_bra, hess_rks.rs:886-909) with
  * fxc_eff [ng, nvar, nvar] (device; must outlive the driver) to the response list. */
@@ -207,6 +209,34 @@ hh_status hh_cphf_finalize(hh_cphf* h, const double* f1mo, const double* s1mo, c
 /* get_cphf_hess, rscf.rs:371-399: de [3, 3, natm, natm] from [.., .., 3, natm] inputs. */
 hh_status hh_cphf_hess(hh_cphf* h, const double* f1mo, const double* s1mo, const double* mo1, const double* mo_e1,
                        int64_t natm, double* de);
+
+/* Unrestricted driver: the same handle type, two spins.  mo_coeff[2] device [nao, nmo[s]];
+ * mo_occ[2], mo_energy[2] HOST [nmo[s]] (occupied orbitals first in each spin).  Replaces
+ * UHessSCF (uscf.rs:32-56).  The hh_cphf_* per-call entry points above are for the restricted
+ * driver; the hh_ucphf_* ones below take one array per spin, [nmo[s], nocc[s], nprop] (mo_e1:
+ * [nocc[s], nocc[s], nprop]), like the reference's [TsrView; 2] arguments. */
+hh_status hh_ucphf_create(const double* const* mo_coeff, int64_t nao, const int64_t* nmo,
+                          const double* const* mo_occ, const double* const* mo_energy, double level_shift, double tol,
+                          int32_t max_cycle, int32_t max_space, double lindep, void* stream, hh_cphf** out);
+/* compute_dimless_cphf_rhs, uscf.rs:165-174. */
+hh_status hh_ucphf_dimless_rhs(hh_cphf* h, const double* const* f1mo, const double* const* s1mo, int64_t nprop,
+                               double* const* rhs);
+/* response_mo, uscf.rs:203-218. */
+hh_status hh_ucphf_response_mo(hh_cphf* h, const double* const* mo1, int64_t nprop, double* const* resp,
+                               hh_precision precision);
+/* response_dimless_cphf, uscf.rs:229-265. */
+hh_status hh_ucphf_response_dimless(hh_cphf* h, const double* const* mo1, int64_t nprop, double* const* resp,
+                                    hh_precision precision);
+/* solve_dimless_cphf, uscf.rs:278-349: one Krylov solve over the spin blocks stacked per column;
+ * precision as hh_cphf_solve_dimless (no preconditioner argument). */
+hh_status hh_ucphf_solve_dimless(hh_cphf* h, const double* const* rhs, int64_t nprop, double* const* mo1,
+                                 hh_precision precision, hh_krylov_stats* stats);
+/* finalize_cphf, uscf.rs:369-407. */
+hh_status hh_ucphf_finalize(hh_cphf* h, const double* const* f1mo, const double* const* s1mo,
+                            const double* const* mo1, int64_t nprop, double* const* mo1_out, double* const* mo_e1);
+/* get_cphf_hess, uscf.rs:409-440: de [3, 3, natm, natm]. */
+hh_status hh_ucphf_hess(hh_cphf* h, const double* const* f1mo, const double* const* s1mo, const double* const* mo1,
+                        const double* const* mo_e1, int64_t natm, double* de);
 
 /* ---------------------------------------------------------------------------------------- */
 /* Learned diagonal CPHF preconditioner (no reference counterpart: the reference preconditions */

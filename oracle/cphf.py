@@ -109,3 +109,97 @@ def cphf_hess(f1mo, s1mo, mo1, mo_e1, mo_occ, mo_energy):
         for B in range(A):
             de[:, :, A, B] += de[:, :, B, A].T
     return de
+
+
+# ---------------------------------------------------------------------------------------------
+# Unrestricted driver: src/hessian/uscf.rs (every argument a pair (alpha, beta))
+#   compute_dimless_cphf_rhs 78-182 (rhs assembly 165-174), response_mo 203-218,
+#   response_dimless_cphf 229-265, solve_dimless_cphf 278-349 (spin blocks stacked per column,
+#   pack_flattened / unpack_flattened 292-316), finalize_cphf 369-407, get_cphf_hess 409-440.
+
+def u_dimless_rhs(f1mo, s1mo, mo_occ, mo_energy, level_shift=0.0):
+    """uscf.rs:165-174: the restricted formula per spin."""
+    return [dimless_rhs(f1mo[s], s1mo[s], mo_occ[s], mo_energy[s], level_shift) for s in range(2)]
+
+
+def u_response_mo(mo_coeff, response_bras, mo1):
+    """uscf.rs:203-218: resp_s = sum_el C_s^T el.get_response_bra([C_a mo1_a, C_b mo1_b])[s]."""
+    shapes = [m.shape for m in mo1]
+    ms = [m.reshape(m.shape[0], m.shape[1], -1) for m in mo1]
+    ubra = [np.einsum("um,mia->uia", mo_coeff[s], ms[s]) for s in range(2)]
+    resp = [np.zeros_like(m) for m in ms]
+    for rb in response_bras:
+        r = rb(ubra)
+        for s in range(2):
+            resp[s] += np.einsum("um,uia->mia", mo_coeff[s], np.asarray(r[s]).reshape(ubra[s].shape))
+    return [r.reshape(sh) for r, sh in zip(resp, shapes)]
+
+
+def u_response_dimless(mo_coeff, mo_occ, mo_energy, response_bras, mo1, level_shift=0.0):
+    """uscf.rs:229-265: the restricted scaling per spin."""
+    resp = u_response_mo(mo_coeff, response_bras, mo1)
+    out = []
+    for s in range(2):
+        nocc, eocc, evir = split(mo_occ[s], mo_energy[s])
+        e_ai = evir[:, None] - eocc[None, :] + level_shift
+        tail = (1,) * (mo1[s].ndim - 2)
+        r = resp[s] - level_shift * mo1[s] if level_shift != 0.0 else resp[s]
+        r[nocc:] /= e_ai.reshape(e_ai.shape + tail)
+        r[:nocc] = 0.0
+        out.append(r)
+    return out
+
+
+def u_solve_dimless(mo_coeff, mo_occ, mo_energy, response_bras, rhs, cfg=None, stats=None):
+    """uscf.rs:278-349: one Krylov solve over [nmo_a nocc_a + nmo_b nocc_b, nprop]."""
+    cfg = dict(DEFAULTS, **(cfg or {}))
+    shapes = [r.shape for r in rhs]
+    dims = [(sh[0], sh[1]) for sh in shapes]
+    split_at = dims[0][0] * dims[0][1]
+
+    def unpack(x):
+        return [x[:split_at].reshape(dims[0] + (-1,)), x[split_at:].reshape(dims[1] + (-1,))]
+
+    def pack(v):
+        return np.concatenate([v[0].reshape(split_at, -1), v[1].reshape(dims[1][0] * dims[1][1], -1)], axis=0)
+
+    def aop(x):
+        return pack(u_response_dimless(mo_coeff, mo_occ, mo_energy, response_bras, unpack(x), cfg["level_shift"]))
+
+    x = krylov.krylov_block(aop, pack([r.reshape(d + (-1,)) for r, d in zip(rhs, dims)]), None, cfg["cphf_tol"],
+                            cfg["cphf_max_cycle"], cfg["cphf_max_space"], cfg["cphf_lindep"], stats=stats)
+    return [v.reshape(sh) for v, sh in zip(unpack(x), shapes)]
+
+
+def u_finalize(mo_coeff, mo_occ, mo_energy, response_bras, f1mo, s1mo, mo1):
+    """uscf.rs:369-407: the restricted finalisation per spin with the two-spin response."""
+    last = u_response_mo(mo_coeff, response_bras, mo1)
+    mo1_out, mo_e1 = [], []
+    for s in range(2):
+        nocc, eocc, evir = split(mo_occ[s], mo_energy[s])
+        tail = (1,) * (mo1[s].ndim - 2)
+        e_ai = (evir[:, None] - eocc[None, :]).reshape((len(evir), nocc) + tail)
+        e_ij = (eocc[:, None] - eocc[None, :]).reshape((nocc, nocc) + tail)
+        b1 = f1mo[s] - s1mo[s] * eocc.reshape((1, -1) + tail) + last[s]
+        m = mo1[s].copy()
+        m[nocc:] = -b1[nocc:] / e_ai
+        mo1_out.append(m)
+        mo_e1.append(b1[:nocc] + m[:nocc] * e_ij)
+    return mo1_out, mo_e1
+
+
+def u_cphf_hess(f1mo, s1mo, mo1, mo_e1, mo_occ, mo_energy):
+    """uscf.rs:409-440: the restricted assembly with factors 2, -2, -1 summed over spins."""
+    natm = f1mo[0].shape[3]
+    de = np.zeros((3, 3, natm, natm))
+    for A in range(natm):
+        for B in range(A + 1):
+            for s in range(2):
+                nocc, eocc, _ = split(mo_occ[s], mo_energy[s])
+                blk = 2.0 * np.einsum("mit,mis->st", f1mo[s][:, :, :, A], mo1[s][:, :, :, B])
+                blk -= 2.0 * np.einsum("mit,mis,i->st", s1mo[s][:, :, :, A], mo1[s][:, :, :, B], eocc)
+                blk -= np.einsum("jit,jis->st", s1mo[s][:nocc, :, :, A], mo_e1[s][:, :, :, B])
+                de[:, :, B, A] += blk
+        for B in range(A):
+            de[:, :, A, B] += de[:, :, B, A].T
+    return de

@@ -24,10 +24,10 @@ namespace {
 struct RijkResponse : hh::ElecResponse {
   hh::Rijk* r;
   explicit RijkResponse(hh::Rijk* x) : r(x) {}
-  void response_bra(const double* bra, int64_t nprop, double* out, bool accumulate, hh::Prec prec) override {
-    const double* b[1] = {bra};
-    double* o[1] = {out};
-    r->get_response_bra(b, nprop, o, prec, accumulate);
+  int nspin() const override { return r->nspin(); }
+  void response_bra(const double* const* bra, int64_t nprop, double* const* out, bool accumulate,
+                    hh::Prec prec) override {
+    r->get_response_bra(bra, nprop, out, prec, accumulate);
   }
 };
 
@@ -40,10 +40,27 @@ struct NumintResponse : hh::ElecResponse {
   int64_t nocc;
   NumintResponse(hh::Numint* n, int d, const double* f, const double* m, int64_t no)
       : ni(n), den(d), fxc(f), mocc(m), nocc(no) {}
-  void response_bra(const double* bra, int64_t nprop, double* out, bool accumulate, hh::Prec prec) override {
-    ni->rks_response_bra(den, fxc, bra, nprop, mocc, nocc, out, prec, accumulate);
+  int nspin() const override { return 1; }
+  void response_bra(const double* const* bra, int64_t nprop, double* const* out, bool accumulate,
+                    hh::Prec prec) override {
+    ni->rks_response_bra(den, fxc, bra[0], nprop, mocc, nocc, out[0], prec, accumulate);
   }
 };
+
+hh::CphfConfig cphf_config(double level_shift, double tol, int32_t max_cycle, int32_t max_space, double lindep) {
+  HH_REQUIRE(tol > 0.0 && max_cycle > 0 && max_space > 0 && lindep >= 0.0, "invalid CPHF configuration");
+  hh::CphfConfig cfg;
+  cfg.level_shift = level_shift;
+  cfg.tol = tol;
+  cfg.max_cycle = max_cycle;
+  cfg.max_space = max_space;
+  cfg.lindep = lindep;
+  return cfg;
+}
+
+void check_prec(hh_precision precision) {
+  HH_REQUIRE(precision >= HH_PREC_FP64 && precision <= HH_PREC_BF16X3, "unknown precision");
+}
 
 void fill_stats(hh_krylov_stats* out, const hh::KrylovStats& k, int refinements, double residual) {
   if (!out) return;
@@ -217,14 +234,25 @@ hh_status hh_cphf_create(const double* mo_coeff, int64_t nao, int64_t nmo, const
   HH_ABI_BEGIN
   HH_REQUIRE(out != nullptr, "out is null");
   *out = nullptr;
-  CphfConfig cfg;
-  cfg.level_shift = level_shift;
-  cfg.tol = tol;
-  cfg.max_cycle = max_cycle;
-  cfg.max_space = max_space;
-  cfg.lindep = lindep;
+  const CphfConfig cfg = cphf_config(level_shift, tol, max_cycle, max_space, lindep);
+  const double* c[1] = {mo_coeff};
+  const double* o[1] = {mo_occ};
+  const double* e[1] = {mo_energy};
   auto h = std::make_unique<hh_cphf>();
-  h->impl = std::make_unique<Cphf>(mo_coeff, nao, nmo, mo_occ, mo_energy, cfg, (hipStream_t)stream);
+  h->impl = std::make_unique<Cphf>(1, c, nao, &nmo, o, e, cfg, (hipStream_t)stream);
+  *out = h.release();
+  HH_ABI_END
+}
+
+hh_status hh_ucphf_create(const double* const* mo_coeff, int64_t nao, const int64_t* nmo,
+                          const double* const* mo_occ, const double* const* mo_energy, double level_shift, double tol,
+                          int32_t max_cycle, int32_t max_space, double lindep, void* stream, hh_cphf** out) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(out != nullptr, "out is null");
+  *out = nullptr;
+  const CphfConfig cfg = cphf_config(level_shift, tol, max_cycle, max_space, lindep);
+  auto h = std::make_unique<hh_cphf>();
+  h->impl = std::make_unique<Cphf>(2, mo_coeff, nao, nmo, mo_occ, mo_energy, cfg, (hipStream_t)stream);
   *out = h.release();
   HH_ABI_END
 }
@@ -248,9 +276,11 @@ hh_status hh_cphf_set_verbose(hh_cphf* h, int32_t on) {
 hh_status hh_cphf_add_rijk(hh_cphf* h, hh_rijk* rijk) {
   HH_ABI_BEGIN
   HH_REQUIRE(h && rijk, "null handle");
-  HH_REQUIRE(rijk->impl->nspin() == 1, "RHF CPHF needs an RHF (nspin = 1) RI-JK component");
+  HH_REQUIRE(rijk->impl->nspin() == h->impl->nspin(),
+             "RI-JK component spin count differs from the driver's (RHF: nspin 1, UHF: nspin 2)");
   if (!rijk->impl->prepared()) throw Error(HH_ERR_STATE, "RI-JK component not prepared (make_response_preparation)");
-  HH_REQUIRE(rijk->impl->nocc(0) == h->impl->nocc(), "RI-JK component occupation differs from the driver's");
+  for (int t = 0; t < h->impl->nspin(); ++t)
+    HH_REQUIRE(rijk->impl->nocc(t) == h->impl->nocc(t), "RI-JK component occupation differs from the driver's");
   h->comps.push_back(std::make_unique<RijkResponse>(rijk->impl.get()));
   h->impl->add_component(h->comps.back().get());
   HH_ABI_END
@@ -259,6 +289,7 @@ hh_status hh_cphf_add_rijk(hh_cphf* h, hh_rijk* rijk) {
 hh_status hh_cphf_add_numint(hh_cphf* h, hh_numint* ni, int32_t den_type, const double* fxc_eff) {
   HH_ABI_BEGIN
   HH_REQUIRE(h && ni && fxc_eff, "null argument");
+  HH_REQUIRE(h->impl->nspin() == 1, "the grid response component is restricted (RKS)");
   HH_REQUIRE(den_type >= HH_DEN_RHO && den_type <= HH_DEN_TAU, "den_type must be RHO, SIGMA or TAU");
   HH_REQUIRE(ni->impl->nao() == h->impl->nao(), "numint nao differs from the driver's");
   h->comps.push_back(std::make_unique<NumintResponse>(ni->impl.get(), den_type, fxc_eff, h->impl->mo_coeff(),
@@ -331,36 +362,30 @@ hh_status hh_xceff_transform(int32_t den_type, int32_t order, const double* rho,
   HH_ABI_END
 }
 
-hh_status hh_cphf_dimless_rhs(hh_cphf* h, const double* f1mo, const double* s1mo, int64_t nprop, double* rhs) {
-  HH_ABI_BEGIN
-  HH_REQUIRE(h && f1mo && s1mo && rhs && nprop > 0, "null argument or nprop <= 0");
-  h->impl->dimless_rhs(f1mo, s1mo, nprop, rhs);
-  HH_ABI_END
+}  // extern "C"
+
+namespace {
+void need_spins(hh_cphf* h, int nspin) {
+  HH_REQUIRE(h != nullptr, "handle is null");
+  HH_REQUIRE(h->impl->nspin() == nspin, nspin == 1 ? "this entry point is for the restricted driver (hh_cphf_create)"
+                                                    : "this entry point is for the unrestricted driver (hh_ucphf_create)");
 }
 
-hh_status hh_cphf_response_mo(hh_cphf* h, const double* mo1, int64_t nprop, double* resp, hh_precision precision) {
-  HH_ABI_BEGIN
-  HH_REQUIRE(h && mo1 && resp && nprop > 0, "null argument or nprop <= 0");
-  HH_REQUIRE(precision >= HH_PREC_FP64 && precision <= HH_PREC_BF16X3, "unknown precision");
-  h->impl->response_mo(mo1, nprop, resp, (Prec)precision);
-  HH_ABI_END
+template <typename T>
+void need_arrays(T* const* v, int nspin) {
+  HH_REQUIRE(v != nullptr, "null spin array");
+  for (int t = 0; t < nspin; ++t) HH_REQUIRE(v[t] != nullptr, "null spin array entry");
 }
 
-hh_status hh_cphf_response_dimless(hh_cphf* h, const double* mo1, int64_t nprop, double* resp,
-                                   hh_precision precision) {
-  HH_ABI_BEGIN
-  HH_REQUIRE(h && mo1 && resp && nprop > 0, "null argument or nprop <= 0");
-  HH_REQUIRE(precision >= HH_PREC_FP64 && precision <= HH_PREC_BF16X3, "unknown precision");
-  h->impl->response_dimless(mo1, nprop, resp, (Prec)precision);
-  HH_ABI_END
-}
-
-hh_status hh_cphf_solve_dimless(hh_cphf* h, const double* rhs, int64_t nprop, double* mo1, hh_precision precision,
-                                const double* diag, hh_krylov_stats* stats) {
+hh_status solve_common(hh_cphf* h, int nspin, const double* const* rhs, int64_t nprop, double* const* mo1,
+                       hh_precision precision, const double* diag, hh_krylov_stats* stats) {
   CphfStats st;
   try {
-    HH_REQUIRE(h && rhs && mo1 && nprop > 0, "null argument or nprop <= 0");
-    HH_REQUIRE(precision >= HH_PREC_FP64 && precision <= HH_PREC_BF16X3, "unknown precision");
+    need_spins(h, nspin);
+    need_arrays(rhs, nspin);
+    need_arrays(mo1, nspin);
+    HH_REQUIRE(nprop > 0, "nprop <= 0");
+    check_prec(precision);
     h->impl->solve_dimless(rhs, nprop, mo1, (Prec)precision, &st, diag);
     fill_stats(stats, st.krylov, st.refinements, st.residual);
     return HH_OK;
@@ -373,19 +398,114 @@ hh_status hh_cphf_solve_dimless(hh_cphf* h, const double* rhs, int64_t nprop, do
     return HH_ERR_INTERNAL;
   }
 }
+}  // namespace
+
+extern "C" {
+
+
+hh_status hh_cphf_dimless_rhs(hh_cphf* h, const double* f1mo, const double* s1mo, int64_t nprop, double* rhs) {
+  HH_ABI_BEGIN
+  need_spins(h, 1);
+  HH_REQUIRE(f1mo && s1mo && rhs && nprop > 0, "null argument or nprop <= 0");
+  h->impl->dimless_rhs(&f1mo, &s1mo, nprop, &rhs);
+  HH_ABI_END
+}
+
+hh_status hh_cphf_response_mo(hh_cphf* h, const double* mo1, int64_t nprop, double* resp, hh_precision precision) {
+  HH_ABI_BEGIN
+  need_spins(h, 1);
+  HH_REQUIRE(mo1 && resp && nprop > 0, "null argument or nprop <= 0");
+  check_prec(precision);
+  h->impl->response_mo(&mo1, nprop, &resp, (Prec)precision);
+  HH_ABI_END
+}
+
+hh_status hh_cphf_response_dimless(hh_cphf* h, const double* mo1, int64_t nprop, double* resp,
+                                   hh_precision precision) {
+  HH_ABI_BEGIN
+  need_spins(h, 1);
+  HH_REQUIRE(mo1 && resp && nprop > 0, "null argument or nprop <= 0");
+  check_prec(precision);
+  h->impl->response_dimless(&mo1, nprop, &resp, (Prec)precision);
+  HH_ABI_END
+}
+
+hh_status hh_cphf_solve_dimless(hh_cphf* h, const double* rhs, int64_t nprop, double* mo1, hh_precision precision,
+                                const double* diag, hh_krylov_stats* stats) {
+  return solve_common(h, 1, &rhs, nprop, &mo1, precision, diag, stats);
+}
 
 hh_status hh_cphf_finalize(hh_cphf* h, const double* f1mo, const double* s1mo, const double* mo1, int64_t nprop,
                            double* mo1_out, double* mo_e1) {
   HH_ABI_BEGIN
-  HH_REQUIRE(h && f1mo && s1mo && mo1 && mo1_out && mo_e1 && nprop > 0, "null argument or nprop <= 0");
-  h->impl->finalize(f1mo, s1mo, mo1, nprop, mo1_out, mo_e1);
+  need_spins(h, 1);
+  HH_REQUIRE(f1mo && s1mo && mo1 && mo1_out && mo_e1 && nprop > 0, "null argument or nprop <= 0");
+  h->impl->finalize(&f1mo, &s1mo, &mo1, nprop, &mo1_out, &mo_e1);
   HH_ABI_END
 }
 
 hh_status hh_cphf_hess(hh_cphf* h, const double* f1mo, const double* s1mo, const double* mo1, const double* mo_e1,
                        int64_t natm, double* de) {
   HH_ABI_BEGIN
-  HH_REQUIRE(h && f1mo && s1mo && mo1 && mo_e1 && de && natm > 0, "null argument or natm <= 0");
+  need_spins(h, 1);
+  HH_REQUIRE(f1mo && s1mo && mo1 && mo_e1 && de && natm > 0, "null argument or natm <= 0");
+  h->impl->cphf_hess(&f1mo, &s1mo, &mo1, &mo_e1, natm, de);
+  HH_ABI_END
+}
+
+hh_status hh_ucphf_dimless_rhs(hh_cphf* h, const double* const* f1mo, const double* const* s1mo, int64_t nprop,
+                               double* const* rhs) {
+  HH_ABI_BEGIN
+  need_spins(h, 2);
+  need_arrays(f1mo, 2); need_arrays(s1mo, 2); need_arrays(rhs, 2);
+  HH_REQUIRE(nprop > 0, "nprop <= 0");
+  h->impl->dimless_rhs(f1mo, s1mo, nprop, rhs);
+  HH_ABI_END
+}
+
+hh_status hh_ucphf_response_mo(hh_cphf* h, const double* const* mo1, int64_t nprop, double* const* resp,
+                               hh_precision precision) {
+  HH_ABI_BEGIN
+  need_spins(h, 2);
+  need_arrays(mo1, 2); need_arrays(resp, 2);
+  HH_REQUIRE(nprop > 0, "nprop <= 0");
+  check_prec(precision);
+  h->impl->response_mo(mo1, nprop, resp, (Prec)precision);
+  HH_ABI_END
+}
+
+hh_status hh_ucphf_response_dimless(hh_cphf* h, const double* const* mo1, int64_t nprop, double* const* resp,
+                                    hh_precision precision) {
+  HH_ABI_BEGIN
+  need_spins(h, 2);
+  need_arrays(mo1, 2); need_arrays(resp, 2);
+  HH_REQUIRE(nprop > 0, "nprop <= 0");
+  check_prec(precision);
+  h->impl->response_dimless(mo1, nprop, resp, (Prec)precision);
+  HH_ABI_END
+}
+
+hh_status hh_ucphf_solve_dimless(hh_cphf* h, const double* const* rhs, int64_t nprop, double* const* mo1,
+                                 hh_precision precision, hh_krylov_stats* stats) {
+  return solve_common(h, 2, rhs, nprop, mo1, precision, nullptr, stats);
+}
+
+hh_status hh_ucphf_finalize(hh_cphf* h, const double* const* f1mo, const double* const* s1mo,
+                            const double* const* mo1, int64_t nprop, double* const* mo1_out, double* const* mo_e1) {
+  HH_ABI_BEGIN
+  need_spins(h, 2);
+  need_arrays(f1mo, 2); need_arrays(s1mo, 2); need_arrays(mo1, 2); need_arrays(mo1_out, 2); need_arrays(mo_e1, 2);
+  HH_REQUIRE(nprop > 0, "nprop <= 0");
+  h->impl->finalize(f1mo, s1mo, mo1, nprop, mo1_out, mo_e1);
+  HH_ABI_END
+}
+
+hh_status hh_ucphf_hess(hh_cphf* h, const double* const* f1mo, const double* const* s1mo, const double* const* mo1,
+                        const double* const* mo_e1, int64_t natm, double* de) {
+  HH_ABI_BEGIN
+  need_spins(h, 2);
+  need_arrays(f1mo, 2); need_arrays(s1mo, 2); need_arrays(mo1, 2); need_arrays(mo_e1, 2);
+  HH_REQUIRE(de && natm > 0, "null argument or natm <= 0");
   h->impl->cphf_hess(f1mo, s1mo, mo1, mo_e1, natm, de);
   HH_ABI_END
 }

@@ -1,8 +1,9 @@
-// Dimensionless CPHF driver (reference: src/hessian/rscf.rs).
+// Dimensionless CPHF driver (reference: src/hessian/rscf.rs, src/hessian/uscf.rs).
 //
 // The dimensionless equation U + resp(U)/(e_a - e_i) = -B/(e_a - e_i) is solved for the vir-occ
-// block with the occ-occ block frozen at -S1/2 (rscf.rs:88-99, 237-263).  All tensors stay on
-// the device; perturbations are the trailing `nprop` dimension ([nmo, nocc, nprop]).
+// block with the occ-occ block frozen at -S1/2 (rscf.rs:88-99, 237-263), per spin for UHF with
+// the spin blocks stacked in one Krylov vector (uscf.rs:288-330).  All tensors stay on the
+// device; perturbations are the trailing `nprop` dimension ([nmo, nocc, nprop] per spin).
 #include "cphf.hpp"
 
 #include <cmath>
@@ -25,20 +26,29 @@ __global__ void dimless_rhs_kernel(const double* f1, const double* s1, const dou
   }
 }
 
-// resp[vir] = (resp - shift mo1)[vir] / (e_ai + shift); resp[occ] = 0
+// resp[vir] = (resp - shift mo1)[vir] / (e_ai + shift); resp[occ] = 0.  Column a of the
+// [nmo, nocc] block starts at ld * a.
 __global__ void dimless_scale_kernel(double* resp, const double* mo1, const double* eocc, const double* evir,
-                                     int64_t nmo, int64_t nocc, int64_t nprop, double shift) {
-  const int64_t total = nmo * nocc * nprop;
+                                     int64_t nmo, int64_t nocc, int64_t nprop, int64_t ld, double shift) {
+  const int64_t nn = nmo * nocc, total = nn * nprop;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t m = e % nmo, i = (e / nmo) % nocc;
+    const int64_t m = e % nmo, i = (e / nmo) % nocc, at = e % nn + ld * (e / nn);
     if (m >= nocc) {
-      double r = resp[e];
-      if (shift != 0.0) r -= shift * mo1[e];
-      resp[e] = r / (evir[m - nocc] - eocc[i] + shift);
+      double r = resp[at];
+      if (shift != 0.0) r -= shift * mo1[at];
+      resp[at] = r / (evir[m - nocc] - eocc[i] + shift);
     } else {
-      resp[e] = 0.0;
+      resp[at] = 0.0;
     }
   }
+}
+
+// dst[r + ld_dst c] = src[r + ld_src c], r < rows, c < ncol
+__global__ void copy_block_kernel(const double* src, int64_t ld_src, double* dst, int64_t ld_dst, int64_t rows,
+                                  int64_t ncol) {
+  const int64_t total = rows * ncol;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x)
+    dst[e % rows + ld_dst * (e / rows)] = src[e % rows + ld_src * (e / rows)];
 }
 
 // b1 = f1 - s1 e_i + resp (in place in resp);  mo1_out[vir] = -b1/e_ai, mo1_out[occ] = mo1[occ];
@@ -225,91 +235,186 @@ __global__ void precond_diag_kernel(const float* y, int64_t nmo, int64_t nocc, d
 
 }  // namespace
 
-Cphf::Cphf(const double* mo_coeff, int64_t nao, int64_t nmo, const double* mo_occ, const double* mo_energy,
-           const CphfConfig& cfg, hipStream_t s)
-    : nao_(nao), nmo_(nmo), cfg_(cfg), stream_(s) {
-  HH_REQUIRE(mo_coeff && mo_occ && mo_energy && nao > 0 && nmo > 0, "cphf: null or empty MO data");
-  std::vector<double> eo, ev;
-  std::vector<int64_t> order;
-  for (int64_t j = 0; j < nmo; ++j)
-    if (mo_occ[j] > 0.0) eo.push_back(mo_energy[j]);
-  for (int64_t j = 0; j < nmo; ++j)
-    if (!(mo_occ[j] > 0.0)) ev.push_back(mo_energy[j]);
-  // the reference slices occ = [0, nocc), vir = [nocc, nmo) (rscf.rs:172-177): occupied orbitals
-  // must come first
-  nocc_ = (int64_t)eo.size();
-  HH_REQUIRE(nocc_ > 0 && nocc_ < nmo, "cphf: need both occupied and virtual orbitals");
-  for (int64_t j = 0; j < nocc_; ++j) HH_REQUIRE(mo_occ[j] > 0.0, "cphf: occupied orbitals must come first");
-  c_.reserve(sizeof(double) * nao * nmo);
-  HH_HIP(hipMemcpyAsync(c_.ptr, mo_coeff, sizeof(double) * nao * nmo, hipMemcpyDeviceToDevice, s));
-  eocc_.reserve(sizeof(double) * nocc_);
-  evir_.reserve(sizeof(double) * (nmo - nocc_));
-  HH_HIP(hipMemcpyAsync(eocc_.ptr, eo.data(), sizeof(double) * nocc_, hipMemcpyHostToDevice, s));
-  HH_HIP(hipMemcpyAsync(evir_.ptr, ev.data(), sizeof(double) * (nmo - nocc_), hipMemcpyHostToDevice, s));
-  HH_HIP(hipStreamSynchronize(s));
+Cphf::Cphf(int nspin, const double* const* mo_coeff, int64_t nao, const int64_t* nmo, const double* const* mo_occ,
+           const double* const* mo_energy, const CphfConfig& cfg, hipStream_t s)
+    : nspin_(nspin), nao_(nao), cfg_(cfg), stream_(s) {
+  HH_REQUIRE(nspin == 1 || nspin == 2, "cphf: nspin must be 1 or 2");
+  HH_REQUIRE(mo_coeff && mo_occ && mo_energy && nmo && nao > 0, "cphf: null MO data");
+  for (int t = 0; t < nspin; ++t) {
+    Spin& sp = sp_[t];
+    HH_REQUIRE(mo_coeff[t] && mo_occ[t] && mo_energy[t] && nmo[t] > 0, "cphf: null or empty MO data");
+    sp.nmo = nmo[t];
+    std::vector<double> eo, ev;
+    for (int64_t j = 0; j < sp.nmo; ++j) (mo_occ[t][j] > 0.0 ? eo : ev).push_back(mo_energy[t][j]);
+    // the reference slices occ = [0, nocc), vir = [nocc, nmo) (rscf.rs:172-177, uscf.rs:165-168):
+    // occupied orbitals must come first
+    sp.nocc = (int64_t)eo.size();
+    HH_REQUIRE(sp.nocc > 0 && sp.nocc < sp.nmo, "cphf: need both occupied and virtual orbitals");
+    for (int64_t j = 0; j < sp.nocc; ++j) HH_REQUIRE(mo_occ[t][j] > 0.0, "cphf: occupied orbitals must come first");
+    sp.off = n_;
+    n_ += sp.nmo * sp.nocc;
+    sp.c.reserve(sizeof(double) * nao * sp.nmo);
+    HH_HIP(hipMemcpyAsync(sp.c.ptr, mo_coeff[t], sizeof(double) * nao * sp.nmo, hipMemcpyDeviceToDevice, s));
+    sp.eocc.reserve(sizeof(double) * sp.nocc);
+    sp.evir.reserve(sizeof(double) * (sp.nmo - sp.nocc));
+    HH_HIP(hipMemcpyAsync(sp.eocc.ptr, eo.data(), sizeof(double) * sp.nocc, hipMemcpyHostToDevice, s));
+    HH_HIP(hipMemcpyAsync(sp.evir.ptr, ev.data(), sizeof(double) * (sp.nmo - sp.nocc), hipMemcpyHostToDevice, s));
+    HH_HIP(hipStreamSynchronize(s));
+  }
 }
 
-void Cphf::dimless_rhs(const double* f1mo, const double* s1mo, int64_t nprop, double* rhs) {
-  hipLaunchKernelGGL(dimless_rhs_kernel, dim3(nblk(nmo_ * nocc_ * nprop)), dim3(256), 0, stream_, f1mo, s1mo,
-                     eocc_.as<double>(), evir_.as<double>(), nmo_, nocc_, nprop, cfg_.level_shift, rhs);
+void Cphf::add_component(ElecResponse* c) {
+  HH_REQUIRE(c && c->nspin() == nspin_, "cphf: component spin count differs from the driver's");
+  comps_.push_back(c);
+}
+
+// ---- stacked vector <-> per-spin arrays (one spin: the same memory, no copy) ----
+const double* Cphf::stack_in(const double* const* v, int64_t nprop, DevBuf& buf) {
+  HH_REQUIRE(v, "cphf: null spin array");
+  for (int t = 0; t < nspin_; ++t) HH_REQUIRE(v[t], "cphf: null spin array");
+  if (nspin_ == 1) return v[0];
+  double* flat = static_cast<double*>(buf.reserve(sizeof(double) * n_ * nprop));
+  for (int t = 0; t < nspin_; ++t) {
+    const int64_t rows = sp_[t].nmo * sp_[t].nocc;
+    hipLaunchKernelGGL(copy_block_kernel, dim3(nblk(rows * nprop)), dim3(256), 0, stream_, v[t], rows,
+                       flat + sp_[t].off, n_, rows, nprop);
+  }
+  HH_HIP(hipGetLastError());
+  return flat;
+}
+
+double* Cphf::stack_out(double* const* v, int64_t nprop, DevBuf& buf) {
+  HH_REQUIRE(v, "cphf: null spin array");
+  for (int t = 0; t < nspin_; ++t) HH_REQUIRE(v[t], "cphf: null spin array");
+  if (nspin_ == 1) return v[0];
+  return static_cast<double*>(buf.reserve(sizeof(double) * n_ * nprop));
+}
+
+void Cphf::unstack(const double* flat, int64_t nprop, double* const* v) {
+  if (nspin_ == 1) return;
+  for (int t = 0; t < nspin_; ++t) {
+    const int64_t rows = sp_[t].nmo * sp_[t].nocc;
+    hipLaunchKernelGGL(copy_block_kernel, dim3(nblk(rows * nprop)), dim3(256), 0, stream_, flat + sp_[t].off, n_,
+                       v[t], rows, rows, nprop);
+  }
   HH_HIP(hipGetLastError());
 }
 
-void Cphf::response_mo(const double* mo1, int64_t nprop, double* resp, Prec prec) {
+void Cphf::dimless_rhs(const double* const* f1mo, const double* const* s1mo, int64_t nprop, double* const* rhs) {
+  HH_REQUIRE(f1mo && s1mo && rhs, "cphf: null spin array");
+  for (int t = 0; t < nspin_; ++t) {
+    const Spin& sp = sp_[t];
+    hipLaunchKernelGGL(dimless_rhs_kernel, dim3(nblk(sp.nmo * sp.nocc * nprop)), dim3(256), 0, stream_, f1mo[t],
+                       s1mo[t], sp.eocc.as<double>(), sp.evir.as<double>(), sp.nmo, sp.nocc, nprop, cfg_.level_shift,
+                       rhs[t]);
+  }
+  HH_HIP(hipGetLastError());
+}
+
+void Cphf::response_mo_stacked(const double* x, int64_t nprop, double* resp, Prec prec) {
   HH_REQUIRE(!comps_.empty(), "cphf: no electron-interaction component added");
-  const int64_t ncol = nocc_ * nprop;
-  double* ubra = static_cast<double*>(ubra_.reserve(sizeof(double) * nao_ * ncol));
-  double* rbra = static_cast<double*>(rbra_.reserve(sizeof(double) * nao_ * ncol));
-  GemmArgs g;  // ubra = C mo1
-  g.M = nao_; g.N = ncol; g.K = nmo_;
-  g.A = c_.ptr; g.lda = nao_; g.a_kmajor = false;
-  g.B = mo1; g.ldb = nmo_; g.b_kmajor = true;
-  g.C = ubra; g.ldc = nao_;
-  gemm<double, double>(g, PREC_F64, stream_, ws_);
-  for (size_t c = 0; c < comps_.size(); ++c) comps_[c]->response_bra(ubra, nprop, rbra, c > 0, prec);
-  GemmArgs h;  // resp = C^T rbra
-  h.M = nmo_; h.N = ncol; h.K = nao_;
-  h.A = c_.ptr; h.lda = nao_; h.a_kmajor = true;
-  h.B = rbra; h.ldb = nao_; h.b_kmajor = true;
-  h.C = resp; h.ldc = nmo_;
-  gemm<double, double>(h, PREC_F64, stream_, ws_);
+  const double* ub[2] = {nullptr, nullptr};
+  double* rb[2] = {nullptr, nullptr};
+  for (int t = 0; t < nspin_; ++t) {
+    Spin& sp = sp_[t];
+    const int64_t ncol = sp.nocc * nprop;
+    double* ubra = static_cast<double*>(sp.ubra.reserve(sizeof(double) * nao_ * ncol));
+    rb[t] = static_cast<double*>(sp.rbra.reserve(sizeof(double) * nao_ * ncol));
+    ub[t] = ubra;
+    GemmArgs g;  // ubra = C mo1 (per perturbation column block when the spins are stacked)
+    g.M = nao_; g.K = sp.nmo;
+    g.A = sp.c.ptr; g.lda = nao_; g.a_kmajor = false;
+    g.B = x + sp.off; g.ldb = sp.nmo; g.b_kmajor = true;
+    g.C = ubra; g.ldc = nao_;
+    if (nspin_ == 1) {
+      g.N = ncol;
+    } else {
+      g.N = sp.nocc; g.nb0 = nprop; g.sA0 = 0; g.sB0 = n_; g.sC0 = nao_ * sp.nocc;
+    }
+    gemm<double, double>(g, PREC_F64, stream_, ws_);
+  }
+  for (size_t c = 0; c < comps_.size(); ++c) comps_[c]->response_bra(ub, nprop, rb, c > 0, prec);
+  for (int t = 0; t < nspin_; ++t) {
+    Spin& sp = sp_[t];
+    GemmArgs h;  // resp = C^T rbra
+    h.M = sp.nmo; h.K = nao_;
+    h.A = sp.c.ptr; h.lda = nao_; h.a_kmajor = true;
+    h.B = rb[t]; h.ldb = nao_; h.b_kmajor = true;
+    h.C = resp + sp.off; h.ldc = sp.nmo;
+    if (nspin_ == 1) {
+      h.N = sp.nocc * nprop;
+    } else {
+      h.N = sp.nocc; h.nb0 = nprop; h.sA0 = 0; h.sB0 = nao_ * sp.nocc; h.sC0 = n_;
+    }
+    gemm<double, double>(h, PREC_F64, stream_, ws_);
+  }
 }
 
-void Cphf::response_dimless(const double* mo1, int64_t nprop, double* resp, Prec prec) {
-  response_mo(mo1, nprop, resp, prec);
-  hipLaunchKernelGGL(dimless_scale_kernel, dim3(nblk(nmo_ * nocc_ * nprop)), dim3(256), 0, stream_, resp, mo1,
-                     eocc_.as<double>(), evir_.as<double>(), nmo_, nocc_, nprop, cfg_.level_shift);
+void Cphf::response_dimless_stacked(const double* x, int64_t nprop, double* resp, Prec prec) {
+  response_mo_stacked(x, nprop, resp, prec);
+  for (int t = 0; t < nspin_; ++t) {
+    const Spin& sp = sp_[t];
+    hipLaunchKernelGGL(dimless_scale_kernel, dim3(nblk(sp.nmo * sp.nocc * nprop)), dim3(256), 0, stream_,
+                       resp + sp.off, x + sp.off, sp.eocc.as<double>(), sp.evir.as<double>(), sp.nmo, sp.nocc, nprop,
+                       n_, cfg_.level_shift);
+  }
   HH_HIP(hipGetLastError());
 }
 
-void Cphf::solve_dimless(const double* rhs, int64_t nprop, double* mo1, Prec prec, CphfStats* st, const double* diag) {
-  const int64_t n = nmo_ * nocc_;
+void Cphf::response_mo(const double* const* mo1, int64_t nprop, double* const* resp, Prec prec) {
+  const double* x = stack_in(mo1, nprop, st_in_);
+  double* r = stack_out(resp, nprop, st_out_);
+  response_mo_stacked(x, nprop, r, prec);
+  unstack(r, nprop, resp);
+}
+
+void Cphf::response_dimless(const double* const* mo1, int64_t nprop, double* const* resp, Prec prec) {
+  const double* x = stack_in(mo1, nprop, st_in_);
+  double* r = stack_out(resp, nprop, st_out_);
+  response_dimless_stacked(x, nprop, r, prec);
+  unstack(r, nprop, resp);
+}
+
+void Cphf::solve_dimless(const double* const* rhs, int64_t nprop, double* const* mo1, Prec prec, CphfStats* st,
+                         const double* diag) {
+  HH_REQUIRE(!diag || nspin_ == 1, "cphf: the diagonal preconditioner is for one spin");
+  const double* b = stack_in(rhs, nprop, st_in_);
+  double* x = stack_out(mo1, nprop, st_out_);
+  solve_stacked(b, nprop, x, prec, st, diag);
+  unstack(x, nprop, mo1);
+}
+
+void Cphf::solve_stacked(const double* rhs, int64_t nprop, double* mo1, Prec prec, CphfStats* st,
+                         const double* diag) {
+  const int64_t n = n_;
   const int64_t blk = n * nprop;
+  const int64_t nmo0 = sp_[0].nmo, nocc0 = sp_[0].nocc;
   CphfStats stats;
   double* y = static_cast<double*>(ydiag_.reserve(sizeof(double) * blk * 2));
   double* ytmp = y + blk;
   // aop on the (optionally right-preconditioned) system; `p` is the response precision
   auto make_aop = [&](Prec p) {
-    return [this, p, diag, y, ytmp](const double* x, int64_t nb, double* ax) {
+    return [this, p, diag, ytmp, nmo0, nocc0](const double* xv, int64_t nb, double* ax) {
       if (!diag) {
-        response_dimless(x, nb, ax, p);
+        response_dimless_stacked(xv, nb, ax, p);
         return;
       }
       // (I + A) D x = (I + A') x with A' x = (D - 1) x + A (D x)
-      hipLaunchKernelGGL(apply_diag_kernel, dim3(nblk(nmo_ * nocc_ * nb)), dim3(256), 0, stream_, x, diag, nmo_,
-                         nocc_, nb, ytmp);
-      response_dimless(ytmp, nb, ax, p);
-      hipLaunchKernelGGL(precond_fix_kernel, dim3(nblk(nmo_ * nocc_ * nb)), dim3(256), 0, stream_, x, diag, nmo_,
-                         nocc_, nb, ax);
+      hipLaunchKernelGGL(apply_diag_kernel, dim3(nblk(nmo0 * nocc0 * nb)), dim3(256), 0, stream_, xv, diag, nmo0,
+                         nocc0, nb, ytmp);
+      response_dimless_stacked(ytmp, nb, ax, p);
+      hipLaunchKernelGGL(precond_fix_kernel, dim3(nblk(nmo0 * nocc0 * nb)), dim3(256), 0, stream_, xv, diag, nmo0,
+                         nocc0, nb, ax);
       HH_HIP(hipGetLastError());
     };
   };
   auto unprecond = [&](const double* yv, double* out, int64_t nb) {
     if (diag) {
-      hipLaunchKernelGGL(apply_diag_kernel, dim3(nblk(n * nb)), dim3(256), 0, stream_, yv, diag, nmo_, nocc_, nb, out);
+      hipLaunchKernelGGL(apply_diag_kernel, dim3(nblk(n * nb)), dim3(256), 0, stream_, yv, diag, nmo0, nocc0, nb,
+                         out);
       HH_HIP(hipGetLastError());
     } else {
-      HH_HIP(hipMemcpyAsync(out, yv, sizeof(double) * n * nb, hipMemcpyDeviceToDevice, stream_));
+      copy_bytes(out, yv, (int64_t)sizeof(double) * n * nb, stream_);
     }
   };
   auto accumulate = [&](const KrylovStats& k) {
@@ -375,7 +480,7 @@ void Cphf::solve_dimless(const double* rhs, int64_t nprop, double* mo1, Prec pre
     unprecond(y, dx, nprop);
     hipLaunchKernelGGL(add_unscaled_kernel, dim3(nblk(blk)), dim3(256), 0, stream_, mo1, dx, n, nprop, csc);
     HH_HIP(hipGetLastError());
-    response_dimless(mo1, nprop, ax, PREC_F64);
+    response_dimless_stacked(mo1, nprop, ax, PREC_F64);
     ++stats.krylov.aop_calls;
     hipLaunchKernelGGL(residual_norm_kernel, dim3((unsigned)nprop), dim3(256), 0, stream_, rhs, mo1, ax, r, n, coln);
     HH_HIP(hipGetLastError());
@@ -397,62 +502,95 @@ void Cphf::solve_dimless(const double* rhs, int64_t nprop, double* mo1, Prec pre
   }
 }
 
-void Cphf::finalize(const double* f1mo, const double* s1mo, const double* mo1, int64_t nprop, double* mo1_out,
-                    double* mo_e1) {
-  const int64_t blk = nmo_ * nocc_ * nprop;
-  double* b1 = static_cast<double*>(tmp_.reserve(sizeof(double) * blk));
-  response_mo(mo1, nprop, b1, PREC_F64);
-  hipLaunchKernelGGL(finalize_kernel, dim3(nblk(blk)), dim3(256), 0, stream_, f1mo, s1mo, mo1, b1, eocc_.as<double>(),
-                     evir_.as<double>(), nmo_, nocc_, nprop, mo1_out, mo_e1);
+void Cphf::finalize(const double* const* f1mo, const double* const* s1mo, const double* const* mo1, int64_t nprop,
+                    double* const* mo1_out, double* const* mo_e1) {
+  HH_REQUIRE(f1mo && s1mo && mo1 && mo1_out && mo_e1, "cphf: null spin array");
+  // last response in FP64 (per spin, into one scratch block)
+  const double* x = stack_in(mo1, nprop, st_in_);
+  double* b1 = static_cast<double*>(tmp_.reserve(sizeof(double) * n_ * nprop));
+  response_mo_stacked(x, nprop, b1, PREC_F64);
+  double* b1s[2] = {b1, nullptr};
+  if (nspin_ == 2) {
+    double* aux = static_cast<double*>(st_aux_.reserve(sizeof(double) * n_ * nprop));
+    b1s[0] = aux;
+    b1s[1] = aux + sp_[0].nmo * sp_[0].nocc * nprop;
+    unstack(b1, nprop, b1s);
+  }
+  for (int t = 0; t < nspin_; ++t) {
+    const Spin& sp = sp_[t];
+    const int64_t blk = sp.nmo * sp.nocc * nprop;
+    hipLaunchKernelGGL(finalize_kernel, dim3(nblk(blk)), dim3(256), 0, stream_, f1mo[t], s1mo[t], mo1[t], b1s[t],
+                       sp.eocc.as<double>(), sp.evir.as<double>(), sp.nmo, sp.nocc, nprop, mo1_out[t], mo_e1[t]);
+  }
   HH_HIP(hipGetLastError());
 }
 
 void Cphf::precond_features(const double* pair, const double* r, int64_t nprop, float* x) {
-  const int64_t ns = (nmo_ - nocc_) * nocc_;
-  hipLaunchKernelGGL(raw_features_kernel, dim3(nblk(ns)), dim3(256), 0, stream_, pair, r, eocc_.as<double>(),
-                     evir_.as<double>(), nmo_, nocc_, nprop, x);
+  HH_REQUIRE(nspin_ == 1, "cphf: the learned preconditioner is for one spin");
+  const Spin& sp = sp_[0];
+  const int64_t ns = (sp.nmo - sp.nocc) * sp.nocc;
+  hipLaunchKernelGGL(raw_features_kernel, dim3(nblk(ns)), dim3(256), 0, stream_, pair, r, sp.eocc.as<double>(),
+                     sp.evir.as<double>(), sp.nmo, sp.nocc, nprop, x);
   hipLaunchKernelGGL(standardise_kernel, dim3(3), dim3(256), 0, stream_, x, ns);
   HH_HIP(hipGetLastError());
 }
 
 void Cphf::precond_target(const double* r, const double* xs, int64_t nprop, float* t, float* w) {
-  const int64_t ns = (nmo_ - nocc_) * nocc_;
-  hipLaunchKernelGGL(lsq_target_kernel, dim3(nblk(ns)), dim3(256), 0, stream_, r, xs, nmo_, nocc_, nprop, t, w);
+  HH_REQUIRE(nspin_ == 1, "cphf: the learned preconditioner is for one spin");
+  const Spin& sp = sp_[0];
+  const int64_t ns = (sp.nmo - sp.nocc) * sp.nocc;
+  hipLaunchKernelGGL(lsq_target_kernel, dim3(nblk(ns)), dim3(256), 0, stream_, r, xs, sp.nmo, sp.nocc, nprop, t, w);
   HH_HIP(hipGetLastError());
 }
 
 void Cphf::precond_diag(const float* y, double* diag) {
-  hipLaunchKernelGGL(precond_diag_kernel, dim3(nblk(nmo_ * nocc_)), dim3(256), 0, stream_, y, nmo_, nocc_, diag);
+  HH_REQUIRE(nspin_ == 1, "cphf: the learned preconditioner is for one spin");
+  const Spin& sp = sp_[0];
+  hipLaunchKernelGGL(precond_diag_kernel, dim3(nblk(sp.nmo * sp.nocc)), dim3(256), 0, stream_, y, sp.nmo, sp.nocc,
+                     diag);
   HH_HIP(hipGetLastError());
 }
 
-void Cphf::cphf_hess(const double* f1mo, const double* s1mo, const double* mo1, const double* mo_e1, int64_t natm,
-                     double* de) {
-  const int64_t n3 = 3 * natm, nn = nmo_ * nocc_, no2 = nocc_ * nocc_;
+void Cphf::cphf_hess(const double* const* f1mo, const double* const* s1mo, const double* const* mo1,
+                     const double* const* mo_e1, int64_t natm, double* de) {
+  HH_REQUIRE(f1mo && s1mo && mo1 && mo_e1, "cphf: null spin array");
+  const int64_t n3 = 3 * natm;
   double* G = static_cast<double*>(g1_.reserve(sizeof(double) * n3 * n3));
-  double* se = static_cast<double*>(sx_.reserve(sizeof(double) * nn * n3));
-  double* soo = static_cast<double*>(g2_.reserve(sizeof(double) * no2 * n3));
-  hipLaunchKernelGGL(scale_by_eocc_kernel, dim3(nblk(nn * n3)), dim3(256), 0, stream_, s1mo, eocc_.as<double>(), nmo_,
-                     nocc_, n3, se);
-  hipLaunchKernelGGL(occ_rows_kernel, dim3(nblk(no2 * n3)), dim3(256), 0, stream_, s1mo, nmo_, nocc_, n3, soo);
-  HH_HIP(hipGetLastError());
-  // G[(t,A),(s,B)] = 4 f1^T mo1 - 4 (s1 e)^T mo1 - 2 s1oo^T mo_e1
-  GemmArgs g;
-  g.M = n3; g.N = n3; g.K = nn;
-  g.A = f1mo; g.lda = nn; g.a_kmajor = true;
-  g.B = mo1; g.ldb = nn; g.b_kmajor = true;
-  g.C = G; g.ldc = n3;
-  g.alpha = 4.0; g.beta = 0.0;
-  gemm<double, double>(g, PREC_F64, stream_, ws_);
-  g.A = se; g.alpha = -4.0; g.beta = 1.0;
-  gemm<double, double>(g, PREC_F64, stream_, ws_);
-  GemmArgs h;
-  h.M = n3; h.N = n3; h.K = no2;
-  h.A = soo; h.lda = no2; h.a_kmajor = true;
-  h.B = mo_e1; h.ldb = no2; h.b_kmajor = true;
-  h.C = G; h.ldc = n3;
-  h.alpha = -2.0; h.beta = 1.0;
-  gemm<double, double>(h, PREC_F64, stream_, ws_);
+  // G[(t,A),(s,B)] = sum_spin f (2 f1^T mo1 - 2 (s1 e)^T mo1 - s1oo^T mo_e1) with f = 2 for one
+  // (doubly occupied) spin, 1 per spin for two (rscf.rs:382-388, uscf.rs:425-431)
+  const double f = nspin_ == 1 ? 2.0 : 1.0;
+  int64_t nn_max = 0, no2_max = 0;  // scratch sized once: no reallocation under queued work
+  for (int t = 0; t < nspin_; ++t) {
+    nn_max = std::max(nn_max, sp_[t].nmo * sp_[t].nocc);
+    no2_max = std::max(no2_max, sp_[t].nocc * sp_[t].nocc);
+  }
+  double* se = static_cast<double*>(sx_.reserve(sizeof(double) * nn_max * n3));
+  double* soo = static_cast<double*>(g2_.reserve(sizeof(double) * no2_max * n3));
+  for (int t = 0; t < nspin_; ++t) {
+    const Spin& sp = sp_[t];
+    const int64_t nn = sp.nmo * sp.nocc, no2 = sp.nocc * sp.nocc;
+    hipLaunchKernelGGL(scale_by_eocc_kernel, dim3(nblk(nn * n3)), dim3(256), 0, stream_, s1mo[t],
+                       sp.eocc.as<double>(), sp.nmo, sp.nocc, n3, se);
+    hipLaunchKernelGGL(occ_rows_kernel, dim3(nblk(no2 * n3)), dim3(256), 0, stream_, s1mo[t], sp.nmo, sp.nocc, n3,
+                       soo);
+    HH_HIP(hipGetLastError());
+    GemmArgs g;
+    g.M = n3; g.N = n3; g.K = nn;
+    g.A = f1mo[t]; g.lda = nn; g.a_kmajor = true;
+    g.B = mo1[t]; g.ldb = nn; g.b_kmajor = true;
+    g.C = G; g.ldc = n3;
+    g.alpha = 2.0 * f; g.beta = t == 0 ? 0.0 : 1.0;
+    gemm<double, double>(g, PREC_F64, stream_, ws_);
+    g.A = se; g.alpha = -2.0 * f; g.beta = 1.0;
+    gemm<double, double>(g, PREC_F64, stream_, ws_);
+    GemmArgs h;
+    h.M = n3; h.N = n3; h.K = no2;
+    h.A = soo; h.lda = no2; h.a_kmajor = true;
+    h.B = mo_e1[t]; h.ldb = no2; h.b_kmajor = true;
+    h.C = G; h.ldc = n3;
+    h.alpha = -f; h.beta = 1.0;
+    gemm<double, double>(h, PREC_F64, stream_, ws_);
+  }
   hipLaunchKernelGGL(assemble_hess_kernel, dim3(nblk(9 * natm * natm)), dim3(256), 0, stream_, G, natm, de);
   HH_HIP(hipGetLastError());
 }

@@ -1,4 +1,5 @@
-// Dimensionless CPHF driver for restricted SCF (reference: src/hessian/rscf.rs:104-425).
+// Dimensionless CPHF driver for restricted (one spin) and unrestricted (two spins) SCF
+// (reference: src/hessian/rscf.rs:104-425, src/hessian/uscf.rs:78-446).
 #pragma once
 #include <vector>
 
@@ -7,11 +8,14 @@
 namespace hh {
 
 // An electron-interaction component that supplies get_response_bra (the reference's
-// RHessElecInteractAPI::get_response_bra, trait_rhess.rs:81-190): out[nao, nocc, nprop]
-// (+)= response of bra[nao, nocc, nprop].
+// RHessElecInteractAPI / UHessElecInteractAPI::get_response_bra, trait_rhess.rs:81-190,
+// trait_uhess.rs:57-172): out[s][nao, nocc_s, nprop] (+)= response of bra[s][nao, nocc_s, nprop]
+// for every spin s < nspin().
 struct ElecResponse {
   virtual ~ElecResponse() = default;
-  virtual void response_bra(const double* bra, int64_t nprop, double* out, bool accumulate, Prec prec) = 0;
+  virtual int nspin() const = 0;
+  virtual void response_bra(const double* const* bra, int64_t nprop, double* const* out, bool accumulate,
+                            Prec prec) = 0;
 };
 
 struct CphfConfig {
@@ -30,39 +34,42 @@ struct CphfStats {
 
 class Cphf {
  public:
-  Cphf(const double* mo_coeff, int64_t nao, int64_t nmo, const double* mo_occ, const double* mo_energy,
-       const CphfConfig& cfg, hipStream_t s);
-  void add_component(ElecResponse* c) { comps_.push_back(c); }
+  // mo_coeff[s] device [nao, nmo_s]; mo_occ[s], mo_energy[s] host [nmo_s] (occupied first)
+  Cphf(int nspin, const double* const* mo_coeff, int64_t nao, const int64_t* nmo, const double* const* mo_occ,
+       const double* const* mo_energy, const CphfConfig& cfg, hipStream_t s);
+  void add_component(ElecResponse* c);
 
-  int64_t nocc() const { return nocc_; }
-  int64_t nmo() const { return nmo_; }
+  int nspin() const { return nspin_; }
+  int64_t nocc(int s = 0) const { return sp_[s].nocc; }
+  int64_t nmo(int s = 0) const { return sp_[s].nmo; }
   int64_t nao() const { return nao_; }
-  const double* mo_coeff() const { return c_.as<double>(); }
+  const double* mo_coeff(int s = 0) const { return sp_[s].c.as<double>(); }
 
-  // rscf.rs:170-178
-  void dimless_rhs(const double* f1mo, const double* s1mo, int64_t nprop, double* rhs);
-  // rscf.rs:210-220
-  void response_mo(const double* mo1, int64_t nprop, double* resp, Prec prec);
-  // rscf.rs:237-263
-  void response_dimless(const double* mo1, int64_t nprop, double* resp, Prec prec);
-  // rscf.rs:278-300.  prec != FP64: the Krylov solve runs on the mixed-precision response and
-  // is wrapped in FP64 iterative refinement until the FP64 residual satisfies the tolerance.
-  // `diag` (optional, device [nmo, nocc]): a diagonal preconditioner applied as
-  // x = diag * y, i.e. the solve runs on (I + A) diag y = b (vir rows; occ rows keep 1).
-  void solve_dimless(const double* rhs, int64_t nprop, double* mo1, Prec prec, CphfStats* st,
+  // Per-spin arrays [nmo_s, nocc_s, nprop] (mo_e1: [nocc_s, nocc_s, nprop]); one pointer per spin.
+  // rscf.rs:170-178 / uscf.rs:165-174
+  void dimless_rhs(const double* const* f1mo, const double* const* s1mo, int64_t nprop, double* const* rhs);
+  // rscf.rs:210-220 / uscf.rs:203-218
+  void response_mo(const double* const* mo1, int64_t nprop, double* const* resp, Prec prec);
+  // rscf.rs:237-263 / uscf.rs:229-265
+  void response_dimless(const double* const* mo1, int64_t nprop, double* const* resp, Prec prec);
+  // rscf.rs:278-300 / uscf.rs:278-349 (the spin blocks stacked per column, as the reference's
+  // pack_flattened).  prec != FP64: Krylov on the mixed-precision response inside FP64 iterative
+  // refinement until the FP64 residual satisfies the tolerance.  `diag` (optional, one spin only,
+  // device [nmo, nocc]): right diagonal preconditioner x = diag * y on the virtual rows.
+  void solve_dimless(const double* const* rhs, int64_t nprop, double* const* mo1, Prec prec, CphfStats* st,
                      const double* diag = nullptr);
-  // rscf.rs:328-353
-  void finalize(const double* f1mo, const double* s1mo, const double* mo1, int64_t nprop, double* mo1_out,
-                double* mo_e1);
-  // rscf.rs:371-399: de [3, 3, natm, natm]; inputs [nmo|nocc, nocc, 3, natm]
-  void cphf_hess(const double* f1mo, const double* s1mo, const double* mo1, const double* mo_e1, int64_t natm,
-                 double* de);
+  // rscf.rs:328-353 / uscf.rs:369-407
+  void finalize(const double* const* f1mo, const double* const* s1mo, const double* const* mo1, int64_t nprop,
+                double* const* mo1_out, double* const* mo_e1);
+  // rscf.rs:371-399 / uscf.rs:409-440: de [3, 3, natm, natm]; inputs [.., .., 3, natm] per spin
+  void cphf_hess(const double* const* f1mo, const double* const* s1mo, const double* const* mo1,
+                 const double* const* mo_e1, int64_t natm, double* de);
 
   hipStream_t stream() const { return stream_; }
   bool verbose = false;  // per-sweep diagnostics on stderr (the reference prints per cycle)
 
-  // Learned-preconditioner support (virtual rows only; N = nvir * nocc samples, sample index
-  // a + nvir * i).  Features x [N, 3] (float, column-major), standardised over the samples:
+  // Learned-preconditioner support (one spin; virtual rows only; N = nvir * nocc samples, sample
+  // index a + nvir * i).  Features x [N, 3] (float, column-major), standardised over the samples:
   // ln(e_a - e_i), ln(pair[a,i]) (RI pair norm (ai|ai)), ln(rms_p |r[a,i,p]|).
   void precond_features(const double* pair, const double* r, int64_t nprop, float* x);
   // Least-squares diagonal mapping residual r to solution x over the perturbations:
@@ -72,12 +79,25 @@ class Cphf {
   void precond_diag(const float* y, double* diag);
 
  private:
-  int64_t nao_, nmo_, nocc_;
+  struct Spin {
+    int64_t nmo = 0, nocc = 0, off = 0;  // off: row offset of the spin block in the stacked vector
+    DevBuf c, eocc, evir, ubra, rbra;
+  };
+  // The stacked vector [n, nprop], n = sum_s nmo_s nocc_s (one spin: the [nmo, nocc, nprop] array).
+  void response_mo_stacked(const double* x, int64_t nprop, double* resp, Prec prec);
+  void response_dimless_stacked(const double* x, int64_t nprop, double* resp, Prec prec);
+  void solve_stacked(const double* rhs, int64_t nprop, double* x, Prec prec, CphfStats* st, const double* diag);
+  const double* stack_in(const double* const* v, int64_t nprop, DevBuf& buf);
+  double* stack_out(double* const* v, int64_t nprop, DevBuf& buf);
+  void unstack(const double* flat, int64_t nprop, double* const* v);
+
+  int nspin_;
+  int64_t nao_, n_ = 0;
+  Spin sp_[2];
   CphfConfig cfg_;
   hipStream_t stream_;
   std::vector<ElecResponse*> comps_;
-  DevBuf c_, eocc_, evir_, mocc_;
-  DevBuf ubra_, rbra_, tmp_, ws_, res_, g1_, g2_, g3_, sx_, ydiag_;
+  DevBuf tmp_, ws_, res_, g1_, g2_, g3_, sx_, ydiag_, st_in_, st_out_, st_aux_;
 };
 
 }  // namespace hh

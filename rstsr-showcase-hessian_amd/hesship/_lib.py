@@ -68,6 +68,13 @@ SIGNATURES.update({
     "hh_cphf_solve_dimless": (_i32, [_p, _p, _i64, _p, _i32, _p, C.POINTER(KrylovStats)]),
     "hh_cphf_finalize": (_i32, [_p, _p, _p, _p, _i64, _p, _p]),
     "hh_cphf_hess": (_i32, [_p, _p, _p, _p, _p, _i64, _p]),
+    "hh_ucphf_create": (_i32, [_pp, _i64, _pi64, _pp, _pp, _d, _d, _i32, _i32, _d, _p, C.POINTER(C.c_void_p)]),
+    "hh_ucphf_dimless_rhs": (_i32, [_p, _pp, _pp, _i64, _pp]),
+    "hh_ucphf_response_mo": (_i32, [_p, _pp, _i64, _pp, _i32]),
+    "hh_ucphf_response_dimless": (_i32, [_p, _pp, _i64, _pp, _i32]),
+    "hh_ucphf_solve_dimless": (_i32, [_p, _pp, _i64, _pp, _i32, C.POINTER(KrylovStats)]),
+    "hh_ucphf_finalize": (_i32, [_p, _pp, _pp, _pp, _i64, _pp, _pp]),
+    "hh_ucphf_hess": (_i32, [_p, _pp, _pp, _pp, _pp, _i64, _p]),
     "hh_linalg_solve": (_i32, [_p, _i64, _p, _i64, _p]),
     "hh_linalg_cholesky": (_i32, [_p, _i64, _p]),
     "hh_linalg_trsm_lower": (_i32, [_p, _i64, _p, _i64, _i64, _i32, _p]),

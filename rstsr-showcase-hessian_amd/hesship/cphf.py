@@ -1,10 +1,11 @@
-"""Dimensionless CPHF driver for restricted SCF (reference: src/hessian/rscf.rs)."""
+"""Dimensionless CPHF drivers for restricted and unrestricted SCF (reference:
+src/hessian/rscf.rs, src/hessian/uscf.rs)."""
 import ctypes as C
 
 import numpy as np
 import torch
 
-from ._lib import PREC, KrylovStats, check, lib
+from ._lib import PREC, KrylovStats, check, lib, ptr_array
 from .tensors import empty_f, fortran, stream_of
 
 DEFAULT_CONFIG = dict(level_shift=0.0, cphf_tol=1e-8, cphf_max_cycle=42, cphf_max_space=14, cphf_lindep=1e-14)
@@ -113,6 +114,126 @@ class RHessCPHF:
 
     def make_cphf_hess(self, f1mo, s1mo, precision="fp64", stats=None):
         """rscf.rs:412-425: rhs -> Krylov -> finalize -> de_cphf."""
+        rhs = self.compute_dimless_rhs(f1mo, s1mo)
+        mo1 = self.solve_dimless_cphf(rhs, precision=precision, stats=stats)
+        mo1, mo_e1 = self.finalize_cphf(f1mo, s1mo, mo1)
+        return self.get_cphf_hess(f1mo, s1mo, mo1, mo_e1)
+
+
+def _host(x):
+    return np.ascontiguousarray(np.asarray(x.cpu() if isinstance(x, torch.Tensor) else x, dtype=np.float64))
+
+
+class UHessCPHF:
+    """The CPHF part of UHessSCF (uscf.rs:32-446) over device tensors: every argument and result
+    is a pair (alpha, beta).  mo_coeff[s] [nao, nmo] (device); mo_occ[s], mo_energy[s] [nmo]."""
+
+    def __init__(self, mo_coeff, mo_occ, mo_energy, config=None):
+        cfg = dict(DEFAULT_CONFIG, **(config or {}))
+        self.device = mo_coeff[0].device
+        self.nao = mo_coeff[0].shape[0]
+        self.nmo = [int(c.shape[1]) for c in mo_coeff]
+        occ = [_host(o) for o in mo_occ]
+        en = [_host(e) for e in mo_energy]
+        self.nocc = [int((o > 0).sum()) for o in occ]
+        self._c = [fortran(c) for c in mo_coeff]
+        self._host = (occ, en)
+        nmo = (C.c_int64 * 2)(*self.nmo)
+        h = C.c_void_p()
+        check(lib().hh_ucphf_create(ptr_array([c.data_ptr() for c in self._c]), self.nao, nmo,
+                                    ptr_array([o.ctypes.data for o in occ]), ptr_array([e.ctypes.data for e in en]),
+                                    cfg["level_shift"], cfg["cphf_tol"], cfg["cphf_max_cycle"], cfg["cphf_max_space"],
+                                    cfg["cphf_lindep"], stream_of(self.device), C.byref(h)))
+        self._h = h
+        self._els = []
+        self.config = cfg
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                lib().hh_cphf_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def set_verbose(self, on=True):
+        check(lib().hh_cphf_set_verbose(self._h, int(bool(on))))
+
+    def add_el(self, rijk):
+        """Append a UHF RI-JK component (UHessRIJK, prepared) to the response list."""
+        check(lib().hh_cphf_add_rijk(self._h, rijk._h))
+        self._els.append(rijk)
+
+    def _flat(self, pair):
+        shapes = [tuple(t.shape) for t in pair]
+        return [fortran(t.reshape(t.shape[0], t.shape[1], -1)) for t in pair], shapes
+
+    def _out(self, like):
+        return [empty_f(tuple(t.shape), self.device) for t in like]
+
+    def compute_dimless_rhs(self, f1mo, s1mo):
+        f, shapes = self._flat(f1mo)
+        s, _ = self._flat(s1mo)
+        out = self._out(f)
+        check(lib().hh_ucphf_dimless_rhs(self._h, ptr_array([x.data_ptr() for x in f]),
+                                         ptr_array([x.data_ptr() for x in s]), f[0].shape[2],
+                                         ptr_array([x.data_ptr() for x in out])))
+        return [o.reshape(sh) for o, sh in zip(out, shapes)]
+
+    def response_mo(self, mo1, precision="fp64"):
+        m, shapes = self._flat(mo1)
+        out = self._out(m)
+        check(lib().hh_ucphf_response_mo(self._h, ptr_array([x.data_ptr() for x in m]), m[0].shape[2],
+                                         ptr_array([x.data_ptr() for x in out]), PREC[precision]))
+        return [o.reshape(sh) for o, sh in zip(out, shapes)]
+
+    def response_dimless_cphf(self, mo1, precision="fp64"):
+        m, shapes = self._flat(mo1)
+        out = self._out(m)
+        check(lib().hh_ucphf_response_dimless(self._h, ptr_array([x.data_ptr() for x in m]), m[0].shape[2],
+                                              ptr_array([x.data_ptr() for x in out]), PREC[precision]))
+        return [o.reshape(sh) for o, sh in zip(out, shapes)]
+
+    def solve_dimless_cphf(self, rhs, precision="fp64", stats=None):
+        r, shapes = self._flat(rhs)
+        out = self._out(r)
+        st = KrylovStats()
+        rc = lib().hh_ucphf_solve_dimless(self._h, ptr_array([x.data_ptr() for x in r]), r[0].shape[2],
+                                          ptr_array([x.data_ptr() for x in out]), PREC[precision], C.byref(st))
+        if stats is not None:
+            stats.update(st.as_dict())
+        check(rc)
+        return [o.reshape(sh) for o, sh in zip(out, shapes)]
+
+    def finalize_cphf(self, f1mo, s1mo, mo1):
+        f, shapes = self._flat(f1mo)
+        s, _ = self._flat(s1mo)
+        m, _ = self._flat(mo1)
+        nprop = f[0].shape[2]
+        mo1_out = self._out(f)
+        mo_e1 = [empty_f((no, no, nprop), self.device) for no in self.nocc]
+        check(lib().hh_ucphf_finalize(self._h, ptr_array([x.data_ptr() for x in f]),
+                                      ptr_array([x.data_ptr() for x in s]), ptr_array([x.data_ptr() for x in m]), nprop,
+                                      ptr_array([x.data_ptr() for x in mo1_out]),
+                                      ptr_array([x.data_ptr() for x in mo_e1])))
+        return ([o.reshape(sh) for o, sh in zip(mo1_out, shapes)],
+                [e.reshape((no, no) + sh[2:]) for e, no, sh in zip(mo_e1, self.nocc, shapes)])
+
+    def get_cphf_hess(self, f1mo, s1mo, mo1, mo_e1):
+        natm = f1mo[0].shape[-1]
+        f = [fortran(x.reshape(nm, no, 3, natm)) for x, nm, no in zip(f1mo, self.nmo, self.nocc)]
+        s = [fortran(x.reshape(nm, no, 3, natm)) for x, nm, no in zip(s1mo, self.nmo, self.nocc)]
+        m = [fortran(x.reshape(nm, no, 3, natm)) for x, nm, no in zip(mo1, self.nmo, self.nocc)]
+        e = [fortran(x.reshape(no, no, 3, natm)) for x, no in zip(mo_e1, self.nocc)]
+        de = empty_f((3, 3, natm, natm), self.device)
+        check(lib().hh_ucphf_hess(self._h, ptr_array([x.data_ptr() for x in f]), ptr_array([x.data_ptr() for x in s]),
+                                  ptr_array([x.data_ptr() for x in m]), ptr_array([x.data_ptr() for x in e]), natm,
+                                  de.data_ptr()))
+        return de
+
+    def make_cphf_hess(self, f1mo, s1mo, precision="fp64", stats=None):
+        """uscf.rs:442-446: rhs -> Krylov -> finalize -> de_cphf."""
         rhs = self.compute_dimless_rhs(f1mo, s1mo)
         mo1 = self.solve_dimless_cphf(rhs, precision=precision, stats=stats)
         mo1, mo_e1 = self.finalize_cphf(f1mo, s1mo, mo1)

@@ -1,0 +1,125 @@
+"""GPU parity: unrestricted dimensionless CPHF driver (uscf.rs) with the UHF RI-JK response,
+against the oracle; closed shells through both drivers give the same Hessian."""
+import numpy as np
+import pytest
+
+from oracle import cphf as ocp
+from oracle import rijk as orc
+from tests.test_gpu_cphf import T, rel
+from tests.test_oracle_rijk import synthetic_ri
+
+pytestmark = pytest.mark.gpu
+
+
+def usystem(nao=24, naux=50, nocc=(5, 4), natm=3, seed=0, sk=0.5):
+    rng = np.random.default_rng(seed)
+    _, _, cderi = synthetic_ri(nao, naux, seed=seed + 1)
+    cderi *= 0.15
+    C = [np.linalg.qr(rng.standard_normal((nao, nao)))[0] for _ in range(2)]
+    occ = [np.where(np.arange(nao) < no, 1.0, 0.0) for no in nocc]
+    e = []
+    for no in nocc:
+        x = np.sort(rng.uniform(-1.0, 0.5, nao)); x[no:] += 1.0
+        e.append(x)
+    f1 = [rng.standard_normal((nao, no, 3, natm)) for no in nocc]
+    s1 = [rng.standard_normal((nao, no, 3, natm)) * 0.1 for no in nocc]
+    rb = lambda b: orc.uhf_response_bra(cderi, C, occ, b, 1.0, sk)
+    return cderi, C, occ, e, f1, s1, rb
+
+
+def umake(dev, cderi, C, occ, e, sk=0.5, cfg=None):
+    import hesship
+    rijk = hesship.UHessRIJK(T(cderi, dev), 1.0, sk)
+    rijk.make_response_preparation([T(c, dev) for c in C], occ)
+    drv = hesship.UHessCPHF([T(c, dev) for c in C], occ, e, cfg)
+    drv.add_el(rijk)
+    return drv, rijk
+
+
+def np_(pair):
+    return [p.cpu().numpy() for p in pair]
+
+
+def test_urhs_and_responses(dev):
+    cderi, C, occ, e, f1, s1, rb = usystem()
+    drv, _ = umake(dev, cderi, C, occ, e)
+    rhs = np_(drv.compute_dimless_rhs([T(x, dev) for x in f1], [T(x, dev) for x in s1]))
+    for s, r in enumerate(ocp.u_dimless_rhs(f1, s1, occ, e)):
+        np.testing.assert_allclose(rhs[s], r, rtol=1e-14, atol=1e-15)
+    rng = np.random.default_rng(1)
+    mo1 = [rng.standard_normal(x.shape) for x in f1]
+    got = np_(drv.response_mo([T(x, dev) for x in mo1]))
+    for s, r in enumerate(ocp.u_response_mo(C, [rb], mo1)):
+        assert rel(got[s], r) < 1e-10
+    got = np_(drv.response_dimless_cphf([T(x, dev) for x in mo1]))
+    for s, r in enumerate(ocp.u_response_dimless(C, occ, e, [rb], mo1)):
+        assert rel(got[s], r) < 1e-10
+
+
+def test_usolve_finalize_hessian_fp64(dev):
+    cderi, C, occ, e, f1, s1, rb = usystem(seed=2)
+    cfg = dict(cphf_tol=1e-10)
+    drv, _ = umake(dev, cderi, C, occ, e, cfg=cfg)
+    rhs_o = ocp.u_dimless_rhs(f1, s1, occ, e)
+    mo1_o = ocp.u_solve_dimless(C, occ, e, [rb], rhs_o, cfg)
+    st = {}
+    mo1 = np_(drv.solve_dimless_cphf([T(x, dev) for x in rhs_o], stats=st))
+    assert st["cycles"] > 0
+    for s in range(2):
+        np.testing.assert_allclose(mo1[s], mo1_o[s], rtol=1e-7, atol=1e-9)
+    m_o, e1_o = ocp.u_finalize(C, occ, e, [rb], f1, s1, mo1_o)
+    m, e1 = drv.finalize_cphf([T(x, dev) for x in f1], [T(x, dev) for x in s1], [T(x, dev) for x in mo1])
+    for s in range(2):
+        np.testing.assert_allclose(m[s].cpu().numpy(), m_o[s], rtol=1e-7, atol=1e-9)
+        np.testing.assert_allclose(e1[s].cpu().numpy(), e1_o[s], rtol=1e-7, atol=1e-9)
+    de_o = ocp.u_cphf_hess(f1, s1, m_o, e1_o, occ, e)
+    de = drv.get_cphf_hess([T(x, dev) for x in f1], [T(x, dev) for x in s1], m, e1).cpu().numpy()
+    assert rel(de, de_o) < 1e-8
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_usolve_mixed_precision_refined(dev, prec):
+    cderi, C, occ, e, f1, s1, rb = usystem(seed=4)
+    tol = 1e-9
+    drv, _ = umake(dev, cderi, C, occ, e, cfg=dict(cphf_tol=tol))
+    rhs = [T(x, dev) for x in ocp.u_dimless_rhs(f1, s1, occ, e)]
+    ref = np_(drv.solve_dimless_cphf(rhs))
+    st = {}
+    got = np_(drv.solve_dimless_cphf(rhs, precision=prec, stats=st))
+    assert st["residual"] < tol and st["refinements"] >= 1
+    for s in range(2):
+        assert np.abs(got[s] - ref[s]).max() < 1e-7
+
+
+def test_closed_shell_through_both_drivers(dev):
+    import hesship
+    nao, nocc, natm = 24, 5, 2
+    rng = np.random.default_rng(9)
+    _, _, cderi = synthetic_ri(nao, 50, seed=10)
+    cderi *= 0.15
+    C = np.linalg.qr(rng.standard_normal((nao, nao)))[0]
+    e = np.sort(rng.uniform(-1.0, 0.5, nao)); e[nocc:] += 1.0
+    f1 = rng.standard_normal((nao, nocc, 3, natm))
+    s1 = rng.standard_normal((nao, nocc, 3, natm)) * 0.1
+    occ2 = np.where(np.arange(nao) < nocc, 2.0, 0.0)
+    occ1 = occ2 / 2
+    cfg = dict(cphf_tol=1e-11)
+    rr = hesship.RHessRIJK(T(cderi, dev), 1.0, 0.5)
+    rr.make_response_preparation(T(C, dev), occ2)
+    rd = hesship.RHessCPHF(T(C, dev), occ2, e, cfg)
+    rd.add_el(rr)
+    de_r = rd.make_cphf_hess(T(f1, dev), T(s1, dev)).cpu().numpy()
+    ud, _ = umake(dev, cderi, [C, C], [occ1, occ1], [e, e], cfg=cfg)
+    de_u = ud.make_cphf_hess([T(f1, dev)] * 2, [T(s1, dev)] * 2).cpu().numpy()
+    assert rel(de_u, de_r) < 1e-8
+
+
+def test_spin_mismatch_is_an_error(dev):
+    import hesship
+    from hesship._lib import HesshipError
+    cderi, C, occ, e, f1, s1, rb = usystem()
+    r_rijk = hesship.RHessRIJK(T(cderi, dev), 1.0, 0.5)
+    r_rijk.make_response_preparation(T(C[0], dev), np.where(occ[0] > 0, 2.0, 0.0))
+    ud = hesship.UHessCPHF([T(c, dev) for c in C], occ, e)
+    with pytest.raises(HesshipError):
+        ud.add_el(r_rijk)  # an RHF component on the UHF driver
