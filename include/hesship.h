@@ -136,12 +136,38 @@ hh_status hh_numint_rks_response_bra(hh_numint* h, int32_t den_type, const doubl
                                      int64_t nprop, const double* mocc, int64_t nocc, double* out,
                                      hh_precision precision);
 
+/* UKS: out[s] [nao, nocc[s], nprop] = uks_fxc_pot_with_eff_bra_trans(fxc_eff, rho1, bra)[s]
+ * (pure_xcpot.rs:921-1028): fxc_eff [ng, nvar, 2, nvar, 2], rho1 [ng, nvar, 2, nprop] (the
+ * two spins' densities stacked on axis 2), bra[s] [nao, nocc[s]]; two arrays per argument. */
+hh_status hh_numint_uks_fxc_pot_bra_trans(hh_numint* h, int32_t den_type, const double* fxc_eff, const double* rho1,
+                                          int64_t nprop, const double* const* bra, const int64_t* nocc,
+                                          double* const* out, hh_precision precision);
+/* out[s] [nao, nocc[s], nprop] = get_uks_response_bra (hess_uks.rs:390-441; rho1 of both spins
+ * from mocc[s] and mo1_bra[s] [nao, nocc[s], nprop], x2 factor), batched over the grid
+ * (get_uks_response_bra_batched, hess_uks.rs:589-650). */
+hh_status hh_numint_uks_response_bra(hh_numint* h, int32_t den_type, const double* fxc_eff,
+                                     const double* const* mo1_bra, int64_t nprop, const double* const* mocc,
+                                     const int64_t* nocc, double* const* out, hh_precision precision);
+
 /* XC effective kernels from LibXC-variable derivatives, unpolarised (transform_xc_inner,
  * xc_deriv.rs:305-428): rho [ng, >= nvar] (ld_rho), xc_val [ng, ncol] (ld_xc) with columns
  * [exc | 1st derivatives | unique 2nd derivatives]; order 1 -> out [ng, nvar] (vxc_eff),
  * order 2 -> out [ng, nvar, nvar] (fxc_eff).  RHO: out = the order's raw column. */
 hh_status hh_xceff_transform(int32_t den_type, int32_t order, const double* rho, int64_t ld_rho, const double* xc_val,
                              int64_t ld_xc, int64_t ng, double* out, void* stream);
+/* Spin-polarised transform (transform_xc_inner with LibXCSpin::Polarized: vxc_unfold_sigma_spin1,
+ * xc_deriv.rs:188-243, and the order-2 sigma pair term, xc_deriv.rs:379-424): rho [ng, nvar, 2]
+ * with element (g, x, s) at g + ld_rho*x + ld_spin*s; xc_val [ng, ncol] (ld_xc) in XCFun order
+ * (rho_a, rho_b, sigma_aa, sigma_ab, sigma_bb, tau_a, tau_b; second derivatives as combinations
+ * with replacement); order 1 -> out [ng, nvar, 2], order 2 -> out [ng, nvar, 2, nvar, 2]. */
+hh_status hh_xceff_transform_polarized(int32_t den_type, int32_t order, const double* rho, int64_t ld_rho,
+                                       int64_t ld_spin, const double* xc_val, int64_t ld_xc, int64_t ng, double* out,
+                                       void* stream);
+/* Columns of LibXC's polarised output [zk | 1st | 2nd derivatives] listed in XCFun order
+ * (libxc_to_xcfun_mapping, xc_deriv.rs:17-72), deriv 0..2.  Host.  *len = map length; map may be
+ * NULL to query it, else it needs capacity >= *len. */
+hh_status hh_xceff_libxc_to_xcfun_map(int32_t den_type, int32_t deriv, int64_t* map, int64_t capacity,
+                                      int64_t* len);
 
 /* ---------------------------------------------------------------------------------------- */
 /* Block Krylov solver for (I + A) x = b (src/util/krylov_block.rs)                         */
@@ -237,6 +263,10 @@ hh_status hh_ucphf_finalize(hh_cphf* h, const double* const* f1mo, const double*
 /* get_cphf_hess, uscf.rs:409-440: de [3, 3, natm, natm]. */
 hh_status hh_ucphf_hess(hh_cphf* h, const double* const* f1mo, const double* const* s1mo, const double* const* mo1,
                         const double* const* mo_e1, int64_t natm, double* de);
+/* Add the UKS grid response (UHessKSNIMatmul::get_response_bra, hess_uks.rs:740-770) with
+ * fxc_eff [ng, nvar, 2, nvar, 2] (device; must outlive the driver) to the unrestricted driver's
+ * response list; the occupied MOs are the driver's. */
+hh_status hh_ucphf_add_numint(hh_cphf* h, hh_numint* ni, int32_t den_type, const double* fxc_eff);
 
 /* ---------------------------------------------------------------------------------------- */
 /* Learned diagonal CPHF preconditioner (no reference counterpart: the reference preconditions */

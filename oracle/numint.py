@@ -110,3 +110,50 @@ def brute_fxc_full(den, fxc_eff, rho1, ao, weights):
                 for t in range(1, 4):
                     V[:, :, a] += 0.5 * wv[4] * np.outer(p[:, t], p[:, t])
     return V
+
+
+# ---------------------------------------------------------------------------------------------
+# UKS (pure_xcpot.rs:921-1028 uks_fxc_pot_with_eff_bra_trans_with_output; hess_uks.rs:390-441
+# get_uks_response_bra, factor 2).  fxc_eff [ng, nvar, 2, nvar, 2], rho1 [ng, nvar, 2, nset].
+
+def uks_fxc_pot_bra_trans(den, fxc_eff, rho1, ao, weights, bra):
+    """wv_s[g, x, a] = w_g sum_{y,t} fxc[g, y, t, x, s] rho1[g, y, t, a]; per spin bra contraction."""
+    ncomp = NCOMP[den]
+    nset = rho1.shape[3]
+    out = []
+    for s in range(2):
+        ao_bra = np.stack([ao[:, :, c] @ bra[s] for c in range(ncomp)], axis=2)
+        o = np.zeros((ao.shape[1], bra[s].shape[1], nset))
+        for a in range(nset):
+            wv = weights[:, None] * np.einsum("gytx,gyt->gx", fxc_eff[:, :, :, :, s], rho1[:, :, :, a])
+            o[:, :, a] = contract_ao_wv_bra(den, wv, ao, ao_bra)
+        out.append(o)
+    return out
+
+
+def uks_response_bra(den, fxc_eff, ao, weights, mo1_bra, mocc):
+    """hess_uks.rs:390-441: rho1 per spin, stacked, then 2 * uks_fxc_pot_bra_trans."""
+    kets = [m.reshape(m.shape[0], m.shape[1], -1) for m in mo1_bra]
+    rho1 = np.stack([rho1_one_bra_mult_ket(ao, mocc[s], kets[s], den) for s in range(2)], axis=2)
+    r = uks_fxc_pot_bra_trans(den, fxc_eff, rho1, ao, weights, mocc)
+    return [2.0 * r[s].reshape(mo1_bra[s].shape) for s in range(2)]
+
+
+def brute_uks_potential(den, fxc_eff, rho1, ao, weights):
+    """Per-grid-point AO-pair potentials V_s[mu, nu, a] with the spin-coupled kernel."""
+    ng, nvar, _, nset = rho1.shape
+    nao = ao.shape[1]
+    V = [np.zeros((nao, nao, nset)) for _ in range(2)]
+    for s in range(2):
+        for a in range(nset):
+            for g in range(ng):
+                wv = weights[g] * np.einsum("ytx,yt->x", fxc_eff[g, :, :, :, s], rho1[g, :, :, a])
+                p = ao[g]
+                V[s][:, :, a] += wv[0] * np.outer(p[:, 0], p[:, 0])
+                if den in ("sigma", "tau"):
+                    for t in range(1, 4):
+                        V[s][:, :, a] += wv[t] * (np.outer(p[:, 0], p[:, t]) + np.outer(p[:, t], p[:, 0]))
+                if den == "tau":
+                    for t in range(1, 4):
+                        V[s][:, :, a] += 0.5 * wv[4] * np.outer(p[:, t], p[:, t])
+    return V

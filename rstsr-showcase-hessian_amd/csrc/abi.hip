@@ -47,6 +47,23 @@ struct NumintResponse : hh::ElecResponse {
   }
 };
 
+// UKS grid response as a CPHF component: mocc[s] = the driver's first nocc[s] MO columns of spin s
+struct UksNumintResponse : hh::ElecResponse {
+  hh::Numint* ni;
+  int den;
+  const double* fxc;
+  const double* mocc[2];
+  int64_t nocc[2];
+  UksNumintResponse(hh::Numint* n, int d, const double* f, const double* ma, const double* mb, int64_t na,
+                    int64_t nb)
+      : ni(n), den(d), fxc(f), mocc{ma, mb}, nocc{na, nb} {}
+  int nspin() const override { return 2; }
+  void response_bra(const double* const* bra, int64_t nprop, double* const* out, bool accumulate,
+                    hh::Prec prec) override {
+    ni->uks_response_bra(den, fxc, bra, nprop, mocc, nocc, out, prec, accumulate);
+  }
+};
+
 hh::CphfConfig cphf_config(double level_shift, double tol, int32_t max_cycle, int32_t max_space, double lindep) {
   HH_REQUIRE(tol > 0.0 && max_cycle > 0 && max_space > 0 && lindep >= 0.0, "invalid CPHF configuration");
   hh::CphfConfig cfg;
@@ -298,6 +315,18 @@ hh_status hh_cphf_add_numint(hh_cphf* h, hh_numint* ni, int32_t den_type, const 
   HH_ABI_END
 }
 
+hh_status hh_ucphf_add_numint(hh_cphf* h, hh_numint* ni, int32_t den_type, const double* fxc_eff) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && ni && fxc_eff, "null argument");
+  HH_REQUIRE(h->impl->nspin() == 2, "this entry point is for the unrestricted driver (hh_ucphf_create)");
+  HH_REQUIRE(den_type >= HH_DEN_RHO && den_type <= HH_DEN_TAU, "den_type must be RHO, SIGMA or TAU");
+  HH_REQUIRE(ni->impl->nao() == h->impl->nao(), "numint nao differs from the driver's");
+  h->comps.push_back(std::make_unique<UksNumintResponse>(ni->impl.get(), den_type, fxc_eff, h->impl->mo_coeff(0),
+                                                         h->impl->mo_coeff(1), h->impl->nocc(0), h->impl->nocc(1)));
+  h->impl->add_component(h->comps.back().get());
+  HH_ABI_END
+}
+
 hh_status hh_numint_create(const double* ao, int64_t ng, int64_t nao, int64_t ncomp, const double* weights,
                            void* stream, hh_numint** out) {
   HH_ABI_BEGIN
@@ -351,6 +380,51 @@ hh_status hh_numint_rks_response_bra(hh_numint* h, int32_t den_type, const doubl
   HH_REQUIRE(h && fxc_eff && mo1_bra && mocc && out && nocc > 0 && nprop > 0, "null argument or empty size");
   HH_REQUIRE(precision >= HH_PREC_FP64 && precision <= HH_PREC_BF16X3, "unknown precision");
   h->impl->rks_response_bra(den_type, fxc_eff, mo1_bra, nprop, mocc, nocc, out, (Prec)precision);
+  HH_ABI_END
+}
+
+hh_status hh_numint_uks_fxc_pot_bra_trans(hh_numint* h, int32_t den_type, const double* fxc_eff, const double* rho1,
+                                          int64_t nprop, const double* const* bra, const int64_t* nocc,
+                                          double* const* out, hh_precision precision) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && fxc_eff && rho1 && bra && nocc && out && nprop > 0, "null argument or empty size");
+  for (int s = 0; s < 2; ++s) HH_REQUIRE(bra[s] && out[s] && nocc[s] > 0, "null spin array or empty occupation");
+  check_prec(precision);
+  h->impl->uks_fxc_pot_bra_trans(den_type, fxc_eff, rho1, nprop, bra, nocc, out, (Prec)precision);
+  HH_ABI_END
+}
+
+hh_status hh_numint_uks_response_bra(hh_numint* h, int32_t den_type, const double* fxc_eff,
+                                     const double* const* mo1_bra, int64_t nprop, const double* const* mocc,
+                                     const int64_t* nocc, double* const* out, hh_precision precision) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && fxc_eff && mo1_bra && mocc && nocc && out && nprop > 0, "null argument or empty size");
+  for (int s = 0; s < 2; ++s)
+    HH_REQUIRE(mo1_bra[s] && mocc[s] && out[s] && nocc[s] > 0, "null spin array or empty occupation");
+  check_prec(precision);
+  h->impl->uks_response_bra(den_type, fxc_eff, mo1_bra, nprop, mocc, nocc, out, (Prec)precision);
+  HH_ABI_END
+}
+
+hh_status hh_xceff_transform_polarized(int32_t den_type, int32_t order, const double* rho, int64_t ld_rho,
+                                       int64_t ld_spin, const double* xc_val, int64_t ld_xc, int64_t ng, double* out,
+                                       void* stream) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(ng >= 0 && xc_val && out && (den_type == HH_DEN_RHO || rho), "null argument");
+  xceff_transform_pol(den_type, order, rho, ld_rho, ld_spin, xc_val, ld_xc, ng, out, (hipStream_t)stream);
+  HH_ABI_END
+}
+
+hh_status hh_xceff_libxc_to_xcfun_map(int32_t den_type, int32_t deriv, int64_t* map, int64_t capacity,
+                                      int64_t* len) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(len != nullptr, "len is null");
+  const std::vector<int64_t> m = libxc_to_xcfun_map(den_type, deriv);
+  *len = (int64_t)m.size();
+  if (map) {
+    HH_REQUIRE(capacity >= (int64_t)m.size(), "map capacity too small");
+    std::copy(m.begin(), m.end(), map);
+  }
   HH_ABI_END
 }
 

@@ -119,6 +119,64 @@ __global__ void wv_s_kernel(const double* aob, const double* rho1, double* S, co
   }
 }
 
+// ---- UKS: the kernel couples the spins (fxc_eff [ng, NV, 2, NV, 2]) ----
+// w_s[x] = w sum_{y,t} fxc[g, y, t, x, s] r_t[y]
+template <int DEN>
+__device__ __forceinline__ void weighted_kernel_uks(const double* fxc, int64_t ld, int64_t g, double w,
+                                                    const double* ra, const double* rb, double* wa, double* wb) {
+  constexpr int NV = DEN == DEN_RHO ? 1 : (DEN == DEN_SIGMA ? 4 : 5);
+#pragma unroll
+  for (int sp = 0; sp < 2; ++sp)
+#pragma unroll
+    for (int x = 0; x < NV; ++x) {
+      double v = 0.0;
+#pragma unroll
+      for (int y = 0; y < NV; ++y) {
+        v += fxc[g + ld * (y + NV * (0 + 2 * (x + NV * sp)))] * ra[y];
+        v += fxc[g + ld * (y + NV * (1 + 2 * (x + NV * sp)))] * rb[y];
+      }
+      (sp == 0 ? wa : wb)[x] = w * v;
+    }
+}
+
+// fused: rho1 of both spins from (aob_s, akb_s), then S_s into akb_s (both rho1 first: S overwrites)
+template <int DEN>
+__global__ void uks_rho1_wv_s_kernel(const double* aob_a, double* akb_a, const double* aob_b, double* akb_b,
+                                     const double* fxc, const double* w, int64_t gb, int64_t nocc_a, int64_t nocc_b,
+                                     int64_t nprop, int64_t ng, int64_t g0) {
+  const int64_t total = gb * nprop, ncol_a = nocc_a * nprop, ncol_b = nocc_b * nprop;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t g = e % gb, a = e / gb;
+    double ra[5], rb[5], wa[5], wb[5];
+    rho1_point<DEN>(aob_a, akb_a, gb, g, nocc_a, ncol_a, a, ra);
+    rho1_point<DEN>(aob_b, akb_b, gb, g, nocc_b, ncol_b, a, rb);
+    weighted_kernel_uks<DEN>(fxc + g0, ng, g, w[g0 + g], ra, rb, wa, wb);
+    write_s<DEN>(aob_a, akb_a, gb, g, nocc_a, ncol_a, a, wa);
+    write_s<DEN>(aob_b, akb_b, gb, g, nocc_b, ncol_b, a, wb);
+  }
+}
+
+// S_s from a given rho1 [ng, NV, 2, nprop]
+template <int DEN>
+__global__ void uks_wv_s_kernel(const double* aob_a, const double* aob_b, const double* rho1, double* Sa, double* Sb,
+                                const double* fxc, const double* w, int64_t gb, int64_t nocc_a, int64_t nocc_b,
+                                int64_t nprop, int64_t ng, int64_t g0) {
+  constexpr int NV = DEN == DEN_RHO ? 1 : (DEN == DEN_SIGMA ? 4 : 5);
+  const int64_t total = gb * nprop;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t g = e % gb, a = e / gb;
+    double ra[5], rb[5], wa[5], wb[5];
+#pragma unroll
+    for (int y = 0; y < NV; ++y) {
+      ra[y] = rho1[(g0 + g) + ng * (y + NV * (0 + 2 * a))];
+      rb[y] = rho1[(g0 + g) + ng * (y + NV * (1 + 2 * a))];
+    }
+    weighted_kernel_uks<DEN>(fxc + g0, ng, g, w[g0 + g], ra, rb, wa, wb);
+    write_s<DEN>(aob_a, Sa, gb, g, nocc_a, nocc_a * nprop, a, wa);
+    write_s<DEN>(aob_b, Sb, gb, g, nocc_b, nocc_b * nprop, a, wb);
+  }
+}
+
 // ---- xceff transform (xc_deriv.rs:305-428, unpolarised) ----
 template <int DEN>
 __global__ void xceff_kernel(int order, const double* rho, int64_t ld_rho, const double* xc, int64_t ld_xc, int64_t ng,
@@ -161,6 +219,74 @@ __global__ void xceff_kernel(int order, const double* rho, int64_t ld_rho, const
 __global__ void xceff_lda_kernel(int order, const double* xc, int64_t ld_xc, int64_t ng, double* out) {
   for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < ng; g += (int64_t)gridDim.x * blockDim.x)
     out[g] = xc[g + ld_xc * order];
+}
+
+// ---- spin-polarised transform ----
+// Column (x, s) of the Jacobian d(XCFun variable)/d(rho_t variable) has at most two entries:
+// rho_s -> variable s; grad_t rho_a -> sigma_aa (2 grad_t rho_a), sigma_ab (grad_t rho_b);
+// grad_t rho_b -> sigma_ab (grad_t rho_a), sigma_bb (2 grad_t rho_b); tau_s -> 5 + s.
+template <int DEN>
+__global__ void xceff_pol_kernel(int order, const double* rho, int64_t ld_rho, int64_t ld_spin, const double* xc,
+                                 int64_t ld_xc, int64_t ng, double* out) {
+  constexpr int NV = DEN == DEN_SIGMA ? 4 : 5;
+  constexpr int XL = DEN == DEN_SIGMA ? 5 : 7;
+  constexpr int NC = 2 * NV;
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < ng; g += (int64_t)gridDim.x * blockDim.x) {
+    int k1[NC], k2[NC];
+    double c1[NC], c2[NC];
+#pragma unroll
+    for (int sp = 0; sp < 2; ++sp) {
+      k1[NV * sp] = sp; c1[NV * sp] = 1.0; k2[NV * sp] = -1; c2[NV * sp] = 0.0;
+#pragma unroll
+      for (int t = 1; t < 4; ++t) {
+        const double ga = rho[g + ld_rho * t], gbv = rho[g + ld_rho * t + ld_spin];
+        const int c = t + NV * sp;
+        if (sp == 0) { k1[c] = 2; c1[c] = 2.0 * ga; k2[c] = 3; c2[c] = gbv; }
+        else { k1[c] = 3; c1[c] = ga; k2[c] = 4; c2[c] = 2.0 * gbv; }
+      }
+      if (NV == 5) { k1[4 + NV * sp] = 5 + sp; c1[4 + NV * sp] = 1.0; k2[4 + NV * sp] = -1; c2[4 + NV * sp] = 0.0; }
+    }
+    if (order == 1) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        double v = c1[c] * xc[g + ld_xc * (1 + k1[c])];
+        if (k2[c] >= 0) v += c2[c] * xc[g + ld_xc * (1 + k2[c])];
+        out[g + ng * c] = v;
+      }
+    } else {
+      auto f2 = [&](int p, int q) {
+        if (p > q) { const int r = p; p = q; q = r; }
+        return xc[g + ld_xc * (1 + XL + p * XL - p * (p - 1) / 2 + (q - p))];
+      };
+      const double fsaa = xc[g + ld_xc * 3], fsab = xc[g + ld_xc * 4], fsbb = xc[g + ld_xc * 5];
+#pragma unroll
+      for (int cy = 0; cy < NC; ++cy)
+#pragma unroll
+        for (int cx = 0; cx < NC; ++cx) {
+          double v = c1[cx] * c1[cy] * f2(k1[cx], k1[cy]);
+          if (k2[cy] >= 0) v += c1[cx] * c2[cy] * f2(k1[cx], k2[cy]);
+          if (k2[cx] >= 0) v += c2[cx] * c1[cy] * f2(k2[cx], k1[cy]);
+          if (k2[cx] >= 0 && k2[cy] >= 0) v += c2[cx] * c2[cy] * f2(k2[cx], k2[cy]);
+          const int x = cx % NV, y = cy % NV, sx = cx / NV, sy = cy / NV;
+          if (x == y && x >= 1 && x <= 3)  // d2 sigma / d grad d grad: [[2 f_aa, f_ab], [f_ab, 2 f_bb]]
+            v += (sx != sy) ? fsab : (sx == 0 ? 2.0 * fsaa : 2.0 * fsbb);
+          out[g + ng * (cx + NC * cy)] = v;
+        }
+    }
+  }
+}
+
+// polarised LDA: the raw block; order 2 unfolds (aa, ab, bb) to [s, t]
+__global__ void xceff_lda_pol_kernel(int order, const double* xc, int64_t ld_xc, int64_t ng, double* out) {
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < ng; g += (int64_t)gridDim.x * blockDim.x) {
+    if (order == 1) {
+      out[g] = xc[g + ld_xc * 1];
+      out[g + ng] = xc[g + ld_xc * 2];
+    } else {
+      for (int t = 0; t < 2; ++t)
+        for (int sp = 0; sp < 2; ++sp) out[g + ng * (sp + 2 * t)] = xc[g + ld_xc * (3 + sp + t)];
+    }
+  }
 }
 
 }  // namespace
@@ -275,6 +401,108 @@ void Numint::rks_response_bra(int den, const double* fxc_eff, const double* mo1_
     back_contract(ao_.as<double>(), ng_, nao_, g0, gb, nc, akb, ncol, out, 4.0, accumulate || g0 > 0, prec, stream_,
                   ws_);
   }
+}
+
+void Numint::uks_fxc_pot_bra_trans(int den, const double* fxc_eff, const double* rho1, int64_t nprop,
+                                   const double* const* bra, const int64_t* nocc, double* const* out, Prec prec,
+                                   double alpha, bool accumulate) {
+  HH_REQUIRE(den >= DEN_RHO && den <= DEN_TAU, "numint: den_type must be RHO, SIGMA or TAU");
+  const int nc = den_ncomp(den);
+  HH_REQUIRE(ncomp_ >= nc, "numint: AO component dimension insufficient");
+  const int64_t ncol_a = nocc[0] * nprop, ncol_b = nocc[1] * nprop;
+  const int64_t gbmax = pick_batch(ncol_a + ncol_b);
+  double* aob_a = static_cast<double*>(aob_.reserve(sizeof(double) * gbmax * nocc[0] * nc));
+  double* aob_b = static_cast<double*>(aob2_.reserve(sizeof(double) * gbmax * nocc[1] * nc));
+  double* Sa = static_cast<double*>(akb_.reserve(sizeof(double) * gbmax * ncol_a * nc));
+  double* Sb = static_cast<double*>(akb2_.reserve(sizeof(double) * gbmax * ncol_b * nc));
+  for (int64_t g0 = 0; g0 < ng_; g0 += gbmax) {
+    const int64_t gb = std::min(gbmax, ng_ - g0);
+    half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, bra[0], nocc[0], aob_a, prec, stream_, ws_);
+    half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, bra[1], nocc[1], aob_b, prec, stream_, ws_);
+    HH_DEN_DISPATCH(den, uks_wv_s_kernel, dim3(nblk(gb * nprop)), dim3(256), 0, stream_, aob_a, aob_b, rho1, Sa, Sb,
+                    fxc_eff, w_.as<double>(), gb, nocc[0], nocc[1], nprop, ng_, g0);
+    back_contract(ao_.as<double>(), ng_, nao_, g0, gb, nc, Sa, ncol_a, out[0], alpha, accumulate || g0 > 0, prec,
+                  stream_, ws_);
+    back_contract(ao_.as<double>(), ng_, nao_, g0, gb, nc, Sb, ncol_b, out[1], alpha, accumulate || g0 > 0, prec,
+                  stream_, ws_);
+  }
+}
+
+void Numint::uks_response_bra(int den, const double* fxc_eff, const double* const* mo1_bra, int64_t nprop,
+                              const double* const* mocc, const int64_t* nocc, double* const* out, Prec prec,
+                              bool accumulate) {
+  HH_REQUIRE(den >= DEN_RHO && den <= DEN_TAU, "numint: den_type must be RHO, SIGMA or TAU");
+  const int nc = den_ncomp(den);
+  HH_REQUIRE(ncomp_ >= nc, "numint: AO component dimension insufficient");
+  const int64_t ncol_a = nocc[0] * nprop, ncol_b = nocc[1] * nprop;
+  const int64_t gbmax = pick_batch(ncol_a + ncol_b);
+  double* aob_a = static_cast<double*>(aob_.reserve(sizeof(double) * gbmax * nocc[0] * nc));
+  double* aob_b = static_cast<double*>(aob2_.reserve(sizeof(double) * gbmax * nocc[1] * nc));
+  double* akb_a = static_cast<double*>(akb_.reserve(sizeof(double) * gbmax * ncol_a * nc));
+  double* akb_b = static_cast<double*>(akb2_.reserve(sizeof(double) * gbmax * ncol_b * nc));
+  for (int64_t g0 = 0; g0 < ng_; g0 += gbmax) {
+    const int64_t gb = std::min(gbmax, ng_ - g0);
+    half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, mocc[0], nocc[0], aob_a, prec, stream_, ws_);
+    half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, mocc[1], nocc[1], aob_b, prec, stream_, ws_);
+    half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, mo1_bra[0], ncol_a, akb_a, prec, stream_, ws_);
+    half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, mo1_bra[1], ncol_b, akb_b, prec, stream_, ws_);
+    HH_DEN_DISPATCH(den, uks_rho1_wv_s_kernel, dim3(nblk(gb * nprop)), dim3(256), 0, stream_, aob_a, akb_a, aob_b,
+                    akb_b, fxc_eff, w_.as<double>(), gb, nocc[0], nocc[1], nprop, ng_, g0);
+    // the x2 factor of hess_uks.rs:436-439
+    back_contract(ao_.as<double>(), ng_, nao_, g0, gb, nc, akb_a, ncol_a, out[0], 2.0, accumulate || g0 > 0, prec,
+                  stream_, ws_);
+    back_contract(ao_.as<double>(), ng_, nao_, g0, gb, nc, akb_b, ncol_b, out[1], 2.0, accumulate || g0 > 0, prec,
+                  stream_, ws_);
+  }
+}
+
+void xceff_transform_pol(int den, int order, const double* rho, int64_t ld_rho, int64_t ld_spin, const double* xc_val,
+                         int64_t ld_xc, int64_t ng, double* out, hipStream_t s) {
+  HH_REQUIRE(order == 1 || order == 2, "xceff: order must be 1 (vxc) or 2 (fxc)");
+  HH_REQUIRE(den >= DEN_RHO && den <= DEN_TAU, "xceff: den_type must be RHO, SIGMA or TAU");
+  if (ng == 0) return;
+  if (den == DEN_RHO) {
+    hipLaunchKernelGGL(xceff_lda_pol_kernel, dim3(nblk(ng)), dim3(256), 0, s, order, xc_val, ld_xc, ng, out);
+  } else if (den == DEN_SIGMA) {
+    hipLaunchKernelGGL(xceff_pol_kernel<DEN_SIGMA>, dim3(nblk(ng)), dim3(256), 0, s, order, rho, ld_rho, ld_spin, xc_val,
+                       ld_xc, ng, out);
+  } else {
+    hipLaunchKernelGGL(xceff_pol_kernel<DEN_TAU>, dim3(nblk(ng)), dim3(256), 0, s, order, rho, ld_rho, ld_spin, xc_val,
+                       ld_xc, ng, out);
+  }
+  HH_HIP(hipGetLastError());
+}
+
+std::vector<int64_t> libxc_to_xcfun_map(int den, int deriv) {
+  HH_REQUIRE(den >= DEN_RHO && den <= DEN_TAU, "xceff: den_type must be RHO, SIGMA or TAU");
+  HH_REQUIRE(deriv >= 0 && deriv <= 2, "xceff: the map is provided for derivative orders 0-2");
+  std::vector<int> kinds = den == DEN_RHO ? std::vector<int>{2}
+                           : den == DEN_SIGMA ? std::vector<int>{2, 3} : std::vector<int>{2, 3, 2};
+  std::vector<std::pair<int, int>> flat;  // (kind, component)
+  for (int k = 0; k < (int)kinds.size(); ++k)
+    for (int c = 0; c < kinds[k]; ++c) flat.push_back({k, c});
+  const int nf = (int)flat.size();
+  std::vector<int64_t> out{0};
+  if (deriv >= 1)
+    for (int i = 0; i < nf; ++i) out.push_back(1 + i);
+  if (deriv >= 2) {
+    // LibXC position of each (kind1, comp1, kind2, comp2) second-derivative entry
+    std::vector<int64_t> pos(nf * nf, -1);
+    auto flat_index = [&](int k, int c) {
+      int i = 0;
+      for (int kk = 0; kk < k; ++kk) i += kinds[kk];
+      return i + c;
+    };
+    int64_t off = 1 + nf;
+    for (int k1 = 0; k1 < (int)kinds.size(); ++k1)
+      for (int k2 = k1; k2 < (int)kinds.size(); ++k2)
+        for (int c1 = 0; c1 < kinds[k1]; ++c1)
+          for (int c2 = (k1 == k2 ? c1 : 0); c2 < kinds[k2]; ++c2)
+            pos[flat_index(k1, c1) * nf + flat_index(k2, c2)] = off++;
+    for (int i = 0; i < nf; ++i)
+      for (int j = i; j < nf; ++j) out.push_back(pos[i * nf + j]);
+  }
+  return out;
 }
 
 void xceff_transform(int den, int order, const double* rho, int64_t ld_rho, const double* xc_val, int64_t ld_xc,

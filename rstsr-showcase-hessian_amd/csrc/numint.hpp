@@ -1,6 +1,8 @@
 // Numerical-integration grid contractions for the CPKS response (reference:
 // src/numint_matmul/{nimatmul.rs, pure_eval_rho.rs, pure_xcpot.rs, hess_rks.rs}).
 #pragma once
+#include <vector>
+
 #include "gemm.hpp"
 
 namespace hh {
@@ -27,6 +29,18 @@ class Numint {
   void rks_response_bra(int den, const double* fxc_eff, const double* mo1_bra, int64_t nprop, const double* mocc,
                         int64_t nocc, double* out, Prec prec, bool accumulate = false);
 
+  // UKS, one array per spin (pure_xcpot.rs:921-1028): out[s][nao, nocc_s, nprop] (+)= alpha *
+  // bra[s]-transformed potential of rho1 [ng, nvar, 2, nprop] under fxc_eff [ng, nvar, 2, nvar, 2],
+  // wv_s[g, x, a] = w_g sum_{y,t} fxc[g, y, t, x, s] rho1[g, y, t, a]
+  void uks_fxc_pot_bra_trans(int den, const double* fxc_eff, const double* rho1, int64_t nprop,
+                             const double* const* bra, const int64_t* nocc, double* const* out, Prec prec,
+                             double alpha = 1.0, bool accumulate = false);
+  // get_uks_response_bra (hess_uks.rs:390-441): rho1 per spin from (mocc_s, mo1_bra_s), then 2x the
+  // bra-transformed potential; fused per grid batch like the RKS response
+  void uks_response_bra(int den, const double* fxc_eff, const double* const* mo1_bra, int64_t nprop,
+                        const double* const* mocc, const int64_t* nocc, double* const* out, Prec prec,
+                        bool accumulate = false);
+
   int64_t ng() const { return ng_; }
   int64_t nao() const { return nao_; }
   hipStream_t stream() const { return stream_; }
@@ -36,7 +50,7 @@ class Numint {
   int64_t pick_batch(int64_t ncol) const;
   int64_t ng_, nao_, ncomp_;
   hipStream_t stream_;
-  DevBuf ao_, w_, aob_, akb_, rho_, ws_;
+  DevBuf ao_, w_, aob_, akb_, aob2_, akb2_, rho_, ws_;
 };
 
 // LibXC-variable derivatives -> rho_t-based effective kernels, unpolarised (reference:
@@ -45,5 +59,19 @@ class Numint {
 // order 1: out [ng, nvar];  order 2: out [ng, nvar, nvar].  RHO: out = the raw derivative block.
 void xceff_transform(int den, int order, const double* rho, int64_t ld_rho, const double* xc_val, int64_t ld_xc,
                      int64_t ng, double* out, hipStream_t s);
+
+// Spin-polarised (xc_deriv.rs:188-243 vxc_unfold_sigma_spin1, 379-424): rho element (g, x, s) at
+// g + ld_rho x + ld_spin s; xc_val columns [exc | 1st (xlen) | 2nd (xlen(xlen+1)/2)] in XCFun order
+// over (rho_a, rho_b, sigma_aa, sigma_ab, sigma_bb[, tau_a, tau_b]), xlen 2 / 5 / 7.
+// order 1: out [ng, nvar, 2];  order 2: out [ng, nvar, 2, nvar, 2].
+void xceff_transform_pol(int den, int order, const double* rho, int64_t ld_rho, int64_t ld_spin, const double* xc_val,
+                         int64_t ld_xc, int64_t ng, double* out, hipStream_t s);
+
+// Columns of LibXC's combined polarised output [zk | 1st | 2nd] in XCFun order (deriv <= 2;
+// xc_deriv.rs:17-88 libxc_to_xcfun_mapping): LibXC groups derivatives by variable kind (rho 2,
+// sigma 3, tau 2) and second-derivative blocks by kind pairs (rho,rho), (rho,sigma), (rho,tau),
+// (sigma,sigma), (sigma,tau), (tau,tau), row-major inside a block (upper triangle on the diagonal
+// blocks); XCFun lists all pairs of the flat variable list as combinations with replacement.
+std::vector<int64_t> libxc_to_xcfun_map(int den, int deriv);
 
 }  // namespace hh

@@ -10,5 +10,5 @@ from ._lib import PREC, HesshipError, KrylovNotConverged, lib  # noqa: F401
 from .tensors import fortran, empty_f  # noqa: F401
 from .krylov import krylov_block, solve, cholesky, trsm_lower  # noqa: F401
 from .cphf import RHessCPHF, UHessCPHF  # noqa: F401
-from .numint import NIMatmul, transform_xc  # noqa: F401
+from .numint import NIMatmul, libxc_to_xcfun_mapping, transform_xc, transform_xc_polarized  # noqa: F401
 from .rijk import RHessRIJK, UHessRIJK, get_rijk_response_bra_separated, pack_triu_tilde, unpack_triu  # noqa: F401

@@ -95,6 +95,7 @@ SIGNATURES.update({
     "hh_cphf_precond_target": (_i32, [_p, _p, _p, _i64, _p, _p]),
     "hh_cphf_precond_diag": (_i32, [_p, _p, _p]),
     "hh_cphf_add_numint": (_i32, [_p, _p, _i32, _p]),
+    "hh_ucphf_add_numint": (_i32, [_p, _p, _i32, _p]),
     "hh_cphf_set_verbose": (_i32, [_p, _i32]),
     "hh_numint_create": (_i32, [_p, _i64, _i64, _i64, _p, _p, C.POINTER(C.c_void_p)]),
     "hh_numint_destroy": (_i32, [_p]),
@@ -103,6 +104,10 @@ SIGNATURES.update({
     "hh_numint_fxc_pot_bra_trans": (_i32, [_p, _i32, _p, _p, _i64, _p, _i64, _p, _i32]),
     "hh_numint_rks_response_bra": (_i32, [_p, _i32, _p, _p, _i64, _p, _i64, _p, _i32]),
     "hh_xceff_transform": (_i32, [_i32, _i32, _p, _i64, _p, _i64, _i64, _p, _p]),
+    "hh_xceff_transform_polarized": (_i32, [_i32, _i32, _p, _i64, _i64, _p, _i64, _i64, _p, _p]),
+    "hh_xceff_libxc_to_xcfun_map": (_i32, [_i32, _i32, _pi64, _i64, _pi64]),
+    "hh_numint_uks_fxc_pot_bra_trans": (_i32, [_p, _i32, _p, _p, _i64, _pp, _pi64, _pp, _i32]),
+    "hh_numint_uks_response_bra": (_i32, [_p, _i32, _p, _pp, _i64, _pp, _pi64, _pp, _i32]),
 })
 
 

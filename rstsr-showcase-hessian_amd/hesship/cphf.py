@@ -165,6 +165,14 @@ class UHessCPHF:
         check(lib().hh_cphf_add_rijk(self._h, rijk._h))
         self._els.append(rijk)
 
+    def add_numint(self, ni, den_type, fxc_eff):
+        """Append the UKS grid response (UHessKSNIMatmul, hess_uks.rs:740-770) with fxc_eff
+        [ng, nvar, 2, nvar, 2]."""
+        from .numint import DEN
+        f = fortran(fxc_eff)
+        check(lib().hh_ucphf_add_numint(self._h, ni._h, DEN[den_type], f.data_ptr()))
+        self._els.append((ni, f))
+
     def _flat(self, pair):
         shapes = [tuple(t.shape) for t in pair]
         return [fortran(t.reshape(t.shape[0], t.shape[1], -1)) for t in pair], shapes

@@ -1,14 +1,15 @@
 """Grid contractions (reference: src/numint_matmul) and the XC kernel transform (src/xceff).
 
 Shapes (reference, column-major): ao [ng, nao, ncomp]; weights [ng]; rho1 [ng, nvar, nprop];
-fxc_eff [ng, nvar, nvar]; bra/mocc [nao, nocc]; kets / mo1_bra [nao, nocc, nprop].
+fxc_eff [ng, nvar, nvar]; bra/mocc [nao, nocc]; kets / mo1_bra [nao, nocc, nprop].  UKS: pairs
+(alpha, beta) of the per-spin arrays; rho1 [ng, nvar, 2, nprop]; fxc_eff [ng, nvar, 2, nvar, 2].
 den_type: "rho" | "sigma" | "tau" (XCDenType, src/xceff/flags.rs:19-24).
 """
 import ctypes as C
 
 import numpy as np
 
-from ._lib import PREC, check, lib
+from ._lib import PREC, check, lib, ptr_array
 from .tensors import empty_f, fortran, stream_of
 
 DEN = {"rho": 0, "sigma": 1, "tau": 2}
@@ -73,11 +74,49 @@ class NIMatmul:
                                                m.data_ptr(), nocc, out.data_ptr(), PREC[precision]))
         return out.reshape(shape)
 
+    def make_uks_fxc_pot_with_eff_bra_trans(self, fxc_eff, rho1, bra, den_type, precision="fp64"):
+        """pure_xcpot.rs:921-1028: [out_a, out_b], out_s [nao, nocc_s, nprop]."""
+        nprop = rho1.shape[3]
+        f, r = fortran(fxc_eff), fortran(rho1)
+        b = [fortran(x) for x in bra]
+        nocc = (C.c_int64 * 2)(*[int(x.shape[1]) for x in b])
+        out = [empty_f((self.nao, int(x.shape[1]), nprop), self.device) for x in b]
+        check(lib().hh_numint_uks_fxc_pot_bra_trans(self._h, DEN[den_type], f.data_ptr(), r.data_ptr(), nprop,
+                                                    ptr_array([x.data_ptr() for x in b]), nocc,
+                                                    ptr_array([o.data_ptr() for o in out]), PREC[precision]))
+        return out
+
+    def get_uks_response_bra(self, den_type, fxc_eff, mo1_bra, mocc, precision="fp64"):
+        """hess_uks.rs:390-441 (x2 factor), batched over the grid: [resp_a, resp_b]."""
+        m = [fortran(x) for x in mocc]
+        nocc_l = [int(x.shape[1]) for x in m]
+        shapes = [tuple(x.shape) for x in mo1_bra]
+        k = [fortran(x.reshape(self.nao, n, -1)) for x, n in zip(mo1_bra, nocc_l)]
+        nprop = k[0].shape[2]
+        if k[1].shape[2] != nprop:
+            raise ValueError("the two spins' mo1_bra differ in the number of perturbations")
+        f = fortran(fxc_eff)
+        out = [empty_f((self.nao, n, nprop), self.device) for n in nocc_l]
+        check(lib().hh_numint_uks_response_bra(self._h, DEN[den_type], f.data_ptr(),
+                                               ptr_array([x.data_ptr() for x in k]), nprop,
+                                               ptr_array([x.data_ptr() for x in m]), (C.c_int64 * 2)(*nocc_l),
+                                               ptr_array([o.data_ptr() for o in out]), PREC[precision]))
+        return [o.reshape(s) for o, s in zip(out, shapes)]
+
+
+def _check_xc_columns(xc_val, xlen, order):
+    if order not in (1, 2):
+        raise ValueError("order must be 1 (vxc) or 2 (fxc)")
+    need = 1 + xlen + (xlen * (xlen + 1) // 2 if order == 2 else 0)
+    if xc_val.dim() != 2 or xc_val.shape[1] < need:
+        raise ValueError(f"xc_val needs [ng, >= {need}] columns for order {order}")
+
 
 def transform_xc(rho, xc_val, den_type, order):
     """xc_deriv.rs:305-428 (unpolarised): vxc_eff [ng, nvar] (order 1) / fxc_eff [ng, nvar, nvar] (order 2)."""
     ng = xc_val.shape[0]
     nv = NVAR[den_type]
+    _check_xc_columns(xc_val, {"rho": 1, "sigma": 2, "tau": 3}[den_type], order)
     shape = (ng,) + (nv,) * order
     out = empty_f(shape, xc_val.device)
     r = fortran(rho) if rho is not None else None
@@ -86,3 +125,33 @@ def transform_xc(rho, xc_val, den_type, order):
                                    r.shape[0] if r is not None else 0, x.data_ptr(), x.shape[0], ng, out.data_ptr(),
                                    stream_of(xc_val.device)))
     return out
+
+
+def transform_xc_polarized(rho, xc_val, den_type, order):
+    """transform_xc_inner with LibXCSpin::Polarized (xc_deriv.rs:188-243, 305-428): rho
+    [ng, >= nvar, 2]; xc_val [ng, ncol] in XCFun order; vxc_eff [ng, nvar, 2] (order 1) /
+    fxc_eff [ng, nvar, 2, nvar, 2] (order 2)."""
+    ng = xc_val.shape[0]
+    nv = NVAR[den_type]
+    _check_xc_columns(xc_val, {"rho": 2, "sigma": 5, "tau": 7}[den_type], order)
+    if rho is not None and (rho.dim() != 3 or rho.shape[1] < nv or rho.shape[2] != 2 or rho.shape[0] != ng):
+        raise ValueError(f"rho must be [ng, >= {nv}, 2]")
+    shape = (ng,) + (nv, 2) * order
+    out = empty_f(shape, xc_val.device)
+    r = fortran(rho) if rho is not None else None
+    x = fortran(xc_val)
+    ld_rho = r.shape[0] if r is not None else 0
+    ld_spin = r.shape[0] * r.shape[1] if r is not None else 0
+    check(lib().hh_xceff_transform_polarized(DEN[den_type], int(order), r.data_ptr() if r is not None else None,
+                                             ld_rho, ld_spin, x.data_ptr(), x.shape[0], ng, out.data_ptr(),
+                                             stream_of(xc_val.device)))
+    return out
+
+
+def libxc_to_xcfun_mapping(den_type, deriv):
+    """xc_deriv.rs:65-72 (spin polarised): the LibXC output columns listed in XCFun order."""
+    n = C.c_int64()
+    check(lib().hh_xceff_libxc_to_xcfun_map(DEN[den_type], int(deriv), None, 0, C.byref(n)))
+    buf = (C.c_int64 * n.value)()
+    check(lib().hh_xceff_libxc_to_xcfun_map(DEN[den_type], int(deriv), buf, n.value, C.byref(n)))
+    return np.array(buf[:], dtype=np.int64)

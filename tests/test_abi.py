@@ -45,3 +45,19 @@ def test_version_and_invalid_argument_paths():
     assert rc == _lib.HH_ERR_INVALID and b"nspin" in L.hh_last_error()
     rc = L.hh_rijk_get_response_bra(None, None, 1, None, 0)
     assert rc == _lib.HH_ERR_INVALID
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libhesship.so not built")
+def test_libxc_to_xcfun_map_host_entry():
+    """The host entry point reproduces the reference's table (xc_deriv.rs:17-40) and rejects
+    derivative orders it does not cover."""
+    from hesship import _lib
+    from hesship.numint import libxc_to_xcfun_mapping
+    from tests.test_oracle_uks_xceff import REF_MAP
+    for den, ref in REF_MAP.items():
+        assert list(libxc_to_xcfun_mapping(den, 2)) == ref
+    assert list(libxc_to_xcfun_mapping("rho", 2)) == [0, 1, 2, 3, 4, 5]
+    n = ctypes.c_int64()
+    assert _lib.lib().hh_xceff_libxc_to_xcfun_map(1, 3, None, 0, ctypes.byref(n)) == _lib.HH_ERR_INVALID
+    buf = (ctypes.c_int64 * 2)()
+    assert _lib.lib().hh_xceff_libxc_to_xcfun_map(1, 2, buf, 2, ctypes.byref(n)) == _lib.HH_ERR_INVALID
