@@ -149,6 +149,31 @@ hh_status hh_numint_uks_response_bra(hh_numint* h, int32_t den_type, const doubl
                                      const double* const* mo1_bra, int64_t nprop, const double* const* mocc,
                                      const int64_t* nocc, double* const* out, hh_precision precision);
 
+/* DFT skeleton-Hessian setup over a fixed grid: nspin 1 = RKS (make_hessian_setup,
+ * hess_rks.rs:71-172, dm0 the closed-shell density), nspin 2 = UKS (make_hessian_setup_uks,
+ * hess_uks.rs:286-384, one dm0 [nao, nao] per spin), batched over the grid as
+ * make_hessian_setup_batched (hess_rks.rs:611-750).  The handle's AO tensor must carry
+ * derivatives to order 2 (RHO: ncomp >= 10) or 3 (SIGMA/TAU: ncomp >= 20), components ordered
+ * value, x, y, z, xx, xy, xz, yy, yz, zz, xxx, xxy, xxz, xyy, xyz, xzz, yyy, yyz, yzz, zzz.
+ * hh_numint_setup_rho: rho [ng, nvar, nspin] (the density of get_rho_vxc_fxc, hess_rks.rs:174-218 /
+ * get_rho_vxc_fxc_uks, hess_uks.rs:56-108) for the caller's functional evaluation (LibXC is
+ * outside this library). */
+hh_status hh_numint_setup_rho(hh_numint* h, int32_t den_type, int32_t nspin, const double* const* dm0,
+                              double* rho);
+/* The additive setup intermediates from the transformed kernels vxc_eff [ng, nvar, nspin] and
+ * fxc_eff [ng, nvar, nspin, nvar, nspin] (weights not applied; hh_xceff_transform /
+ * hh_xceff_transform_polarized): de_fxc [3, 3, natm, natm] and per spin de_vxc_diag,
+ * de_vxc_off [3, 3, natm, natm], vmat_ip [nao, nao, 3], vmat_deriv1 [nao, nao, 3, natm] (get_de_fxc,
+ * get_de_vxc_diag, get_de_vxc_off, get_vmat_ip, get_vmat_deriv1: hess_rks.rs:220-565; UKS
+ * hess_uks.rs:110-284).  aoslice: HOST [natm, 2] AO ranges (p0, p1) of the atoms in the list
+ * (aoslice_by_atom()[atm_list], columns 2-3).  The skeleton Hessian is
+ * de_fxc + sum_s (de_vxc_diag[s] + de_vxc_off[s]) (make_skeleton_hess, hess_rks.rs:866-872,
+ * hess_uks.rs:705-720); get_deriv1_ao is vmat_deriv1. */
+hh_status hh_numint_hessian_setup(hh_numint* h, int32_t den_type, int32_t nspin, const double* const* dm0,
+                                  int64_t natm, const int64_t* aoslice, const double* vxc_eff, const double* fxc_eff,
+                                  double* de_fxc, double* const* de_vxc_diag, double* const* de_vxc_off,
+                                  double* const* vmat_ip, double* const* vmat_deriv1);
+
 /* XC effective kernels from LibXC-variable derivatives, unpolarised (transform_xc_inner,
  * xc_deriv.rs:305-428): rho [ng, >= nvar] (ld_rho), xc_val [ng, ncol] (ld_xc) with columns
  * [exc | 1st derivatives | unique 2nd derivatives]; order 1 -> out [ng, nvar] (vxc_eff),

@@ -406,6 +406,35 @@ hh_status hh_numint_uks_response_bra(hh_numint* h, int32_t den_type, const doubl
   HH_ABI_END
 }
 
+hh_status hh_numint_setup_rho(hh_numint* h, int32_t den_type, int32_t nspin, const double* const* dm0,
+                              double* rho) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && dm0 && rho && (nspin == 1 || nspin == 2), "null argument or nspin not 1 / 2");
+  for (int s = 0; s < nspin; ++s) HH_REQUIRE(dm0[s], "null dm0");
+  h->impl->setup_rho(den_type, nspin, dm0, rho);
+  HH_ABI_END
+}
+
+hh_status hh_numint_hessian_setup(hh_numint* h, int32_t den_type, int32_t nspin, const double* const* dm0,
+                                  int64_t natm, const int64_t* aoslice, const double* vxc_eff, const double* fxc_eff,
+                                  double* de_fxc, double* const* de_vxc_diag, double* const* de_vxc_off,
+                                  double* const* vmat_ip, double* const* vmat_deriv1) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && dm0 && aoslice && vxc_eff && fxc_eff && de_fxc && de_vxc_diag && de_vxc_off && vmat_ip &&
+                 vmat_deriv1 && (nspin == 1 || nspin == 2) && natm > 0,
+             "null argument, nspin not 1 / 2 or no atoms");
+  Numint::SetupOut o;
+  o.de_fxc = de_fxc;
+  for (int s = 0; s < nspin; ++s) {
+    o.de_vxc_diag[s] = de_vxc_diag[s];
+    o.de_vxc_off[s] = de_vxc_off[s];
+    o.vmat_ip[s] = vmat_ip[s];
+    o.vmat_deriv1[s] = vmat_deriv1[s];
+  }
+  h->impl->hessian_setup(den_type, nspin, dm0, natm, aoslice, vxc_eff, fxc_eff, o);
+  HH_ABI_END
+}
+
 hh_status hh_xceff_transform_polarized(int32_t den_type, int32_t order, const double* rho, int64_t ld_rho,
                                        int64_t ld_spin, const double* xc_val, int64_t ld_xc, int64_t ng, double* out,
                                        void* stream) {

@@ -11,6 +11,8 @@ namespace hh {
 enum DenType : int { DEN_RHO = 0, DEN_SIGMA = 1, DEN_TAU = 2 };
 inline int den_nvar(int d) { return d == DEN_RHO ? 1 : (d == DEN_SIGMA ? 4 : 5); }
 inline int den_ncomp(int d) { return d == DEN_RHO ? 1 : 4; }
+// AO components the Hessian setup reads (derivative order 2 for RHO, 3 for SIGMA / TAU)
+int setup_ao_ncomp(int den);
 
 class Numint {
  public:
@@ -40,6 +42,23 @@ class Numint {
   void uks_response_bra(int den, const double* fxc_eff, const double* const* mo1_bra, int64_t nprop,
                         const double* const* mocc, const int64_t* nocc, double* const* out, Prec prec,
                         bool accumulate = false);
+
+  // DFT skeleton-Hessian setup over the grid (hess_rks.rs:71-565; hess_uks.rs:56-384), nspin 1
+  // (RKS, dm0 = the closed-shell density) or 2 (UKS, one density per spin).  The AO tensor needs
+  // derivatives to order 2 (RHO, 10 components) or 3 (SIGMA / TAU, 20 components).
+  // setup_rho: rho [ng, nvar, nspin] for the caller's functional evaluation (get_rho_vxc_fxc).
+  void setup_rho(int den, int nspin, const double* const* dm0, double* rho);
+  struct SetupOut {
+    double* de_fxc = nullptr;                           // [3, 3, natm, natm]
+    double* de_vxc_diag[2] = {nullptr, nullptr};        // [3, 3, natm, natm] per spin
+    double* de_vxc_off[2] = {nullptr, nullptr};         // [3, 3, natm, natm] per spin
+    double* vmat_ip[2] = {nullptr, nullptr};            // [nao, nao, 3] per spin
+    double* vmat_deriv1[2] = {nullptr, nullptr};        // [nao, nao, 3, natm] per spin
+  };
+  // aoslice: host [natm, 2] AO ranges (p0, p1) of the atoms in the list; vxc [ng, nvar, nspin],
+  // fxc [ng, nvar, nspin, nvar, nspin] (the transformed kernels, weights not applied).
+  void hessian_setup(int den, int nspin, const double* const* dm0, int64_t natm, const int64_t* aoslice,
+                     const double* vxc, const double* fxc, const SetupOut& out);
 
   int64_t ng() const { return ng_; }
   int64_t nao() const { return nao_; }

@@ -103,6 +103,51 @@ class NIMatmul:
                                                ptr_array([o.data_ptr() for o in out]), PREC[precision]))
         return [o.reshape(s) for o, s in zip(out, shapes)]
 
+    # ---- DFT skeleton-Hessian setup (hess_rks.rs:71-750, hess_uks.rs:56-384) ----
+    @staticmethod
+    def _spins(dm0):
+        return list(dm0) if isinstance(dm0, (list, tuple)) else [dm0]
+
+    def setup_rho(self, den_type, dm0):
+        """rho [ng, nvar] (RKS, dm0 a matrix) or [ng, nvar, 2] (UKS, dm0 a pair)."""
+        d = [fortran(x) for x in self._spins(dm0)]
+        ns = len(d)
+        out = empty_f((self.ng, NVAR[den_type], ns), self.device)
+        check(lib().hh_numint_setup_rho(self._h, DEN[den_type], ns, ptr_array([x.data_ptr() for x in d]),
+                                        out.data_ptr()))
+        return out if ns == 2 else out[:, :, 0]
+
+    def make_hessian_setup(self, den_type, dm0, aoslices, vxc_eff, fxc_eff):
+        """The additive intermediates of make_hessian_setup_batched with the reference's keys
+        (UKS: per-spin keys with suffix _a / _b).  aoslices: (p0, p1) per listed atom."""
+        d = [fortran(x) for x in self._spins(dm0)]
+        ns = len(d)
+        natm = len(aoslices)
+        sl = (C.c_int64 * (2 * natm))(*[int(v) for p in aoslices for v in p])
+        v, f = fortran(vxc_eff), fortran(fxc_eff)
+        nao, dev = self.nao, self.device
+        de_fxc = empty_f((3, 3, natm, natm), dev)
+        per = {k: [empty_f(shape, dev) for _ in range(ns)] for k, shape in
+               [("de_vxc_diag", (3, 3, natm, natm)), ("de_vxc_off", (3, 3, natm, natm)), ("vmat_ip", (nao, nao, 3)),
+                ("vmat_deriv1", (nao, nao, 3, natm))]}
+        pa = lambda k: ptr_array([t.data_ptr() for t in per[k]])
+        check(lib().hh_numint_hessian_setup(self._h, DEN[den_type], ns, ptr_array([x.data_ptr() for x in d]), natm, sl,
+                                            v.data_ptr(), f.data_ptr(), de_fxc.data_ptr(), pa("de_vxc_diag"),
+                                            pa("de_vxc_off"), pa("vmat_ip"), pa("vmat_deriv1")))
+        out = {"de_fxc": de_fxc}
+        for k, ts in per.items():
+            if ns == 1:
+                out[k] = ts[0]
+            else:
+                out[k + "_a"], out[k + "_b"] = ts
+        return out
+
+    @staticmethod
+    def skeleton_hess(setup):
+        """make_skeleton_hess (hess_rks.rs:866-872 / hess_uks.rs:705-720)."""
+        tags = [""] if "de_vxc_diag" in setup else ["_a", "_b"]
+        return setup["de_fxc"] + sum(setup["de_vxc_diag" + t] + setup["de_vxc_off" + t] for t in tags)
+
 
 def _check_xc_columns(xc_val, xlen, order):
     if order not in (1, 2):
