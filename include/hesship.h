@@ -103,6 +103,36 @@ hh_status hh_rijk_set_profiling(hh_rijk* h, int32_t on);
 hh_status hh_rijk_get_timing(const hh_rijk* h, double* out5);
 hh_status hh_rijk_reset_timing(hh_rijk* h);
 
+/* Skeleton (fixed-orbital) second derivative of the RI-J/K energy and the first-derivative Fock
+ * terms (get_rijk_skeleton_decomposed_separated, hess_r.rs:202-1718, as consumed by
+ * RHessRIJK::make_skeleton_hess / get_deriv1_bra, hess_r.rs:1858-1909, and UHessRIJK,
+ * hess_u.rs:150-261).  The 3-centre derivative integrals come from a generator called per
+ * auxiliary range (generator_hess_intor_j3c_by_aux, cint_handling.rs:80-92): it fills `out`
+ * (device, column-major) with int3c2e_<kind> over all AO pairs and the auxiliary functions
+ * [p0, p1): [nao, nao, p1-p0, 3] for IP1 / IP2 and [nao, nao, p1-p0, 3, 3] for the second
+ * derivatives (libcint's 9 components reshaped as hess_intor does, cint_handling.rs:20-44), and
+ * must enqueue its work on `stream` (or finish before returning).  Non-zero return aborts. */
+#define HH_INT3C_IP1 0
+#define HH_INT3C_IP2 1
+#define HH_INT3C_IPIP1 2
+#define HH_INT3C_IPVIP1 3
+#define HH_INT3C_IP1IP2 4
+#define HH_INT3C_IPIP2 5
+typedef int32_t (*hh_int3c_deriv_fn)(void* ctx, int32_t kind, int64_t p0, int64_t p1, double* out, void* stream);
+/* mo_coeff[s] device [nao, nmo[s]], mo_occ[s] HOST [nmo[s]] (one entry per handle spin);
+ * j2c device [naux, naux] (the metric the handle's cderi was decomposed with by Cholesky; NULL
+ * for a handle made by hh_rijk_create_from_j3c, which keeps its factor); j2c_ip1 [naux, naux, 3],
+ * j2c_ipip1 and j2c_ip1ip2 [naux, naux, 3, 3] (int2c2e_ip1 / ipip1 / ip1ip2, device);
+ * aux_ranges HOST [nrange, 2] covering [0, naux) in order (the reference's aux_ranges, shell
+ * aligned); aoslice / auxslice HOST [natm, 2] AO and auxiliary ranges of the atoms in the list.
+ * Outputs (device): de [3, 3, natm, natm] = make_skeleton_hess (scale_j de_J - 0.5 scale_k de_K
+ * RHF, scale_j de_J - scale_k sum_s de_K UHF); deriv1_bra[s] [nao, nocc_s, 3, natm] =
+ * get_deriv1_bra (NULL to skip).  Synchronous. */
+hh_status hh_rijk_skeleton(hh_rijk* h, const double* const* mo_coeff, const int64_t* nmo, const double* const* mo_occ,
+                           const double* j2c, const double* j2c_ip1, const double* j2c_ipip1, const double* j2c_ip1ip2,
+                           hh_int3c_deriv_fn int3c, void* ctx, int64_t nrange, const int64_t* aux_ranges, int64_t natm,
+                           const int64_t* aoslice, const int64_t* auxslice, double* de, double* const* deriv1_bra);
+
 /* ---------------------------------------------------------------------------------------- */
 /* Numerical-integration grid contractions (src/numint_matmul)                               */
 

@@ -5,6 +5,7 @@
 #include "linalg.hpp"
 #include "mlp.hpp"
 #include "numint.hpp"
+#include "rijk_skeleton.hpp"
 #include "rijk.hpp"
 #include "vib.hpp"
 
@@ -300,6 +301,31 @@ hh_status hh_cphf_add_rijk(hh_cphf* h, hh_rijk* rijk) {
     HH_REQUIRE(rijk->impl->nocc(t) == h->impl->nocc(t), "RI-JK component occupation differs from the driver's");
   h->comps.push_back(std::make_unique<RijkResponse>(rijk->impl.get()));
   h->impl->add_component(h->comps.back().get());
+  HH_ABI_END
+}
+
+hh_status hh_rijk_skeleton(hh_rijk* h, const double* const* mo_coeff, const int64_t* nmo, const double* const* mo_occ,
+                           const double* j2c, const double* j2c_ip1, const double* j2c_ipip1, const double* j2c_ip1ip2,
+                           hh_int3c_deriv_fn int3c, void* ctx, int64_t nrange, const int64_t* aux_ranges, int64_t natm,
+                           const int64_t* aoslice, const int64_t* auxslice, double* de, double* const* deriv1_bra) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && mo_coeff && nmo && mo_occ, "null argument");
+  hh::SkeletonInput in;
+  in.mo_coeff = mo_coeff;
+  in.nmo = nmo;
+  in.mo_occ = mo_occ;
+  in.j2c = j2c;
+  in.j2c_ip1 = j2c_ip1;
+  in.j2c_ipip1 = j2c_ipip1;
+  in.j2c_ip1ip2 = j2c_ip1ip2;
+  in.int3c = int3c;
+  in.ctx = ctx;
+  in.nrange = nrange;
+  in.aux_ranges = aux_ranges;
+  in.natm = natm;
+  in.aoslice = aoslice;
+  in.auxslice = auxslice;
+  hh::rijk_skeleton(*h->impl, in, de, deriv1_bra);
   HH_ABI_END
 }
 

@@ -21,7 +21,9 @@ class Rijk {
   // Cholesky factor has a diagonal element <= threshold.
   static std::unique_ptr<Rijk> from_j3c(const double* j3c, const double* j2c, int64_t nao, int64_t naux,
                                         double threshold, double scale_j, double scale_k, int nspin, hipStream_t s);
-  const double* j2c_l() const { return j2c_l_.as<double>(); }
+  const double* j2c_l() const { return j2c_l_.ptr ? j2c_l_.as<double>() : nullptr; }
+  // B [nao, nao, naux] (cderi unpacked, FP64)
+  const double* bfull() const { return bfull_.as<double>(); }
 
   void prepare_response(const double* const* mo_coeff, const double* const* mo_occ, const int64_t* nmo);
   // resp[s] = scale_j * f_s * J_ao mocc_s + scale_k * K_s  (f = 1 RHF, 0.5 UHF)

@@ -104,6 +104,8 @@ SIGNATURES.update({
     "hh_numint_fxc_pot_bra_trans": (_i32, [_p, _i32, _p, _p, _i64, _p, _i64, _p, _i32]),
     "hh_numint_rks_response_bra": (_i32, [_p, _i32, _p, _p, _i64, _p, _i64, _p, _i32]),
     "hh_xceff_transform": (_i32, [_i32, _i32, _p, _i64, _p, _i64, _i64, _p, _p]),
+    "hh_rijk_skeleton": (_i32, [_p, _pp, _pi64, _pp, _p, _p, _p, _p, _p, _p, _i64, _pi64, _i64, _pi64, _pi64, _p,
+                                 _pp]),
     "hh_numint_setup_rho": (_i32, [_p, _i32, _i32, _pp, _p]),
     "hh_numint_hessian_setup": (_i32, [_p, _i32, _i32, _pp, _i64, _pi64, _p, _p, _p, _pp, _pp, _pp, _pp]),
     "hh_xceff_transform_polarized": (_i32, [_i32, _i32, _p, _i64, _i64, _p, _i64, _i64, _p, _p]),

@@ -63,6 +63,11 @@ class _RIJKBase:
         self._h = h
         self.nocc = None
 
+    @classmethod
+    def from_j3c(cls, j3c_tp, j2c, scale_j, scale_k, threshold=1e-14):
+        """cderi generated on the device from j3c [nao_tp, naux] and j2c (hess_r.rs:1725-1735)."""
+        return cls(j3c_tp, scale_j, scale_k, _j2c=j2c, _threshold=threshold)
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
@@ -125,21 +130,73 @@ class _RIJKBase:
         return j_ao, [k.reshape((self.nao,) + sh[1:]) for k, sh in zip(ks, shapes)]
 
 
+INT3C_KINDS = {"ip1": 0, "ip2": 1, "ipip1": 2, "ipvip1": 3, "ip1ip2": 4, "ipip2": 5}
+_INT3C_FN = C.CFUNCTYPE(C.c_int32, C.c_void_p, C.c_int32, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p)
+
+
+def _skeleton(h, device, nao, mo_coeff, mo_occ, int3c, j2c_ip1, j2c_ipip1, j2c_ip1ip2, aux_ranges, aoslices,
+              auxslices, j2c=None, with_deriv1=True):
+    """hh_rijk_skeleton over device tensors.  int3c(kind, p0, p1) returns the derivative integrals
+    int3c2e_<kind> for auxiliary functions [p0, p1) as a device tensor [nao, nao, p1-p0, 3] or
+    [nao, nao, p1-p0, 3, 3] (kind in INT3C_KINDS)."""
+    names = {v: k for k, v in INT3C_KINDS.items()}
+    keep = []
+    err = []
+
+    def cb(_ctx, kind, p0, p1, out, stream):
+        try:
+            t = fortran(int3c(names[kind], int(p0), int(p1)))
+            ncomp = 3 if kind < 2 else 9
+            if t.numel() != nao * nao * (p1 - p0) * ncomp:
+                raise ValueError(f"int3c({names[kind]}) returned {tuple(t.shape)}")
+            keep.append(t)
+            check(lib().hh_util_copy(out, t.data_ptr(), t.numel() * 8, stream))
+            return 0
+        except Exception as e:  # reported after the call returns
+            err.append(e)
+            return 1
+
+    fn = _INT3C_FN(cb)
+    cs = [fortran(c) for c in mo_coeff]
+    occ = [_host_occ(o) for o in mo_occ]
+    nmo = (C.c_int64 * len(cs))(*[c.shape[1] for c in cs])
+    nocc = [int((o > 0).sum()) for o in occ]
+    natm = len(aoslices)
+    rng = (C.c_int64 * (2 * len(aux_ranges)))(*[int(v) for p in aux_ranges for v in p])
+    asl = (C.c_int64 * (2 * natm))(*[int(v) for p in aoslices for v in p])
+    xsl = (C.c_int64 * (2 * natm))(*[int(v) for p in auxslices for v in p])
+    v1, v2, v3 = fortran(j2c_ip1), fortran(j2c_ipip1), fortran(j2c_ip1ip2)
+    j2 = fortran(j2c) if j2c is not None else None
+    de = empty_f((3, 3, natm, natm), device)
+    d1 = [empty_f((nao, no, 3, natm), device) for no in nocc] if with_deriv1 else None
+    rc = lib().hh_rijk_skeleton(h, ptr_array([c.data_ptr() for c in cs]), nmo, ptr_array([o.ctypes.data for o in occ]),
+                                j2.data_ptr() if j2 is not None else None, v1.data_ptr(), v2.data_ptr(), v3.data_ptr(),
+                                C.cast(fn, C.c_void_p), None, len(aux_ranges), rng, natm, asl, xsl, de.data_ptr(),
+                                ptr_array([t.data_ptr() for t in d1]) if d1 is not None else None)
+    if err:
+        raise err[0]
+    check(rc)
+    return de, d1
+
+
 class RHessRIJK(_RIJKBase):
     """RHessRIJK (hess_r.rs:1737-1954), response side, built from an existing cderi
     (new_with_cderi, hess_r.rs:1775-1794)."""
     nspin = 1
-
-    @classmethod
-    def from_j3c(cls, j3c_tp, j2c, scale_j, scale_k, threshold=1e-14):
-        """cderi generated on the device from j3c [nao_tp, naux] and j2c (hess_r.rs:1725-1735)."""
-        return cls(j3c_tp, scale_j, scale_k, _j2c=j2c, _threshold=threshold)
 
     def make_response_preparation(self, mo_coeff, mo_occ):
         self._prepare([mo_coeff], [mo_occ])
 
     def get_response_bra(self, bra, precision="fp64"):
         return self._response([bra], precision)[0]
+
+    def make_skeleton(self, mo_coeff, mo_occ, int3c, j2c_ip1, j2c_ipip1, j2c_ip1ip2, aux_ranges, aoslices,
+                      auxslices, j2c=None, with_deriv1=True):
+        """(make_skeleton_hess, get_deriv1_bra) of hess_r.rs:1858-1909: de [3, 3, natm, natm] and
+        deriv1_bra [nao, nocc, 3, natm] (None if with_deriv1 is False)."""
+        de, d1 = _skeleton(self._h, self.device, self.nao, [mo_coeff], [mo_occ], int3c, j2c_ip1, j2c_ipip1,
+                           j2c_ip1ip2, aux_ranges, aoslices, auxslices, j2c, with_deriv1)
+        return de, (d1[0] if d1 is not None else None)
 
 
 class UHessRIJK(_RIJKBase):
@@ -151,6 +208,12 @@ class UHessRIJK(_RIJKBase):
 
     def get_response_bra(self, bra, precision="fp64"):
         return self._response(list(bra), precision)
+
+    def make_skeleton(self, mo_coeff, mo_occ, int3c, j2c_ip1, j2c_ipip1, j2c_ip1ip2, aux_ranges, aoslices,
+                      auxslices, j2c=None, with_deriv1=True):
+        """(make_skeleton_hess, get_deriv1_bra) of hess_u.rs:150-261: de and [deriv1_bra_a, _b]."""
+        return _skeleton(self._h, self.device, self.nao, list(mo_coeff), list(mo_occ), int3c, j2c_ip1, j2c_ipip1,
+                         j2c_ip1ip2, aux_ranges, aoslices, auxslices, j2c, with_deriv1)
 
 
 def get_rijk_response_bra_separated(cderi, mo_coeff, mo_occ, bra, do_j, do_k, nbatch_aux=72, precision="fp64"):
