@@ -10,6 +10,11 @@ algorithmic count F_K = 2 nao^2 naux nocc + 2 nao naux nocc^2 + nprop (2 nao^2 n
 against the FP64 result is reported.
 
   python bench.py [--steps K] [--warmup W] [--precision bf16x3|fp32|fp64] [--gpus 1]
+
+`--workload` selects a measurement of one of the other hot-path pieces instead (one JSON line
+each, same fields): `numint` (RKS grid response at the (H2O)_8 B3LYP sizes on synthetic AO
+values), `dft_setup` (DFT skeleton setup passes at the same sizes, FP64), `skeleton` (RI-JK
+skeleton Hessian and first-derivative terms on synthetic integrals, FP64).
 """
 import argparse
 import json
@@ -114,6 +119,204 @@ def run_cphf(drv, f1, s1, prec):
     return mo1f, moe1, time.perf_counter() - t0, st
 
 
+# ---- other hot-path pieces (--workload) ----
+C3 = dict(ng=24 * 75 * 302, nao=192, natm=24, nocc=40, nprop=72)   # (H2O)_8 def2-SVP, 75 x 302 grid
+
+
+def _timed(fn, steps, warmup):
+    import torch
+    out = None
+    for _ in range(warmup):
+        out = fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        out = fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return out, e0.elapsed_time(e1) / steps
+
+
+def _line(metric, value, unit, ms, steps, warmup, dtype, config, roofline, higher=True):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": round(ms, 3), "higher_is_better": higher, "scaling": "weak", "vs_baseline": None,
+            "dtype": dtype, "data": "synthetic (seeded)", "config": config, "roofline": roofline}
+
+
+def _blas_threads():
+    try:
+        from threadpoolctl import threadpool_info
+        return max([i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"] or [1])
+    except Exception:
+        return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def numint_flops(ng, nao, nocc, nprop):
+    """Reference-form count of the GGA response: 28 ng nao nocc per perturbation + the shared
+    AO-times-occupied transforms (8 ng nao nocc)."""
+    return nprop * 28.0 * ng * nao * nocc + 8.0 * ng * nao * nocc
+
+
+def workload_numint(args):
+    """RKS CPKS grid response (get_rks_response_bra, hess_rks.rs:570-605) at (H2O)_8 sizes: one
+    step = one response for all 72 perturbations over the 543 600-point grid (GGA)."""
+    import numpy as np
+    import torch
+    import hesship
+    from hesship.tensors import empty_f
+    from oracle import numint as oni
+    c = C3
+    ng, nao, nocc, nprop = c["ng"], c["nao"], c["nocc"], c["nprop"]
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(0x5EED0005)
+    ao = empty_f((ng, nao, 4), dev)
+    ao.normal_(generator=g)
+    ao *= 0.3
+    w = torch.rand(ng, generator=g, device=dev, dtype=torch.float64) * 0.1
+    f = torch.randn((ng, 4, 4), generator=g, device=dev, dtype=torch.float64) * 0.1
+    f = f + f.transpose(1, 2)
+    mocc = torch.randn((nao, nocc), generator=g, device=dev, dtype=torch.float64) * 0.1
+    mo1 = torch.randn((nao, nocc, nprop), generator=g, device=dev, dtype=torch.float64) * 0.1
+    ni = hesship.NIMatmul(ao, w)
+    del ao
+    prec = args.precision
+    log(f"[bench] numint response, {prec}")
+    out, ms = _timed(lambda: ni.get_rks_response_bra("sigma", f, mo1, mocc, precision=prec), args.steps, args.warmup)
+    ref, ms64 = _timed(lambda: ni.get_rks_response_bra("sigma", f, mo1, mocc), 1, 0)
+    value = numint_flops(ng, nao, nocc, nprop) / (ms * 1e-3) / 1e12
+    peak = PEAK_TFLOPS[prec]
+    line = _line("numint RKS response TFLOP/s (GGA, all perturbations)", round(value, 3), "TFLOP/s", ms, args.steps,
+                 args.warmup, prec, {"workload": "RKS grid response on synthetic AO values", "ngrid": ng, "nao": nao,
+                                     "nocc": nocc, "nprop": nprop, "den_type": "sigma"},
+                 {"bound": "mfma", "achieved": round(value, 3), "peak": peak, "unit": "TFLOP/s",
+                  "frac": round(value / peak, 4), "traffic": None,
+                  "kernel": "whole call (half-transform and back-contraction GEMMs + fused pointwise pass)"})
+    line["fp64_ms"] = round(ms64, 3)
+    line["rel_err_vs_fp64"] = ((out - ref).abs().max() / ref.abs().max()).item()
+    if not args.no_cpu_baseline:
+        # the oracle's reference-form response on a bounded slice of the grid, scaled per point
+        nsamp = 3000
+        rng = np.random.default_rng(1)
+        aos = rng.standard_normal((nsamp, nao, 4)) * 0.3
+        fs = rng.standard_normal((nsamp, 4, 4)) * 0.1
+        fs = fs + fs.transpose(0, 2, 1)
+        mo, m1 = mocc.cpu().numpy(), mo1.cpu().numpy()
+        t0 = time.perf_counter()
+        oni.rks_response_bra("sigma", fs, aos, rng.uniform(0, 0.1, nsamp), m1, mo)
+        dt = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": round(numint_flops(nsamp, nao, nocc, nprop) / dt / 1e12, 5),
+                                "unit": "TFLOP/s", "cores": _blas_threads(), "kind": "port",
+                                "sample": f"oracle rks_response_bra (NumPy FP64) on {nsamp} of {ng} grid points, "
+                                          f"all {nprop} perturbations, {dt:.1f} s"}
+    return line
+
+
+def setup_flops(ng, nao, natm, nvar=4):
+    """Algorithmic FP64 work of the GGA setup passes: ao_dm0 (4 components), de_fxc, the
+    de_vxc_off AO-pair tensor, vmat_ip, and the dominant vmat_fxc product 2 ng nao^2 3 natm."""
+    n3 = 3 * natm
+    return (8.0 * ng * nao * nao + 2.0 * ng * nvar * n3 * n3 + 2.0 * ng * (3 * nao) ** 2 + 4.0 * ng * 3 * nao * nao
+            + 2.0 * ng * nao * nao * n3)
+
+
+def workload_dft_setup(args):
+    """DFT skeleton setup (make_hessian_setup_batched, hess_rks.rs:611-750) at (H2O)_8 sizes, GGA,
+    FP64, AO derivatives to third order (20 components)."""
+    import numpy as np
+    import torch
+    import hesship
+    from hesship.tensors import empty_f
+    from oracle import dft_setup as ods
+    c = C3
+    ng, nao, natm = c["ng"], c["nao"], c["natm"]
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(0x5EED0006)
+    ao = empty_f((ng, nao, 20), dev)
+    ao.normal_(generator=g)
+    ao *= 0.3
+    w = torch.rand(ng, generator=g, device=dev, dtype=torch.float64) * 0.1
+    vxc = torch.randn((ng, 4), generator=g, device=dev, dtype=torch.float64) * 0.1
+    f = torch.randn((ng, 4, 4), generator=g, device=dev, dtype=torch.float64) * 0.1
+    f = f + f.transpose(1, 2)
+    Cm = torch.randn((nao, c["nocc"]), generator=g, device=dev, dtype=torch.float64) * 0.1
+    dm0 = Cm @ Cm.T
+    per = nao // natm
+    slices = [(A * per, (A + 1) * per) for A in range(natm)]
+    ni = hesship.NIMatmul(ao, w)
+    del ao
+    log("[bench] dft setup")
+    _, ms = _timed(lambda: ni.make_hessian_setup("sigma", dm0, slices, vxc, f), args.steps, args.warmup)
+    value = setup_flops(ng, nao, natm) / (ms * 1e-3) / 1e12
+    line = _line("DFT skeleton setup TFLOP/s (GGA, FP64)", round(value, 3), "TFLOP/s", ms, args.steps, args.warmup,
+                 "f64", {"workload": "DFT Hessian setup passes on synthetic AO values (derivative order 3)",
+                         "ngrid": ng, "nao": nao, "natm": natm, "den_type": "sigma"},
+                 {"bound": "mfma", "achieved": round(value, 3), "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
+                  "frac": round(value / PEAK_TFLOPS["fp64"], 4), "traffic": None,
+                  "kernel": "whole call (GEMMs dominated by vmat_fxc, pointwise drho and AO-diagonal passes)"})
+    if not args.no_cpu_baseline:
+        nsamp = 2000
+        rng = np.random.default_rng(2)
+        aos = rng.standard_normal((nsamp, nao, 20)) * 0.3
+        t0 = time.perf_counter()
+        ods.make_hessian_setup("sigma", aos, rng.uniform(0, 0.1, nsamp), dm0.cpu().numpy(), slices,
+                               rng.standard_normal((nsamp, 4)) * 0.1, rng.standard_normal((nsamp, 4, 4)) * 0.1)
+        dt = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": round(setup_flops(nsamp, nao, natm) / dt / 1e12, 5), "unit": "TFLOP/s",
+                                "cores": _blas_threads(), "kind": "port",
+                                "sample": f"oracle make_hessian_setup (NumPy FP64) on {nsamp} of {ng} grid points, "
+                                          f"{dt:.1f} s"}
+    return line
+
+
+def workload_skeleton(args):
+    """RI-JK skeleton (make_skeleton_hess + get_deriv1_bra, hess_r.rs:1858-1909) on synthetic
+    integrals at (H2O)_8 def2-SVP / jkfit-like sizes, FP64; the derivative integrals are served
+    from device-resident tensors through the generator interface in aux ranges of 72."""
+    import numpy as np
+    import torch
+    import hesship
+    from hesship.tensors import empty_f
+    nao, natm, nocc, naux = 192, 24, 40, 8 * 112
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(0x5EED0007)
+    rn = lambda *shape: torch.randn(shape, generator=g, device=dev, dtype=torch.float64) * 0.05
+    a = rn(naux, naux)
+    j2c = a @ a.T + naux * 0.05 * torch.eye(naux, device=dev, dtype=torch.float64)
+    j3c = rn(nao, nao, naux)
+    j3c = j3c + j3c.transpose(0, 1)
+    il = torch.tril_indices(nao, nao, device=dev)   # (nu, mu), mu <= nu, nu-major: the packed-upper order
+    j3c_tp = j3c[il[1], il[0]].contiguous()
+    del j3c
+    ints = {}
+    for k, nc in (("ip1", 3), ("ip2", 3), ("ipip1", 9), ("ipvip1", 9), ("ip1ip2", 9), ("ipip2", 9)):
+        t = empty_f((nao, nao, naux) + ((3,) if nc == 3 else (3, 3)), dev)
+        t.normal_(generator=g)
+        ints[k] = t
+    v2 = [rn(naux, naux, 3), rn(naux, naux, 3, 3), rn(naux, naux, 3, 3)]
+    C, _ = torch.linalg.qr(rn(nao, nao))
+    occ = np.where(np.arange(nao) < nocc, 2.0, 0.0)
+    per, pera = nao // natm, naux // natm
+    aos = [(A * per, (A + 1) * per) for A in range(natm)]
+    auxs = [(A * pera, (A + 1) * pera) for A in range(natm)]
+    ranges = [(p, min(naux, p + 72)) for p in range(0, naux, 72)]
+    h = hesship.RHessRIJK.from_j3c(j3c_tp, j2c, 1.0, 0.2)
+    fn = lambda kind, p0, p1: ints[kind][:, :, p0:p1]
+    log("[bench] rijk skeleton")
+    _, ms = _timed(lambda: h.make_skeleton(C, occ, fn, *v2, ranges, aos, auxs), args.steps, args.warmup)
+    nbytes = 8.0 * nao * nao * naux * (2 * 3 + 4 * 9)  # every derivative integral element read once
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    return _line("RI-JK skeleton wall ms (make_skeleton_hess + get_deriv1_bra, FP64)", round(ms, 3), "ms", ms,
+                 args.steps, args.warmup, "f64",
+                 {"workload": "RI-JK skeleton on synthetic derivative integrals", "nao": nao, "naux": naux,
+                  "natm": natm, "nocc": nocc, "aux_range": 72},
+                 {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
+                  "frac": round(gbs / 8000.0, 4), "traffic": None,
+                  "kernel": "whole call: derivative-integral bytes (each element read once, generator copies "
+                            "included) / time"},
+                 higher=False)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,6 +328,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cphf", action="store_true")
+    ap.add_argument("--workload", default="ri_jk", choices=["ri_jk", "numint", "dft_setup", "skeleton"])
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run the library's 16-byte copy kernel over GIB GiB three times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes) and exit")
@@ -139,6 +343,11 @@ def main():
 
     import hesship
     from hesship import synth
+
+    if args.workload != "ri_jk":
+        fn = {"numint": workload_numint, "dft_setup": workload_dft_setup, "skeleton": workload_skeleton}[args.workload]
+        print(json.dumps(fn(args)), flush=True)
+        return
 
     if args.copy_calibration > 0:
         nbytes = int(args.copy_calibration * 2**30) // 16 * 16
