@@ -29,6 +29,7 @@ struct NtParams {
   const void* Blo;
   int64_t ldb;
   int64_t tiles_m, tiles_n;
+  int64_t arows, brows;  // tiled: rows per K block of the A / B arrays
   int64_t kchunk;
   int ksplit;
   int64_t nwg;
@@ -100,7 +101,7 @@ __global__ __launch_bounds__(512) void nt_bf16x3_kernel(NtParams p) {
         const int r = c / (BK / 8), kk = (c % (BK / 8)) * 8;
         const int64_t gm = m0 + r;
         if (gm < p.M) {
-          const int64_t off = TILED ? ((k0 >> 5) * p.M + gm) * 32 + kk : gm * p.lda + k0 + kk;
+          const int64_t off = TILED ? ((k0 >> 5) * p.arows + gm) * 32 + kk : gm * p.lda + k0 + kk;
           vh = *reinterpret_cast<const uint4*>(Ah + off);
           vl = *reinterpret_cast<const uint4*>(Al + off);
         }
@@ -116,7 +117,7 @@ __global__ __launch_bounds__(512) void nt_bf16x3_kernel(NtParams p) {
         const int r = c / (BK / 8), kk = (c % (BK / 8)) * 8;
         const int64_t gn = n0 + r;
         if (gn < p.N) {
-          const int64_t off = TILED ? ((k0 >> 5) * p.N + gn) * 32 + kk : gn * p.ldb + k0 + kk;
+          const int64_t off = TILED ? ((k0 >> 5) * p.brows + gn) * 32 + kk : gn * p.ldb + k0 + kk;
           vh = *reinterpret_cast<const uint4*>(Bh + off);
           vl = *reinterpret_cast<const uint4*>(Bl + off);
         }
@@ -450,6 +451,8 @@ void launch_nt(const NtArgs& a, hipStream_t s, DevBuf& ws) {
   p.C = a.C; p.Cf = a.C_f32; p.ldc = a.ldc; p.alpha = a.alpha; p.beta = a.beta;
   p.tiles_m = ceil_div(a.M, BM);
   p.tiles_n = ceil_div(a.N, BN);
+  p.arows = a.a_rows > 0 ? a.a_rows : a.M;
+  p.brows = a.b_rows > 0 ? a.b_rows : a.N;
   const int64_t tiles = p.tiles_m * p.tiles_n;
   int ks = a.ksplit;
   if (ks <= 0) {
@@ -492,6 +495,8 @@ void gemm_nt(const NtArgs& a, NtFormat fmt, hipStream_t s, DevBuf& ws) {
                                                                   "multiples of 8");
   if (fmt == NT_BF16X3) {
     HH_REQUIRE(a.K == 0 || (a.A_lo && a.B_lo), "gemm_nt: bf16x3 needs lo arrays");
+    HH_REQUIRE((a.a_rows == 0 || a.a_rows >= a.M) && (a.b_rows == 0 || a.b_rows >= a.N),
+               "gemm_nt: tiled row stride smaller than the operand");
     // 208 x 256 tiles: M = 400 is two tiles with 4 % padding
     if (a.tiled) launch_nt<1, 8, 13, 2, true, true>(a, s, ws);
     else launch_nt<1, 8, 13, 2, true, false>(a, s, ws);

@@ -6,6 +6,7 @@
 // by their producers.  With `tiled` set (BF16X3 only) both operands are stored in K blocks of 32:
 // A[((k / 32) * M + m) * 32 + k % 32], B[((k / 32) * N + n) * 32 + k % 32] (lda, ldb unused), so
 // each K step of a workgroup reads one contiguous run of 64 bytes per row for all its rows.
+// (a_rows / b_rows replace M / N as the K-block stride for row ranges of larger tiled arrays.)
 // Element formats:
 //   NT_F64    : A, B double; v_mfma_f64_16x16x4f64, FP64 accumulate.
 //   NT_BF16X3 : A, B given as two bf16 arrays each (hi and lo = x - hi); hi*hi + hi*lo + lo*hi
@@ -30,6 +31,9 @@ struct NtArgs {
   int64_t ldc = 0;
   double alpha = 1.0, beta = 0.0;
   bool tiled = false;  // K-block tiled operands (BF16X3)
+  // Tiled operands that are a row range of a larger tiled array: rows per K block of the array
+  // (0 = M for A, N for B); the operand pointer then addresses its first row in K block 0.
+  int64_t a_rows = 0, b_rows = 0;
   int ksplit = 0;  // 0: automatic
   hipEvent_t ev_begin = nullptr, ev_main_end = nullptr;
 };

@@ -12,6 +12,8 @@
 #include "numint.hpp"
 
 #include "elementwise.hpp"
+#include "gemm_nt.hpp"
+#include <hip/hip_bf16.h>
 
 namespace hh {
 
@@ -19,20 +21,20 @@ namespace {
 unsigned nblk(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 65535)); }
 
 // rho1 of one (g, a) pair from the half-transformed AOs.
-template <int DEN>
-__device__ __forceinline__ void rho1_point(const double* aob, const double* akb, int64_t gb, int64_t g, int64_t nocc,
+template <int DEN, typename T>
+__device__ __forceinline__ void rho1_point(const double* aob, const T* akb, int64_t gb, int64_t g, int64_t nocc,
                                            int64_t ncol, int64_t a, double* r) {
   const int64_t sc_b = gb * nocc, sc_k = gb * ncol;
   for (int x = 0; x < 5; ++x) r[x] = 0.0;
   for (int64_t i = 0; i < nocc; ++i) {
     const double b0 = aob[g + gb * i];
-    const double k0 = akb[g + gb * (i + nocc * a)];
+    const double k0 = (double)akb[g + gb * (i + nocc * a)];
     r[0] += b0 * k0;
     if (DEN != DEN_RHO) {
 #pragma unroll
       for (int t = 1; t < 4; ++t) {
         const double bt = aob[g + gb * i + sc_b * t];
-        const double kt = akb[g + gb * (i + nocc * a) + sc_k * t];
+        const double kt = (double)akb[g + gb * (i + nocc * a) + sc_k * t];
         r[t] += b0 * kt + bt * k0;
         if (DEN == DEN_TAU) r[4] += 0.5 * bt * kt;
       }
@@ -55,29 +57,30 @@ __device__ __forceinline__ void weighted_kernel(const double* fxc, int64_t ld_fx
 }
 
 // write S_c[g, (i,a)] (into `S`, which may alias akb: each (g, a) owns its entries)
-template <int DEN>
-__device__ __forceinline__ void write_s(const double* aob, double* S, int64_t gb, int64_t g, int64_t nocc,
+template <int DEN, typename T>
+__device__ __forceinline__ void write_s(const double* aob, T* S, int64_t gb, int64_t g, int64_t nocc,
                                         int64_t ncol, int64_t a, const double* wv) {
   const int64_t sc_b = gb * nocc, sc_k = gb * ncol;
   for (int64_t i = 0; i < nocc; ++i) {
     const double b0 = aob[g + gb * i];
     const int64_t o = g + gb * (i + nocc * a);
     if (DEN == DEN_RHO) {
-      S[o] = b0 * wv[0];
+      S[o] = (T)(b0 * wv[0]);
     } else {
       const double b1 = aob[g + gb * i + sc_b], b2 = aob[g + gb * i + 2 * sc_b], b3 = aob[g + gb * i + 3 * sc_b];
-      S[o] = b0 * wv[0] + b1 * wv[1] + b2 * wv[2] + b3 * wv[3];
+      S[o] = (T)(b0 * wv[0] + b1 * wv[1] + b2 * wv[2] + b3 * wv[3]);
       const double h = (DEN == DEN_TAU) ? 0.5 * wv[4] : 0.0;
-      S[o + sc_k] = b0 * wv[1] + h * b1;
-      S[o + 2 * sc_k] = b0 * wv[2] + h * b2;
-      S[o + 3 * sc_k] = b0 * wv[3] + h * b3;
+      S[o + sc_k] = (T)(b0 * wv[1] + h * b1);
+      S[o + 2 * sc_k] = (T)(b0 * wv[2] + h * b2);
+      S[o + 3 * sc_k] = (T)(b0 * wv[3] + h * b3);
     }
   }
 }
 
-template <int DEN>
-__global__ void rho1_kernel(const double* aob, const double* akb, int64_t gb, int64_t nocc, int64_t nprop,
+template <int DEN, typename T>
+__global__ void rho1_kernel(const double* aob, const void* akb_v, int64_t gb, int64_t nocc, int64_t nprop,
                             double* rho1, int64_t ng, int64_t g0) {
+  const T* akb = static_cast<const T*>(akb_v);
   constexpr int NV = DEN == DEN_RHO ? 1 : (DEN == DEN_SIGMA ? 4 : 5);
   const int64_t total = gb * nprop, ncol = nocc * nprop;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
@@ -90,9 +93,10 @@ __global__ void rho1_kernel(const double* aob, const double* akb, int64_t gb, in
 }
 
 // fused: rho1 from (aob, akb), then S into akb
-template <int DEN>
-__global__ void rho1_wv_s_kernel(const double* aob, double* akb, const double* fxc, const double* w, int64_t gb,
+template <int DEN, typename T>
+__global__ void rho1_wv_s_kernel(const double* aob, void* akb_v, const double* fxc, const double* w, int64_t gb,
                                  int64_t nocc, int64_t nprop, int64_t ng, int64_t g0) {
+  T* akb = static_cast<T*>(akb_v);
   const int64_t total = gb * nprop, ncol = nocc * nprop;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t g = e % gb, a = e / gb;
@@ -104,9 +108,10 @@ __global__ void rho1_wv_s_kernel(const double* aob, double* akb, const double* f
 }
 
 // S from a given rho1 [ng, nvar, nprop]
-template <int DEN>
-__global__ void wv_s_kernel(const double* aob, const double* rho1, double* S, const double* fxc, const double* w,
+template <int DEN, typename T>
+__global__ void wv_s_kernel(const double* aob, const double* rho1, void* S_v, const double* fxc, const double* w,
                             int64_t gb, int64_t nocc, int64_t nprop, int64_t ng, int64_t g0) {
+  T* S = static_cast<T*>(S_v);
   constexpr int NV = DEN == DEN_RHO ? 1 : (DEN == DEN_SIGMA ? 4 : 5);
   const int64_t total = gb * nprop, ncol = nocc * nprop;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
@@ -140,10 +145,12 @@ __device__ __forceinline__ void weighted_kernel_uks(const double* fxc, int64_t l
 }
 
 // fused: rho1 of both spins from (aob_s, akb_s), then S_s into akb_s (both rho1 first: S overwrites)
-template <int DEN>
-__global__ void uks_rho1_wv_s_kernel(const double* aob_a, double* akb_a, const double* aob_b, double* akb_b,
+template <int DEN, typename T>
+__global__ void uks_rho1_wv_s_kernel(const double* aob_a, void* akb_a_v, const double* aob_b, void* akb_b_v,
                                      const double* fxc, const double* w, int64_t gb, int64_t nocc_a, int64_t nocc_b,
                                      int64_t nprop, int64_t ng, int64_t g0) {
+  T* akb_a = static_cast<T*>(akb_a_v);
+  T* akb_b = static_cast<T*>(akb_b_v);
   const int64_t total = gb * nprop, ncol_a = nocc_a * nprop, ncol_b = nocc_b * nprop;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t g = e % gb, a = e / gb;
@@ -157,10 +164,12 @@ __global__ void uks_rho1_wv_s_kernel(const double* aob_a, double* akb_a, const d
 }
 
 // S_s from a given rho1 [ng, NV, 2, nprop]
-template <int DEN>
-__global__ void uks_wv_s_kernel(const double* aob_a, const double* aob_b, const double* rho1, double* Sa, double* Sb,
+template <int DEN, typename T>
+__global__ void uks_wv_s_kernel(const double* aob_a, const double* aob_b, const double* rho1, void* Sa_v, void* Sb_v,
                                 const double* fxc, const double* w, int64_t gb, int64_t nocc_a, int64_t nocc_b,
                                 int64_t nprop, int64_t ng, int64_t g0) {
+  T* Sa = static_cast<T*>(Sa_v);
+  T* Sb = static_cast<T*>(Sb_v);
   constexpr int NV = DEN == DEN_RHO ? 1 : (DEN == DEN_SIGMA ? 4 : 5);
   const int64_t total = gb * nprop;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
@@ -308,16 +317,23 @@ int64_t Numint::pick_batch(int64_t ncol) const {
   return std::min(gb, ng_);
 }
 
-#define HH_DEN_DISPATCH(den, KERNEL, ...)                                   \
-  do {                                                                      \
-    if ((den) == DEN_RHO) hipLaunchKernelGGL(KERNEL<DEN_RHO>, __VA_ARGS__); \
-    else if ((den) == DEN_SIGMA) hipLaunchKernelGGL(KERNEL<DEN_SIGMA>, __VA_ARGS__); \
-    else hipLaunchKernelGGL(KERNEL<DEN_TAU>, __VA_ARGS__);                  \
-    HH_HIP(hipGetLastError());                                              \
+// launch KERNEL<den, T> with T = float (bf) or double; the T* arguments are cast by the caller
+#define HH_DEN_DISPATCH_T(den, bf, KERNEL, ...)                                          \
+  do {                                                                                   \
+    if (bf) {                                                                            \
+      if ((den) == DEN_RHO) hipLaunchKernelGGL((KERNEL<DEN_RHO, float>), __VA_ARGS__);   \
+      else if ((den) == DEN_SIGMA) hipLaunchKernelGGL((KERNEL<DEN_SIGMA, float>), __VA_ARGS__); \
+      else hipLaunchKernelGGL((KERNEL<DEN_TAU, float>), __VA_ARGS__);                    \
+    } else {                                                                             \
+      if ((den) == DEN_RHO) hipLaunchKernelGGL((KERNEL<DEN_RHO, double>), __VA_ARGS__);  \
+      else if ((den) == DEN_SIGMA) hipLaunchKernelGGL((KERNEL<DEN_SIGMA, double>), __VA_ARGS__); \
+      else hipLaunchKernelGGL((KERNEL<DEN_TAU, double>), __VA_ARGS__);                   \
+    }                                                                                    \
+    HH_HIP(hipGetLastError());                                                           \
   } while (0)
 
 namespace {
-// aob[g, i, c] and akb[g, (i,a), c] for one grid batch
+// out[g, n, c] = sum_mu ao[g0+g, mu, c] B[mu, n] for one grid batch (c < ncomp)
 void half_transform(const double* ao, int64_t ng, int64_t nao, int64_t g0, int64_t gb, int ncomp, const double* B,
                     int64_t ncolB, double* out, Prec prec, hipStream_t s, DevBuf& ws) {
   GemmArgs g;
@@ -343,23 +359,138 @@ void back_contract(const double* ao, int64_t ng, int64_t nao, int64_t g0, int64_
     gemm<double, double>(g, prec, s, ws);
   }
 }
+
+// AO values of one component as the tiled BF16X3 operand with K = mu (rows = grid points):
+// hi/lo[((mu / 32) * ngp + g) * 32 + mu % 32] = split(ao[g + ng mu]), zero for mu >= nao or g >= ng.
+// One thread per (K block, g): reads are consecutive in g, each thread writes 64 contiguous bytes.
+__global__ void ao_tiled_kmu_kernel(const double* __restrict__ ao, int64_t ng, int64_t nao, int64_t ngp, int64_t nkb,
+                                    unsigned short* __restrict__ hi, unsigned short* __restrict__ lo) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= nkb * ngp) return;
+  const int64_t g = e % ngp, kb = e / ngp;
+  unsigned short h[32], l[32];
+#pragma unroll
+  for (int kk = 0; kk < 32; ++kk) {
+    const int64_t mu = kb * 32 + kk;
+    const float x = (g < ng && mu < nao) ? (float)ao[g + ng * mu] : 0.0f;
+    const __hip_bfloat16 bh = __float2bfloat16(x);
+    const float r = x - __bfloat162float(bh);
+    const __hip_bfloat16 bl = __float2bfloat16(r);
+    h[kk] = *reinterpret_cast<const unsigned short*>(&bh);
+    l[kk] = *reinterpret_cast<const unsigned short*>(&bl);
+  }
+  uint4* ph = reinterpret_cast<uint4*>(hi + e * 32);
+  uint4* pl = reinterpret_cast<uint4*>(lo + e * 32);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    ph[q] = make_uint4(h[8 * q] | (unsigned)h[8 * q + 1] << 16, h[8 * q + 2] | (unsigned)h[8 * q + 3] << 16,
+                       h[8 * q + 4] | (unsigned)h[8 * q + 5] << 16, h[8 * q + 6] | (unsigned)h[8 * q + 7] << 16);
+    pl[q] = make_uint4(l[8 * q] | (unsigned)l[8 * q + 1] << 16, l[8 * q + 2] | (unsigned)l[8 * q + 3] << 16,
+                       l[8 * q + 4] | (unsigned)l[8 * q + 5] << 16, l[8 * q + 6] | (unsigned)l[8 * q + 7] << 16);
+  }
+}
 }  // namespace
+
+// Tiled bf16 hi/lo copies of the AO values (components 0..3) for the BF16X3 grid GEMMs:
+// aok_* with K = grid point (back-contraction A operand, rows mu), aot_* with K = mu
+// (half-transform A operand, rows = grid points).  Built once per handle.
+void Numint::ensure_bf16() {
+  if (bf_ready_) return;
+  ngp_ = ceil_div(ng_, 32) * 32;
+  naop_ = ceil_div(nao_, 32) * 32;
+  const int nc = (int)std::min<int64_t>(ncomp_, 4);
+  const size_t nk = (size_t)ngp_ * nao_, nt = (size_t)naop_ * ngp_;
+  unsigned short* kh = static_cast<unsigned short*>(aok_hi_.reserve(sizeof(unsigned short) * nk * nc));
+  unsigned short* kl = static_cast<unsigned short*>(aok_lo_.reserve(sizeof(unsigned short) * nk * nc));
+  unsigned short* th = static_cast<unsigned short*>(aot_hi_.reserve(sizeof(unsigned short) * nt * nc));
+  unsigned short* tl = static_cast<unsigned short*>(aot_lo_.reserve(sizeof(unsigned short) * nt * nc));
+  for (int c = 0; c < nc; ++c) {
+    const double* a = ao_.as<double>() + ng_ * nao_ * c;
+    split_bf16_tiled<double>(a, ng_, ng_, nao_, ngp_, kh + nk * c, kl + nk * c, stream_);
+    const int64_t n = (naop_ / 32) * ngp_;
+    hipLaunchKernelGGL(ao_tiled_kmu_kernel, dim3(nblk(n)), dim3(256), 0, stream_, a, ng_, nao_, ngp_, naop_ / 32,
+                       th + nt * c, tl + nt * c);
+    HH_HIP(hipGetLastError());
+  }
+  bf_ready_ = true;
+}
+
+// BF16X3 half-transform of one batch: out[g, n, c] (FP32) = sum_mu ao[g0+g, mu, c] B[mu, n], with
+// B given as its tiled split (K = mu, kpad naop_)
+void Numint::kets_bf16(int nc, int64_t g0, int64_t gb, const unsigned short* bh, const unsigned short* bl,
+                       int64_t ncol, float* out) {
+  const size_t nt = (size_t)naop_ * ngp_;
+  for (int c = 0; c < nc; ++c) {
+    NtArgs a;
+    a.M = gb; a.N = ncol; a.K = naop_;
+    a.A = aot_hi_.as<unsigned short>() + nt * c + g0 * 32;
+    a.A_lo = aot_lo_.as<unsigned short>() + nt * c + g0 * 32;
+    a.a_rows = ngp_;
+    a.B = bh; a.B_lo = bl;
+    a.C_f32 = out + gb * ncol * c; a.ldc = gb;
+    a.tiled = true;
+    gemm_nt(a, NT_BF16X3, stream_, ws_);
+  }
+}
+
+// BF16X3 back-contraction of one batch: out[mu, n] (+)= alpha sum_c sum_g ao[g0+g, mu, c] S_c[g, n]
+void Numint::back_bf16(int nc, int64_t g0, int64_t gb, const float* S, int64_t ncol, double* out, double alpha,
+                       bool accumulate) {
+  const int64_t gbp = ceil_div(gb, 32) * 32;
+  const size_t nk = (size_t)ngp_ * nao_;
+  unsigned short* sh = static_cast<unsigned short*>(sh_.reserve(sizeof(unsigned short) * gbp * ncol));
+  unsigned short* sl = static_cast<unsigned short*>(sl_.reserve(sizeof(unsigned short) * gbp * ncol));
+  for (int c = 0; c < nc; ++c) {
+    split_bf16_tiled<float>(S + gb * ncol * c, gb, gb, ncol, gbp, sh, sl, stream_);
+    NtArgs a;
+    a.M = nao_; a.N = ncol; a.K = gbp;
+    a.A = aok_hi_.as<unsigned short>() + nk * c + (g0 / 32) * nao_ * 32;
+    a.A_lo = aok_lo_.as<unsigned short>() + nk * c + (g0 / 32) * nao_ * 32;
+    a.B = sh; a.B_lo = sl;
+    a.C = out; a.ldc = nao_;
+    a.alpha = alpha;
+    a.beta = (accumulate || c > 0) ? 1.0 : 0.0;
+    a.tiled = true;
+    gemm_nt(a, NT_BF16X3, stream_, ws_);
+  }
+}
+
+// tiled split of a [nao, ncol] operand (K = mu) into the given buffers
+void Numint::split_kets(const double* B, int64_t ncol, DevBuf& hi, DevBuf& lo) {
+  unsigned short* h = static_cast<unsigned short*>(hi.reserve(sizeof(unsigned short) * naop_ * ncol));
+  unsigned short* l = static_cast<unsigned short*>(lo.reserve(sizeof(unsigned short) * naop_ * ncol));
+  split_bf16_tiled<double>(B, nao_, nao_, ncol, naop_, h, l, stream_);
+}
+
+// Batch size: from memory, or the user's; a multiple of 32 for the BF16X3 path (K blocks).
+int64_t Numint::batch_for(int64_t ncol, bool bf) const {
+  int64_t gb = pick_batch(ncol);
+  if (bf) gb = std::min<int64_t>(ceil_div(gb, 32) * 32, ceil_div(ng_, 32) * 32);
+  return gb;
+}
 
 void Numint::rho1_one_bra_mult_ket(int den, const double* bra, int64_t nocc, const double* kets, int64_t nprop,
                                    double* rho1, Prec prec) {
   HH_REQUIRE(den >= DEN_RHO && den <= DEN_TAU, "numint: den_type must be RHO, SIGMA or TAU");
   const int nc = den_ncomp(den);
   HH_REQUIRE(ncomp_ >= nc, "numint: AO component dimension insufficient");
+  const bool bf = prec == PREC_BF16X3;
   const int64_t ncol = nocc * nprop;
-  const int64_t gbmax = pick_batch(ncol);
+  const int64_t gbmax = batch_for(ncol, bf);
+  const size_t es = bf ? sizeof(float) : sizeof(double);
   double* aob = static_cast<double*>(aob_.reserve(sizeof(double) * gbmax * nocc * nc));
-  double* akb = static_cast<double*>(akb_.reserve(sizeof(double) * gbmax * ncol * nc));
+  void* akb = akb_.reserve(es * gbmax * ncol * nc);
+  if (bf) {
+    ensure_bf16();
+    split_kets(kets, ncol, kh_, kl_);
+  }
   for (int64_t g0 = 0; g0 < ng_; g0 += gbmax) {
     const int64_t gb = std::min(gbmax, ng_ - g0);
-    half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, bra, nocc, aob, prec, stream_, ws_);
-    half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, kets, ncol, akb, prec, stream_, ws_);
-    HH_DEN_DISPATCH(den, rho1_kernel, dim3(nblk(gb * nprop)), dim3(256), 0, stream_, aob, akb, gb, nocc, nprop, rho1,
-                    ng_, g0);
+    half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, bra, nocc, aob, PREC_F64, stream_, ws_);
+    if (bf) kets_bf16(nc, g0, gb, kh_.as<unsigned short>(), kl_.as<unsigned short>(), ncol, static_cast<float*>(akb));
+    else half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, kets, ncol, static_cast<double*>(akb), prec, stream_, ws_);
+    HH_DEN_DISPATCH_T(den, bf, rho1_kernel, dim3(nblk(gb * nprop)), dim3(256), 0, stream_, aob, akb, gb, nocc, nprop,
+                      rho1, ng_, g0);
   }
 }
 
@@ -368,17 +499,21 @@ void Numint::fxc_pot_bra_trans(int den, const double* fxc_eff, const double* rho
   HH_REQUIRE(den >= DEN_RHO && den <= DEN_TAU, "numint: den_type must be RHO, SIGMA or TAU");
   const int nc = den_ncomp(den);
   HH_REQUIRE(ncomp_ >= nc, "numint: AO component dimension insufficient");
+  const bool bf = prec == PREC_BF16X3;
   const int64_t ncol = nocc * nprop;
-  const int64_t gbmax = pick_batch(ncol);
+  const int64_t gbmax = batch_for(ncol, bf);
+  const size_t es = bf ? sizeof(float) : sizeof(double);
   double* aob = static_cast<double*>(aob_.reserve(sizeof(double) * gbmax * nocc * nc));
-  double* S = static_cast<double*>(akb_.reserve(sizeof(double) * gbmax * ncol * nc));
+  void* S = akb_.reserve(es * gbmax * ncol * nc);
+  if (bf) ensure_bf16();
   for (int64_t g0 = 0; g0 < ng_; g0 += gbmax) {
     const int64_t gb = std::min(gbmax, ng_ - g0);
-    half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, bra, nocc, aob, prec, stream_, ws_);
-    HH_DEN_DISPATCH(den, wv_s_kernel, dim3(nblk(gb * nprop)), dim3(256), 0, stream_, aob, rho1, S, fxc_eff,
-                    w_.as<double>(), gb, nocc, nprop, ng_, g0);
-    back_contract(ao_.as<double>(), ng_, nao_, g0, gb, nc, S, ncol, out, alpha, accumulate || g0 > 0, prec, stream_,
-                  ws_);
+    half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, bra, nocc, aob, PREC_F64, stream_, ws_);
+    HH_DEN_DISPATCH_T(den, bf, wv_s_kernel, dim3(nblk(gb * nprop)), dim3(256), 0, stream_, aob, rho1, S, fxc_eff,
+                      w_.as<double>(), gb, nocc, nprop, ng_, g0);
+    if (bf) back_bf16(nc, g0, gb, static_cast<float*>(S), ncol, out, alpha, accumulate || g0 > 0);
+    else back_contract(ao_.as<double>(), ng_, nao_, g0, gb, nc, static_cast<double*>(S), ncol, out, alpha,
+                       accumulate || g0 > 0, prec, stream_, ws_);
   }
 }
 
@@ -387,19 +522,28 @@ void Numint::rks_response_bra(int den, const double* fxc_eff, const double* mo1_
   HH_REQUIRE(den >= DEN_RHO && den <= DEN_TAU, "numint: den_type must be RHO, SIGMA or TAU");
   const int nc = den_ncomp(den);
   HH_REQUIRE(ncomp_ >= nc, "numint: AO component dimension insufficient");
+  const bool bf = prec == PREC_BF16X3;
   const int64_t ncol = nocc * nprop;
-  const int64_t gbmax = pick_batch(ncol);
+  const int64_t gbmax = batch_for(ncol, bf);
+  const size_t es = bf ? sizeof(float) : sizeof(double);
   double* aob = static_cast<double*>(aob_.reserve(sizeof(double) * gbmax * nocc * nc));
-  double* akb = static_cast<double*>(akb_.reserve(sizeof(double) * gbmax * ncol * nc));
+  void* akb = akb_.reserve(es * gbmax * ncol * nc);
+  if (bf) {
+    ensure_bf16();
+    split_kets(mo1_bra, ncol, kh_, kl_);
+  }
   for (int64_t g0 = 0; g0 < ng_; g0 += gbmax) {
     const int64_t gb = std::min(gbmax, ng_ - g0);
-    half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, mocc, nocc, aob, prec, stream_, ws_);
-    half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, mo1_bra, ncol, akb, prec, stream_, ws_);
-    HH_DEN_DISPATCH(den, rho1_wv_s_kernel, dim3(nblk(gb * nprop)), dim3(256), 0, stream_, aob, akb, fxc_eff,
-                    w_.as<double>(), gb, nocc, nprop, ng_, g0);
+    half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, mocc, nocc, aob, PREC_F64, stream_, ws_);
+    if (bf) kets_bf16(nc, g0, gb, kh_.as<unsigned short>(), kl_.as<unsigned short>(), ncol, static_cast<float*>(akb));
+    else half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, mo1_bra, ncol, static_cast<double*>(akb), prec,
+                        stream_, ws_);
+    HH_DEN_DISPATCH_T(den, bf, rho1_wv_s_kernel, dim3(nblk(gb * nprop)), dim3(256), 0, stream_, aob, akb, fxc_eff,
+                      w_.as<double>(), gb, nocc, nprop, ng_, g0);
     // the x4 closed-shell factor of hess_rks.rs:603
-    back_contract(ao_.as<double>(), ng_, nao_, g0, gb, nc, akb, ncol, out, 4.0, accumulate || g0 > 0, prec, stream_,
-                  ws_);
+    if (bf) back_bf16(nc, g0, gb, static_cast<float*>(akb), ncol, out, 4.0, accumulate || g0 > 0);
+    else back_contract(ao_.as<double>(), ng_, nao_, g0, gb, nc, static_cast<double*>(akb), ncol, out, 4.0,
+                       accumulate || g0 > 0, prec, stream_, ws_);
   }
 }
 
@@ -409,22 +553,30 @@ void Numint::uks_fxc_pot_bra_trans(int den, const double* fxc_eff, const double*
   HH_REQUIRE(den >= DEN_RHO && den <= DEN_TAU, "numint: den_type must be RHO, SIGMA or TAU");
   const int nc = den_ncomp(den);
   HH_REQUIRE(ncomp_ >= nc, "numint: AO component dimension insufficient");
+  const bool bf = prec == PREC_BF16X3;
   const int64_t ncol_a = nocc[0] * nprop, ncol_b = nocc[1] * nprop;
-  const int64_t gbmax = pick_batch(ncol_a + ncol_b);
+  const int64_t gbmax = batch_for(ncol_a + ncol_b, bf);
+  const size_t es = bf ? sizeof(float) : sizeof(double);
   double* aob_a = static_cast<double*>(aob_.reserve(sizeof(double) * gbmax * nocc[0] * nc));
   double* aob_b = static_cast<double*>(aob2_.reserve(sizeof(double) * gbmax * nocc[1] * nc));
-  double* Sa = static_cast<double*>(akb_.reserve(sizeof(double) * gbmax * ncol_a * nc));
-  double* Sb = static_cast<double*>(akb2_.reserve(sizeof(double) * gbmax * ncol_b * nc));
+  void* Sa = akb_.reserve(es * gbmax * ncol_a * nc);
+  void* Sb = akb2_.reserve(es * gbmax * ncol_b * nc);
+  if (bf) ensure_bf16();
   for (int64_t g0 = 0; g0 < ng_; g0 += gbmax) {
     const int64_t gb = std::min(gbmax, ng_ - g0);
-    half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, bra[0], nocc[0], aob_a, prec, stream_, ws_);
-    half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, bra[1], nocc[1], aob_b, prec, stream_, ws_);
-    HH_DEN_DISPATCH(den, uks_wv_s_kernel, dim3(nblk(gb * nprop)), dim3(256), 0, stream_, aob_a, aob_b, rho1, Sa, Sb,
-                    fxc_eff, w_.as<double>(), gb, nocc[0], nocc[1], nprop, ng_, g0);
-    back_contract(ao_.as<double>(), ng_, nao_, g0, gb, nc, Sa, ncol_a, out[0], alpha, accumulate || g0 > 0, prec,
-                  stream_, ws_);
-    back_contract(ao_.as<double>(), ng_, nao_, g0, gb, nc, Sb, ncol_b, out[1], alpha, accumulate || g0 > 0, prec,
-                  stream_, ws_);
+    half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, bra[0], nocc[0], aob_a, PREC_F64, stream_, ws_);
+    half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, bra[1], nocc[1], aob_b, PREC_F64, stream_, ws_);
+    HH_DEN_DISPATCH_T(den, bf, uks_wv_s_kernel, dim3(nblk(gb * nprop)), dim3(256), 0, stream_, aob_a, aob_b, rho1, Sa,
+                      Sb, fxc_eff, w_.as<double>(), gb, nocc[0], nocc[1], nprop, ng_, g0);
+    if (bf) {
+      back_bf16(nc, g0, gb, static_cast<float*>(Sa), ncol_a, out[0], alpha, accumulate || g0 > 0);
+      back_bf16(nc, g0, gb, static_cast<float*>(Sb), ncol_b, out[1], alpha, accumulate || g0 > 0);
+    } else {
+      back_contract(ao_.as<double>(), ng_, nao_, g0, gb, nc, static_cast<double*>(Sa), ncol_a, out[0], alpha,
+                    accumulate || g0 > 0, prec, stream_, ws_);
+      back_contract(ao_.as<double>(), ng_, nao_, g0, gb, nc, static_cast<double*>(Sb), ncol_b, out[1], alpha,
+                    accumulate || g0 > 0, prec, stream_, ws_);
+    }
   }
 }
 
@@ -434,25 +586,44 @@ void Numint::uks_response_bra(int den, const double* fxc_eff, const double* cons
   HH_REQUIRE(den >= DEN_RHO && den <= DEN_TAU, "numint: den_type must be RHO, SIGMA or TAU");
   const int nc = den_ncomp(den);
   HH_REQUIRE(ncomp_ >= nc, "numint: AO component dimension insufficient");
+  const bool bf = prec == PREC_BF16X3;
   const int64_t ncol_a = nocc[0] * nprop, ncol_b = nocc[1] * nprop;
-  const int64_t gbmax = pick_batch(ncol_a + ncol_b);
+  const int64_t gbmax = batch_for(ncol_a + ncol_b, bf);
+  const size_t es = bf ? sizeof(float) : sizeof(double);
   double* aob_a = static_cast<double*>(aob_.reserve(sizeof(double) * gbmax * nocc[0] * nc));
   double* aob_b = static_cast<double*>(aob2_.reserve(sizeof(double) * gbmax * nocc[1] * nc));
-  double* akb_a = static_cast<double*>(akb_.reserve(sizeof(double) * gbmax * ncol_a * nc));
-  double* akb_b = static_cast<double*>(akb2_.reserve(sizeof(double) * gbmax * ncol_b * nc));
+  void* akb_a = akb_.reserve(es * gbmax * ncol_a * nc);
+  void* akb_b = akb2_.reserve(es * gbmax * ncol_b * nc);
+  if (bf) {
+    ensure_bf16();
+    split_kets(mo1_bra[0], ncol_a, kh_, kl_);
+    split_kets(mo1_bra[1], ncol_b, kh2_, kl2_);
+  }
   for (int64_t g0 = 0; g0 < ng_; g0 += gbmax) {
     const int64_t gb = std::min(gbmax, ng_ - g0);
-    half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, mocc[0], nocc[0], aob_a, prec, stream_, ws_);
-    half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, mocc[1], nocc[1], aob_b, prec, stream_, ws_);
-    half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, mo1_bra[0], ncol_a, akb_a, prec, stream_, ws_);
-    half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, mo1_bra[1], ncol_b, akb_b, prec, stream_, ws_);
-    HH_DEN_DISPATCH(den, uks_rho1_wv_s_kernel, dim3(nblk(gb * nprop)), dim3(256), 0, stream_, aob_a, akb_a, aob_b,
-                    akb_b, fxc_eff, w_.as<double>(), gb, nocc[0], nocc[1], nprop, ng_, g0);
+    half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, mocc[0], nocc[0], aob_a, PREC_F64, stream_, ws_);
+    half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, mocc[1], nocc[1], aob_b, PREC_F64, stream_, ws_);
+    if (bf) {
+      kets_bf16(nc, g0, gb, kh_.as<unsigned short>(), kl_.as<unsigned short>(), ncol_a, static_cast<float*>(akb_a));
+      kets_bf16(nc, g0, gb, kh2_.as<unsigned short>(), kl2_.as<unsigned short>(), ncol_b, static_cast<float*>(akb_b));
+    } else {
+      half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, mo1_bra[0], ncol_a, static_cast<double*>(akb_a), prec,
+                     stream_, ws_);
+      half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, mo1_bra[1], ncol_b, static_cast<double*>(akb_b), prec,
+                     stream_, ws_);
+    }
+    HH_DEN_DISPATCH_T(den, bf, uks_rho1_wv_s_kernel, dim3(nblk(gb * nprop)), dim3(256), 0, stream_, aob_a, akb_a,
+                      aob_b, akb_b, fxc_eff, w_.as<double>(), gb, nocc[0], nocc[1], nprop, ng_, g0);
     // the x2 factor of hess_uks.rs:436-439
-    back_contract(ao_.as<double>(), ng_, nao_, g0, gb, nc, akb_a, ncol_a, out[0], 2.0, accumulate || g0 > 0, prec,
-                  stream_, ws_);
-    back_contract(ao_.as<double>(), ng_, nao_, g0, gb, nc, akb_b, ncol_b, out[1], 2.0, accumulate || g0 > 0, prec,
-                  stream_, ws_);
+    if (bf) {
+      back_bf16(nc, g0, gb, static_cast<float*>(akb_a), ncol_a, out[0], 2.0, accumulate || g0 > 0);
+      back_bf16(nc, g0, gb, static_cast<float*>(akb_b), ncol_b, out[1], 2.0, accumulate || g0 > 0);
+    } else {
+      back_contract(ao_.as<double>(), ng_, nao_, g0, gb, nc, static_cast<double*>(akb_a), ncol_a, out[0], 2.0,
+                    accumulate || g0 > 0, prec, stream_, ws_);
+      back_contract(ao_.as<double>(), ng_, nao_, g0, gb, nc, static_cast<double*>(akb_b), ncol_b, out[1], 2.0,
+                    accumulate || g0 > 0, prec, stream_, ws_);
+    }
   }
 }
 

@@ -67,9 +67,20 @@ class Numint {
 
  private:
   int64_t pick_batch(int64_t ncol) const;
+  int64_t batch_for(int64_t ncol, bool bf) const;
+  // BF16X3 grid GEMMs on the tiled NT kernel (FP32 intermediates)
+  void ensure_bf16();
+  void split_kets(const double* B, int64_t ncol, DevBuf& hi, DevBuf& lo);
+  void kets_bf16(int nc, int64_t g0, int64_t gb, const unsigned short* bh, const unsigned short* bl, int64_t ncol,
+                 float* out);
+  void back_bf16(int nc, int64_t g0, int64_t gb, const float* S, int64_t ncol, double* out, double alpha,
+                 bool accumulate);
   int64_t ng_, nao_, ncomp_;
   hipStream_t stream_;
   DevBuf ao_, w_, aob_, akb_, aob2_, akb2_, rho_, ws_;
+  bool bf_ready_ = false;
+  int64_t ngp_ = 0, naop_ = 0;
+  DevBuf aok_hi_, aok_lo_, aot_hi_, aot_lo_, kh_, kl_, kh2_, kl2_, sh_, sl_;
 };
 
 // LibXC-variable derivatives -> rho_t-based effective kernels, unpolarised (reference:
