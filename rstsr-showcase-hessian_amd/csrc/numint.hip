@@ -124,6 +124,67 @@ __global__ void wv_s_kernel(const double* aob, const double* rho1, void* S_v, co
   }
 }
 
+// ---- BF16X3: S written directly as the tiled bf16 hi/lo back-contraction operand ----
+// component c of element (g, n) at c * cs + ((g / 32) * ncol + n) * 32 + g % 32 (K = grid point)
+__device__ __forceinline__ void put_split(unsigned short* hi, unsigned short* lo, int64_t idx, double v) {
+  const float x = (float)v;
+  const __bf16 h = (__bf16)x;
+  hi[idx] = __builtin_bit_cast(unsigned short, h);
+  lo[idx] = __builtin_bit_cast(unsigned short, (__bf16)(x - (float)h));
+}
+
+template <int DEN>
+__device__ __forceinline__ void write_s_tiled(const double* aob, unsigned short* hi, unsigned short* lo, int64_t cs,
+                                              int64_t gb, int64_t g, int64_t nocc, int64_t ncol, int64_t a,
+                                              const double* wv) {
+  const int64_t sc_b = gb * nocc;
+  const int64_t tb = (g >> 5) * ncol * 32 + (g & 31);
+  for (int64_t i = 0; i < nocc; ++i) {
+    const double b0 = aob[g + gb * i];
+    const int64_t t = tb + (i + nocc * a) * 32;
+    if (DEN == DEN_RHO) {
+      put_split(hi, lo, t, b0 * wv[0]);
+    } else {
+      const double b1 = aob[g + gb * i + sc_b], b2 = aob[g + gb * i + 2 * sc_b], b3 = aob[g + gb * i + 3 * sc_b];
+      const double h = (DEN == DEN_TAU) ? 0.5 * wv[4] : 0.0;
+      put_split(hi, lo, t, b0 * wv[0] + b1 * wv[1] + b2 * wv[2] + b3 * wv[3]);
+      put_split(hi, lo, t + cs, b0 * wv[1] + h * b1);
+      put_split(hi, lo, t + 2 * cs, b0 * wv[2] + h * b2);
+      put_split(hi, lo, t + 3 * cs, b0 * wv[3] + h * b3);
+    }
+  }
+}
+
+template <int DEN>
+__global__ void rho1_wv_s_tiled_kernel(const double* aob, const float* akb, unsigned short* hi, unsigned short* lo,
+                                       int64_t cs, const double* fxc, const double* w, int64_t gb, int64_t nocc,
+                                       int64_t nprop, int64_t ng, int64_t g0) {
+  const int64_t total = gb * nprop, ncol = nocc * nprop;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t g = e % gb, a = e / gb;
+    double r[5], wv[5];
+    rho1_point<DEN, float>(aob, akb, gb, g, nocc, ncol, a, r);
+    weighted_kernel<DEN>(fxc + g0, ng, g, w[g0 + g], r, wv);
+    write_s_tiled<DEN>(aob, hi, lo, cs, gb, g, nocc, ncol, a, wv);
+  }
+}
+
+template <int DEN>
+__global__ void wv_s_tiled_kernel(const double* aob, const double* rho1, unsigned short* hi, unsigned short* lo,
+                                  int64_t cs, const double* fxc, const double* w, int64_t gb, int64_t nocc,
+                                  int64_t nprop, int64_t ng, int64_t g0) {
+  constexpr int NV = DEN == DEN_RHO ? 1 : (DEN == DEN_SIGMA ? 4 : 5);
+  const int64_t total = gb * nprop, ncol = nocc * nprop;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t g = e % gb, a = e / gb;
+    double r[5], wv[5];
+#pragma unroll
+    for (int x = 0; x < NV; ++x) r[x] = rho1[(g0 + g) + ng * (x + NV * a)];
+    weighted_kernel<DEN>(fxc + g0, ng, g, w[g0 + g], r, wv);
+    write_s_tiled<DEN>(aob, hi, lo, cs, gb, g, nocc, ncol, a, wv);
+  }
+}
+
 // ---- UKS: the kernel couples the spins (fxc_eff [ng, NV, 2, NV, 2]) ----
 // w_s[x] = w sum_{y,t} fxc[g, y, t, x, s] r_t[y]
 template <int DEN>
@@ -183,6 +244,46 @@ __global__ void uks_wv_s_kernel(const double* aob_a, const double* aob_b, const 
     weighted_kernel_uks<DEN>(fxc + g0, ng, g, w[g0 + g], ra, rb, wa, wb);
     write_s<DEN>(aob_a, Sa, gb, g, nocc_a, nocc_a * nprop, a, wa);
     write_s<DEN>(aob_b, Sb, gb, g, nocc_b, nocc_b * nprop, a, wb);
+  }
+}
+
+template <int DEN>
+__global__ void uks_rho1_wv_s_tiled_kernel(const double* aob_a, const float* akb_a, const double* aob_b,
+                                           const float* akb_b, unsigned short* hi_a, unsigned short* lo_a,
+                                           int64_t cs_a, unsigned short* hi_b, unsigned short* lo_b, int64_t cs_b,
+                                           const double* fxc, const double* w, int64_t gb, int64_t nocc_a,
+                                           int64_t nocc_b, int64_t nprop, int64_t ng, int64_t g0) {
+  const int64_t total = gb * nprop, ncol_a = nocc_a * nprop, ncol_b = nocc_b * nprop;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t g = e % gb, a = e / gb;
+    double ra[5], rb[5], wa[5], wb[5];
+    rho1_point<DEN, float>(aob_a, akb_a, gb, g, nocc_a, ncol_a, a, ra);
+    rho1_point<DEN, float>(aob_b, akb_b, gb, g, nocc_b, ncol_b, a, rb);
+    weighted_kernel_uks<DEN>(fxc + g0, ng, g, w[g0 + g], ra, rb, wa, wb);
+    write_s_tiled<DEN>(aob_a, hi_a, lo_a, cs_a, gb, g, nocc_a, ncol_a, a, wa);
+    write_s_tiled<DEN>(aob_b, hi_b, lo_b, cs_b, gb, g, nocc_b, ncol_b, a, wb);
+  }
+}
+
+template <int DEN>
+__global__ void uks_wv_s_tiled_kernel(const double* aob_a, const double* aob_b, const double* rho1,
+                                      unsigned short* hi_a, unsigned short* lo_a, int64_t cs_a, unsigned short* hi_b,
+                                      unsigned short* lo_b, int64_t cs_b, const double* fxc, const double* w,
+                                      int64_t gb, int64_t nocc_a, int64_t nocc_b, int64_t nprop, int64_t ng,
+                                      int64_t g0) {
+  constexpr int NV = DEN == DEN_RHO ? 1 : (DEN == DEN_SIGMA ? 4 : 5);
+  const int64_t total = gb * nprop;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t g = e % gb, a = e / gb;
+    double ra[5], rb[5], wa[5], wb[5];
+#pragma unroll
+    for (int y = 0; y < NV; ++y) {
+      ra[y] = rho1[(g0 + g) + ng * (y + NV * (0 + 2 * a))];
+      rb[y] = rho1[(g0 + g) + ng * (y + NV * (1 + 2 * a))];
+    }
+    weighted_kernel_uks<DEN>(fxc + g0, ng, g, w[g0 + g], ra, rb, wa, wb);
+    write_s_tiled<DEN>(aob_a, hi_a, lo_a, cs_a, gb, g, nocc_a, nocc_a * nprop, a, wa);
+    write_s_tiled<DEN>(aob_b, hi_b, lo_b, cs_b, gb, g, nocc_b, nocc_b * nprop, a, wb);
   }
 }
 
@@ -317,6 +418,14 @@ int64_t Numint::pick_batch(int64_t ncol) const {
   return std::min(gb, ng_);
 }
 
+#define HH_DEN_DISPATCH(den, KERNEL, ...)                                     \
+  do {                                                                        \
+    if ((den) == DEN_RHO) hipLaunchKernelGGL(KERNEL<DEN_RHO>, __VA_ARGS__);   \
+    else if ((den) == DEN_SIGMA) hipLaunchKernelGGL(KERNEL<DEN_SIGMA>, __VA_ARGS__); \
+    else hipLaunchKernelGGL(KERNEL<DEN_TAU>, __VA_ARGS__);                    \
+    HH_HIP(hipGetLastError());                                                \
+  } while (0)
+
 // launch KERNEL<den, T> with T = float (bf) or double; the T* arguments are cast by the caller
 #define HH_DEN_DISPATCH_T(den, bf, KERNEL, ...)                                          \
   do {                                                                                   \
@@ -433,20 +542,32 @@ void Numint::kets_bf16(int nc, int64_t g0, int64_t gb, const unsigned short* bh,
   }
 }
 
-// BF16X3 back-contraction of one batch: out[mu, n] (+)= alpha sum_c sum_g ao[g0+g, mu, c] S_c[g, n]
-void Numint::back_bf16(int nc, int64_t g0, int64_t gb, const float* S, int64_t ncol, double* out, double alpha,
-                       bool accumulate) {
-  const int64_t gbp = ceil_div(gb, 32) * 32;
+// Tiled bf16 hi/lo buffers for S of one batch ([nc][gbp / 32][ncol][32] each); rows of the last K
+// block beyond gb are cleared (the pointwise kernels write rows < gb).
+void Numint::s_tiled(int nc, int64_t gb, int64_t ncol, DevBuf& hi, DevBuf& lo, unsigned short** h,
+                     unsigned short** l) {
+  const int64_t gbp = ceil_div(gb, 32) * 32, cs = gbp * ncol;
+  *h = static_cast<unsigned short*>(hi.reserve(sizeof(unsigned short) * cs * nc));
+  *l = static_cast<unsigned short*>(lo.reserve(sizeof(unsigned short) * cs * nc));
+  if (gb % 32 != 0)
+    for (int c = 0; c < nc; ++c) {
+      fill<unsigned short>(*h + cs * c + (gbp - 32) * ncol, 32 * ncol, 0, stream_);
+      fill<unsigned short>(*l + cs * c + (gbp - 32) * ncol, 32 * ncol, 0, stream_);
+    }
+}
+
+// BF16X3 back-contraction of one batch: out[mu, n] (+)= alpha sum_c sum_g ao[g0+g, mu, c] S_c[g, n],
+// S given tiled (s_tiled layout)
+void Numint::back_bf16(int nc, int64_t g0, int64_t gb, const unsigned short* sh, const unsigned short* sl,
+                       int64_t ncol, double* out, double alpha, bool accumulate) {
+  const int64_t gbp = ceil_div(gb, 32) * 32, cs = gbp * ncol;
   const size_t nk = (size_t)ngp_ * nao_;
-  unsigned short* sh = static_cast<unsigned short*>(sh_.reserve(sizeof(unsigned short) * gbp * ncol));
-  unsigned short* sl = static_cast<unsigned short*>(sl_.reserve(sizeof(unsigned short) * gbp * ncol));
   for (int c = 0; c < nc; ++c) {
-    split_bf16_tiled<float>(S + gb * ncol * c, gb, gb, ncol, gbp, sh, sl, stream_);
     NtArgs a;
     a.M = nao_; a.N = ncol; a.K = gbp;
     a.A = aok_hi_.as<unsigned short>() + nk * c + (g0 / 32) * nao_ * 32;
     a.A_lo = aok_lo_.as<unsigned short>() + nk * c + (g0 / 32) * nao_ * 32;
-    a.B = sh; a.B_lo = sl;
+    a.B = sh + cs * c; a.B_lo = sl + cs * c;
     a.C = out; a.ldc = nao_;
     a.alpha = alpha;
     a.beta = (accumulate || c > 0) ? 1.0 : 0.0;
@@ -504,16 +625,23 @@ void Numint::fxc_pot_bra_trans(int den, const double* fxc_eff, const double* rho
   const int64_t gbmax = batch_for(ncol, bf);
   const size_t es = bf ? sizeof(float) : sizeof(double);
   double* aob = static_cast<double*>(aob_.reserve(sizeof(double) * gbmax * nocc * nc));
-  void* S = akb_.reserve(es * gbmax * ncol * nc);
+  void* S = bf ? nullptr : akb_.reserve(es * gbmax * ncol * nc);
   if (bf) ensure_bf16();
   for (int64_t g0 = 0; g0 < ng_; g0 += gbmax) {
     const int64_t gb = std::min(gbmax, ng_ - g0);
     half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, bra, nocc, aob, PREC_F64, stream_, ws_);
-    HH_DEN_DISPATCH_T(den, bf, wv_s_kernel, dim3(nblk(gb * nprop)), dim3(256), 0, stream_, aob, rho1, S, fxc_eff,
-                      w_.as<double>(), gb, nocc, nprop, ng_, g0);
-    if (bf) back_bf16(nc, g0, gb, static_cast<float*>(S), ncol, out, alpha, accumulate || g0 > 0);
-    else back_contract(ao_.as<double>(), ng_, nao_, g0, gb, nc, static_cast<double*>(S), ncol, out, alpha,
-                       accumulate || g0 > 0, prec, stream_, ws_);
+    if (bf) {
+      unsigned short *h, *l;
+      s_tiled(nc, gb, ncol, sh_, sl_, &h, &l);
+      HH_DEN_DISPATCH(den, wv_s_tiled_kernel, dim3(nblk(gb * nprop)), dim3(256), 0, stream_, aob, rho1, h, l,
+                      ceil_div(gb, 32) * 32 * ncol, fxc_eff, w_.as<double>(), gb, nocc, nprop, ng_, g0);
+      back_bf16(nc, g0, gb, h, l, ncol, out, alpha, accumulate || g0 > 0);
+    } else {
+      HH_DEN_DISPATCH_T(den, false, wv_s_kernel, dim3(nblk(gb * nprop)), dim3(256), 0, stream_, aob, rho1, S, fxc_eff,
+                        w_.as<double>(), gb, nocc, nprop, ng_, g0);
+      back_contract(ao_.as<double>(), ng_, nao_, g0, gb, nc, static_cast<double*>(S), ncol, out, alpha,
+                    accumulate || g0 > 0, prec, stream_, ws_);
+    }
   }
 }
 
@@ -538,12 +666,20 @@ void Numint::rks_response_bra(int den, const double* fxc_eff, const double* mo1_
     if (bf) kets_bf16(nc, g0, gb, kh_.as<unsigned short>(), kl_.as<unsigned short>(), ncol, static_cast<float*>(akb));
     else half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, mo1_bra, ncol, static_cast<double*>(akb), prec,
                         stream_, ws_);
-    HH_DEN_DISPATCH_T(den, bf, rho1_wv_s_kernel, dim3(nblk(gb * nprop)), dim3(256), 0, stream_, aob, akb, fxc_eff,
-                      w_.as<double>(), gb, nocc, nprop, ng_, g0);
     // the x4 closed-shell factor of hess_rks.rs:603
-    if (bf) back_bf16(nc, g0, gb, static_cast<float*>(akb), ncol, out, 4.0, accumulate || g0 > 0);
-    else back_contract(ao_.as<double>(), ng_, nao_, g0, gb, nc, static_cast<double*>(akb), ncol, out, 4.0,
-                       accumulate || g0 > 0, prec, stream_, ws_);
+    if (bf) {
+      unsigned short *h, *l;
+      s_tiled(nc, gb, ncol, sh_, sl_, &h, &l);
+      HH_DEN_DISPATCH(den, rho1_wv_s_tiled_kernel, dim3(nblk(gb * nprop)), dim3(256), 0, stream_, aob,
+                      static_cast<const float*>(akb), h, l, ceil_div(gb, 32) * 32 * ncol, fxc_eff, w_.as<double>(),
+                      gb, nocc, nprop, ng_, g0);
+      back_bf16(nc, g0, gb, h, l, ncol, out, 4.0, accumulate || g0 > 0);
+    } else {
+      HH_DEN_DISPATCH_T(den, false, rho1_wv_s_kernel, dim3(nblk(gb * nprop)), dim3(256), 0, stream_, aob, akb,
+                        fxc_eff, w_.as<double>(), gb, nocc, nprop, ng_, g0);
+      back_contract(ao_.as<double>(), ng_, nao_, g0, gb, nc, static_cast<double*>(akb), ncol, out, 4.0,
+                    accumulate || g0 > 0, prec, stream_, ws_);
+    }
   }
 }
 
@@ -559,19 +695,26 @@ void Numint::uks_fxc_pot_bra_trans(int den, const double* fxc_eff, const double*
   const size_t es = bf ? sizeof(float) : sizeof(double);
   double* aob_a = static_cast<double*>(aob_.reserve(sizeof(double) * gbmax * nocc[0] * nc));
   double* aob_b = static_cast<double*>(aob2_.reserve(sizeof(double) * gbmax * nocc[1] * nc));
-  void* Sa = akb_.reserve(es * gbmax * ncol_a * nc);
-  void* Sb = akb2_.reserve(es * gbmax * ncol_b * nc);
+  void* Sa = bf ? nullptr : akb_.reserve(es * gbmax * ncol_a * nc);
+  void* Sb = bf ? nullptr : akb2_.reserve(es * gbmax * ncol_b * nc);
   if (bf) ensure_bf16();
   for (int64_t g0 = 0; g0 < ng_; g0 += gbmax) {
     const int64_t gb = std::min(gbmax, ng_ - g0);
     half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, bra[0], nocc[0], aob_a, PREC_F64, stream_, ws_);
     half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, bra[1], nocc[1], aob_b, PREC_F64, stream_, ws_);
-    HH_DEN_DISPATCH_T(den, bf, uks_wv_s_kernel, dim3(nblk(gb * nprop)), dim3(256), 0, stream_, aob_a, aob_b, rho1, Sa,
-                      Sb, fxc_eff, w_.as<double>(), gb, nocc[0], nocc[1], nprop, ng_, g0);
     if (bf) {
-      back_bf16(nc, g0, gb, static_cast<float*>(Sa), ncol_a, out[0], alpha, accumulate || g0 > 0);
-      back_bf16(nc, g0, gb, static_cast<float*>(Sb), ncol_b, out[1], alpha, accumulate || g0 > 0);
+      unsigned short *ha, *la, *hb, *lb;
+      s_tiled(nc, gb, ncol_a, sh_, sl_, &ha, &la);
+      s_tiled(nc, gb, ncol_b, sh2_, sl2_, &hb, &lb);
+      const int64_t gbp = ceil_div(gb, 32) * 32;
+      HH_DEN_DISPATCH(den, uks_wv_s_tiled_kernel, dim3(nblk(gb * nprop)), dim3(256), 0, stream_, aob_a, aob_b, rho1, ha,
+                      la, gbp * ncol_a, hb, lb, gbp * ncol_b, fxc_eff, w_.as<double>(), gb, nocc[0], nocc[1], nprop,
+                      ng_, g0);
+      back_bf16(nc, g0, gb, ha, la, ncol_a, out[0], alpha, accumulate || g0 > 0);
+      back_bf16(nc, g0, gb, hb, lb, ncol_b, out[1], alpha, accumulate || g0 > 0);
     } else {
+      HH_DEN_DISPATCH_T(den, false, uks_wv_s_kernel, dim3(nblk(gb * nprop)), dim3(256), 0, stream_, aob_a, aob_b, rho1,
+                        Sa, Sb, fxc_eff, w_.as<double>(), gb, nocc[0], nocc[1], nprop, ng_, g0);
       back_contract(ao_.as<double>(), ng_, nao_, g0, gb, nc, static_cast<double*>(Sa), ncol_a, out[0], alpha,
                     accumulate || g0 > 0, prec, stream_, ws_);
       back_contract(ao_.as<double>(), ng_, nao_, g0, gb, nc, static_cast<double*>(Sb), ncol_b, out[1], alpha,
@@ -612,13 +755,21 @@ void Numint::uks_response_bra(int den, const double* fxc_eff, const double* cons
       half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, mo1_bra[1], ncol_b, static_cast<double*>(akb_b), prec,
                      stream_, ws_);
     }
-    HH_DEN_DISPATCH_T(den, bf, uks_rho1_wv_s_kernel, dim3(nblk(gb * nprop)), dim3(256), 0, stream_, aob_a, akb_a,
-                      aob_b, akb_b, fxc_eff, w_.as<double>(), gb, nocc[0], nocc[1], nprop, ng_, g0);
     // the x2 factor of hess_uks.rs:436-439
     if (bf) {
-      back_bf16(nc, g0, gb, static_cast<float*>(akb_a), ncol_a, out[0], 2.0, accumulate || g0 > 0);
-      back_bf16(nc, g0, gb, static_cast<float*>(akb_b), ncol_b, out[1], 2.0, accumulate || g0 > 0);
+      unsigned short *ha, *la, *hb, *lb;
+      s_tiled(nc, gb, ncol_a, sh_, sl_, &ha, &la);
+      s_tiled(nc, gb, ncol_b, sh2_, sl2_, &hb, &lb);
+      const int64_t gbp = ceil_div(gb, 32) * 32;
+      HH_DEN_DISPATCH(den, uks_rho1_wv_s_tiled_kernel, dim3(nblk(gb * nprop)), dim3(256), 0, stream_, aob_a,
+                      static_cast<const float*>(akb_a), aob_b, static_cast<const float*>(akb_b), ha, la,
+                      gbp * ncol_a, hb, lb, gbp * ncol_b, fxc_eff, w_.as<double>(), gb, nocc[0], nocc[1], nprop,
+                      ng_, g0);
+      back_bf16(nc, g0, gb, ha, la, ncol_a, out[0], 2.0, accumulate || g0 > 0);
+      back_bf16(nc, g0, gb, hb, lb, ncol_b, out[1], 2.0, accumulate || g0 > 0);
     } else {
+      HH_DEN_DISPATCH_T(den, false, uks_rho1_wv_s_kernel, dim3(nblk(gb * nprop)), dim3(256), 0, stream_, aob_a, akb_a,
+                        aob_b, akb_b, fxc_eff, w_.as<double>(), gb, nocc[0], nocc[1], nprop, ng_, g0);
       back_contract(ao_.as<double>(), ng_, nao_, g0, gb, nc, static_cast<double*>(akb_a), ncol_a, out[0], 2.0,
                     accumulate || g0 > 0, prec, stream_, ws_);
       back_contract(ao_.as<double>(), ng_, nao_, g0, gb, nc, static_cast<double*>(akb_b), ncol_b, out[1], 2.0,

@@ -73,14 +73,15 @@ class Numint {
   void split_kets(const double* B, int64_t ncol, DevBuf& hi, DevBuf& lo);
   void kets_bf16(int nc, int64_t g0, int64_t gb, const unsigned short* bh, const unsigned short* bl, int64_t ncol,
                  float* out);
-  void back_bf16(int nc, int64_t g0, int64_t gb, const float* S, int64_t ncol, double* out, double alpha,
-                 bool accumulate);
+  void s_tiled(int nc, int64_t gb, int64_t ncol, DevBuf& hi, DevBuf& lo, unsigned short** h, unsigned short** l);
+  void back_bf16(int nc, int64_t g0, int64_t gb, const unsigned short* sh, const unsigned short* sl, int64_t ncol,
+                 double* out, double alpha, bool accumulate);
   int64_t ng_, nao_, ncomp_;
   hipStream_t stream_;
   DevBuf ao_, w_, aob_, akb_, aob2_, akb2_, rho_, ws_;
   bool bf_ready_ = false;
   int64_t ngp_ = 0, naop_ = 0;
-  DevBuf aok_hi_, aok_lo_, aot_hi_, aot_lo_, kh_, kl_, kh2_, kl2_, sh_, sl_;
+  DevBuf aok_hi_, aok_lo_, aot_hi_, aot_lo_, kh_, kl_, kh2_, kl2_, sh_, sl_, sh2_, sl2_;
 };
 
 // LibXC-variable derivatives -> rho_t-based effective kernels, unpolarised (reference:

@@ -106,12 +106,21 @@ __global__ void xj_ip2_kernel(const double* Y2, int64_t n, int64_t nb, const int
   XJ[(p0 + P) + m * (t + 3 * A)] -= acc;
 }
 
-// XK[p0 + P, i, j, t + 3A] -= HA[i, j, P, t] + HA[j, i, P, t]
-__global__ void xk_ip1_kernel(const double* HA, int64_t no, int64_t nb, int64_t A, double* XK, int64_t m, int64_t p0) {
+// Cm[u, i + no A] = C[u, i] for AOs u of atom A, else 0 (the per-atom restriction as one GEMM)
+__global__ void mask_cols_kernel(const double* C, const int64_t* aoatom, int64_t n, int64_t no, int64_t N, double* Cm) {
   const int64_t e = ELEM;
-  if (e >= nb * no * no * 3) return;
-  const int64_t P = e % nb, r = e / nb, i = r % no, j = (r / no) % no, t = r / (no * no);
-  const double v = HA[i + no * (j + no * (P + nb * t))] + HA[j + no * (i + no * (P + nb * t))];
+  if (e >= n * no * N) return;
+  const int64_t u = e % n, r = e / n, i = r % no, A = r / no;
+  Cm[e] = aoatom[u] == A ? C[u + n * i] : 0.0;
+}
+
+// XK[p0 + P, i, j, t + 3A] -= HA[(i, A), j, P, t] + HA[(j, A), i, P, t]   (HA rows i + no A, ld no N)
+__global__ void xk_ip1_kernel(const double* HA, int64_t no, int64_t nb, int64_t N, double* XK, int64_t m, int64_t p0) {
+  const int64_t e = ELEM;
+  if (e >= nb * no * no * 3 * N) return;
+  const int64_t P = e % nb, r = e / nb, i = r % no, j = (r / no) % no, r2 = r / (no * no), t = r2 % 3, A = r2 / 3;
+  const int64_t ld = no * N;
+  const double v = HA[(i + no * A) + ld * (j + no * (P + nb * t))] + HA[(j + no * A) + ld * (i + no * (P + nb * t))];
   XK[(p0 + P) + m * (i + no * (j + no * (t + 3 * A)))] -= v;
 }
 
@@ -453,7 +462,15 @@ void rijk_skeleton(Rijk& r, const SkeletonInput& in, double* de, double* const* 
   double* IT = D(bIT, n * n * nbmax * 3);
   double* Yb = D(bY, n * nbmax * 9);
   double* Gs = D(bGs, n * omax * nbmax * 3);
-  double* HA = D(bHA, omax * omax * nbmax * 3);
+  double* HA = D(bHA, omax * omax * nbmax * 3 * N);
+  double* Cm[2] = {nullptr, nullptr};
+  DevBuf bCm[2];
+  if (doK)
+    for (int s = 0; s < ns; ++s) {
+      Cm[s] = D(bCm[s], n * no[s] * N);
+      hipLaunchKernelGGL(mask_cols_kernel, GRID(n * no[s] * N), C[s], d_aoatom, n, no[s], N, Cm[s]);
+      launch_check();
+    }
   double* M = D(bM, n * n * nbmax);
   double* Mp = D(bMp, n * n * nbmax);
   double* Mpp = D(bMpp, n * n * nbmax);
@@ -494,13 +511,10 @@ void rijk_skeleton(Rijk& r, const SkeletonInput& in, double* de, double* const* 
         const int64_t o = no[s];
         // G[u, j, P, t] = sum_v ip1[u, v, P, t] C[v, j]
         gm(n, o, n, I, n, false, C[s], n, true, Gs, n, 1.0, 0.0, st, ws, nb * 3, n * n, 0, n * o);
-        for (int64_t A = 0; A < N; ++A) {
-          const int64_t a0 = in.aoslice[2 * A], na = in.aoslice[2 * A + 1] - a0;
-          if (na == 0) continue;
-          gm(o, o * nb * 3, na, C[s] + a0, n, true, Gs + a0, n, true, HA, o, 1.0, 0.0, st, ws);
-          hipLaunchKernelGGL(xk_ip1_kernel, GRID(nb * o * o * 3), HA, o, nb, A, XK[s], m, p0);
-          launch_check();
-        }
+        // HA[(i, A), (j, P, t)] = sum_{u in A} C[u, i] G[u, j, P, t], all atoms in one product
+        gm(o * N, o * nb * 3, n, Cm[s], n, true, Gs, n, true, HA, o * N, 1.0, 0.0, st, ws);
+        hipLaunchKernelGGL(xk_ip1_kernel, GRID(nb * o * o * 3 * N), HA, o, nb, N, XK[s], m, p0);
+        launch_check();
         if (deriv1_bra) {
           // M'[k, P, v] = sum_j C[k, j] T~[v, j, p0 + P]  and its [P, k, v] copy
           gm(n, n, o, C[s], n, false, Tt[s] + n * o * p0, n, false, Mp, n * nb, 1.0, 0.0, st, ws, nb, 0, n * o, n);
@@ -511,9 +525,8 @@ void rijk_skeleton(Rijk& r, const SkeletonInput& in, double* de, double* const* 
           // k1acc[u, v, t, A] -= sum_{k in A, P} ip1[k, u, P, t] M'[k, P, v]
           for (int64_t A = 0; A < N; ++A) {
             const int64_t a0 = in.aoslice[2 * A], na = in.aoslice[2 * A + 1] - a0;
-            for (int t = 0; t < 3; ++t)
-              gm(n, n, nb * na, IT + n * n * nb * t + nb * a0, nb * n, true, Mpp + nb * a0, nb * n, true,
-                 k1acc[s] + n * n * (t + 3 * A), n, -1.0, 1.0, st, ws);
+            gm(n, n, nb * na, IT + nb * a0, nb * n, true, Mpp + nb * a0, nb * n, true, k1acc[s] + n * n * 3 * A, n,
+               -1.0, 1.0, st, ws, 3, n * n * nb, 0, n * n);
           }
         }
       }
@@ -530,12 +543,9 @@ void rijk_skeleton(Rijk& r, const SkeletonInput& in, double* de, double* const* 
     for (int64_t A = 0; A < N; ++A) {
       const int64_t q0 = std::max(p0, in.auxslice[2 * A]), q1 = std::min(p1, in.auxslice[2 * A + 1]);
       if (q1 <= q0) continue;
-      for (int t = 0; t < 3; ++t) {
-        const double* Ib = I + n * n * ((q0 - p0) + nb * t);
-        if (doJ && deriv1_bra)  // j1acc[u, v, t, A] -= sum_{P in A} ip2[u, v, P, t] d_P
-          gm(n * n, 1, q1 - q0, Ib, n * n, false, d + q0, q1 - q0, true, j1acc + n * n * (t + 3 * A), n * n, -1.0, 1.0,
-             st, ws);
-      }
+      if (doJ && deriv1_bra)  // j1acc[u, v, t, A] -= sum_{P in A} ip2[u, v, P, t] d_P
+        gm(n * n, 1, q1 - q0, I + n * n * (q0 - p0), n * n, false, d + q0, q1 - q0, true, j1acc + n * n * 3 * A,
+           n * n, -1.0, 1.0, st, ws, 3, n * n * nb, 0, n * n);
     }
     if (doK)
       for (int s = 0; s < ns; ++s) {
@@ -549,9 +559,8 @@ void rijk_skeleton(Rijk& r, const SkeletonInput& in, double* de, double* const* 
           for (int64_t A = 0; A < N; ++A) {
             const int64_t q0 = std::max(p0, in.auxslice[2 * A]), q1 = std::min(p1, in.auxslice[2 * A + 1]);
             if (q1 <= q0) continue;
-            for (int t = 0; t < 3; ++t)
-              gm(n, n, n * (q1 - q0), I + n * n * ((q0 - p0) + nb * t), n, false, Mp + n * (q0 - p0), n * nb, true,
-                 k1acc[s] + n * n * (t + 3 * A), n, -1.0, 1.0, st, ws);
+            gm(n, n, n * (q1 - q0), I + n * n * (q0 - p0), n, false, Mp + n * (q0 - p0), n * nb, true,
+               k1acc[s] + n * n * 3 * A, n, -1.0, 1.0, st, ws, 3, n * n * nb, 0, n * n);
           }
         }
       }
@@ -614,9 +623,8 @@ void rijk_skeleton(Rijk& r, const SkeletonInput& in, double* de, double* const* 
         // S[u, v, t, A] = sum_{j, P in A} T~[u, j, P] g[v, j, P, t], added to the accumulator
         for (int64_t A = 0; A < N; ++A) {
           const int64_t q0 = in.auxslice[2 * A], mq = in.auxslice[2 * A + 1] - q0;
-          for (int t = 0; t < 3; ++t)
-            gm(n, n, o * mq, Tt[s] + n * o * q0, n, false, g[s] + n * o * (q0 + m * t), n, false,
-               k1acc[s] + n * n * (t + 3 * A), n, 1.0, 1.0, st, ws);
+          gm(n, n, o * mq, Tt[s] + n * o * q0, n, false, g[s] + n * o * q0, n, false, k1acc[s] + n * n * 3 * A, n,
+             1.0, 1.0, st, ws, 3, 0, n * o * m, n * n);
         }
         hipLaunchKernelGGL(k1_final_kernel, GRID(n * n * nx), k1acc[s], r1[s], d_aoatom, n, N, F1);
         launch_check();
