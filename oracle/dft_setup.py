@@ -90,9 +90,8 @@ def get_de_fxc(wf, drho):
     return np.einsum("gxy,gxta,gysb->tsab", wf, drho, drho)
 
 
-def get_de_vxc_diag(den, ao, aod, wv, aoslices):
-    """hess_rks.rs:302-352."""
-    natm = len(aoslices)
+def make_dao_vxc_diag(den, ao, aod, wv):
+    """hess_rks.rs:302-339 (hess_rks_becke.rs:321-369): the per-AO kernel [nao, 6] over PAIRS6."""
     nao = ao.shape[1]
     dao = np.zeros((nao, 6))
     aow = aod[:, :, 0] * wv[:, 0:1]
@@ -110,6 +109,13 @@ def get_de_vxc_diag(den, ao, aod, wv, aoslices):
             aw = aod[:, :, r + 1] * wv[:, 4:5]
             for k, (t, s) in enumerate(PAIRS6):
                 dao[:, k] += np.sum(ao[:, :, comp(r, t, s)] * aw, axis=0)
+    return dao
+
+
+def get_de_vxc_diag(den, ao, aod, wv, aoslices):
+    """hess_rks.rs:302-352."""
+    natm = len(aoslices)
+    dao = make_dao_vxc_diag(den, ao, aod, wv)
     de6 = np.zeros((6, natm, natm))
     for A, (p0, p1) in enumerate(aoslices):
         de6[:, A, A] = dao[p0:p1].sum(axis=0)

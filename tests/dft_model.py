@@ -34,9 +34,9 @@ class Model:
         self.aoslices = [(int(off[A]), int(off[A + 1])) for A in range(natm)]
 
     # -- AO values and r-derivatives to third order, [ng, nao, 20]; differentiable in R --
-    def ao(self, R):
+    def ao(self, R, grid=None):
         R = torch.as_tensor(R, dtype=torch.float64)
-        g = torch.as_tensor(self.grid)
+        g = torch.as_tensor(self.grid) if grid is None else grid
         al = torch.as_tensor(self.alpha)
         a0 = torch.as_tensor(self.a0)
         a = torch.as_tensor(self.a)
