@@ -204,6 +204,49 @@ hh_status hh_numint_hessian_setup(hh_numint* h, int32_t den_type, int32_t nspin,
                                   double* de_fxc, double* const* de_vxc_diag, double* const* de_vxc_off,
                                   double* const* vmat_ip, double* const* vmat_deriv1);
 
+/* ---------------------------------------------------------------------------------------- */
+/* Becke partition and the grid-shift setup (src/numint_matmul/becke_partition.rs,           */
+/* hess_rks_becke.rs, hess_uks_becke.rs)                                                     */
+
+typedef struct hh_becke hh_becke;
+/* Molecular tables (BeckeMolTables::new, becke_partition.rs:163-205): atm_coords HOST [natm, 3];
+ * adjustment HOST [natm, natm] row-major, entry (A, B) = adjustment_factor[A][B] (antisymmetric;
+ * Becke 1988 eqs. A2, A5).  Work is enqueued on `stream`. */
+hh_status hh_becke_create(int64_t natm, const double* atm_coords, const double* adjustment, void* stream,
+                          hh_becke** out);
+hh_status hh_becke_destroy(hh_becke* b);
+/* becke_partition_with_tables (becke_partition.rs:259-340).  grid_coords DEVICE [ngrids, 3];
+ * quadrature_weights DEVICE [ngrids].  by_atom != 0: atm_indices HOST [natm + 1] cumulative grid
+ * boundaries (AtmIndices::ByAtom; must start at 0, end at ngrids, be non-decreasing); else HOST
+ * [ngrids] generating atoms (ByGrid; values >= natm are padding and carry no weight).  deriv 0..2.
+ * Outputs (DEVICE, NULL = not produced; derivative outputs above `deriv` are ignored):
+ * w [ngrids]; dw [natm, 3, ngrids]; ddw [natm, 3, natm, 3, ngrids] (C order); contractions with
+ * contract_x [nset, ngrids]: c [nset_w], dc [natm, 3, nset_dw], ddc [natm, 3, natm, 3, nset_ddw].
+ * Second derivatives support at most 48 atoms (per-point atom-pair tables in LDS). */
+hh_status hh_becke_partition(hh_becke* b, const double* grid_coords, int64_t ngrids, int32_t by_atom,
+                             const int64_t* atm_indices, const double* quadrature_weights, int32_t hardness,
+                             int32_t deriv, double* w, double* dw, double* ddw, const double* contract_w,
+                             int64_t nset_w, double* c, const double* contract_dw, int64_t nset_dw, double* dc,
+                             const double* contract_ddw, int64_t nset_ddw, double* ddc);
+/* The setup with the Becke grid shift (make_hessian_setup_becke, hess_rks_becke.rs:1106-1266;
+ * make_hessian_setup_becke_uks, hess_uks_becke.rs:726-930), as consumed by
+ * RHessKSNIMatmulBecke::make_skeleton_hess / get_deriv1_ao (hess_rks_becke.rs:1387-1401): the
+ * handle's grid is grouped by generating atom (atm_quad_split HOST [natm + 1]) and its weights are
+ * the partitioned ones; grid_coords DEVICE [ng, 3]; quadrature_weights DEVICE [ng] (before the
+ * partition); exc DEVICE [ng] the energy per particle (energy density exc * sum_s rho_s[0]);
+ * vxc_eff / fxc_eff as hh_numint_hessian_setup.  The full atom list is required (aoslice covers
+ * every AO; the Becke tables hold the same natm atoms and use the handle's stream).
+ * Outputs: de_xc_skeleton [3, 3, natm, natm] and per spin vmat_deriv1_grid [nao, nao, 3, natm]
+ * (both translation invariant); optional grid-fixed keys (NULL array or entry = skipped):
+ * de_fxc, de_vxc_diag, de_vxc_off, vmat_ip, vmat_deriv1 as hh_numint_hessian_setup. */
+hh_status hh_numint_hessian_setup_becke(hh_numint* h, int32_t den_type, int32_t nspin, const double* const* dm0,
+                                        int64_t natm, const int64_t* aoslice, const double* exc, const double* vxc_eff,
+                                        const double* fxc_eff, hh_becke* becke, const double* grid_coords,
+                                        const double* quadrature_weights, const int64_t* atm_quad_split,
+                                        int32_t hardness, double* de_xc_skeleton, double* const* vmat_deriv1_grid,
+                                        double* de_fxc, double* const* de_vxc_diag, double* const* de_vxc_off,
+                                        double* const* vmat_ip, double* const* vmat_deriv1);
+
 /* XC effective kernels from LibXC-variable derivatives, unpolarised (transform_xc_inner,
  * xc_deriv.rs:305-428): rho [ng, >= nvar] (ld_rho), xc_val [ng, ncol] (ld_xc) with columns
  * [exc | 1st derivatives | unique 2nd derivatives]; order 1 -> out [ng, nvar] (vxc_eff),

@@ -1,4 +1,5 @@
 // extern "C" boundary of libhesship: argument checks, exception -> status translation.
+#include "becke.hpp"
 #include "elementwise.hpp"
 #include "gemm.hpp"
 #include "cphf.hpp"
@@ -93,6 +94,10 @@ void fill_stats(hh_krylov_stats* out, const hh::KrylovStats& k, int refinements,
 
 struct hh_numint {
   std::unique_ptr<hh::Numint> impl;
+};
+
+struct hh_becke {
+  std::unique_ptr<hh::Becke> impl;
 };
 
 struct hh_mlp {
@@ -458,6 +463,83 @@ hh_status hh_numint_hessian_setup(hh_numint* h, int32_t den_type, int32_t nspin,
     o.vmat_deriv1[s] = vmat_deriv1[s];
   }
   h->impl->hessian_setup(den_type, nspin, dm0, natm, aoslice, vxc_eff, fxc_eff, o);
+  HH_ABI_END
+}
+
+hh_status hh_numint_hessian_setup_becke(hh_numint* h, int32_t den_type, int32_t nspin, const double* const* dm0,
+                                        int64_t natm, const int64_t* aoslice, const double* exc, const double* vxc_eff,
+                                        const double* fxc_eff, hh_becke* becke, const double* grid_coords,
+                                        const double* quadrature_weights, const int64_t* atm_quad_split,
+                                        int32_t hardness, double* de_xc_skeleton, double* const* vmat_deriv1_grid,
+                                        double* de_fxc, double* const* de_vxc_diag, double* const* de_vxc_off,
+                                        double* const* vmat_ip, double* const* vmat_deriv1) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && dm0 && aoslice && exc && vxc_eff && fxc_eff && becke && grid_coords && quadrature_weights &&
+                 atm_quad_split && de_xc_skeleton && vmat_deriv1_grid && (nspin == 1 || nspin == 2) && natm > 0,
+             "null argument, nspin not 1 / 2 or no atoms");
+  Numint::SetupOut o;
+  o.de_fxc = de_fxc;
+  Numint::BeckeOut bo;
+  bo.de_xc_skeleton = de_xc_skeleton;
+  for (int s = 0; s < nspin; ++s) {
+    o.de_vxc_diag[s] = de_vxc_diag ? de_vxc_diag[s] : nullptr;
+    o.de_vxc_off[s] = de_vxc_off ? de_vxc_off[s] : nullptr;
+    o.vmat_ip[s] = vmat_ip ? vmat_ip[s] : nullptr;
+    o.vmat_deriv1[s] = vmat_deriv1 ? vmat_deriv1[s] : nullptr;
+    bo.vmat_deriv1_grid[s] = vmat_deriv1_grid[s];
+  }
+  Numint::BeckeIn bi;
+  bi.becke = becke->impl.get();
+  bi.coords = grid_coords;
+  bi.wquad = quadrature_weights;
+  bi.split = atm_quad_split;
+  bi.hardness = hardness;
+  bi.exc = exc;
+  h->impl->hessian_setup_becke(den_type, nspin, dm0, natm, aoslice, vxc_eff, fxc_eff, bi, o, bo);
+  HH_ABI_END
+}
+
+hh_status hh_becke_create(int64_t natm, const double* atm_coords, const double* adjustment, void* stream,
+                          hh_becke** out) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(out != nullptr, "out is null");
+  *out = nullptr;
+  auto b = std::make_unique<hh_becke>();
+  b->impl = std::make_unique<Becke>(natm, atm_coords, adjustment, (hipStream_t)stream);
+  *out = b.release();
+  HH_ABI_END
+}
+
+hh_status hh_becke_destroy(hh_becke* b) {
+  HH_ABI_BEGIN
+  if (b) {
+    HH_HIP(hipStreamSynchronize(b->impl->stream()));
+    delete b;
+  }
+  HH_ABI_END
+}
+
+hh_status hh_becke_partition(hh_becke* b, const double* grid_coords, int64_t ngrids, int32_t by_atom,
+                             const int64_t* atm_indices, const double* quadrature_weights, int32_t hardness,
+                             int32_t deriv, double* w, double* dw, double* ddw, const double* contract_w,
+                             int64_t nset_w, double* c, const double* contract_dw, int64_t nset_dw, double* dc,
+                             const double* contract_ddw, int64_t nset_ddw, double* ddc) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(b != nullptr, "handle is null");
+  Becke::Out o;
+  o.w = w;
+  o.dw = dw;
+  o.ddw = ddw;
+  o.contract_w = contract_w;
+  o.nset_w = nset_w;
+  o.c = c;
+  o.contract_dw = contract_dw;
+  o.nset_dw = nset_dw;
+  o.dc = dc;
+  o.contract_ddw = contract_ddw;
+  o.nset_ddw = nset_ddw;
+  o.ddc = ddc;
+  b->impl->partition(grid_coords, ngrids, by_atom != 0, atm_indices, quadrature_weights, hardness, deriv, o);
   HH_ABI_END
 }
 

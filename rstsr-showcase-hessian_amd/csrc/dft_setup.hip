@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "becke.hpp"
 #include "elementwise.hpp"
 #include "numint.hpp"
 
@@ -387,6 +388,149 @@ __global__ void vmat_deriv1_final_kernel(const double* __restrict__ Cfx, const d
   out[e] = v;
 }
 
+// ---- Becke grid shift (hess_rks_becke.rs:916-1266, hess_uks_becke.rs:501-930) ----
+// A point of atom G moves with G: its total derivative is d/dR_A + [A == G] d/dr, and
+// d/dr = -sum_A d/dR_A on the fixed-grid density.  On the batch's drho and W (all points of
+// atom G) that is the row of G replaced by minus the sum of the other rows; one thread per
+// (g, v, s, t).  Element (g, j, t, A) at g + gb (j + nvns (t + 3 A)), j = v + NV s.
+__global__ void shift_fix_kernel(double* __restrict__ drho, double* __restrict__ W, int64_t gb, int64_t nvns,
+                                 int64_t natm, int64_t G) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= gb * nvns * 3) return;
+  const int64_t g = e % gb, j = (e / gb) % nvns, t = e / (gb * nvns);
+  double sd = 0.0, sw = 0.0;
+  for (int64_t A = 0; A < natm; ++A) {
+    if (A == G) continue;
+    const int64_t i = g + gb * (j + nvns * (t + 3 * A));
+    sd += drho[i];
+    sw += W[i];
+  }
+  const int64_t iG = g + gb * (j + nvns * (t + 3 * G));
+  drho[iG] = -sd;
+  W[iG] = -sw;
+}
+
+// Phi = W_t + vxc (x) dw: the field of vmat_fxc + vmat_becke_fxc + vmat_becke_dw
+// (hess_rks_becke.rs:850-908); dw [natm, 3, ng] (A, t, g).
+__global__ void phi_kernel(double* __restrict__ W, const double* __restrict__ vxc, const double* __restrict__ dw,
+                           int64_t ng, int64_t g0, int64_t gb, int64_t nvns, int64_t natm) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= gb * nvns * 3 * natm) return;
+  const int64_t g = e % gb, j = (e / gb) % nvns, tA = e / (gb * nvns);
+  W[e] += vxc[g0 + g + ng * j] * dw[tA * ng + g0 + g];
+}
+
+// c[g] = exc[g] sum_s rho_s[g, 0], the weights of the ddw contraction (hess_rks_becke.rs:976-978)
+__global__ void cddw_kernel(const double* __restrict__ ao, int64_t ng, int64_t nao, int64_t g0, int64_t gb,
+                            const double* __restrict__ aod0, const double* __restrict__ aod1, int nspin,
+                            const double* __restrict__ exc, double* __restrict__ c) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= gb) return;
+  double r = 0.0;
+  for (int64_t mu = 0; mu < nao; ++mu) {
+    const double a = ao[g0 + g + ng * mu];
+    r += a * aod0[g + gb * mu];
+    if (nspin == 2) r += a * aod1[g + gb * mu];
+  }
+  c[g0 + g] = exc[g0 + g] * r;
+}
+
+// out[i] = sum_G in[G n + i]
+__global__ void sum_blocks_kernel(const double* __restrict__ in, int64_t nblk, int64_t n, double* __restrict__ out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v = 0.0;
+  for (int64_t b = 0; b < nblk; ++b) v += in[b * n + i];
+  out[i] = v;
+}
+
+// Per-grid-atom kernel of the vxc grid-shift parts (hess_rks_becke.rs:736-767, spin-summed as
+// hess_uks_becke.rs:357-394): pv[G][u][t][s] = sum_spin dao_diag_G[u][pair(t, s)]
+//   + sum_v (C_G[u + nao t, v + nao s] + C_G[v + nao s, u + nao t]) dm0[u, v].
+// diag [G][split][u][6]; C_G at G * 9 nao^2.
+__global__ void becke_pv_kernel(const double* __restrict__ diag0, const double* __restrict__ diag1,
+                                const double* __restrict__ C0, const double* __restrict__ C1,
+                                const double* __restrict__ D0, const double* __restrict__ D1, int nspin,
+                                int64_t nsplit, int64_t nao, int64_t natm, double* __restrict__ pv) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= 9 * nao * natm) return;
+  const int64_t ts = e % 9, u = (e / 9) % nao, G = e / (9 * nao);
+  const int t = (int)(ts / 3), s = (int)(ts % 3);
+  const int lo = t < s ? t : s, hi = t < s ? s : t;
+  const int k = lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);
+  const int64_t ld = 3 * nao;
+  double v = 0.0;
+  for (int sp = 0; sp < nspin; ++sp) {
+    const double* dg = sp ? diag1 : diag0;
+    for (int64_t p = 0; p < nsplit; ++p) v += dg[((G * nsplit + p) * nao + u) * 6 + k];
+    const double* C = (sp ? C1 : C0) + G * 9 * nao * nao;
+    const double* D = sp ? D1 : D0;
+    for (int64_t w = 0; w < nao; ++w)
+      v += (C[(u + nao * t) + ld * (w + nao * s)] + C[(w + nao * s) + ld * (u + nao * t)]) * D[u + nao * w];
+  }
+  pv[e] = v;
+}
+
+// PA[G][A][t][s] = sum_{u in A} pv[G][u][t][s]
+__global__ void becke_pa_kernel(const double* __restrict__ pv, const int64_t* __restrict__ slices, int64_t nao,
+                                int64_t natm, double* __restrict__ PA) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= 9 * natm * natm) return;
+  const int64_t ts = e % 9, A = (e / 9) % natm, G = e / (9 * natm);
+  double v = 0.0;
+  for (int64_t u = slices[2 * A]; u < slices[2 * A + 1]; ++u) v += pv[(G * nao + u) * 9 + ts];
+  PA[e] = v;
+}
+
+// de_xc_skeleton[t, s, A, B] (hess_rks_becke.rs:1227-1237), a = t + 3 A, b = s + 3 B:
+//   X[a, b] + X[b, a] - Xt[a, b]       de_fxc + de_becke_atom_1/2/3 + de_becke_full_1
+//                                      (X = Phi^T drho_t, Xt = W_t^T drho_t)
+//   + ddc[a n3 + b]                    de_becke_full_2
+//   + Y[t, s, A, B] + Y[s, t, B, A]    de_becke_vxc_diag / off, Y[t, s, A, B] =
+//                                      [A == B] / 2 sum_A' PA[B][A'][t][s] - PA[B][A][t][s]
+//   + sum_spin de_vxc_diag + de_vxc_off
+__global__ void becke_de_final_kernel(const double* __restrict__ X, const double* __restrict__ Xt,
+                                      const double* __restrict__ ddc, const double* __restrict__ PA,
+                                      const double* __restrict__ f0, const double* __restrict__ f1,
+                                      const double* __restrict__ f2, const double* __restrict__ f3, int nfix,
+                                      int64_t natm, double* __restrict__ de) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t n3 = 3 * natm;
+  if (e >= n3 * n3) return;
+  const int64_t t = e % 3, s = (e / 3) % 3, A = (e / 9) % natm, B = e / (9 * natm);
+  const int64_t a = t + 3 * A, b = s + 3 * B;
+  double v = X[a + n3 * b] + X[b + n3 * a] - Xt[a + n3 * b] + ddc[a * n3 + b];
+  double y1 = -PA[(B * natm + A) * 9 + 3 * t + s], y2 = -PA[(A * natm + B) * 9 + 3 * s + t];
+  if (A == B)
+    for (int64_t Ap = 0; Ap < natm; ++Ap) {
+      y1 += 0.5 * PA[(B * natm + Ap) * 9 + 3 * t + s];
+      y2 += 0.5 * PA[(A * natm + Ap) * 9 + 3 * s + t];
+    }
+  v += y1 + y2;
+  const double* f[4] = {f0, f1, f2, f3};
+  for (int i = 0; i < nfix; ++i) v += f[i][e];
+  de[e] = v;
+}
+
+// vmat_deriv1_grid[mu, nu, t, A] (hess_rks_becke.rs:1238): the bra + ket assembly of the Phi
+// field, the basis part -[mu in A] ip - [nu in A] ip^T, and vmat_becke_vxc = ip_A + ip_A^T of the
+// points of atom A (hess_rks_becke.rs:893-895).
+__global__ void vmat_grid_final_kernel(const double* __restrict__ Cfx, const double* __restrict__ Cip,
+                                       const double* __restrict__ CipG, const int64_t* __restrict__ atom_of,
+                                       int64_t nao, int64_t natm, double* __restrict__ out) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= nao * nao * 3 * natm) return;
+  const int64_t mu = e % nao, nu = (e / nao) % nao, tA = e / (nao * nao);
+  const int64_t t = tA % 3, A = tA / 3;
+  const int64_t ld = 3 * nao;
+  double v = Cfx[(mu + nao * tA) + nao * 3 * natm * nu] + Cfx[(nu + nao * tA) + nao * 3 * natm * mu];
+  if (atom_of[mu] == A) v -= Cip[(mu + nao * t) + ld * nu];
+  if (atom_of[nu] == A) v -= Cip[(nu + nao * t) + ld * mu];
+  const double* ipA = CipG + A * 3 * nao * nao;
+  v += ipA[(mu + nao * t) + ld * nu] + ipA[(nu + nao * t) + ld * mu];
+  out[e] = v;
+}
+
 // C[m, n] += alpha sum_k A[k, m] B[k, n] (+ A2/B2 over the second K range), FP64
 void gemm_tn(int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
              int64_t ldc, double alpha, hipStream_t s, DevBuf& ws, int64_t K1 = -1, const double* A2 = nullptr,
@@ -579,6 +723,220 @@ void Numint::hessian_setup(int den, int nspin, const double* const* dm0, int64_t
                        acc_ip(s), datom, nao_, natm, out.vmat_deriv1[s]);
     HH_HIP(hipGetLastError());
   }
+}
+
+void Numint::hessian_setup_becke(int den, int nspin, const double* const* dm0, int64_t natm, const int64_t* aoslice,
+                                 const double* vxc, const double* fxc, const BeckeIn& bin, const SetupOut& out,
+                                 const BeckeOut& bout) {
+  HH_REQUIRE(den >= DEN_RHO && den <= DEN_TAU, "numint: den_type must be RHO, SIGMA or TAU");
+  HH_REQUIRE(nspin == 1 || nspin == 2, "numint: nspin must be 1 or 2");
+  HH_REQUIRE(ncomp_ >= setup_ao_ncomp(den),
+             "numint: the Hessian setup needs AO derivatives to order 2 (RHO, 10 components) or 3 (SIGMA/TAU, 20)");
+  HH_REQUIRE(natm > 0 && aoslice && vxc && fxc && bout.de_xc_skeleton, "numint: null argument or no atoms");
+  HH_REQUIRE(bin.becke && bin.coords && bin.wquad && bin.split && bin.exc, "numint: null Becke argument");
+  HH_REQUIRE(bin.becke->natm() == natm, "numint: the grid shift needs the full atom list (Becke tables of natm atoms)");
+  HH_REQUIRE(bin.becke->stream() == stream_, "numint: the Becke tables must use the grid handle's stream");
+  HH_REQUIRE(bin.split[0] == 0 && bin.split[natm] == ng_, "numint: atom grid boundaries must run from 0 to ngrids");
+  for (int64_t A = 0; A < natm; ++A) HH_REQUIRE(bin.split[A] <= bin.split[A + 1], "numint: atom grid boundaries must be non-decreasing");
+  std::vector<int64_t> sl(aoslice, aoslice + 2 * natm), atom_of(nao_, -1);
+  for (int64_t A = 0; A < natm; ++A) {
+    HH_REQUIRE(sl[2 * A] >= 0 && sl[2 * A] <= sl[2 * A + 1] && sl[2 * A + 1] <= nao_, "numint: AO slice out of range");
+    for (int64_t mu = sl[2 * A]; mu < sl[2 * A + 1]; ++mu) {
+      HH_REQUIRE(atom_of[mu] < 0, "numint: AO slices overlap");
+      atom_of[mu] = A;
+    }
+  }
+  for (int64_t mu = 0; mu < nao_; ++mu) HH_REQUIRE(atom_of[mu] >= 0, "numint: the grid shift needs every AO in an atom slice");
+  bool fixed_vmat = true;
+  for (int s = 0; s < nspin; ++s) {
+    HH_REQUIRE(dm0[s] && bout.vmat_deriv1_grid[s], "numint: null per-spin argument");
+    fixed_vmat = fixed_vmat && out.vmat_deriv1[s];
+  }
+  const int nc = den_ncomp(den), nv = den_nvar(den);
+  const bool tau = den == DEN_TAU;
+  const int nblk_aw = tau ? 4 : 1;
+  const int64_t n3 = 3 * natm, M_fx = nao_ * n3, nvns = (int64_t)nv * nspin;
+
+  const int64_t per_point = nspin * nao_ * nc + 2 * nv * nspin * n3 + 4 * nao_ + (tau ? 21 * nao_ : 0) +
+                            nblk_aw * M_fx;
+  int64_t gbmax = batch > 0 ? batch : (int64_t)(1LL << 30) / (8 * per_point);
+  gbmax = std::min(ng_, std::max<int64_t>(64, (gbmax / 64) * 64));
+
+  // accumulators: [Xf | Xt | X] then per spin [diag_G | C_G | Cip_G | Fgrid | Ffixed]
+  const int64_t nsplit = std::max<int64_t>(1, std::min<int64_t>(64, ceil_div(2048, nao_)));
+  const int64_t n_d = natm * nsplit * nao_ * 6, n_o = natm * 9 * nao_ * nao_, n_i = natm * 3 * nao_ * nao_,
+                n_f = M_fx * nao_, n_ff = fixed_vmat ? M_fx * nao_ : 0;
+  const int64_t n_spin = n_d + n_o + n_i + n_f + n_ff;
+  const int64_t n_acc = 3 * n3 * n3 + nspin * n_spin;
+  double* acc = static_cast<double*>(rho_.reserve(sizeof(double) * n_acc));
+  fill<double>(acc, n_acc, 0.0, stream_);
+  double* acc_xf = acc;
+  double* acc_xt = acc + n3 * n3;
+  double* acc_x = acc + 2 * n3 * n3;
+  auto acc_d = [&](int s) { return acc + 3 * n3 * n3 + s * n_spin; };
+  auto acc_o = [&](int s) { return acc_d(s) + n_d; };
+  auto acc_i = [&](int s) { return acc_o(s) + n_o; };
+  auto acc_f = [&](int s) { return acc_i(s) + n_i; };
+  auto acc_ff = [&](int s) { return acc_f(s) + n_f; };
+
+  // whole-grid buffers: dw, ddw weights c, ddc, C / Cip sums, pv, PA, grid-fixed de scratch
+  const int64_t n_fixde = 9 * natm * natm;
+  const int64_t n_aux = n3 * ng_ + ng_ + n3 * n3 + nspin * 12 * nao_ * nao_ + 9 * natm * nao_ + 9 * natm * natm +
+                        2 * nspin * n_fixde;
+  double* aux = static_cast<double*>(aob_.reserve(sizeof(double) * n_aux));
+  double* dw = aux;
+  double* cw = dw + n3 * ng_;
+  double* ddc = cw + ng_;
+  double* csum = ddc + n3 * n3;                 // per spin [C sum 9 nao^2 | Cip sum 3 nao^2]
+  double* pv = csum + nspin * 12 * nao_ * nao_;
+  double* PA = pv + 9 * natm * nao_;
+  double* fixde = PA + 9 * natm * natm;         // per spin [diag | off]
+  auto c_sum = [&](int s) { return csum + s * 12 * nao_ * nao_; };
+  auto ip_sum = [&](int s) { return c_sum(s) + 9 * nao_ * nao_; };
+
+  int64_t* dsl = static_cast<int64_t*>(akb2_.reserve(sizeof(int64_t) * (2 * natm + nao_)));
+  HH_HIP(hipMemcpyAsync(dsl, sl.data(), sizeof(int64_t) * 2 * natm, hipMemcpyHostToDevice, stream_));
+  HH_HIP(hipMemcpyAsync(dsl + 2 * natm, atom_of.data(), sizeof(int64_t) * nao_, hipMemcpyHostToDevice, stream_));
+  const int64_t* datom = dsl + 2 * natm;
+
+  const int64_t n_aod = gbmax * nao_ * nc, n_dr = gbmax * nvns * n3;
+  const int64_t n_scr = nspin * n_aod + 2 * n_dr + 4 * gbmax * nao_ + (tau ? 21 * gbmax * nao_ : 0) +
+                        nblk_aw * gbmax * M_fx;
+  double* scr = static_cast<double*>(akb_.reserve(sizeof(double) * n_scr));
+  double* aod[2] = {scr, scr + n_aod};
+  double* drho = scr + nspin * n_aod;
+  double* W = drho + n_dr;
+  double* aowv = W + n_dr;
+  double* aow = aowv + 3 * gbmax * nao_;
+  double* H2 = aow + gbmax * nao_;
+  double* H2w = H2 + (tau ? 9 * gbmax * nao_ : 0);
+  double* AT = H2w + (tau ? 9 * gbmax * nao_ : 0);
+  double* AW = AT + (tau ? 3 * gbmax * nao_ : 0);
+
+  // Becke weight derivatives dw [natm, 3, ng] of the atom-grouped grid
+  {
+    Becke::Out bo;
+    bo.dw = dw;
+    bin.becke->partition(bin.coords, ng_, true, bin.split, bin.wquad, bin.hardness, 1, bo);
+  }
+
+  const double* ao = ao_.as<double>();
+  const double* w = w_.as<double>();
+  const double fo = den == DEN_RHO ? 1.0 : 2.0;
+  for (int64_t G = 0; G < natm; ++G)
+    for (int64_t g0 = bin.split[G]; g0 < bin.split[G + 1]; g0 += gbmax) {
+      const int64_t gb = std::min(gbmax, bin.split[G + 1] - g0);
+      for (int s = 0; s < nspin; ++s) ao_dm0(ao, ng_, nao_, g0, gb, nc, dm0[s], aod[s], stream_, ws_);
+      HH_SETUP_DISPATCH_NS(den, nspin, drho_w_kernel, dim3(nblocks(gb * natm)), dim3(256), 0, stream_, ao, ng_, nao_,
+                           g0, gb, aod[0], aod[nspin - 1], dsl, natm, fxc, w, drho, W);
+      const int64_t Kf = gb * nvns;
+      auto fxc_vmat = [&](int s, double* C) {
+        HH_SETUP_DISPATCH_NS(den, nspin, fxc_ops_kernel, dim3(nblocks(gb * nao_)), dim3(256), 0, stream_, ao, ng_,
+                             nao_, g0, gb, W, s, natm, AW);
+        const double* ao0 = ao + g0;
+        if (!tau) {
+          gemm_tn(M_fx, nao_, gb, AW, gb, ao0, ng_, C, M_fx, 1.0, stream_, ws_);
+        } else {
+          const int64_t blk = gb * M_fx;
+          gemm_tn(M_fx, nao_, 2 * gb, AW, gb, ao0, ng_, C, M_fx, 1.0, stream_, ws_, gb, AW + blk, gb,
+                  ao0 + ng_ * nao_, ng_);
+          gemm_tn(M_fx, nao_, 2 * gb, AW + 2 * blk, gb, ao0 + 2 * ng_ * nao_, ng_, C, M_fx, 1.0, stream_, ws_, gb,
+                  AW + 3 * blk, gb, ao0 + 3 * ng_ * nao_, ng_);
+        }
+      };
+      // grid-fixed de_fxc and vmat_deriv1 (fxc part)
+      if (out.de_fxc) gemm_tn(n3, n3, Kf, W, Kf, drho, Kf, acc_xf, n3, 1.0, stream_, ws_);
+      if (fixed_vmat)
+        for (int s = 0; s < nspin; ++s) fxc_vmat(s, acc_ff(s));
+      // the point carried by G: total drho and W, their product, then the Phi field
+      hipLaunchKernelGGL(shift_fix_kernel, dim3(nblocks(gb * nvns * 3)), dim3(256), 0, stream_, drho, W, gb, nvns,
+                         natm, G);
+      HH_HIP(hipGetLastError());
+      gemm_tn(n3, n3, Kf, W, Kf, drho, Kf, acc_xt, n3, 1.0, stream_, ws_);
+      hipLaunchKernelGGL(phi_kernel, dim3(nblocks(gb * nvns * n3)), dim3(256), 0, stream_, W, vxc, dw, ng_, g0, gb,
+                         nvns, natm);
+      HH_HIP(hipGetLastError());
+      gemm_tn(n3, n3, Kf, W, Kf, drho, Kf, acc_x, n3, 1.0, stream_, ws_);
+      hipLaunchKernelGGL(cddw_kernel, dim3(nblocks(gb)), dim3(256), 0, stream_, ao, ng_, nao_, g0, gb, aod[0],
+                         aod[nspin - 1], nspin, bin.exc, cw);
+      HH_HIP(hipGetLastError());
+      for (int s = 0; s < nspin; ++s) {
+        const double* vxc_s = vxc + ng_ * nv * s;
+        HH_SETUP_DISPATCH(den, vxc_diag_kernel, dim3((unsigned)nao_, (unsigned)nsplit), dim3(256), 0, stream_, ao, ng_,
+                          nao_, g0, gb, aod[s], vxc_s, w, acc_d(s) + G * nsplit * nao_ * 6, nsplit);
+        HH_SETUP_DISPATCH(den, vxc_ops_kernel, dim3(nblocks(gb * nao_)), dim3(256), 0, stream_, ao, ng_, nao_, g0, gb,
+                          vxc_s, w, aowv, aow, H2, H2w, AT);
+        const double* ao0 = ao + g0;
+        const double* ao1 = ao + g0 + ng_ * nao_;
+        double* CG = acc_o(s) + G * 9 * nao_ * nao_;
+        double* IG = acc_i(s) + G * 3 * nao_ * nao_;
+        gemm_tn(3 * nao_, 3 * nao_, gb, aowv, gb, ao1, ng_, CG, 3 * nao_, fo, stream_, ws_);
+        if (tau) gemm_tn(3 * nao_, 3 * nao_, 3 * gb, H2w, 3 * gb, H2, 3 * gb, CG, 3 * nao_, 1.0, stream_, ws_);
+        if (den == DEN_RHO)
+          gemm_tn(3 * nao_, nao_, gb, ao1, ng_, aow, gb, IG, 3 * nao_, 1.0, stream_, ws_);
+        else
+          gemm_tn(3 * nao_, nao_, 2 * gb, ao1, ng_, aow, gb, IG, 3 * nao_, 1.0, stream_, ws_, gb, aowv, gb, ao0, ng_);
+        if (tau) gemm_tn(3 * nao_, nao_, 3 * gb, H2, 3 * gb, AT, 3 * gb, IG, 3 * nao_, 1.0, stream_, ws_);
+        fxc_vmat(s, acc_f(s));
+      }
+    }
+
+  // de_becke_full_2: ddw contracted with exc rho[0]
+  {
+    Becke::Out bo;
+    bo.contract_ddw = cw;
+    bo.nset_ddw = 1;
+    bo.ddc = ddc;
+    bin.becke->partition(bin.coords, ng_, true, bin.split, bin.wquad, bin.hardness, 2, bo);
+  }
+
+  // grid-fixed keys
+  if (out.de_fxc) {
+    hipLaunchKernelGGL(de_fxc_final_kernel, dim3(nblocks(n3 * n3)), dim3(256), 0, stream_, acc_xf, natm, out.de_fxc);
+    HH_HIP(hipGetLastError());
+  }
+  const double* fx[4] = {nullptr, nullptr, nullptr, nullptr};
+  for (int s = 0; s < nspin; ++s) {
+    hipLaunchKernelGGL(sum_blocks_kernel, dim3(nblocks(9 * nao_ * nao_)), dim3(256), 0, stream_, acc_o(s), natm,
+                       9 * nao_ * nao_, c_sum(s));
+    HH_HIP(hipGetLastError());
+    hipLaunchKernelGGL(sum_blocks_kernel, dim3(nblocks(3 * nao_ * nao_)), dim3(256), 0, stream_, acc_i(s), natm,
+                       3 * nao_ * nao_, ip_sum(s));
+    HH_HIP(hipGetLastError());
+    double* dd = out.de_vxc_diag[s] ? out.de_vxc_diag[s] : fixde + 2 * s * n_fixde;
+    double* doff = out.de_vxc_off[s] ? out.de_vxc_off[s] : fixde + (2 * s + 1) * n_fixde;
+    hipLaunchKernelGGL(de_diag_final_kernel, dim3(nblocks(9 * natm * natm)), dim3(256), 0, stream_, acc_d(s),
+                       natm * nsplit, nao_, dsl, natm, dd);
+    HH_HIP(hipGetLastError());
+    hipLaunchKernelGGL(de_off_final_kernel, dim3((unsigned)(9 * natm * natm)), dim3(256), 0, stream_, c_sum(s), nao_,
+                       dm0[s], dsl, natm, doff);
+    HH_HIP(hipGetLastError());
+    fx[2 * s] = dd;
+    fx[2 * s + 1] = doff;
+    if (out.vmat_ip[s]) {
+      hipLaunchKernelGGL(vmat_ip_final_kernel, dim3(nblocks(3 * nao_ * nao_)), dim3(256), 0, stream_, ip_sum(s), nao_,
+                         out.vmat_ip[s]);
+      HH_HIP(hipGetLastError());
+    }
+    if (fixed_vmat) {
+      hipLaunchKernelGGL(vmat_deriv1_final_kernel, dim3(nblocks(nao_ * nao_ * n3)), dim3(256), 0, stream_, acc_ff(s),
+                         ip_sum(s), datom, nao_, natm, out.vmat_deriv1[s]);
+      HH_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(vmat_grid_final_kernel, dim3(nblocks(nao_ * nao_ * n3)), dim3(256), 0, stream_, acc_f(s),
+                       ip_sum(s), acc_i(s), datom, nao_, natm, bout.vmat_deriv1_grid[s]);
+    HH_HIP(hipGetLastError());
+  }
+  // grid-shift vxc parts and the assembly
+  hipLaunchKernelGGL(becke_pv_kernel, dim3(nblocks(9 * nao_ * natm)), dim3(256), 0, stream_, acc_d(0),
+                     acc_d(nspin - 1), acc_o(0), acc_o(nspin - 1), dm0[0], dm0[nspin - 1], nspin, nsplit, nao_, natm,
+                     pv);
+  HH_HIP(hipGetLastError());
+  hipLaunchKernelGGL(becke_pa_kernel, dim3(nblocks(9 * natm * natm)), dim3(256), 0, stream_, pv, dsl, nao_, natm, PA);
+  HH_HIP(hipGetLastError());
+  hipLaunchKernelGGL(becke_de_final_kernel, dim3(nblocks(n3 * n3)), dim3(256), 0, stream_, acc_x, acc_xt, ddc, PA,
+                     fx[0], fx[1], fx[2], fx[3], 2 * nspin, natm, bout.de_xc_skeleton);
+  HH_HIP(hipGetLastError());
 }
 
 }  // namespace hh

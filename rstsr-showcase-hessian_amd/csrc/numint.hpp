@@ -7,6 +7,8 @@
 
 namespace hh {
 
+class Becke;
+
 // XCDenType (src/xceff/flags.rs:19-24): RHO 0, SIGMA 1, TAU 2.
 enum DenType : int { DEN_RHO = 0, DEN_SIGMA = 1, DEN_TAU = 2 };
 inline int den_nvar(int d) { return d == DEN_RHO ? 1 : (d == DEN_SIGMA ? 4 : 5); }
@@ -59,6 +61,26 @@ class Numint {
   // fxc [ng, nvar, nspin, nvar, nspin] (the transformed kernels, weights not applied).
   void hessian_setup(int den, int nspin, const double* const* dm0, int64_t natm, const int64_t* aoslice,
                      const double* vxc, const double* fxc, const SetupOut& out);
+
+  // The setup with the Becke grid shift (hess_rks_becke.rs:916-1266, hess_uks_becke.rs:501-930):
+  // the grid is grouped by generating atom (split host [natm + 1]), every point moves with its
+  // atom and its weight w = wquad P_A / Z follows the atoms.  Needs the full atom list.
+  struct BeckeIn {
+    Becke* becke = nullptr;          // tables of the natm atoms, on this handle's stream
+    const double* coords = nullptr;  // device [ng, 3]
+    const double* wquad = nullptr;   // device [ng], before the partition
+    const int64_t* split = nullptr;  // host [natm + 1]
+    int hardness = 3;
+    const double* exc = nullptr;     // device [ng], energy per particle (energy density exc * rho)
+  };
+  struct BeckeOut {
+    double* de_xc_skeleton = nullptr;                  // [3, 3, natm, natm]
+    double* vmat_deriv1_grid[2] = {nullptr, nullptr};  // [nao, nao, 3, natm] per spin
+  };
+  // `out` (the grid-fixed keys) is optional per field.
+  void hessian_setup_becke(int den, int nspin, const double* const* dm0, int64_t natm, const int64_t* aoslice,
+                           const double* vxc, const double* fxc, const BeckeIn& bin, const SetupOut& out,
+                           const BeckeOut& bout);
 
   int64_t ng() const { return ng_; }
   int64_t nao() const { return nao_; }

@@ -3,6 +3,7 @@
 Mirrors the reference's (ajz34/rstsr-showcase-hessian) Rust API names:
   RHessRIJK, UHessRIJK, get_rijk_response_bra_separated     (src/ri_jk)
   krylov_block                                               (src/util/krylov_block.rs)
+  BeckeMolTables.partition                                   (src/numint_matmul/becke_partition.rs)
 Device tensors are torch tensors on a ROCm device, in the reference's logical shapes; the
 binding hands column-major device buffers to the C ABI.
 """
@@ -10,5 +11,6 @@ from ._lib import PREC, HesshipError, KrylovNotConverged, lib  # noqa: F401
 from .tensors import fortran, empty_f  # noqa: F401
 from .krylov import krylov_block, solve, cholesky, trsm_lower  # noqa: F401
 from .cphf import RHessCPHF, UHessCPHF  # noqa: F401
+from .becke import BeckeMolTables  # noqa: F401
 from .numint import NIMatmul, libxc_to_xcfun_mapping, transform_xc, transform_xc_polarized  # noqa: F401
 from .rijk import RHessRIJK, UHessRIJK, get_rijk_response_bra_separated, pack_triu_tilde, unpack_triu  # noqa: F401

@@ -112,6 +112,12 @@ SIGNATURES.update({
     "hh_xceff_libxc_to_xcfun_map": (_i32, [_i32, _i32, _pi64, _i64, _pi64]),
     "hh_numint_uks_fxc_pot_bra_trans": (_i32, [_p, _i32, _p, _p, _i64, _pp, _pi64, _pp, _i32]),
     "hh_numint_uks_response_bra": (_i32, [_p, _i32, _p, _pp, _i64, _pp, _pi64, _pp, _i32]),
+    "hh_becke_create": (_i32, [_i64, _p, _p, _p, C.POINTER(C.c_void_p)]),
+    "hh_becke_destroy": (_i32, [_p]),
+    "hh_becke_partition": (_i32, [_p, _p, _i64, _i32, _pi64, _p, _i32, _i32, _p, _p, _p, _p, _i64, _p, _p, _i64, _p,
+                                  _p, _i64, _p]),
+    "hh_numint_hessian_setup_becke": (_i32, [_p, _i32, _i32, _pp, _i64, _pi64, _p, _p, _p, _p, _p, _p, _pi64, _i32,
+                                             _p, _pp, _p, _pp, _pp, _pp, _pp]),
 })
 
 

@@ -8,6 +8,7 @@ den_type: "rho" | "sigma" | "tau" (XCDenType, src/xceff/flags.rs:19-24).
 import ctypes as C
 
 import numpy as np
+import torch
 
 from ._lib import PREC, check, lib, ptr_array
 from .tensors import empty_f, fortran, stream_of
@@ -140,6 +141,56 @@ class NIMatmul:
                 out[k] = ts[0]
             else:
                 out[k + "_a"], out[k + "_b"] = ts
+        return out
+
+    def make_hessian_setup_becke(self, den_type, dm0, aoslices, exc, vxc_eff, fxc_eff, becke, grid_coords,
+                                 quadrature_weights, atm_quad_split, hardness=3, grid_fixed_keys=True):
+        """make_hessian_setup_becke (hess_rks_becke.rs:1106-1266; UKS hess_uks_becke.rs:726-930):
+        the grid is grouped by generating atom (atm_quad_split [natm + 1]) and moves with the atoms;
+        the handle's weights are the partitioned ones, quadrature_weights the ones before the
+        partition.  exc [ng] is the energy per particle.  Returns de_xc_skeleton and
+        vmat_deriv1_grid (UKS: _a / _b) and, with grid_fixed_keys, the keys of
+        make_hessian_setup and de_xc_skeleton_no_becke.  becke: BeckeMolTables of all atoms."""
+        d = [fortran(x) for x in self._spins(dm0)]
+        ns = len(d)
+        natm = len(aoslices)
+        sl = (C.c_int64 * (2 * natm))(*[int(v) for p in aoslices for v in p])
+        split = np.ascontiguousarray(np.asarray(atm_quad_split, dtype=np.int64))
+        if split.shape != (natm + 1,):
+            raise ValueError("atm_quad_split must have length natm + 1")
+        v, f, e = fortran(vxc_eff), fortran(fxc_eff), fortran(exc)
+        coords = grid_coords.to(device=self.device, dtype=torch.float64).contiguous()
+        wq = fortran(quadrature_weights)
+        nao, dev = self.nao, self.device
+        de = empty_f((3, 3, natm, natm), dev)
+        grid = [empty_f((nao, nao, 3, natm), dev) for _ in range(ns)]
+        per = {}
+        de_fxc = None
+        if grid_fixed_keys:
+            de_fxc = empty_f((3, 3, natm, natm), dev)
+            per = {k: [empty_f(shape, dev) for _ in range(ns)] for k, shape in
+                   [("de_vxc_diag", (3, 3, natm, natm)), ("de_vxc_off", (3, 3, natm, natm)),
+                    ("vmat_ip", (nao, nao, 3)), ("vmat_deriv1", (nao, nao, 3, natm))]}
+        pa = lambda k: ptr_array([t.data_ptr() for t in per[k]]) if k in per else None
+        check(lib().hh_numint_hessian_setup_becke(
+            self._h, DEN[den_type], ns, ptr_array([x.data_ptr() for x in d]), natm, sl, e.data_ptr(), v.data_ptr(),
+            f.data_ptr(), becke._h, coords.data_ptr(), wq.data_ptr(), split.ctypes.data_as(C.POINTER(C.c_int64)),
+            int(hardness), de.data_ptr(), ptr_array([g.data_ptr() for g in grid]),
+            de_fxc.data_ptr() if de_fxc is not None else None, pa("de_vxc_diag"), pa("de_vxc_off"), pa("vmat_ip"),
+            pa("vmat_deriv1")))
+        out = {"de_xc_skeleton": de}
+        if ns == 1:
+            out["vmat_deriv1_grid"] = grid[0]
+        else:
+            out["vmat_deriv1_grid_a"], out["vmat_deriv1_grid_b"] = grid
+        if grid_fixed_keys:
+            out["de_fxc"] = de_fxc
+            for k, ts in per.items():
+                if ns == 1:
+                    out[k] = ts[0]
+                else:
+                    out[k + "_a"], out[k + "_b"] = ts
+            out["de_xc_skeleton_no_becke"] = self.skeleton_hess(out)
         return out
 
     @staticmethod
