@@ -13,8 +13,9 @@ against the FP64 result is reported.
 
 `--workload` selects a measurement of one of the other hot-path pieces instead (one JSON line
 each, same fields): `numint` (RKS grid response at the (H2O)_8 B3LYP sizes on synthetic AO
-values), `dft_setup` (DFT skeleton setup passes at the same sizes, FP64), `skeleton` (RI-JK
-skeleton Hessian and first-derivative terms on synthetic integrals, FP64).
+values), `dft_setup` (DFT skeleton setup passes at the same sizes, FP64), `dft_becke` (the same
+setup with the Becke grid shift on an atom-grouped grid, FP64), `skeleton` (RI-JK skeleton
+Hessian and first-derivative terms on synthetic integrals, FP64).
 """
 import argparse
 import json
@@ -269,6 +270,96 @@ def workload_dft_setup(args):
     return line
 
 
+def becke_geometry(natm, npa, seed):
+    """Atoms in a box with a minimum separation; each atom's points at random directions and radii
+    around it (the layout of an atom-grouped Becke grid); antisymmetric radii adjustments."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    R = []
+    while len(R) < natm:
+        x = rng.uniform(-4.5, 4.5, 3)
+        if all(np.linalg.norm(x - y) > 1.8 for y in R):
+            R.append(x)
+    R = np.array(R)
+    attr = np.repeat(np.arange(natm), npa)
+    u = rng.standard_normal((len(attr), 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    coords = R[attr] + u * np.exp(rng.uniform(np.log(0.05), np.log(6.0), (len(attr), 1)))
+    a = rng.uniform(-0.3, 0.3, (natm, natm))
+    adj = np.triu(a, 1) - np.triu(a, 1).T
+    split = np.concatenate([[0], np.cumsum(np.full(natm, npa) if np.isscalar(npa) else npa)]).astype(np.int64)
+    return R, attr, coords, rng.uniform(1e-3, 0.5, len(attr)), adj, split
+
+
+def becke_flops(ng, nao, natm, nvar=4):
+    """The setup passes (vmat on the combined field once) plus the two grid-shift n3 x n3 products
+    and the rank-natm cross term of the contracted ddw (symmetric half, natm (3 natm)^2 per point)."""
+    n3 = 3 * natm
+    return setup_flops(ng, nao, natm, nvar) + 4.0 * ng * nvar * n3 * n3 + 1.0 * ng * natm * n3 * n3
+
+
+def workload_dft_becke(args):
+    """DFT skeleton setup with the Becke grid shift (make_hessian_setup_becke,
+    hess_rks_becke.rs:1106-1266) at (H2O)_8 sizes, GGA, FP64: de_xc_skeleton and
+    vmat_deriv1_grid, the keys its consumer reads (hess_rks_becke.rs:1387-1401)."""
+    import numpy as np
+    import torch
+    import hesship
+    from hesship.tensors import empty_f
+    from oracle import dft_becke as odb
+    c = C3
+    ng, nao, natm = c["ng"], c["nao"], c["natm"]
+    dev = torch.device("cuda:0")
+    R, attr, coords, wq, adj, split = becke_geometry(natm, ng // natm, seed=8)
+    tab = hesship.BeckeMolTables(R, adj, dev)
+    d_coords = torch.from_numpy(coords).to(dev)
+    d_wq = torch.from_numpy(wq).to(dev)
+    w = tab.partition(d_coords, split, d_wq, deriv=0, by_atom=True)["w"]
+    g = torch.Generator(device=dev).manual_seed(0x5EED0008)
+    ao = empty_f((ng, nao, 20), dev)
+    ao.normal_(generator=g)
+    ao *= 0.3
+    vxc = torch.randn((ng, 4), generator=g, device=dev, dtype=torch.float64) * 0.1
+    f = torch.randn((ng, 4, 4), generator=g, device=dev, dtype=torch.float64) * 0.1
+    f = f + f.transpose(1, 2)
+    exc = torch.randn(ng, generator=g, device=dev, dtype=torch.float64) * 0.1
+    Cm = torch.randn((nao, c["nocc"]), generator=g, device=dev, dtype=torch.float64) * 0.1
+    dm0 = Cm @ Cm.T
+    per = nao // natm
+    slices = [(A * per, (A + 1) * per) for A in range(natm)]
+    ni = hesship.NIMatmul(ao, w)
+    del ao
+    log("[bench] dft setup with the Becke grid shift")
+    run = lambda: ni.make_hessian_setup_becke("sigma", dm0, slices, exc, vxc, f, tab, d_coords, d_wq, split,
+                                              grid_fixed_keys=False)
+    _, ms = _timed(run, args.steps, args.warmup)
+    value = becke_flops(ng, nao, natm) / (ms * 1e-3) / 1e12
+    line = _line("DFT skeleton setup with Becke grid shift TFLOP/s (GGA, FP64)", round(value, 3), "TFLOP/s", ms,
+                 args.steps, args.warmup, "f64",
+                 {"workload": "DFT Hessian setup with the Becke grid shift on synthetic AO values (derivative "
+                              "order 3) and an atom-grouped grid", "ngrid": ng, "nao": nao, "natm": natm,
+                  "den_type": "sigma"},
+                 {"bound": "mfma", "achieved": round(value, 3), "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
+                  "frac": round(value / PEAK_TFLOPS["fp64"], 4), "traffic": None,
+                  "kernel": "whole call (vmat GEMM on the combined field, AO-pair GEMMs per atom, contracted ddw)"})
+    if not args.no_cpu_baseline:
+        nsamp = 600
+        rng = np.random.default_rng(3)
+        sel = slice(0, nsamp)                       # points of atom 0
+        aos = rng.standard_normal((nsamp, nao, 20)) * 0.3
+        t0 = time.perf_counter()
+        odb.make_hessian_setup_chunk_becke("sigma", aos, w[sel].cpu().numpy(), wq[sel], coords[sel], R, adj, 3,
+                                           dm0.cpu().numpy(), slices, 0, rng.standard_normal(nsamp) * 0.1,
+                                           rng.standard_normal((nsamp, 4)) * 0.1,
+                                           rng.standard_normal((nsamp, 4, 4)) * 0.1)
+        dt = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": round(becke_flops(nsamp, nao, natm) / dt / 1e12, 5), "unit": "TFLOP/s",
+                                "cores": _blas_threads(), "kind": "port",
+                                "sample": f"oracle make_hessian_setup_chunk_becke (NumPy FP64) on {nsamp} of {ng} "
+                                          f"grid points (one atom's chunk), {dt:.1f} s"}
+    return line
+
+
 def workload_skeleton(args):
     """RI-JK skeleton (make_skeleton_hess + get_deriv1_bra, hess_r.rs:1858-1909) on synthetic
     integrals at (H2O)_8 def2-SVP / jkfit-like sizes, FP64; the derivative integrals are served
@@ -328,7 +419,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cphf", action="store_true")
-    ap.add_argument("--workload", default="ri_jk", choices=["ri_jk", "numint", "dft_setup", "skeleton"])
+    ap.add_argument("--workload", default="ri_jk", choices=["ri_jk", "numint", "dft_setup", "dft_becke", "skeleton"])
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run the library's 16-byte copy kernel over GIB GiB three times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes) and exit")
@@ -345,7 +436,8 @@ def main():
     from hesship import synth
 
     if args.workload != "ri_jk":
-        fn = {"numint": workload_numint, "dft_setup": workload_dft_setup, "skeleton": workload_skeleton}[args.workload]
+        fn = {"numint": workload_numint, "dft_setup": workload_dft_setup, "dft_becke": workload_dft_becke,
+              "skeleton": workload_skeleton}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
 

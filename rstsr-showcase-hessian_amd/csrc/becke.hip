@@ -50,11 +50,12 @@ __device__ inline Sw switch_eval(double mu, double a, int h) {
 }
 
 // table layout: R [3 natm], adj [natm^2], irab [natm^2] (1 / |R_AB|, 0 on the diagonal),
-// U [natm^2, 3] ((R_A - R_B) / |R_AB|)
+// U [natm^2, 3] ((R_A - R_B) / |R_AB|), rab [natm^2] (|R_AB|)
 struct Tab {
-  const double *R, *adj, *irab, *U;
+  const double *R, *adj, *irab, *U, *rab;
   __device__ Tab(const double* t, int64_t natm)
-      : R(t), adj(t + 3 * natm), irab(t + 3 * natm + natm * natm), U(t + 3 * natm + 2 * natm * natm) {}
+      : R(t), adj(t + 3 * natm), irab(t + 3 * natm + natm * natm), U(t + 3 * natm + 2 * natm * natm),
+        rab(t + 3 * natm + 5 * natm * natm) {}
 };
 
 // generating atom of sorted index i: the last A with bounds[A] <= i
@@ -140,15 +141,16 @@ __global__ void becke_w_dw_kernel(const double* __restrict__ coords, const doubl
 
 // ---- contracted second derivative, one workgroup per chunk of one atom's points ----
 // chunks [nchunk][3] = (first sorted index, end, G); part [nchunk][n3 * n3] receives S.
-// Per-point LDS record (doubles): dist [natm], P [natm], e [natm][3], L [natm][n3], gam [natm],
-// dZ [n3], dq [n3], sc [4] = (Z, q, c~).
+// Per-point LDS record (doubles): dist [natm], 1 / dist [natm], P [natm], e [natm][3],
+// L [natm][n3], gam [natm], dZ [n3], dq [n3], sc [4] = (Z, q, c~).
 struct Rec {
-  int64_t natm, n3, stride;
-  int64_t o_P, o_e, o_L, o_gam, o_dZ, o_dq, o_sc;
-  __host__ __device__ Rec(int64_t natm_) : natm(natm_), n3(3 * natm_) {
-    o_P = natm;
-    o_e = 2 * natm;
-    o_L = 5 * natm;
+  int natm, n3, stride;
+  int o_id, o_P, o_e, o_L, o_gam, o_dZ, o_dq, o_sc;
+  __host__ __device__ Rec(int natm_) : natm(natm_), n3(3 * natm_) {
+    o_id = natm;
+    o_P = 2 * natm;
+    o_e = 3 * natm;
+    o_L = 6 * natm;
     o_gam = o_L + natm * n3;
     o_dZ = o_gam + natm;
     o_dq = o_dZ + n3;
@@ -157,46 +159,56 @@ struct Rec {
   }
 };
 
-// (A, t)-(B, s) pair part of one block from the pair (A, N) at one point, A in the first role:
-// coefficient (gam_A ddl_A + gam_N ddl_N) on d mu / dR_A (x) d mu / dR_X and (gam_A l_A + gam_N l_N)
-// on d2 mu / dR_A dR_X, X = N (off-diagonal block) or X = A (diagonal block);
-// becke_partition.rs:960-1090.
+// Work units of a workgroup: units [0, nblk) own the atom-pair blocks (A <= B): the rank-natm
+// and rank-2 parts and, for A != B, the pair (A, B); units [nblk, nblk + natm * ngrp) carry the
+// pairs (A, N), N in a group of DIAG_GRP atoms, of the diagonal block (A, A) (its natm - 1 pairs
+// would otherwise serialise on one thread).
+constexpr int DIAG_GRP = 3;
+
+// Pair part of one block from the pair (A, N) at one point, A in the first role (the reference
+// pair loop, becke_partition.rs:960-1090): coefficient (gam_A ddl_A + gam_N ddl_N) on
+// d mu / dR_A (x) d mu / dR_X and (gam_A l_A + gam_N l_N) on d2 mu / dR_A dR_X, X = N (off-diagonal
+// block) or X = A (diagonal block).
 template <bool DIAG>
-__device__ inline void pair_block(const Tab& T, int64_t natm, int h, const double* rec, const Rec& L, int64_t A,
-                                  int64_t N, double* a9) {
-  const double* dist = rec;
+__device__ inline void pair_block(const Tab& T, int natm, int h, const double* rec, const Rec& L, int A, int N,
+                                  double* a9) {
   const double* e = rec + L.o_e;
   const double* gam = rec + L.o_gam;
-  const double ir = T.irab[A * natm + N], a = T.adj[A * natm + N];
-  const double dA = dist[A], dN = dist[N];
+  const int an = A * natm + N;
+  const double ir = T.irab[an], a = T.adj[an], gab = T.rab[an];
+  const double dA = rec[A], dN = rec[N];
   const double fab = dA - dN, mu = fab * ir;
   const Sw s = switch_eval<2>(mu, a, h);
-  const double sA = 0.5 * (1.0 - s.f), sN = 0.5 * (1.0 + s.f);
+  const double sAs = fmax(0.5 * (1.0 - s.f), INVTOL), sNs = fmax(0.5 * (1.0 + s.f), INVTOL);
+  const double inv = 1.0 / (sAs * sNs);
+  const double isA = sNs * inv, isN = sAs * inv;
   const double dmu_nu = 1.0 - 2.0 * mu * a;
   const double dsA = -0.5 * s.df * dmu_nu;
-  const double sAs = fmax(sA, INVTOL), sNs = fmax(sN, INVTOL);
-  const double lA = dsA / sAs, lN = -dsA / sNs;
+  const double lA = dsA * isA, lN = -dsA * isN;
   const double ddsA = -0.5 * s.ddf * dmu_nu * dmu_nu + s.df * a;
-  const double ddlA = ddsA / sAs - lA * lA, ddlN = -ddsA / sNs - lN * lN;
+  const double ddlA = ddsA * isA - lA * lA, ddlN = -ddsA * isN - lN * lN;
   const double cdd = gam[A] * ddlA + gam[N] * ddlN;
   const double c1 = gam[A] * lA + gam[N] * lN;
-  const double* U = T.U + 3 * (A * natm + N);
+  const double* U = T.U + 3 * an;
   const double* eA = e + 3 * A;
   const double* eN = e + 3 * N;
   double rA[3], rX[3];
+#pragma unroll
   for (int t = 0; t < 3; ++t) {
     rA[t] = (eA[t] - mu * U[t]) * ir;
     rX[t] = DIAG ? rA[t] : (-eN[t] + mu * U[t]) * ir;
   }
-  const double ir2 = ir * ir, ir3 = ir2 * ir;
+  const double ir2 = ir * ir, ir3 = ir2 * ir, idA = rec[L.o_id + A];
+#pragma unroll
   for (int t = 0; t < 3; ++t)
+#pragma unroll
     for (int u = 0; u < 3; ++u) {
       const double dl = t == u ? 1.0 : 0.0;
       const double PU = (dl - U[t] * U[u]) * ir;
       double hm;
       if (DIAG) {
-        const double PrA = (dl - eA[t] * eA[u]) / dA;
-        hm = (PrA / ir - (eA[t] * U[u] + U[t] * eA[u]) - fab * PU) * ir2 + 2.0 * fab * U[t] * U[u] * ir3;
+        const double PrA = (dl - eA[t] * eA[u]) * idA;
+        hm = (PrA * gab - (eA[t] * U[u] + U[t] * eA[u]) - fab * PU) * ir2 + 2.0 * fab * U[t] * U[u] * ir3;
       } else {
         hm = (eA[t] * U[u] + U[t] * eN[u] + fab * PU) * ir2 - 2.0 * fab * U[t] * U[u] * ir3;
       }
@@ -209,56 +221,62 @@ __global__ void __launch_bounds__(256) becke_dd_kernel(const double* __restrict_
                                                        const double* __restrict__ wquad,
                                                        const double* __restrict__ cw, const int64_t* __restrict__ perm,
                                                        const int64_t* __restrict__ chunks,
-                                                       const double* __restrict__ tab, int64_t natm, int h, int GB,
+                                                       const double* __restrict__ tab, int natm, int h, int GB,
                                                        double* __restrict__ part) {
   extern __shared__ double sm[];
   const int tid = threadIdx.x;
   const Tab T(tab, natm);
   const Rec L(natm);
-  const int64_t n3 = L.n3;
+  const int n3 = L.n3;
   const int64_t c = blockIdx.x;
-  const int64_t i0 = chunks[3 * c], i1 = chunks[3 * c + 1], G = chunks[3 * c + 2];
-  const int64_t nblk = natm * (natm + 1) / 2;
-  int64_t bA[NBPT], bB[NBPT];
+  const int64_t i0 = chunks[3 * c], i1 = chunks[3 * c + 1];
+  const int G = (int)chunks[3 * c + 2];
+  const int nblk = natm * (natm + 1) / 2;
+  const int ngrp = (natm + DIAG_GRP - 1) / DIAG_GRP;
+  const int nunit = nblk + natm * ngrp;
+  int uA[NBPT], uB[NBPT];
   double acc[NBPT][9];
 #pragma unroll
   for (int k = 0; k < NBPT; ++k) {
-    int64_t p = tid + 256 * k, A = 0;
+    int p = tid + 256 * k, A = 0, B = 0;
     if (p < nblk) {
       while (p >= natm - A) {
         p -= natm - A;
         ++A;
       }
+      B = A + p;
+    } else if (p < nunit) {
+      A = (p - nblk) / ngrp;
+      B = ((p - nblk) % ngrp) * DIAG_GRP;  // first N of the group
     }
-    bA[k] = A;
-    bB[k] = A + p;
+    uA[k] = A;
+    uB[k] = B;
 #pragma unroll
     for (int j = 0; j < 9; ++j) acc[k][j] = 0.0;
   }
   for (int64_t base = i0; base < i1; base += GB) {
     const int nb = (int)(i1 - base < GB ? i1 - base : GB);
     // distances and unit vectors towards the atoms
-    for (int64_t t = tid; t < nb * natm; t += 256) {
-      const int gi = (int)(t / natm);
-      const int64_t A = t % natm;
+    for (int t = tid; t < nb * natm; t += 256) {
+      const int gi = t / natm, A = t % natm;
       double* rec = sm + gi * L.stride;
       const int64_t g = perm ? perm[base + gi] : base + gi;
       const double d0 = T.R[3 * A] - coords[3 * g], d1 = T.R[3 * A + 1] - coords[3 * g + 1],
                    d2 = T.R[3 * A + 2] - coords[3 * g + 2];
-      const double d = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+      const double d = sqrt(d0 * d0 + d1 * d1 + d2 * d2), id = 1.0 / d;
       rec[A] = d;
-      rec[L.o_e + 3 * A] = d0 / d;
-      rec[L.o_e + 3 * A + 1] = d1 / d;
-      rec[L.o_e + 3 * A + 2] = d2 / d;
+      rec[L.o_id + A] = id;
+      rec[L.o_e + 3 * A] = d0 * id;
+      rec[L.o_e + 3 * A + 1] = d1 * id;
+      rec[L.o_e + 3 * A + 2] = d2 * id;
     }
     __syncthreads();
     // cell functions P_M
-    for (int64_t t = tid; t < nb * natm; t += 256) {
-      const int gi = (int)(t / natm);
-      const int64_t M = t % natm;
+    for (int t = tid; t < nb * natm; t += 256) {
+      const int gi = t / natm, M = t % natm;
       double* rec = sm + gi * L.stride;
       double p = 1.0;
-      for (int64_t N = 0; N < natm; ++N) {
+      for (int N = 0; N < natm; ++N) {
         if (N == M) continue;
         const double mu = (rec[M] - rec[N]) * T.irab[M * natm + N];
         p *= 0.5 * (1.0 - switch_eval<0>(mu, T.adj[M * natm + N], h).f);
@@ -270,21 +288,20 @@ __global__ void __launch_bounds__(256) becke_dd_kernel(const double* __restrict_
       double* rec = sm + gi * L.stride;
       const int64_t g = perm ? perm[base + gi] : base + gi;
       double Z = 0.0;
-      for (int64_t M = 0; M < natm; ++M) Z += rec[L.o_P + M];
+      for (int M = 0; M < natm; ++M) Z += rec[L.o_P + M];
       rec[L.o_sc] = Z;
       rec[L.o_sc + 1] = rec[L.o_P + G] / Z;
       rec[L.o_sc + 2] = wquad[g] * (cw ? cw[g] : 1.0);
     }
     __syncthreads();
     // l_M = d log P_M: block N != M is l_MN d mu_MN / dR_N, block M sums l_MN d mu_MN / dR_M
-    for (int64_t t = tid; t < nb * natm; t += 256) {
-      const int gi = (int)(t / natm);
-      const int64_t M = t % natm;
+    for (int t = tid; t < nb * natm; t += 256) {
+      const int gi = t / natm, M = t % natm;
       double* rec = sm + gi * L.stride;
       const double* e = rec + L.o_e;
       double* Lrow = rec + L.o_L + M * n3;
       double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-      for (int64_t N = 0; N < natm; ++N) {
+      for (int N = 0; N < natm; ++N) {
         if (N == M) continue;
         const double ir = T.irab[M * natm + N], a = T.adj[M * natm + N];
         const double mu = (rec[M] - rec[N]) * ir;
@@ -305,13 +322,12 @@ __global__ void __launch_bounds__(256) becke_dd_kernel(const double* __restrict_
       rec[L.o_gam + M] = ct * rec[L.o_P + M] * ((M == G ? 1.0 : 0.0) - q) / Z;
     }
     __syncthreads();
-    for (int64_t t = tid; t < nb * n3; t += 256) {
-      const int gi = (int)(t / n3);
-      const int64_t a = t % n3;
+    for (int t = tid; t < nb * n3; t += 256) {
+      const int gi = t / n3, a = t % n3;
       double* rec = sm + gi * L.stride;
       const double* Lm = rec + L.o_L;
       double z = 0.0;
-      for (int64_t M = 0; M < natm; ++M) z += rec[L.o_P + M] * Lm[M * n3 + a];
+      for (int M = 0; M < natm; ++M) z += rec[L.o_P + M] * Lm[M * n3 + a];
       const double Z = rec[L.o_sc], q = rec[L.o_sc + 1];
       rec[L.o_dZ + a] = z;
       rec[L.o_dq + a] = (rec[L.o_P + G] * Lm[G * n3 + a] - q * z) / Z;
@@ -326,10 +342,17 @@ __global__ void __launch_bounds__(256) becke_dd_kernel(const double* __restrict_
       const double fac = -rec[L.o_sc + 2] / rec[L.o_sc];
 #pragma unroll
       for (int k = 0; k < NBPT; ++k) {
-        if (tid + 256 * k >= nblk) continue;
-        const int64_t A = bA[k], B = bB[k];
+        const int p = tid + 256 * k;
+        if (p >= nunit) continue;
+        const int A = uA[k], B = uB[k];
         double* a9 = acc[k];
-        for (int64_t M = 0; M < natm; ++M) {
+        if (p >= nblk) {
+          const int n1 = B + DIAG_GRP < natm ? B + DIAG_GRP : natm;
+          for (int N = B; N < n1; ++N)
+            if (N != A) pair_block<true>(T, natm, h, rec, L, A, N, a9);
+          continue;
+        }
+        for (int M = 0; M < natm; ++M) {
           const double gm = gam[M];
           const double* la = Lm + M * n3 + 3 * A;
           const double* lb = Lm + M * n3 + 3 * B;
@@ -339,28 +362,39 @@ __global__ void __launch_bounds__(256) becke_dd_kernel(const double* __restrict_
           a9[3] += x1 * y0; a9[4] += x1 * y1; a9[5] += x1 * y2;
           a9[6] += x2 * y0; a9[7] += x2 * y1; a9[8] += x2 * y2;
         }
+#pragma unroll
         for (int t = 0; t < 3; ++t)
+#pragma unroll
           for (int s = 0; s < 3; ++s)
             a9[3 * t + s] += fac * (dq[3 * A + t] * dZ[3 * B + s] + dZ[3 * A + t] * dq[3 * B + s]);
-        if (A != B) {
-          pair_block<false>(T, natm, h, rec, L, A, B, a9);
-        } else {
-          for (int64_t N = 0; N < natm; ++N)
-            if (N != A) pair_block<true>(T, natm, h, rec, L, A, N, a9);
-        }
+        if (A != B) pair_block<false>(T, natm, h, rec, L, A, B, a9);
       }
     }
     __syncthreads();
   }
-  double* S = part + c * n3 * n3;
+  // diagonal-block pair units add into their block through LDS
+  double* D = sm;
+  for (int i = tid; i < 9 * natm; i += 256) D[i] = 0.0;
+  __syncthreads();
 #pragma unroll
   for (int k = 0; k < NBPT; ++k) {
-    if (tid + 256 * k >= nblk) continue;
-    const int64_t A = bA[k], B = bB[k];
+    const int p = tid + 256 * k;
+    if (p >= nblk && p < nunit)
+      for (int j = 0; j < 9; ++j) atomicAdd(&D[9 * uA[k] + j], acc[k][j]);
+  }
+  __syncthreads();
+  double* S = part + c * (int64_t)n3 * n3;
+#pragma unroll
+  for (int k = 0; k < NBPT; ++k) {
+    const int p = tid + 256 * k;
+    if (p >= nblk) continue;
+    const int A = uA[k], B = uB[k];
     for (int t = 0; t < 3; ++t)
       for (int s = 0; s < 3; ++s) {
-        S[(3 * A + t) * n3 + 3 * B + s] = acc[k][3 * t + s];
-        if (A != B) S[(3 * B + s) * n3 + 3 * A + t] = acc[k][3 * t + s];
+        double v = acc[k][3 * t + s];
+        if (A == B) v += D[9 * A + 3 * t + s];
+        S[(3 * A + t) * n3 + 3 * B + s] = v;
+        if (A != B) S[(3 * B + s) * n3 + 3 * A + t] = v;
       }
   }
 }
@@ -434,11 +468,12 @@ void gemm_kk(int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, cons
 Becke::Becke(int64_t natm, const double* atm_coords, const double* adjustment, hipStream_t s)
     : natm_(natm), stream_(s) {
   HH_REQUIRE(natm >= 1 && atm_coords && adjustment, "becke: null tables or no atoms");
-  std::vector<double> t(3 * natm + 2 * natm * natm + 3 * natm * natm, 0.0);
+  std::vector<double> t(3 * natm + 6 * natm * natm, 0.0);
   double* R = t.data();
   double* adj = R + 3 * natm;
   double* irab = adj + natm * natm;
   double* U = irab + natm * natm;
+  double* rab = U + 3 * natm * natm;
   for (int64_t i = 0; i < 3 * natm; ++i) R[i] = atm_coords[i];
   for (int64_t A = 0; A < natm; ++A)
     for (int64_t B = 0; B < natm; ++B) {
@@ -448,6 +483,7 @@ Becke::Becke(int64_t natm, const double* atm_coords, const double* adjustment, h
       const double d = std::sqrt(d0 * d0 + d1 * d1 + d2 * d2);
       HH_REQUIRE(d > 0.0, "becke: two atoms share a position");
       irab[A * natm + B] = 1.0 / d;
+      rab[A * natm + B] = d;
       U[3 * (A * natm + B)] = d0 / d;
       U[3 * (A * natm + B) + 1] = d1 / d;
       U[3 * (A * natm + B) + 2] = d2 / d;
@@ -551,24 +587,25 @@ void Becke::partition(const double* coords, int64_t ng, bool by_atom, const int6
   if (want_dc) gemm_kk(o.nset_dw, n3, ng, o.contract_dw, ng, dw, ng, o.dc, o.nset_dw, stream_, ws_);
 
   if (want_ddw || want_ddc) {
-    const Rec L(natm);
+    const Rec L((int)natm);
     const int64_t gb = std::max<int64_t>(1, std::min<int64_t>(8, 8192 / L.stride));
     const size_t lds = sizeof(double) * gb * L.stride;
-    const int64_t nblk = natm * (natm + 1) / 2;
+    const int64_t nunit = natm * (natm + 1) / 2 + natm * ((natm + DIAG_GRP - 1) / DIAG_GRP);
+    const int na = (int)natm;
     auto run = [&](const int64_t* d_ch, int64_t nchunk, const double* cwv, double* part) {
       if (nchunk == 0) return;
-      if (nblk <= 256)
+      if (nunit <= 256)
         hipLaunchKernelGGL(becke_dd_kernel<1>, dim3((unsigned)nchunk), dim3(256), lds, stream_, coords, wquad, cwv,
-                           d_perm, d_ch, tab, natm, hardness, (int)gb, part);
-      else if (nblk <= 512)
+                           d_perm, d_ch, tab, na, hardness, (int)gb, part);
+      else if (nunit <= 512)
         hipLaunchKernelGGL(becke_dd_kernel<2>, dim3((unsigned)nchunk), dim3(256), lds, stream_, coords, wquad, cwv,
-                           d_perm, d_ch, tab, natm, hardness, (int)gb, part);
-      else if (nblk <= 1024)
+                           d_perm, d_ch, tab, na, hardness, (int)gb, part);
+      else if (nunit <= 1024)
         hipLaunchKernelGGL(becke_dd_kernel<4>, dim3((unsigned)nchunk), dim3(256), lds, stream_, coords, wquad, cwv,
-                           d_perm, d_ch, tab, natm, hardness, (int)gb, part);
+                           d_perm, d_ch, tab, na, hardness, (int)gb, part);
       else
         hipLaunchKernelGGL(becke_dd_kernel<8>, dim3((unsigned)nchunk), dim3(256), lds, stream_, coords, wquad, cwv,
-                           d_perm, d_ch, tab, natm, hardness, (int)gb, part);
+                           d_perm, d_ch, tab, na, hardness, (int)gb, part);
       HH_HIP(hipGetLastError());
     };
     const size_t lds_fix = sizeof(double) * 3 * n3;

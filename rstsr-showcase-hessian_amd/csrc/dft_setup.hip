@@ -19,6 +19,7 @@
 
 #include "becke.hpp"
 #include "elementwise.hpp"
+#include "gemm_nt.hpp"
 #include "numint.hpp"
 
 namespace hh {
@@ -227,19 +228,26 @@ __global__ void __launch_bounds__(256) vxc_diag_kernel(const double* __restrict_
 }
 
 // ---- operands of the vxc products, one thread per (g, mu) ----
-// aowv[g, mu, t] (g + gb (mu + nao t)): RHO w v0 ao_t; else 0.5 w v0 ao_t + sum_r w v_r ao_tr
-// aow[g, mu]: RHO w v0 ao_0; else 0.5 w v0 ao_0 + sum_r w v_r ao_r
+// aowv[g, mu, t] (g + ldg (mu + nao t)): RHO w v0 ao_t; else 0.5 w v0 ao_t + sum_r w v_r ao_tr
+// aow[g, mu] (g + ldg mu): RHO w v0 ao_0; else 0.5 w v0 ao_0 + sum_r w v_r ao_r
+// (rows gb <= g < ldg are zero: the K padding of the matrix-core products)
 // TAU: H2[(g, r), (mu, t)] = ao_tr, H2w = 0.5 w v_tau H2, AT[(g, r), mu] = 0.5 w v_tau ao_r
 // (row (g, r) = g + gb r, leading dimension 3 gb)
 template <int DEN>
 __global__ void vxc_ops_kernel(const double* __restrict__ ao, int64_t ng, int64_t nao, int64_t g0, int64_t gb,
-                               const double* __restrict__ vxc, const double* __restrict__ w,
+                               int64_t ldg, const double* __restrict__ vxc, const double* __restrict__ w,
                                double* __restrict__ aowv, double* __restrict__ aow, double* __restrict__ H2,
                                double* __restrict__ H2w, double* __restrict__ AT) {
   constexpr int NV = Den<DEN>::NV;
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= gb * nao) return;
-  const int64_t g = e % gb, mu = e / gb, gg = g0 + g;
+  if (e >= ldg * nao) return;
+  const int64_t g = e % ldg, mu = e / ldg, gg = g0 + g;
+  if (g >= gb) {  // K padding rows of aowv / aow
+    aow[g + ldg * mu] = 0.0;
+#pragma unroll
+    for (int t = 0; t < 3; ++t) aowv[g + ldg * (mu + nao * t)] = 0.0;
+    return;
+  }
   const int64_t sa = ng * nao;
   const double* a = ao + gg + ng * mu;
   const double wg = w[gg];
@@ -247,20 +255,20 @@ __global__ void vxc_ops_kernel(const double* __restrict__ ao, int64_t ng, int64_
 #pragma unroll
   for (int x = 0; x < NV; ++x) wv[x] = wg * vxc[gg + ng * x];
   if constexpr (DEN == DEN_RHO) {
-    aow[g + gb * mu] = wv[0] * a[0];
+    aow[g + ldg * mu] = wv[0] * a[0];
 #pragma unroll
-    for (int t = 0; t < 3; ++t) aowv[g + gb * (mu + nao * t)] = wv[0] * a[(1 + t) * sa];
+    for (int t = 0; t < 3; ++t) aowv[g + ldg * (mu + nao * t)] = wv[0] * a[(1 + t) * sa];
   } else {
     double v = 0.5 * wv[0] * a[0];
 #pragma unroll
     for (int r = 0; r < 3; ++r) v += wv[1 + r] * a[(1 + r) * sa];
-    aow[g + gb * mu] = v;
+    aow[g + ldg * mu] = v;
 #pragma unroll
     for (int t = 0; t < 3; ++t) {
       double u = 0.5 * wv[0] * a[(1 + t) * sa];
 #pragma unroll
       for (int r = 0; r < 3; ++r) u += wv[1 + r] * a[c2(t, r) * sa];
-      aowv[g + gb * (mu + nao * t)] = u;
+      aowv[g + ldg * (mu + nao * t)] = u;
     }
   }
   if constexpr (DEN == DEN_TAU) {
@@ -283,27 +291,40 @@ __global__ void vxc_ops_kernel(const double* __restrict__ ao, int64_t ng, int64_
 // with W' = W of output spin s, component 0 halved and tau quartered:
 // AW[blk][g, (mu, t, A)] = sum_{c < 4 (RHO: c = 0)} W'[g, c, t, A] ao_c[g, mu]    (blk 0)
 //                        = W'[g, 4, t, A] ao_r[g, mu]                             (blk r = 1..3, TAU)
-// element g + gb (mu + nao (t + 3 A)) of block blk at offset blk * gb * nao * 3 natm.
+// element g + ldg (mu + nao (t + 3 A)) of block blk at offset blk * ldg * nao * 3 natm; rows
+// gb <= g < ldg are zero (the K padding of the matrix-core product).
 template <int DEN, int NS>
 __global__ void fxc_ops_kernel(const double* __restrict__ ao, int64_t ng, int64_t nao, int64_t g0, int64_t gb,
-                               const double* __restrict__ W, int s, int64_t natm, double* __restrict__ AW) {
+                               int64_t ldg, const double* __restrict__ W, int s, int64_t natm,
+                               double* __restrict__ AW) {
   constexpr int NV = Den<DEN>::NV;
   constexpr int NC = DEN == DEN_RHO ? 1 : 4;
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= gb * nao) return;
-  const int64_t g = e % gb, mu = e / gb;
+  if (e >= ldg * nao) return;
+  const int64_t g = e % ldg, mu = e / ldg;
+  const int64_t blk = ldg * nao * 3 * natm;
+  if (g >= gb) {  // K padding rows
+    for (int64_t tA = 0; tA < 3 * natm; ++tA) {
+      const int64_t o = g + ldg * (mu + nao * tA);
+      AW[o] = 0.0;
+      if constexpr (DEN == DEN_TAU) {
+#pragma unroll
+        for (int r = 1; r < 4; ++r) AW[o + r * blk] = 0.0;
+      }
+    }
+    return;
+  }
   const int64_t sa = ng * nao;
   const double* a = ao + (g0 + g) + ng * mu;
   double ac[4];
 #pragma unroll
   for (int c = 0; c < NC; ++c) ac[c] = a[c * sa];
-  const int64_t blk = gb * nao * 3 * natm;
   for (int64_t tA = 0; tA < 3 * natm; ++tA) {
     const double* Wp = W + g + gb * NV * (s + NS * tA);  // component c at Wp[gb * c]
     double v = 0.5 * Wp[0] * ac[0];
 #pragma unroll
     for (int c = 1; c < NC; ++c) v += Wp[gb * c] * ac[c];
-    const int64_t o = g + gb * (mu + nao * tA);
+    const int64_t o = g + ldg * (mu + nao * tA);
     AW[o] = v;
     if constexpr (DEN == DEN_TAU) {
       const double w4 = 0.25 * Wp[gb * 4];
@@ -311,6 +332,16 @@ __global__ void fxc_ops_kernel(const double* __restrict__ ao, int64_t ng, int64_
       for (int r = 1; r < 4; ++r) AW[o + r * blk] = w4 * ac[r];
     }
   }
+}
+
+// AO components 0..3 of the batch with the point axis padded to ldg (zero rows):
+// out[(c nao + mu) ldg + g] = ao[g0 + g, mu, c]
+__global__ void stage_ao_kernel(const double* __restrict__ ao, int64_t ng, int64_t nao, int64_t g0, int64_t gb,
+                                int64_t ldg, double* __restrict__ out) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= ldg * nao * 4) return;
+  const int64_t g = e % ldg, cm = e / ldg;
+  out[e] = g < gb ? ao[g0 + g + ng * cm] : 0.0;
 }
 
 // ---- finalisation ----
@@ -374,7 +405,7 @@ __global__ void vmat_ip_final_kernel(const double* __restrict__ Cip, int64_t nao
 }
 
 // vmat_deriv1[mu, nu, t, A] = F[mu, nu, tA] + F[nu, mu, tA] - [mu in A] ip[mu, nu, t] - [nu in A] ip[nu, mu, t]
-// with F[mu, nu, tA] = Cfx[mu + nao tA, nu] (hess_rks.rs:471-565, bra + ket assembly)
+// with F[mu, nu, tA] = CfxT[nu, mu + nao tA] (hess_rks.rs:471-565, bra + ket assembly)
 __global__ void vmat_deriv1_final_kernel(const double* __restrict__ Cfx, const double* __restrict__ Cip,
                                          const int64_t* __restrict__ atom_of, int64_t nao, int64_t natm,
                                          double* __restrict__ out) {
@@ -382,7 +413,7 @@ __global__ void vmat_deriv1_final_kernel(const double* __restrict__ Cfx, const d
   if (e >= nao * nao * 3 * natm) return;
   const int64_t mu = e % nao, nu = (e / nao) % nao, tA = e / (nao * nao);
   const int64_t t = tA % 3, A = tA / 3;
-  double v = Cfx[(mu + nao * tA) + nao * 3 * natm * nu] + Cfx[(nu + nao * tA) + nao * 3 * natm * mu];
+  double v = Cfx[nu + nao * (mu + nao * tA)] + Cfx[mu + nao * (nu + nao * tA)];
   if (atom_of[mu] == A) v -= Cip[(mu + nao * t) + 3 * nao * nu];
   if (atom_of[nu] == A) v -= Cip[(nu + nao * t) + 3 * nao * mu];
   out[e] = v;
@@ -420,19 +451,29 @@ __global__ void phi_kernel(double* __restrict__ W, const double* __restrict__ vx
   W[e] += vxc[g0 + g + ng * j] * dw[tA * ng + g0 + g];
 }
 
-// c[g] = exc[g] sum_s rho_s[g, 0], the weights of the ddw contraction (hess_rks_becke.rs:976-978)
-__global__ void cddw_kernel(const double* __restrict__ ao, int64_t ng, int64_t nao, int64_t g0, int64_t gb,
-                            const double* __restrict__ aod0, const double* __restrict__ aod1, int nspin,
-                            const double* __restrict__ exc, double* __restrict__ c) {
-  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (g >= gb) return;
+// c[g] = exc[g] sum_s rho_s[g, 0], the weights of the ddw contraction (hess_rks_becke.rs:976-978);
+// 32 points per workgroup, 8 threads per point split the AO sum.
+__global__ void __launch_bounds__(256) cddw_kernel(const double* __restrict__ ao, int64_t ng, int64_t nao, int64_t g0,
+                                                   int64_t gb, const double* __restrict__ aod0,
+                                                   const double* __restrict__ aod1, int nspin,
+                                                   const double* __restrict__ exc, double* __restrict__ c) {
+  __shared__ double red[8][32];
+  const int lane = threadIdx.x & 31, part = threadIdx.x >> 5;
+  const int64_t g = (int64_t)blockIdx.x * 32 + lane;
   double r = 0.0;
-  for (int64_t mu = 0; mu < nao; ++mu) {
-    const double a = ao[g0 + g + ng * mu];
-    r += a * aod0[g + gb * mu];
-    if (nspin == 2) r += a * aod1[g + gb * mu];
+  if (g < gb)
+    for (int64_t mu = part; mu < nao; mu += 8) {
+      const double a = ao[g0 + g + ng * mu];
+      r += a * aod0[g + gb * mu];
+      if (nspin == 2) r += a * aod1[g + gb * mu];
+    }
+  red[part][lane] = r;
+  __syncthreads();
+  if (part == 0 && g < gb) {
+    double v = 0.0;
+    for (int p = 0; p < 8; ++p) v += red[p][lane];
+    c[g0 + g] = exc[g0 + g] * v;
   }
-  c[g0 + g] = exc[g0 + g] * r;
 }
 
 // out[i] = sum_G in[G n + i]
@@ -513,7 +554,7 @@ __global__ void becke_de_final_kernel(const double* __restrict__ X, const double
 }
 
 // vmat_deriv1_grid[mu, nu, t, A] (hess_rks_becke.rs:1238): the bra + ket assembly of the Phi
-// field, the basis part -[mu in A] ip - [nu in A] ip^T, and vmat_becke_vxc = ip_A + ip_A^T of the
+// field (CfxT[nu, mu + nao tA]), the basis part -[mu in A] ip - [nu in A] ip^T, and vmat_becke_vxc = ip_A + ip_A^T of the
 // points of atom A (hess_rks_becke.rs:893-895).
 __global__ void vmat_grid_final_kernel(const double* __restrict__ Cfx, const double* __restrict__ Cip,
                                        const double* __restrict__ CipG, const int64_t* __restrict__ atom_of,
@@ -523,7 +564,7 @@ __global__ void vmat_grid_final_kernel(const double* __restrict__ Cfx, const dou
   const int64_t mu = e % nao, nu = (e / nao) % nao, tA = e / (nao * nao);
   const int64_t t = tA % 3, A = tA / 3;
   const int64_t ld = 3 * nao;
-  double v = Cfx[(mu + nao * tA) + nao * 3 * natm * nu] + Cfx[(nu + nao * tA) + nao * 3 * natm * mu];
+  double v = Cfx[nu + nao * (mu + nao * tA)] + Cfx[mu + nao * (nu + nao * tA)];
   if (atom_of[mu] == A) v -= Cip[(mu + nao * t) + ld * nu];
   if (atom_of[nu] == A) v -= Cip[(nu + nao * t) + ld * mu];
   const double* ipA = CipG + A * 3 * nao * nao;
@@ -545,6 +586,24 @@ void gemm_tn(int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, cons
     g.K1 = K1; g.A2 = A2; g.lda2 = lda2; g.B2 = B2; g.ldb2 = ldb2;
   }
   gemm<double, double>(g, PREC_F64, s, ws);
+}
+
+// C[m, n] += alpha sum_k A[k + ld m] B[k + ld n] on the matrix cores (NT FP64); K = ld is the
+// padded point count (a multiple of 32, zero rows beyond the batch)
+void nt_acc(int64_t M, int64_t N, int64_t ld, const double* A, const double* B, double* C, int64_t ldc, double alpha,
+            hipStream_t s, DevBuf& ws) {
+  NtArgs a;
+  a.M = M; a.N = N; a.K = ld;
+  a.A = A; a.lda = ld; a.B = B; a.ldb = ld;
+  a.C = C; a.ldc = ldc; a.alpha = alpha; a.beta = 1.0;
+  gemm_nt(a, NT_F64, s, ws);
+}
+
+// fxc part of vmat_deriv1, transposed: CT[nu, mu + nao tA] += sum_g ao_b[g, nu] AW_b[g, (mu, tA)]
+// over the AW blocks b (1, TAU 4) with the staged AO components b
+void vmat_fx(int nblk, const double* stage, const double* AW, int64_t ldg, int64_t nao, int64_t M_fx, double* CT,
+             hipStream_t s, DevBuf& ws) {
+  for (int b = 0; b < nblk; ++b) nt_acc(nao, M_fx, ldg, stage + b * nao * ldg, AW + b * ldg * M_fx, CT, nao, 1.0, s, ws);
 }
 
 // out[g, mu, c] = sum_nu ao[g0 + g, nu, c] dm0[nu, mu], c < nc
@@ -624,11 +683,11 @@ void Numint::hessian_setup(int den, int nspin, const double* const* dm0, int64_t
   const int nblk_aw = tau ? 4 : 1;
   const int64_t n3 = 3 * natm, M_fx = nao_ * n3;
 
-  // per-point scratch (doubles): ao_dm0 per spin, drho + W, vxc operands, fxc operand blocks
-  const int64_t per_point = nspin * nao_ * nc + 2 * nv * nspin * n3 + 4 * nao_ + (tau ? 21 * nao_ : 0) +
+  // per-point scratch (doubles): ao_dm0 per spin, drho + W, vxc operands, staged AO, fxc operand blocks
+  const int64_t per_point = nspin * nao_ * nc + 2 * nv * nspin * n3 + 8 * nao_ + (tau ? 21 * nao_ : 0) +
                             nblk_aw * M_fx;
   int64_t gbmax = batch > 0 ? batch : (int64_t)(1LL << 30) / (8 * per_point);
-  gbmax = std::min(ng_, std::max<int64_t>(64, (gbmax / 64) * 64));
+  gbmax = std::max<int64_t>(64, (std::min(ng_, gbmax) + 63) / 64 * 64);
 
   // persistent accumulators
   const int64_t nsplit = std::max<int64_t>(1, std::min<int64_t>(64, ceil_div(2048, nao_)));
@@ -651,25 +710,29 @@ void Numint::hessian_setup(int den, int nspin, const double* const* dm0, int64_t
 
   // batch scratch
   const int64_t n_aod = gbmax * nao_ * nc, n_dr = gbmax * nv * nspin * n3;
-  const int64_t n_scr = nspin * n_aod + 2 * n_dr + 4 * gbmax * nao_ + (tau ? 21 * gbmax * nao_ : 0) +
+  const int64_t n_scr = nspin * n_aod + 2 * n_dr + 8 * gbmax * nao_ + (tau ? 21 * gbmax * nao_ : 0) +
                         nblk_aw * gbmax * M_fx;
   double* scr = static_cast<double*>(akb_.reserve(sizeof(double) * n_scr));
   double* aod[2] = {scr, scr + n_aod};
   double* drho = scr + nspin * n_aod;
   double* W = drho + n_dr;
-  double* aowv = W + n_dr;                 // [gb, nao, 3]
-  double* aow = aowv + 3 * gbmax * nao_;   // [gb, nao]
+  double* aowv = W + n_dr;                 // [ldg, nao, 3]
+  double* aow = aowv + 3 * gbmax * nao_;   // [ldg, nao]
   double* H2 = aow + gbmax * nao_;         // TAU: [3 gb, 3 nao]
   double* H2w = H2 + (tau ? 9 * gbmax * nao_ : 0);
   double* AT = H2w + (tau ? 9 * gbmax * nao_ : 0);  // TAU: [3 gb, nao]
-  double* AW = AT + (tau ? 3 * gbmax * nao_ : 0);   // [nblk_aw][gb, M_fx]
+  double* stage = AT + (tau ? 3 * gbmax * nao_ : 0);  // [4][nao, ldg]
+  double* AW = stage + 4 * gbmax * nao_;            // [nblk_aw][ldg, M_fx]
 
   const double* ao = ao_.as<double>();
   const double* w = w_.as<double>();
   const double fo = den == DEN_RHO ? 1.0 : 2.0;  // hess_rks.rs:364 / 377
   for (int64_t g0 = 0; g0 < ng_; g0 += gbmax) {
-    const int64_t gb = std::min(gbmax, ng_ - g0);
+    const int64_t gb = std::min(gbmax, ng_ - g0), ldg = ceil_div(gb, 32) * 32;
     for (int s = 0; s < nspin; ++s) ao_dm0(ao, ng_, nao_, g0, gb, nc, dm0[s], aod[s], stream_, ws_);
+    hipLaunchKernelGGL(stage_ao_kernel, dim3(nblocks(ldg * nao_ * 4)), dim3(256), 0, stream_, ao, ng_, nao_, g0, gb,
+                       ldg, stage);
+    HH_HIP(hipGetLastError());
     HH_SETUP_DISPATCH_NS(den, nspin, drho_w_kernel, dim3(nblocks(gb * natm)), dim3(256), 0, stream_, ao, ng_, nao_,
                          g0, gb, aod[0], aod[nspin - 1], dsl, natm, fxc, w, drho, W);
     // de_fxc: sum over (g, y, s) of W[(g,y,s), (t,A)] drho[(g,y,s), (s',B)]
@@ -679,32 +742,24 @@ void Numint::hessian_setup(int den, int nspin, const double* const* dm0, int64_t
       const double* vxc_s = vxc + ng_ * nv * s;
       HH_SETUP_DISPATCH(den, vxc_diag_kernel, dim3((unsigned)nao_, (unsigned)nsplit), dim3(256), 0, stream_, ao, ng_,
                         nao_, g0, gb, aod[s], vxc_s, w, acc_diag(s), nsplit);
-      HH_SETUP_DISPATCH(den, vxc_ops_kernel, dim3(nblocks(gb * nao_)), dim3(256), 0, stream_, ao, ng_, nao_, g0, gb,
-                        vxc_s, w, aowv, aow, H2, H2w, AT);
+      HH_SETUP_DISPATCH(den, vxc_ops_kernel, dim3(nblocks(ldg * nao_)), dim3(256), 0, stream_, ao, ng_, nao_, g0, gb,
+                        ldg, vxc_s, w, aowv, aow, H2, H2w, AT);
       const double* ao0 = ao + g0;
       const double* ao1 = ao + g0 + ng_ * nao_;  // components 1..3 as [gb, 3 nao], ld ng
       // de_vxc_off AO-pair tensor C[(mu, t), (nu, s)] (hess_rks.rs:359-400)
-      gemm_tn(3 * nao_, 3 * nao_, gb, aowv, gb, ao1, ng_, acc_off(s), 3 * nao_, fo, stream_, ws_);
+      nt_acc(3 * nao_, 3 * nao_, ldg, aowv, stage + nao_ * ldg, acc_off(s), 3 * nao_, fo, stream_, ws_);
       if (tau) gemm_tn(3 * nao_, 3 * nao_, 3 * gb, H2w, 3 * gb, H2, 3 * gb, acc_off(s), 3 * nao_, 1.0, stream_, ws_);
       // vmat_ip Cip[(mu, t), nu] (hess_rks.rs:421-469)
       if (den == DEN_RHO)
-        gemm_tn(3 * nao_, nao_, gb, ao1, ng_, aow, gb, acc_ip(s), 3 * nao_, 1.0, stream_, ws_);
+        gemm_tn(3 * nao_, nao_, gb, ao1, ng_, aow, ldg, acc_ip(s), 3 * nao_, 1.0, stream_, ws_);
       else
-        gemm_tn(3 * nao_, nao_, 2 * gb, ao1, ng_, aow, gb, acc_ip(s), 3 * nao_, 1.0, stream_, ws_, gb, aowv, gb, ao0,
+        gemm_tn(3 * nao_, nao_, 2 * gb, ao1, ng_, aow, ldg, acc_ip(s), 3 * nao_, 1.0, stream_, ws_, gb, aowv, ldg, ao0,
                 ng_);
       if (tau) gemm_tn(3 * nao_, nao_, 3 * gb, H2, 3 * gb, AT, 3 * gb, acc_ip(s), 3 * nao_, 1.0, stream_, ws_);
-      // fxc part of vmat_deriv1: Cfx[(mu, t, A), nu]
-      HH_SETUP_DISPATCH_NS(den, nspin, fxc_ops_kernel, dim3(nblocks(gb * nao_)), dim3(256), 0, stream_, ao, ng_, nao_,
-                           g0, gb, W, s, natm, AW);
-      if (!tau) {
-        gemm_tn(M_fx, nao_, gb, AW, gb, ao0, ng_, acc_fx(s), M_fx, 1.0, stream_, ws_);
-      } else {
-        const int64_t blk = gb * M_fx;
-        gemm_tn(M_fx, nao_, 2 * gb, AW, gb, ao0, ng_, acc_fx(s), M_fx, 1.0, stream_, ws_, gb, AW + blk, gb,
-                ao0 + ng_ * nao_, ng_);
-        gemm_tn(M_fx, nao_, 2 * gb, AW + 2 * blk, gb, ao0 + 2 * ng_ * nao_, ng_, acc_fx(s), M_fx, 1.0, stream_, ws_,
-                gb, AW + 3 * blk, gb, ao0 + 3 * ng_ * nao_, ng_);
-      }
+      // fxc part of vmat_deriv1: CfxT[nu, (mu, t, A)]
+      HH_SETUP_DISPATCH_NS(den, nspin, fxc_ops_kernel, dim3(nblocks(ldg * nao_)), dim3(256), 0, stream_, ao, ng_, nao_,
+                           g0, gb, ldg, W, s, natm, AW);
+      vmat_fx(nblk_aw, stage, AW, ldg, nao_, M_fx, acc_fx(s), stream_, ws_);
     }
   }
   hipLaunchKernelGGL(de_fxc_final_kernel, dim3(nblocks(n3 * n3)), dim3(256), 0, stream_, acc_fxc, natm, out.de_fxc);
@@ -757,10 +812,10 @@ void Numint::hessian_setup_becke(int den, int nspin, const double* const* dm0, i
   const int nblk_aw = tau ? 4 : 1;
   const int64_t n3 = 3 * natm, M_fx = nao_ * n3, nvns = (int64_t)nv * nspin;
 
-  const int64_t per_point = nspin * nao_ * nc + 2 * nv * nspin * n3 + 4 * nao_ + (tau ? 21 * nao_ : 0) +
+  const int64_t per_point = nspin * nao_ * nc + 2 * nv * nspin * n3 + 8 * nao_ + (tau ? 21 * nao_ : 0) +
                             nblk_aw * M_fx;
   int64_t gbmax = batch > 0 ? batch : (int64_t)(1LL << 30) / (8 * per_point);
-  gbmax = std::min(ng_, std::max<int64_t>(64, (gbmax / 64) * 64));
+  gbmax = std::max<int64_t>(64, (std::min(ng_, gbmax) + 63) / 64 * 64);
 
   // accumulators: [Xf | Xt | X] then per spin [diag_G | C_G | Cip_G | Fgrid | Ffixed]
   const int64_t nsplit = std::max<int64_t>(1, std::min<int64_t>(64, ceil_div(2048, nao_)));
@@ -800,7 +855,7 @@ void Numint::hessian_setup_becke(int den, int nspin, const double* const* dm0, i
   const int64_t* datom = dsl + 2 * natm;
 
   const int64_t n_aod = gbmax * nao_ * nc, n_dr = gbmax * nvns * n3;
-  const int64_t n_scr = nspin * n_aod + 2 * n_dr + 4 * gbmax * nao_ + (tau ? 21 * gbmax * nao_ : 0) +
+  const int64_t n_scr = nspin * n_aod + 2 * n_dr + 8 * gbmax * nao_ + (tau ? 21 * gbmax * nao_ : 0) +
                         nblk_aw * gbmax * M_fx;
   double* scr = static_cast<double*>(akb_.reserve(sizeof(double) * n_scr));
   double* aod[2] = {scr, scr + n_aod};
@@ -811,7 +866,8 @@ void Numint::hessian_setup_becke(int den, int nspin, const double* const* dm0, i
   double* H2 = aow + gbmax * nao_;
   double* H2w = H2 + (tau ? 9 * gbmax * nao_ : 0);
   double* AT = H2w + (tau ? 9 * gbmax * nao_ : 0);
-  double* AW = AT + (tau ? 3 * gbmax * nao_ : 0);
+  double* stage = AT + (tau ? 3 * gbmax * nao_ : 0);
+  double* AW = stage + 4 * gbmax * nao_;
 
   // Becke weight derivatives dw [natm, 3, ng] of the atom-grouped grid
   {
@@ -825,24 +881,18 @@ void Numint::hessian_setup_becke(int den, int nspin, const double* const* dm0, i
   const double fo = den == DEN_RHO ? 1.0 : 2.0;
   for (int64_t G = 0; G < natm; ++G)
     for (int64_t g0 = bin.split[G]; g0 < bin.split[G + 1]; g0 += gbmax) {
-      const int64_t gb = std::min(gbmax, bin.split[G + 1] - g0);
+      const int64_t gb = std::min(gbmax, bin.split[G + 1] - g0), ldg = ceil_div(gb, 32) * 32;
       for (int s = 0; s < nspin; ++s) ao_dm0(ao, ng_, nao_, g0, gb, nc, dm0[s], aod[s], stream_, ws_);
+      hipLaunchKernelGGL(stage_ao_kernel, dim3(nblocks(ldg * nao_ * 4)), dim3(256), 0, stream_, ao, ng_, nao_, g0, gb,
+                         ldg, stage);
+      HH_HIP(hipGetLastError());
       HH_SETUP_DISPATCH_NS(den, nspin, drho_w_kernel, dim3(nblocks(gb * natm)), dim3(256), 0, stream_, ao, ng_, nao_,
                            g0, gb, aod[0], aod[nspin - 1], dsl, natm, fxc, w, drho, W);
       const int64_t Kf = gb * nvns;
       auto fxc_vmat = [&](int s, double* C) {
-        HH_SETUP_DISPATCH_NS(den, nspin, fxc_ops_kernel, dim3(nblocks(gb * nao_)), dim3(256), 0, stream_, ao, ng_,
-                             nao_, g0, gb, W, s, natm, AW);
-        const double* ao0 = ao + g0;
-        if (!tau) {
-          gemm_tn(M_fx, nao_, gb, AW, gb, ao0, ng_, C, M_fx, 1.0, stream_, ws_);
-        } else {
-          const int64_t blk = gb * M_fx;
-          gemm_tn(M_fx, nao_, 2 * gb, AW, gb, ao0, ng_, C, M_fx, 1.0, stream_, ws_, gb, AW + blk, gb,
-                  ao0 + ng_ * nao_, ng_);
-          gemm_tn(M_fx, nao_, 2 * gb, AW + 2 * blk, gb, ao0 + 2 * ng_ * nao_, ng_, C, M_fx, 1.0, stream_, ws_, gb,
-                  AW + 3 * blk, gb, ao0 + 3 * ng_ * nao_, ng_);
-        }
+        HH_SETUP_DISPATCH_NS(den, nspin, fxc_ops_kernel, dim3(nblocks(ldg * nao_)), dim3(256), 0, stream_, ao, ng_,
+                             nao_, g0, gb, ldg, W, s, natm, AW);
+        vmat_fx(nblk_aw, stage, AW, ldg, nao_, M_fx, C, stream_, ws_);
       };
       // grid-fixed de_fxc and vmat_deriv1 (fxc part)
       if (out.de_fxc) gemm_tn(n3, n3, Kf, W, Kf, drho, Kf, acc_xf, n3, 1.0, stream_, ws_);
@@ -857,25 +907,26 @@ void Numint::hessian_setup_becke(int den, int nspin, const double* const* dm0, i
                          nvns, natm);
       HH_HIP(hipGetLastError());
       gemm_tn(n3, n3, Kf, W, Kf, drho, Kf, acc_x, n3, 1.0, stream_, ws_);
-      hipLaunchKernelGGL(cddw_kernel, dim3(nblocks(gb)), dim3(256), 0, stream_, ao, ng_, nao_, g0, gb, aod[0],
+      hipLaunchKernelGGL(cddw_kernel, dim3((unsigned)ceil_div(gb, 32)), dim3(256), 0, stream_, ao, ng_, nao_, g0, gb, aod[0],
                          aod[nspin - 1], nspin, bin.exc, cw);
       HH_HIP(hipGetLastError());
       for (int s = 0; s < nspin; ++s) {
         const double* vxc_s = vxc + ng_ * nv * s;
         HH_SETUP_DISPATCH(den, vxc_diag_kernel, dim3((unsigned)nao_, (unsigned)nsplit), dim3(256), 0, stream_, ao, ng_,
                           nao_, g0, gb, aod[s], vxc_s, w, acc_d(s) + G * nsplit * nao_ * 6, nsplit);
-        HH_SETUP_DISPATCH(den, vxc_ops_kernel, dim3(nblocks(gb * nao_)), dim3(256), 0, stream_, ao, ng_, nao_, g0, gb,
-                          vxc_s, w, aowv, aow, H2, H2w, AT);
+        HH_SETUP_DISPATCH(den, vxc_ops_kernel, dim3(nblocks(ldg * nao_)), dim3(256), 0, stream_, ao, ng_, nao_, g0,
+                          gb, ldg, vxc_s, w, aowv, aow, H2, H2w, AT);
         const double* ao0 = ao + g0;
         const double* ao1 = ao + g0 + ng_ * nao_;
         double* CG = acc_o(s) + G * 9 * nao_ * nao_;
         double* IG = acc_i(s) + G * 3 * nao_ * nao_;
-        gemm_tn(3 * nao_, 3 * nao_, gb, aowv, gb, ao1, ng_, CG, 3 * nao_, fo, stream_, ws_);
+        nt_acc(3 * nao_, 3 * nao_, ldg, aowv, stage + nao_ * ldg, CG, 3 * nao_, fo, stream_, ws_);
         if (tau) gemm_tn(3 * nao_, 3 * nao_, 3 * gb, H2w, 3 * gb, H2, 3 * gb, CG, 3 * nao_, 1.0, stream_, ws_);
         if (den == DEN_RHO)
-          gemm_tn(3 * nao_, nao_, gb, ao1, ng_, aow, gb, IG, 3 * nao_, 1.0, stream_, ws_);
+          gemm_tn(3 * nao_, nao_, gb, ao1, ng_, aow, ldg, IG, 3 * nao_, 1.0, stream_, ws_);
         else
-          gemm_tn(3 * nao_, nao_, 2 * gb, ao1, ng_, aow, gb, IG, 3 * nao_, 1.0, stream_, ws_, gb, aowv, gb, ao0, ng_);
+          gemm_tn(3 * nao_, nao_, 2 * gb, ao1, ng_, aow, ldg, IG, 3 * nao_, 1.0, stream_, ws_, gb, aowv, ldg, ao0,
+                  ng_);
         if (tau) gemm_tn(3 * nao_, nao_, 3 * gb, H2, 3 * gb, AT, 3 * gb, IG, 3 * nao_, 1.0, stream_, ws_);
         fxc_vmat(s, acc_f(s));
       }
