@@ -301,7 +301,7 @@ def becke_flops(ng, nao, natm, nvar=4):
 def workload_dft_becke(args):
     """DFT skeleton setup with the Becke grid shift (make_hessian_setup_becke,
     hess_rks_becke.rs:1106-1266) at (H2O)_8 sizes, GGA, FP64: de_xc_skeleton and
-    vmat_deriv1_grid, the keys its consumer reads (hess_rks_becke.rs:1387-1401)."""
+    vmat_deriv1_grid, the keys its consumer reads (hess_rks_becke.rs:1389-1401)."""
     import numpy as np
     import torch
     import hesship

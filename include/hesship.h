@@ -230,7 +230,7 @@ hh_status hh_becke_partition(hh_becke* b, const double* grid_coords, int64_t ngr
                              const double* contract_ddw, int64_t nset_ddw, double* ddc);
 /* The setup with the Becke grid shift (make_hessian_setup_becke, hess_rks_becke.rs:1106-1266;
  * make_hessian_setup_becke_uks, hess_uks_becke.rs:726-930), as consumed by
- * RHessKSNIMatmulBecke::make_skeleton_hess / get_deriv1_ao (hess_rks_becke.rs:1387-1401): the
+ * RHessKSNIMatmulBecke::make_skeleton_hess / get_deriv1_ao (hess_rks_becke.rs:1389-1401): the
  * handle's grid is grouped by generating atom (atm_quad_split HOST [natm + 1]) and its weights are
  * the partitioned ones; grid_coords DEVICE [ng, 3]; quadrature_weights DEVICE [ng] (before the
  * partition); exc DEVICE [ng] the energy per particle (energy density exc * sum_s rho_s[0]);

@@ -297,39 +297,55 @@ template <int DEN, int NS>
 __global__ void fxc_ops_kernel(const double* __restrict__ ao, int64_t ng, int64_t nao, int64_t g0, int64_t gb,
                                int64_t ldg, const double* __restrict__ W, int s, int64_t natm,
                                double* __restrict__ AW) {
+  // one thread per (g, group of FX_MU AOs): the W values of a point are read once per group
+  constexpr int FX_MU = 4;
   constexpr int NV = Den<DEN>::NV;
   constexpr int NC = DEN == DEN_RHO ? 1 : 4;
+  const int64_t nq = (nao + FX_MU - 1) / FX_MU;
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= ldg * nao) return;
-  const int64_t g = e % ldg, mu = e / ldg;
+  if (e >= ldg * nq) return;
+  const int64_t g = e % ldg, mu0 = (e / ldg) * FX_MU;
+  const int nm = (int)(nao - mu0 < FX_MU ? nao - mu0 : FX_MU);
   const int64_t blk = ldg * nao * 3 * natm;
   if (g >= gb) {  // K padding rows
-    for (int64_t tA = 0; tA < 3 * natm; ++tA) {
-      const int64_t o = g + ldg * (mu + nao * tA);
-      AW[o] = 0.0;
-      if constexpr (DEN == DEN_TAU) {
+    for (int64_t tA = 0; tA < 3 * natm; ++tA)
+      for (int i = 0; i < nm; ++i) {
+        const int64_t o = g + ldg * (mu0 + i + nao * tA);
+        AW[o] = 0.0;
+        if constexpr (DEN == DEN_TAU) {
 #pragma unroll
-        for (int r = 1; r < 4; ++r) AW[o + r * blk] = 0.0;
+          for (int r = 1; r < 4; ++r) AW[o + r * blk] = 0.0;
+        }
       }
-    }
     return;
   }
   const int64_t sa = ng * nao;
-  const double* a = ao + (g0 + g) + ng * mu;
-  double ac[4];
+  double ac[FX_MU][4];
 #pragma unroll
-  for (int c = 0; c < NC; ++c) ac[c] = a[c * sa];
+  for (int i = 0; i < FX_MU; ++i) {
+    const double* a = ao + (g0 + g) + ng * (mu0 + (i < nm ? i : 0));
+#pragma unroll
+    for (int c = 0; c < NC; ++c) ac[i][c] = a[c * sa];
+  }
   for (int64_t tA = 0; tA < 3 * natm; ++tA) {
     const double* Wp = W + g + gb * NV * (s + NS * tA);  // component c at Wp[gb * c]
-    double v = 0.5 * Wp[0] * ac[0];
+    double wc[NC];
+    wc[0] = 0.5 * Wp[0];
 #pragma unroll
-    for (int c = 1; c < NC; ++c) v += Wp[gb * c] * ac[c];
-    const int64_t o = g + ldg * (mu + nao * tA);
-    AW[o] = v;
-    if constexpr (DEN == DEN_TAU) {
-      const double w4 = 0.25 * Wp[gb * 4];
+    for (int c = 1; c < NC; ++c) wc[c] = Wp[gb * c];
+    const double w4 = DEN == DEN_TAU ? 0.25 * Wp[gb * (NV - 1)] : 0.0;
 #pragma unroll
-      for (int r = 1; r < 4; ++r) AW[o + r * blk] = w4 * ac[r];
+    for (int i = 0; i < FX_MU; ++i) {
+      if (i >= nm) break;
+      double v = wc[0] * ac[i][0];
+#pragma unroll
+      for (int c = 1; c < NC; ++c) v += wc[c] * ac[i][c];
+      const int64_t o = g + ldg * (mu0 + i + nao * tA);
+      AW[o] = v;
+      if constexpr (DEN == DEN_TAU) {
+#pragma unroll
+        for (int r = 1; r < 4; ++r) AW[o + r * blk] = w4 * ac[i][r];
+      }
     }
   }
 }
@@ -572,6 +588,90 @@ __global__ void vmat_grid_final_kernel(const double* __restrict__ Cfx, const dou
   out[e] = v;
 }
 
+// ---- products with a small output: C[a, b] += sum_k X[k + ld a] Y[k + ld b], a < na, b < nb <= 96 ----
+// (the n3 x n3 contractions over grid points and rho components).  One workgroup per K chunk holds
+// the whole output in registers, a thread owning a 6 x 6 sub-block; the chunk partials are summed
+// in a fixed order by skinny_reduce_kernel.
+constexpr int SK_T = 32;  // K tile
+__global__ void __launch_bounds__(256) skinny_kernel(const double* __restrict__ X, const double* __restrict__ Y,
+                                                     int64_t ld, int64_t K, int na, int nb, int64_t kchunk,
+                                                     double* __restrict__ part) {
+  __shared__ double xs[SK_T][96 + 1], ys[SK_T][96 + 1];
+  const int tid = threadIdx.x;
+  const int ra = (tid % 16) * 6, rb = (tid / 16) * 6;
+  const int64_t k0 = blockIdx.x * kchunk, k1 = min(K, k0 + kchunk);
+  double acc[6][6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) acc[i][j] = 0.0;
+  for (int64_t kt = k0; kt < k1; kt += SK_T) {
+    for (int e = tid; e < SK_T * 96; e += 256) {
+      const int kk = e % SK_T, c = e / SK_T;
+      const int64_t k = kt + kk;
+      xs[kk][c] = (k < k1 && c < na) ? X[k + ld * c] : 0.0;
+      ys[kk][c] = (k < k1 && c < nb) ? Y[k + ld * c] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < SK_T; ++kk) {
+      double x[6], y[6];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        x[i] = xs[kk][ra + i];
+        y[i] = ys[kk][rb + i];
+      }
+#pragma unroll
+      for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) acc[i][j] += x[i] * y[j];
+    }
+    __syncthreads();
+  }
+  double* P = part + blockIdx.x * (int64_t)na * nb;
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j)
+      if (ra + i < na && rb + j < nb) P[(ra + i) + na * (rb + j)] = acc[i][j];
+}
+
+// 64 outputs per workgroup, the chunks split over 4 thread rows and summed in LDS (fixed order)
+__global__ void __launch_bounds__(256) skinny_reduce_kernel(const double* __restrict__ part, int nchunk, int na,
+                                                            int nb, double alpha, double* __restrict__ C,
+                                                            int64_t ldc) {
+  __shared__ double red[4][64];
+  const int el = threadIdx.x & 63, row = threadIdx.x >> 6;
+  const int e = blockIdx.x * 64 + el;
+  const int nn = na * nb;
+  double v0 = 0.0, v1 = 0.0;
+  if (e < nn) {
+    int c = row;
+    for (; c + 4 < nchunk; c += 8) {
+      v0 += part[(int64_t)c * nn + e];
+      v1 += part[(int64_t)(c + 4) * nn + e];
+    }
+    if (c < nchunk) v0 += part[(int64_t)c * nn + e];
+  }
+  red[row][el] = v0 + v1;
+  __syncthreads();
+  if (row == 0 && e < nn) C[(e % na) + ldc * (e / na)] += alpha * (red[0][el] + red[1][el] + red[2][el] + red[3][el]);
+}
+
+void skinny_tn(int na, int nb, int64_t K, const double* X, const double* Y, int64_t ld, double* C, int64_t ldc,
+               double alpha, hipStream_t s, DevBuf& ws) {
+  HH_REQUIRE(na <= 96 && nb <= 96, "skinny product: output dimensions above 96");
+  if (K <= 0) return;
+  int64_t kchunk = std::max<int64_t>(SK_T * 4, ceil_div(ceil_div(K, 256), SK_T) * SK_T);
+  const int nchunk = (int)ceil_div(K, kchunk);
+  double* part = static_cast<double*>(ws.reserve(sizeof(double) * (size_t)nchunk * na * nb));
+  hipLaunchKernelGGL(skinny_kernel, dim3((unsigned)nchunk), dim3(256), 0, s, X, Y, ld, K, na, nb, kchunk, part);
+  HH_HIP(hipGetLastError());
+  hipLaunchKernelGGL(skinny_reduce_kernel, dim3((unsigned)ceil_div(na * nb, 64)), dim3(256), 0, s, part, nchunk, na,
+                     nb, alpha, C, ldc);
+  HH_HIP(hipGetLastError());
+}
+
 // C[m, n] += alpha sum_k A[k, m] B[k, n] (+ A2/B2 over the second K range), FP64
 void gemm_tn(int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
              int64_t ldc, double alpha, hipStream_t s, DevBuf& ws, int64_t K1 = -1, const double* A2 = nullptr,
@@ -596,6 +696,7 @@ void nt_acc(int64_t M, int64_t N, int64_t ld, const double* A, const double* B, 
   a.M = M; a.N = N; a.K = ld;
   a.A = A; a.lda = ld; a.B = B; a.ldb = ld;
   a.C = C; a.ldc = ldc; a.alpha = alpha; a.beta = 1.0;
+  a.ksplit = -1;
   gemm_nt(a, NT_F64, s, ws);
 }
 
@@ -737,7 +838,8 @@ void Numint::hessian_setup(int den, int nspin, const double* const* dm0, int64_t
                          g0, gb, aod[0], aod[nspin - 1], dsl, natm, fxc, w, drho, W);
     // de_fxc: sum over (g, y, s) of W[(g,y,s), (t,A)] drho[(g,y,s), (s',B)]
     const int64_t Kf = gb * nv * nspin;
-    gemm_tn(n3, n3, Kf, W, Kf, drho, Kf, acc_fxc, n3, 1.0, stream_, ws_);
+    if (n3 <= 96) skinny_tn((int)n3, (int)n3, Kf, W, drho, Kf, acc_fxc, n3, 1.0, stream_, ws2_);
+    else gemm_tn(n3, n3, Kf, W, Kf, drho, Kf, acc_fxc, n3, 1.0, stream_, ws_);
     for (int s = 0; s < nspin; ++s) {
       const double* vxc_s = vxc + ng_ * nv * s;
       HH_SETUP_DISPATCH(den, vxc_diag_kernel, dim3((unsigned)nao_, (unsigned)nsplit), dim3(256), 0, stream_, ao, ng_,
@@ -757,8 +859,8 @@ void Numint::hessian_setup(int den, int nspin, const double* const* dm0, int64_t
                 ng_);
       if (tau) gemm_tn(3 * nao_, nao_, 3 * gb, H2, 3 * gb, AT, 3 * gb, acc_ip(s), 3 * nao_, 1.0, stream_, ws_);
       // fxc part of vmat_deriv1: CfxT[nu, (mu, t, A)]
-      HH_SETUP_DISPATCH_NS(den, nspin, fxc_ops_kernel, dim3(nblocks(ldg * nao_)), dim3(256), 0, stream_, ao, ng_, nao_,
-                           g0, gb, ldg, W, s, natm, AW);
+      HH_SETUP_DISPATCH_NS(den, nspin, fxc_ops_kernel, dim3(nblocks(ldg * ceil_div(nao_, 4))), dim3(256), 0, stream_, ao,
+                           ng_, nao_, g0, gb, ldg, W, s, natm, AW);
       vmat_fx(nblk_aw, stage, AW, ldg, nao_, M_fx, acc_fx(s), stream_, ws_);
     }
   }
@@ -890,23 +992,27 @@ void Numint::hessian_setup_becke(int den, int nspin, const double* const* dm0, i
                            g0, gb, aod[0], aod[nspin - 1], dsl, natm, fxc, w, drho, W);
       const int64_t Kf = gb * nvns;
       auto fxc_vmat = [&](int s, double* C) {
-        HH_SETUP_DISPATCH_NS(den, nspin, fxc_ops_kernel, dim3(nblocks(ldg * nao_)), dim3(256), 0, stream_, ao, ng_,
-                             nao_, g0, gb, ldg, W, s, natm, AW);
+        HH_SETUP_DISPATCH_NS(den, nspin, fxc_ops_kernel, dim3(nblocks(ldg * ceil_div(nao_, 4))), dim3(256), 0, stream_,
+                             ao, ng_, nao_, g0, gb, ldg, W, s, natm, AW);
         vmat_fx(nblk_aw, stage, AW, ldg, nao_, M_fx, C, stream_, ws_);
       };
       // grid-fixed de_fxc and vmat_deriv1 (fxc part)
-      if (out.de_fxc) gemm_tn(n3, n3, Kf, W, Kf, drho, Kf, acc_xf, n3, 1.0, stream_, ws_);
+      auto xprod = [&](double* C) {
+        if (n3 <= 96) skinny_tn((int)n3, (int)n3, Kf, W, drho, Kf, C, n3, 1.0, stream_, ws2_);
+        else gemm_tn(n3, n3, Kf, W, Kf, drho, Kf, C, n3, 1.0, stream_, ws_);
+      };
+      if (out.de_fxc) xprod(acc_xf);
       if (fixed_vmat)
         for (int s = 0; s < nspin; ++s) fxc_vmat(s, acc_ff(s));
       // the point carried by G: total drho and W, their product, then the Phi field
       hipLaunchKernelGGL(shift_fix_kernel, dim3(nblocks(gb * nvns * 3)), dim3(256), 0, stream_, drho, W, gb, nvns,
                          natm, G);
       HH_HIP(hipGetLastError());
-      gemm_tn(n3, n3, Kf, W, Kf, drho, Kf, acc_xt, n3, 1.0, stream_, ws_);
+      xprod(acc_xt);
       hipLaunchKernelGGL(phi_kernel, dim3(nblocks(gb * nvns * n3)), dim3(256), 0, stream_, W, vxc, dw, ng_, g0, gb,
                          nvns, natm);
       HH_HIP(hipGetLastError());
-      gemm_tn(n3, n3, Kf, W, Kf, drho, Kf, acc_x, n3, 1.0, stream_, ws_);
+      xprod(acc_x);
       hipLaunchKernelGGL(cddw_kernel, dim3((unsigned)ceil_div(gb, 32)), dim3(256), 0, stream_, ao, ng_, nao_, g0, gb, aod[0],
                          aod[nspin - 1], nspin, bin.exc, cw);
       HH_HIP(hipGetLastError());

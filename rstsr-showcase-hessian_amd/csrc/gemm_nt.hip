@@ -455,7 +455,22 @@ void launch_nt(const NtArgs& a, hipStream_t s, DevBuf& ws) {
   p.brows = a.b_rows > 0 ? a.b_rows : a.N;
   const int64_t tiles = p.tiles_m * p.tiles_n;
   int ks = a.ksplit;
-  if (ks <= 0) {
+  if (ks < 0) {
+    // balance the last wave of workgroups: the split count in [1, 16] (>= 16 K steps each) whose
+    // workgroup count fills the largest fraction of whole waves over the CUs
+    const int64_t cus = cu_count();
+    double best = -1.0;
+    ks = 1;
+    for (int k = 1; k <= 16; ++k) {
+      if (k > 1 && a.K / ((int64_t)BK * k) < 16) break;
+      const int64_t nwg = tiles * k;
+      const double eff = (double)nwg / (double)(ceil_div(nwg, cus) * cus);
+      if (eff > best + 1e-9) {
+        best = eff;
+        ks = k;
+      }
+    }
+  } else if (ks == 0) {
     // about one workgroup per CU, whole multiple of the XCD count, >= 16 K steps per split
     ks = (int)std::max<int64_t>(1, (int64_t)cu_count() / std::max<int64_t>(tiles, 1));
     while (ks > 1 && a.K / ((int64_t)BK * ks) < 16) --ks;

@@ -34,7 +34,7 @@ struct NtArgs {
   // Tiled operands that are a row range of a larger tiled array: rows per K block of the array
   // (0 = M for A, N for B); the operand pointer then addresses its first row in K block 0.
   int64_t a_rows = 0, b_rows = 0;
-  int ksplit = 0;  // 0: automatic
+  int ksplit = 0;  // 0: automatic; < 0: balance the last wave of workgroups over the CUs
   hipEvent_t ev_begin = nullptr, ev_main_end = nullptr;
 };
 

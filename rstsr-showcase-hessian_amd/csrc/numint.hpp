@@ -100,7 +100,7 @@ class Numint {
                  double* out, double alpha, bool accumulate);
   int64_t ng_, nao_, ncomp_;
   hipStream_t stream_;
-  DevBuf ao_, w_, aob_, akb_, aob2_, akb2_, rho_, ws_;
+  DevBuf ao_, w_, aob_, akb_, aob2_, akb2_, rho_, ws_, ws2_;
   bool bf_ready_ = false;
   int64_t ngp_ = 0, naop_ = 0;
   DevBuf aok_hi_, aok_lo_, aot_hi_, aot_lo_, kh_, kl_, kh2_, kl2_, sh_, sl_, sh2_, sl2_;
