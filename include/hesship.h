@@ -472,6 +472,20 @@ hh_status hh_util_gemm(int64_t M, int64_t N, int64_t K, double alpha, const doub
                        int64_t ldc, int64_t nb0, int64_t nb1, const int64_t* strides6, hh_precision precision,
                        void* stream);
 
+/* The matrix-core NT product C[m,n] = alpha sum_k A[m*lda+k] B[n*ldb+k] + beta C[m+n*ldc] in FP64
+ * (no reference counterpart; the building block of the FP64 contractions).  K a multiple of 32,
+ * lda / ldb multiples of 8.  config 0..2 selects a tile configuration (208x128, 192x128,
+ * 192x192) with split rule ksplit (> 0 fixed, 0 / -1 automatic); config -1 uses the autotuner:
+ * the first product of a shape class (M, N and K rounded up to powers of two) times every
+ * configuration on `stream` and the choice is kept for the process (hh_util_gemm_nt_choice
+ * reports it, config -1 if none).  hh_util_set_autotune(0) keeps the default configuration for
+ * classes not yet tuned. */
+hh_status hh_util_gemm_nt(int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
+                          const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int32_t config,
+                          int32_t ksplit, void* stream);
+hh_status hh_util_set_autotune(int32_t enable);
+hh_status hh_util_gemm_nt_choice(int64_t M, int64_t N, int64_t K, int32_t* config, int32_t* ksplit);
+
 /* Device-to-device copy of `bytes` on `stream` (no reference counterpart; lets foreign
  * callers of hh_krylov_block move data in and out of the solver's buffers). */
 hh_status hh_util_copy(void* dst, const void* src, int64_t bytes, void* stream);

@@ -2,6 +2,7 @@
 #include "becke.hpp"
 #include "elementwise.hpp"
 #include "gemm.hpp"
+#include "gemm_nt.hpp"
 #include "cphf.hpp"
 #include "linalg.hpp"
 #include "mlp.hpp"
@@ -833,6 +834,37 @@ hh_status hh_linalg_trsm_lower(const double* L, int64_t nd, double* X, int64_t n
   DevBuf ws;
   trsm_lower(L, nd, X, n2, ldx, side_right != 0, (hipStream_t)stream, ws);
   HH_HIP(hipStreamSynchronize((hipStream_t)stream));
+  HH_ABI_END
+}
+
+hh_status hh_util_gemm_nt(int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
+                          const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int32_t config,
+                          int32_t ksplit, void* stream) {
+  HH_ABI_BEGIN
+  static thread_local DevBuf* ws = new DevBuf;
+  NtArgs a;
+  a.M = M; a.N = N; a.K = K;
+  a.A = A; a.lda = lda; a.B = B; a.ldb = ldb;
+  a.C = C; a.ldc = ldc; a.alpha = alpha; a.beta = beta;
+  a.ksplit = ksplit;
+  if (config < 0) gemm_nt(a, NT_F64, (hipStream_t)stream, *ws);
+  else gemm_nt_f64_config(a, config, (hipStream_t)stream, *ws);
+  HH_ABI_END
+}
+
+hh_status hh_util_set_autotune(int32_t enable) {
+  HH_ABI_BEGIN
+  gemm_nt_set_autotune(enable != 0);
+  HH_ABI_END
+}
+
+hh_status hh_util_gemm_nt_choice(int64_t M, int64_t N, int64_t K, int32_t* config, int32_t* ksplit) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(config && ksplit, "null output");
+  int c = -1, k = 0;
+  if (!gemm_nt_choice(M, N, K, &c, &k)) c = -1;
+  *config = c;
+  *ksplit = k;
   HH_ABI_END
 }
 

@@ -11,6 +11,10 @@
 // of that split are then fetched from HBM about once per XCD and reused from its L2.
 #include "gemm_nt.hpp"
 
+#include <map>
+#include <mutex>
+#include <tuple>
+
 namespace hh {
 
 namespace {
@@ -499,7 +503,105 @@ void launch_nt(const NtArgs& a, hipStream_t s, DevBuf& ws) {
   }
 }
 
+// ---- autotuner (F64 tile configurations) ----
+// Candidates (WGM, WGN, FM, FN): 0 = 208 x 128 (two tiles cover M = 400), 1 = 192 x 128,
+// 2 = 192 x 192 (2 x 4 waves); each with the two automatic split rules (0: about one workgroup per
+// CU, -1: fill the last wave).  The first product of a shape class (M exactly, N and K rounded up
+// to powers of two) times every candidate once, after one warm-up launch, on at most 65536 of its
+// K indices, into a scratch output on the caller's stream, and keeps the fastest for the process
+// (the CPHF solves shrink N from cycle to cycle: without the classes and the K cap every new width
+// would pay a full tuning pass); products issued while the stream is being captured into a graph
+// use the cached choice or the default without timing.
+constexpr int NT_F64_NCFG = 3;
+
+void launch_f64_cfg(int cfg, const NtArgs& a, hipStream_t s, DevBuf& ws) {
+  switch (cfg) {
+    case 1: launch_nt<1, 8, 12, 1, false>(a, s, ws); break;
+    case 2: launch_nt<2, 4, 6, 3, false>(a, s, ws); break;
+    default: launch_nt<1, 8, 13, 1, false>(a, s, ws);
+  }
+}
+
+struct NtChoice {
+  int cfg, ks;
+};
+typedef std::tuple<int64_t, int64_t, int64_t, int64_t, int64_t> NtKey;
+std::mutex g_tune_mu;
+std::map<NtKey, NtChoice>* g_tune_cache = new std::map<NtKey, NtChoice>;
+bool g_autotune = true;
+
+int64_t pow2_ceil(int64_t x) {
+  int64_t p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+NtKey shape_class(int64_t M, int64_t N, int64_t K) { return NtKey(M, pow2_ceil(N), pow2_ceil(K), 0, 0); }
+
+NtChoice tuned_f64(const NtArgs& a, hipStream_t s, DevBuf& ws) {
+  const NtKey key = shape_class(a.M, a.N, a.K);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune_cache->find(key);
+  if (it != g_tune_cache->end()) return it->second;
+  NtChoice best{0, a.ksplit};
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  HH_HIP(hipStreamIsCapturing(s, &cap));
+  if (!g_autotune || cap != hipStreamCaptureStatusNone || a.K == 0) return best;
+  static DevBuf* scratch = new DevBuf;
+  double* C = static_cast<double*>(scratch->reserve(sizeof(double) * a.M * a.N));
+  hipEvent_t e0, e1;
+  HH_HIP(hipEventCreate(&e0));
+  HH_HIP(hipEventCreate(&e1));
+  float best_ms = 1e30f;
+  for (int cfg = 0; cfg < NT_F64_NCFG; ++cfg)
+    for (int ks : {0, -1}) {
+      NtArgs t = a;
+      t.K = std::min<int64_t>(a.K, 65536);
+      t.C = C; t.ldc = a.M; t.alpha = 1.0; t.beta = 0.0; t.ksplit = ks;
+      t.ev_begin = t.ev_main_end = nullptr;
+      if (cfg == 0 && ks == 0) launch_f64_cfg(cfg, t, s, ws);
+      HH_HIP(hipEventRecord(e0, s));
+      launch_f64_cfg(cfg, t, s, ws);
+      HH_HIP(hipEventRecord(e1, s));
+      HH_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HH_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = NtChoice{cfg, ks};
+      }
+    }
+  HH_HIP(hipEventDestroy(e0));
+  HH_HIP(hipEventDestroy(e1));
+  (*g_tune_cache)[key] = best;
+  return best;
+}
+
 }  // namespace
+
+void gemm_nt_set_autotune(bool on) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_autotune = on;
+}
+
+bool gemm_nt_choice(int64_t M, int64_t N, int64_t K, int* cfg, int* ksplit) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune_cache->find(shape_class(M, N, K));
+  if (it == g_tune_cache->end()) return false;
+  *cfg = it->second.cfg;
+  *ksplit = it->second.ks;
+  return true;
+}
+
+void gemm_nt_f64_config(const NtArgs& a, int cfg, hipStream_t s, DevBuf& ws) {
+  HH_REQUIRE(cfg >= 0 && cfg < NT_F64_NCFG, "gemm_nt: unknown F64 tile configuration");
+  HH_REQUIRE(a.M >= 0 && a.N >= 0 && a.K >= 0, "gemm_nt: negative size");
+  if (a.M == 0 || a.N == 0) return;
+  HH_REQUIRE(a.C && (a.K == 0 || (a.A && a.B)), "gemm_nt: null operand");
+  HH_REQUIRE(a.K % 32 == 0 && a.lda % 8 == 0 && a.ldb % 8 == 0 && !a.tiled,
+             "gemm_nt: K must be a multiple of 32 and lda, ldb multiples of 8");
+  launch_f64_cfg(cfg, a, s, ws);
+}
 
 void gemm_nt(const NtArgs& a, NtFormat fmt, hipStream_t s, DevBuf& ws) {
   HH_REQUIRE(a.M >= 0 && a.N >= 0 && a.K >= 0, "gemm_nt: negative size");
@@ -517,7 +619,14 @@ void gemm_nt(const NtArgs& a, NtFormat fmt, hipStream_t s, DevBuf& ws) {
     else launch_nt<1, 8, 13, 2, true, false>(a, s, ws);
   } else {
     HH_REQUIRE(!a.tiled, "gemm_nt: tiled operands are BF16X3 only");
-    launch_nt<1, 8, 13, 1, false>(a, s, ws);
+    if (a.ksplit > 0) {
+      launch_nt<1, 8, 13, 1, false>(a, s, ws);
+    } else {
+      const NtChoice c = tuned_f64(a, s, ws);
+      NtArgs t = a;
+      t.ksplit = c.ks;
+      launch_f64_cfg(c.cfg, t, s, ws);
+    }
   }
 }
 

@@ -66,3 +66,61 @@ def test_gemm_mixed_precision(dev, prec, tol, ak, bk):
     assert err < tol, err
     # and it is genuinely reduced precision (not silently FP64)
     assert err > 1e-12
+
+
+# ---- NT product (hh_util_gemm_nt): tile configurations and the autotuner ----
+
+def run_nt(M, N, K, config, ksplit, alpha=1.0, beta=0.0, dev=None, seed=0, pad=8):
+    from hesship import _lib
+    gen = torch.Generator().manual_seed(seed)
+    lda = ldb = K + pad
+    A = torch.randn(M, lda, generator=gen, dtype=torch.float64).to(dev)
+    B = torch.randn(N, ldb, generator=gen, dtype=torch.float64).to(dev)
+    ldc = M + 3
+    Cm = torch.randn(N, ldc, generator=gen, dtype=torch.float64).to(dev)
+    ref = alpha * (A[:, :K] @ B[:, :K].T) + beta * Cm[:, :M].T
+    _lib.check(_lib.lib().hh_util_gemm_nt(M, N, K, alpha, A.data_ptr(), lda, B.data_ptr(), ldb, beta, Cm.data_ptr(),
+                                          ldc, config, ksplit, torch.cuda.current_stream().cuda_stream))
+    torch.cuda.synchronize()
+    got = Cm[:, :M].T
+    scale = (A[:, :K].abs() @ B[:, :K].abs().T).max().item() * abs(alpha) + 1e-300
+    return (got - ref).abs().max().item() / scale
+
+
+@pytest.mark.parametrize("config", [0, 1, 2])
+@pytest.mark.parametrize("ksplit", [1, 3, 0, -1])
+@pytest.mark.parametrize("shape", [(400, 300, 1024), (192, 1000, 2048), (37, 29, 96), (576, 576, 640)])
+def test_gemm_nt_configs(dev, config, ksplit, shape):
+    M, N, K = shape
+    assert run_nt(M, N, K, config, ksplit, alpha=0.7, beta=-1.3, dev=dev) < 1e-13
+
+
+def test_gemm_nt_autotuner(dev):
+    import ctypes as C
+    from hesship import _lib
+    L = _lib.lib()
+    cfg, ks = C.c_int32(), C.c_int32()
+    M, N, K = 208, 640, 3072
+    _lib.check(L.hh_util_gemm_nt_choice(M, N, K, C.byref(cfg), C.byref(ks)))
+    assert cfg.value == -1
+    assert run_nt(M, N, K, -1, 0, beta=0.5, dev=dev) < 1e-13      # tunes, then the product itself
+    _lib.check(L.hh_util_gemm_nt_choice(M, N, K, C.byref(cfg), C.byref(ks)))
+    assert cfg.value in (0, 1, 2) and ks.value in (0, -1)
+    first = (cfg.value, ks.value)
+    assert run_nt(M, N, K, -1, 0, beta=0.5, dev=dev, seed=1) < 1e-13  # cached choice
+    _lib.check(L.hh_util_gemm_nt_choice(M, N, K, C.byref(cfg), C.byref(ks)))
+    assert (cfg.value, ks.value) == first
+    # a shape of the same class (N, K rounded up to powers of two) shares the choice
+    _lib.check(L.hh_util_gemm_nt_choice(M, 600, 2100, C.byref(cfg), C.byref(ks)))
+    assert (cfg.value, ks.value) == first
+    # with the tuner off a new class keeps the default configuration and is not cached
+    _lib.check(L.hh_util_set_autotune(0))
+    try:
+        assert run_nt(160, 96, 512, -1, 0, dev=dev) < 1e-13
+        _lib.check(L.hh_util_gemm_nt_choice(160, 96, 512, C.byref(cfg), C.byref(ks)))
+        assert cfg.value == -1
+    finally:
+        _lib.check(L.hh_util_set_autotune(1))
+    with pytest.raises(_lib.HesshipError, match="configuration"):
+        _lib.check(L.hh_util_gemm_nt(32, 32, 32, 1.0, None, 40, None, 40, 0.0, None, 32, 5, 0,
+                                     torch.cuda.current_stream().cuda_stream))
