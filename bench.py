@@ -15,7 +15,8 @@ against the FP64 result is reported.
 each, same fields): `numint` (RKS grid response at the (H2O)_8 B3LYP sizes on synthetic AO
 values), `dft_setup` (DFT skeleton setup passes at the same sizes, FP64), `dft_becke` (the same
 setup with the Becke grid shift on an atom-grouped grid, FP64), `skeleton` (RI-JK skeleton
-Hessian and first-derivative terms on synthetic integrals, FP64).
+Hessian and first-derivative terms on synthetic integrals, FP64), `cphf_batch` (64 uneven
+alkane-sized RI-JK CPHF solves on one stream, eager vs HIP-graph replay of the response).
 """
 import argparse
 import json
@@ -360,6 +361,68 @@ def workload_dft_becke(args):
     return line
 
 
+def workload_cphf_batch(args):
+    """Many uneven molecules: 64 seeded alkane-sized RI-JK CPHF systems (C_nH_2n+2 with
+    cc-pVDZ / -jkfit counts: nao 24n+10, naux 84n+28, nocc 4n+1, 3(3n+2) perturbations, n uniform
+    in 1..12), synthetic tensors, FP64 CPHF to 1e-8 on one created stream, solved one after the
+    other with the response operator run eagerly and then replayed from HIP graphs."""
+    import numpy as np
+    import torch
+    import hesship
+    from hesship import synth
+    dev = torch.device("cuda:0")
+    ns = np.random.default_rng(64).integers(1, 13, 64)
+    stream = torch.cuda.Stream(device=dev)
+    mols = []
+    with torch.cuda.stream(stream):
+        for i, n in enumerate(ns):
+            nao, naux, nocc = 24 * int(n) + 10, 84 * int(n) + 28, 4 * int(n) + 1
+            nprop = 3 * (3 * int(n) + 2)
+            cd = torch.from_numpy(synth.cderi(nao, naux, seed=synth.SEED_B + 16 * i).T.copy()).to(dev).T
+            C = torch.from_numpy(synth.mo_coeff(nao, seed=synth.SEED_C + 16 * i)).to(dev)
+            occ = np.where(np.arange(nao) < nocc, 2.0, 0.0)
+            probe = hesship.RHessRIJK(cd, 1.0, 1.0)
+            probe.make_response_preparation(C, occ)
+            _, _, _, _, ratio = cphf_case(probe, C, occ, nao, nocc, nprop, dev)
+            del probe
+            obj = hesship.RHessRIJK(cd * (0.3 / ratio) ** 0.5, 1.0, 1.0)
+            obj.make_response_preparation(C, occ)
+            drv, _, f1, s1, _ = cphf_case(obj, C, occ, nao, nocc, nprop, dev)
+            mols.append((obj, drv, f1, s1))
+        torch.cuda.synchronize()
+        out = {}
+        for mode in ("eager", "graphs"):
+            for _, drv, _, _ in mols:
+                drv.set_graphs(mode == "graphs")
+            if mode == "eager":     # warm-up pass: sizes every buffer and tunes the products
+                for _, drv, f1, s1 in mols:
+                    run_cphf(drv, f1, s1, "fp64")
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            cycles = 0
+            for _, drv, f1, s1 in mols:
+                _, _, _, st = run_cphf(drv, f1, s1, "fp64")
+                cycles += st.get("cycles", 0)
+            torch.cuda.synchronize()
+            out[mode] = (time.perf_counter() - t0, cycles)
+            log(f"[bench] cphf batch {mode}: {out[mode][0]:.3f} s, {cycles} Krylov cycles")
+        caps = sum(drv.graph_stats()[0] for _, drv, _, _ in mols)
+        launches = sum(drv.graph_stats()[1] for _, drv, _, _ in mols)
+    wall = out["graphs"][0]
+    line = _line("CPHF batch wall s (64 uneven RI-JK molecules, FP64, graph replay)", round(wall, 4), "s",
+                 wall * 1e3, 1, 1, "f64",
+                 {"workload": "64 alkane-sized synthetic RI-JK CPHF systems solved back to back on one stream",
+                  "n_carbon": [int(n) for n in ns], "cphf_tol": 1e-8},
+                 {"bound": "launch", "achieved": None, "peak": None, "unit": None, "frac": None, "traffic": None,
+                  "kernel": "whole solves"}, higher=False)
+    line["eager_wall_s"] = round(out["eager"][0], 4)
+    line["speedup_graphs_vs_eager"] = round(out["eager"][0] / wall, 3)
+    line["krylov_cycles"] = out["graphs"][1]
+    line["graphs_captured"] = caps
+    line["graph_launches"] = launches
+    return line
+
+
 def workload_skeleton(args):
     """RI-JK skeleton (make_skeleton_hess + get_deriv1_bra, hess_r.rs:1858-1909) on synthetic
     integrals at (H2O)_8 def2-SVP / jkfit-like sizes, FP64; the derivative integrals are served
@@ -419,7 +482,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cphf", action="store_true")
-    ap.add_argument("--workload", default="ri_jk", choices=["ri_jk", "numint", "dft_setup", "dft_becke", "skeleton"])
+    ap.add_argument("--workload", default="ri_jk", choices=["ri_jk", "numint", "dft_setup", "dft_becke", "skeleton", "cphf_batch"])
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run the library's 16-byte copy kernel over GIB GiB three times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes) and exit")
@@ -437,7 +500,7 @@ def main():
 
     if args.workload != "ri_jk":
         fn = {"numint": workload_numint, "dft_setup": workload_dft_setup, "dft_becke": workload_dft_becke,
-              "skeleton": workload_skeleton}[args.workload]
+              "skeleton": workload_skeleton, "cphf_batch": workload_cphf_batch}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
 

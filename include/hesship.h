@@ -306,6 +306,14 @@ hh_status hh_cphf_destroy(hh_cphf* h);
 /* Per-sweep / per-cycle diagnostics on stderr (the reference prints per Krylov cycle,
  * krylov_block.rs:148-151). */
 hh_status hh_cphf_set_verbose(hh_cphf* h, int32_t on);
+/* Replay the response operator from HIP graphs (default on; no reference counterpart): within a
+ * solve, an operator call with the same operands as an earlier one (the Krylov trial block of a
+ * given width) is captured once and replayed as one graph launch.  Applies when the driver runs on
+ * a created stream (the legacy default stream cannot be captured) and every component enqueues on
+ * it; graphs are dropped whenever a library buffer is freed.
+ * hh_cphf_graph_stats: graphs captured and graph launches so far. */
+hh_status hh_cphf_set_graphs(hh_cphf* h, int32_t on);
+hh_status hh_cphf_graph_stats(hh_cphf* h, int64_t* captures, int64_t* launches);
 /* Add an RI-JK component (nspin 1 for the restricted driver, 2 for the unrestricted one,
  * prepared with the same MOs) to the response list (the reference's el_list, rscf.rs:28,
  * uscf.rs:24).  The component must outlive the driver. */

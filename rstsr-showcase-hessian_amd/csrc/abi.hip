@@ -32,6 +32,7 @@ struct RijkResponse : hh::ElecResponse {
                     hh::Prec prec) override {
     r->get_response_bra(bra, nprop, out, prec, accumulate);
   }
+  hipStream_t stream() const override { return r->stream(); }
 };
 
 // RKS grid response as a CPHF component: mocc = the driver's first nocc MO columns
@@ -48,6 +49,7 @@ struct NumintResponse : hh::ElecResponse {
                     hh::Prec prec) override {
     ni->rks_response_bra(den, fxc, bra[0], nprop, mocc, nocc, out[0], prec, accumulate);
   }
+  hipStream_t stream() const override { return ni->stream(); }
 };
 
 // UKS grid response as a CPHF component: mocc[s] = the driver's first nocc[s] MO columns of spin s
@@ -65,6 +67,7 @@ struct UksNumintResponse : hh::ElecResponse {
                     hh::Prec prec) override {
     ni->uks_response_bra(den, fxc, bra, nprop, mocc, nocc, out, prec, accumulate);
   }
+  hipStream_t stream() const override { return ni->stream(); }
 };
 
 hh::CphfConfig cphf_config(double level_shift, double tol, int32_t max_cycle, int32_t max_space, double lindep) {
@@ -294,6 +297,21 @@ hh_status hh_cphf_set_verbose(hh_cphf* h, int32_t on) {
   HH_ABI_BEGIN
   HH_REQUIRE(h != nullptr, "handle is null");
   h->impl->verbose = on != 0;
+  HH_ABI_END
+}
+
+hh_status hh_cphf_set_graphs(hh_cphf* h, int32_t on) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h != nullptr, "handle is null");
+  h->impl->use_graphs = on != 0;
+  HH_ABI_END
+}
+
+hh_status hh_cphf_graph_stats(hh_cphf* h, int64_t* captures, int64_t* launches) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && captures && launches, "null argument");
+  *captures = h->impl->graph_captures();
+  *launches = h->impl->graph_launches();
   HH_ABI_END
 }
 

@@ -361,6 +361,91 @@ void Cphf::response_dimless_stacked(const double* x, int64_t nprop, double* resp
   HH_HIP(hipGetLastError());
 }
 
+// ---- the response operator, optionally replayed from HIP graphs ----
+void Cphf::aop_body(const double* x, int64_t nb, double* ax, Prec p, const double* diag, double* ytmp) {
+  if (!diag) {
+    response_dimless_stacked(x, nb, ax, p);
+    return;
+  }
+  // (I + A) D x = (I + A') x with A' x = (D - 1) x + A (D x)
+  const int64_t nmo0 = sp_[0].nmo, nocc0 = sp_[0].nocc;
+  hipLaunchKernelGGL(apply_diag_kernel, dim3(nblk(nmo0 * nocc0 * nb)), dim3(256), 0, stream_, x, diag, nmo0, nocc0,
+                     nb, ytmp);
+  response_dimless_stacked(ytmp, nb, ax, p);
+  hipLaunchKernelGGL(precond_fix_kernel, dim3(nblk(nmo0 * nocc0 * nb)), dim3(256), 0, stream_, x, diag, nmo0, nocc0,
+                     nb, ax);
+  HH_HIP(hipGetLastError());
+}
+
+void Cphf::drop_graphs() {
+  for (auto& g : graphs_)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  graphs_.clear();
+}
+
+Cphf::~Cphf() { drop_graphs(); }
+
+void Cphf::aop(const double* x, int64_t nb, double* ax, Prec p, const double* diag, double* ytmp) {
+  // (the legacy default stream cannot be captured)
+  bool graphs = use_graphs && stream_ != nullptr;
+  for (ElecResponse* c : comps_) graphs = graphs && c->stream() == stream_;
+  if (!graphs) {
+    aop_body(x, nb, ax, p, diag, ytmp);
+    return;
+  }
+  if (graph_gen_ != devbuf_generation().load()) {  // a buffer a graph may refer to was freed
+    drop_graphs();
+    graph_gen_ = devbuf_generation().load();
+  }
+  AopGraph* e = nullptr;
+  for (auto& g : graphs_)
+    if (g.x == x && g.resp == ax && g.nprop == nb && g.prec == (int)p && g.diag == diag && g.ytmp == ytmp) e = &g;
+  if (!e) {
+    if (graphs_.size() >= 32) drop_graphs();  // operands that never repeat: keep the cache small
+    graphs_.push_back(AopGraph{x, ax, nb, (int)p, diag, ytmp, 0, nullptr});
+    e = &graphs_.back();
+  }
+  if (e->exec) {
+    HH_HIP(hipGraphLaunch(e->exec, stream_));
+    ++n_launches_;
+    return;
+  }
+  if (e->uses == 0) {  // first call eager: sizes the buffers and tunes the products
+    e->uses = 1;
+    aop_body(x, nb, ax, p, diag, ytmp);
+    if (graph_gen_ != devbuf_generation().load()) {  // it moved buffers other graphs may refer to
+      drop_graphs();
+      graph_gen_ = devbuf_generation().load();
+    }
+    return;
+  }
+  hipGraph_t g = nullptr;
+  HH_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+  try {
+    aop_body(x, nb, ax, p, diag, ytmp);
+  } catch (...) {
+    (void)hipStreamEndCapture(stream_, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  HH_HIP(hipStreamEndCapture(stream_, &g));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HH_HIP(ie);
+  if (graph_gen_ != devbuf_generation().load()) {  // a buffer moved while capturing: not replayable
+    (void)hipGraphExecDestroy(exec);
+    drop_graphs();
+    graph_gen_ = devbuf_generation().load();
+    aop_body(x, nb, ax, p, diag, ytmp);
+    return;
+  }
+  e->exec = exec;
+  ++n_captures_;
+  HH_HIP(hipGraphLaunch(exec, stream_));
+  ++n_launches_;
+}
+
 void Cphf::response_mo(const double* const* mo1, int64_t nprop, double* const* resp, Prec prec) {
   const double* x = stack_in(mo1, nprop, st_in_);
   double* r = stack_out(resp, nprop, st_out_);
@@ -371,7 +456,7 @@ void Cphf::response_mo(const double* const* mo1, int64_t nprop, double* const* r
 void Cphf::response_dimless(const double* const* mo1, int64_t nprop, double* const* resp, Prec prec) {
   const double* x = stack_in(mo1, nprop, st_in_);
   double* r = stack_out(resp, nprop, st_out_);
-  response_dimless_stacked(x, nprop, r, prec);
+  aop(x, nprop, r, prec, nullptr, nullptr);
   unstack(r, nprop, resp);
 }
 
@@ -393,20 +478,10 @@ void Cphf::solve_stacked(const double* rhs, int64_t nprop, double* mo1, Prec pre
   double* y = static_cast<double*>(ydiag_.reserve(sizeof(double) * blk * 2));
   double* ytmp = y + blk;
   // aop on the (optionally right-preconditioned) system; `p` is the response precision
+  (void)nmo0;
+  (void)nocc0;
   auto make_aop = [&](Prec p) {
-    return [this, p, diag, ytmp, nmo0, nocc0](const double* xv, int64_t nb, double* ax) {
-      if (!diag) {
-        response_dimless_stacked(xv, nb, ax, p);
-        return;
-      }
-      // (I + A) D x = (I + A') x with A' x = (D - 1) x + A (D x)
-      hipLaunchKernelGGL(apply_diag_kernel, dim3(nblk(nmo0 * nocc0 * nb)), dim3(256), 0, stream_, xv, diag, nmo0,
-                         nocc0, nb, ytmp);
-      response_dimless_stacked(ytmp, nb, ax, p);
-      hipLaunchKernelGGL(precond_fix_kernel, dim3(nblk(nmo0 * nocc0 * nb)), dim3(256), 0, stream_, xv, diag, nmo0,
-                         nocc0, nb, ax);
-      HH_HIP(hipGetLastError());
-    };
+    return [this, p, diag, ytmp](const double* xv, int64_t nb, double* ax) { aop(xv, nb, ax, p, diag, ytmp); };
   };
   auto unprecond = [&](const double* yv, double* out, int64_t nb) {
     if (diag) {

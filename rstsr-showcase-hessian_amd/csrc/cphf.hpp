@@ -16,6 +16,7 @@ struct ElecResponse {
   virtual int nspin() const = 0;
   virtual void response_bra(const double* const* bra, int64_t nprop, double* const* out, bool accumulate,
                             Prec prec) = 0;
+  virtual hipStream_t stream() const = 0;  // the stream its work is enqueued on
 };
 
 struct CphfConfig {
@@ -67,6 +68,15 @@ class Cphf {
 
   hipStream_t stream() const { return stream_; }
   bool verbose = false;  // per-sweep diagnostics on stderr (the reference prints per cycle)
+  // Replay the response operator from HIP graphs: an operator call with the same operands (input
+  // and output addresses, width, precision) as an earlier one is captured once and replayed
+  // afterwards (one graph launch instead of its tens of kernel launches).  Used when the driver
+  // runs on a created stream (not the legacy default stream) and every component enqueues on it;
+  // graphs are dropped whenever a library buffer is freed.  Counters: graphs captured and replayed.
+  bool use_graphs = true;
+  int64_t graph_captures() const { return n_captures_; }
+  int64_t graph_launches() const { return n_launches_; }
+  ~Cphf();
 
   // Learned-preconditioner support (one spin; virtual rows only; N = nvir * nocc samples, sample
   // index a + nvir * i).  Features x [N, 3] (float, column-major), standardised over the samples:
@@ -90,6 +100,24 @@ class Cphf {
   const double* stack_in(const double* const* v, int64_t nprop, DevBuf& buf);
   double* stack_out(double* const* v, int64_t nprop, DevBuf& buf);
   void unstack(const double* flat, int64_t nprop, double* const* v);
+
+  // the response operator (x -> resp) through the graph cache
+  void aop(const double* x, int64_t nprop, double* resp, Prec prec, const double* diag, double* ytmp);
+  void aop_body(const double* x, int64_t nprop, double* resp, Prec prec, const double* diag, double* ytmp);
+  struct AopGraph {
+    const double* x;
+    double* resp;
+    int64_t nprop;
+    int prec;
+    const double* diag;
+    double* ytmp;
+    int uses;
+    hipGraphExec_t exec;
+  };
+  std::vector<AopGraph> graphs_;
+  uint64_t graph_gen_ = 0;
+  int64_t n_captures_ = 0, n_launches_ = 0;
+  void drop_graphs();
 
   int nspin_;
   int64_t nao_, n_ = 0;

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <stdexcept>
@@ -34,6 +35,13 @@ void set_last_error(const std::string& msg);
   } while (0)
 
 // Owning device allocation (grown on demand, never shrunk).
+// Counts frees of device buffers: a captured graph that refers to library buffers is valid only
+// while the count is unchanged.
+inline std::atomic<uint64_t>& devbuf_generation() {
+  static std::atomic<uint64_t> g{0};
+  return g;
+}
+
 struct DevBuf {
   void* ptr = nullptr;
   size_t bytes = 0;
@@ -42,7 +50,10 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { release(); }
   void release() {
-    if (ptr) (void)hipFree(ptr);
+    if (ptr) {
+      (void)hipFree(ptr);
+      ++devbuf_generation();
+    }
     ptr = nullptr;
     bytes = 0;
   }

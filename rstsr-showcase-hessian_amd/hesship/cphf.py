@@ -44,6 +44,16 @@ class RHessCPHF:
     def set_verbose(self, on=True):
         check(lib().hh_cphf_set_verbose(self._h, int(bool(on))))
 
+    def set_graphs(self, on=True):
+        """Replay the response operator from HIP graphs (default on)."""
+        check(lib().hh_cphf_set_graphs(self._h, int(bool(on))))
+
+    def graph_stats(self):
+        """(graphs captured, graph launches) so far."""
+        c, l = C.c_int64(), C.c_int64()
+        check(lib().hh_cphf_graph_stats(self._h, C.byref(c), C.byref(l)))
+        return c.value, l.value
+
     def add_el(self, rijk):
         """Append an RHF RI-JK component (prepared) to the response list."""
         check(lib().hh_cphf_add_rijk(self._h, rijk._h))
@@ -159,6 +169,16 @@ class UHessCPHF:
 
     def set_verbose(self, on=True):
         check(lib().hh_cphf_set_verbose(self._h, int(bool(on))))
+
+    def set_graphs(self, on=True):
+        """Replay the response operator from HIP graphs (default on)."""
+        check(lib().hh_cphf_set_graphs(self._h, int(bool(on))))
+
+    def graph_stats(self):
+        """(graphs captured, graph launches) so far."""
+        c, l = C.c_int64(), C.c_int64()
+        check(lib().hh_cphf_graph_stats(self._h, C.byref(c), C.byref(l)))
+        return c.value, l.value
 
     def add_el(self, rijk):
         """Append a UHF RI-JK component (UHessRIJK, prepared) to the response list."""

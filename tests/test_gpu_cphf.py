@@ -108,3 +108,61 @@ def test_right_preconditioner_gives_same_solution(dev):
     d = torch.full((24, 5), 0.8, dtype=torch.float64, device=dev)
     x1 = drv.solve_dimless_cphf(rhs, diag=d).cpu().numpy()
     np.testing.assert_allclose(x1, x0, rtol=1e-6, atol=1e-8)
+
+
+def test_graph_replay_matches_eager(dev):
+    """The response operator replayed from a HIP graph: four calls on the same operand addresses
+    with new input values written (by torch, eagerly) between them, each compared with the eager
+    operator; then whole solves with and without graphs."""
+    stream = torch.cuda.Stream(device=dev)    # graphs need a created stream (not the default one)
+    with torch.cuda.stream(stream):
+        _graph_replay_body(dev)
+    torch.cuda.synchronize()
+    # on the legacy default stream the operator runs eagerly
+    cderi, C_, occ, e, f1, s1, rb = system(seed=5)
+    drv, _ = make(dev, cderi, C_, occ, e)
+    drv.solve_dimless_cphf(T(ocp.dimless_rhs(f1, s1, occ, e), dev))
+    assert drv.graph_stats() == (0, 0)
+
+
+def _graph_replay_body(dev):
+    from hesship import _lib
+    from hesship.tensors import empty_f, fortran
+    cderi, C_, occ, e, f1, s1, rb = system(seed=5)
+    drv, _ = make(dev, cderi, C_, occ, e)
+    L = _lib.lib()
+    nmo, nocc = C_.shape[1], int((occ > 0).sum())
+    nprop = f1.shape[2] * f1.shape[3]
+    x = fortran(torch.zeros((nmo, nocc, nprop), dtype=torch.float64, device=dev))
+    out = empty_f((nmo, nocc, nprop), dev)
+    rng = np.random.default_rng(11)
+    inputs = [rng.standard_normal((nmo, nocc, nprop)) for _ in range(4)]
+
+    def call():
+        _lib.check(L.hh_cphf_response_dimless(drv._h, x.data_ptr(), nprop, out.data_ptr(), 0))
+        torch.cuda.current_stream().synchronize()
+        return out.cpu().numpy().copy()
+
+    drv.set_graphs(False)
+    ref = []
+    for v in inputs:
+        x.copy_(T(v, dev))
+        ref.append(call())
+    drv.set_graphs(True)
+    for v, r in zip(inputs, ref):
+        x.copy_(T(v, dev))
+        x.mul_(2.0).mul_(0.5)          # eager work on the operand between replays
+        got = call()
+        assert rel(got, r) < 1e-14
+    assert drv.graph_stats() == (1, 3)   # eager, capture + launch, two replays
+    # whole solves: identical cycles and solutions
+    rhs = T(ocp.dimless_rhs(f1, s1, occ, e), dev)
+    st0, st1 = {}, {}
+    drv.set_graphs(False)
+    a = drv.solve_dimless_cphf(rhs, stats=st0).cpu().numpy()
+    drv.set_graphs(True)
+    c0 = drv.graph_stats()
+    b = drv.solve_dimless_cphf(rhs, stats=st1).cpu().numpy()
+    c1 = drv.graph_stats()
+    assert st0["cycles"] == st1["cycles"] and c1[1] > c0[1]
+    assert rel(b, a) < 1e-13
