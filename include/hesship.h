@@ -290,6 +290,13 @@ hh_status hh_krylov_block(hh_aop_fn aop, void* ctx, const double* b, const doubl
                           double tol, int32_t max_cycle, int32_t max_space, double lindep, double* x,
                           hh_krylov_stats* stats, void* stream);
 
+/* Non-normalised Gram-Schmidt of the columns of V [n, m] in column order with the drop rule
+ * ||v||^2 <= lindep (replaces orth_block, krylov_block.rs:225-253).  Kept columns go to R in
+ * order (R holds m columns; those past the kept ones are left zero or unspecified), their squared
+ * norms to norms_out (host, m entries), the count to *kept.  V is used as scratch.  Synchronous. */
+hh_status hh_krylov_orth_block(double* V, int64_t n, int64_t m, double lindep, double* R, double* norms_out,
+                               int64_t* kept, void* stream);
+
 /* ---------------------------------------------------------------------------------------- */
 /* Dimensionless CPHF driver, restricted (src/hessian/rscf.rs) and unrestricted              */
 /* (src/hessian/uscf.rs) SCF                                                                  */

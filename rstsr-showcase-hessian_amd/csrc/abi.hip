@@ -231,6 +231,18 @@ hh_status hh_rijk_reset_timing(hh_rijk* h) {
   HH_ABI_END
 }
 
+hh_status hh_krylov_orth_block(double* V, int64_t n, int64_t m, double lindep, double* R, double* norms_out,
+                               int64_t* kept, void* stream) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(V && R && norms_out && kept && n > 0 && m >= 0, "orth_block: bad argument");
+  static thread_local DevBuf* ws = new DevBuf;
+  std::vector<double> nr;
+  const int64_t k = orth_block(V, n, m, lindep, R, 0, nr, (hipStream_t)stream, *ws);
+  for (int64_t i = 0; i < k; ++i) norms_out[i] = nr[i];
+  *kept = k;
+  HH_ABI_END
+}
+
 hh_status hh_krylov_block(hh_aop_fn aop, void* ctx, const double* b, const double* x0, int64_t n, int64_t nset,
                           double tol, int32_t max_cycle, int32_t max_space, double lindep, double* x,
                           hh_krylov_stats* stats, void* stream) {

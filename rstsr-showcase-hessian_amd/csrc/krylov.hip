@@ -2,11 +2,17 @@
 //
 // The basis is kept non-normalised, as in the reference, so the squared norms of the new trial
 // vectors are the convergence signal.  Orthogonalisation of a block (the reference's serial
-// column-by-column MGS, krylov_block.rs:225-253) runs in chunks of CB columns: each chunk is
-// projected against the columns already kept with two classical Gram-Schmidt passes on the
-// matrix cores (CGS2), then the chunk itself is swept column by column (MGS) by one workgroup,
-// which also applies the reference's drop rule (||v||^2 <= lindep) in column order.  Reductions
-// run in a fixed order, so results are reproducible run to run.
+// column-by-column MGS, krylov_block.rs:225-253) runs in groups of up to GB columns: a group is
+// first projected against the columns already kept with two classical Gram-Schmidt passes on
+// the matrix cores (CGS2); inside the group, Gram-Schmidt in column order is the LDL^T
+// factorisation of the group's Gram matrix G = V^T V: the Schur complement d_t of pivot t is the
+// squared norm MGS would find for column t, the reference's drop rule (||v||^2 <= lindep) drops
+// pivot t (its column of L is zeroed, so it projects nothing), and the kept vectors are
+// V L^{-T}.  The factorisation runs in one workgroup on the group's m x m Gram matrix in LDS;
+// the products over the long axis run on the whole chip.  Two such passes (the second on the
+// first pass's output, whose Gram matrix is nearly diagonal) restore orthogonality to working
+// precision; the second pass's Schur complements are the returned norms.  Reductions run in a
+// fixed order, so results are reproducible run to run.
 #include "krylov.hpp"
 
 #include "elementwise.hpp"
@@ -15,58 +21,79 @@
 namespace hh {
 
 namespace {
-constexpr int CB = 8;       // columns per MGS chunk
-constexpr int MGS_T = 1024;
+constexpr int GB = 128;       // columns per Gram group
+constexpr int GLD = GB + 1;   // LDS leading dimension (odd: row walks hit distinct banks)
+constexpr int LDL_T = 1024;
 
-__device__ double block_sum(double v, double* red) {
-  // 1024 threads = 16 wave64
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) red[w] = v;
-  __syncthreads();
-  double s = 0.0;
-  if (threadIdx.x == 0) {
-    for (int i = 0; i < MGS_T / 64; ++i) s += red[i];
-    red[16] = s;
+// LDL^T with the drop rule on the group's Gram matrix G [m, m] (column-major, lower triangle
+// read), m <= GB.  Outputs: Tm [m, m] with column c (c < kept) = row k_c of L^{-1} (k_c = c-th
+// kept pivot), so V Tm holds the kept Gram-Schmidt vectors in order; columns c >= kept are zero.
+// dout[c] = d_{k_c} (c < kept); *kout = kept.
+__global__ __launch_bounds__(LDL_T) void ldl_drop_kernel(const double* __restrict__ G, int m, double lindep,
+                                                         double* __restrict__ Tm, double* __restrict__ dout,
+                                                         int* __restrict__ kout) {
+  __shared__ double S[GB * GLD];
+  __shared__ double dv[GB];
+  __shared__ int kidx[GB];
+  __shared__ unsigned char kf[GB];
+  __shared__ int nkept;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < m * m; e += LDL_T) {
+    const int i = e % m, j = e / m;
+    if (j <= i) S[i + GLD * j] = G[e];
   }
   __syncthreads();
-  s = red[16];
-  __syncthreads();
-  return s;
-}
-
-// One workgroup: sweep cb columns of V (ld n) in order; kept columns go to Rd[:, 0..), their
-// squared norms to nd_out[0..); the count to *kept_out.
-__global__ __launch_bounds__(MGS_T) void chunk_mgs_kernel(double* V, int64_t n, int cb, double lindep, double* Rd,
-                                                         double* nd_out, int* kept_out) {
-  __shared__ double red[17];
-  __shared__ double nrm[CB];
-  int kept = 0;
-  for (int t = 0; t < cb; ++t) {
-    double* v = V + (int64_t)t * n;
-    for (int q = 0; q < kept; ++q) {
-      const double* r = Rd + (int64_t)q * n;
-      double p = 0.0;
-      for (int64_t i = threadIdx.x; i < n; i += MGS_T) p += v[i] * r[i];
-      const double c = block_sum(p, red) / nrm[q];
-      for (int64_t i = threadIdx.x; i < n; i += MGS_T) v[i] -= c * r[i];
-      __syncthreads();
-    }
-    double p = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += MGS_T) p += v[i] * v[i];
-    const double nsq = block_sum(p, red);
-    if (nsq > lindep) {
-      double* r = Rd + (int64_t)kept * n;
-      for (int64_t i = threadIdx.x; i < n; i += MGS_T) r[i] = v[i];
-      if (threadIdx.x == 0) {
-        nrm[kept] = nsq;
-        nd_out[kept] = nsq;
+  for (int t = 0; t < m; ++t) {
+    const double d = S[t + GLD * t];  // final: step t touches only rows / columns > t
+    const bool keep = d > lindep;
+    const int r = m - t - 1;
+    if (keep) {
+      const double inv = 1.0 / d;
+      for (int e = tid; e < r * r; e += LDL_T) {
+        const int i = t + 1 + e % r, j = t + 1 + e / r;
+        if (j <= i) S[i + GLD * j] -= S[i + GLD * t] * S[j + GLD * t] * inv;
       }
-      ++kept;
       __syncthreads();
+      for (int i = t + 1 + tid; i < m; i += LDL_T) S[i + GLD * t] *= inv;
+    } else {
+      for (int i = t + 1 + tid; i < m; i += LDL_T) S[i + GLD * t] = 0.0;
     }
+    if (tid == 0) {
+      dv[t] = d;
+      kf[t] = keep ? 1 : 0;
+    }
+    __syncthreads();
   }
-  if (threadIdx.x == 0) *kept_out = kept;
+  if (tid == 0) {
+    int k = 0;
+    for (int t = 0; t < m; ++t)
+      if (kf[t]) kidx[k++] = t;
+    nkept = k;
+    *kout = k;
+  }
+  __syncthreads();
+  const int kept = nkept;
+  // row k of L^{-1}: x = e_k; for q = k .. 1: x[s] -= L[q][s] x[q] (s < q).  One wave per row,
+  // x[s] held by lane s % 64 in slot s / 64.
+  const int lane = tid & 63, wave = tid >> 6;
+  for (int c = wave; c < m; c += LDL_T / 64) {
+    double* out = Tm + (int64_t)c * m;
+    if (c >= kept) {
+      for (int s2 = lane; s2 < m; s2 += 64) out[s2] = 0.0;
+      continue;
+    }
+    const int k = kidx[c];
+    double x0 = (lane == k) ? 1.0 : 0.0;
+    double x1 = (lane + 64 == k) ? 1.0 : 0.0;
+    for (int q = k; q >= 1; --q) {
+      const double xq = __shfl(q < 64 ? x0 : x1, q & 63, 64);
+      if (lane < q) x0 -= S[q + GLD * lane] * xq;
+      if (lane + 64 < q) x1 -= S[q + GLD * (lane + 64)] * xq;
+    }
+    if (lane < m) out[lane] = x0;
+    if (lane + 64 < m) out[lane + 64] = x1;
+    if (lane == 0) dout[c] = dv[k];
+  }
 }
 
 __global__ void scale_rows_kernel(double* D, int64_t rows, int64_t cols, const double* dn) {
@@ -89,7 +116,7 @@ __global__ void residual_kernel(const double* b, const double* x, const double* 
 unsigned nblk(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 8192)); }
 
 struct Scratch {
-  DevBuf d, dn, dk, h, g, diag, tmp;
+  DevBuf d, dn, dk, h, g, diag, tmp, g2, tm, w;
 };
 Scratch& scratch() {
   static thread_local Scratch* s = new Scratch;  // lives for the process
@@ -137,17 +164,39 @@ void projected_solve_add(const double* xs, const double* ax, const std::vector<d
 int64_t orth_block(double* V, int64_t n, int64_t m, double lindep, double* R, int64_t r_off,
                    std::vector<double>& norms, hipStream_t s, DevBuf& ws) {
   Scratch& sc = scratch();
-  double* D = static_cast<double*>(sc.d.reserve(sizeof(double) * std::max<int64_t>(m, 1) * CB));
-  double* dn = static_cast<double*>(sc.dn.reserve(sizeof(double) * std::max<int64_t>(m, 1)));
+  const int64_t gmax = std::min<int64_t>(std::max<int64_t>(m, 1), GB);
+  double* D = static_cast<double*>(sc.d.reserve(sizeof(double) * std::max<int64_t>(m, 1) * gmax));
+  double* dn = static_cast<double*>(sc.dn.reserve(sizeof(double) * (std::max<int64_t>(m, 1) + GB)));
   int* dk = static_cast<int*>(sc.dk.reserve(64));
+  double* Gm = static_cast<double*>(sc.g2.reserve(sizeof(double) * gmax * gmax));
+  double* Tm = static_cast<double*>(sc.tm.reserve(sizeof(double) * gmax * gmax));
+  double* W = static_cast<double*>(sc.w.reserve(sizeof(double) * n * gmax));
+  double* d1 = dn + std::max<int64_t>(m, 1);  // first-pass pivots (not returned)
   double* Rk = R + r_off * n;
   int64_t kept = 0;
-  std::vector<double> hn(CB);
-  for (int64_t c0 = 0; c0 < m; c0 += CB) {
-    const int cb = (int)std::min<int64_t>(CB, m - c0);
+  std::vector<double> hn(GB);
+  // G = X^T X (cb x cb), then Tm / pivots / count from the LDL kernel, then Y = X Tm
+  auto pass = [&](const double* X, int cb, double* Y, double* dout) {
+    GemmArgs g;
+    g.M = cb; g.N = cb; g.K = n;
+    g.A = X; g.lda = n; g.a_kmajor = true;
+    g.B = X; g.ldb = n; g.b_kmajor = true;
+    g.C = Gm; g.ldc = cb;
+    gemm<double, double>(g, PREC_F64, s, ws);
+    hipLaunchKernelGGL(ldl_drop_kernel, dim3(1), dim3(LDL_T), 0, s, Gm, cb, lindep, Tm, dout, dk);
+    HH_HIP(hipGetLastError());
+    GemmArgs u;
+    u.M = n; u.N = cb; u.K = cb;
+    u.A = X; u.lda = n; u.a_kmajor = false;
+    u.B = Tm; u.ldb = cb; u.b_kmajor = true;
+    u.C = Y; u.ldc = n;
+    gemm<double, double>(u, PREC_F64, s, ws);
+  };
+  for (int64_t c0 = 0; c0 < m; c0 += GB) {
+    const int cb = (int)std::min<int64_t>(GB, m - c0);
     double* Vc = V + c0 * n;
     if (kept > 0) {
-      for (int pass = 0; pass < 2; ++pass) {
+      for (int p = 0; p < 2; ++p) {
         GemmArgs g;  // D = Rk^T Vc
         g.M = kept; g.N = cb; g.K = n;
         g.A = Rk; g.lda = n; g.a_kmajor = true;
@@ -165,8 +214,10 @@ int64_t orth_block(double* V, int64_t n, int64_t m, double lindep, double* R, in
         gemm<double, double>(u, PREC_F64, s, ws);
       }
     }
-    hipLaunchKernelGGL(chunk_mgs_kernel, dim3(1), dim3(MGS_T), 0, s, Vc, n, cb, lindep, Rk + kept * n, dn + kept, dk);
-    HH_HIP(hipGetLastError());
+    // two Gram / LDL passes: Vc -> W (kept columns first, zero columns after), W -> R[:, kept..)
+    // (columns past the kept ones are zero and are overwritten by the next group)
+    pass(Vc, cb, W, d1);
+    pass(W, cb, Rk + kept * n, dn + kept);
     int k_new = 0;
     HH_HIP(hipMemcpyAsync(&k_new, dk, sizeof(int), hipMemcpyDeviceToHost, s));
     HH_HIP(hipStreamSynchronize(s));
@@ -271,12 +322,17 @@ void krylov_block(const Aop& aop, const double* b, const double* x0, int64_t n, 
       // keep only directions above the convergence threshold, in order
       ip.clear();
       int64_t nk = 0;
-      for (int64_t c = 0; c < kept; ++c)
-        if (nip[c] > conv) {
-          d2d(x1 + nk * n, xtmp + c * n, n, s);
-          ip.push_back(nip[c]);
-          ++nk;
+      for (int64_t c = 0; c < kept;) {  // runs of kept columns, one copy each
+        if (!(nip[c] > conv)) {
+          ++c;
+          continue;
         }
+        int64_t e = c;
+        while (e < kept && nip[e] > conv) ip.push_back(nip[e++]);
+        d2d(x1 + nk * n, xtmp + c * n, (e - c) * n, s);
+        nk += e - c;
+        c = e;
+      }
       nb = nk;
       last_max = max_ip;
       if (max_ip < conv) {

@@ -59,6 +59,7 @@ class KrylovStats(C.Structure):
 AOP_FN = C.CFUNCTYPE(C.c_int32, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p)
 SIGNATURES.update({
     "hh_krylov_block": (_i32, [AOP_FN, _p, _p, _p, _i64, _i64, _d, _i32, _i32, _d, _p, C.POINTER(KrylovStats), _p]),
+    "hh_krylov_orth_block": (_i32, [_p, _i64, _i64, _d, _p, _p, _pi64, _p]),
     "hh_cphf_create": (_i32, [_p, _i64, _i64, _p, _p, _d, _d, _i32, _i32, _d, _p, C.POINTER(C.c_void_p)]),
     "hh_cphf_destroy": (_i32, [_p]),
     "hh_cphf_add_rijk": (_i32, [_p, _p]),

@@ -1,6 +1,7 @@
 """Block Krylov solver and dense helpers (reference: src/util/krylov_block.rs)."""
 import ctypes as C
 
+import numpy as np
 import torch
 
 from ._lib import AOP_FN, KrylovStats, check, lib
@@ -43,6 +44,20 @@ def krylov_block(aop, b, x0=None, tol=1e-8, max_cycle=42, max_space=14, lindep=1
         raise err[0]
     check(rc)
     return x
+
+
+def orth_block(vec, lindep=1e-14):
+    """orth_block (krylov_block.rs:225-253) on the device: non-normalised Gram-Schmidt of the
+    columns of vec [n, m] in order, dropping columns with ||v||^2 <= lindep.  Returns (kept
+    columns [n, kept], list of their squared norms)."""
+    n, m = vec.shape
+    V = fortran(vec).clone()
+    R = empty_f((n, max(m, 1)), vec.device)
+    nr = np.zeros(max(m, 1))
+    k = C.c_int64()
+    check(lib().hh_krylov_orth_block(V.data_ptr(), n, m, float(lindep), R.data_ptr(), nr.ctypes.data, C.byref(k),
+                                     stream_of(vec.device)))
+    return R[:, :k.value], list(nr[:k.value])
 
 
 def solve(h, g):

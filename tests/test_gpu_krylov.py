@@ -83,6 +83,26 @@ def test_krylov_larger_block_matches_oracle(dev):
     assert st["cycles"] == so["cycles"]
 
 
+@pytest.mark.parametrize("n,m", [(50, 1), (200, 37), (500, 300)])
+def test_orth_block_matches_oracle(dev, n, m):
+    """Columns over six orders of magnitude, exact and near (1e-9) dependencies, and m = 300
+    (three Gram groups): same kept set, vectors and norms as the serial MGS of the oracle."""
+    import hesship
+    rng = np.random.default_rng(m)
+    V = rng.standard_normal((n, m)) * np.exp(rng.uniform(-7, 0, m))
+    if m > 10:
+        V[:, 3] = 0.5 * V[:, 0] + V[:, 1]
+        V[:, 7] = V[:, 2] + 1e-9 * rng.standard_normal(n)
+        V[:, m - 1] = V[:, 5] - 2.0 * V[:, m - 2]
+    R, nr = hesship.orth_block(T(V, dev), 1e-14)
+    Ro, no = okr.orth_block(V, 1e-14)
+    assert R.shape[1] == Ro.shape[1] == len(nr)
+    np.testing.assert_allclose(R.cpu().numpy(), Ro, rtol=0, atol=1e-12 * np.abs(Ro).max())
+    np.testing.assert_allclose(nr, no, rtol=1e-12)
+    Rn = R.cpu().numpy() / np.sqrt(nr)
+    assert np.abs(Rn.T @ Rn - np.eye(len(nr))).max() < 1e-12
+
+
 @pytest.mark.parametrize("nd,nrhs", [(1, 1), (63, 5), (64, 64), (130, 7), (333, 90)])
 def test_dense_solve(dev, nd, nrhs):
     import hesship
