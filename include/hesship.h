@@ -321,6 +321,13 @@ hh_status hh_cphf_set_verbose(hh_cphf* h, int32_t on);
  * hh_cphf_graph_stats: graphs captured and graph launches so far. */
 hh_status hh_cphf_set_graphs(hh_cphf* h, int32_t on);
 hh_status hh_cphf_graph_stats(hh_cphf* h, int64_t* captures, int64_t* launches);
+/* Number of finalize calls (hh_cphf_finalize / hh_ucphf_finalize) that took the FP64 response of
+ * mo1 from the last mixed-precision solve instead of evaluating it again (no reference
+ * counterpart; the reference's finalize, rscf.rs:328-353, always evaluates it).  The solve's
+ * final FP64 residual is that response; it is reused when the mo1 passed to finalize is
+ * bit-identical to the solve's output and no component was added since (components must not be
+ * re-prepared between the solve and finalize). */
+hh_status hh_cphf_finalize_reuses(hh_cphf* h, int64_t* count);
 /* Add an RI-JK component (nspin 1 for the restricted driver, 2 for the unrestricted one,
  * prepared with the same MOs) to the response list (the reference's el_list, rscf.rs:28,
  * uscf.rs:24).  The component must outlive the driver. */

@@ -327,6 +327,13 @@ hh_status hh_cphf_graph_stats(hh_cphf* h, int64_t* captures, int64_t* launches) 
   HH_ABI_END
 }
 
+hh_status hh_cphf_finalize_reuses(hh_cphf* h, int64_t* count) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && count, "null argument");
+  *count = h->impl->finalize_reuses();
+  HH_ABI_END
+}
+
 hh_status hh_cphf_add_rijk(hh_cphf* h, hh_rijk* rijk) {
   HH_ABI_BEGIN
   HH_REQUIRE(h && rijk, "null handle");

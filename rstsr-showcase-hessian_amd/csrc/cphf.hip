@@ -28,6 +28,21 @@ __global__ void dimless_rhs_kernel(const double* f1, const double* s1, const dou
 
 // resp[vir] = (resp - shift mo1)[vir] / (e_ai + shift); resp[occ] = 0.  Column a of the
 // [nmo, nocc] block starts at ld * a.
+// *flag = 1 when a and b hold the same values (one workgroup, fixed order)
+__global__ __launch_bounds__(1024) void equal_kernel(const double* a, const double* b, int64_t n, int* flag) {
+  __shared__ int bad[16];
+  int d = 0;
+  for (int64_t i = threadIdx.x; i < n; i += 1024) d |= (a[i] != b[i]);
+  for (int o = 32; o > 0; o >>= 1) d |= __shfl_down(d, o, 64);
+  if ((threadIdx.x & 63) == 0) bad[threadIdx.x >> 6] = d;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int any = 0;
+    for (int w = 0; w < 16; ++w) any |= bad[w];
+    *flag = any ? 0 : 1;
+  }
+}
+
 __global__ void dimless_scale_kernel(double* resp, const double* mo1, const double* eocc, const double* evir,
                                      int64_t nmo, int64_t nocc, int64_t nprop, int64_t ld, double shift) {
   const int64_t nn = nmo * nocc, total = nn * nprop;
@@ -266,6 +281,7 @@ Cphf::Cphf(int nspin, const double* const* mo_coeff, int64_t nao, const int64_t*
 void Cphf::add_component(ElecResponse* c) {
   HH_REQUIRE(c && c->nspin() == nspin_, "cphf: component spin count differs from the driver's");
   comps_.push_back(c);
+  fin_nprop_ = -1;
 }
 
 // ---- stacked vector <-> per-spin arrays (one spin: the same memory, no copy) ----
@@ -352,6 +368,10 @@ void Cphf::response_mo_stacked(const double* x, int64_t nprop, double* resp, Pre
 
 void Cphf::response_dimless_stacked(const double* x, int64_t nprop, double* resp, Prec prec) {
   response_mo_stacked(x, nprop, resp, prec);
+  dimless_scale(x, nprop, resp);
+}
+
+void Cphf::dimless_scale(const double* x, int64_t nprop, double* resp) {
   for (int t = 0; t < nspin_; ++t) {
     const Spin& sp = sp_[t];
     hipLaunchKernelGGL(dimless_scale_kernel, dim3(nblk(sp.nmo * sp.nocc * nprop)), dim3(256), 0, stream_,
@@ -475,6 +495,7 @@ void Cphf::solve_stacked(const double* rhs, int64_t nprop, double* mo1, Prec pre
   const int64_t blk = n * nprop;
   const int64_t nmo0 = sp_[0].nmo, nocc0 = sp_[0].nocc;
   CphfStats stats;
+  fin_nprop_ = -1;
   double* y = static_cast<double*>(ydiag_.reserve(sizeof(double) * blk * 2));
   double* ytmp = y + blk;
   // aop on the (optionally right-preconditioned) system; `p` is the response precision
@@ -533,6 +554,7 @@ void Cphf::solve_stacked(const double* rhs, int64_t nprop, double* mo1, Prec pre
   hipLaunchKernelGGL(residual_norm_kernel, dim3((unsigned)nprop), dim3(256), 0, stream_, rhs, mo1, ax, r, n, coln);
   HH_HIP(hipGetLastError());
   double res = max_colnorm2();
+  double* fin_r = static_cast<double*>(fin_r_.reserve(sizeof(double) * blk));
   const double op_tol = (prec == PREC_BF16X3) ? 1e-5 : 1e-7;  // inner target, relative to the rescaled rhs
   const int max_sweeps = 10;
   for (int sweep = 0; sweep < max_sweeps && std::sqrt(res) >= cfg_.tol && res > 0.0; ++sweep) {
@@ -555,7 +577,10 @@ void Cphf::solve_stacked(const double* rhs, int64_t nprop, double* mo1, Prec pre
     unprecond(y, dx, nprop);
     hipLaunchKernelGGL(add_unscaled_kernel, dim3(nblk(blk)), dim3(256), 0, stream_, mo1, dx, n, nprop, csc);
     HH_HIP(hipGetLastError());
-    response_dimless_stacked(mo1, nprop, ax, PREC_F64);
+    // FP64 response of mo1, kept raw for finalize, then put in dimensionless form
+    response_mo_stacked(mo1, nprop, fin_r, PREC_F64);
+    copy_bytes(ax, fin_r, (int64_t)sizeof(double) * blk, stream_);
+    dimless_scale(mo1, nprop, ax);
     ++stats.krylov.aop_calls;
     hipLaunchKernelGGL(residual_norm_kernel, dim3((unsigned)nprop), dim3(256), 0, stream_, rhs, mo1, ax, r, n, coln);
     HH_HIP(hipGetLastError());
@@ -569,6 +594,10 @@ void Cphf::solve_stacked(const double* rhs, int64_t nprop, double* mo1, Prec pre
   }
   stats.residual = std::sqrt(res);
   if (st) *st = stats;
+  if (stats.refinements > 0 && std::sqrt(res) < cfg_.tol) {
+    copy_bytes(fin_x_.reserve(sizeof(double) * blk), mo1, (int64_t)sizeof(double) * blk, stream_);
+    fin_nprop_ = nprop;
+  }
   if (!(std::sqrt(res) < cfg_.tol)) {
     char buf[200];
     snprintf(buf, sizeof buf, "cphf mixed-precision refinement did not reach tol %.3e (residual %.3e after %d sweeps)",
@@ -583,7 +612,22 @@ void Cphf::finalize(const double* const* f1mo, const double* const* s1mo, const 
   // last response in FP64 (per spin, into one scratch block)
   const double* x = stack_in(mo1, nprop, st_in_);
   double* b1 = static_cast<double*>(tmp_.reserve(sizeof(double) * n_ * nprop));
-  response_mo_stacked(x, nprop, b1, PREC_F64);
+  bool reuse = false;
+  if (fin_nprop_ == nprop) {
+    int* flag = static_cast<int*>(fin_flag_.reserve(64));
+    hipLaunchKernelGGL(equal_kernel, dim3(1), dim3(1024), 0, stream_, x, fin_x_.as<double>(), n_ * nprop, flag);
+    HH_HIP(hipGetLastError());
+    int h = 0;
+    HH_HIP(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    HH_HIP(hipStreamSynchronize(stream_));
+    reuse = h == 1;
+  }
+  if (reuse) {
+    copy_bytes(b1, fin_r_.ptr, (int64_t)sizeof(double) * n_ * nprop, stream_);
+    ++n_fin_reuse_;
+  } else {
+    response_mo_stacked(x, nprop, b1, PREC_F64);
+  }
   double* b1s[2] = {b1, nullptr};
   if (nspin_ == 2) {
     double* aux = static_cast<double*>(st_aux_.reserve(sizeof(double) * n_ * nprop));

@@ -76,6 +76,11 @@ class Cphf {
   bool use_graphs = true;
   int64_t graph_captures() const { return n_captures_; }
   int64_t graph_launches() const { return n_launches_; }
+  // finalize() calls that took the FP64 response from the last mixed-precision solve: its final
+  // residual evaluates the FP64 response of the returned mo1, which finalize needs again.  Reused
+  // when the mo1 given to finalize is bit-identical to that solve's output and no component was
+  // added since (components must not be re-prepared in between).
+  int64_t finalize_reuses() const { return n_fin_reuse_; }
   ~Cphf();
 
   // Learned-preconditioner support (one spin; virtual rows only; N = nvir * nocc samples, sample
@@ -96,6 +101,7 @@ class Cphf {
   // The stacked vector [n, nprop], n = sum_s nmo_s nocc_s (one spin: the [nmo, nocc, nprop] array).
   void response_mo_stacked(const double* x, int64_t nprop, double* resp, Prec prec);
   void response_dimless_stacked(const double* x, int64_t nprop, double* resp, Prec prec);
+  void dimless_scale(const double* x, int64_t nprop, double* resp);  // in place, per spin
   void solve_stacked(const double* rhs, int64_t nprop, double* x, Prec prec, CphfStats* st, const double* diag);
   const double* stack_in(const double* const* v, int64_t nprop, DevBuf& buf);
   double* stack_out(double* const* v, int64_t nprop, DevBuf& buf);
@@ -118,6 +124,8 @@ class Cphf {
   uint64_t graph_gen_ = 0;
   int64_t n_captures_ = 0, n_launches_ = 0;
   void drop_graphs();
+  DevBuf fin_x_, fin_r_, fin_flag_;
+  int64_t fin_nprop_ = -1, n_fin_reuse_ = 0;
 
   int nspin_;
   int64_t nao_, n_ = 0;

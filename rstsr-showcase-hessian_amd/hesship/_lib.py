@@ -118,6 +118,7 @@ SIGNATURES.update({
     "hh_util_gemm_nt_choice": (_i32, [_i64, _i64, _i64, C.POINTER(_i32), C.POINTER(_i32)]),
     "hh_cphf_set_graphs": (_i32, [_p, _i32]),
     "hh_cphf_graph_stats": (_i32, [_p, _pi64, _pi64]),
+    "hh_cphf_finalize_reuses": (_i32, [_p, _pi64]),
     "hh_becke_create": (_i32, [_i64, _p, _p, _p, C.POINTER(C.c_void_p)]),
     "hh_becke_destroy": (_i32, [_p]),
     "hh_becke_partition": (_i32, [_p, _p, _i64, _i32, _pi64, _p, _i32, _i32, _p, _p, _p, _p, _i64, _p, _p, _i64, _p,

@@ -54,6 +54,12 @@ class RHessCPHF:
         check(lib().hh_cphf_graph_stats(self._h, C.byref(c), C.byref(l)))
         return c.value, l.value
 
+    def finalize_reuses(self):
+        """finalize calls that reused the FP64 response from the last mixed-precision solve."""
+        c = C.c_int64()
+        check(lib().hh_cphf_finalize_reuses(self._h, C.byref(c)))
+        return c.value
+
     def add_el(self, rijk):
         """Append an RHF RI-JK component (prepared) to the response list."""
         check(lib().hh_cphf_add_rijk(self._h, rijk._h))
@@ -179,6 +185,12 @@ class UHessCPHF:
         c, l = C.c_int64(), C.c_int64()
         check(lib().hh_cphf_graph_stats(self._h, C.byref(c), C.byref(l)))
         return c.value, l.value
+
+    def finalize_reuses(self):
+        """finalize calls that reused the FP64 response from the last mixed-precision solve."""
+        c = C.c_int64()
+        check(lib().hh_cphf_finalize_reuses(self._h, C.byref(c)))
+        return c.value
 
     def add_el(self, rijk):
         """Append a UHF RI-JK component (UHessRIJK, prepared) to the response list."""

@@ -100,6 +100,30 @@ def test_mixed_precision_refinement_reaches_fp64_tolerance(dev, prec):
     assert st["refinements"] >= 1 and st["residual"] < 1e-9
 
 
+def test_finalize_reuses_mixed_solve_response(dev):
+    """finalize after a mixed-precision solve takes the FP64 response of mo1 from the solve's
+    final residual: the same results (to rounding) as evaluating it again (after an FP64 solve
+    the cache is empty), equal to the oracle's finalize; a changed mo1 is not reused."""
+    cderi, C, occ, e, f1, s1, rb = system(seed=8)
+    drv, _ = make(dev, cderi, C, occ, e, cfg=dict(cphf_tol=1e-9))
+    rhs = T(ocp.dimless_rhs(f1, s1, occ, e), dev)
+    F1, S1 = T(f1, dev), T(s1, dev)
+    mo1 = drv.solve_dimless_cphf(rhs, precision="bf16x3")
+    m_a, e_a = drv.finalize_cphf(F1, S1, mo1)
+    assert drv.finalize_reuses() == 1
+    m_o, e_o = ocp.finalize(C, occ, e, [rb], f1, s1, mo1.cpu().numpy())
+    np.testing.assert_allclose(m_a.cpu().numpy(), m_o, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(e_a.cpu().numpy(), e_o, rtol=1e-10, atol=1e-12)
+    bumped = mo1.clone()
+    bumped[7, 1, 0, 0] += 1e-12
+    drv.finalize_cphf(F1, S1, bumped)
+    assert drv.finalize_reuses() == 1
+    drv.solve_dimless_cphf(rhs)  # FP64: no cached response
+    m_b, e_b = drv.finalize_cphf(F1, S1, mo1)
+    assert drv.finalize_reuses() == 1
+    assert rel(m_b.cpu().numpy(), m_a.cpu().numpy()) < 1e-14 and rel(e_b.cpu().numpy(), e_a.cpu().numpy()) < 1e-14
+
+
 def test_right_preconditioner_gives_same_solution(dev):
     cderi, C, occ, e, f1, s1, rb = system(seed=6)
     drv, _ = make(dev, cderi, C, occ, e, cfg=dict(cphf_tol=1e-10))
