@@ -889,6 +889,42 @@ hh_status hh_util_gemm_nt(int64_t M, int64_t N, int64_t K, double alpha, const d
   HH_ABI_END
 }
 
+hh_status hh_util_gemm_nt_bf16x3(int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
+                                 const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int32_t config,
+                                 int32_t ksplit, void* stream) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(M >= 0 && N >= 0 && K >= 0 && K % 32 == 0 && lda >= K && ldb >= K, "gemm_nt_bf16x3: bad sizes");
+  HH_REQUIRE(A && B && C, "null operand");
+  static thread_local DevBuf* ws = new DevBuf;
+  static thread_local DevBuf* sp = new DevBuf;
+  hipStream_t s = (hipStream_t)stream;
+  unsigned short* ah = static_cast<unsigned short*>(sp->reserve(2 * sizeof(unsigned short) * K * (M + N) + 64));
+  unsigned short* al = ah + K * M;
+  unsigned short* bh = al + K * M;
+  unsigned short* bl = bh + K * N;
+  split_bf16_tiled<double>(A, lda, K, M, K, ah, al, s);
+  split_bf16_tiled<double>(B, ldb, K, N, K, bh, bl, s);
+  NtArgs a;
+  a.M = M; a.N = N; a.K = K;
+  a.A = ah; a.A_lo = al; a.lda = K; a.B = bh; a.B_lo = bl; a.ldb = K;
+  a.tiled = true;
+  a.C = C; a.ldc = ldc; a.alpha = alpha; a.beta = beta;
+  a.ksplit = ksplit;
+  if (config < 0) gemm_nt(a, NT_BF16X3, s, *ws);
+  else gemm_nt_bf16x3_config(a, config, s, *ws);
+  HH_ABI_END
+}
+
+hh_status hh_util_gemm_nt_bf16x3_choice(int64_t M, int64_t N, int64_t K, int32_t* config, int32_t* ksplit) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(config && ksplit, "null output");
+  int c = -1, k = 0;
+  if (!gemm_nt_choice(M, N, K, &c, &k, NT_BF16X3)) c = -1;
+  *config = c;
+  *ksplit = k;
+  HH_ABI_END
+}
+
 hh_status hh_util_set_autotune(int32_t enable) {
   HH_ABI_BEGIN
   gemm_nt_set_autotune(enable != 0);

@@ -59,15 +59,19 @@ __device__ __forceinline__ void lds_barrier() {
 // ---------------------------------------------------------------- bf16 x3 ----
 // BMT 16-row tiles per workgroup (wave wm owns tiles wm*FM .. min(BMT, (wm+1)*FM) - 1, so the
 // waves of a column may own unequal row counts); PD: K steps of global loads in flight (1 or 2).
+// 4 or 8 waves per workgroup (waves_per_eu follows: one or two waves per SIMD).
 template <int WGM, int WGN, int FM, int FN, bool TILED, int BMT = WGM * FM, int PD = 2>
-__global__ __launch_bounds__(512) void nt_bf16x3_kernel(NtParams p) {
+__global__ __launch_bounds__(64 * WGM * WGN) __attribute__((amdgpu_waves_per_eu(WGM * WGN / 4)))
+void nt_bf16x3_kernel(NtParams p) {
   static_assert(BMT <= WGM * FM && BMT > (WGM - 1) * FM && (PD == 1 || PD == 2), "tile shape");
+  static_assert(WGM * WGN == 4 || WGM * WGN == 8, "4 or 8 waves");
+  constexpr int NTH = 64 * WGM * WGN;
   constexpr int BM = BMT * 16, BN = WGN * FN * 16, BK = 32;
   constexpr int LD = BK + 8;                 // bf16 elements per LDS row (80 B)
   constexpr int A_ELEMS = BM * LD, B_ELEMS = BN * LD;
   constexpr int STAGE = 2 * A_ELEMS + 2 * B_ELEMS;  // hi, lo for A and B
   constexpr int A_CH = BM * (BK / 8), B_CH = BN * (BK / 8);  // 16-byte chunks per array
-  constexpr int A_PT = (A_CH + 511) / 512, B_PT = (B_CH + 511) / 512;
+  constexpr int A_PT = (A_CH + NTH - 1) / NTH, B_PT = (B_CH + NTH - 1) / NTH;
   __shared__ __attribute__((aligned(16))) u16 smem[2 * STAGE];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -99,7 +103,7 @@ __global__ __launch_bounds__(512) void nt_bf16x3_kernel(NtParams p) {
   auto gload = [&](int64_t k0, Regs& R) {
 #pragma unroll
     for (int e = 0; e < A_PT; ++e) {
-      const int c = tid + 512 * e;
+      const int c = tid + NTH * e;
       uint4 vh = {0, 0, 0, 0}, vl = {0, 0, 0, 0};
       if (c < A_CH) {
         const int r = c / (BK / 8), kk = (c % (BK / 8)) * 8;
@@ -115,7 +119,7 @@ __global__ __launch_bounds__(512) void nt_bf16x3_kernel(NtParams p) {
     }
 #pragma unroll
     for (int e = 0; e < B_PT; ++e) {
-      const int c = tid + 512 * e;
+      const int c = tid + NTH * e;
       uint4 vh = {0, 0, 0, 0}, vl = {0, 0, 0, 0};
       if (c < B_CH) {
         const int r = c / (BK / 8), kk = (c % (BK / 8)) * 8;
@@ -134,7 +138,7 @@ __global__ __launch_bounds__(512) void nt_bf16x3_kernel(NtParams p) {
     u16* s = smem + buf * STAGE;
 #pragma unroll
     for (int e = 0; e < A_PT; ++e) {
-      const int c = tid + 512 * e;
+      const int c = tid + NTH * e;
       if (c < A_CH) {
         const int r = c / (BK / 8), kk = (c % (BK / 8)) * 8;
         *reinterpret_cast<uint4*>(s + r * LD + kk) = R.ah[e];
@@ -143,7 +147,7 @@ __global__ __launch_bounds__(512) void nt_bf16x3_kernel(NtParams p) {
     }
 #pragma unroll
     for (int e = 0; e < B_PT; ++e) {
-      const int c = tid + 512 * e;
+      const int c = tid + NTH * e;
       if (c < B_CH) {
         const int r = c / (BK / 8), kk = (c % (BK / 8)) * 8;
         *reinterpret_cast<uint4*>(s + 2 * A_ELEMS + r * LD + kk) = R.bh[e];
@@ -489,8 +493,8 @@ void launch_nt(const NtArgs& a, hipStream_t s, DevBuf& ws) {
   HH_REQUIRE(p.nwg < (1LL << 31), "gemm_nt: grid too large");
   if (a.ev_begin) HH_HIP(hipEventRecord(a.ev_begin, s));
   if constexpr (BF)
-    hipLaunchKernelGGL((nt_bf16x3_kernel<WGM, WGN, FM, FN, TILED, BMT, PD>), dim3((unsigned)p.nwg), dim3(512), 0, s,
-                       p);
+    hipLaunchKernelGGL((nt_bf16x3_kernel<WGM, WGN, FM, FN, TILED, BMT, PD>), dim3((unsigned)p.nwg),
+                       dim3(64 * WGM * WGN), 0, s, p);
   else
     hipLaunchKernelGGL((nt_f64_kernel<WGM, WGN, FM, FN>), dim3((unsigned)p.nwg), dim3(512), 0, s, p);
   HH_HIP(hipGetLastError());
@@ -522,6 +526,21 @@ void launch_f64_cfg(int cfg, const NtArgs& a, hipStream_t s, DevBuf& ws) {
   }
 }
 
+// BF16X3 candidates, both 208 x 256 tiles on 8 waves: 0 = 1 x 8 waves of 13 x 2 blocks (two K
+// steps of loads in flight), 1 = 2 x 4 waves of 7 (6) x 4 blocks (fewer LDS fragment reads per
+// MFMA, one K step of loads in flight to fit the registers)
+constexpr int NT_BF_NCFG = 2;
+
+void launch_bf_cfg(int cfg, const NtArgs& a, hipStream_t s, DevBuf& ws) {
+  if (cfg == 1) {
+    if (a.tiled) launch_nt<2, 4, 7, 4, true, true, 13, 1>(a, s, ws);
+    else launch_nt<2, 4, 7, 4, true, false, 13, 1>(a, s, ws);
+  } else {
+    if (a.tiled) launch_nt<1, 8, 13, 2, true, true>(a, s, ws);
+    else launch_nt<1, 8, 13, 2, true, false>(a, s, ws);
+  }
+}
+
 struct NtChoice {
   int cfg, ks;
 };
@@ -536,10 +555,13 @@ int64_t pow2_ceil(int64_t x) {
   return p;
 }
 
-NtKey shape_class(int64_t M, int64_t N, int64_t K) { return NtKey(M, pow2_ceil(N), pow2_ceil(K), 0, 0); }
+NtKey shape_class(int64_t M, int64_t N, int64_t K, int fmt = NT_F64) {
+  return NtKey(M, pow2_ceil(N), pow2_ceil(K), fmt, 0);
+}
 
-NtChoice tuned_f64(const NtArgs& a, hipStream_t s, DevBuf& ws) {
-  const NtKey key = shape_class(a.M, a.N, a.K);
+NtChoice tuned(const NtArgs& a, NtFormat fmt, hipStream_t s, DevBuf& ws) {
+  const bool bf = fmt == NT_BF16X3;
+  const NtKey key = shape_class(a.M, a.N, a.K, fmt);
   std::lock_guard<std::mutex> lk(g_tune_mu);
   auto it = g_tune_cache->find(key);
   if (it != g_tune_cache->end()) return it->second;
@@ -548,20 +570,31 @@ NtChoice tuned_f64(const NtArgs& a, hipStream_t s, DevBuf& ws) {
   HH_HIP(hipStreamIsCapturing(s, &cap));
   if (!g_autotune || cap != hipStreamCaptureStatusNone || a.K == 0) return best;
   static DevBuf* scratch = new DevBuf;
-  double* C = static_cast<double*>(scratch->reserve(sizeof(double) * a.M * a.N));
+  void* C = scratch->reserve(sizeof(double) * a.M * a.N);
   hipEvent_t e0, e1;
   HH_HIP(hipEventCreate(&e0));
   HH_HIP(hipEventCreate(&e1));
   float best_ms = 1e30f;
-  for (int cfg = 0; cfg < NT_F64_NCFG; ++cfg)
+  const int ncfg = bf ? NT_BF_NCFG : NT_F64_NCFG;
+  auto run = [&](int cfg, const NtArgs& t) {
+    if (bf) launch_bf_cfg(cfg, t, s, ws);
+    else launch_f64_cfg(cfg, t, s, ws);
+  };
+  for (int cfg = 0; cfg < ncfg; ++cfg)
     for (int ks : {0, -1}) {
       NtArgs t = a;
       t.K = std::min<int64_t>(a.K, 65536);
-      t.C = C; t.ldc = a.M; t.alpha = 1.0; t.beta = 0.0; t.ksplit = ks;
+      t.alpha = 1.0; t.beta = 0.0; t.ksplit = ks;
+      t.ldc = a.M;
+      if (a.C_f32) {
+        t.C = nullptr; t.C_f32 = static_cast<float*>(C);
+      } else {
+        t.C = static_cast<double*>(C); t.C_f32 = nullptr;
+      }
       t.ev_begin = t.ev_main_end = nullptr;
-      if (cfg == 0 && ks == 0) launch_f64_cfg(cfg, t, s, ws);
+      if (cfg == 0 && ks == 0) run(cfg, t);
       HH_HIP(hipEventRecord(e0, s));
-      launch_f64_cfg(cfg, t, s, ws);
+      run(cfg, t);
       HH_HIP(hipEventRecord(e1, s));
       HH_HIP(hipEventSynchronize(e1));
       float ms = 0.f;
@@ -584,13 +617,25 @@ void gemm_nt_set_autotune(bool on) {
   g_autotune = on;
 }
 
-bool gemm_nt_choice(int64_t M, int64_t N, int64_t K, int* cfg, int* ksplit) {
+bool gemm_nt_choice(int64_t M, int64_t N, int64_t K, int* cfg, int* ksplit, NtFormat fmt) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  auto it = g_tune_cache->find(shape_class(M, N, K));
+  auto it = g_tune_cache->find(shape_class(M, N, K, fmt));
   if (it == g_tune_cache->end()) return false;
   *cfg = it->second.cfg;
   *ksplit = it->second.ks;
   return true;
+}
+
+void gemm_nt_bf16x3_config(const NtArgs& a, int cfg, hipStream_t s, DevBuf& ws) {
+  HH_REQUIRE(cfg >= 0 && cfg < NT_BF_NCFG, "gemm_nt: unknown BF16X3 tile configuration");
+  HH_REQUIRE(a.M >= 0 && a.N >= 0 && a.K >= 0, "gemm_nt: negative size");
+  if (a.M == 0 || a.N == 0) return;
+  HH_REQUIRE((a.C || a.C_f32) && (a.K == 0 || (a.A && a.B && a.A_lo && a.B_lo)), "gemm_nt: null operand");
+  HH_REQUIRE(a.K % 32 == 0 && a.lda % 8 == 0 && a.ldb % 8 == 0,
+             "gemm_nt: K must be a multiple of 32 and lda, ldb multiples of 8");
+  HH_REQUIRE((a.a_rows == 0 || a.a_rows >= a.M) && (a.b_rows == 0 || a.b_rows >= a.N),
+             "gemm_nt: tiled row stride smaller than the operand");
+  launch_bf_cfg(cfg, a, s, ws);
 }
 
 void gemm_nt_f64_config(const NtArgs& a, int cfg, hipStream_t s, DevBuf& ws) {
@@ -615,14 +660,20 @@ void gemm_nt(const NtArgs& a, NtFormat fmt, hipStream_t s, DevBuf& ws) {
     HH_REQUIRE((a.a_rows == 0 || a.a_rows >= a.M) && (a.b_rows == 0 || a.b_rows >= a.N),
                "gemm_nt: tiled row stride smaller than the operand");
     // 208 x 256 tiles: M = 400 is two tiles with 4 % padding
-    if (a.tiled) launch_nt<1, 8, 13, 2, true, true>(a, s, ws);
-    else launch_nt<1, 8, 13, 2, true, false>(a, s, ws);
+    if (a.ksplit > 0) {
+      launch_bf_cfg(0, a, s, ws);
+    } else {
+      const NtChoice c = tuned(a, NT_BF16X3, s, ws);
+      NtArgs t = a;
+      t.ksplit = c.ks;
+      launch_bf_cfg(c.cfg, t, s, ws);
+    }
   } else {
     HH_REQUIRE(!a.tiled, "gemm_nt: tiled operands are BF16X3 only");
     if (a.ksplit > 0) {
       launch_nt<1, 8, 13, 1, false>(a, s, ws);
     } else {
-      const NtChoice c = tuned_f64(a, s, ws);
+      const NtChoice c = tuned(a, NT_F64, s, ws);
       NtArgs t = a;
       t.ksplit = c.ks;
       launch_f64_cfg(c.cfg, t, s, ws);

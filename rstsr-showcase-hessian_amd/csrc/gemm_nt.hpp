@@ -38,15 +38,17 @@ struct NtArgs {
   hipEvent_t ev_begin = nullptr, ev_main_end = nullptr;
 };
 
-// NT_F64 products whose caller leaves ksplit <= 0 are autotuned: the first product of each shape
-// class (M, N and K rounded up to powers of two) times the tile configurations and split rules on
-// the stream and keeps the fastest (no timing while the stream is being captured into a graph).
+// Products whose caller leaves ksplit <= 0 are autotuned: the first product of each shape class
+// (format, M, and N and K rounded up to powers of two) times the tile configurations and split
+// rules on the stream and keeps the fastest (no timing while the stream is being captured into a
+// graph).
 void gemm_nt(const NtArgs& a, NtFormat fmt, hipStream_t s, DevBuf& ws);
 void gemm_nt_set_autotune(bool on);
 // the cached choice for a shape's class (false if none yet): configuration index and split rule
-bool gemm_nt_choice(int64_t M, int64_t N, int64_t K, int* cfg, int* ksplit);
-// one NT_F64 product with configuration `cfg` (0 .. 2) and the split rule in a.ksplit
+bool gemm_nt_choice(int64_t M, int64_t N, int64_t K, int* cfg, int* ksplit, NtFormat fmt = NT_F64);
+// one product with configuration `cfg` (F64: 0 .. 2; BF16X3: 0 .. 1) and the split rule in a.ksplit
 void gemm_nt_f64_config(const NtArgs& a, int cfg, hipStream_t s, DevBuf& ws);
+void gemm_nt_bf16x3_config(const NtArgs& a, int cfg, hipStream_t s, DevBuf& ws);
 
 // Split the column-major [nrow, ncol] matrix src (leading dimension ld) into bf16 hi / lo arrays
 // (hi = bf16(x), lo = bf16(x - hi)) in the K-block tiled operand layout with K = rows padded to

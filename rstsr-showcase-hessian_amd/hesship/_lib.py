@@ -116,6 +116,8 @@ SIGNATURES.update({
     "hh_util_gemm_nt": (_i32, [_i64, _i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64, _i32, _i32, _p]),
     "hh_util_set_autotune": (_i32, [_i32]),
     "hh_util_gemm_nt_choice": (_i32, [_i64, _i64, _i64, C.POINTER(_i32), C.POINTER(_i32)]),
+    "hh_util_gemm_nt_bf16x3": (_i32, [_i64, _i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64, _i32, _i32, _p]),
+    "hh_util_gemm_nt_bf16x3_choice": (_i32, [_i64, _i64, _i64, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "hh_cphf_set_graphs": (_i32, [_p, _i32]),
     "hh_cphf_graph_stats": (_i32, [_p, _pi64, _pi64]),
     "hh_cphf_finalize_reuses": (_i32, [_p, _pi64]),

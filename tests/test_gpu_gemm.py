@@ -124,3 +124,47 @@ def test_gemm_nt_autotuner(dev):
     with pytest.raises(_lib.HesshipError, match="configuration"):
         _lib.check(L.hh_util_gemm_nt(32, 32, 32, 1.0, None, 40, None, 40, 0.0, None, 32, 5, 0,
                                      torch.cuda.current_stream().cuda_stream))
+
+
+def run_nt_bf16x3(M, N, K, config, ksplit, alpha=1.0, beta=0.0, dev=None, seed=0, pad=8):
+    from hesship import _lib
+    gen = torch.Generator().manual_seed(seed)
+    lda = ldb = K + pad
+    A = torch.randn(M, lda, generator=gen, dtype=torch.float64).to(dev)
+    B = torch.randn(N, ldb, generator=gen, dtype=torch.float64).to(dev)
+    ldc = M + 3
+    Cm = torch.randn(N, ldc, generator=gen, dtype=torch.float64).to(dev)
+    ref = alpha * (A[:, :K] @ B[:, :K].T) + beta * Cm[:, :M].T
+    _lib.check(_lib.lib().hh_util_gemm_nt_bf16x3(M, N, K, alpha, A.data_ptr(), lda, B.data_ptr(), ldb, beta,
+                                                 Cm.data_ptr(), ldc, config, ksplit,
+                                                 torch.cuda.current_stream().cuda_stream))
+    torch.cuda.synchronize()
+    got = Cm[:, :M].T
+    scale = (A[:, :K].abs() @ B[:, :K].abs().T).max().item() * abs(alpha) + 1e-300
+    return (got - ref).abs().max().item() / scale
+
+
+@pytest.mark.parametrize("config", [0, 1])
+@pytest.mark.parametrize("ksplit", [1, 3, 0, -1])
+@pytest.mark.parametrize("shape", [(400, 1800, 2048), (37, 29, 96), (208, 256, 512), (417, 300, 640)])
+def test_gemm_nt_bf16x3_configs(dev, config, ksplit, shape):
+    """Split-BF16 x3 NT product, both tile configurations (1 x 8 and 2 x 4 waves): error relative to
+    |A| |B| at the scheme's level (missing lo*lo term ~2^-16, FP32 accumulation)."""
+    M, N, K = shape
+    assert run_nt_bf16x3(M, N, K, config, ksplit, alpha=0.7, beta=-1.3, dev=dev) < 3e-5
+
+
+def test_gemm_nt_bf16x3_autotuner(dev):
+    import ctypes as C
+    from hesship import _lib
+    L = _lib.lib()
+    cfg, ks = C.c_int32(), C.c_int32()
+    M, N, K = 400, 1000, 4096
+    _lib.check(L.hh_util_gemm_nt_bf16x3_choice(M, N, K, C.byref(cfg), C.byref(ks)))
+    assert cfg.value == -1
+    assert run_nt_bf16x3(M, N, K, -1, 0, dev=dev) < 3e-5
+    _lib.check(L.hh_util_gemm_nt_bf16x3_choice(M, N, K, C.byref(cfg), C.byref(ks)))
+    assert cfg.value in (0, 1) and ks.value in (0, -1)
+    # the F64 classes are separate
+    _lib.check(L.hh_util_gemm_nt_choice(M, N, K, C.byref(cfg), C.byref(ks)))
+    assert cfg.value == -1
