@@ -321,12 +321,13 @@ hh_status hh_cphf_set_verbose(hh_cphf* h, int32_t on);
  * hh_cphf_graph_stats: graphs captured and graph launches so far. */
 hh_status hh_cphf_set_graphs(hh_cphf* h, int32_t on);
 hh_status hh_cphf_graph_stats(hh_cphf* h, int64_t* captures, int64_t* launches);
-/* Number of finalize calls (hh_cphf_finalize / hh_ucphf_finalize) that took the FP64 response of
- * mo1 from the last mixed-precision solve instead of evaluating it again (no reference
+/* Number of finalize calls (hh_cphf_finalize / hh_ucphf_finalize) that took the response of mo1
+ * from the last mixed-precision solve instead of evaluating it in FP64 again (no reference
  * counterpart; the reference's finalize, rscf.rs:328-353, always evaluates it).  The solve's
- * final FP64 residual is that response; it is reused when the mo1 passed to finalize is
- * bit-identical to the solve's output and no component was added since (components must not be
- * re-prepared between the solve and finalize). */
+ * final residual holds that response: FP64, except that the responses of late, small corrections
+ * may be taken in the reduced precision when their error bound is below 5 % of cphf_tol.  It is
+ * reused when the mo1 passed to finalize is bit-identical to the solve's output and no component
+ * was added since (components must not be re-prepared between the solve and finalize). */
 hh_status hh_cphf_finalize_reuses(hh_cphf* h, int64_t* count);
 /* Add an RI-JK component (nspin 1 for the restricted driver, 2 for the unrestricted one,
  * prepared with the same MOs) to the response list (the reference's el_list, rscf.rs:28,

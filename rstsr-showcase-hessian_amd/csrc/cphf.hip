@@ -167,13 +167,31 @@ __global__ void scale_columns_kernel(double* r, int64_t n, int64_t ncol, const d
   }
 }
 
-// y[:, j] += x[:, j] / sc[j]  (sc[j] = 0: column untouched)
-__global__ void add_unscaled_kernel(double* y, const double* x, int64_t n, int64_t ncol, const double* sc) {
+// x[:, j] /= sc[j]  (sc[j] = 0: column zeroed)
+__global__ void unscale_columns_kernel(double* x, int64_t n, int64_t ncol, const double* sc) {
   const int64_t total = n * ncol;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
     const double f = sc[e / n];
-    if (f != 0.0) y[e] += x[e] / f;
+    x[e] = f != 0.0 ? x[e] / f : 0.0;
   }
+}
+
+// colnorm[j] = sum_i x[i, j]^2 (one workgroup per column, fixed order)
+__global__ void colnorm2_kernel(const double* x, int64_t n, double* colnorm) {
+  __shared__ double red[256];
+  const int64_t a = blockIdx.x;
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const double v = x[i + n * a];
+    acc += v * v;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s2 = 128; s2 > 0; s2 >>= 1) {
+    if ((int)threadIdx.x < s2) red[threadIdx.x] += red[threadIdx.x + s2];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) colnorm[a] = red[0];
 }
 
 __global__ void axpy_kernel(double* y, const double* x, double alpha, int64_t n) {
@@ -534,30 +552,37 @@ void Cphf::solve_stacked(const double* rhs, int64_t nprop, double* mo1, Prec pre
   // residual shrinks, and the non-normalised Krylov basis does not mix columns whose norms
   // differ by orders of magnitude (that makes the projected system ill-conditioned).  Columns
   // already far below the tolerance are left out.  Inner solves go to about the
-  // reduced-precision operator's own accuracy; the outer loop stops when every column's FP64
-  // residual 2-norm is below tol.
-  double* r = static_cast<double*>(res_.reserve(sizeof(double) * (3 * blk + 2 * nprop)));
+  // reduced-precision operator's own accuracy; the outer loop stops when every column's residual
+  // 2-norm plus the bound on its error is below tol (the residual is FP64 up to the
+  // reduced-precision responses of small corrections, see below).
+  double* r = static_cast<double*>(res_.reserve(sizeof(double) * (4 * blk + 3 * nprop)));
   double* ax = r + blk;
   double* dx = ax + blk;
-  double* coln = dx + blk;
+  double* rdx = dx + blk;
+  double* coln = rdx + blk;
   double* csc = coln + nprop;
+  double* cnd = csc + nprop;
   std::vector<double> hn(nprop);
-  auto max_colnorm2 = [&]() {
-    HH_HIP(hipMemcpyAsync(hn.data(), coln, sizeof(double) * nprop, hipMemcpyDeviceToHost, stream_));
+  auto max_colnorm2 = [&](const double* cn) {
+    HH_HIP(hipMemcpyAsync(hn.data(), cn, sizeof(double) * nprop, hipMemcpyDeviceToHost, stream_));
     HH_HIP(hipStreamSynchronize(stream_));
     double m = 0.0;
     for (double v : hn) m = std::max(m, v);
     return m;
   };
+  // Relative error of the reduced-precision response (measured: 6e-6 for BF16x3 against FP64 on
+  // the whole response, 1e-7 for FP32), taken 10-20x larger as the bound used below.
+  const double rel_err = (prec == PREC_BF16X3) ? 1e-4 : 2e-6;
   fill<double>(mo1, blk, 0.0, stream_);
   fill<double>(ax, blk, 0.0, stream_);
   hipLaunchKernelGGL(residual_norm_kernel, dim3((unsigned)nprop), dim3(256), 0, stream_, rhs, mo1, ax, r, n, coln);
   HH_HIP(hipGetLastError());
-  double res = max_colnorm2();
+  double res = max_colnorm2(coln);
+  double bound = 0.0;  // bound on the error of the residual's 2-norm (any column)
   double* fin_r = static_cast<double*>(fin_r_.reserve(sizeof(double) * blk));
   const double op_tol = (prec == PREC_BF16X3) ? 1e-5 : 1e-7;  // inner target, relative to the rescaled rhs
   const int max_sweeps = 10;
-  for (int sweep = 0; sweep < max_sweeps && std::sqrt(res) >= cfg_.tol && res > 0.0; ++sweep) {
+  for (int sweep = 0; sweep < max_sweeps && std::sqrt(res) + bound >= cfg_.tol && res > 0.0; ++sweep) {
     const double sc = 1.0 / std::sqrt(res);
     hipLaunchKernelGGL(scale_columns_kernel, dim3(nblk(blk)), dim3(256), 0, stream_, r, n, nprop, coln,
                        0.01 * cfg_.tol, csc);
@@ -575,16 +600,47 @@ void Cphf::solve_stacked(const double* rhs, int64_t nprop, double* mo1, Prec pre
     }
     accumulate(ks);
     unprecond(y, dx, nprop);
-    hipLaunchKernelGGL(add_unscaled_kernel, dim3(nblk(blk)), dim3(256), 0, stream_, mo1, dx, n, nprop, csc);
+    // the correction itself (columns left out: zero), added to mo1
+    hipLaunchKernelGGL(unscale_columns_kernel, dim3(nblk(blk)), dim3(256), 0, stream_, dx, n, nprop, csc);
     HH_HIP(hipGetLastError());
-    // FP64 response of mo1, kept raw for finalize, then put in dimensionless form
-    response_mo_stacked(mo1, nprop, fin_r, PREC_F64);
+    hipLaunchKernelGGL(axpy_kernel, dim3(nblk(blk)), dim3(256), 0, stream_, mo1, dx, 1.0, blk);
+    HH_HIP(hipGetLastError());
+    // The raw response of mo1 (kept for finalize) is updated by the response of the correction.
+    // The first sweep's is FP64.  Later corrections are small: their response is taken in the
+    // reduced precision when rel_err times its largest dimensionless column norm stays below
+    // 5 % of tol (the residual test then allows for that bound), else in FP64.
+    // (a correction's dimensionless response is about the size of the residual it corrects: a
+    // reduced-precision attempt that would fail the test by 10x is not made)
+    if (sweep == 0) {
+      response_mo_stacked(mo1, nprop, fin_r, PREC_F64);
+      ++stats.krylov.aop_calls;
+    } else if (rel_err * std::sqrt(res) > 0.5 * cfg_.tol) {
+      response_mo_stacked(dx, nprop, rdx, PREC_F64);
+      ++stats.krylov.aop_calls;
+      hipLaunchKernelGGL(axpy_kernel, dim3(nblk(blk)), dim3(256), 0, stream_, fin_r, rdx, 1.0, blk);
+      HH_HIP(hipGetLastError());
+    } else {
+      response_mo_stacked(dx, nprop, rdx, prec);
+      ++stats.krylov.aop_calls;
+      copy_bytes(ax, rdx, (int64_t)sizeof(double) * blk, stream_);
+      dimless_scale(dx, nprop, ax);
+      hipLaunchKernelGGL(colnorm2_kernel, dim3((unsigned)nprop), dim3(256), 0, stream_, ax, n, cnd);
+      HH_HIP(hipGetLastError());
+      const double b = rel_err * std::sqrt(max_colnorm2(cnd));
+      if (b <= 0.05 * cfg_.tol) {
+        bound += b;
+      } else {
+        response_mo_stacked(dx, nprop, rdx, PREC_F64);
+        ++stats.krylov.aop_calls;
+      }
+      hipLaunchKernelGGL(axpy_kernel, dim3(nblk(blk)), dim3(256), 0, stream_, fin_r, rdx, 1.0, blk);
+      HH_HIP(hipGetLastError());
+    }
     copy_bytes(ax, fin_r, (int64_t)sizeof(double) * blk, stream_);
     dimless_scale(mo1, nprop, ax);
-    ++stats.krylov.aop_calls;
     hipLaunchKernelGGL(residual_norm_kernel, dim3((unsigned)nprop), dim3(256), 0, stream_, rhs, mo1, ax, r, n, coln);
     HH_HIP(hipGetLastError());
-    res = max_colnorm2();
+    res = max_colnorm2(coln);
     stats.refinements = sweep + 1;
     if (verbose) {
       fprintf(stderr, "cphf refinement sweep %d: inner cycles %d (tol %.1e), FP64 residual %.3e\n", sweep + 1,
@@ -592,16 +648,17 @@ void Cphf::solve_stacked(const double* rhs, int64_t nprop, double* mo1, Prec pre
       fflush(stderr);
     }
   }
-  stats.residual = std::sqrt(res);
+  stats.residual = std::sqrt(res) + bound;
   if (st) *st = stats;
-  if (stats.refinements > 0 && std::sqrt(res) < cfg_.tol) {
+  const bool ok = std::sqrt(res) + bound < cfg_.tol;
+  if (stats.refinements > 0 && ok) {
     copy_bytes(fin_x_.reserve(sizeof(double) * blk), mo1, (int64_t)sizeof(double) * blk, stream_);
     fin_nprop_ = nprop;
   }
-  if (!(std::sqrt(res) < cfg_.tol)) {
+  if (!ok) {
     char buf[200];
     snprintf(buf, sizeof buf, "cphf mixed-precision refinement did not reach tol %.3e (residual %.3e after %d sweeps)",
-             cfg_.tol, std::sqrt(res), stats.refinements);
+             cfg_.tol, std::sqrt(res) + bound, stats.refinements);
     throw Error(HH_ERR_NOT_CONVERGED, buf);
   }
 }

@@ -76,10 +76,11 @@ class Cphf {
   bool use_graphs = true;
   int64_t graph_captures() const { return n_captures_; }
   int64_t graph_launches() const { return n_launches_; }
-  // finalize() calls that took the FP64 response from the last mixed-precision solve: its final
-  // residual evaluates the FP64 response of the returned mo1, which finalize needs again.  Reused
-  // when the mo1 given to finalize is bit-identical to that solve's output and no component was
-  // added since (components must not be re-prepared in between).
+  // finalize() calls that took the response of mo1 from the last mixed-precision solve: its final
+  // residual holds the response of the returned mo1 (FP64, up to corrections whose response was
+  // taken in the reduced precision with an error below 5 % of tol), which finalize needs again.
+  // Reused when the mo1 given to finalize is bit-identical to that solve's output and no
+  // component was added since (components must not be re-prepared in between).
   int64_t finalize_reuses() const { return n_fin_reuse_; }
   ~Cphf();
 

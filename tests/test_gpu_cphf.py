@@ -101,9 +101,10 @@ def test_mixed_precision_refinement_reaches_fp64_tolerance(dev, prec):
 
 
 def test_finalize_reuses_mixed_solve_response(dev):
-    """finalize after a mixed-precision solve takes the FP64 response of mo1 from the solve's
-    final residual: the same results (to rounding) as evaluating it again (after an FP64 solve
-    the cache is empty), equal to the oracle's finalize; a changed mo1 is not reused."""
+    """finalize after a mixed-precision solve takes the response of mo1 from the solve's final
+    residual (FP64 up to small corrections whose reduced-precision response is within 5 % of
+    cphf_tol): the same results, to that level, as evaluating it again in FP64 (after an FP64
+    solve the cache is empty) and as the oracle's finalize; a changed mo1 is not reused."""
     cderi, C, occ, e, f1, s1, rb = system(seed=8)
     drv, _ = make(dev, cderi, C, occ, e, cfg=dict(cphf_tol=1e-9))
     rhs = T(ocp.dimless_rhs(f1, s1, occ, e), dev)
@@ -112,8 +113,8 @@ def test_finalize_reuses_mixed_solve_response(dev):
     m_a, e_a = drv.finalize_cphf(F1, S1, mo1)
     assert drv.finalize_reuses() == 1
     m_o, e_o = ocp.finalize(C, occ, e, [rb], f1, s1, mo1.cpu().numpy())
-    np.testing.assert_allclose(m_a.cpu().numpy(), m_o, rtol=1e-10, atol=1e-12)
-    np.testing.assert_allclose(e_a.cpu().numpy(), e_o, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(m_a.cpu().numpy(), m_o, rtol=1e-8, atol=1e-9)
+    np.testing.assert_allclose(e_a.cpu().numpy(), e_o, rtol=1e-8, atol=1e-9)
     bumped = mo1.clone()
     bumped[7, 1, 0, 0] += 1e-12
     drv.finalize_cphf(F1, S1, bumped)
@@ -121,7 +122,8 @@ def test_finalize_reuses_mixed_solve_response(dev):
     drv.solve_dimless_cphf(rhs)  # FP64: no cached response
     m_b, e_b = drv.finalize_cphf(F1, S1, mo1)
     assert drv.finalize_reuses() == 1
-    assert rel(m_b.cpu().numpy(), m_a.cpu().numpy()) < 1e-14 and rel(e_b.cpu().numpy(), e_a.cpu().numpy()) < 1e-14
+    np.testing.assert_allclose(m_b.cpu().numpy(), m_a.cpu().numpy(), rtol=1e-8, atol=1e-9)
+    np.testing.assert_allclose(e_b.cpu().numpy(), e_a.cpu().numpy(), rtol=1e-8, atol=1e-9)
 
 
 def test_right_preconditioner_gives_same_solution(dev):
