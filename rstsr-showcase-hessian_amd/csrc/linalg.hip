@@ -11,37 +11,20 @@ namespace {
 constexpr int NB = 64;
 constexpr int PT = 1024;  // panel kernel threads
 
-__device__ __forceinline__ void argmax_reduce(double& v, int64_t& idx, double* sv, int64_t* si) {
-  // block-wide (PT threads) argmax of |v| (v already absolute), ties -> smaller idx
-  const int t = threadIdx.x;
-  sv[t] = v;
-  si[t] = idx;
-  __syncthreads();
-  for (int s = PT / 2; s > 0; s >>= 1) {
-    if (t < s) {
-      const double a = sv[t], b = sv[t + s];
-      const int64_t ia = si[t], ib = si[t + s];
-      if (b > a || (b == a && ib < ia)) {
-        sv[t] = b;
-        si[t] = ib;
-      }
-    }
-    __syncthreads();
-  }
-  v = sv[0];
-  idx = si[0];
-  __syncthreads();
-}
-
 // Unblocked LU of panel columns [k0, k0+nb) over rows [k0, nd); records pivots (global row
-// indices) in piv[k0..k0+nb); row swaps applied within the panel only.
+// indices) in piv[k0..k0+nb); row swaps applied within the panel only.  Per column: the argmax
+// (largest |a|, ties to the smaller row) by wave shuffles and one exchange of the 16 wave results
+// in LDS; the rank-1 update of the panel's remaining columns row by row, eight columns' loads
+// issued together.
 __global__ __launch_bounds__(PT) void lu_panel_kernel(double* A, int64_t nd, int64_t k0, int nb, int64_t* piv,
                                                       int* info) {
-  __shared__ double sv[PT];
-  __shared__ int64_t si[PT];
-  __shared__ double prow[NB];
-  const int t = threadIdx.x;
+  constexpr int NW = PT / 64;
+  __shared__ double wbest[2][NW];
+  __shared__ int64_t widx[2][NW];
+  __shared__ double prow[2][NB];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   for (int c = 0; c < nb; ++c) {
+    const int par = c & 1;  // double-buffered LDS: no barrier between the last reads and the next writes
     const int64_t col = k0 + c;
     double* a = A + col * nd;
     double best = -1.0;
@@ -53,29 +36,61 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(double* A, int64_t nd, int
         bi = i;
       }
     }
-    argmax_reduce(best, bi, sv, si);
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ob = __shfl_xor(best, o, 64);
+      const int64_t oi = __shfl_xor(bi, o, 64);
+      if (ob > best || (ob == best && oi < bi)) {
+        best = ob;
+        bi = oi;
+      }
+    }
+    if (lane == 0) {
+      wbest[par][w] = best;
+      widx[par][w] = bi;
+    }
+    __syncthreads();
+    best = wbest[par][0];
+    bi = widx[par][0];
+    for (int x = 1; x < NW; ++x) {
+      const double ob = wbest[par][x];
+      const int64_t oi = widx[par][x];
+      if (ob > best || (ob == best && oi < bi)) {
+        best = ob;
+        bi = oi;
+      }
+    }
     if (t == 0) {
       piv[col] = bi;
       if (!(best > 0.0) && *info == 0) *info = (int)(col + 1);
     }
-    // swap rows col <-> bi inside the panel
-    if (bi != col && t < nb) {
+    // swap rows col <-> bi inside the panel; thread t also keeps the pivot row's entry of its
+    // column for the update
+    if (t < nb) {
       double* x = A + (k0 + t) * nd;
-      const double tmp = x[col];
-      x[col] = x[bi];
-      x[bi] = tmp;
+      if (bi != col) {
+        const double tmp = x[col];
+        x[col] = x[bi];
+        x[bi] = tmp;
+      }
+      prow[par][t] = x[col];
     }
     __syncthreads();
-    const double d = a[col];
+    const double d = prow[par][c];
     const double inv = (d != 0.0) ? 1.0 / d : 0.0;
-    for (int64_t i = col + 1 + t; i < nd; i += PT) a[i] *= inv;
-    if (t < nb) prow[t] = A[(k0 + t) * nd + col];
-    __syncthreads();
-    // rank-1 update of the remaining panel columns
-    for (int j = c + 1; j < nb; ++j) {
-      const double u = prow[j];
-      double* x = A + (k0 + j) * nd;
-      for (int64_t i = col + 1 + t; i < nd; i += PT) x[i] -= a[i] * u;
+    // scale the column and apply the rank-1 update to the panel's remaining columns, row by row
+    // (each row belongs to one thread, so the scaled value is reused without a barrier)
+    for (int64_t i = col + 1 + t; i < nd; i += PT) {
+      const double ai = a[i] * inv;
+      a[i] = ai;
+      for (int j0 = c + 1; j0 < nb; j0 += 8) {
+        double xv[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          if (j0 + q < nb) xv[q] = A[(k0 + j0 + q) * nd + i];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          if (j0 + q < nb) A[(k0 + j0 + q) * nd + i] = xv[q] - ai * prow[par][j0 + q];
+      }
     }
     __syncthreads();
   }
@@ -101,47 +116,50 @@ __global__ void lu_swap_kernel(double* H, int64_t nd, double* G, int64_t nrhs, i
 }
 
 // X[k0..k0+nb, j] <- L11^-1 X[..., j] for L11 = unit-lower block of A at (k0, k0) (forward
-// substitution), for the trailing H columns and all G columns.
+// substitution), for the trailing H columns and all G columns.  One wave per column, lane r
+// holding row r (NB = 64 = the wave size): the right-hand side never leaves registers.
 __global__ void lu_trsm_kernel(const double* A, int64_t nd, double* H, double* G, int64_t nrhs, int64_t k0, int nb) {
   __shared__ double L[NB * NB];
   for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
     const int r = e % nb, c = e / nb;
-    L[e] = A[(k0 + r) + (k0 + c) * nd];
+    L[r + c * NB] = A[(k0 + r) + (k0 + c) * nd];
   }
   __syncthreads();
+  const int lane = threadIdx.x & 63;
   const int64_t ntrail = nd - (k0 + nb);
   const int64_t ncol = ntrail + nrhs;
-  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < ncol; j += (int64_t)gridDim.x * blockDim.x) {
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t j = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; j < ncol; j += nw) {
     double* x = ((j < ntrail) ? H + (k0 + nb + j) * nd : G + (j - ntrail) * nd) + k0;
-    double y[NB];
-    for (int r = 0; r < nb; ++r) y[r] = x[r];
-    for (int r = 0; r < nb; ++r) {
-      double v = y[r];
-      for (int c = 0; c < r; ++c) v -= L[r + c * nb] * y[c];
-      y[r] = v;
+    double y = lane < nb ? x[lane] : 0.0;
+    for (int c = 0; c < nb; ++c) {
+      const double yc = __shfl(y, c, 64);
+      if (lane > c && lane < nb) y -= L[lane + c * NB] * yc;
     }
-    for (int r = 0; r < nb; ++r) x[r] = y[r];
+    if (lane < nb) x[lane] = y;
   }
 }
 
 // Backward substitution for the diagonal block U_pp at (r0, r0) of size nb: X[r0.., j] <- U^-1 X.
+// One wave per column (lane r holds row r).
 __global__ void upper_block_solve_kernel(const double* U, int64_t nd, double* G, int64_t nrhs, int64_t r0, int nb) {
   __shared__ double S[NB * NB];
   for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
     const int r = e % nb, c = e / nb;
-    S[e] = U[(r0 + r) + (r0 + c) * nd];
+    S[r + c * NB] = U[(r0 + r) + (r0 + c) * nd];
   }
   __syncthreads();
-  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nrhs; j += (int64_t)gridDim.x * blockDim.x) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t j = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; j < nrhs; j += nw) {
     double* x = G + j * nd + r0;
-    double y[NB];
-    for (int r = 0; r < nb; ++r) y[r] = x[r];
+    double y = lane < nb ? x[lane] : 0.0;
     for (int r = nb - 1; r >= 0; --r) {
-      double v = y[r];
-      for (int c = r + 1; c < nb; ++c) v -= S[r + c * nb] * y[c];
-      y[r] = v / S[r + r * nb];
+      if (lane == r) y = y / S[r + r * NB];
+      const double yr = __shfl(y, r, 64);
+      if (lane < r) y -= S[lane + r * NB] * yr;
     }
-    for (int r = 0; r < nb; ++r) x[r] = y[r];
+    if (lane < nb) x[lane] = y;
   }
 }
 
@@ -178,40 +196,43 @@ __global__ __launch_bounds__(256) void chol_block_kernel(double* A, int64_t nd, 
 }
 
 // Row-wise forward substitution against a lower block Lb (nb x nb at (k0,k0) of L, ld = nd):
-// for each row x of X[:, k0..k0+nb):  x <- x Lb^-T
+// for each row x of X[:, k0..k0+nb):  x <- x Lb^-T.  One wave per row (lane j holds entry j).
 __global__ void trsm_right_lt_kernel(const double* L, int64_t nd, double* X, int64_t nrow, int64_t ldx, int64_t k0,
                                      int nb) {
   __shared__ double S[NB * NB];
-  for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) S[e] = L[(k0 + e % nb) + (k0 + e / nb) * nd];
+  for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) S[e % nb + (e / nb) * NB] = L[(k0 + e % nb) + (k0 + e / nb) * nd];
   __syncthreads();
-  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nrow; r += (int64_t)gridDim.x * blockDim.x) {
-    double y[NB];
-    for (int j = 0; j < nb; ++j) y[j] = X[r + (k0 + j) * ldx];
-    for (int j = 0; j < nb; ++j) {
-      double v = y[j];
-      for (int c = 0; c < j; ++c) v -= y[c] * S[j + c * nb];
-      y[j] = v / S[j + j * nb];
+  const int lane = threadIdx.x & 63;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; r < nrow; r += nw) {
+    double y = lane < nb ? X[r + (k0 + lane) * ldx] : 0.0;
+    for (int c = 0; c < nb; ++c) {
+      if (lane == c) y = y / S[c + c * NB];
+      const double yc = __shfl(y, c, 64);
+      if (lane > c && lane < nb) y -= yc * S[lane + c * NB];
     }
-    for (int j = 0; j < nb; ++j) X[r + (k0 + j) * ldx] = y[j];
+    if (lane < nb) X[r + (k0 + lane) * ldx] = y;
   }
 }
 
-// Column-wise forward substitution: X[k0..k0+nb, j] <- Lb^-1 X[k0..k0+nb, j] (non-unit lower)
+// Column-wise forward substitution: X[k0..k0+nb, j] <- Lb^-1 X[k0..k0+nb, j] (non-unit lower).
+// One wave per column (lane r holds row r).
 __global__ void trsm_left_l_kernel(const double* L, int64_t nd, double* X, int64_t ncol, int64_t ldx, int64_t k0,
                                    int nb) {
   __shared__ double S[NB * NB];
-  for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) S[e] = L[(k0 + e % nb) + (k0 + e / nb) * nd];
+  for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) S[e % nb + (e / nb) * NB] = L[(k0 + e % nb) + (k0 + e / nb) * nd];
   __syncthreads();
-  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < ncol; j += (int64_t)gridDim.x * blockDim.x) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t j = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; j < ncol; j += nw) {
     double* x = X + j * ldx + k0;
-    double y[NB];
-    for (int r = 0; r < nb; ++r) y[r] = x[r];
-    for (int r = 0; r < nb; ++r) {
-      double v = y[r];
-      for (int c = 0; c < r; ++c) v -= S[r + c * nb] * y[c];
-      y[r] = v / S[r + r * nb];
+    double y = lane < nb ? x[lane] : 0.0;
+    for (int c = 0; c < nb; ++c) {
+      if (lane == c) y = y / S[c + c * NB];
+      const double yc = __shfl(y, c, 64);
+      if (lane > c && lane < nb) y -= S[lane + c * NB] * yc;
     }
-    for (int r = 0; r < nb; ++r) x[r] = y[r];
+    if (lane < nb) x[lane] = y;
   }
 }
 
@@ -249,7 +270,7 @@ void lu_solve(double* H, int64_t nd, double* G, int64_t nrhs, hipStream_t s, Dev
     const int64_t ncol = ntrail + nrhs;
     if (ncol > 0) {
       hipLaunchKernelGGL(lu_swap_kernel, dim3(blocks_for(ncol, 256)), dim3(256), 0, s, H, nd, G, nrhs, k0, nb, piv);
-      hipLaunchKernelGGL(lu_trsm_kernel, dim3(blocks_for(ncol, 64)), dim3(64), 0, s, H, nd, H, G, nrhs, k0, nb);
+      hipLaunchKernelGGL(lu_trsm_kernel, dim3(blocks_for(ncol * 64, 256)), dim3(256), 0, s, H, nd, H, G, nrhs, k0, nb);
       HH_HIP(hipGetLastError());
     }
     const int64_t mrows = nd - (k0 + nb);
@@ -280,7 +301,8 @@ void lu_solve(double* H, int64_t nd, double* G, int64_t nrhs, hipStream_t s, Dev
   for (int64_t b = nblk - 1; b >= 0; --b) {
     const int64_t r0 = b * NB;
     const int nb = (int)std::min<int64_t>(NB, nd - r0);
-    hipLaunchKernelGGL(upper_block_solve_kernel, dim3(blocks_for(nrhs, 64)), dim3(64), 0, s, H, nd, G, nrhs, r0, nb);
+    hipLaunchKernelGGL(upper_block_solve_kernel, dim3(blocks_for(nrhs * 64, 256)), dim3(256), 0, s, H, nd, G, nrhs, r0,
+                       nb);
     HH_HIP(hipGetLastError());
     if (r0 > 0) {
       GemmArgs g;  // G[0:r0] -= U[0:r0, r0:r0+nb] X[r0:r0+nb]
@@ -306,7 +328,7 @@ bool cholesky(double* H, int64_t nd, hipStream_t s, DevBuf& ws) {
     const int64_t m = nd - (k0 + nb);
     if (m > 0) {
       // L21 = A21 L11^-T
-      hipLaunchKernelGGL(trsm_right_lt_kernel, dim3(blocks_for(m, 64)), dim3(64), 0, s, H, nd, H + (k0 + nb), m, nd,
+      hipLaunchKernelGGL(trsm_right_lt_kernel, dim3(blocks_for(m * 64, 256)), dim3(256), 0, s, H, nd, H + (k0 + nb), m, nd,
                          k0, nb);
       HH_HIP(hipGetLastError());
       // A22 -= L21 L21^T  (full trailing block; the strict upper part is cleared at the end)
@@ -333,7 +355,7 @@ void trsm_lower(const double* L, int64_t nd, double* X, int64_t n2, int64_t ldx,
     const int64_t rest = nd - (k0 + nb);
     if (side_right) {
       // X[:, p] <- X[:, p] Lpp^-T ; X[:, after] -= X[:, p] L[after, p]^T
-      hipLaunchKernelGGL(trsm_right_lt_kernel, dim3(blocks_for(n2, 64)), dim3(64), 0, s, L, nd, X, n2, ldx, k0, nb);
+      hipLaunchKernelGGL(trsm_right_lt_kernel, dim3(blocks_for(n2 * 64, 256)), dim3(256), 0, s, L, nd, X, n2, ldx, k0, nb);
       HH_HIP(hipGetLastError());
       if (rest > 0) {
         GemmArgs g;
@@ -345,7 +367,7 @@ void trsm_lower(const double* L, int64_t nd, double* X, int64_t n2, int64_t ldx,
         gemm<double, double>(g, PREC_F64, s, ws);
       }
     } else {
-      hipLaunchKernelGGL(trsm_left_l_kernel, dim3(blocks_for(n2, 64)), dim3(64), 0, s, L, nd, X, n2, ldx, k0, nb);
+      hipLaunchKernelGGL(trsm_left_l_kernel, dim3(blocks_for(n2 * 64, 256)), dim3(256), 0, s, L, nd, X, n2, ldx, k0, nb);
       HH_HIP(hipGetLastError());
       if (rest > 0) {
         GemmArgs g;  // X[after, :] -= L[after, p] X[p, :]
