@@ -89,7 +89,7 @@ __global__ void setup_rho_kernel(const double* __restrict__ ao, int64_t ng, int6
 // drho / W element (g, v, s, t, A) at g + gb (v + NV (s + NS (t + 3 A))).
 // W[g, y, s2, t, A] = w_g sum_{x, s1} fxc[g, x, s1, y, s2] drho[g, x, s1, t, A].
 template <int DEN, int NS>
-__global__ void drho_w_kernel(const double* __restrict__ ao, int64_t ng, int64_t nao, int64_t g0, int64_t gb,
+__global__ __launch_bounds__(256) void drho_w_kernel(const double* __restrict__ ao, int64_t ng, int64_t nao, int64_t g0, int64_t gb,
                               const double* __restrict__ aod0, const double* __restrict__ aod1,
                               const int64_t* __restrict__ slices, int64_t natm, const double* __restrict__ fxc,
                               const double* __restrict__ w, double* __restrict__ drho, double* __restrict__ W) {

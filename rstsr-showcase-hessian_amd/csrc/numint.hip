@@ -201,13 +201,14 @@ __device__ __forceinline__ void weighted_kernel_uks(const double* fxc, int64_t l
         v += fxc[g + ld * (y + NV * (0 + 2 * (x + NV * sp)))] * ra[y];
         v += fxc[g + ld * (y + NV * (1 + 2 * (x + NV * sp)))] * rb[y];
       }
-      (sp == 0 ? wa : wb)[x] = w * v;
+      if (sp == 0) wa[x] = w * v;  // (a select of the two arrays would put them in scratch memory)
+      else wb[x] = w * v;
     }
 }
 
 // fused: rho1 of both spins from (aob_s, akb_s), then S_s into akb_s (both rho1 first: S overwrites)
 template <int DEN, typename T>
-__global__ void uks_rho1_wv_s_kernel(const double* aob_a, void* akb_a_v, const double* aob_b, void* akb_b_v,
+__global__ __launch_bounds__(256) void uks_rho1_wv_s_kernel(const double* aob_a, void* akb_a_v, const double* aob_b, void* akb_b_v,
                                      const double* fxc, const double* w, int64_t gb, int64_t nocc_a, int64_t nocc_b,
                                      int64_t nprop, int64_t ng, int64_t g0) {
   T* akb_a = static_cast<T*>(akb_a_v);
@@ -226,7 +227,7 @@ __global__ void uks_rho1_wv_s_kernel(const double* aob_a, void* akb_a_v, const d
 
 // S_s from a given rho1 [ng, NV, 2, nprop]
 template <int DEN, typename T>
-__global__ void uks_wv_s_kernel(const double* aob_a, const double* aob_b, const double* rho1, void* Sa_v, void* Sb_v,
+__global__ __launch_bounds__(256) void uks_wv_s_kernel(const double* aob_a, const double* aob_b, const double* rho1, void* Sa_v, void* Sb_v,
                                 const double* fxc, const double* w, int64_t gb, int64_t nocc_a, int64_t nocc_b,
                                 int64_t nprop, int64_t ng, int64_t g0) {
   T* Sa = static_cast<T*>(Sa_v);
@@ -248,7 +249,7 @@ __global__ void uks_wv_s_kernel(const double* aob_a, const double* aob_b, const 
 }
 
 template <int DEN>
-__global__ void uks_rho1_wv_s_tiled_kernel(const double* aob_a, const float* akb_a, const double* aob_b,
+__global__ __launch_bounds__(256) void uks_rho1_wv_s_tiled_kernel(const double* aob_a, const float* akb_a, const double* aob_b,
                                            const float* akb_b, unsigned short* hi_a, unsigned short* lo_a,
                                            int64_t cs_a, unsigned short* hi_b, unsigned short* lo_b, int64_t cs_b,
                                            const double* fxc, const double* w, int64_t gb, int64_t nocc_a,
@@ -266,7 +267,7 @@ __global__ void uks_rho1_wv_s_tiled_kernel(const double* aob_a, const float* akb
 }
 
 template <int DEN>
-__global__ void uks_wv_s_tiled_kernel(const double* aob_a, const double* aob_b, const double* rho1,
+__global__ __launch_bounds__(256) void uks_wv_s_tiled_kernel(const double* aob_a, const double* aob_b, const double* rho1,
                                       unsigned short* hi_a, unsigned short* lo_a, int64_t cs_a, unsigned short* hi_b,
                                       unsigned short* lo_b, int64_t cs_b, const double* fxc, const double* w,
                                       int64_t gb, int64_t nocc_a, int64_t nocc_b, int64_t nprop, int64_t ng,
@@ -336,7 +337,7 @@ __global__ void xceff_lda_kernel(int order, const double* xc, int64_t ld_xc, int
 // rho_s -> variable s; grad_t rho_a -> sigma_aa (2 grad_t rho_a), sigma_ab (grad_t rho_b);
 // grad_t rho_b -> sigma_ab (grad_t rho_a), sigma_bb (2 grad_t rho_b); tau_s -> 5 + s.
 template <int DEN>
-__global__ void xceff_pol_kernel(int order, const double* rho, int64_t ld_rho, int64_t ld_spin, const double* xc,
+__global__ __launch_bounds__(256) void xceff_pol_kernel(int order, const double* rho, int64_t ld_rho, int64_t ld_spin, const double* xc,
                                  int64_t ld_xc, int64_t ng, double* out) {
   constexpr int NV = DEN == DEN_SIGMA ? 4 : 5;
   constexpr int XL = DEN == DEN_SIGMA ? 5 : 7;
