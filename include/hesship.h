@@ -151,6 +151,9 @@ hh_status hh_numint_create(const double* ao, int64_t ng, int64_t nao, int64_t nc
 hh_status hh_numint_destroy(hh_numint* h);
 /* Grid points per batch (the reference's nbatch, nimatmul.rs:18-36); 0 = size from memory. */
 hh_status hh_numint_set_batch(hh_numint* h, int64_t nbatch);
+/* BF16X3 RKS response: compute the kets product inside the pointwise pass (default 1; no
+ * reference counterpart, results bit-identical either way; nocc <= 128, else unfused). */
+hh_status hh_numint_set_fused(hh_numint* h, int32_t on);
 /* rho1 [ng, nvar, nprop] from bra [nao, nocc] and kets [nao, nocc, nprop]
  * (make_rho_from_one_bra_mult_ket, nimatmul.rs:196-229 / pure_eval_rho.rs:216-334). */
 hh_status hh_numint_rho1_one_bra_mult_ket(hh_numint* h, int32_t den_type, const double* bra, int64_t nocc,

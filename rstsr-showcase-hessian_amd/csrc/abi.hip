@@ -423,6 +423,13 @@ hh_status hh_numint_set_batch(hh_numint* h, int64_t nbatch) {
   HH_ABI_END
 }
 
+hh_status hh_numint_set_fused(hh_numint* h, int32_t on) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h, "null handle");
+  h->impl->fused_kets = on != 0;
+  HH_ABI_END
+}
+
 hh_status hh_numint_rho1_one_bra_mult_ket(hh_numint* h, int32_t den_type, const double* bra, int64_t nocc,
                                           const double* kets, int64_t nprop, double* rho1, hh_precision precision) {
   HH_ABI_BEGIN

@@ -169,6 +169,135 @@ __global__ void rho1_wv_s_tiled_kernel(const double* aob, const float* akb, unsi
   }
 }
 
+// ---- BF16X3, RKS: the kets product fused with the pointwise pass ----
+// One workgroup (2 waves) takes 32 points of a grid batch and NA perturbations with all their
+// occupied orbitals (columns n = i + nocc a, 16 NFR at most).  akb[g, n, c] is computed on the
+// matrix cores from the tiled AO values (K = mu) and the tiled split kets: the same split-bf16
+// products, in the same order per output, as the NT kernel that writes akb for the unfused path
+// (one FP32 accumulation over mu, which is what that kernel does at K = naop).  akb stays in
+// LDS; rho1, the weighted kernel and the tiled S operand follow with the arithmetic of
+// rho1_wv_s_tiled_kernel, so S is bit-identical to the unfused path's and the FP32 akb array
+// (0.5 GB per batch at 72 perturbations) never reaches HBM.
+typedef float v4f_ni __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf_ni __attribute__((ext_vector_type(8)));
+
+template <int DEN, int NFR>
+__global__ __launch_bounds__(128) void kets_rho1_s_kernel(const unsigned short* __restrict__ aoth,
+                                                          const unsigned short* __restrict__ aotl, int64_t nt,
+                                                          int64_t ngp, const unsigned short* __restrict__ kh,
+                                                          const unsigned short* __restrict__ kl, int64_t ncol, int nkb,
+                                                          const double* __restrict__ aob, const double* __restrict__ fxc,
+                                                          const double* __restrict__ w, unsigned short* __restrict__ hi,
+                                                          unsigned short* __restrict__ lo, int64_t cs, int64_t gb,
+                                                          int64_t nocc, int64_t nprop, int na, int64_t ng, int64_t g0) {
+  constexpr int NC = DEN == DEN_RHO ? 1 : 4;
+  constexpr int NV = DEN == DEN_RHO ? 1 : (DEN == DEN_SIGMA ? 4 : 5);
+  constexpr int LDA = NFR * 16 + 4;  // LDS row stride (floats): conflict-free staging stores
+  __shared__ float akbs[NC][32 * LDA];
+  __shared__ double wvs[32 * 8 * NV];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t gt0 = (int64_t)blockIdx.x * 32;  // first point of the tile (batch-local)
+  const int64_t a0 = (int64_t)blockIdx.y * na;
+  const int nat = (int)min((int64_t)na, nprop - a0);
+  const int64_t n0 = nocc * a0;
+  const int nct = (int)(nocc * nat);  // columns of this tile
+  // kets: wave `wave` owns points gt0 + 16 wave .. + 15
+  v4f_ni acc[NFR][NC];
+#pragma unroll
+  for (int f = 0; f < NFR; ++f)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[f][c] = v4f_ni{0.f, 0.f, 0.f, 0.f};
+  const int64_t grow = g0 + gt0 + 16 * wave + (lane & 15);
+  const int kq = 8 * (lane >> 4);
+  for (int kb = 0; kb < nkb; ++kb) {
+    v8bf_ni bh[NFR], bl[NFR];
+#pragma unroll
+    for (int f = 0; f < NFR; ++f) {
+      const int nl = 16 * f + (lane & 15);
+      uint4 h = {0, 0, 0, 0}, l = {0, 0, 0, 0};
+      if (nl < nct) {
+        const int64_t off = ((int64_t)kb * ncol + n0 + nl) * 32 + kq;
+        h = *reinterpret_cast<const uint4*>(kh + off);
+        l = *reinterpret_cast<const uint4*>(kl + off);
+      }
+      bh[f] = __builtin_bit_cast(v8bf_ni, h);
+      bl[f] = __builtin_bit_cast(v8bf_ni, l);
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int64_t off = c * nt + ((int64_t)kb * ngp + grow) * 32 + kq;
+      const v8bf_ni ah = *reinterpret_cast<const v8bf_ni*>(aoth + off);
+      const v8bf_ni al = *reinterpret_cast<const v8bf_ni*>(aotl + off);
+#pragma unroll
+      for (int f = 0; f < NFR; ++f) {
+        acc[f][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[f], acc[f][c], 0, 0, 0);
+        acc[f][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[f], acc[f][c], 0, 0, 0);
+        acc[f][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[f], acc[f][c], 0, 0, 0);
+      }
+    }
+  }
+  // stage: the lane holds akb[point 16 wave + 4 (lane / 16) + r][column 16 f + lane % 16]
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+#pragma unroll
+    for (int f = 0; f < NFR; ++f)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        akbs[c][(16 * wave + 4 * (lane >> 4) + r) * LDA + 16 * f + (lane & 15)] = acc[f][c][r];
+  __syncthreads();
+  // rho1 and the weighted kernel per (point, perturbation)
+  const int64_t sc_b = gb * nocc;
+  for (int pr = tid; pr < 32 * nat; pr += 128) {
+    const int gl = pr & 31, al = pr >> 5;
+    const int64_t g = gt0 + gl;
+    if (g < gb) {
+      double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      for (int64_t i = 0; i < nocc; ++i) {
+        const double b0 = aob[g + gb * i];
+        const int col = (int)(i + nocc * al);
+        const double k0 = (double)akbs[0][gl * LDA + col];
+        r[0] += b0 * k0;
+        if (DEN != DEN_RHO) {
+#pragma unroll
+          for (int t = 1; t < 4; ++t) {
+            const double bt = aob[g + gb * i + sc_b * t];
+            const double kt = (double)akbs[t < NC ? t : 0][gl * LDA + col];
+            r[t] += b0 * kt + bt * k0;
+            if (DEN == DEN_TAU) r[4] += 0.5 * bt * kt;
+          }
+        }
+      }
+      double wv[5];
+      weighted_kernel<DEN>(fxc + g0, ng, g, w[g0 + g], r, wv);
+#pragma unroll
+      for (int x = 0; x < NV; ++x) wvs[(gl * 8 + al) * NV + x] = wv[x];
+    }
+  }
+  __syncthreads();
+  // tiled S: 4 groups of 32 lanes (one 32-point block: 64 contiguous bytes per column)
+  const int gl = tid & 31, q = tid >> 5;
+  const int64_t g = gt0 + gl;
+  if (g >= gb) return;
+  const int64_t tb = (g >> 5) * ncol * 32 + (g & 31);
+  for (int p = q; p < nct; p += 4) {
+    const int al = (int)(p / nocc);
+    const int64_t i = p - (int64_t)al * nocc;
+    const double* wv = wvs + (gl * 8 + al) * NV;
+    const double b0 = aob[g + gb * i];
+    const int64_t t = tb + (n0 + p) * 32;
+    if (DEN == DEN_RHO) {
+      put_split(hi, lo, t, b0 * wv[0]);
+    } else {
+      const double b1 = aob[g + gb * i + sc_b], b2 = aob[g + gb * i + 2 * sc_b], b3 = aob[g + gb * i + 3 * sc_b];
+      const double h = (DEN == DEN_TAU) ? 0.5 * wv[4] : 0.0;
+      put_split(hi, lo, t, b0 * wv[0] + b1 * wv[1] + b2 * wv[2] + b3 * wv[3]);
+      put_split(hi, lo, t + cs, b0 * wv[1] + h * b1);
+      put_split(hi, lo, t + 2 * cs, b0 * wv[2] + h * b2);
+      put_split(hi, lo, t + 3 * cs, b0 * wv[3] + h * b3);
+    }
+  }
+}
+
 template <int DEN>
 __global__ void wv_s_tiled_kernel(const double* aob, const double* rho1, unsigned short* hi, unsigned short* lo,
                                   int64_t cs, const double* fxc, const double* w, int64_t gb, int64_t nocc,
@@ -501,6 +630,41 @@ __global__ void ao_tiled_kmu_kernel(const double* __restrict__ ao, int64_t ng, i
 }
 }  // namespace
 
+template <int DEN>
+void launch_kets_rho1_s_den(int nfr, dim3 grid, hipStream_t s, const unsigned short* aoth, const unsigned short* aotl,
+                            int64_t nt, int64_t ngp, const unsigned short* kh, const unsigned short* kl, int64_t ncol,
+                            int nkb, const double* aob, const double* fxc, const double* w, unsigned short* hi,
+                            unsigned short* lo, int64_t cs, int64_t gb, int64_t nocc, int64_t nprop, int na, int64_t ng,
+                            int64_t g0) {
+#define HH_KRS(N)                                                                                                   \
+  case N:                                                                                                           \
+    hipLaunchKernelGGL((kets_rho1_s_kernel<DEN, N>), grid, dim3(128), 0, s, aoth, aotl, nt, ngp, kh, kl, ncol, nkb, \
+                       aob, fxc, w, hi, lo, cs, gb, nocc, nprop, na, ng, g0);                                       \
+    break;
+  switch (nfr) {
+    HH_KRS(1) HH_KRS(2) HH_KRS(3) HH_KRS(4) HH_KRS(5) HH_KRS(6) HH_KRS(7) HH_KRS(8)
+    default: throw Error(HH_ERR_INVALID, "numint: fused tile too wide");
+  }
+#undef HH_KRS
+  HH_HIP(hipGetLastError());
+}
+
+static void launch_kets_rho1_s(int den, int nfr, dim3 grid, hipStream_t s, const unsigned short* aoth,
+                        const unsigned short* aotl, int64_t nt, int64_t ngp, const unsigned short* kh,
+                        const unsigned short* kl, int64_t ncol, int nkb, const double* aob, const double* fxc,
+                        const double* w, unsigned short* hi, unsigned short* lo, int64_t cs, int64_t gb, int64_t nocc,
+                        int64_t nprop, int na, int64_t ng, int64_t g0) {
+  if (den == DEN_RHO)
+    launch_kets_rho1_s_den<DEN_RHO>(nfr, grid, s, aoth, aotl, nt, ngp, kh, kl, ncol, nkb, aob, fxc, w, hi, lo, cs, gb,
+                                    nocc, nprop, na, ng, g0);
+  else if (den == DEN_SIGMA)
+    launch_kets_rho1_s_den<DEN_SIGMA>(nfr, grid, s, aoth, aotl, nt, ngp, kh, kl, ncol, nkb, aob, fxc, w, hi, lo, cs,
+                                      gb, nocc, nprop, na, ng, g0);
+  else
+    launch_kets_rho1_s_den<DEN_TAU>(nfr, grid, s, aoth, aotl, nt, ngp, kh, kl, ncol, nkb, aob, fxc, w, hi, lo, cs, gb,
+                                    nocc, nprop, na, ng, g0);
+}
+
 // Tiled bf16 hi/lo copies of the AO values (components 0..3) for the BF16X3 grid GEMMs:
 // aok_* with K = grid point (back-contraction A operand, rows mu), aot_* with K = mu
 // (half-transform A operand, rows = grid points).  Built once per handle.
@@ -656,7 +820,13 @@ void Numint::rks_response_bra(int den, const double* fxc_eff, const double* mo1_
   const int64_t gbmax = batch_for(ncol, bf);
   const size_t es = bf ? sizeof(float) : sizeof(double);
   double* aob = static_cast<double*>(aob_.reserve(sizeof(double) * gbmax * nocc * nc));
-  void* akb = akb_.reserve(es * gbmax * ncol * nc);
+  // BF16X3: the kets product fused with the pointwise pass when a tile of whole perturbations
+  // fits 128 columns (nocc <= 128): fna perturbations, fnfr 16-column fragments per tile
+  const bool fused = bf && fused_kets && nocc <= 128;
+  const int fna = fused ? (int)std::max<int64_t>(1, std::min<int64_t>(8, 96 / nocc)) : 0;
+  const int fnfr = fused ? (int)ceil_div(fna * nocc, (int64_t)16) : 0;
+  if (fused) HH_REQUIRE(ceil_div(nprop, (int64_t)fna) < 65536, "numint: too many perturbations for one launch");
+  void* akb = fused ? nullptr : akb_.reserve(es * gbmax * ncol * nc);
   if (bf) {
     ensure_bf16();
     split_kets(mo1_bra, ncol, kh_, kl_);
@@ -664,16 +834,25 @@ void Numint::rks_response_bra(int den, const double* fxc_eff, const double* mo1_
   for (int64_t g0 = 0; g0 < ng_; g0 += gbmax) {
     const int64_t gb = std::min(gbmax, ng_ - g0);
     half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, mocc, nocc, aob, PREC_F64, stream_, ws_);
-    if (bf) kets_bf16(nc, g0, gb, kh_.as<unsigned short>(), kl_.as<unsigned short>(), ncol, static_cast<float*>(akb));
-    else half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, mo1_bra, ncol, static_cast<double*>(akb), prec,
-                        stream_, ws_);
+    if (bf && !fused)
+      kets_bf16(nc, g0, gb, kh_.as<unsigned short>(), kl_.as<unsigned short>(), ncol, static_cast<float*>(akb));
+    else if (!bf)
+      half_transform(ao_.as<double>(), ng_, nao_, g0, gb, nc, mo1_bra, ncol, static_cast<double*>(akb), prec, stream_,
+                     ws_);
     // the x4 closed-shell factor of hess_rks.rs:603
     if (bf) {
       unsigned short *h, *l;
       s_tiled(nc, gb, ncol, sh_, sl_, &h, &l);
-      HH_DEN_DISPATCH(den, rho1_wv_s_tiled_kernel, dim3(nblk(gb * nprop)), dim3(256), 0, stream_, aob,
-                      static_cast<const float*>(akb), h, l, ceil_div(gb, 32) * 32 * ncol, fxc_eff, w_.as<double>(),
-                      gb, nocc, nprop, ng_, g0);
+      const int64_t cs = ceil_div(gb, 32) * 32 * ncol;
+      if (fused) {
+        const dim3 grid((unsigned)ceil_div(gb, 32), (unsigned)ceil_div(nprop, (int64_t)fna));
+        launch_kets_rho1_s(den, fnfr, grid, stream_, aot_hi_.as<unsigned short>(), aot_lo_.as<unsigned short>(),
+                           naop_ * ngp_, ngp_, kh_.as<unsigned short>(), kl_.as<unsigned short>(), ncol,
+                           (int)(naop_ / 32), aob, fxc_eff, w_.as<double>(), h, l, cs, gb, nocc, nprop, fna, ng_, g0);
+      } else {
+        HH_DEN_DISPATCH(den, rho1_wv_s_tiled_kernel, dim3(nblk(gb * nprop)), dim3(256), 0, stream_, aob,
+                        static_cast<const float*>(akb), h, l, cs, fxc_eff, w_.as<double>(), gb, nocc, nprop, ng_, g0);
+      }
       back_bf16(nc, g0, gb, h, l, ncol, out, 4.0, accumulate || g0 > 0);
     } else {
       HH_DEN_DISPATCH_T(den, false, rho1_wv_s_kernel, dim3(nblk(gb * nprop)), dim3(256), 0, stream_, aob, akb,

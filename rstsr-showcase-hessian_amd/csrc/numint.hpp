@@ -32,6 +32,9 @@ class Numint {
   // fused per grid batch so rho1 never leaves the batch.
   void rks_response_bra(int den, const double* fxc_eff, const double* mo1_bra, int64_t nprop, const double* mocc,
                         int64_t nocc, double* out, Prec prec, bool accumulate = false);
+  // BF16X3 rks_response_bra: the kets product fused with the pointwise pass (nocc <= 128; the
+  // results are bit-identical to the unfused kernels').  Default on.
+  bool fused_kets = true;
 
   // UKS, one array per spin (pure_xcpot.rs:921-1028): out[s][nao, nocc_s, nprop] (+)= alpha *
   // bra[s]-transformed potential of rho1 [ng, nvar, 2, nprop] under fxc_eff [ng, nvar, 2, nvar, 2],

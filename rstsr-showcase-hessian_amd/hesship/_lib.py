@@ -101,6 +101,7 @@ SIGNATURES.update({
     "hh_numint_create": (_i32, [_p, _i64, _i64, _i64, _p, _p, C.POINTER(C.c_void_p)]),
     "hh_numint_destroy": (_i32, [_p]),
     "hh_numint_set_batch": (_i32, [_p, _i64]),
+    "hh_numint_set_fused": (_i32, [_p, _i32]),
     "hh_numint_rho1_one_bra_mult_ket": (_i32, [_p, _i32, _p, _i64, _p, _i64, _p, _i32]),
     "hh_numint_fxc_pot_bra_trans": (_i32, [_p, _i32, _p, _p, _i64, _p, _i64, _p, _i32]),
     "hh_numint_rks_response_bra": (_i32, [_p, _i32, _p, _p, _i64, _p, _i64, _p, _i32]),

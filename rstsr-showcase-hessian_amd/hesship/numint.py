@@ -32,6 +32,10 @@ class NIMatmul:
         if nbatch:
             check(lib().hh_numint_set_batch(self._h, int(nbatch)))
 
+    def set_fused(self, on=True):
+        """BF16X3 RKS response: kets product fused with the pointwise pass (default on)."""
+        check(lib().hh_numint_set_fused(self._h, int(bool(on))))
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:

@@ -57,6 +57,26 @@ def test_rks_response_mixed(dev, prec, tol):
     assert 1e-14 < e < tol, e
 
 
+@pytest.mark.parametrize("den", ["rho", "sigma", "tau"])
+@pytest.mark.parametrize("nocc,nset,nbatch", [(6, 4, 0), (5, 23, 96), (40, 5, 0), (50, 3, 160), (97, 2, 0),
+                                              (130, 2, 0)])
+def test_rks_response_bf16x3_fused_matches_unfused(dev, den, nocc, nset, nbatch):
+    """The BF16X3 response with the kets product fused into the pointwise pass equals the
+    unfused kernels' result bit for bit (tiles of 1..8 perturbations, partial tiles, batches
+    that end inside a 32-point block; nocc 130 falls back to the unfused path), and both match
+    the oracle at the BF16x3 level."""
+    import hesship
+    nao = 37
+    ao, w, bra, kets, f = grid_case(ng=1111, nao=nao, nocc=nocc, nset=nset, seed=nocc + nset, den=den)
+    ni = hesship.NIMatmul(T(ao, dev), T(w, dev), nbatch=nbatch)
+    ni.set_fused(True)
+    fused = ni.get_rks_response_bra(den, T(f, dev), T(kets, dev), T(bra, dev), precision="bf16x3")
+    ni.set_fused(False)
+    plain = ni.get_rks_response_bra(den, T(f, dev), T(kets, dev), T(bra, dev), precision="bf16x3")
+    assert torch.equal(fused, plain)
+    assert rel(fused.cpu().numpy(), oni.rks_response_bra(den, f, ao, w, kets, bra)) < 1e-4
+
+
 @pytest.mark.parametrize("den", ["sigma", "tau"])
 def test_xceff_transform(dev, den):
     import hesship
