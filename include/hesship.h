@@ -58,12 +58,27 @@ typedef struct hh_rijk hh_rijk;
  * hess_r.rs:1775-1794) or 2 (UHF, UHessRIJK, hess_u.rs:38-146). */
 hh_status hh_rijk_create(const double* cderi, int64_t nao, int64_t naux, double scale_j, double scale_k,
                          int32_t nspin, void* stream, hh_rijk** out);
+/* Decomposition policy of the RI metric (J2CDecompPolicy, src/ri_jk/decompose.rs:8-16). */
+#define HH_J2C_CD 0               /* Cholesky j2c = L L^T, cderi = j3c L^-T */
+#define HH_J2C_EIG 1              /* eigen: cderi = j3c J^-1/2 over eigenvalues >= threshold */
+
 /* The same from the 3-centre integrals: j3c [nao_tp, naux] (packed upper, column-major) and the
- * metric j2c [naux, naux] (device).  j2c = L L^T by Cholesky on the device in FP64, then
- * cderi = j3c L^-T (replaces generate_cderi_with_decomp, hess_r.rs:1725-1735, with
- * J2CDecompOption {Cd, threshold, ..}, decomp_j2c_cd pure_decompose.rs:28-55).  A factor with a
- * diagonal element <= threshold gives HH_ERR_INVALID: the reference's eigenvalue-floor fallback
- * is not provided. */
+ * metric j2c [naux, naux] (device), decomposed on the device in FP64 (generate_cderi_with_decomp,
+ * hess_r.rs:1725-1735, with get_j2c_decomp, pure_decompose.rs:61-73; threshold > 0 is
+ * Some(threshold), threshold <= 0 is None):
+ *   HH_J2C_CD  (decomp_j2c_cd, pure_decompose.rs:28-55): j2c = L L^T, cderi = j3c L^-T.  If the
+ *              Cholesky factorisation fails or a diagonal element of L is <= threshold, j2c is
+ *              rebuilt as V max(e, threshold) V^T from its eigendecomposition and factorised again;
+ *              with no threshold a failure gives HH_ERR_INVALID (the reference panics).
+ *   HH_J2C_EIG (decomp_j2c_eig, pure_decompose.rs:11-20): J^-1/2 = V_k e_k^-1/2 V_k^T over the
+ *              eigenvalues e_k >= threshold (no threshold: J2C_THRESH = 1e-13, decompose.rs:5),
+ *              cderi = j3c J^-1/2 (solve_by_j2c with J2CDecompose::Eig, side Right).
+ * The handle keeps the factor (L, or J^-1/2) for the skeleton's metric terms. */
+hh_status hh_rijk_create_from_j3c_decomp(const double* j3c, const double* j2c, int64_t nao, int64_t naux,
+                                         int32_t policy, double threshold, double scale_j, double scale_k,
+                                         int32_t nspin, void* stream, hh_rijk** out);
+/* hh_rijk_create_from_j3c_decomp with HH_J2C_CD (RHessRIJK::new_without_cderi uses
+ * J2CDecompOption {Cd, Some(1e-14), Upper}, hess_r.rs:1751-1753). */
 hh_status hh_rijk_create_from_j3c(const double* j3c, const double* j2c, int64_t nao, int64_t naux, double threshold,
                                   double scale_j, double scale_k, int32_t nspin, void* stream, hh_rijk** out);
 hh_status hh_rijk_destroy(hh_rijk* h);

@@ -179,3 +179,40 @@ def cderi_from_j3c(j3c_tp, j2c):
     """pure_decompose.rs:107-133 + hess_r.rs:1725-1735: cderi = j3c L^-T with j2c = L L^T."""
     L = np.linalg.cholesky(j2c)
     return np.linalg.solve(L, j3c_tp.T).T
+
+
+J2C_THRESH = 1e-13  # decompose.rs:5
+
+
+def decomp_j2c_eig(j2c, threshold=J2C_THRESH):
+    """decomp_j2c_eig (pure_decompose.rs:11-20): eigenvalues ascending, the first n = #(e < threshold)
+    discarded, J^-1/2 = V_k diag(e_k^-1/2) V_k^T over the rest (symmetric)."""
+    e, v = np.linalg.eigh(j2c)
+    n = int((e < threshold).sum())
+    vk = v[:, n:]
+    return (vk * e[n:] ** -0.5) @ vk.T
+
+
+def decomp_j2c_cd(j2c, threshold=None):
+    """decomp_j2c_cd (pure_decompose.rs:28-55), lower factor: Cholesky; with a threshold, when the
+    factorisation fails or min(diag L) <= threshold, j2c is rebuilt as V max(e, threshold) V^T and
+    factorised again; with none, a failure raises (the reference panics)."""
+    try:
+        L = np.linalg.cholesky(j2c)
+    except np.linalg.LinAlgError:
+        if threshold is None:
+            raise
+        L = None
+    if threshold is None or (L is not None and L.diagonal().min() > threshold):
+        return L
+    e, v = np.linalg.eigh(j2c)
+    return np.linalg.cholesky((v * np.maximum(e, threshold)) @ v.T)
+
+
+def cderi_from_j3c_decomp(j3c_tp, j2c, policy="cd", threshold=None):
+    """generate_cderi_with_decomp (hess_r.rs:1725-1735) with solve_by_j2c side Right
+    (pure_decompose.rs:179-380): cderi = j3c L^-T (policy "cd") or j3c J^-1/2 (policy "eig")."""
+    if policy == "eig":
+        return j3c_tp @ decomp_j2c_eig(j2c, J2C_THRESH if threshold is None else threshold)
+    L = decomp_j2c_cd(j2c, threshold)
+    return np.linalg.solve(L, j3c_tp.T).T

@@ -147,16 +147,23 @@ hh_status hh_rijk_create(const double* cderi, int64_t nao, int64_t naux, double 
   HH_ABI_END
 }
 
-hh_status hh_rijk_create_from_j3c(const double* j3c, const double* j2c, int64_t nao, int64_t naux, double threshold,
-                                  double scale_j, double scale_k, int32_t nspin, void* stream, hh_rijk** out) {
+hh_status hh_rijk_create_from_j3c_decomp(const double* j3c, const double* j2c, int64_t nao, int64_t naux,
+                                         int32_t policy, double threshold, double scale_j, double scale_k,
+                                         int32_t nspin, void* stream, hh_rijk** out) {
   HH_ABI_BEGIN
   HH_REQUIRE(out != nullptr, "out is null");
   *out = nullptr;
   HH_REQUIRE(nspin == 1 || nspin == 2, "nspin must be 1 (RHF) or 2 (UHF)");
   auto h = std::make_unique<hh_rijk>();
-  h->impl = Rijk::from_j3c(j3c, j2c, nao, naux, threshold, scale_j, scale_k, nspin, (hipStream_t)stream);
+  h->impl = Rijk::from_j3c(j3c, j2c, nao, naux, policy, threshold, scale_j, scale_k, nspin, (hipStream_t)stream);
   *out = h.release();
   HH_ABI_END
+}
+
+hh_status hh_rijk_create_from_j3c(const double* j3c, const double* j2c, int64_t nao, int64_t naux, double threshold,
+                                  double scale_j, double scale_k, int32_t nspin, void* stream, hh_rijk** out) {
+  return hh_rijk_create_from_j3c_decomp(j3c, j2c, nao, naux, HH_J2C_CD, threshold, scale_j, scale_k, nspin, stream,
+                                        out);
 }
 
 hh_status hh_rijk_destroy(hh_rijk* h) {

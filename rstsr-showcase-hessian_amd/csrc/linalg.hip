@@ -5,6 +5,9 @@
 // matrix-core GEMM.  Pivot choice: largest |a|, ties to the smallest row index (deterministic).
 #include "linalg.hpp"
 
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 namespace hh {
 
 namespace {
@@ -380,6 +383,28 @@ void trsm_lower(const double* L, int64_t nd, double* X, int64_t n2, int64_t ldx,
       }
     }
   }
+}
+
+void eigh(double* H, int64_t nd, double* w, hipStream_t s, DevBuf& ws) {
+  HH_REQUIRE(nd > 0 && nd < (int64_t)1 << 31, "eigh: dimension out of range");
+  rocblas_handle handle = nullptr;
+  HH_REQUIRE(rocblas_create_handle(&handle) == rocblas_status_success, "eigh: rocblas_create_handle failed");
+  struct Guard {
+    rocblas_handle h;
+    ~Guard() { (void)rocblas_destroy_handle(h); }
+  } guard{handle};
+  HH_REQUIRE(rocblas_set_stream(handle, s) == rocblas_status_success, "eigh: rocblas_set_stream failed");
+  // workspace: the off-diagonal of the tridiagonal form (nd doubles) and the info word
+  char* base = static_cast<char*>(ws.reserve(sizeof(double) * nd + 256));
+  double* e = reinterpret_cast<double*>(base);
+  rocblas_int* info = reinterpret_cast<rocblas_int*>(base + sizeof(double) * nd);
+  const rocblas_status st = rocsolver_dsyevd(handle, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)nd, H,
+                                             (rocblas_int)nd, w, e, info);
+  HH_REQUIRE(st == rocblas_status_success, "eigh: rocsolver_dsyevd returned status " + std::to_string((int)st));
+  rocblas_int hinfo = 0;
+  HH_HIP(hipMemcpyAsync(&hinfo, info, sizeof(rocblas_int), hipMemcpyDeviceToHost, s));
+  HH_HIP(hipStreamSynchronize(s));
+  if (hinfo != 0) throw Error(HH_ERR_INVALID, "eigh: syevd did not converge (info " + std::to_string(hinfo) + ")");
 }
 
 }  // namespace hh

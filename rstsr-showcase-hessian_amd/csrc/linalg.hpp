@@ -19,4 +19,11 @@ bool cholesky(double* H, int64_t nd, hipStream_t s, DevBuf& ws);
 void trsm_lower(const double* L, int64_t nd, double* X, int64_t nrow_or_ncol, int64_t ldx, bool side_right,
                 hipStream_t s, DevBuf& ws);
 
+// Symmetric eigendecomposition H = V diag(w) V^T in FP64 on the device (rstsr linalg eigh as
+// used by decomp_j2c_eig / decomp_j2c_cd, pure_decompose.rs:11-20, 41-45): the lower triangle
+// of H [nd, nd] is read, H is overwritten by the orthonormal eigenvectors (columns), w [nd]
+// (device) receives the eigenvalues in ascending order.  rocSOLVER syevd (tridiagonal reduction
+// and divide and conquer); throws Error(HH_ERR_INVALID) if it does not converge.
+void eigh(double* H, int64_t nd, double* w, hipStream_t s, DevBuf& ws);
+
 }  // namespace hh

@@ -51,33 +51,92 @@ __global__ void diag_min_kernel(const double* L, int64_t n, double* out) {
   }
   if (threadIdx.x == 0) *out = red[0];
 }
+
+// V[:, j] *= f(e_j): mode 0 (eigenvalue floor of decomp_j2c_cd) f = max(e, thr);
+// mode 1 (decomp_j2c_eig) f = e^-1/2 for e >= thr, 0 for the discarded eigenvalues.
+__global__ void eig_scale_cols_kernel(const double* V, const double* e, int64_t n, double thr, int mode, double* W) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * n) return;
+  const double ej = e[t / n];
+  const double f = mode == 0 ? fmax(ej, thr) : (ej >= thr ? 1.0 / sqrt(ej) : 0.0);
+  W[t] = V[t] * f;
+}
 }  // namespace
 
-std::unique_ptr<Rijk> Rijk::from_j3c(const double* j3c, const double* j2c, int64_t nao, int64_t naux,
+namespace {
+// A = W V^T  (n x n x n FP64 product on the matrix cores)
+void outer_wvt(const double* W, const double* V, int64_t n, double* A, hipStream_t s, DevBuf& ws) {
+  GemmArgs g;
+  g.M = n, g.N = n, g.K = n;
+  g.A = W, g.lda = n, g.a_kmajor = false;  // W[m, k] at m + k n
+  g.B = V, g.ldb = n, g.b_kmajor = false;  // V^T[k, j] = V[j, k] at j + k n
+  g.C = A, g.ldc = n;
+  gemm<double, double>(g, PREC_F64, s, ws);
+}
+}  // namespace
+
+std::unique_ptr<Rijk> Rijk::from_j3c(const double* j3c, const double* j2c, int64_t nao, int64_t naux, int policy,
                                      double threshold, double scale_j, double scale_k, int nspin, hipStream_t s) {
   HH_REQUIRE(j3c && j2c && nao > 0 && naux > 0, "from_j3c: null or empty input");
+  HH_REQUIRE(policy == HH_J2C_CD || policy == HH_J2C_EIG, "from_j3c: policy must be HH_J2C_CD or HH_J2C_EIG");
+  const bool has_thr = threshold > 0.0;  // Some(threshold); <= 0 or NaN: None
   const int64_t ntp = nao * (nao + 1) / 2;
-  DevBuf L, cd, ws, dmin;
-  double* Lp = static_cast<double*>(L.reserve(sizeof(double) * naux * naux));
-  HH_HIP(hipMemcpyAsync(Lp, j2c, sizeof(double) * naux * naux, hipMemcpyDeviceToDevice, s));
-  const bool ok = cholesky(Lp, naux, s, ws);
-  if (!ok) throw Error(HH_ERR_INVALID, "j2c is not positive definite (Cholesky failed); eigenvalue-floor fallback "
-                                       "of decomp_j2c_cd is not provided by this library");
-  double* dm = static_cast<double*>(dmin.reserve(sizeof(double)));
-  hipLaunchKernelGGL(diag_min_kernel, dim3(1), dim3(256), 0, s, Lp, naux, dm);
-  double hmin = 0.0;
-  HH_HIP(hipMemcpyAsync(&hmin, dm, sizeof(double), hipMemcpyDeviceToHost, s));
-  HH_HIP(hipStreamSynchronize(s));
-  if (!(hmin > threshold))
-    throw Error(HH_ERR_INVALID, "j2c Cholesky factor has a diagonal element <= threshold (" + std::to_string(hmin) +
-                                    "); eigenvalue-floor fallback of decomp_j2c_cd is not provided by this library");
+  const int64_t nn = naux * naux;
+  const dim3 grid((unsigned)((nn + 255) / 256)), block(256);
+  DevBuf F, cd, ws, aux;
+  double* Fp = static_cast<double*>(F.reserve(sizeof(double) * nn));
   double* cdp = static_cast<double*>(cd.reserve(sizeof(double) * ntp * naux));
-  HH_HIP(hipMemcpyAsync(cdp, j3c, sizeof(double) * ntp * naux, hipMemcpyDeviceToDevice, s));
-  trsm_lower(Lp, naux, cdp, ntp, ntp, true, s, ws);
+  if (policy == HH_J2C_EIG) {
+    const double thr = has_thr ? threshold : 1e-13;  // J2C_THRESH (decompose.rs:5)
+    double* V = static_cast<double*>(aux.reserve(sizeof(double) * (2 * nn + naux)));
+    double* W = V + nn;
+    double* e = W + nn;
+    HH_HIP(hipMemcpyAsync(V, j2c, sizeof(double) * nn, hipMemcpyDeviceToDevice, s));
+    eigh(V, naux, e, s, ws);
+    hipLaunchKernelGGL(eig_scale_cols_kernel, grid, block, 0, s, V, e, naux, thr, 1, W);
+    HH_HIP(hipGetLastError());
+    outer_wvt(W, V, naux, Fp, s, ws);  // J^-1/2
+    // cderi = j3c J^-1/2  [ntp, naux]
+    GemmArgs g;
+    g.M = ntp, g.N = naux, g.K = naux;
+    g.A = j3c, g.lda = ntp, g.a_kmajor = false;
+    g.B = Fp, g.ldb = naux, g.b_kmajor = true;
+    g.C = cdp, g.ldc = ntp;
+    gemm<double, double>(g, PREC_F64, s, ws);
+  } else {
+    HH_HIP(hipMemcpyAsync(Fp, j2c, sizeof(double) * nn, hipMemcpyDeviceToDevice, s));
+    bool ok = cholesky(Fp, naux, s, ws);
+    double hmin = 0.0;
+    if (ok) {
+      double* dm = static_cast<double*>(aux.reserve(sizeof(double)));
+      hipLaunchKernelGGL(diag_min_kernel, dim3(1), dim3(256), 0, s, Fp, naux, dm);
+      HH_HIP(hipMemcpyAsync(&hmin, dm, sizeof(double), hipMemcpyDeviceToHost, s));
+      HH_HIP(hipStreamSynchronize(s));
+    }
+    if (!ok && !has_thr)
+      throw Error(HH_ERR_INVALID, "Cholesky decomposition failed for j2c matrix, and no threshold is provided for "
+                                  "fallback");
+    if (has_thr && !(ok && hmin > threshold)) {
+      // eigenvalue floor: j2c <- V max(e, threshold) V^T, then Cholesky again
+      double* V = static_cast<double*>(aux.reserve(sizeof(double) * (2 * nn + naux)));
+      double* W = V + nn;
+      double* e = W + nn;
+      HH_HIP(hipMemcpyAsync(V, j2c, sizeof(double) * nn, hipMemcpyDeviceToDevice, s));
+      eigh(V, naux, e, s, ws);
+      hipLaunchKernelGGL(eig_scale_cols_kernel, grid, block, 0, s, V, e, naux, threshold, 0, W);
+      HH_HIP(hipGetLastError());
+      outer_wvt(W, V, naux, Fp, s, ws);
+      if (!cholesky(Fp, naux, s, ws))
+        throw Error(HH_ERR_INVALID, "j2c with its eigenvalues floored at the threshold is not positive definite");
+    }
+    HH_HIP(hipMemcpyAsync(cdp, j3c, sizeof(double) * ntp * naux, hipMemcpyDeviceToDevice, s));
+    trsm_lower(Fp, naux, cdp, ntp, ntp, true, s, ws);
+  }
   auto r = std::make_unique<Rijk>(cdp, nao, naux, scale_j, scale_k, nspin, s);
   HH_HIP(hipStreamSynchronize(s));
-  std::swap(r->j2c_l_.ptr, L.ptr);
-  std::swap(r->j2c_l_.bytes, L.bytes);
+  DevBuf& keep = policy == HH_J2C_EIG ? r->j2c_linv_ : r->j2c_l_;
+  std::swap(keep.ptr, F.ptr);
+  std::swap(keep.bytes, F.bytes);
   return r;
 }
 

@@ -16,12 +16,18 @@ class Rijk {
   ~Rijk();
   Rijk(const double* cderi, int64_t nao, int64_t naux, double scale_j, double scale_k, int nspin, hipStream_t s);
   // generate_cderi_with_decomp (hess_r.rs:1725-1735) from device j3c [nao_tp, naux] (packed
-  // upper, column-major) and j2c [naux, naux]: j2c = L L^T (decomp_j2c_cd,
-  // pure_decompose.rs:28-55), cderi = j3c L^-T (solve_by_j2c, side Right).  Throws if the
-  // Cholesky factor has a diagonal element <= threshold.
-  static std::unique_ptr<Rijk> from_j3c(const double* j3c, const double* j2c, int64_t nao, int64_t naux,
+  // upper, column-major) and j2c [naux, naux].  policy HH_J2C_CD: j2c = L L^T (decomp_j2c_cd,
+  // pure_decompose.rs:28-55, with the eigenvalue-floor refactorisation when threshold > 0),
+  // cderi = j3c L^-T; HH_J2C_EIG: cderi = j3c J^-1/2 over eigenvalues >= threshold
+  // (decomp_j2c_eig, pure_decompose.rs:11-20).  Throws if the Cholesky factorisation fails with
+  // no threshold.
+  static std::unique_ptr<Rijk> from_j3c(const double* j3c, const double* j2c, int64_t nao, int64_t naux, int policy,
                                         double threshold, double scale_j, double scale_k, int nspin, hipStream_t s);
+  // Lower Cholesky factor L of the metric (HH_J2C_CD handles), or null.
   const double* j2c_l() const { return j2c_l_.ptr ? j2c_l_.as<double>() : nullptr; }
+  // Symmetric J^-1/2 (HH_J2C_EIG handles), or null.  Either factor F gives V^-1 = F^-T F^-1
+  // (L) or F^T F (J^-1/2) and cderi = j3c F^-T (L) or j3c F (J^-1/2).
+  const double* j2c_l_inv() const { return j2c_linv_.ptr ? j2c_linv_.as<double>() : nullptr; }
   // B [nao, nao, naux] (cderi unpacked, FP64)
   const double* bfull() const { return bfull_.as<double>(); }
 
@@ -86,7 +92,8 @@ class Rijk {
   bool xt_ready_ = false;
   int64_t nocc_[2] = {0, 0};
 
-  DevBuf j2c_l_;      // [naux, naux] lower Cholesky factor of the metric (from_j3c only)
+  DevBuf j2c_l_;      // [naux, naux] lower Cholesky factor of the metric (from_j3c, HH_J2C_CD)
+  DevBuf j2c_linv_;   // [naux, naux] J^-1/2 of the metric (from_j3c, HH_J2C_EIG)
   DevBuf cderi_;      // [nao_tp, naux] f64
   DevBuf bfull_;      // [nao, nao, naux] f64 (unpacked)
   int64_t kpad_ = 0;

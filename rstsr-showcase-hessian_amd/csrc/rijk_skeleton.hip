@@ -280,7 +280,7 @@ void rijk_skeleton(Rijk& r, const SkeletonInput& in, double* de, double* const* 
   const bool doJ = sj != 0.0, doK = sk != 0.0;
   HH_REQUIRE(in.int3c && in.j2c_ip1 && in.j2c_ipip1 && in.j2c_ip1ip2 && de, "skeleton: null argument");
   HH_REQUIRE(N > 0 && in.aoslice && in.auxslice, "skeleton: no atoms");
-  HH_REQUIRE(in.j2c || r.j2c_l(), "skeleton: j2c is required for a handle made from cderi");
+  HH_REQUIRE(in.j2c || r.j2c_l() || r.j2c_l_inv(), "skeleton: j2c is required for a handle made from cderi");
   // ranges and slices
   HH_REQUIRE(in.nrange > 0 && in.aux_ranges, "skeleton: no auxiliary ranges");
   int64_t nbmax = 0;
@@ -344,18 +344,26 @@ void rijk_skeleton(Rijk& r, const SkeletonInput& in, double* de, double* const* 
     HH_HIP(hipMemcpyAsync(d_sc[s], occsq[s].data(), sizeof(double) * no[s], hipMemcpyHostToDevice, st));
   }
 
-  // metric factor and its inverse
+  // metric factor inverse Li with V^-1 = Li^T Li: L^-1 of the Cholesky factor, or the handle's
+  // symmetric J^-1/2 (eigen decomposition, the pseudo-inverse over the kept eigenvalues)
   const double* L = r.j2c_l();
+  const double* Li = nullptr;
   if (in.j2c) {
     double* Lw = D(bL, m * m);
     copy_bytes(Lw, in.j2c, (int64_t)sizeof(double) * m * m, st);
     HH_REQUIRE(cholesky(Lw, m, st, ws), "skeleton: j2c is not positive definite");
     L = Lw;
+  } else if (!L) {
+    Li = r.j2c_l_inv();
   }
-  double* Li = D(bLi, m * m);
-  hipLaunchKernelGGL(eye_kernel, GRID(m * m), Li, m);
-  launch_check();
-  trsm_lower(L, m, Li, m, m, false, st, ws);
+  HH_REQUIRE(L || Li, "skeleton: no metric given and the handle keeps no factor of it");
+  if (!Li) {
+    double* Lw = D(bLi, m * m);
+    hipLaunchKernelGGL(eye_kernel, GRID(m * m), Lw, m);
+    launch_check();
+    trsm_lower(L, m, Lw, m, m, false, st, ws);
+    Li = Lw;
+  }
 
   const double* Bf = r.bfull();
   double* mo[2];

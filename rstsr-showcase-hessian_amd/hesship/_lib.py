@@ -35,6 +35,8 @@ SIGNATURES = {
     "hh_rijk_create": (_i32, [_p, _i64, _i64, _d, _d, _i32, _p, C.POINTER(C.c_void_p)]),
     "hh_rijk_destroy": (_i32, [_p]),
     "hh_rijk_create_from_j3c": (_i32, [_p, _p, _i64, _i64, _d, _d, _d, _i32, _p, C.POINTER(C.c_void_p)]),
+    "hh_rijk_create_from_j3c_decomp": (_i32, [_p, _p, _i64, _i64, _i32, _d, _d, _d, _i32, _p,
+                                              C.POINTER(C.c_void_p)]),
     "hh_rijk_prepare_response": (_i32, [_p, _pp, _pp, _pi64]),
     "hh_rijk_nocc": (_i32, [_p, _i32, _pi64]),
     "hh_rijk_get_response_bra": (_i32, [_p, _pp, _i64, _pp, _i32]),

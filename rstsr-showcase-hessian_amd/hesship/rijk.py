@@ -11,6 +11,9 @@ import torch
 from ._lib import PREC, check, lib, ptr_array
 from .tensors import empty_f, fortran, stream_of
 
+# RI metric decomposition policies (include/hesship.h HH_J2C_*)
+J2C_CD, J2C_EIG = 0, 1
+
 
 def _host_occ(mo_occ):
     if isinstance(mo_occ, torch.Tensor):
@@ -42,7 +45,7 @@ def unpack_triu(tp, nao):
 class _RIJKBase:
     nspin = 1
 
-    def __init__(self, cderi, scale_j, scale_k, nao=None, _j2c=None, _threshold=1e-14):
+    def __init__(self, cderi, scale_j, scale_k, nao=None, _j2c=None, _threshold=1e-14, _policy="cd"):
         nao_tp, naux = cderi.shape
         if nao is None:
             nao = int(round((np.sqrt(8 * nao_tp + 1) - 1) / 2))
@@ -57,16 +60,22 @@ class _RIJKBase:
                                        stream_of(self.device), C.byref(h)))
         else:
             j2 = fortran(_j2c)  # held across the call
-            check(lib().hh_rijk_create_from_j3c(cd.data_ptr(), j2.data_ptr(), nao, naux, float(_threshold),
-                                                self.scale_j, self.scale_k, self.nspin, stream_of(self.device),
-                                                C.byref(h)))
+            pol = {"cd": J2C_CD, "eig": J2C_EIG}[_policy]
+            thr = 0.0 if _threshold is None else float(_threshold)  # None: no threshold
+            check(lib().hh_rijk_create_from_j3c_decomp(cd.data_ptr(), j2.data_ptr(), nao, naux, pol, thr,
+                                                       self.scale_j, self.scale_k, self.nspin,
+                                                       stream_of(self.device), C.byref(h)))
         self._h = h
         self.nocc = None
 
     @classmethod
-    def from_j3c(cls, j3c_tp, j2c, scale_j, scale_k, threshold=1e-14):
-        """cderi generated on the device from j3c [nao_tp, naux] and j2c (hess_r.rs:1725-1735)."""
-        return cls(j3c_tp, scale_j, scale_k, _j2c=j2c, _threshold=threshold)
+    def from_j3c(cls, j3c_tp, j2c, scale_j, scale_k, threshold=1e-14, policy="cd"):
+        """cderi generated on the device from j3c [nao_tp, naux] and j2c (generate_cderi_with_decomp,
+        hess_r.rs:1725-1735).  policy "cd" (decomp_j2c_cd, pure_decompose.rs:28-55: Cholesky, and the
+        eigenvalue floor at `threshold` when it fails; threshold None: raise instead) or "eig"
+        (decomp_j2c_eig, pure_decompose.rs:11-20: J^-1/2 over eigenvalues >= threshold, default
+        1e-13).  The defaults are RHessRIJK::new_without_cderi's option (hess_r.rs:1752)."""
+        return cls(j3c_tp, scale_j, scale_k, _j2c=j2c, _threshold=threshold, _policy=policy)
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -166,4 +166,58 @@ def test_cderi_generated_from_j3c_and_metric(dev):
     ref = orc.response_bra_naive(j3c, j2c, C, occ, bra, 1.0, 1.0)
     assert rel(got, ref) < 1e-9
     with pytest.raises(HesshipError):
-        RHessRIJK.from_j3c(T(orc.pack_triu(j3c), dev), T(-j2c, dev), 1.0, 1.0)
+        RHessRIJK.from_j3c(T(orc.pack_triu(j3c), dev), T(-j2c, dev), 1.0, 1.0, threshold=None)
+
+
+def _decomp_case(policy):
+    rng = np.random.default_rng(21)
+    nao, naux, nocc = 12, 70, 3
+    j3c = rng.standard_normal((nao, nao, naux)); j3c = 0.5 * (j3c + j3c.transpose(1, 0, 2))
+    q, _ = np.linalg.qr(rng.standard_normal((naux, naux)))
+    if policy == "eig":  # two discarded directions (one below the threshold, one negative)
+        ev = np.concatenate([[-1e-3, 1e-9], np.geomspace(1e-2, 50.0, naux - 2)])
+    else:  # not positive definite: the Cholesky factorisation fails, the eigenvalue floor applies
+        ev = np.concatenate([[-0.5, 2e-4], np.geomspace(1e-1, 20.0, naux - 2)])
+    j2c = (q * ev) @ q.T
+    j2c = 0.5 * (j2c + j2c.T)
+    C, _ = np.linalg.qr(rng.standard_normal((nao, nao)))
+    occ = np.zeros(nao); occ[:nocc] = 2.0
+    bra = rng.standard_normal((nao, nocc, 5))
+    return j3c, j2c, C, occ, bra
+
+
+@pytest.mark.parametrize("policy,thr", [("eig", 1e-6), ("cd", 1e-3)])
+def test_cderi_metric_fallback_policies(dev, policy, thr):
+    """decomp_j2c_eig and the eigenvalue floor of decomp_j2c_cd on the device (rocSOLVER syevd +
+    FP64 products) against the oracle's decompositions: the response of the resulting cderi."""
+    from hesship import RHessRIJK
+    j3c, j2c, C, occ, bra = _decomp_case(policy)
+    tp = orc.pack_triu(j3c)
+    obj = RHessRIJK.from_j3c(T(tp, dev), T(j2c, dev), 1.0, 1.0, threshold=thr, policy=policy)
+    obj.make_response_preparation(T(C, dev), occ)
+    got = obj.get_response_bra(T(bra, dev)).cpu().numpy()
+    cderi = orc.cderi_from_j3c_decomp(tp, j2c, policy, thr)
+    ref = orc.rhf_response_bra(cderi, C, occ, bra, 1.0, 1.0)
+    assert rel(got, ref) < 1e-9
+
+
+def test_cderi_eig_on_spd_metric_equals_cholesky(dev):
+    """On a positive definite metric both policies give the same response (B B^T = j3c j2c^-1 j3c^T)."""
+    from hesship import RHessRIJK, HesshipError
+    rng = np.random.default_rng(12)
+    nao, naux, nocc = 10, 90, 4
+    j3c = rng.standard_normal((nao, nao, naux)); j3c = 0.5 * (j3c + j3c.transpose(1, 0, 2))
+    a = rng.standard_normal((naux, naux)); j2c = a @ a.T + naux * np.eye(naux)
+    C, _ = np.linalg.qr(rng.standard_normal((nao, nao)))
+    occ = np.zeros(nao); occ[:nocc] = 2.0
+    bra = rng.standard_normal((nao, nocc, 3))
+    out = []
+    for policy in ("cd", "eig"):
+        obj = RHessRIJK.from_j3c(T(orc.pack_triu(j3c), dev), T(j2c, dev), 1.0, 1.0, policy=policy)
+        obj.make_response_preparation(T(C, dev), occ)
+        out.append(obj.get_response_bra(T(bra, dev)).cpu().numpy())
+    ref = orc.response_bra_naive(j3c, j2c, C, occ, bra, 1.0, 1.0)
+    assert rel(out[0], ref) < 1e-9 and rel(out[1], ref) < 1e-9
+    # no threshold: a metric that is not positive definite is an error (the reference panics)
+    with pytest.raises(HesshipError):
+        RHessRIJK.from_j3c(T(orc.pack_triu(j3c), dev), T(-j2c, dev), 1.0, 1.0, threshold=None)

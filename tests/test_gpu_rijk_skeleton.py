@@ -48,6 +48,21 @@ def test_rhf_skeleton_from_j3c(dev, sj, sk):
     assert rel(d1.cpu().numpy(), d1_o) < 1e-10
 
 
+def test_rhf_skeleton_from_eig_decomposed_handle(dev):
+    """A handle whose cderi came from the eigen decomposition (J^-1/2 kept for the metric terms)
+    gives the same skeleton as the Cholesky one on a positive definite metric."""
+    import hesship
+    model = Model(nbas=(4, 3, 5), naux_per=(6, 5, 7), seed=2)
+    I, j3c_tp, cderi, int3c = setup(dev, model)
+    C, occ = model.orbitals(4, seed=3)
+    h = hesship.RHessRIJK.from_j3c(T(j3c_tp, dev), T(I["j2c"], dev), 1.0, 0.7, threshold=1e-13, policy="eig")
+    de, d1 = h.make_skeleton(T(C, dev), occ, int3c, T(I["j2c_ip1"], dev), T(I["j2c_ipip1"], dev),
+                             T(I["j2c_ip1ip2"], dev), ranges(model.naux, [5, 7, 4]), model.aoslices, model.auxslices)
+    de_o, d1_o = osk.skeleton_rhf(I, C, occ, 1.0, 0.7, model.aoslices, model.auxslices)
+    assert rel(de.cpu().numpy(), de_o) < 1e-9
+    assert rel(d1.cpu().numpy(), d1_o) < 1e-9
+
+
 def test_rhf_skeleton_from_cderi_atom_subset(dev):
     """Handle made from cderi (the metric passed to the call); a reordered subset of atoms; one
     auxiliary range."""

@@ -134,3 +134,54 @@ def test_j_depends_only_on_symmetric_dm1():
     j1, _ = rijk.response_bra_separated(cderi, [C], [occ], [bra], True, False)
     j2, _ = rijk.response_bra_separated(cderi, [C], [occ], [bra2], True, False)
     np.testing.assert_allclose(j1, j2, rtol=1e-11, atol=1e-11)
+
+
+def metric_with_spectrum(naux, evals, seed):
+    """j2c = Q diag(evals) Q^T with a random orthogonal Q: the decompositions are pinned by the
+    known eigenpairs rather than by a second eigensolver."""
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.standard_normal((naux, naux)))
+    return (q * evals) @ q.T, q
+
+
+def test_decomp_eig_is_inverse_sqrt_on_kept_eigenvalues():
+    naux = 40
+    ev = np.concatenate([[-1e-3, 1e-9, 3e-9], np.geomspace(1e-2, 50.0, naux - 3)])
+    j2c, q = metric_with_spectrum(naux, ev, 1)
+    thr = 1e-6
+    w = rijk.decomp_j2c_eig(j2c, thr)
+    keep = ev >= thr
+    ref = (q[:, keep] * ev[keep] ** -0.5) @ q[:, keep].T
+    np.testing.assert_allclose(w, ref, atol=1e-9 * np.abs(ref).max())
+    np.testing.assert_allclose(w, w.T, atol=1e-12 * np.abs(w).max())
+    # w j2c w is the projector onto the kept eigenvectors
+    np.testing.assert_allclose(w @ j2c @ w, q[:, keep] @ q[:, keep].T, atol=1e-9)
+
+
+def test_decomp_eig_on_spd_metric_matches_cholesky_cderi():
+    j3c, j2c, cderi = synthetic_ri(9, 50, seed=3)
+    tp = rijk.pack_triu(j3c)
+    b_eig = rijk.cderi_from_j3c_decomp(tp, j2c, "eig")
+    # B B^T = j3c j2c^-1 j3c^T for either factor (pure_decompose.rs:382-407 pattern)
+    np.testing.assert_allclose(b_eig @ b_eig.T, cderi @ cderi.T, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(b_eig @ b_eig.T, tp @ np.linalg.solve(j2c, tp.T), rtol=1e-10, atol=1e-12)
+
+
+def test_decomp_cd_floor_and_no_threshold():
+    naux = 30
+    ev = np.concatenate([[-0.5, 2e-4], np.geomspace(1e-1, 20.0, naux - 2)])
+    j2c, q = metric_with_spectrum(naux, ev, 2)
+    with pytest.raises(np.linalg.LinAlgError):
+        rijk.decomp_j2c_cd(j2c, None)
+    thr = 1e-3
+    L = rijk.decomp_j2c_cd(j2c, thr)
+    np.testing.assert_allclose(np.tril(L), L)
+    np.testing.assert_allclose(L @ L.T, (q * np.maximum(ev, thr)) @ q.T, atol=1e-12 * 20)
+    # a well-conditioned metric is factorised directly (the floor changes nothing)
+    _, spd, _ = synthetic_ri(4, 25, seed=5)
+    np.testing.assert_allclose(rijk.decomp_j2c_cd(spd, 1e-14), np.linalg.cholesky(spd))
+    # a positive definite metric whose factor has a diagonal element <= threshold takes the floor
+    ev2 = np.geomspace(1e-6, 10.0, naux)
+    j2c2, q2 = metric_with_spectrum(naux, ev2, 4)
+    L2 = rijk.decomp_j2c_cd(j2c2, 1e-2)
+    np.testing.assert_allclose(L2 @ L2.T, (q2 * np.maximum(ev2, 1e-2)) @ q2.T, atol=1e-11)
