@@ -35,10 +35,26 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}  # MI355X dense ma
 # HBM bytes per launch of the hot kernel, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per
 # dispatch) and scaled by the same counters on a 1 GiB copy16_kernel copy (factors 1.99994 and 1.00000);
 # profiles/nt_bf16x3_kernel_ri_k_nao400_naux2400_nocc40_nprop90_hbm_traffic.txt.  Keyed by
-# (precision, nao, naux, nocc, nprop); other workloads report null.
-MEASURED_TRAFFIC = {("bf16x3", 400, 2400, 40, 90): 13881227.0 * 1024 * 1.99994 + 91504.8 * 1024 * 1.00000}
+# (precision, nao, naux, nocc, nprop, BF16X3 tile configuration); other workloads report null.
+MEASURED_TRAFFIC = {("bf16x3", 400, 2400, 40, 90, 0): 13881227.0 * 1024 * 1.99994 + 91504.8 * 1024 * 1.00000,
+                    ("bf16x3", 400, 2400, 40, 90, 2): 7991991.2 * 1024 * 1.99994 + 91596.1 * 1024 * 1.00000}
 HOT_KERNEL = {"fp64": "nt_f64_kernel<1,8,13,1>", "fp32": "gemm_kernel<1,float,double>",
               "bf16x3": "nt_bf16x3_kernel<1,8,13,2,true,13,2>"}
+# BF16X3 tile configurations of the NT kernel (the autotuner's candidates), by index
+BF16X3_CONFIGS = {0: "nt_bf16x3_kernel<1,8,13,2,true,13,2> (208x256 tile)",
+                  1: "nt_bf16x3_kernel<2,4,7,4,true,13,1> (208x256 tile)",
+                  2: "nt_bf16x3_kernel<2,4,13,2,true,25,2,swizzled> (400x128 tile)"}
+SPLIT_RULES = {0: "about one workgroup per CU", -1: "last wave filled"}
+
+
+def hot_choice(nao, naux, nocc, nprop):
+    """The autotuner's choice for the BF16X3 hot contraction (M nao, N nocc nprop, K kpad naux)."""
+    import ctypes as C
+    from hesship import _lib
+    cfg, ks = C.c_int32(), C.c_int32()
+    kpad = -(-nao // 32) * 32
+    _lib.check(_lib.lib().hh_util_gemm_nt_bf16x3_choice(nao, nocc * nprop, kpad * naux, C.byref(cfg), C.byref(ks)))
+    return cfg.value, ks.value
 SIZES = {
     "full": dict(nao=400, naux=2400, nocc=40, nprop=90),
     "small": dict(nao=96, naux=480, nocc=12, nprop=18),
@@ -574,6 +590,11 @@ def main():
 
     tk = tm["k_total_ms"] * 1e-3
     value = fk / tk / 1e12
+    hot_cfg, hot_name = None, HOT_KERNEL[args.precision]
+    if args.precision == "bf16x3":
+        hot_cfg, hot_ks = hot_choice(nao, naux, nocc, nprop)
+        if hot_cfg in BF16X3_CONFIGS:
+            hot_name = f"{BF16X3_CONFIGS[hot_cfg]}, split rule: {SPLIT_RULES.get(hot_ks, hot_ks)}"
     # dominant kernel: the hot K contraction GEMM; per launch it covers one aux chunk, and its
     # algorithmic work per aux index is 2 nao^2 nocc nprop.
     hot_flops_call = 2.0 * nao**2 * nocc * nprop * naux
@@ -608,8 +629,8 @@ def main():
         "rel_err_vs_fp64": diff / scale,
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4),
-                     "traffic": MEASURED_TRAFFIC.get((args.precision, nao, naux, nocc, nprop)),
-                     "kernel": HOT_KERNEL[args.precision] + " (hot K contraction, one perturbation chunk per launch)",
+                     "traffic": MEASURED_TRAFFIC.get((args.precision, nao, naux, nocc, nprop, hot_cfg)),
+                     "kernel": hot_name + " (hot K contraction, one perturbation chunk per launch)",
                      "launches_per_step": nl, "avg_launch_ms": round(t_launch * 1e3, 4)},
     }
     if not args.no_cphf:

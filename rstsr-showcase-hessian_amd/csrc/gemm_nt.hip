@@ -60,19 +60,26 @@ __device__ __forceinline__ void lds_barrier() {
 // BMT 16-row tiles per workgroup (wave wm owns tiles wm*FM .. min(BMT, (wm+1)*FM) - 1, so the
 // waves of a column may own unequal row counts); PD: K steps of global loads in flight (1 or 2).
 // 4 or 8 waves per workgroup (waves_per_eu follows: one or two waves per SIMD).
-template <int WGM, int WGN, int FM, int FN, bool TILED, int BMT = WGM * FM, int PD = 2>
+// LDS rows hold one K step (32 bf16 = four 16-byte chunks).  SWZ = false: rows padded to 80 bytes;
+// SWZ = true: unpadded 64-byte rows with chunk q of row r stored at slot q ^ ((r >> 2) & 3), so the
+// 16 rows of a fragment read (one chunk each) still fall on 16 distinct 16-byte bank groups; this
+// is what lets a 400 x 128 tile (T read once for all 400 rows of the RI-K contraction) keep two
+// LDS stages within 160 KB.
+template <int WGM, int WGN, int FM, int FN, bool TILED, int BMT = WGM * FM, int PD = 2, bool SWZ = false>
 __global__ __launch_bounds__(64 * WGM * WGN) __attribute__((amdgpu_waves_per_eu(WGM * WGN / 4)))
 void nt_bf16x3_kernel(NtParams p) {
   static_assert(BMT <= WGM * FM && BMT > (WGM - 1) * FM && (PD == 1 || PD == 2), "tile shape");
   static_assert(WGM * WGN == 4 || WGM * WGN == 8, "4 or 8 waves");
   constexpr int NTH = 64 * WGM * WGN;
   constexpr int BM = BMT * 16, BN = WGN * FN * 16, BK = 32;
-  constexpr int LD = BK + 8;                 // bf16 elements per LDS row (80 B)
+  constexpr int LD = SWZ ? BK : BK + 8;      // bf16 elements per LDS row (64 or 80 B)
   constexpr int A_ELEMS = BM * LD, B_ELEMS = BN * LD;
   constexpr int STAGE = 2 * A_ELEMS + 2 * B_ELEMS;  // hi, lo for A and B
   constexpr int A_CH = BM * (BK / 8), B_CH = BN * (BK / 8);  // 16-byte chunks per array
   constexpr int A_PT = (A_CH + NTH - 1) / NTH, B_PT = (B_CH + NTH - 1) / NTH;
   __shared__ __attribute__((aligned(16))) u16 smem[2 * STAGE];
+  // element offset of 16-byte chunk q (0 .. 3) of LDS row r
+  auto lds_off = [](int r, int q) -> int { return r * LD + ((SWZ ? (q ^ ((r >> 2) & 3)) : q) << 3); };
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WGM, wn = wave / WGM;
@@ -140,36 +147,36 @@ void nt_bf16x3_kernel(NtParams p) {
     for (int e = 0; e < A_PT; ++e) {
       const int c = tid + NTH * e;
       if (c < A_CH) {
-        const int r = c / (BK / 8), kk = (c % (BK / 8)) * 8;
-        *reinterpret_cast<uint4*>(s + r * LD + kk) = R.ah[e];
-        *reinterpret_cast<uint4*>(s + A_ELEMS + r * LD + kk) = R.al[e];
+        const int r = c / (BK / 8), o = lds_off(r, c % (BK / 8));
+        *reinterpret_cast<uint4*>(s + o) = R.ah[e];
+        *reinterpret_cast<uint4*>(s + A_ELEMS + o) = R.al[e];
       }
     }
 #pragma unroll
     for (int e = 0; e < B_PT; ++e) {
       const int c = tid + NTH * e;
       if (c < B_CH) {
-        const int r = c / (BK / 8), kk = (c % (BK / 8)) * 8;
-        *reinterpret_cast<uint4*>(s + 2 * A_ELEMS + r * LD + kk) = R.bh[e];
-        *reinterpret_cast<uint4*>(s + 2 * A_ELEMS + B_ELEMS + r * LD + kk) = R.bl[e];
+        const int r = c / (BK / 8), o = lds_off(r, c % (BK / 8));
+        *reinterpret_cast<uint4*>(s + 2 * A_ELEMS + o) = R.bh[e];
+        *reinterpret_cast<uint4*>(s + 2 * A_ELEMS + B_ELEMS + o) = R.bl[e];
       }
     }
   };
   auto compute = [&](int buf) {
     const u16* s = smem + buf * STAGE;
-    const int kc = 8 * (lane >> 4);
+    const int q = lane >> 4;  // K chunk of this lane (k = 8 q .. 8 q + 7 of the step)
     v8bf bh[FN], bl[FN];
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
       const int r = (wn * FN + j) * 16 + (lane & 15);
-      bh[j] = *reinterpret_cast<const v8bf*>(s + 2 * A_ELEMS + r * LD + kc);
-      bl[j] = *reinterpret_cast<const v8bf*>(s + 2 * A_ELEMS + B_ELEMS + r * LD + kc);
+      bh[j] = *reinterpret_cast<const v8bf*>(s + 2 * A_ELEMS + lds_off(r, q));
+      bl[j] = *reinterpret_cast<const v8bf*>(s + 2 * A_ELEMS + B_ELEMS + lds_off(r, q));
     }
     // A fragments software-pipelined: row tile i + 1 is read while tile i's MFMAs run
     auto aload = [&](int i, v8bf& h, v8bf& l) {
       const int r = (wm * FM + i) * 16 + (lane & 15);
-      h = *reinterpret_cast<const v8bf*>(s + r * LD + kc);
-      l = *reinterpret_cast<const v8bf*>(s + A_ELEMS + r * LD + kc);
+      h = *reinterpret_cast<const v8bf*>(s + lds_off(r, q));
+      l = *reinterpret_cast<const v8bf*>(s + A_ELEMS + lds_off(r, q));
     };
     v8bf ah, al, nh, nl;
     aload(0, ah, al);
@@ -449,7 +456,8 @@ int cu_count() {
   return n;
 }
 
-template <int WGM, int WGN, int FM, int FN, bool BF, bool TILED = false, int BMT = WGM * FM, int PD = 2>
+template <int WGM, int WGN, int FM, int FN, bool BF, bool TILED = false, int BMT = WGM * FM, int PD = 2,
+          bool SWZ = false>
 void launch_nt(const NtArgs& a, hipStream_t s, DevBuf& ws) {
   constexpr int BM = BMT * 16, BN = WGN * FN * 16, BK = BF ? 32 : 16;
   NtParams p{};
@@ -493,7 +501,7 @@ void launch_nt(const NtArgs& a, hipStream_t s, DevBuf& ws) {
   HH_REQUIRE(p.nwg < (1LL << 31), "gemm_nt: grid too large");
   if (a.ev_begin) HH_HIP(hipEventRecord(a.ev_begin, s));
   if constexpr (BF)
-    hipLaunchKernelGGL((nt_bf16x3_kernel<WGM, WGN, FM, FN, TILED, BMT, PD>), dim3((unsigned)p.nwg),
+    hipLaunchKernelGGL((nt_bf16x3_kernel<WGM, WGN, FM, FN, TILED, BMT, PD, SWZ>), dim3((unsigned)p.nwg),
                        dim3(64 * WGM * WGN), 0, s, p);
   else
     hipLaunchKernelGGL((nt_f64_kernel<WGM, WGN, FM, FN>), dim3((unsigned)p.nwg), dim3(512), 0, s, p);
@@ -526,13 +534,20 @@ void launch_f64_cfg(int cfg, const NtArgs& a, hipStream_t s, DevBuf& ws) {
   }
 }
 
-// BF16X3 candidates, both 208 x 256 tiles on 8 waves: 0 = 1 x 8 waves of 13 x 2 blocks (two K
-// steps of loads in flight), 1 = 2 x 4 waves of 7 (6) x 4 blocks (fewer LDS fragment reads per
-// MFMA, one K step of loads in flight to fit the registers)
-constexpr int NT_BF_NCFG = 2;
+// BF16X3 candidates on 8 waves: 0 = 208 x 256, 1 x 8 waves of 13 x 2 blocks (two K steps of
+// loads in flight); 1 = 208 x 256, 2 x 4 waves of 7 (6) x 4 blocks (fewer LDS fragment reads per
+// MFMA, one K step of loads in flight to fit the registers); 2 = 400 x 128, 2 x 4 waves of 13 (12)
+// x 2 blocks, swizzled LDS rows (an M of 400 in one tile: the B operand is read once).  Measured and
+// not kept: 400 x 128 on 2 x 2 waves of 13 (12) x 4 blocks (one wave per SIMD, accumulators in AGPRs,
+// half the LDS fragment reads per MFMA): 10.8 ms against 8.6 ms for configuration 2 on the RI-K
+// hot shape (M 400, N 3600, K 998400)
+constexpr int NT_BF_NCFG = 3;
 
 void launch_bf_cfg(int cfg, const NtArgs& a, hipStream_t s, DevBuf& ws) {
-  if (cfg == 1) {
+  if (cfg == 2) {
+    if (a.tiled) launch_nt<2, 4, 13, 2, true, true, 25, 2, true>(a, s, ws);
+    else launch_nt<2, 4, 13, 2, true, false, 25, 1, true>(a, s, ws);  // one step in flight: no spills
+  } else if (cfg == 1) {
     if (a.tiled) launch_nt<2, 4, 7, 4, true, true, 13, 1>(a, s, ws);
     else launch_nt<2, 4, 7, 4, true, false, 13, 1>(a, s, ws);
   } else {

@@ -46,7 +46,7 @@ void gemm_nt(const NtArgs& a, NtFormat fmt, hipStream_t s, DevBuf& ws);
 void gemm_nt_set_autotune(bool on);
 // the cached choice for a shape's class (false if none yet): configuration index and split rule
 bool gemm_nt_choice(int64_t M, int64_t N, int64_t K, int* cfg, int* ksplit, NtFormat fmt = NT_F64);
-// one product with configuration `cfg` (F64: 0 .. 2; BF16X3: 0 .. 1) and the split rule in a.ksplit
+// one product with configuration `cfg` (F64: 0 .. 2; BF16X3: 0 .. 2) and the split rule in a.ksplit
 void gemm_nt_f64_config(const NtArgs& a, int cfg, hipStream_t s, DevBuf& ws);
 void gemm_nt_bf16x3_config(const NtArgs& a, int cfg, hipStream_t s, DevBuf& ws);
 
