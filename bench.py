@@ -16,7 +16,8 @@ each, same fields): `numint` (RKS grid response at the (H2O)_8 B3LYP sizes on sy
 values), `dft_setup` (DFT skeleton setup passes at the same sizes, FP64), `dft_becke` (the same
 setup with the Becke grid shift on an atom-grouped grid, FP64), `skeleton` (RI-JK skeleton
 Hessian and first-derivative terms on synthetic integrals, FP64), `cphf_batch` (64 uneven
-alkane-sized RI-JK CPHF solves on one stream, eager vs HIP-graph replay of the response).
+alkane-sized RI-JK CPHF solves, half of them doublet cations (UHF), on one stream: plain FP64 vs
+mixed precision with HIP-graph replay of the response).
 """
 import argparse
 import json
@@ -377,14 +378,42 @@ def workload_dft_becke(args):
     return line
 
 
+def ucphf_case(obj, d_C, occ, nao, nprop, dev, config=None, seed=0x5EED0005):
+    """Unrestricted counterpart of cphf_case: both spins share the synthetic MO coefficients;
+    orbital energies and first-order Fock/overlap derivatives per spin from the seeded generator."""
+    import numpy as np
+    import torch
+    import hesship
+    from hesship import synth
+    noc = [int((o > 0).sum()) for o in occ]
+    z = synth.normal(seed, 0, 2 * nao + sum(2 * nao * no * nprop for no in noc))
+    en, f1, s1, p = [], [], [], 2 * nao
+    for sp, no in enumerate(noc):
+        zz = z[sp * nao:(sp + 1) * nao]
+        en.append(np.sort(np.concatenate([-1.4 + 1.1 * np.abs(np.tanh(zz[:no])), 0.2 + 2.5 * np.abs(np.tanh(zz[no:]))])))
+        f1.append(torch.from_numpy(z[p:p + nao * no * nprop].reshape(nprop, no, nao).T.copy()).to(dev))
+        p += nao * no * nprop
+        s1.append(torch.from_numpy(0.1 * z[p:p + nao * no * nprop].reshape(nprop, no, nao).T.copy()).to(dev))
+        p += nao * no * nprop
+    drv = hesship.UHessCPHF([d_C, d_C], occ, en, dict(cphf_tol=1e-8, **(config or {})))
+    drv.add_el(obj)
+    rng = np.random.default_rng(1)
+    v = [torch.from_numpy(rng.standard_normal((nao, no, 4))).to(dev) for no in noc]
+    r = drv.response_dimless_cphf(v)
+    ratio = (sum(x.norm() ** 2 for x in r) ** 0.5 / sum(x.norm() ** 2 for x in v) ** 0.5).item()
+    return drv, en, f1, s1, ratio
+
+
 def workload_cphf_batch(args):
     """Many uneven molecules: 64 seeded alkane-sized RI-JK CPHF systems (C_nH_2n+2 with
-    cc-pVDZ / -jkfit counts: nao 24n+10, naux 84n+28, nocc 4n+1, 3(3n+2) perturbations, n uniform
-    in 1..12), synthetic tensors, FP64 CPHF to 1e-8 (one subspace of up to 42 cycles) on one
-    created stream.  Each molecule is solved once untimed (buffers sized, products tuned), then
-    timed with the response operator run eagerly, with graphs from a cold cache (capture on the
-    second operator call, replays after), and once more with every graph cached.  The molecules
-    are handled one at a time (a 12-carbon system holds ~15 GB of workspace)."""
+    cc-pVDZ / -jkfit counts: nao 24n+10, naux 84n+28, 3(3n+2) perturbations, n uniform in 1..12),
+    synthetic tensors; every second molecule is a doublet cation (UHF, 4n+1 alpha and 4n beta
+    occupied orbitals), the others closed shells (RHF, 4n+1 doubly occupied).  Each molecule is
+    solved to 1e-8 (one subspace of up to 42 cycles) on one created stream, plain FP64 with the
+    response launched eagerly and mixed precision (BF16x3 operator, FP64 refinement) with the
+    response replayed from HIP graphs; each leg runs once untimed first (buffers sized, products
+    tuned, graphs captured).  The molecules are handled one at a time (a 12-carbon system holds
+    ~15 GB of workspace)."""
     import numpy as np
     import torch
     import hesship
@@ -393,56 +422,82 @@ def workload_cphf_batch(args):
     ns = np.random.default_rng(64).integers(1, 13, 64)
     stream = torch.cuda.Stream(device=dev)
     cfg = dict(cphf_max_space=42)
-    tot = dict(eager=0.0, graphs=0.0, cached=0.0)
-    cycles = caps = launches = 0
+    tot = dict(fp64=0.0, mixed=0.0)
+    cyc = dict(fp64=0, mixed=0)
+    caps = launches = 0
     err = 0.0
+    nuhf = fallbacks = 0
+
+    def diff(a, b):
+        if isinstance(a, (list, tuple)):
+            return max((x - y).abs().max().item() for x, y in zip(a, b))
+        return (a - b).abs().max().item()
+
     with torch.cuda.stream(stream):
         for i, n in enumerate(ns):
-            nao, naux, nocc = 24 * int(n) + 10, 84 * int(n) + 28, 4 * int(n) + 1
-            nprop = 3 * (3 * int(n) + 2)
+            n = int(n)
+            nao, naux = 24 * n + 10, 84 * n + 28
+            nprop = 3 * (3 * n + 2)
+            uhf = i % 2 == 1
+            nuhf += uhf
             cd = torch.from_numpy(synth.cderi(nao, naux, seed=synth.SEED_B + 16 * i).T.copy()).to(dev).T
             C = torch.from_numpy(synth.mo_coeff(nao, seed=synth.SEED_C + 16 * i)).to(dev)
-            occ = np.where(np.arange(nao) < nocc, 2.0, 0.0)
-            probe = hesship.RHessRIJK(cd, 1.0, 1.0)
-            probe.make_response_preparation(C, occ)
-            pdrv, _, _, _, ratio = cphf_case(probe, C, occ, nao, nocc, nprop, dev, cfg)
-            del pdrv, probe
-            obj = hesship.RHessRIJK(cd * (0.3 / ratio) ** 0.5, 1.0, 1.0)
-            obj.make_response_preparation(C, occ)
-            drv, _, f1, s1, _ = cphf_case(obj, C, occ, nao, nocc, nprop, dev, cfg)
+            if uhf:
+                occ = [np.where(np.arange(nao) < 4 * n + 1, 1.0, 0.0), np.where(np.arange(nao) < 4 * n, 1.0, 0.0)]
+                probe = hesship.UHessRIJK(cd, 1.0, 1.0)
+                probe.make_response_preparation([C, C], occ)
+                pdrv, _, _, _, ratio = ucphf_case(probe, C, occ, nao, nprop, dev, cfg)
+                del pdrv, probe
+                obj = hesship.UHessRIJK(cd * (0.3 / ratio) ** 0.5, 1.0, 1.0)
+                obj.make_response_preparation([C, C], occ)
+                drv, _, f1, s1, _ = ucphf_case(obj, C, occ, nao, nprop, dev, cfg)
+            else:
+                nocc = 4 * n + 1
+                occ = np.where(np.arange(nao) < nocc, 2.0, 0.0)
+                probe = hesship.RHessRIJK(cd, 1.0, 1.0)
+                probe.make_response_preparation(C, occ)
+                pdrv, _, _, _, ratio = cphf_case(probe, C, occ, nao, nocc, nprop, dev, cfg)
+                del pdrv, probe
+                obj = hesship.RHessRIJK(cd * (0.3 / ratio) ** 0.5, 1.0, 1.0)
+                obj.make_response_preparation(C, occ)
+                drv, _, f1, s1, _ = cphf_case(obj, C, occ, nao, nocc, nprop, dev, cfg)
             drv.set_graphs(False)
             run_cphf(drv, f1, s1, "fp64")
-            ref, _, t_e, st = run_cphf(drv, f1, s1, "fp64")
+            ref, _, t64, st64 = run_cphf(drv, f1, s1, "fp64")
             drv.set_graphs(True)
-            got, _, t_g, _ = run_cphf(drv, f1, s1, "fp64")
-            _, _, t_c, _ = run_cphf(drv, f1, s1, "fp64")
+            run_cphf(drv, f1, s1, "bf16x3")
+            got, _, tmx, stmx = run_cphf(drv, f1, s1, "bf16x3")
             c, l = drv.graph_stats()
-            err = max(err, (got - ref).abs().max().item())
-            tot["eager"] += t_e
-            tot["graphs"] += t_g
-            tot["cached"] += t_c
-            cycles += st["cycles"]
+            err = max(err, diff(got, ref))
+            tot["fp64"] += t64
+            tot["mixed"] += tmx
+            cyc["fp64"] += st64["cycles"]
+            cyc["mixed"] += stmx["cycles"]
+            fallbacks += stmx.get("fp64_fallback", 0)
             caps += c
             launches += l
-            log(f"[bench] molecule {i}: C{n} nao {nao} nprop {nprop}: {st['cycles']} cycles, eager {t_e * 1e3:.1f} ms, "
-                f"graphs {t_g * 1e3:.1f} ms, cached {t_c * 1e3:.1f} ms ({c} captures, {l} launches)")
+            log(f"[bench] molecule {i}: C{n} {'UHF' if uhf else 'RHF'} nao {nao} nprop {nprop}: FP64 {st64['cycles']} "
+                f"cycles {t64 * 1e3:.1f} ms, mixed {stmx['cycles']} cycles / {stmx['refinements']} sweeps "
+                f"{' (FP64 finish)' if stmx.get('fp64_fallback') else ''} {tmx * 1e3:.1f} ms")
             del drv, obj, cd, C, f1, s1, ref, got
             torch.cuda.synchronize()
-    wall = tot["graphs"]
-    log(f"[bench] cphf batch: eager {tot['eager']:.3f} s, graphs {wall:.3f} s, cached {tot['cached']:.3f} s")
-    line = _line("CPHF batch wall s (64 uneven RI-JK molecules, FP64, graph replay)", round(wall, 4), "s",
-                 wall * 1e3, 1, 1, "f64",
-                 {"workload": "64 alkane-sized synthetic RI-JK CPHF systems solved one after the other on one stream",
+    wall = tot["mixed"]
+    log(f"[bench] cphf batch: FP64 {tot['fp64']:.3f} s, mixed {wall:.3f} s")
+    line = _line("CPHF batch wall s (64 uneven RI-JK molecules, RHF + UHF, mixed precision, graph replay)",
+                 round(wall, 4), "s", wall * 1e3, 1, 1, "bf16x3 operator, f64 refinement",
+                 {"workload": "64 alkane-sized synthetic RI-JK CPHF systems (32 closed shells, 32 doublet cations) "
+                              "solved one after the other on one stream",
                   "n_carbon": [int(n) for n in ns], "cphf_tol": 1e-8},
                  {"bound": "launch", "achieved": None, "peak": None, "unit": None, "frac": None, "traffic": None,
                   "kernel": "whole solves"}, higher=False)
-    line["eager_wall_s"] = round(tot["eager"], 4)
-    line["cached_graphs_wall_s"] = round(tot["cached"], 4)
-    line["speedup_graphs_vs_eager"] = round(tot["eager"] / wall, 3)
-    line["krylov_cycles"] = cycles
+    line["uhf_molecules"] = nuhf
+    line["fp64_eager_wall_s"] = round(tot["fp64"], 4)
+    line["speedup_mixed_vs_fp64"] = round(tot["fp64"] / wall, 3)
+    line["krylov_cycles"] = cyc
     line["graphs_captured"] = caps
     line["graph_launches"] = launches
-    line["max_abs_mo1_diff_graphs_vs_eager"] = err
+    line["max_abs_mo1_diff_mixed_vs_fp64"] = err
+    line["mixed_solves_finished_in_fp64"] = fallbacks
     return line
 
 

@@ -295,6 +295,8 @@ typedef struct {
   int32_t refinements;  /* FP64 refinement sweeps (CPHF mixed precision; 0 otherwise) */
   double max_innerprod; /* last max ||v||^2 of the new trial block */
   double residual;      /* final max column 2-norm of the FP64 residual (CPHF mixed precision) */
+  int32_t fp64_fallback; /* 1 if a mixed-precision CPHF solve stagnated (a sweep gained less than
+                            10x) and was replaced by the plain FP64 solve from scratch */
 } hh_krylov_stats;
 
 /* ax[n, nblock] = A x[n, nblock]; device pointers, column-major; must enqueue its work on the

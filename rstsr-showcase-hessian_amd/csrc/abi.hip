@@ -85,8 +85,10 @@ void check_prec(hh_precision precision) {
   HH_REQUIRE(precision >= HH_PREC_FP64 && precision <= HH_PREC_BF16X3, "unknown precision");
 }
 
-void fill_stats(hh_krylov_stats* out, const hh::KrylovStats& k, int refinements, double residual) {
+void fill_stats(hh_krylov_stats* out, const hh::KrylovStats& k, int refinements, double residual,
+                int fp64_fallback = 0) {
   if (!out) return;
+  out->fp64_fallback = fp64_fallback;
   out->cycles = k.cycles;
   out->restarts = k.restarts;
   out->aop_calls = k.aop_calls;
@@ -650,10 +652,10 @@ hh_status solve_common(hh_cphf* h, int nspin, const double* const* rhs, int64_t 
     HH_REQUIRE(nprop > 0, "nprop <= 0");
     check_prec(precision);
     h->impl->solve_dimless(rhs, nprop, mo1, (Prec)precision, &st, diag);
-    fill_stats(stats, st.krylov, st.refinements, st.residual);
+    fill_stats(stats, st.krylov, st.refinements, st.residual, st.fallbacks);
     return HH_OK;
   } catch (const hh::Error& e) {
-    fill_stats(stats, st.krylov, st.refinements, st.residual);
+    fill_stats(stats, st.krylov, st.refinements, st.residual, st.fallbacks);
     hh::set_last_error(e.what());
     return e.code;
   } catch (const std::exception& e) {

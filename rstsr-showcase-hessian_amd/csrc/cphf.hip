@@ -581,8 +581,31 @@ void Cphf::solve_stacked(const double* rhs, int64_t nprop, double* mo1, Prec pre
   double bound = 0.0;  // bound on the error of the residual's 2-norm (any column)
   double* fin_r = static_cast<double*>(fin_r_.reserve(sizeof(double) * blk));
   const double op_tol = (prec == PREC_BF16X3) ? 1e-5 : 1e-7;  // inner target, relative to the rescaled rhs
+  // Stagnation fallback: a sweep that gains less than 10x (restarted inner solves that stop
+  // resolving the hard part of the spectrum; the residual rebuilt in FP64 agrees with the
+  // incremental one there, so it is not the reduced precision) ends the mixed solve, and the
+  // reference's plain FP64 block Krylov solves the system from scratch.  A warm start from mo1
+  // is no option: krylov_block restarts from the residual, whose trial vectors then sit at the
+  // absolute lindep floor at once.
   const int max_sweeps = 10;
+  double prev_res = -1.0;
   for (int sweep = 0; sweep < max_sweeps && std::sqrt(res) + bound >= cfg_.tol && res > 0.0; ++sweep) {
+    if (prev_res > 0.0 && res > 0.01 * prev_res) {
+      if (verbose) {
+        fprintf(stderr, "cphf refinement stagnated at %.3e: plain FP64 solve\n", std::sqrt(res));
+        fflush(stderr);
+      }
+      KrylovStats ks;
+      krylov_block(make_aop(PREC_F64), rhs, nullptr, n, nprop, cfg_.tol, cfg_.max_cycle, cfg_.max_space,
+                   cfg_.lindep, y, &ks, stream_);
+      accumulate(ks);
+      unprecond(y, mo1, nprop);
+      stats.fallbacks = 1;
+      stats.residual = 0.0;
+      if (st) *st = stats;
+      return;
+    }
+    prev_res = res;
     const double sc = 1.0 / std::sqrt(res);
     hipLaunchKernelGGL(scale_columns_kernel, dim3(nblk(blk)), dim3(256), 0, stream_, r, n, nprop, coln,
                        0.01 * cfg_.tol, csc);

@@ -31,6 +31,7 @@ struct CphfStats {
   KrylovStats krylov;       // summed over refinement sweeps
   int refinements = 0;      // FP64 residual evaluations after the first mixed solve
   double residual = 0.0;    // final max column 2-norm of rhs - (x + A x), FP64
+  int fallbacks = 0;        // 1 if a stagnating mixed solve was replaced by the plain FP64 solve
 };
 
 class Cphf {

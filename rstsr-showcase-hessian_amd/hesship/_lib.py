@@ -52,7 +52,8 @@ SIGNATURES = {
 
 class KrylovStats(C.Structure):
     _fields_ = [("cycles", C.c_int32), ("restarts", C.c_int32), ("aop_calls", C.c_int32),
-                ("refinements", C.c_int32), ("max_innerprod", C.c_double), ("residual", C.c_double)]
+                ("refinements", C.c_int32), ("max_innerprod", C.c_double), ("residual", C.c_double),
+                ("fp64_fallback", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
