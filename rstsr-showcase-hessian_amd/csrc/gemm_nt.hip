@@ -445,6 +445,33 @@ __global__ void split_bf16_tiled_kernel(const T* __restrict__ src, int64_t ld, i
   }
 }
 
+// The transposed split: K runs along the columns of src.  A 32-column x 64-row block goes through
+// LDS so that both the column-major reads (along rows) and the tiled writes (32 K values of a row
+// contiguous) are coalesced.
+template <typename T>
+__global__ __launch_bounds__(256) void split_bf16_tiled_t_kernel(const T* __restrict__ src, int64_t ld, int64_t nrow,
+                                                                 int64_t ncol, u16* __restrict__ hi,
+                                                                 u16* __restrict__ lo) {
+  __shared__ float t[32][65];
+  const int64_t r0 = (int64_t)blockIdx.x * 64, kb = blockIdx.y, c0 = kb * 32;
+  for (int e = threadIdx.x; e < 32 * 64; e += 256) {
+    const int c = e / 64, r = e % 64;
+    const int64_t gr = r0 + r, gc = c0 + c;
+    t[c][r] = (gr < nrow && gc < ncol) ? (float)src[gr + ld * gc] : 0.f;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 32 * 64; e += 256) {
+    const int r = e / 32, c = e % 32;
+    const int64_t gr = r0 + r;
+    if (gr >= nrow) continue;
+    const float x = t[c][r];
+    const __bf16 h = (__bf16)x;
+    const int64_t o = (kb * nrow + gr) * 32 + c;
+    hi[o] = __builtin_bit_cast(u16, h);
+    lo[o] = __builtin_bit_cast(u16, (__bf16)(x - (float)h));
+  }
+}
+
 int cu_count() {
   static int n = 0;
   if (n == 0) {
@@ -712,6 +739,19 @@ void split_bf16_tiled(const T* src, int64_t ld, int64_t nrow, int64_t ncol, int6
   HH_HIP(hipGetLastError());
 }
 
+template <typename T>
+void split_bf16_tiled_t(const T* src, int64_t ld, int64_t nrow, int64_t ncol, int64_t kpad, unsigned short* hi,
+                        unsigned short* lo, hipStream_t s) {
+  HH_REQUIRE(kpad % 32 == 0 && kpad >= ncol && ld >= nrow, "split_bf16_tiled_t: bad sizes");
+  if (nrow == 0 || kpad == 0) return;
+  HH_REQUIRE(kpad / 32 < 65536, "split_bf16_tiled_t: too many K blocks");
+  hipLaunchKernelGGL((split_bf16_tiled_t_kernel<T>), dim3((unsigned)ceil_div(nrow, 64), (unsigned)(kpad / 32)),
+                     dim3(256), 0, s, src, ld, nrow, ncol, hi, lo);
+  HH_HIP(hipGetLastError());
+}
+
+template void split_bf16_tiled_t<double>(const double*, int64_t, int64_t, int64_t, int64_t, unsigned short*,
+                                         unsigned short*, hipStream_t);
 template void split_bf16_tiled<double>(const double*, int64_t, int64_t, int64_t, int64_t, unsigned short*,
                                        unsigned short*, hipStream_t);
 template void split_bf16_tiled<float>(const float*, int64_t, int64_t, int64_t, int64_t, unsigned short*,

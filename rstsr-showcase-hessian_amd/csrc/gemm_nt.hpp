@@ -57,4 +57,10 @@ template <typename T>
 void split_bf16_tiled(const T* src, int64_t ld, int64_t nrow, int64_t ncol, int64_t kpad, unsigned short* hi,
                       unsigned short* lo, hipStream_t s);
 
+// The same split with K along the columns: src column-major [nrow, ncol] (leading dimension ld),
+// K = columns padded to kpad: hi/lo[((c / 32) * nrow + r) * 32 + c % 32], zero for c >= ncol.
+template <typename T>
+void split_bf16_tiled_t(const T* src, int64_t ld, int64_t nrow, int64_t ncol, int64_t kpad, unsigned short* hi,
+                        unsigned short* lo, hipStream_t s);
+
 }  // namespace hh

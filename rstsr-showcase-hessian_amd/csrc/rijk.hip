@@ -312,6 +312,39 @@ void Rijk::build_j_ao(const double* const* bra, int64_t nprop, Prec prec, double
     h.B = itm; h.ldb = naux_; h.b_kmajor = true;
     h.C = jtp; h.ldc = nao_tp_; h.alpha = 2.0;
     gemm<double, double>(h, PREC_F64, stream_, ws_);
+  } else if (prec == PREC_BF16X3) {
+    // both products on the NT kernel: itm = cderi^T dm_tp (K = AO pairs), then
+    // resp_tp = 2 cderi itm (K = P), split-bf16 operands, FP32 accumulation, FP64 outputs
+    const int64_t kp1 = (nao_tp_ + 31) / 32 * 32, kp2 = (naux_ + 31) / 32 * 32;
+    if (!j_bf16_ready_) {
+      const size_t e1 = sizeof(unsigned short) * (size_t)kp1 * naux_, e2 = sizeof(unsigned short) * (size_t)kp2 * nao_tp_;
+      auto* a = static_cast<unsigned short*>(cj_hi_.reserve(e1));
+      auto* b = static_cast<unsigned short*>(cj_lo_.reserve(e1));
+      split_bf16_tiled<double>(cderi_.as<double>(), nao_tp_, nao_tp_, naux_, kp1, a, b, stream_);
+      auto* c = static_cast<unsigned short*>(cjt_hi_.reserve(e2));
+      auto* d = static_cast<unsigned short*>(cjt_lo_.reserve(e2));
+      split_bf16_tiled_t<double>(cderi_.as<double>(), nao_tp_, nao_tp_, naux_, kp2, c, d, stream_);
+      j_bf16_ready_ = true;
+    }
+    const int64_t e1 = kp1 * nprop, e2 = kp2 * nprop;
+    auto* sp = static_cast<unsigned short*>(jsp_.reserve(sizeof(unsigned short) * 2 * (size_t)(e1 + e2)));
+    unsigned short *dh = sp, *dl = sp + e1, *ih = sp + 2 * e1, *il = ih + e2;
+    split_bf16_tiled<double>(dmtp, nao_tp_, nao_tp_, nprop, kp1, dh, dl, stream_);
+    NtArgs g;  // itm[P, a] = sum_tp cderi[tp, P] dm_tp[tp, a]
+    g.M = naux_; g.N = nprop; g.K = kp1;
+    g.A = cj_hi_.ptr; g.A_lo = cj_lo_.ptr; g.lda = kp1;
+    g.B = dh; g.B_lo = dl; g.ldb = kp1;
+    g.tiled = true;
+    g.C = itm; g.ldc = naux_;
+    gemm_nt(g, NT_BF16X3, stream_, ws_);
+    split_bf16_tiled<double>(itm, naux_, naux_, nprop, kp2, ih, il, stream_);
+    NtArgs h;  // resp_tp[tp, a] = 2 sum_P cderi[tp, P] itm[P, a]
+    h.M = nao_tp_; h.N = nprop; h.K = kp2;
+    h.A = cjt_hi_.ptr; h.A_lo = cjt_lo_.ptr; h.lda = kp2;
+    h.B = ih; h.B_lo = il; h.ldb = kp2;
+    h.tiled = true;
+    h.C = jtp; h.ldc = nao_tp_; h.alpha = 2.0;
+    gemm_nt(h, NT_BF16X3, stream_, ws_);
   } else {
     ensure_mixed();
     float* dmtp32 = static_cast<float*>(bra32_.reserve(sizeof(float) * std::max(nao_tp_, naux_) * nprop));

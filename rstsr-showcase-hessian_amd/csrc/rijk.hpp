@@ -113,6 +113,11 @@ class Rijk {
   // per-call scratch
   DevBuf brat_;                 // bra as BF16x3 NT operand (hi then lo), K-block tiled
   DevBuf w_, t_, bra32_, dm_, dmtp_, itm_, jtp_, jao_, ws_;
+  // BF16x3 RI-J operands (built on the first BF16x3 J): cderi split into bf16 hi / lo in the
+  // K-block tiled NT layout twice, K = the AO pair (cj, rows P) and K = P (cjt, rows the AO pair);
+  // jsp_: the per-call split dm_tp and itm
+  DevBuf cj_hi_, cj_lo_, cjt_hi_, cjt_lo_, jsp_;
+  bool j_bf16_ready_ = false;
 };
 
 }  // namespace hh
