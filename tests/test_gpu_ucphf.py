@@ -123,3 +123,50 @@ def test_spin_mismatch_is_an_error(dev):
     ud = hesship.UHessCPHF([T(c, dev) for c in C], occ, e)
     with pytest.raises(HesshipError):
         ud.add_el(r_rijk)  # an RHF component on the UHF driver
+
+
+def test_mixed_refinement_stagnation_falls_back_to_fp64(dev):
+    """A doublet-cation system (alkane C6 sizes: nao 154, naux 532, 25 / 24 occupied, 60
+    perturbations; seeded synthetic RI tensors) on which the refinement sweeps of the mixed solve
+    stall near 1e-7: the solve must still reach the FP64 residual tolerance, through the plain FP64
+    solve when a sweep gains less than 10x, and agree with the plain FP64 solution."""
+    import torch
+    import hesship
+    from hesship import synth
+    n, i = 6, 21
+    nao, naux, nprop = 24 * n + 10, 84 * n + 28, 3 * (3 * n + 2)
+    cd = torch.from_numpy(synth.cderi(nao, naux, seed=synth.SEED_B + 16 * i).T.copy()).to(dev).T
+    C = torch.from_numpy(synth.mo_coeff(nao, seed=synth.SEED_C + 16 * i)).to(dev)
+    occ = [np.where(np.arange(nao) < 4 * n + 1, 1.0, 0.0), np.where(np.arange(nao) < 4 * n, 1.0, 0.0)]
+    noc = [int(o.sum()) for o in occ]
+    z = synth.normal(0x5EED0005, 0, 2 * nao + sum(2 * nao * no * nprop for no in noc))
+    en, f1, s1, p = [], [], [], 2 * nao
+    for sp, no in enumerate(noc):
+        zz = z[sp * nao:(sp + 1) * nao]
+        en.append(np.sort(np.concatenate([-1.4 + 1.1 * np.abs(np.tanh(zz[:no])), 0.2 + 2.5 * np.abs(np.tanh(zz[no:]))])))
+        f1.append(torch.from_numpy(z[p:p + nao * no * nprop].reshape(nprop, no, nao).T.copy()).to(dev))
+        p += nao * no * nprop
+        s1.append(torch.from_numpy(0.1 * z[p:p + nao * no * nprop].reshape(nprop, no, nao).T.copy()).to(dev))
+        p += nao * no * nprop
+
+    def driver(scale):
+        rijk = hesship.UHessRIJK(cd * scale, 1.0, 1.0)
+        rijk.make_response_preparation([C, C], occ)
+        drv = hesship.UHessCPHF([C, C], occ, en, dict(cphf_tol=1e-8, cphf_max_space=42))
+        drv.add_el(rijk)
+        return drv, rijk
+
+    # RI scale: the dimensionless response of a random vector has 0.3 of its norm
+    drv, rijk = driver(1.0)
+    rng = np.random.default_rng(1)
+    v = [torch.from_numpy(rng.standard_normal((nao, no, 4))).to(dev) for no in noc]
+    r = drv.response_dimless_cphf(v)
+    ratio = (sum(x.norm() ** 2 for x in r) ** 0.5 / sum(x.norm() ** 2 for x in v) ** 0.5).item()
+    drv, rijk = driver((0.3 / ratio) ** 0.5)
+    rhs = drv.compute_dimless_rhs(f1, s1)
+    ref = drv.solve_dimless_cphf(rhs, precision="fp64")
+    st = {}
+    got = drv.solve_dimless_cphf(rhs, precision="bf16x3", stats=st)
+    print(f"mixed solve: {st['refinements']} sweeps, fp64_fallback {st['fp64_fallback']}, residual {st['residual']:.2e}")
+    assert st["fp64_fallback"] == 1 or st["residual"] < 1e-8
+    assert max((a - b).abs().max().item() for a, b in zip(got, ref)) < 1e-6
