@@ -48,6 +48,18 @@ BF16X3_CONFIGS = {0: "nt_bf16x3_kernel<1,8,13,2,true,13,2> (208x256 tile)",
 SPLIT_RULES = {0: "about one workgroup per CU", -1: "last wave filled"}
 
 
+def measured_mfma_peak(precision):
+    """Issue-rate ceiling of the MFMA instruction the hot kernel uses, measured on this device
+    (hh_util_mfma_peak: independent accumulator chains, no memory traffic), TFLOP/s."""
+    import ctypes as C
+    import torch
+    from hesship import _lib
+    kind, iters = (1, 20000) if precision == "fp64" else (0, 500)
+    out = C.c_double()
+    _lib.check(_lib.lib().hh_util_mfma_peak(kind, iters, C.byref(out), torch.cuda.current_stream().cuda_stream))
+    return out.value
+
+
 def hot_choice(nao, naux, nocc, nprop):
     """The autotuner's choice for the BF16X3 hot contraction (M nao, N nocc nprop, K kpad naux)."""
     import ctypes as C
@@ -725,6 +737,13 @@ def main():
                      "kernel": hot_name + " (hot K contraction, one perturbation chunk per launch)",
                      "launches_per_step": nl, "avg_launch_ms": round(t_launch * 1e3, 4)},
     }
+    if args.precision in ("bf16x3", "fp64"):
+        mp = measured_mfma_peak(args.precision)
+        scale = 3.0 if args.precision == "bf16x3" else 1.0  # MFMA flops per algorithmic flop
+        line["roofline"]["measured_mfma_peak"] = {
+            "instruction": "v_mfma_f32_16x16x32_bf16" if args.precision == "bf16x3" else "v_mfma_f64_16x16x4f64",
+            "tflops": round(mp, 1), "mfma_flops_per_algorithmic_flop": scale,
+            "frac": round(achieved * scale / mp, 4)}
     if not args.no_cphf:
         # CPHF wall time: same tensors, RI-JK scaled so the dimensionless response is ~0.3 of its
         # input (the benchmark's raw N(0,1) cderi would make I + A far from a physical CPHF)

@@ -527,6 +527,11 @@ hh_status hh_util_gemm_nt(int64_t M, int64_t N, int64_t K, double alpha, const d
                           const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int32_t config,
                           int32_t ksplit, void* stream);
 hh_status hh_util_set_autotune(int32_t enable);
+/* Measured MFMA issue-rate ceiling in TFLOP/s (no reference counterpart; the denominator beside
+ * the published peak): kind 0 = v_mfma_f32_16x16x32_bf16, 1 = v_mfma_f64_16x16x4f64,
+ * 2 = v_mfma_f32_32x32x16_bf16; every wave runs `iters` trips of 8 (kind 2: 4) independent
+ * accumulator chains, 8 waves per SIMD on every CU. */
+hh_status hh_util_mfma_peak(int32_t kind, int32_t iters, double* tflops, void* stream);
 hh_status hh_util_gemm_nt_choice(int64_t M, int64_t N, int64_t K, int32_t* config, int32_t* ksplit);
 /* The same product from FP64 operands in split-BF16 x3 (the RI-K hot product's arithmetic: A and
  * B split into bf16 hi / lo, three MFMA products, FP32 accumulation, split-K partials summed in

@@ -941,6 +941,13 @@ hh_status hh_util_gemm_nt_bf16x3_choice(int64_t M, int64_t N, int64_t K, int32_t
   HH_ABI_END
 }
 
+hh_status hh_util_mfma_peak(int32_t kind, int32_t iters, double* tflops, void* stream) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(tflops != nullptr, "null output");
+  *tflops = mfma_peak_tflops(kind, iters, (hipStream_t)stream);
+  HH_ABI_END
+}
+
 hh_status hh_util_set_autotune(int32_t enable) {
   HH_ABI_BEGIN
   gemm_nt_set_autotune(enable != 0);

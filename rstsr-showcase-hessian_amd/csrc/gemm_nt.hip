@@ -21,6 +21,7 @@ namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef float v16f __attribute__((ext_vector_type(16)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef unsigned short u16;
 
@@ -472,6 +473,58 @@ __global__ __launch_bounds__(256) void split_bf16_tiled_t_kernel(const T* __rest
   }
 }
 
+// MFMA issue-rate probe: every wave runs independent accumulator chains of one MFMA instruction
+// (no memory traffic in the loop); out[] receives one sum per thread so nothing is optimised
+// away.  KIND 0: v_mfma_f32_16x16x32_bf16 (16384 flop, 8 chains), 1: v_mfma_f64_16x16x4f64
+// (2048 flop, 8 chains), 2: v_mfma_f32_32x32x16_bf16 (32768 flop, 4 chains).
+template <int KIND>
+__global__ __launch_bounds__(256) void mfma_peak_kernel(int iters, float* __restrict__ out) {
+  const int t = threadIdx.x;
+  float s = 0.f;
+  if constexpr (KIND == 0 || KIND == 2) {
+    v8bf a, b;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      a[k] = (__bf16)(1e-3f * (float)((t + k) % 7));
+      b[k] = (__bf16)(1e-3f * (float)((t + 3 * k) % 5));
+    }
+    if constexpr (KIND == 0) {
+      v4f acc[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[j] = v4f{0, 0, 0, 0};
+      for (int it = 0; it < iters; ++it)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) s += acc[j][0] + acc[j][1] + acc[j][2] + acc[j][3];
+    } else {
+      v16f acc[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[j][r] = 0.f;
+      for (int it = 0; it < iters; ++it)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) s += acc[j][r];
+    }
+  } else {
+    const double a = 1e-3 * (double)(t % 7), b = 1e-3 * (double)(t % 5);
+    v4d acc[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[j] = v4d{0, 0, 0, 0};
+    for (int it = 0; it < iters; ++it)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[j], 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s += (float)(acc[j][0] + acc[j][1] + acc[j][2] + acc[j][3]);
+  }
+  out[(int64_t)blockIdx.x * blockDim.x + t] = s;
+}
+
 int cu_count() {
   static int n = 0;
   if (n == 0) {
@@ -567,7 +620,8 @@ void launch_f64_cfg(int cfg, const NtArgs& a, hipStream_t s, DevBuf& ws) {
 // x 2 blocks, swizzled LDS rows (an M of 400 in one tile: the B operand is read once).  Measured and
 // not kept: 400 x 128 on 2 x 2 waves of 13 (12) x 4 blocks (one wave per SIMD, accumulators in AGPRs,
 // half the LDS fragment reads per MFMA): 10.8 ms against 8.6 ms for configuration 2 on the RI-K
-// hot shape (M 400, N 3600, K 998400)
+// hot shape (M 400, N 3600, K 998400); 416 x 128 on 32 x 32 x 16 MFMA blocks (the instruction's
+// measured ceiling is 1.55x that of 16 x 16 x 32): 9.5 ms against 8.55 ms
 constexpr int NT_BF_NCFG = 3;
 
 void launch_bf_cfg(int cfg, const NtArgs& a, hipStream_t s, DevBuf& ws) {
@@ -748,6 +802,35 @@ void split_bf16_tiled_t(const T* src, int64_t ld, int64_t nrow, int64_t ncol, in
   hipLaunchKernelGGL((split_bf16_tiled_t_kernel<T>), dim3((unsigned)ceil_div(nrow, 64), (unsigned)(kpad / 32)),
                      dim3(256), 0, s, src, ld, nrow, ncol, hi, lo);
   HH_HIP(hipGetLastError());
+}
+
+double mfma_peak_tflops(int kind, int iters, hipStream_t s) {
+  HH_REQUIRE(kind >= 0 && kind <= 2, "mfma_peak: kind must be 0 (bf16 16x16x32), 1 (f64 16x16x4) or 2 (bf16 32x32x16)");
+  HH_REQUIRE(iters > 0 && iters <= (1 << 20), "mfma_peak: iters out of range");
+  const int64_t nwg = 8 * (int64_t)cu_count();  // 8 workgroups of 4 waves per CU: 8 waves per SIMD
+  static DevBuf* buf = new DevBuf;
+  float* out = static_cast<float*>(buf->reserve(sizeof(float) * nwg * 256));
+  hipEvent_t e0, e1;
+  HH_HIP(hipEventCreate(&e0));
+  HH_HIP(hipEventCreate(&e1));
+  auto launch = [&]() {
+    if (kind == 0) hipLaunchKernelGGL(mfma_peak_kernel<0>, dim3((unsigned)nwg), dim3(256), 0, s, iters, out);
+    else if (kind == 2) hipLaunchKernelGGL(mfma_peak_kernel<2>, dim3((unsigned)nwg), dim3(256), 0, s, iters, out);
+    else hipLaunchKernelGGL(mfma_peak_kernel<1>, dim3((unsigned)nwg), dim3(256), 0, s, iters, out);
+    HH_HIP(hipGetLastError());
+  };
+  launch();  // warm-up (clocks, code object)
+  HH_HIP(hipEventRecord(e0, s));
+  launch();
+  HH_HIP(hipEventRecord(e1, s));
+  HH_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HH_HIP(hipEventElapsedTime(&ms, e0, e1));
+  HH_HIP(hipEventDestroy(e0));
+  HH_HIP(hipEventDestroy(e1));
+  const double flop_per_instr = kind == 0 ? 16384.0 : kind == 2 ? 32768.0 : 2048.0;
+  const double chains = kind == 2 ? 4.0 : 8.0;
+  return flop_per_instr * chains * (double)iters * (double)(nwg * 4) / (ms * 1e-3) / 1e12;
 }
 
 template void split_bf16_tiled_t<double>(const double*, int64_t, int64_t, int64_t, int64_t, unsigned short*,

@@ -63,4 +63,10 @@ template <typename T>
 void split_bf16_tiled_t(const T* src, int64_t ld, int64_t nrow, int64_t ncol, int64_t kpad, unsigned short* hi,
                         unsigned short* lo, hipStream_t s);
 
+// Measured issue-rate ceiling of one MFMA instruction on this device (TFLOP/s): kind 0 =
+// v_mfma_f32_16x16x32_bf16 (the BF16x3 products), 1 = v_mfma_f64_16x16x4f64 (the FP64 products),
+// 2 = v_mfma_f32_32x32x16_bf16; `iters` loop trips of 8 (kind 2: 4) independent chains per wave,
+// 8 waves per SIMD on every CU.
+double mfma_peak_tflops(int kind, int iters, hipStream_t s);
+
 }  // namespace hh

@@ -119,6 +119,7 @@ SIGNATURES.update({
     "hh_numint_uks_response_bra": (_i32, [_p, _i32, _p, _pp, _i64, _pp, _pi64, _pp, _i32]),
     "hh_util_gemm_nt": (_i32, [_i64, _i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64, _i32, _i32, _p]),
     "hh_util_set_autotune": (_i32, [_i32]),
+    "hh_util_mfma_peak": (_i32, [_i32, _i32, C.POINTER(C.c_double), _p]),
     "hh_util_gemm_nt_choice": (_i32, [_i64, _i64, _i64, C.POINTER(_i32), C.POINTER(_i32)]),
     "hh_util_gemm_nt_bf16x3": (_i32, [_i64, _i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64, _i32, _i32, _p]),
     "hh_util_gemm_nt_bf16x3_choice": (_i32, [_i64, _i64, _i64, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),

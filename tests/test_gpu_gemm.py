@@ -170,3 +170,17 @@ def test_gemm_nt_bf16x3_autotuner(dev):
     # the F64 classes are separate
     _lib.check(L.hh_util_gemm_nt_choice(M, N, K, C.byref(cfg), C.byref(ks)))
     assert cfg.value == -1
+
+
+def test_mfma_peak_probe(dev):
+    """The MFMA issue-rate probe returns a positive rate per instruction, bf16 above f64."""
+    import ctypes as C
+    from hesship import _lib
+    out = {}
+    for kind, iters in ((0, 50), (1, 500), (2, 25)):
+        v = C.c_double()
+        _lib.check(_lib.lib().hh_util_mfma_peak(kind, iters, C.byref(v), torch.cuda.current_stream().cuda_stream))
+        out[kind] = v.value
+    assert out[0] > 0 and out[1] > 0 and out[2] > 0 and out[0] > out[1]
+    with pytest.raises(_lib.HesshipError):
+        _lib.check(_lib.lib().hh_util_mfma_peak(3, 10, C.byref(C.c_double()), None))
