@@ -15,6 +15,12 @@ from . import krylov
 
 DEFAULTS = dict(level_shift=0.0, cphf_tol=1e-8, cphf_max_cycle=42, cphf_max_space=14, cphf_lindep=1e-14)
 
+# The reference's literal factors, named so that the tests can plant a wrong one (monkeypatch) and
+# show that the mathematical pins catch it.
+RHS_OCC = -0.5                        # rscf.rs:178   rhs[occ] = -0.5 s1mo[occ]
+HESS_R = (4.0, -4.0, -2.0)            # rscf.rs:387-391  f1.mo1, s1.mo1.e_occ, s1oo.mo_e1
+HESS_U = (2.0, -2.0, -1.0)            # uscf.rs:409-440  the same terms per spin
+
 
 def split(mo_occ, mo_energy):
     occ = np.asarray(mo_occ) > 0
@@ -32,7 +38,7 @@ def dimless_rhs(f1mo, s1mo, mo_occ, mo_energy, level_shift=0.0):
     b1 = f1mo - s1mo * eocc.reshape((1, -1) + tail)
     rhs = np.zeros_like(f1mo)
     rhs[nocc:] = -b1[nocc:] / e_ai.reshape(e_ai.shape + tail)
-    rhs[:nocc] = -0.5 * s1mo[:nocc]
+    rhs[:nocc] = RHS_OCC * s1mo[:nocc]
     return rhs
 
 
@@ -85,8 +91,13 @@ def finalize(mo_coeff, mo_occ, mo_energy, response_bras, f1mo, s1mo, mo1):
     b1 = f1mo - s1mo * eocc.reshape((1, -1) + tail) + response_mo(mo_coeff, response_bras, mo1)
     mo1 = mo1.copy()
     mo1[nocc:] = -b1[nocc:] / e_ai
-    mo_e1 = b1[:nocc] + mo1[:nocc] * e_ij
+    mo_e1 = mo_e1_of(b1, mo1, e_ij, nocc)
     return mo1, mo_e1
+
+
+def mo_e1_of(b1, mo1, e_ij, nocc):
+    """rscf.rs:350: mo_e1 = b1mo[occ] + mo1[occ] * (e_i - e_j)."""
+    return b1[:nocc] + mo1[:nocc] * e_ij
 
 
 def cphf_hess(f1mo, s1mo, mo1, mo_e1, mo_occ, mo_energy):
@@ -102,9 +113,10 @@ def cphf_hess(f1mo, s1mo, mo1, mo_e1, mo_occ, mo_energy):
         for B in range(A + 1):
             # f1mo[.., .., None, .., A] * mo1[.., .., .., None, B] -> [nmo, nocc, 3(s from mo1), 3(t from f1mo)]
             # summed over axes 0,1 gives de_BA[s, t] with s <-> mo1's xyz, t <-> f1mo's xyz.
-            blk = 4.0 * np.einsum("mit,mis->st", f1mo[:, :, :, A], mo1[:, :, :, B])
-            blk -= 4.0 * np.einsum("mit,mis,i->st", s1mo[:, :, :, A], mo1[:, :, :, B], eocc)
-            blk -= 2.0 * np.einsum("jit,jis->st", s1oo[:, :, :, A], mo_e1[:, :, :, B])
+            c_f, c_s, c_e = HESS_R
+            blk = c_f * np.einsum("mit,mis->st", f1mo[:, :, :, A], mo1[:, :, :, B])
+            blk += c_s * np.einsum("mit,mis,i->st", s1mo[:, :, :, A], mo1[:, :, :, B], eocc)
+            blk += c_e * np.einsum("jit,jis->st", s1oo[:, :, :, A], mo_e1[:, :, :, B])
             de[:, :, B, A] += blk
         for B in range(A):
             de[:, :, A, B] += de[:, :, B, A].T
@@ -184,7 +196,7 @@ def u_finalize(mo_coeff, mo_occ, mo_energy, response_bras, f1mo, s1mo, mo1):
         m = mo1[s].copy()
         m[nocc:] = -b1[nocc:] / e_ai
         mo1_out.append(m)
-        mo_e1.append(b1[:nocc] + m[:nocc] * e_ij)
+        mo_e1.append(mo_e1_of(b1, m, e_ij, nocc))
     return mo1_out, mo_e1
 
 
@@ -196,9 +208,10 @@ def u_cphf_hess(f1mo, s1mo, mo1, mo_e1, mo_occ, mo_energy):
         for B in range(A + 1):
             for s in range(2):
                 nocc, eocc, _ = split(mo_occ[s], mo_energy[s])
-                blk = 2.0 * np.einsum("mit,mis->st", f1mo[s][:, :, :, A], mo1[s][:, :, :, B])
-                blk -= 2.0 * np.einsum("mit,mis,i->st", s1mo[s][:, :, :, A], mo1[s][:, :, :, B], eocc)
-                blk -= np.einsum("jit,jis->st", s1mo[s][:nocc, :, :, A], mo_e1[s][:, :, :, B])
+                c_f, c_s, c_e = HESS_U
+                blk = c_f * np.einsum("mit,mis->st", f1mo[s][:, :, :, A], mo1[s][:, :, :, B])
+                blk += c_s * np.einsum("mit,mis,i->st", s1mo[s][:, :, :, A], mo1[s][:, :, :, B], eocc)
+                blk += c_e * np.einsum("jit,jis->st", s1mo[s][:nocc, :, :, A], mo_e1[s][:, :, :, B])
                 de[:, :, B, A] += blk
         for B in range(A):
             de[:, :, A, B] += de[:, :, B, A].T

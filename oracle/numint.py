@@ -12,6 +12,11 @@ nvar: RHO 1, SIGMA 4, TAU 5 (src/xceff/flags.rs:31-38).
 import numpy as np
 
 NVAR = {"rho": 1, "sigma": 4, "tau": 5}
+
+# The reference's closed-shell / open-shell response factors, named so that the tests can plant a
+# wrong one (monkeypatch) and show that the mathematical pins catch it.
+RKS_RESPONSE = 4.0   # hess_rks.rs:602-603
+UKS_RESPONSE = 2.0   # hess_uks.rs:436-439
 NCOMP = {"rho": 1, "sigma": 4, "tau": 4}
 
 
@@ -71,7 +76,7 @@ def rks_response_bra(den, fxc_eff, ao, weights, mo1_bra, mocc):
     nao, nocc = mocc.shape
     kets = mo1_bra.reshape(nao, nocc, -1)
     rho1 = rho1_one_bra_mult_ket(ao, mocc, kets, den)
-    return (4.0 * fxc_pot_bra_trans(den, fxc_eff, rho1, ao, weights, mocc)).reshape(shape)
+    return (RKS_RESPONSE * fxc_pot_bra_trans(den, fxc_eff, rho1, ao, weights, mocc)).reshape(shape)
 
 
 def brute_rho1(ao, bra, kets, den):
@@ -136,7 +141,7 @@ def uks_response_bra(den, fxc_eff, ao, weights, mo1_bra, mocc):
     kets = [m.reshape(m.shape[0], m.shape[1], -1) for m in mo1_bra]
     rho1 = np.stack([rho1_one_bra_mult_ket(ao, mocc[s], kets[s], den) for s in range(2)], axis=2)
     r = uks_fxc_pot_bra_trans(den, fxc_eff, rho1, ao, weights, mocc)
-    return [2.0 * r[s].reshape(mo1_bra[s].shape) for s in range(2)]
+    return [UKS_RESPONSE * r[s].reshape(mo1_bra[s].shape) for s in range(2)]
 
 
 def brute_uks_potential(den, fxc_eff, rho1, ao, weights):

@@ -216,6 +216,7 @@ hh_status hh_rijk_set_batch(hh_rijk* h, int64_t nprop_batch) {
   HH_ABI_BEGIN
   HH_REQUIRE(h != nullptr && nprop_batch >= 0, "null handle or negative batch");
   h->impl->k_chunk = nprop_batch;
+  bump_response_state();
   HH_ABI_END
 }
 
@@ -223,6 +224,7 @@ hh_status hh_rijk_set_profiling(hh_rijk* h, int32_t on) {
   HH_ABI_BEGIN
   HH_REQUIRE(h != nullptr, "handle is null");
   h->impl->set_profiling(on != 0);
+  bump_response_state();
   HH_ABI_END
 }
 
@@ -429,6 +431,7 @@ hh_status hh_numint_set_batch(hh_numint* h, int64_t nbatch) {
   HH_ABI_BEGIN
   HH_REQUIRE(h && nbatch >= 0, "null handle or negative batch");
   h->impl->batch = nbatch;
+  bump_response_state();
   HH_ABI_END
 }
 
@@ -436,6 +439,7 @@ hh_status hh_numint_set_fused(hh_numint* h, int32_t on) {
   HH_ABI_BEGIN
   HH_REQUIRE(h, "null handle");
   h->impl->fused_kets = on != 0;
+  bump_response_state();
   HH_ABI_END
 }
 

@@ -300,6 +300,7 @@ void Cphf::add_component(ElecResponse* c) {
   HH_REQUIRE(c && c->nspin() == nspin_, "cphf: component spin count differs from the driver's");
   comps_.push_back(c);
   fin_nprop_ = -1;
+  drop_graphs();  // a captured operator leaves the new component out
 }
 
 // ---- stacked vector <-> per-spin arrays (one spin: the same memory, no copy) ----
@@ -431,9 +432,12 @@ void Cphf::aop(const double* x, int64_t nb, double* ax, Prec p, const double* di
     aop_body(x, nb, ax, p, diag, ytmp);
     return;
   }
-  if (graph_gen_ != devbuf_generation().load()) {  // a buffer a graph may refer to was freed
+  if (graph_gen_ != devbuf_generation().load() || graph_state_ != response_state_epoch().load()) {
+    // a buffer a graph may refer to was freed, or a component's state changed (re-prepared
+    // orbitals: the graphs skip the reduced-precision copies made on the first eager call)
     drop_graphs();
     graph_gen_ = devbuf_generation().load();
+    graph_state_ = response_state_epoch().load();
   }
   AopGraph* e = nullptr;
   for (auto& g : graphs_)
@@ -601,7 +605,15 @@ void Cphf::solve_stacked(const double* rhs, int64_t nprop, double* mo1, Prec pre
       accumulate(ks);
       unprecond(y, mo1, nprop);
       stats.fallbacks = 1;
-      stats.residual = 0.0;
+      // the true FP64 residual of the returned solution (krylov_block's own test is on the
+      // trial-vector norms, about an order of magnitude below it)
+      response_mo_stacked(mo1, nprop, ax, PREC_F64);
+      ++stats.krylov.aop_calls;
+      dimless_scale(mo1, nprop, ax);
+      hipLaunchKernelGGL(residual_norm_kernel, dim3((unsigned)nprop), dim3(256), 0, stream_, rhs, mo1, ax, r, n,
+                         coln);
+      HH_HIP(hipGetLastError());
+      stats.residual = std::sqrt(max_colnorm2(coln));
       if (st) *st = stats;
       return;
     }
@@ -677,6 +689,7 @@ void Cphf::solve_stacked(const double* rhs, int64_t nprop, double* mo1, Prec pre
   if (stats.refinements > 0 && ok) {
     copy_bytes(fin_x_.reserve(sizeof(double) * blk), mo1, (int64_t)sizeof(double) * blk, stream_);
     fin_nprop_ = nprop;
+    fin_state_ = response_state_epoch().load();
   }
   if (!ok) {
     char buf[200];
@@ -693,7 +706,7 @@ void Cphf::finalize(const double* const* f1mo, const double* const* s1mo, const 
   const double* x = stack_in(mo1, nprop, st_in_);
   double* b1 = static_cast<double*>(tmp_.reserve(sizeof(double) * n_ * nprop));
   bool reuse = false;
-  if (fin_nprop_ == nprop) {
+  if (fin_nprop_ == nprop && fin_state_ == response_state_epoch().load()) {
     int* flag = static_cast<int*>(fin_flag_.reserve(64));
     hipLaunchKernelGGL(equal_kernel, dim3(1), dim3(1024), 0, stream_, x, fin_x_.as<double>(), n_ * nprop, flag);
     HH_HIP(hipGetLastError());

@@ -73,7 +73,8 @@ class Cphf {
   // and output addresses, width, precision) as an earlier one is captured once and replayed
   // afterwards (one graph launch instead of its tens of kernel launches).  Used when the driver
   // runs on a created stream (not the legacy default stream) and every component enqueues on it;
-  // graphs are dropped whenever a library buffer is freed.  Counters: graphs captured and replayed.
+  // graphs are dropped whenever a library buffer is freed, a component is added or a component's
+  // state changes (response_state_epoch).  Counters: graphs captured and replayed.
   bool use_graphs = true;
   int64_t graph_captures() const { return n_captures_; }
   int64_t graph_launches() const { return n_launches_; }
@@ -81,7 +82,7 @@ class Cphf {
   // residual holds the response of the returned mo1 (FP64, up to corrections whose response was
   // taken in the reduced precision with an error below 5 % of tol), which finalize needs again.
   // Reused when the mo1 given to finalize is bit-identical to that solve's output and no
-  // component was added since (components must not be re-prepared in between).
+  // component was added or re-prepared since.
   int64_t finalize_reuses() const { return n_fin_reuse_; }
   ~Cphf();
 
@@ -123,11 +124,12 @@ class Cphf {
     hipGraphExec_t exec;
   };
   std::vector<AopGraph> graphs_;
-  uint64_t graph_gen_ = 0;
+  uint64_t graph_gen_ = 0, graph_state_ = 0;
   int64_t n_captures_ = 0, n_launches_ = 0;
   void drop_graphs();
   DevBuf fin_x_, fin_r_, fin_flag_;
   int64_t fin_nprop_ = -1, n_fin_reuse_ = 0;
+  uint64_t fin_state_ = 0;  // component-state count when fin_r_ was taken
 
   int nspin_;
   int64_t nao_, n_ = 0;

@@ -42,6 +42,15 @@ inline std::atomic<uint64_t>& devbuf_generation() {
   return g;
 }
 
+// Counts changes of component state that a captured response graph bakes in: re-prepared
+// orbitals (and with them the reduced-precision copies made on first use), batch sizes, the fused
+// grid path, profiling events.  A graph captured under one count is not replayed under another.
+inline std::atomic<uint64_t>& response_state_epoch() {
+  static std::atomic<uint64_t> g{0};
+  return g;
+}
+inline void bump_response_state() { ++response_state_epoch(); }
+
 struct DevBuf {
   void* ptr = nullptr;
   size_t bytes = 0;

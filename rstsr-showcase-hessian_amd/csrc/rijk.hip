@@ -224,6 +224,7 @@ void Rijk::prepare_response(const double* const* mo_coeff, const double* const* 
   prepared_ = true;
   mixed_ready_ = false;
   xt_ready_ = false;
+  bump_response_state();
 }
 
 namespace {

@@ -192,3 +192,60 @@ def _graph_replay_body(dev):
     c1 = drv.graph_stats()
     assert st0["cycles"] == st1["cycles"] and c1[1] > c0[1]
     assert rel(b, a) < 1e-13
+
+
+def test_graph_cache_follows_reprepare_and_new_component(dev):
+    """A response graph must not be replayed after the component it captured changed: re-preparing
+    the RI-JK component with other orbitals (its reduced-precision copies are made on the first
+    eager call and skipped by the graph), or adding a component, gives the eager result."""
+    stream = torch.cuda.Stream(device=dev)
+    with torch.cuda.stream(stream):
+        _graph_reprepare_body(dev)
+    torch.cuda.synchronize()
+
+
+def _graph_reprepare_body(dev):
+    import hesship
+    from hesship import _lib
+    from hesship.tensors import empty_f, fortran
+    cderi, C_, occ, e, f1, s1, rb = system(seed=6)
+    drv, rijk = make(dev, cderi, C_, occ, e)
+    L = _lib.lib()
+    nmo, nocc = C_.shape[1], int((occ > 0).sum())
+    nprop = f1.shape[2] * f1.shape[3]
+    rng = np.random.default_rng(12)
+    x = fortran(T(rng.standard_normal((nmo, nocc, nprop)), dev))
+    out = empty_f((nmo, nocc, nprop), dev)
+    BF16X3 = 2
+
+    def call(graphs):
+        drv.set_graphs(graphs)
+        _lib.check(L.hh_cphf_response_dimless(drv._h, x.data_ptr(), nprop, out.data_ptr(), BF16X3))
+        torch.cuda.current_stream().synchronize()
+        return out.cpu().numpy().copy()
+
+    for _ in range(3):           # eager, capture + launch, replay
+        call(True)
+    assert drv.graph_stats()[1] >= 2
+    C2, _ = np.linalg.qr(np.random.default_rng(13).standard_normal(C_.shape))
+    rijk.make_response_preparation(T(C2, dev), occ)
+    got = [call(True) for _ in range(3)]
+    ref = call(False)
+    assert rel(ref, call(False)) == 0.0
+    before = drv.graph_stats()
+    for g in got:
+        assert rel(g, ref) < 1e-12
+    # the re-prepared component really changed the operator
+    rijk.make_response_preparation(T(C_, dev), occ)
+    assert rel(call(False), ref) > 1e-3
+    # a component added after capture is part of the replayed operator
+    for _ in range(3):
+        call(True)
+    extra = hesship.RHessRIJK(T(cderi, dev), 0.5, 0.0)
+    extra.make_response_preparation(T(C_, dev), occ)
+    drv.add_el(extra)
+    got = [call(True) for _ in range(3)]
+    ref = call(False)
+    for g in got:
+        assert rel(g, ref) < 1e-12
+    assert drv.graph_stats()[0] > before[0]
