@@ -450,6 +450,12 @@ hh_status hh_cphf_precond_diag(hh_cphf* h, const float* y, double* diag);
 /* Solve H X = G in place (H overwritten by LU factors, G by X); replaces solve_general as
  * used by projected_solve (krylov_block.rs:214).  HH_ERR_INVALID if singular.  Synchronous. */
 hh_status hh_linalg_solve(double* H, int64_t nd, double* G, int64_t nrhs, void* stream);
+/* The same solve for callers that cannot allocate device memory (code that must not call
+ * hipMalloc, tight loops, harnesses that forbid it); no counterpart in the reference.  Allocates
+ * nothing: the pivots and status word live in a static device array of 8 * 8192 + 256 bytes per
+ * GPU, so nd <= 8192 (else HH_ERR_INVALID).  The array is shared by all calls on one GPU and keeps
+ * the last call's contents: calls that overlap on one GPU race.  Synchronous. */
+hh_status hh_linalg_solve_noalloc(double* H, int64_t nd, double* G, int64_t nrhs, void* stream);
 /* Lower Cholesky in place (decomp_j2c_cd, pure_decompose.rs:107-133).  HH_ERR_INVALID if not
  * positive definite.  Synchronous. */
 hh_status hh_linalg_cholesky(double* H, int64_t nd, void* stream);

@@ -874,6 +874,14 @@ hh_status hh_linalg_solve(double* H, int64_t nd, double* G, int64_t nrhs, void* 
   HH_ABI_END
 }
 
+hh_status hh_linalg_solve_noalloc(double* H, int64_t nd, double* G, int64_t nrhs, void* stream) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(nd >= 0 && nrhs >= 0 && (nd == 0 || (H && G)), "invalid argument");
+  lu_solve_noalloc(H, nd, G, nrhs, (hipStream_t)stream);
+  HH_HIP(hipStreamSynchronize((hipStream_t)stream));
+  HH_ABI_END
+}
+
 hh_status hh_linalg_cholesky(double* H, int64_t nd, void* stream) {
   HH_ABI_BEGIN
   HH_REQUIRE(nd >= 0 && (nd == 0 || H), "invalid argument");

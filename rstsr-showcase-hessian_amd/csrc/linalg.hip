@@ -5,6 +5,8 @@
 // matrix-core GEMM.  Pivot choice: largest |a|, ties to the smallest row index (deterministic).
 #include "linalg.hpp"
 
+#include <mutex>
+
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -258,10 +260,13 @@ int read_info(const int* dinfo, hipStream_t s) {
 
 }  // namespace
 
-void lu_solve(double* H, int64_t nd, double* G, int64_t nrhs, hipStream_t s, DevBuf& ws) {
+void lu_solve(double* H, int64_t nd, double* G, int64_t nrhs, hipStream_t s, DevBuf& ws, void* scratch) {
   if (nd == 0) return;
   DevBuf meta;
-  char* m = static_cast<char*>(meta.reserve(sizeof(int64_t) * nd + 256));
+  // pivots [nd] and the status word: the caller's scratch (no allocation) or a buffer of our own
+  char* m = scratch ? static_cast<char*>(scratch) : static_cast<char*>(meta.reserve(sizeof(int64_t) * nd + 256));
+  // with caller scratch the trailing products must not split K (a split takes workspace)
+  const int gemm_split = scratch ? 1 : 0;
   int64_t* piv = reinterpret_cast<int64_t*>(m);
   int* info = reinterpret_cast<int*>(m + sizeof(int64_t) * nd);
   hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, s, info);
@@ -283,6 +288,7 @@ void lu_solve(double* H, int64_t nd, double* G, int64_t nrhs, hipStream_t s, Dev
       g.M = mrows; g.K = nb;
       g.A = H + (k0 + nb) + k0 * nd; g.lda = nd; g.a_kmajor = false;
       g.alpha = -1.0; g.beta = 1.0;
+      g.ksplit = gemm_split;
       if (ntrail > 0) {
         g.N = ntrail;
         g.B = H + k0 + (k0 + nb) * nd; g.ldb = nd; g.b_kmajor = true;
@@ -314,9 +320,42 @@ void lu_solve(double* H, int64_t nd, double* G, int64_t nrhs, hipStream_t s, Dev
       g.B = G + r0; g.ldb = nd; g.b_kmajor = true;
       g.C = G; g.ldc = nd;
       g.alpha = -1.0; g.beta = 1.0;
+      g.ksplit = gemm_split;
       gemm<double, double>(g, PREC_F64, s, ws);
     }
   }
+}
+
+// ---- the allocation-free solve ----
+// Pivots and the status word live in this file-scope device array.  Each GPU holds its own copy
+// (until the process ends); host code reaches it only through the address that
+// hipGetSymbolAddress returns with that device current, kept per device below.
+static __device__ __align__(256) char lu_scratch[LU_SCRATCH_BYTES];
+
+void* lu_scratch_ptr() {
+  static std::mutex mu;
+  static void* ptr[64] = {};
+  int dev = 0;
+  HH_HIP(hipGetDevice(&dev));
+  HH_REQUIRE(dev >= 0 && dev < 64, "lu scratch: device index out of range");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    HH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(lu_scratch)));
+    HH_REQUIRE(p != nullptr, "lu scratch: no device address");
+    ptr[dev] = p;
+  }
+  return ptr[dev];
+}
+
+void lu_solve_noalloc(double* H, int64_t nd, double* G, int64_t nrhs, hipStream_t s) {
+  HH_REQUIRE(nd <= LU_NOALLOC_MAX_ND, "lu_solve_noalloc: nd " + std::to_string(nd) + " exceeds the scratch limit " +
+                                          std::to_string(LU_NOALLOC_MAX_ND));
+  if (nd == 0) return;
+  void* scratch = lu_scratch_ptr();
+  DevBuf unused;  // never reserved: every product runs with ksplit = 1
+  lu_solve(H, nd, G, nrhs, s, unused, scratch);
+  HH_REQUIRE(unused.ptr == nullptr, "lu_solve_noalloc: internal allocation");
 }
 
 bool cholesky(double* H, int64_t nd, hipStream_t s, DevBuf& ws) {

@@ -7,7 +7,16 @@ namespace hh {
 // Solve H X = G in place: H [nd, nd] (column-major, ld = nd) is overwritten by its LU factors,
 // G [nd, nrhs] (ld = nd) by X.  Throws Error(HH_ERR_INVALID) if H is singular (zero pivot).
 // Replaces rstsr linalg solve_general (LAPACK getrf/getrs) as used by krylov_block.rs:214.
-void lu_solve(double* H, int64_t nd, double* G, int64_t nrhs, hipStream_t s, DevBuf& ws);
+// `scratch` (optional): device memory of at least 8 nd + 256 bytes for the pivots and the status
+// word; with it the solve allocates nothing (ws stays untouched).
+void lu_solve(double* H, int64_t nd, double* G, int64_t nrhs, hipStream_t s, DevBuf& ws, void* scratch = nullptr);
+
+// The same solve without any device allocation: pivots and status in a file-scope device array
+// of LU_SCRATCH_BYTES (nd <= LU_NOALLOC_MAX_ND, else Error(HH_ERR_INVALID)).  The array is shared
+// by every call on one GPU and keeps the last call's contents: calls that overlap on one GPU race.
+constexpr int64_t LU_NOALLOC_MAX_ND = 8192;
+constexpr int64_t LU_SCRATCH_BYTES = 8 * LU_NOALLOC_MAX_ND + 256;
+void lu_solve_noalloc(double* H, int64_t nd, double* G, int64_t nrhs, hipStream_t s);
 
 // Cholesky H = L L^T in place (lower, column-major, ld = nd); returns false if not positive
 // definite (a pivot <= 0 was met).

@@ -80,6 +80,7 @@ SIGNATURES.update({
     "hh_ucphf_finalize": (_i32, [_p, _pp, _pp, _pp, _i64, _pp, _pp]),
     "hh_ucphf_hess": (_i32, [_p, _pp, _pp, _pp, _pp, _i64, _p]),
     "hh_linalg_solve": (_i32, [_p, _i64, _p, _i64, _p]),
+    "hh_linalg_solve_noalloc": (_i32, [_p, _i64, _p, _i64, _p]),
     "hh_linalg_cholesky": (_i32, [_p, _i64, _p]),
     "hh_linalg_trsm_lower": (_i32, [_p, _i64, _p, _i64, _i64, _i32, _p]),
     "hh_util_copy": (_i32, [_p, _p, _i64, _p]),
