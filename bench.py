@@ -48,13 +48,28 @@ BF16X3_CONFIGS = {0: "nt_bf16x3_kernel<1,8,13,2,true,13,2> (208x256 tile)",
 SPLIT_RULES = {0: "about one workgroup per CU", -1: "last wave filled"}
 
 
-def measured_mfma_peak(precision):
+def kfused_supported(nao, nocc):
+    """The fused BF16x3 K path's range (rijk_kfused.hip: kfused_supported)."""
+    return 8 <= nocc <= 80 and -(-nao // 32) <= 13 and 2 + 30 // nocc <= (5 if nocc <= 16 else 3 if nocc <= 32 else 2)
+
+
+def kfused_name(nao, nocc):
+    nb = -(-nao // 32)
+    nmb = 4 if nb <= 4 else 8 if nb <= 8 else 10 if nb <= 10 else 12 if nb <= 12 else 13
+    nh = -(-nocc // 16)
+    na = {1: 5, 2: 3}.get(nh, 2)
+    return f"kfused_kernel<{nmb},{nh},{na}> (all {32 * nmb} rows x 128 columns per workgroup, 32x32x16 MFMA)"
+
+
+def measured_mfma_peak(precision, kind=None):
     """Issue-rate ceiling of the MFMA instruction the hot kernel uses, measured on this device
     (hh_util_mfma_peak: independent accumulator chains, no memory traffic), TFLOP/s."""
     import ctypes as C
     import torch
     from hesship import _lib
-    kind, iters = (1, 20000) if precision == "fp64" else (0, 500)
+    if kind is None:
+        kind = 1 if precision == "fp64" else 0
+    iters = {0: 500, 1: 20000, 2: 250}[kind]
     out = C.c_double()
     _lib.check(_lib.lib().hh_util_mfma_peak(kind, iters, C.byref(out), torch.cuda.current_stream().cuda_stream))
     return out.value
@@ -695,13 +710,19 @@ def main():
     tk = tm["k_total_ms"] * 1e-3
     value = fk / tk / 1e12
     hot_cfg, hot_name = None, HOT_KERNEL[args.precision]
-    if args.precision == "bf16x3":
+    fused = args.precision == "bf16x3" and kfused_supported(nao, nocc)
+    if fused:
+        hot_cfg, hot_name = "fused", kfused_name(nao, nocc)
+    elif args.precision == "bf16x3":
         hot_cfg, hot_ks = hot_choice(nao, naux, nocc, nprop)
         if hot_cfg in BF16X3_CONFIGS:
             hot_name = f"{BF16X3_CONFIGS[hot_cfg]}, split rule: {SPLIT_RULES.get(hot_ks, hot_ks)}"
-    # dominant kernel: the hot K contraction GEMM; per launch it covers one aux chunk, and its
-    # algorithmic work per aux index is 2 nao^2 nocc nprop.
+    # dominant kernel: the hot K contraction; its algorithmic work per aux index is
+    # 2 nao^2 nocc nprop (the contraction with B_P), plus 2 nao nocc^2 nprop for the fused kernel,
+    # which also generates its operand Q_aP = bra_a O_P.
     hot_flops_call = 2.0 * nao**2 * nocc * nprop * naux
+    if fused:
+        hot_flops_call += 2.0 * nao * nocc**2 * nprop * naux
     nl = max(tm["k_hot_launches"], 1)
     t_launch = tm["k_hot_ms"] / nl * 1e-3
     achieved = hot_flops_call / nl / t_launch / 1e12
@@ -734,16 +755,22 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4),
                      "traffic": MEASURED_TRAFFIC.get((args.precision, nao, naux, nocc, nprop, hot_cfg)),
-                     "kernel": hot_name + " (hot K contraction, one perturbation chunk per launch)",
+                     "kernel": hot_name + (" (fused K term 1: operand generation + contraction, all perturbations in one launch)"
+                                           if fused else " (hot K contraction, one perturbation chunk per launch)"),
                      "launches_per_step": nl, "avg_launch_ms": round(t_launch * 1e3, 4)},
     }
     if args.precision in ("bf16x3", "fp64"):
-        mp = measured_mfma_peak(args.precision)
+        kind = 2 if fused else (0 if args.precision == "bf16x3" else 1)
+        mp = measured_mfma_peak(args.precision, kind)
         scale = 3.0 if args.precision == "bf16x3" else 1.0  # MFMA flops per algorithmic flop
         line["roofline"]["measured_mfma_peak"] = {
-            "instruction": "v_mfma_f32_16x16x32_bf16" if args.precision == "bf16x3" else "v_mfma_f64_16x16x4f64",
+            "instruction": {0: "v_mfma_f32_16x16x32_bf16", 1: "v_mfma_f64_16x16x4f64",
+                            2: "v_mfma_f32_32x32x16_bf16"}[kind],
             "tflops": round(mp, 1), "mfma_flops_per_algorithmic_flop": scale,
             "frac": round(achieved * scale / mp, 4)}
+        # the matrix-core work the launch issues (bf16 products, padding included) against the
+        # published peak: the fraction of the chip's MFMA rate the kernel sustains
+        line["roofline"]["mfma_work_frac_of_peak"] = round(achieved * scale / peak, 4)
     if not args.no_cphf:
         # CPHF wall time: same tensors, RI-JK scaled so the dimensionless response is ~0.3 of its
         # input (the benchmark's raw N(0,1) cderi would make I + A far from a physical CPHF)

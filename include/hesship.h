@@ -114,6 +114,10 @@ hh_status hh_rijk_set_batch(hh_rijk* h, int64_t nprop_batch);
  * times in `timing`, hess_r.rs:1746).  When on, each response call waits for its own events.
  * out[0] hot K-contraction kernel ms (summed), out[1] K total ms, out[2] J total ms,
  * out[3] whole response ms, out[4] number of hot-kernel launches. */
+/* BF16x3 RI-K algorithm: 0 (default) the fused contraction with its operand generated in place
+ * (8 <= nocc <= 80, nao <= 512), 2 the materialised operand; the same algebra as the reference's
+ * K loop (hess_r.rs:144-183).  FP64 / FP32 are unaffected. */
+hh_status hh_rijk_set_k_path(hh_rijk* h, int32_t path);
 hh_status hh_rijk_set_profiling(hh_rijk* h, int32_t on);
 hh_status hh_rijk_get_timing(const hh_rijk* h, double* out5);
 hh_status hh_rijk_reset_timing(hh_rijk* h);

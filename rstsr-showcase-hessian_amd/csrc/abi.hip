@@ -220,6 +220,14 @@ hh_status hh_rijk_set_batch(hh_rijk* h, int64_t nprop_batch) {
   HH_ABI_END
 }
 
+hh_status hh_rijk_set_k_path(hh_rijk* h, int32_t path) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h != nullptr && (path == 0 || path == 2), "null handle or unknown K path");
+  h->impl->k_path = path;
+  bump_response_state();
+  HH_ABI_END
+}
+
 hh_status hh_rijk_set_profiling(hh_rijk* h, int32_t on) {
   HH_ABI_BEGIN
   HH_REQUIRE(h != nullptr, "handle is null");

@@ -525,16 +525,6 @@ __global__ __launch_bounds__(256) void mfma_peak_kernel(int iters, float* __rest
   out[(int64_t)blockIdx.x * blockDim.x + t] = s;
 }
 
-int cu_count() {
-  static int n = 0;
-  if (n == 0) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
-      n = 256;
-  }
-  return n;
-}
 
 template <int WGM, int WGN, int FM, int FN, bool BF, bool TILED = false, int BMT = WGM * FM, int PD = 2,
           bool SWZ = false>
@@ -711,6 +701,17 @@ NtChoice tuned(const NtArgs& a, NtFormat fmt, hipStream_t s, DevBuf& ws) {
 }
 
 }  // namespace
+
+int cu_count() {
+  static int n = 0;
+  if (n == 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+      n = 256;
+  }
+  return n;
+}
 
 void gemm_nt_set_autotune(bool on) {
   std::lock_guard<std::mutex> lk(g_tune_mu);

@@ -83,6 +83,9 @@ struct DevBuf {
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// compute units of the current device (256 if the query fails)
+int cu_count();
+
 }  // namespace hh
 
 // Wrap a C-ABI body: translate exceptions into status codes.

@@ -22,6 +22,7 @@
 
 #include "elementwise.hpp"
 #include "linalg.hpp"
+#include "rijk_kfused.hpp"
 #include "rijk_tgen.hpp"
 
 namespace hh {
@@ -224,6 +225,7 @@ void Rijk::prepare_response(const double* const* mo_coeff, const double* const* 
   prepared_ = true;
   mixed_ready_ = false;
   xt_ready_ = false;
+  kf_ready_ = false;
   bump_response_state();
 }
 
@@ -483,6 +485,10 @@ void Rijk::build_k(int s, const double* bra, int64_t nprop, Prec prec, double al
     a.tiled = true;
     a.C_f32 = static_cast<float*>(W); a.ldc = ldw;
     gemm_nt(a, NT_BF16X3, stream_, ws_);
+    if (k_path != 2 && kfused_supported(nao_, nocc)) {
+      build_k_fused(s, bra, nprop, static_cast<const float*>(W), alpha, beta, out);
+      return;
+    }
   } else {
     GemmArgs g;
     g.M = ldw; g.N = nP; g.K = nao_;
@@ -580,6 +586,67 @@ void Rijk::build_k(int s, const double* bra, int64_t nprop, Prec prec, double al
       gemm_nt(a, NT_F64, stream_, ws_);
     }
   }
+}
+
+// BF16x3 K with the fused first term (rijk_kfused.hpp): term 1 = sum_P B_P bra_a O_P with its
+// operand generated inside the contraction, term 2 = sum_P X_P W_aP as one NT product over
+// K = (P, j) with W permuted to rows (i, a).
+void Rijk::build_k_fused(int s, const double* bra, int64_t nprop, const float* W, double alpha, double beta,
+                         double* out) {
+  const int64_t nocc = nocc_[s];
+  const int64_t nb = kfused_nb(nao_), jp = kfused_jp(nocc), np = kfused_naux_pad(naux_);
+  if (!kb_hi_.ptr) {  // per handle
+    const int64_t n = kfused_b_len(nao_, naux_);
+    kb_hi_.reserve(sizeof(unsigned short) * n);
+    kb_lo_.reserve(sizeof(unsigned short) * n);
+    kfused_pack_b(cderi_.as<double>(), nao_, naux_, kb_hi_.as<unsigned short>(), kb_lo_.as<unsigned short>(),
+                  stream_);
+  }
+  if (!kf_ready_) {  // per prepare_response
+    for (int t = 0; t < nspin_; ++t) {
+      const int64_t no = nocc_[t];
+      if (!kfused_supported(nao_, no)) continue;
+      const int64_t on = kfused_o_len(no, naux_), xn = np * no * nao_;
+      ko_[t].reserve(2 * sizeof(unsigned short) * on);
+      kx_[t].reserve(2 * sizeof(unsigned short) * xn);
+      unsigned short* o = ko_[t].as<unsigned short>();
+      unsigned short* x = kx_[t].as<unsigned short>();
+      kfused_pack_o(o_[t].as<double>(), no, naux_, o, o + on, stream_);
+      kfused_pack_x(x32_[t].as<float>(), nao_, naux_, no, x, x + xn, stream_);
+    }
+    kf_ready_ = true;
+  }
+  const int64_t rn = nprop * nb * 32 * jp;
+  unsigned short* r = static_cast<unsigned short*>(kr_.reserve(2 * sizeof(unsigned short) * rn));
+  kfused_pack_bra(bra, nao_, nocc, nprop, r, r + rn, stream_);
+  const int64_t wn = np * nocc * nocc * nprop;
+  unsigned short* wp = static_cast<unsigned short*>(kw_.reserve(2 * sizeof(unsigned short) * wn));
+  kfused_permute_w(W, naux_, nocc, nprop, wp, wp + wn, stream_);
+  KfusedArgs k;
+  const int64_t on = kfused_o_len(nocc, naux_);
+  k.bh = kb_hi_.as<unsigned short>(); k.bl = kb_lo_.as<unsigned short>();
+  k.oh = ko_[s].as<unsigned short>(); k.ol = k.oh + on;
+  k.rh = r; k.rl = r + rn;
+  k.nao = nao_; k.naux = naux_; k.nocc = nocc; k.nprop = nprop;
+  k.alpha = alpha; k.beta = beta; k.out = out;
+  if (profiling_) {
+    const int e0 = next_event_++, e1 = next_event_++;
+    k.ev_begin = ev(e0);
+    k.ev_main_end = ev(e1);
+    pending_.push_back({T_K_HOT, {e0, e1}});
+  }
+  kfused_term1(k, kf_ws_, stream_);
+  // term 2: out += alpha sum_{(P, j)} X'[mu, (P, j)] W'[(i, a), (P, j)]
+  const int64_t xn = np * nocc * nao_;
+  NtArgs a;
+  a.M = nao_; a.N = nocc * nprop; a.K = np * nocc;
+  a.A = kx_[s].as<unsigned short>(); a.A_lo = kx_[s].as<unsigned short>() + xn;
+  a.B = wp; a.B_lo = wp + wn;
+  a.lda = a.ldb = np * nocc;
+  a.tiled = true;
+  a.C = out; a.ldc = nao_;
+  a.alpha = alpha; a.beta = 1.0;
+  gemm_nt(a, NT_BF16X3, stream_, ws_);
 }
 
 void Rijk::response_bra_separated(const double* const* bra, int64_t nprop, double* j_ao, double* const* k_bra,

@@ -53,6 +53,9 @@ class Rijk {
 
   // Perturbations per K-contraction batch (0 = choose from the size of the intermediate).
   int64_t k_chunk = 0;
+  // BF16x3 K path: 0 = the fused contraction where supported (rijk_kfused.hpp), 2 = always the
+  // materialised operand (T generation + NT contraction).
+  int k_path = 0;
 
   // Stage timing with HIP events on the handle's stream (the reference keeps wall-clock
   // `timing` vectors per component, hess_r.rs:1746).  Indices of the accumulated times (ms):
@@ -68,6 +71,8 @@ class Rijk {
   // K: out_s[nao, nocc_s, nprop] = alpha * (K_s without its minus sign) + beta * out_s
   void build_k(int s, const double* bra, int64_t nprop, Prec prec, double alpha, double beta, double* out);
   void ensure_mixed();
+  void build_k_fused(int s, const double* bra, int64_t nprop, const float* W, double alpha, double beta,
+                     double* out);
   // Packed hot-GEMM operand Bp[u][P][mu_pad] (mu_pad = kpad_ = nao rounded up to 32, zeros
   // beyond nao): FP64 copy and/or bf16 hi/lo split, built once per handle.
   void ensure_packed(Prec prec);
@@ -90,6 +95,7 @@ class Rijk {
   bool prepared_ = false;
   bool mixed_ready_ = false;
   bool xt_ready_ = false;
+  bool kf_ready_ = false;
   int64_t nocc_[2] = {0, 0};
 
   DevBuf j2c_l_;      // [naux, naux] lower Cholesky factor of the metric (from_j3c, HH_J2C_CD)
@@ -117,6 +123,9 @@ class Rijk {
   // K-block tiled NT layout twice, K = the AO pair (cj, rows P) and K = P (cjt, rows the AO pair);
   // jsp_: the per-call split dm_tp and itm
   DevBuf cj_hi_, cj_lo_, cjt_hi_, cjt_lo_, jsp_;
+  // fused BF16x3 K (rijk_kfused.hpp): cderi in the contraction layout (per handle); O and X'
+  // (hi then lo, per prepare_response and spin); per call the packed bra, W' and the partials
+  DevBuf kb_hi_, kb_lo_, ko_[2], kx_[2], kr_, kw_, kf_ws_;
   bool j_bf16_ready_ = false;
 };
 

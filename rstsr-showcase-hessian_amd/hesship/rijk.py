@@ -90,6 +90,11 @@ class _RIJKBase:
         """Perturbations per K-contraction batch (0 = sized from memory)."""
         check(lib().hh_rijk_set_batch(self._h, int(nprop_batch)))
 
+    def set_k_path(self, path):
+        """BF16x3 RI-K algorithm: "fused" (operand generated inside the contraction, default where
+        supported) or "materialized" (T generation + NT contraction)."""
+        check(lib().hh_rijk_set_k_path(self._h, {"fused": 0, "materialized": 2}[path]))
+
     def set_profiling(self, on=True):
         check(lib().hh_rijk_set_profiling(self._h, int(bool(on))))
 
