@@ -153,6 +153,45 @@ def cphf_case(obj, d_C, occ, nao, nocc, nprop, dev, config=None):
     return drv, e, torch.from_numpy(f1).to(dev), torch.from_numpy(s1).to(dev), ratio
 
 
+def resident_leg(d_cderi, d_C, occ, d_bra, r64, nao, nocc, nprop, steps):
+    """The resident RHF response (rijk_resident.hip): M [nao nocc, nao nocc] built from the same
+    tensors (timed: the first call after make_response_preparation), then `steps` calls, each one
+    product that streams M once.  Roofline: HBM, M's bytes per call / call time."""
+    import torch
+    import hesship
+    out = {}
+    R = nao * nocc
+    kpad = -(-R // 32) * 32
+    scale = r64.abs().max().item()
+    for prec in ("bf16x3", "fp64"):
+        obj = hesship.RHessRIJK(d_cderi, 1.0, 1.0)
+        obj.set_response_path("resident")
+        obj.make_response_preparation(d_C, occ)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = obj.get_response_bra(d_bra, precision=prec)
+        torch.cuda.synchronize()
+        build_s = time.perf_counter() - t0
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            r = obj.get_response_bra(d_bra, precision=prec)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        mbytes = R * kpad * (4 if prec == "bf16x3" else 8)
+        gbs = mbytes / (ms * 1e-3) / 1e9
+        out[prec] = {"build_plus_first_call_s": round(build_s, 4), "ms_per_call": round(ms, 4),
+                     "matrix_bytes": mbytes, "achieved_GBps": round(gbs, 1), "hbm_frac": round(gbs / 8000.0, 4),
+                     "build_flops": 4.0 * R * R * (d_cderi.shape[1] if d_cderi.dim() > 1 else 0),
+                     "rel_err_vs_fp64_direct": (r - r64).abs().max().item() / scale}
+        del obj
+        torch.cuda.empty_cache()
+    out["note"] = ("resp = M bra with M = 4 sj sum_P X_P (x) X_P - sk sum_P (B_P (x) O_P + X_P (x) X_P) built once per "
+                   "orbital set (two products of 2 (nao nocc)^2 naux flop); per call 2 (nao nocc)^2 nprop flop")
+    return out
+
+
 def model_frequency_shift(dde):
     """Largest change (cm^-1) of the harmonic frequencies of a model (H2O)_n Hessian when the
     CPHF Hessian difference dde [3, 3, natm, natm] (mixed - FP64, Eh/bohr^2) is added to it.  The
@@ -190,16 +229,82 @@ def model_frequency_shift(dde):
     return float(np.abs(v1[:n] - v0[:n]).max())
 
 
-def run_cphf(drv, f1, s1, prec):
+def run_cphf(drv, f1, s1, prec, prepare=None):
+    """rhs -> Krylov -> finalize, timed; `prepare` (re-preparing the RI-JK component, which for the
+    resident operator includes building its response matrix on the first response) inside the
+    timed region."""
     import torch
     st = {}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    if prepare is not None:
+        prepare()
     rhs = drv.compute_dimless_rhs(f1, s1)
     mo1 = drv.solve_dimless_cphf(rhs, precision=prec, stats=st)
     mo1f, moe1 = drv.finalize_cphf(f1, s1, mo1)
     torch.cuda.synchronize()
     return mo1f, moe1, time.perf_counter() - t0, st
+
+
+def cphf_legs(obj, d_cderi, d_C, occ, nao, nocc, nprop, dev):
+    """The wall-time half of the metric on a synthetic CPHF system built on the benchmark tensors
+    (RI-JK scaled so the dimensionless response is ~0.3 of its input; the raw N(0,1) cderi would
+    make I + A far from a physical CPHF).  Every leg's wall time covers prepare_response of the
+    RI-JK component (X, O; for the resident operator also its response matrix), the right-hand
+    side, the Krylov solve and finalize.  Legs: the plain FP64 path (the reference's per-call
+    algorithm and stopping rule), FP64 asked for the mixed solves' accuracy, FP64 with the
+    resident operator, and mixed precision (BF16x3 inner solves, FP64 residuals) with the per-call
+    and the resident operator at residual tolerances 1e-8 and 1e-9.  Accuracy against the same
+    system solved in FP64 to 1e-11."""
+    import hesship
+    probe, _, _, _, ratio = cphf_case(obj, d_C, occ, nao, nocc, nprop, dev)
+    del probe
+    scale = (0.3 / ratio) ** 0.5
+    cd = d_cderi * scale
+    direct = hesship.RHessRIJK(cd, 1.0, 1.0)
+    direct.make_response_preparation(d_C, occ)
+    resident = hesship.RHessRIJK(cd, 1.0, 1.0)
+    resident.set_response_path("resident")
+    resident.make_response_preparation(d_C, occ)
+    res = {"natm_equiv": nprop // 3}
+    ta = lambda x: x.reshape(*x.shape[:-1], nprop // 3, 3).transpose(-2, -1)  # p = t + 3 A
+    tight, e, f1, s1, ratio2 = cphf_case(direct, d_C, occ, nao, nocc, nprop, dev,
+                                         {"cphf_tol": 1e-11, "cphf_lindep": 1e-24, "cphf_max_cycle": 100})
+    res["response_ratio"] = round(ratio2, 4)
+    mo1t, moe1t, _, stt = run_cphf(tight, f1, s1, "fp64")
+    de_t = tight.get_cphf_hess(ta(f1), ta(s1), ta(mo1t), ta(moe1t))
+    res["accuracy_reference"] = {"cphf_tol": 1e-11, "cphf_lindep": 1e-24, "cycles": stt.get("cycles")}
+    del tight
+    legs = [  # name, component, precision, config
+        ("fp64", direct, "fp64", {"cphf_tol": 1e-8}),
+        ("fp64_matched", direct, "fp64", {"cphf_tol": 1e-9, "cphf_lindep": 1e-24, "cphf_max_cycle": 100}),
+        ("fp64_resident", resident, "fp64", {"cphf_tol": 1e-8}),
+        ("mixed_direct", direct, "bf16x3", {"cphf_tol": 1e-8}),
+        ("mixed_resident", resident, "bf16x3", {"cphf_tol": 1e-8}),
+        ("mixed_resident_tol1e-9", resident, "bf16x3", {"cphf_tol": 1e-9}),
+    ]
+    for name, comp, prec, cfg in legs:
+        drv, _, _, _, _ = cphf_case(comp, d_C, occ, nao, nocc, nprop, dev, cfg)
+        drv.set_verbose(name.startswith("mixed"))
+        prep = lambda c=comp: c.make_response_preparation(d_C, occ)
+        run_cphf(drv, f1, s1, prec, prep)  # untimed: products of new shapes are tuned on first use
+        mo1, moe1, wall, st = run_cphf(drv, f1, s1, prec, prep)
+        de = drv.get_cphf_hess(ta(f1), ta(s1), ta(mo1), ta(moe1))
+        res[name] = {"precision": prec, "operator": "resident" if comp is resident else "per call",
+                     "cphf_tol": cfg["cphf_tol"], "wall_s": round(wall, 4), "cycles": st.get("cycles"),
+                     "aop_calls": st.get("aop_calls"), "refinements": st.get("refinements"),
+                     "residual": st.get("residual"), "fp64_fallback": st.get("fp64_fallback"),
+                     "max_abs_mo1_err_vs_tight": (mo1 - mo1t).abs().max().item(),
+                     "max_abs_de_cphf_err_vs_tight": (de - de_t).abs().max().item(),
+                     "freq_shift_vs_tight_cm1": model_frequency_shift((de - de_t).cpu().numpy())}
+        log(f"[bench] cphf {name}: {wall:.3f} s, cycles {st.get('cycles')}, aop calls {st.get('aop_calls')}, "
+            f"fallback {st.get('fp64_fallback')}")
+        del drv
+    base = res["fp64"]["wall_s"]
+    res["speedup_vs_fp64"] = {k: round(base / res[k]["wall_s"], 3) for k, *_ in legs if k != "fp64"}
+    res["speedup_vs_fp64_matched"] = {k: round(res["fp64_matched"]["wall_s"] / res[k]["wall_s"], 3)
+                                      for k, *_ in legs if k.startswith("mixed") or k.endswith("resident")}
+    return res
 
 
 # ---- other hot-path pieces (--workload) ----
@@ -771,74 +876,10 @@ def main():
         # the matrix-core work the launch issues (bf16 products, padding included) against the
         # published peak: the fraction of the chip's MFMA rate the kernel sustains
         line["roofline"]["mfma_work_frac_of_peak"] = round(achieved * scale / peak, 4)
+    # resident response matrix: build once per orbital set, then one HBM-bound product per call
+    line["resident"] = resident_leg(d_cderi, d_C, occ, d_bra, r64, nao, nocc, nprop, args.steps)
     if not args.no_cphf:
-        # CPHF wall time: same tensors, RI-JK scaled so the dimensionless response is ~0.3 of its
-        # input (the benchmark's raw N(0,1) cderi would make I + A far from a physical CPHF)
-        probe, _, _, _, ratio = cphf_case(obj, d_C, occ, nao, nocc, nprop, dev)
-        del probe
-        scale = (0.3 / ratio) ** 0.5
-        obj2 = hesship.RHessRIJK(d_cderi * scale, 1.0, 1.0)
-        obj2.make_response_preparation(d_C, occ)
-        drv, e, f1, s1, ratio2 = cphf_case(obj2, d_C, occ, nao, nocc, nprop, dev)
-        drv.set_verbose(True)
-        res = {"natm_equiv": nprop // 3, "cphf_tol": 1e-8, "response_ratio": round(ratio2, 4)}
-        # perturbation p = t + 3 A (the reference's [3, natm] column-major axes)
-        ta = lambda x: x.reshape(*x.shape[:-1], nprop // 3, 3).transpose(-2, -1)
-        # accuracy reference: the same system solved in FP64 to 1e-11 (both 1e-8 solves below are
-        # compared with it: at 1e-8 the two differ from each other by about as much as from it)
-        tight, _, _, _, _ = cphf_case(obj2, d_C, occ, nao, nocc, nprop, dev, {"cphf_tol": 1e-11, "cphf_lindep": 1e-24, "cphf_max_cycle": 100})
-        mo1t, moe1t, _, stt = run_cphf(tight, f1, s1, "fp64")
-        de_t = tight.get_cphf_hess(ta(f1), ta(s1), ta(mo1t), ta(moe1t))
-        res["accuracy_reference"] = {"cphf_tol": 1e-11, "cphf_lindep": 1e-24, "cycles": stt.get("cycles")}
-        del tight
-        # plain FP64 asked for the accuracy the mixed solve reaches: trial vectors to 1e-9 with the
-        # lindep floor lowered so that the tolerance, not the floor, ends the solve
-        eq, _, _, _, _ = cphf_case(obj2, d_C, occ, nao, nocc, nprop, dev,
-                                   {"cphf_tol": 1e-9, "cphf_lindep": 1e-24, "cphf_max_cycle": 100})
-        run_cphf(eq, f1, s1, "fp64")
-        mo1e, moe1e, wall_e, ste = run_cphf(eq, f1, s1, "fp64")
-        de_e = eq.get_cphf_hess(ta(f1), ta(s1), ta(mo1e), ta(moe1e))
-        res["fp64_matched"] = {"cphf_tol": 1e-9, "cphf_lindep": 1e-24, "wall_s": round(wall_e, 4),
-                               "cycles": ste.get("cycles"),
-                               "max_abs_mo1_err_vs_tight": (mo1e - mo1t).abs().max().item(),
-                               "max_abs_de_cphf_err_vs_tight": (de_e - de_t).abs().max().item(),
-                               "freq_shift_vs_tight_cm1": model_frequency_shift((de_e - de_t).cpu().numpy())}
-        del eq
-        ref = None
-        # one untimed mixed solve: the products of the Krylov block widths it meets are tuned per
-        # shape class on first use (the FP64 classes were tuned by the two solves above)
-        if args.precision != "fp64":
-            run_cphf(drv, f1, s1, args.precision)
-        for prec in ("fp64", args.precision):
-            mo1, moe1, wall, st = run_cphf(drv, f1, s1, prec)
-            de = drv.get_cphf_hess(ta(f1), ta(s1), ta(mo1), ta(moe1))
-            if ref is None:
-                ref = mo1
-            res[prec] = {"wall_s": round(wall, 4), "cycles": st.get("cycles"), "aop_calls": st.get("aop_calls"),
-                         "refinements": st.get("refinements"), "residual": st.get("residual"),
-                         "fp64_fallback": st.get("fp64_fallback"),
-                         "max_abs_mo1_err_vs_fp64": (mo1 - ref).abs().max().item(),
-                         "max_abs_mo1_err_vs_tight": (mo1 - mo1t).abs().max().item(),
-                         "max_abs_de_cphf_err_vs_tight": (de - de_t).abs().max().item(),
-                         "freq_shift_vs_tight_cm1": model_frequency_shift((de - de_t).cpu().numpy())}
-            log(f"[bench] cphf {prec}: {wall:.3f} s, cycles {st.get('cycles')}, aop calls {st.get('aop_calls')}")
-        res["speedup_vs_fp64"] = round(res["fp64"]["wall_s"] / res[args.precision]["wall_s"], 3)
-        res["speedup_vs_fp64_matched"] = round(res["fp64_matched"]["wall_s"] / res[args.precision]["wall_s"], 3)
-        if args.precision != "fp64":
-            # the mixed solve with the residual tolerance at 1e-9 (Hessian error well inside 1e-6 on
-            # this system, where 1e-8 allows a few 1e-6)
-            d9, _, _, _, _ = cphf_case(obj2, d_C, occ, nao, nocc, nprop, dev, {"cphf_tol": 1e-9})
-            run_cphf(d9, f1, s1, args.precision)
-            mo9, moe9, wall9, st9 = run_cphf(d9, f1, s1, args.precision)
-            de9 = d9.get_cphf_hess(ta(f1), ta(s1), ta(mo9), ta(moe9))
-            res[args.precision + "_tol1e-9"] = {
-                "wall_s": round(wall9, 4), "cycles": st9.get("cycles"), "refinements": st9.get("refinements"),
-                "residual": st9.get("residual"), "fp64_fallback": st9.get("fp64_fallback"),
-                "max_abs_mo1_err_vs_tight": (mo9 - mo1t).abs().max().item(),
-                "max_abs_de_cphf_err_vs_tight": (de9 - de_t).abs().max().item(),
-                "freq_shift_vs_tight_cm1": model_frequency_shift((de9 - de_t).cpu().numpy())}
-            del d9
-        line["cphf"] = res
+        line["cphf"] = cphf_legs(obj, d_cderi, d_C, occ, nao, nocc, nprop, dev)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(cderi, C, occ, bra, args.cpu_budget)
     print(json.dumps(line), flush=True)

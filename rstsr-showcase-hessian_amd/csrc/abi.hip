@@ -220,6 +220,15 @@ hh_status hh_rijk_set_batch(hh_rijk* h, int64_t nprop_batch) {
   HH_ABI_END
 }
 
+hh_status hh_rijk_set_response_path(hh_rijk* h, int32_t path, int64_t budget_bytes) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h != nullptr && (path == 0 || path == 1), "null handle or unknown response path");
+  h->impl->response_path = path;
+  if (budget_bytes > 0) h->impl->resident_budget = budget_bytes;
+  bump_response_state();
+  HH_ABI_END
+}
+
 hh_status hh_rijk_set_k_path(hh_rijk* h, int32_t path) {
   HH_ABI_BEGIN
   HH_REQUIRE(h != nullptr && (path == 0 || path == 2), "null handle or unknown K path");

@@ -556,16 +556,14 @@ void Cphf::solve_stacked(const double* rhs, int64_t nprop, double* mo1, Prec pre
   // residual shrinks, and the non-normalised Krylov basis does not mix columns whose norms
   // differ by orders of magnitude (that makes the projected system ill-conditioned).  Columns
   // already far below the tolerance are left out.  Inner solves go to about the
-  // reduced-precision operator's own accuracy; the outer loop stops when every column's residual
-  // 2-norm plus the bound on its error is below tol (the residual is FP64 up to the
-  // reduced-precision responses of small corrections, see below).
-  double* r = static_cast<double*>(res_.reserve(sizeof(double) * (4 * blk + 3 * nprop)));
+  // reduced-precision operator's own accuracy; after every sweep the residual is rebuilt from one
+  // FP64 response of the new solution, and the outer loop stops when every column's FP64 residual
+  // 2-norm is below tol.
+  double* r = static_cast<double*>(res_.reserve(sizeof(double) * (3 * blk + 2 * nprop)));
   double* ax = r + blk;
   double* dx = ax + blk;
-  double* rdx = dx + blk;
-  double* coln = rdx + blk;
+  double* coln = dx + blk;
   double* csc = coln + nprop;
-  double* cnd = csc + nprop;
   std::vector<double> hn(nprop);
   auto max_colnorm2 = [&](const double* cn) {
     HH_HIP(hipMemcpyAsync(hn.data(), cn, sizeof(double) * nprop, hipMemcpyDeviceToHost, stream_));
@@ -574,26 +572,20 @@ void Cphf::solve_stacked(const double* rhs, int64_t nprop, double* mo1, Prec pre
     for (double v : hn) m = std::max(m, v);
     return m;
   };
-  // Relative error of the reduced-precision response (measured: 6e-6 for BF16x3 against FP64 on
-  // the whole response, 1e-7 for FP32), taken 10-20x larger as the bound used below.
-  const double rel_err = (prec == PREC_BF16X3) ? 1e-4 : 2e-6;
   fill<double>(mo1, blk, 0.0, stream_);
   fill<double>(ax, blk, 0.0, stream_);
   hipLaunchKernelGGL(residual_norm_kernel, dim3((unsigned)nprop), dim3(256), 0, stream_, rhs, mo1, ax, r, n, coln);
   HH_HIP(hipGetLastError());
   double res = max_colnorm2(coln);
-  double bound = 0.0;  // bound on the error of the residual's 2-norm (any column)
   double* fin_r = static_cast<double*>(fin_r_.reserve(sizeof(double) * blk));
   const double op_tol = (prec == PREC_BF16X3) ? 1e-5 : 1e-7;  // inner target, relative to the rescaled rhs
-  // Stagnation fallback: a sweep that gains less than 10x (restarted inner solves that stop
-  // resolving the hard part of the spectrum; the residual rebuilt in FP64 agrees with the
-  // incremental one there, so it is not the reduced precision) ends the mixed solve, and the
+  // Stagnation fallback: a sweep that gains less than 10x ends the mixed solve, and the
   // reference's plain FP64 block Krylov solves the system from scratch.  A warm start from mo1
   // is no option: krylov_block restarts from the residual, whose trial vectors then sit at the
   // absolute lindep floor at once.
   const int max_sweeps = 10;
   double prev_res = -1.0;
-  for (int sweep = 0; sweep < max_sweeps && std::sqrt(res) + bound >= cfg_.tol && res > 0.0; ++sweep) {
+  for (int sweep = 0; sweep < max_sweeps && std::sqrt(res) >= cfg_.tol && res > 0.0; ++sweep) {
     if (prev_res > 0.0 && res > 0.01 * prev_res) {
       if (verbose) {
         fprintf(stderr, "cphf refinement stagnated at %.3e: plain FP64 solve\n", std::sqrt(res));
@@ -640,37 +632,10 @@ void Cphf::solve_stacked(const double* rhs, int64_t nprop, double* mo1, Prec pre
     HH_HIP(hipGetLastError());
     hipLaunchKernelGGL(axpy_kernel, dim3(nblk(blk)), dim3(256), 0, stream_, mo1, dx, 1.0, blk);
     HH_HIP(hipGetLastError());
-    // The raw response of mo1 (kept for finalize) is updated by the response of the correction.
-    // The first sweep's is FP64.  Later corrections are small: their response is taken in the
-    // reduced precision when rel_err times its largest dimensionless column norm stays below
-    // 5 % of tol (the residual test then allows for that bound), else in FP64.
-    // (a correction's dimensionless response is about the size of the residual it corrects: a
-    // reduced-precision attempt that would fail the test by 10x is not made)
-    if (sweep == 0) {
-      response_mo_stacked(mo1, nprop, fin_r, PREC_F64);
-      ++stats.krylov.aop_calls;
-    } else if (rel_err * std::sqrt(res) > 0.5 * cfg_.tol) {
-      response_mo_stacked(dx, nprop, rdx, PREC_F64);
-      ++stats.krylov.aop_calls;
-      hipLaunchKernelGGL(axpy_kernel, dim3(nblk(blk)), dim3(256), 0, stream_, fin_r, rdx, 1.0, blk);
-      HH_HIP(hipGetLastError());
-    } else {
-      response_mo_stacked(dx, nprop, rdx, prec);
-      ++stats.krylov.aop_calls;
-      copy_bytes(ax, rdx, (int64_t)sizeof(double) * blk, stream_);
-      dimless_scale(dx, nprop, ax);
-      hipLaunchKernelGGL(colnorm2_kernel, dim3((unsigned)nprop), dim3(256), 0, stream_, ax, n, cnd);
-      HH_HIP(hipGetLastError());
-      const double b = rel_err * std::sqrt(max_colnorm2(cnd));
-      if (b <= 0.05 * cfg_.tol) {
-        bound += b;
-      } else {
-        response_mo_stacked(dx, nprop, rdx, PREC_F64);
-        ++stats.krylov.aop_calls;
-      }
-      hipLaunchKernelGGL(axpy_kernel, dim3(nblk(blk)), dim3(256), 0, stream_, fin_r, rdx, 1.0, blk);
-      HH_HIP(hipGetLastError());
-    }
+    // The raw response of mo1 (kept for finalize) is recomputed in FP64 after every sweep, so the
+    // residual the outer loop tests is the exact FP64 residual of the returned solution.
+    response_mo_stacked(mo1, nprop, fin_r, PREC_F64);
+    ++stats.krylov.aop_calls;
     copy_bytes(ax, fin_r, (int64_t)sizeof(double) * blk, stream_);
     dimless_scale(mo1, nprop, ax);
     hipLaunchKernelGGL(residual_norm_kernel, dim3((unsigned)nprop), dim3(256), 0, stream_, rhs, mo1, ax, r, n, coln);
@@ -683,9 +648,9 @@ void Cphf::solve_stacked(const double* rhs, int64_t nprop, double* mo1, Prec pre
       fflush(stderr);
     }
   }
-  stats.residual = std::sqrt(res) + bound;
+  stats.residual = std::sqrt(res);
   if (st) *st = stats;
-  const bool ok = std::sqrt(res) + bound < cfg_.tol;
+  const bool ok = std::sqrt(res) < cfg_.tol;
   if (stats.refinements > 0 && ok) {
     copy_bytes(fin_x_.reserve(sizeof(double) * blk), mo1, (int64_t)sizeof(double) * blk, stream_);
     fin_nprop_ = nprop;
@@ -694,7 +659,7 @@ void Cphf::solve_stacked(const double* rhs, int64_t nprop, double* mo1, Prec pre
   if (!ok) {
     char buf[200];
     snprintf(buf, sizeof buf, "cphf mixed-precision refinement did not reach tol %.3e (residual %.3e after %d sweeps)",
-             cfg_.tol, std::sqrt(res) + bound, stats.refinements);
+             cfg_.tol, std::sqrt(res), stats.refinements);
     throw Error(HH_ERR_NOT_CONVERGED, buf);
   }
 }

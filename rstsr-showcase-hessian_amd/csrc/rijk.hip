@@ -226,6 +226,7 @@ void Rijk::prepare_response(const double* const* mo_coeff, const double* const* 
   mixed_ready_ = false;
   xt_ready_ = false;
   kf_ready_ = false;
+  res_ready_[0] = res_ready_[1] = false;
   bump_response_state();
 }
 
@@ -680,6 +681,14 @@ void Rijk::get_response_bra(const double* const* bra, int64_t nprop, double* con
   for (int s = 0; s < nspin_; ++s) HH_REQUIRE(bra[s] && resp[s], "bra[s] / resp[s] is null");
   const bool do_j = scale_j_ != 0.0, do_k = scale_k_ != 0.0;
   const int r0 = profiling_ ? mark() : -1;
+  if (response_path == 1 && resident_fits(prec)) {
+    apply_resident(bra[0], nprop, resp[0], prec, accumulate);
+    if (profiling_) {
+      span(T_RESPONSE, r0);
+      flush_timing();
+    }
+    return;
+  }
   if (do_k)
     for (int s = 0; s < nspin_; ++s) {
       const int a = profiling_ ? mark() : -1;

@@ -56,6 +56,13 @@ class Rijk {
   // BF16x3 K path: 0 = the fused contraction where supported (rijk_kfused.hpp), 2 = always the
   // materialised operand (T generation + NT contraction).
   int k_path = 0;
+  // Response algorithm: 0 = per call (the reference's algebra, default); 1 = resident response
+  // matrix (RHF, FP64 and BF16x3; rijk_resident.hip): M [nao nocc, nao nocc] built on the first
+  // call after prepare_response and kept, each call one product with M.  Falls back to the
+  // per-call algorithm when M would exceed resident_budget bytes or for UHF / FP32.
+  int response_path = 0;
+  int64_t resident_budget = 24LL << 30;
+  bool resident_fits(Prec prec) const;
 
   // Stage timing with HIP events on the handle's stream (the reference keeps wall-clock
   // `timing` vectors per component, hess_r.rs:1746).  Indices of the accumulated times (ms):
@@ -73,6 +80,8 @@ class Rijk {
   void ensure_mixed();
   void build_k_fused(int s, const double* bra, int64_t nprop, const float* W, double alpha, double beta,
                      double* out);
+  void build_resident(Prec prec);
+  void apply_resident(const double* bra, int64_t nprop, double* resp, Prec prec, bool accumulate);
   // Packed hot-GEMM operand Bp[u][P][mu_pad] (mu_pad = kpad_ = nao rounded up to 32, zeros
   // beyond nao): FP64 copy and/or bf16 hi/lo split, built once per handle.
   void ensure_packed(Prec prec);
@@ -126,6 +135,10 @@ class Rijk {
   // fused BF16x3 K (rijk_kfused.hpp): cderi in the contraction layout (per handle); O and X'
   // (hi then lo, per prepare_response and spin); per call the packed bra, W' and the partials
   DevBuf kb_hi_, kb_lo_, ko_[2], kx_[2], kr_, kw_, kf_ws_;
+  // resident response matrix (per prepare_response): BF16x3 tiled hi / lo, FP64 row-major; the
+  // per-call split / padded bra
+  DevBuf res_hi_, res_lo_, res64_, res_b_;
+  bool res_ready_[2] = {false, false};  // FP64, BF16x3
   bool j_bf16_ready_ = false;
 };
 

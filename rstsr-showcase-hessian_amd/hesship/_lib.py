@@ -44,6 +44,7 @@ SIGNATURES = {
     "hh_rijk_set_batch": (_i32, [_p, _i64]),
     "hh_rijk_set_profiling": (_i32, [_p, _i32]),
     "hh_rijk_set_k_path": (_i32, [_p, _i32]),
+    "hh_rijk_set_response_path": (_i32, [_p, _i32, _i64]),
     "hh_rijk_get_timing": (_i32, [_p, C.POINTER(C.c_double)]),
     "hh_rijk_reset_timing": (_i32, [_p]),
     "hh_util_gemm": (_i32, [_i64, _i64, _i64, _d, _p, _i64, _i32, _p, _i64, _i32, _d, _p, _i64, _i64, _i64,

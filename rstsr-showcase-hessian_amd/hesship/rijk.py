@@ -95,6 +95,12 @@ class _RIJKBase:
         supported) or "materialized" (T generation + NT contraction)."""
         check(lib().hh_rijk_set_k_path(self._h, {"fused": 0, "materialized": 2}[path]))
 
+    def set_response_path(self, path, budget_bytes=0):
+        """Response algorithm: "direct" (per call, the reference's algebra; default) or "resident"
+        (RHF, FP64 / BF16x3: the response matrix built once per prepare_response, one product
+        per call; falls back to direct beyond budget_bytes)."""
+        check(lib().hh_rijk_set_response_path(self._h, {"direct": 0, "resident": 1}[path], int(budget_bytes)))
+
     def set_profiling(self, on=True):
         check(lib().hh_rijk_set_profiling(self._h, int(bool(on))))
 

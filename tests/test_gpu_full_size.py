@@ -66,3 +66,19 @@ def test_rks_response_c3_shape(dev, precision):
     err = np.abs(got.cpu().numpy() - ref).max() / np.abs(ref).max()
     print(f"RKS response ng {ng} nao {nao} nocc {nocc} nprop {nprop} ({precision}): rel err {err:.2e}")
     assert err < TOL[precision]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+def test_rijk_resident_response_full_size(dev, c2, precision):
+    """The resident response matrix (16000 x 16000 at these sizes) built and applied at full size."""
+    import hesship
+    cderi, C, occ, bra, ref = c2
+    d_cderi = torch.from_numpy(np.asfortranarray(cderi).T.copy()).to(dev).T
+    obj = hesship.RHessRIJK(d_cderi, 1.0, 1.0)
+    obj.set_response_path("resident")
+    obj.make_response_preparation(torch.from_numpy(C).to(dev), occ)
+    d_bra = hesship.fortran(torch.from_numpy(bra).to(dev))
+    got = obj.get_response_bra(d_bra, precision=precision)[:, :, COLS].cpu().numpy()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"resident RI-JK response, full size ({precision}): rel err of columns {COLS} {err:.2e}")
+    assert err < TOL[precision]
