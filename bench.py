@@ -38,7 +38,14 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}  # MI355X dense ma
 # profiles/nt_bf16x3_kernel_ri_k_nao400_naux2400_nocc40_nprop90_hbm_traffic.txt.  Keyed by
 # (precision, nao, naux, nocc, nprop, BF16X3 tile configuration); other workloads report null.
 MEASURED_TRAFFIC = {("bf16x3", 400, 2400, 40, 90, 0): 13881227.0 * 1024 * 1.99994 + 91504.8 * 1024 * 1.00000,
-                    ("bf16x3", 400, 2400, 40, 90, 2): 7991991.2 * 1024 * 1.99994 + 91596.1 * 1024 * 1.00000}
+                    ("bf16x3", 400, 2400, 40, 90, 2): 7991991.2 * 1024 * 1.99994 + 91596.1 * 1024 * 1.00000,
+                    # fused term-1 kernel: profiles/kfused_kernel_ri_k_nao400_naux2400_nocc40_nprop90_counters.txt
+                    ("bf16x3", 400, 2400, 40, 90, "fused"): 1720391.2 * 1024 * 1.99994 + 152100.0 * 1024 * 1.00000}
+# rocprofv3 --pmc counters of the same launches (same profile): SQ_VALU_MFMA_BUSY_CYCLES over
+# GRBM_GUI_ACTIVE x 1024 SIMDs, SQ_LDS_IDX_ACTIVE over GRBM_GUI_ACTIVE x 256 CUs, SQ_LDS_BANK_CONFLICT
+MEASURED_COUNTERS = {("bf16x3", 400, 2400, 40, 90, "fused"): {"mfma_busy": 0.559, "lds_busy": 0.186,
+                                                            "lds_bank_conflict_cycles": 0,
+                                                            "engine_clock_ghz": 2.03}}
 HOT_KERNEL = {"fp64": "nt_f64_kernel<1,8,13,1>", "fp32": "gemm_kernel<1,float,double>",
               "bf16x3": "nt_bf16x3_kernel<1,8,13,2,true,13,2>"}
 # BF16X3 tile configurations of the NT kernel (the autotuner's candidates), by index
@@ -860,6 +867,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4),
                      "traffic": MEASURED_TRAFFIC.get((args.precision, nao, naux, nocc, nprop, hot_cfg)),
+                     "counters": MEASURED_COUNTERS.get((args.precision, nao, naux, nocc, nprop, hot_cfg)),
                      "kernel": hot_name + (" (fused K term 1: operand generation + contraction, all perturbations in one launch)"
                                            if fused else " (hot K contraction, one perturbation chunk per launch)"),
                      "launches_per_step": nl, "avg_launch_ms": round(t_launch * 1e3, 4)},
