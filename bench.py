@@ -160,6 +160,29 @@ def cphf_case(obj, d_C, occ, nao, nocc, nprop, dev, config=None):
     return drv, e, torch.from_numpy(f1).to(dev), torch.from_numpy(s1).to(dev), ratio
 
 
+def response_spectrum(drv, nao, nocc, dev):
+    """Lowest and highest eigenvalue of the dimensionless response operator A on the vir-occ block
+    (I + A is the CPHF matrix; the occupied rows of the response are zero): ARPACK (scipy eigs) with
+    one FP64 response per matvec.  A = D^-1 K with D the orbital-energy gaps and K symmetric, so
+    the spectrum is real."""
+    import numpy as np
+    import torch
+    from scipy.sparse.linalg import LinearOperator, eigs
+    nv = nao - nocc
+
+    def mv(v):
+        x = np.zeros((nao, nocc, 1))
+        x[nocc:, :, 0] = np.real(v).reshape(nv, nocc)
+        y = drv.response_dimless_cphf(torch.from_numpy(x).to(dev))
+        return y[nocc:, :, 0].cpu().numpy().ravel()
+
+    op = LinearOperator((nv * nocc, nv * nocc), matvec=mv, dtype=np.float64)
+    v0 = np.random.default_rng(7).standard_normal(nv * nocc)
+    lo = eigs(op, k=2, which="SR", return_eigenvectors=False, tol=1e-6, ncv=24, v0=v0)
+    hi = eigs(op, k=2, which="LR", return_eigenvectors=False, tol=1e-6, ncv=24, v0=v0)
+    return float(np.real(lo).min()), float(np.real(hi).max())
+
+
 def resident_leg(d_cderi, d_C, occ, d_bra, r64, nao, nocc, nprop, steps):
     """The resident RHF response (rijk_resident.hip): M [nao nocc, nao nocc] built from the same
     tensors (timed: the first call after make_response_preparation), then `steps` calls, each one
@@ -253,10 +276,13 @@ def run_cphf(drv, f1, s1, prec, prepare=None):
     return mo1f, moe1, time.perf_counter() - t0, st
 
 
+LOWEST_A = 0.75  # the synthetic CPHF matrix I + A has smallest eigenvalue 0.25
+
+
 def cphf_legs(obj, d_cderi, d_C, occ, nao, nocc, nprop, dev):
     """The wall-time half of the metric on a synthetic CPHF system built on the benchmark tensors
-    (RI-JK scaled so the dimensionless response is ~0.3 of its input; the raw N(0,1) cderi would
-    make I + A far from a physical CPHF).  Every leg's wall time covers prepare_response of the
+    (RI-JK scaled so the dimensionless response is ~0.3 of its input, or less where that would
+    leave I + A indefinite; the raw N(0,1) cderi would make I + A far from a physical CPHF).  Every leg's wall time covers prepare_response of the
     RI-JK component (X, O; for the resident operator also its response matrix), the right-hand
     side, the Krylov solve and finalize.  Legs: the plain FP64 path (the reference's per-call
     algorithm and stopping rule), FP64 asked for the mixed solves' accuracy, FP64 with the
@@ -266,14 +292,33 @@ def cphf_legs(obj, d_cderi, d_C, occ, nao, nocc, nprop, dev):
     import hesship
     probe, _, _, _, ratio = cphf_case(obj, d_C, occ, nao, nocc, nprop, dev)
     del probe
-    scale = (0.3 / ratio) ** 0.5
-    cd = d_cderi * scale
+    # A is quadratic in cderi: scale2 multiplies its spectrum.  First |A v| ~ 0.3 |v| for random v,
+    # then, if A has eigenvalues below -LOWEST_A, shrink it so that I + A stays positive definite
+    # with smallest eigenvalue 1 - LOWEST_A, as for a stable closed-shell ground state.
+    scale2 = 0.3 / ratio
+    pre = hesship.RHessRIJK(d_cderi * scale2 ** 0.5, 1.0, 1.0)
+    pre.set_response_path("resident")
+    pre.make_response_preparation(d_C, occ)
+    drv0 = cphf_case(pre, d_C, occ, nao, nocc, nprop, dev)[0]
+    lo0, hi0 = response_spectrum(drv0, nao, nocc, dev)
+    del drv0, pre
+    if lo0 < -LOWEST_A:
+        scale2 *= LOWEST_A / -lo0
+    cd = d_cderi * scale2 ** 0.5
     direct = hesship.RHessRIJK(cd, 1.0, 1.0)
     direct.make_response_preparation(d_C, occ)
     resident = hesship.RHessRIJK(cd, 1.0, 1.0)
     resident.set_response_path("resident")
     resident.make_response_preparation(d_C, occ)
     res = {"natm_equiv": nprop // 3}
+    drv1 = cphf_case(resident, d_C, occ, nao, nocc, nprop, dev)[0]
+    lo1, hi1 = response_spectrum(drv1, nao, nocc, dev)
+    del drv1
+    res["spectrum_A"] = {"min": round(lo1, 4), "max": round(hi1, 4),
+                         "at_ratio_0.3": [round(lo0, 4), round(hi0, 4)],
+                         "note": "eigenvalues of the dimensionless response A (I + A is solved); the cderi "
+                                 "scale puts min(A) at -%.2f when |A v| ~ 0.3 |v| would make I + A "
+                                 "indefinite" % LOWEST_A}
     ta = lambda x: x.reshape(*x.shape[:-1], nprop // 3, 3).transpose(-2, -1)  # p = t + 3 A
     tight, e, f1, s1, ratio2 = cphf_case(direct, d_C, occ, nao, nocc, nprop, dev,
                                          {"cphf_tol": 1e-11, "cphf_lindep": 1e-24, "cphf_max_cycle": 100})
@@ -809,6 +854,14 @@ def main():
     fk, fj = flops_k(nao, naux, nocc, nprop), flops_j(nao, naux, nocc, nprop)
     log(f"[bench] F_K = {fk / 1e12:.4f} TFLOP, F_J = {fj / 1e12:.4f} TFLOP per step")
     r64, ms64, wall64, tm64 = run("fp64", args.fp64_steps, 1)
+    fp32_leg = None
+    if args.precision == "bf16x3":
+        r32, ms32, _, tm32 = run("fp32", args.fp64_steps, 1)
+        fp32_leg = {"ms_per_step": round(ms32, 3), "ri_k_ms": round(tm32["k_total_ms"], 3),
+                    "ri_k_tflops": round(flops_k(nao, naux, nocc, nprop) / (tm32["k_total_ms"] * 1e-3) / 1e12, 3),
+                    "rel_err_vs_fp64": (r32 - r64).abs().max().item() / r64.abs().max().item()}
+        del r32
+        log(f"[bench] fp32: {ms32:.2f} ms/step, K {tm32['k_total_ms']:.2f} ms")
     log(f"[bench] fp64: {ms64:.2f} ms/step, K {tm64['k_total_ms']:.2f} ms, J {tm64['j_total_ms']:.2f} ms")
     if args.precision == "fp64":
         rp, ms, wall, tm = run("fp64", args.steps, args.warmup)
@@ -861,6 +914,7 @@ def main():
         "ri_j_ms": round(tm["j_total_ms"], 3),
         "fp64": {"ms_per_step": round(ms64, 3), "ri_k_ms": round(tm64["k_total_ms"], 3),
                  "ri_k_tflops": round(fk / (tm64["k_total_ms"] * 1e-3) / 1e12, 3)},
+        "fp32": fp32_leg,
         "speedup_vs_fp64": round(ms64 / ms, 3),
         "max_abs_err_vs_fp64": diff,
         "rel_err_vs_fp64": diff / scale,
@@ -890,6 +944,17 @@ def main():
         line["cphf"] = cphf_legs(obj, d_cderi, d_C, occ, nao, nocc, nprop, dev)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(cderi, C, occ, bra, args.cpu_budget)
+        if "cphf" in line:
+            # the plain FP64 solve's operator applications at the oracle's measured RI-K rate
+            # (RI-J and the Krylov algebra left out: a lower bound on the host's wall time)
+            calls = line["cphf"]["fp64"]["aop_calls"] or 0
+            per_call = flops_k(nao, naux, nocc, nprop) / (line["cpu_baseline"]["value"] * 1e12)
+            line["cphf"]["cpu_oracle_estimate"] = {
+                "wall_s": round(calls * per_call, 1), "aop_calls": calls, "s_per_call": round(per_call, 2),
+                "cores": line["cpu_baseline"]["cores"],
+                "basis": "RI-K flops per response call / cpu_baseline TFLOP/s x the FP64 leg's response calls",
+                "speedup_fp64_resident": round(calls * per_call / line["cphf"]["fp64_resident"]["wall_s"], 1),
+                "speedup_mixed_resident": round(calls * per_call / line["cphf"]["mixed_resident"]["wall_s"], 1)}
     print(json.dumps(line), flush=True)
 
 

@@ -43,6 +43,9 @@ struct NtParams {
   float* Cf;  // non-null: FP32 output (C unused)
   int64_t ldc;
   double alpha, beta;
+  unsigned short* Wh;  // non-null: RI-K term-2 operand output (NtArgs::W_hi)
+  unsigned short* Wl;
+  int w_nocc, w_naux, w_naux_pad;
 };
 
 __device__ __forceinline__ int64_t remap_xcd(int64_t w, int64_t nwg) {
@@ -234,6 +237,32 @@ void nt_bf16x3_kernel(NtParams p) {
   // epilogue: D[4*(l/16) + r][l%16]
   const int64_t col0 = n0 + wn * FN * 16 + (lane & 15);
   const int64_t row0 = m0 + wm * FM * 16 + 4 * (lane >> 4);
+  if (p.Wh) {
+    // the term-2 operand: 16 lanes of a row hold 16 consecutive P, one 32-byte run per array
+    const int nocc = p.w_nocc, naux = p.w_naux, nkb = p.w_naux_pad / 32;
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int64_t gn = col0 + j * 16;
+      if (gn >= p.N) continue;
+      const int P = (int)(gn % naux), ii = (int)(gn / naux);
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        if (wm * FM + i >= BMT) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t gm = row0 + i * 16 + r;
+          if (gm >= p.M) continue;
+          const int jo = (int)(gm % nocc), a = (int)(gm / nocc);
+          const int64_t o = (((int64_t)(P / 32) + (int64_t)nkb * jo) * p.M + ii + (int64_t)nocc * a) * 32 + P % 32;
+          const float x = (float)(p.alpha * (double)acc[i][j][r]);
+          const __bf16 h = (__bf16)x;
+          p.Wh[o] = __builtin_bit_cast(u16, h);
+          p.Wl[o] = __builtin_bit_cast(u16, (__bf16)(x - (float)h));
+        }
+      }
+    }
+    return;
+  }
   if (p.Cf && p.ksplit == 1) {
     // FP32 output: the lane's 4 consecutive rows are one 16-byte store where they fit
     const bool vec = (p.ldc % 4) == 0 && p.beta == 0.0;
@@ -535,6 +564,8 @@ void launch_nt(const NtArgs& a, hipStream_t s, DevBuf& ws) {
   p.A = a.A; p.Alo = a.A_lo; p.lda = a.lda;
   p.B = a.B; p.Blo = a.B_lo; p.ldb = a.ldb;
   p.C = a.C; p.Cf = a.C_f32; p.ldc = a.ldc; p.alpha = a.alpha; p.beta = a.beta;
+  p.Wh = a.W_hi; p.Wl = a.W_lo;
+  p.w_nocc = (int)a.w_nocc; p.w_naux = (int)a.w_naux; p.w_naux_pad = (int)a.w_naux_pad;
   p.tiles_m = ceil_div(a.M, BM);
   p.tiles_n = ceil_div(a.N, BN);
   p.arows = a.a_rows > 0 ? a.a_rows : a.M;
@@ -562,6 +593,7 @@ void launch_nt(const NtArgs& a, hipStream_t s, DevBuf& ws) {
     while (ks > 1 && a.K / ((int64_t)BK * ks) < 16) --ks;
     while (ks > 1 && (tiles * ks) % 8 != 0) --ks;
   }
+  if (a.W_hi) ks = 1;  // the operand output is written by the epilogue of the only K slice
   int64_t kchunk = ceil_div(ceil_div(a.K, ks), BK) * BK;
   ks = (int)ceil_div(std::max<int64_t>(a.K, 1), kchunk);
   p.kchunk = kchunk;
@@ -656,7 +688,9 @@ NtChoice tuned(const NtArgs& a, NtFormat fmt, hipStream_t s, DevBuf& ws) {
   HH_HIP(hipStreamIsCapturing(s, &cap));
   if (!g_autotune || cap != hipStreamCaptureStatusNone || a.K == 0) return best;
   static DevBuf* scratch = new DevBuf;
-  void* C = scratch->reserve(sizeof(double) * a.M * a.N);
+  // the output: M N doubles, or the two bf16 arrays of the term-2 operand layout
+  const int64_t w_elems = a.W_hi ? a.w_naux_pad * (a.N / a.w_naux) * a.M : 0;
+  void* C = scratch->reserve(std::max(sizeof(double) * a.M * a.N, 2 * sizeof(unsigned short) * w_elems));
   hipEvent_t e0, e1;
   HH_HIP(hipEventCreate(&e0));
   HH_HIP(hipEventCreate(&e1));
@@ -672,7 +706,10 @@ NtChoice tuned(const NtArgs& a, NtFormat fmt, hipStream_t s, DevBuf& ws) {
       t.K = std::min<int64_t>(a.K, 65536);
       t.alpha = 1.0; t.beta = 0.0; t.ksplit = ks;
       t.ldc = a.M;
-      if (a.C_f32) {
+      if (a.W_hi) {
+        t.W_hi = static_cast<unsigned short*>(C);
+        t.W_lo = t.W_hi + w_elems;
+      } else if (a.C_f32) {
         t.C = nullptr; t.C_f32 = static_cast<float*>(C);
       } else {
         t.C = static_cast<double*>(C); t.C_f32 = nullptr;
@@ -752,8 +789,14 @@ void gemm_nt_f64_config(const NtArgs& a, int cfg, hipStream_t s, DevBuf& ws) {
 void gemm_nt(const NtArgs& a, NtFormat fmt, hipStream_t s, DevBuf& ws) {
   HH_REQUIRE(a.M >= 0 && a.N >= 0 && a.K >= 0, "gemm_nt: negative size");
   if (a.M == 0 || a.N == 0) return;
-  HH_REQUIRE((a.C || a.C_f32) && (a.K == 0 || (a.A && a.B)), "gemm_nt: null operand");
+  HH_REQUIRE((a.C || a.C_f32 || a.W_hi) && (a.K == 0 || (a.A && a.B)), "gemm_nt: null operand");
   HH_REQUIRE(!a.C_f32 || fmt == NT_BF16X3, "gemm_nt: FP32 output is BF16X3 only");
+  if (a.W_hi) {
+    HH_REQUIRE(fmt == NT_BF16X3 && a.W_lo && a.w_nocc > 0 && a.w_naux > 0 && a.M % a.w_nocc == 0 &&
+                   a.N % a.w_naux == 0 && a.w_naux_pad % 32 == 0 && a.w_naux_pad >= a.w_naux &&
+                   a.M < (1LL << 31) && a.N < (1LL << 31),
+               "gemm_nt: bad term-2 operand output");
+  }
   HH_REQUIRE(a.K % 32 == 0 && a.lda % 8 == 0 && a.ldb % 8 == 0, "gemm_nt: K must be a multiple of 32 and lda, ldb "
                                                                   "multiples of 8");
   if (fmt == NT_BF16X3) {

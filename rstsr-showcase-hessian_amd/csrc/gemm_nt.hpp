@@ -35,6 +35,12 @@ struct NtArgs {
   // (0 = M for A, N for B); the operand pointer then addresses its first row in K block 0.
   int64_t a_rows = 0, b_rows = 0;
   int ksplit = 0;  // 0: automatic; < 0: balance the last wave of workgroups over the CUs
+  // BF16X3, instead of C: the bf16 split of alpha C[m, n] with m = j + w_nocc a, n = P + w_naux i
+  // written as the RI-K term-2 operand hi / lo[(kk / 32) M + i + w_nocc a][32], kk = P + w_naux_pad j
+  // (K split 1; entries with w_naux <= P < w_naux_pad are not written)
+  unsigned short* W_hi = nullptr;
+  unsigned short* W_lo = nullptr;
+  int64_t w_nocc = 0, w_naux = 0, w_naux_pad = 0;
   hipEvent_t ev_begin = nullptr, ev_main_end = nullptr;
 };
 

@@ -463,7 +463,7 @@ void Rijk::build_k(int s, const double* bra, int64_t nprop, Prec prec, double al
   const void* Co = f64 ? mocc_[s].ptr : mocc32_[s].ptr;
   // W[(k + nocc*a) + nocc*nprop*(P + naux*i)] = sum_mu bra[mu,(k,a)] X[mu,(P,i)]
   const int64_t ldw = nocc * nprop;
-  void* W = w_.reserve(esz * ldw * nP);
+  void* W = nullptr;
   if (bf) {
     // NT GEMM on K-block tiled bf16 pairs (K = mu padded to kpad), FP32 output
     if (!xt_ready_) {
@@ -484,13 +484,22 @@ void Rijk::build_k(int s, const double* bra, int64_t nprop, Prec prec, double al
     a.A = bth; a.A_lo = btl; a.B = xt_hi_[s].ptr; a.B_lo = xt_lo_[s].ptr;
     a.lda = a.ldb = kpad_;
     a.tiled = true;
-    a.C_f32 = static_cast<float*>(W); a.ldc = ldw;
-    gemm_nt(a, NT_BF16X3, stream_, ws_);
     if (k_path != 2 && kfused_supported(nao_, nocc)) {
-      build_k_fused(s, bra, nprop, static_cast<const float*>(W), alpha, beta, out);
+      // fused path: the product's epilogue writes W directly as the term-2 operand W'
+      const int64_t np = kfused_naux_pad(naux_), wn = np * nocc * ldw;
+      unsigned short* wp = static_cast<unsigned short*>(kw_.reserve(2 * sizeof(unsigned short) * wn));
+      kfused_zero_w_pad(naux_, nocc, nprop, wp, wp + wn, stream_);
+      a.W_hi = wp; a.W_lo = wp + wn;
+      a.w_nocc = nocc; a.w_naux = naux_; a.w_naux_pad = np;
+      gemm_nt(a, NT_BF16X3, stream_, ws_);
+      build_k_fused(s, bra, nprop, wp, alpha, beta, out);
       return;
     }
+    W = w_.reserve(esz * ldw * nP);
+    a.C_f32 = static_cast<float*>(W); a.ldc = ldw;
+    gemm_nt(a, NT_BF16X3, stream_, ws_);
   } else {
+    W = w_.reserve(esz * ldw * nP);
     GemmArgs g;
     g.M = ldw; g.N = nP; g.K = nao_;
     g.A = braX; g.lda = nao_; g.a_kmajor = true;
@@ -592,7 +601,7 @@ void Rijk::build_k(int s, const double* bra, int64_t nprop, Prec prec, double al
 // BF16x3 K with the fused first term (rijk_kfused.hpp): term 1 = sum_P B_P bra_a O_P with its
 // operand generated inside the contraction, term 2 = sum_P X_P W_aP as one NT product over
 // K = (P, j) with W permuted to rows (i, a).
-void Rijk::build_k_fused(int s, const double* bra, int64_t nprop, const float* W, double alpha, double beta,
+void Rijk::build_k_fused(int s, const double* bra, int64_t nprop, const unsigned short* wp, double alpha, double beta,
                          double* out) {
   const int64_t nocc = nocc_[s];
   const int64_t nb = kfused_nb(nao_), jp = kfused_jp(nocc), np = kfused_naux_pad(naux_);
@@ -621,8 +630,6 @@ void Rijk::build_k_fused(int s, const double* bra, int64_t nprop, const float* W
   unsigned short* r = static_cast<unsigned short*>(kr_.reserve(2 * sizeof(unsigned short) * rn));
   kfused_pack_bra(bra, nao_, nocc, nprop, r, r + rn, stream_);
   const int64_t wn = np * nocc * nocc * nprop;
-  unsigned short* wp = static_cast<unsigned short*>(kw_.reserve(2 * sizeof(unsigned short) * wn));
-  kfused_permute_w(W, naux_, nocc, nprop, wp, wp + wn, stream_);
   KfusedArgs k;
   const int64_t on = kfused_o_len(nocc, naux_);
   k.bh = kb_hi_.as<unsigned short>(); k.bl = kb_lo_.as<unsigned short>();

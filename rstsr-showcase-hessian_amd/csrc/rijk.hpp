@@ -78,7 +78,7 @@ class Rijk {
   // K: out_s[nao, nocc_s, nprop] = alpha * (K_s without its minus sign) + beta * out_s
   void build_k(int s, const double* bra, int64_t nprop, Prec prec, double alpha, double beta, double* out);
   void ensure_mixed();
-  void build_k_fused(int s, const double* bra, int64_t nprop, const float* W, double alpha, double beta,
+  void build_k_fused(int s, const double* bra, int64_t nprop, const unsigned short* wp, double alpha, double beta,
                      double* out);
   void build_resident(Prec prec);
   void apply_resident(const double* bra, int64_t nprop, double* resp, Prec prec, bool accumulate);

@@ -107,29 +107,18 @@ __global__ void pack_x_kernel(const float* __restrict__ X, int64_t nao, int64_t 
   }
 }
 
-// grid (m chunks of 256, P blocks of 32, i): a thread owns m = j + nocc a and writes the 32 P
-// values of its (i, a) row and K block as 64 contiguous bytes per array
-__global__ __launch_bounds__(256) void permute_w_kernel(const float* __restrict__ W, int64_t naux, int64_t nocc,
-                                                        int64_t nprop, int64_t naux_pad, u16* __restrict__ hi,
-                                                        u16* __restrict__ lo) {
-  const int64_t ldw = nocc * nprop, R = ldw;
-  const int64_t m = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int64_t P0 = (int64_t)blockIdx.y * 32, i = blockIdx.z;
-  if (m >= ldw) return;
-  const int64_t j = m % nocc, a = m / nocc;
-  u16 h[32], l[32];
-#pragma unroll
-  for (int q = 0; q < 32; ++q) {
-    const int64_t P = P0 + q;
-    const float x = P < naux ? W[m + ldw * (P + naux * i)] : 0.f;
-    split1(x, h[q], l[q]);
-  }
-  const int64_t kb = (P0 + naux_pad * j) / 32;
-  const int64_t o = (kb * R + i + nocc * a) * 32;
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    *reinterpret_cast<uint4*>(hi + o + 8 * q) = *reinterpret_cast<const uint4*>(h + 8 * q);
-    *reinterpret_cast<uint4*>(lo + o + 8 * q) = *reinterpret_cast<const uint4*>(l + 8 * q);
+// zero the entries naux <= P < naux_pad of the term-2 operand W' (the W product writes P < naux):
+// the last K block of each j, rows i + nocc a
+__global__ void zero_w_pad_kernel(int64_t naux, int64_t nocc, int64_t R, int64_t naux_pad, u16* __restrict__ hi,
+                                  u16* __restrict__ lo) {
+  const int64_t p0 = naux % 32, npad = 32 - p0;
+  const int64_t total = nocc * R * npad;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t q = e % npad, r = (e / npad) % R, j = e / (npad * R);
+    const int64_t kb = (naux_pad * j + naux - p0) / 32;
+    const int64_t o = (kb * R + r) * 32 + p0 + q;
+    hi[o] = 0;
+    lo[o] = 0;
   }
 }
 
@@ -451,13 +440,12 @@ void kfused_pack_x(const float* X, int64_t nao, int64_t naux, int64_t nocc, unsi
   HH_HIP(hipGetLastError());
 }
 
-void kfused_permute_w(const float* W, int64_t naux, int64_t nocc, int64_t nprop, unsigned short* hi,
-                      unsigned short* lo, hipStream_t s) {
-  const int64_t np = kfused_naux_pad(naux);
-  const int64_t ldw = nocc * nprop;
-  HH_REQUIRE(nocc < 65536 && np / 32 < 65536 && ceil_div(ldw, 256) < (1LL << 31), "kfused: W permute grid");
-  hipLaunchKernelGGL(permute_w_kernel, dim3((unsigned)ceil_div(ldw, 256), (unsigned)(np / 32), (unsigned)nocc),
-                     dim3(256), 0, s, W, naux, nocc, nprop, np, hi, lo);
+void kfused_zero_w_pad(int64_t naux, int64_t nocc, int64_t nprop, unsigned short* hi, unsigned short* lo,
+                       hipStream_t s) {
+  if (naux % 32 == 0) return;
+  const int64_t R = nocc * nprop, total = nocc * R * (32 - naux % 32);
+  const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(total, 256), 65536));
+  hipLaunchKernelGGL(zero_w_pad_kernel, dim3(blocks), dim3(256), 0, s, naux, nocc, R, kfused_naux_pad(naux), hi, lo);
   HH_HIP(hipGetLastError());
 }
 

@@ -9,8 +9,8 @@
 // and O_P (16-byte loads), splits it into bf16 hi / lo in registers and uses it directly as the B
 // operand of the contraction with the 32-row blocks of B_P staged in LDS.
 //
-// Term 2 operands: X' (X as the K-block tiled bf16 pair with K = P + naux_pad j) and W' (W
-// permuted to rows i + nocc a, the same K) for one NT product.
+// Term 2 operands: X' (X as the K-block tiled bf16 pair with K = P + naux_pad j) and W' (W with
+// rows i + nocc a, the same K) for one NT product.
 #pragma once
 #include "hh_common.hpp"
 
@@ -44,10 +44,11 @@ void kfused_pack_bra(const double* bra, int64_t nao, int64_t nocc, int64_t nprop
 // X[mu + nao (P + naux j)] (FP32) -> hi / lo [(kk / 32) nao + mu][32], kk = P + naux_pad j
 void kfused_pack_x(const float* X, int64_t nao, int64_t naux, int64_t nocc, unsigned short* hi, unsigned short* lo,
                    hipStream_t s);
-// W[(j + nocc a) + nocc nprop (P + naux i)] (FP32) -> hi / lo [(kk / 32) R + i + nocc a][32],
-// R = nocc nprop, kk = P + naux_pad j
-void kfused_permute_w(const float* W, int64_t naux, int64_t nocc, int64_t nprop, unsigned short* hi,
-                      unsigned short* lo, hipStream_t s);
+// The term-2 operand W' = hi / lo [(kk / 32) R + i + nocc a][32], R = nocc nprop, kk = P + naux_pad j,
+// holds W_aP[j, i] = sum_mu bra_a[mu, j] X_P[mu, i]; the W product writes it from its epilogue
+// (NtArgs::W_hi) for P < naux, and this zeroes the padding naux <= P < naux_pad.
+void kfused_zero_w_pad(int64_t naux, int64_t nocc, int64_t nprop, unsigned short* hi, unsigned short* lo,
+                       hipStream_t s);
 
 struct KfusedArgs {
   const unsigned short *bh = nullptr, *bl = nullptr;  // kfused_pack_b

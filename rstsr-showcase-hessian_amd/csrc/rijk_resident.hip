@@ -13,8 +13,10 @@
 // falls from O(nao^2 naux nocc nprop) to O(nao^2 nocc^2 nprop); the memory is (nao nocc)^2
 // elements (c.f. 288 GB of HBM).
 //
-// Symmetries used by the assembly kernel (coalesced reads of all three terms): G is symmetric,
-// and V'[(mu,nu),(j,i)] = V'[(nu,mu),(i,j)] (B_P and O_P are symmetric).
+// Build: G is one NT product of X arranged as Xr[(mu,i)][P] with itself (M = N = nao nocc,
+// K = naux); V needs only the packed triangles of its two symmetric index pairs,
+// Vtp[(mu<=nu), (i<=j)] = sum_P B_P[mu,nu] O_P[i,j], a quarter of the full product's flops, and is
+// unpacked over (mu, nu) before the assembly, which reads all three terms coalesced (kk fastest).
 #include "elementwise.hpp"
 #include "gemm_nt.hpp"
 #include "rijk.hpp"
@@ -25,19 +27,49 @@ namespace {
 
 unsigned rblocks(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 65536)); }
 
-// O2[(j + nocc i) + nocc^2 P] = O[j + nocc (P + naux i)]
-template <typename T>
-__global__ void permute_o_kernel(const double* __restrict__ O, int64_t nocc, int64_t naux, T* __restrict__ O2) {
-  const int64_t total = nocc * nocc * naux;
+// packed index of the pair (i, j) of a symmetric index pair: hi (hi + 1) / 2 + lo
+__device__ __forceinline__ int64_t tp_index(int64_t i, int64_t j) {
+  const int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+  return hi * (hi + 1) / 2 + lo;
+}
+
+// O2t[tp(i,j) + ntp P] = O[j + nocc (P + naux i)] for i <= j (O_P is symmetric), ntp = nocc (nocc + 1) / 2
+__global__ void pack_o_tp_kernel(const double* __restrict__ O, int64_t nocc, int64_t naux, double* __restrict__ O2t) {
+  const int64_t ntp = nocc * (nocc + 1) / 2;
+  const int64_t total = ntp * naux;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t j = e % nocc, i = (e / nocc) % nocc, P = e / (nocc * nocc);
-    O2[e] = (T)O[j + nocc * (P + naux * i)];
+    const int64_t t = e % ntp, P = e / ntp;
+    int64_t j = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while (j * (j + 1) / 2 > t) --j;
+    while ((j + 1) * (j + 2) / 2 <= t) ++j;
+    const int64_t i = t - j * (j + 1) / 2;
+    O2t[e] = O[j + nocc * (P + naux * i)];
   }
 }
 
-// M[(mu,i),(nu,j)] = cg G[r + R kk] - cv (V'[(nu + nao mu) + nao^2 (i + nocc j)] + G[(nu + nao i) + R (mu + nao j)])
-// with r = mu + nao i, kk = nu + nao j.  Output BF16: tiled hi / lo [(kk / 32) R + r][32] (kk < kpad,
-// zero past R); FP64: row-major [r][kpad].  One thread per output element, kk fastest.
+// Xr[(mu + nao i) ldk + P] = X[mu + nao (P + naux i)], zero for naux <= P < ldk: the [nao, naux]
+// slice of each i transposed through a 64 x 64 LDS tile (grid: mu tiles, P tiles, i)
+__global__ __launch_bounds__(256) void transpose_x_kernel(const double* __restrict__ X, int64_t nao, int64_t naux,
+                                                          int64_t ldk, double* __restrict__ Xr) {
+  __shared__ double t[64][65];
+  const int64_t mu0 = (int64_t)blockIdx.x * 64, P0 = (int64_t)blockIdx.y * 64, i = blockIdx.z;
+  const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
+  const double* src = X + nao * naux * i;
+  for (int pp = ty; pp < 64; pp += 4) {
+    const int64_t P = P0 + pp, mu = mu0 + tx;
+    t[pp][tx] = (mu < nao && P < naux) ? src[mu + nao * P] : 0.0;
+  }
+  __syncthreads();
+  for (int mm = ty; mm < 64; mm += 4) {
+    const int64_t mu = mu0 + mm, P = P0 + tx;
+    if (mu < nao && P < ldk) Xr[(mu + nao * i) * ldk + P] = t[tx][mm];
+  }
+}
+
+// M[(mu,i),(nu,j)] = cg G[r + R kk] - cv (Vu[(nu + nao mu) + nao^2 tp(i,j)] + G[(nu + nao i) + R (mu + nao j)])
+// with r = mu + nao i, kk = nu + nao j (Vu: the (mu, nu)-unpacked Vtp).  Output BF16: tiled hi / lo
+// [(kk / 32) R + r][32] (kk < kpad, zero past R); FP64: row-major [r][kpad].  One thread per output
+// element, kk fastest.
 template <typename T, bool BF>
 __global__ void assemble_m_kernel(const T* __restrict__ G, const T* __restrict__ Vp, int64_t nao, int64_t nocc,
                                   int64_t kpad, double cg, double cv, unsigned short* __restrict__ hi,
@@ -59,7 +91,7 @@ __global__ void assemble_m_kernel(const T* __restrict__ G, const T* __restrict__
       const int64_t mu = r % nao, i = r / nao, nu = kk % nao, j = kk / nao;
       const double g = (double)G[kk + R * r];
       const double gt = (double)G[(nu + nao * i) + R * (mu + nao * j)];
-      const double vv = (double)Vp[(nu + nao * mu) + n2 * (i + nocc * j)];
+      const double vv = (double)Vp[(nu + nao * mu) + n2 * tp_index(i, j)];
       v = cg * g - cv * (vv + gt);
     }
     if (BF) {
@@ -121,38 +153,43 @@ void Rijk::build_resident(Prec prec) {
   const int64_t nocc = nocc_[0], R = nao_ * nocc, kpad = ceil_div(R, 32) * 32;
   if (!res_ready_[0]) {
     // FP64 matrix: the products in FP64 (the matrix is exact to FP64 reassociation)
-    DevBuf G, Vp, O2;
+    const int64_t ldk = ceil_div(naux_, 32) * 32, ntp = nocc * (nocc + 1) / 2;
+    DevBuf G, Vp, Xr, O2t, Vtp;
     double* g = static_cast<double*>(G.reserve(sizeof(double) * R * R));
-    double* vp = static_cast<double*>(Vp.reserve(sizeof(double) * R * R));
-    // G[(mu,i),(nu,j)] = sum_P X[mu,P,i] X[nu,P,j]: one [nao x nao] block per (i, j), batched
+    // G[(mu,i),(nu,j)] = sum_P Xr[(mu,i)][P] Xr[(nu,j)][P]: one NT product, K = naux padded to 32
     {
-      GemmArgs a;
-      a.M = nao_; a.N = nao_; a.K = naux_;
-      a.A = x_[0].ptr; a.lda = nao_; a.a_kmajor = false;   // A[mu, P] = X[mu + nao P (+ nao naux i)]
-      a.B = x_[0].ptr; a.ldb = nao_; a.b_kmajor = false;   // B[P, nu] = X[nu + nao P (+ nao naux j)]
+      double* xr = static_cast<double*>(Xr.reserve(sizeof(double) * R * ldk));
+      hipLaunchKernelGGL(transpose_x_kernel, dim3((unsigned)ceil_div(nao_, 64), (unsigned)ceil_div(ldk, 64), (unsigned)nocc),
+                         dim3(256), 0, stream_, x_[0].as<double>(), nao_, naux_, ldk, xr);
+      HH_HIP(hipGetLastError());
+      NtArgs a;
+      a.M = R; a.N = R; a.K = ldk;
+      a.A = xr; a.B = xr; a.lda = a.ldb = ldk;
       a.C = g; a.ldc = R;
-      a.nb0 = nocc; a.sA0 = nao_ * naux_; a.sB0 = 0; a.sC0 = nao_;
-      a.nb1 = nocc; a.sA1 = 0; a.sB1 = nao_ * naux_; a.sC1 = R * nao_;
-      gemm<double, double>(a, PREC_F64, stream_, ws_);
+      a.ksplit = 1;
+      gemm_nt(a, NT_F64, stream_, ws_);
     }
-    // V'[(mu,nu),(j,i)] = sum_P B[mu,nu,P] O2[P,(j,i)]
+    // Vtp[(mu<=nu), (i<=j)] = sum_P cderi[(mu,nu), P] O2t[P, (i,j)], then unpacked over (mu, nu)
+    double* vp = static_cast<double*>(Vp.reserve(sizeof(double) * nao_ * nao_ * ntp));
     {
-      double* o2 = static_cast<double*>(O2.reserve(sizeof(double) * nocc * nocc * naux_));
-      hipLaunchKernelGGL(permute_o_kernel<double>, dim3(rblocks(nocc * nocc * naux_)), dim3(256), 0, stream_,
-                         o_[0].as<double>(), nocc, naux_, o2);
+      double* o2 = static_cast<double*>(O2t.reserve(sizeof(double) * ntp * naux_));
+      double* vtp = static_cast<double*>(Vtp.reserve(sizeof(double) * nao_tp_ * ntp));
+      hipLaunchKernelGGL(pack_o_tp_kernel, dim3(rblocks(ntp * naux_)), dim3(256), 0, stream_, o_[0].as<double>(), nocc,
+                         naux_, o2);
       HH_HIP(hipGetLastError());
       GemmArgs a;
-      a.M = nao_ * nao_; a.N = nocc * nocc; a.K = naux_;
-      a.A = bfull_.ptr; a.lda = nao_ * nao_; a.a_kmajor = false;
-      a.B = o2; a.ldb = nocc * nocc; a.b_kmajor = false;
-      a.C = vp; a.ldc = nao_ * nao_;
+      a.M = nao_tp_; a.N = ntp; a.K = naux_;
+      a.A = cderi_.ptr; a.lda = nao_tp_; a.a_kmajor = false;
+      a.B = o2; a.ldb = ntp; a.b_kmajor = false;
+      a.C = vtp; a.ldc = nao_tp_;
       gemm<double, double>(a, PREC_F64, stream_, ws_);
+      unpack_triu<double, double>(vtp, nao_, nao_tp_, ntp, vp, stream_);
     }
     res64_.reserve(sizeof(double) * R * kpad);
     hipLaunchKernelGGL((assemble_m_kernel<double, false>), dim3(rblocks(R * kpad)), dim3(256), 0, stream_, g, vp, nao_,
                        nocc, kpad, 4.0 * scale_j_, scale_k_, nullptr, nullptr, res64_.as<double>());
     HH_HIP(hipGetLastError());
-    HH_HIP(hipStreamSynchronize(stream_));  // G, V' and O2 are freed on return
+    HH_HIP(hipStreamSynchronize(stream_));  // the intermediates are freed on return
     res_ready_[0] = true;
   }
   if (prec == PREC_BF16X3 && !res_ready_[1]) {
