@@ -160,27 +160,37 @@ def cphf_case(obj, d_C, occ, nao, nocc, nprop, dev, config=None):
     return drv, e, torch.from_numpy(f1).to(dev), torch.from_numpy(s1).to(dev), ratio
 
 
-def response_spectrum(drv, nao, nocc, dev):
-    """Lowest and highest eigenvalue of the dimensionless response operator A on the vir-occ block
-    (I + A is the CPHF matrix; the occupied rows of the response are zero): ARPACK (scipy eigs) with
-    one FP64 response per matvec.  A = D^-1 K with D the orbital-energy gaps and K symmetric, so
-    the spectrum is real."""
+def response_spectrum(drv, nao, nocc, dev, which=("SR", "LR")):
+    """Lowest and highest eigenvalue of the dimensionless response operator A on the vir-occ
+    block(s) (I + A is the CPHF matrix; the occupied rows of the response are zero): ARPACK (scipy
+    eigs) with one FP64 response per matvec.  A = D^-1 K with D the orbital-energy gaps and K
+    symmetric, so the spectrum is real.  `nocc`: an int (RHF driver) or one count per spin (UHF)."""
     import numpy as np
     import torch
     from scipy.sparse.linalg import LinearOperator, eigs
-    nv = nao - nocc
+    noc = [nocc] if isinstance(nocc, (int, np.integer)) else list(nocc)
+    sizes = [(nao - no) * no for no in noc]
+    n = sum(sizes)
 
     def mv(v):
-        x = np.zeros((nao, nocc, 1))
-        x[nocc:, :, 0] = np.real(v).reshape(nv, nocc)
-        y = drv.response_dimless_cphf(torch.from_numpy(x).to(dev))
-        return y[nocc:, :, 0].cpu().numpy().ravel()
+        v = np.real(v)
+        xs, p = [], 0
+        for no, sz in zip(noc, sizes):
+            x = np.zeros((nao, no, 1))
+            x[no:, :, 0] = v[p:p + sz].reshape(nao - no, no)
+            xs.append(torch.from_numpy(x).to(dev))
+            p += sz
+        ys = drv.response_dimless_cphf(xs[0] if len(noc) == 1 else xs)
+        ys = [ys] if len(noc) == 1 else ys
+        return np.concatenate([y[no:, :, 0].cpu().numpy().ravel() for y, no in zip(ys, noc)])
 
-    op = LinearOperator((nv * nocc, nv * nocc), matvec=mv, dtype=np.float64)
-    v0 = np.random.default_rng(7).standard_normal(nv * nocc)
-    lo = eigs(op, k=2, which="SR", return_eigenvectors=False, tol=1e-6, ncv=24, v0=v0)
-    hi = eigs(op, k=2, which="LR", return_eigenvectors=False, tol=1e-6, ncv=24, v0=v0)
-    return float(np.real(lo).min()), float(np.real(hi).max())
+    op = LinearOperator((n, n), matvec=mv, dtype=np.float64)
+    v0 = np.random.default_rng(7).standard_normal(n)
+    out = []
+    for w in which:
+        ev = eigs(op, k=2, which=w, return_eigenvectors=False, tol=1e-6, ncv=min(24, n - 1), v0=v0)
+        out.append(float(np.real(ev).min() if w == "SR" else np.real(ev).max()))
+    return tuple(out)
 
 
 def resident_leg(d_cderi, d_C, occ, d_bra, r64, nao, nocc, nprop, steps):
@@ -629,7 +639,9 @@ def workload_cphf_batch(args):
     """Many uneven molecules: 64 seeded alkane-sized RI-JK CPHF systems (C_nH_2n+2 with
     cc-pVDZ / -jkfit counts: nao 24n+10, naux 84n+28, 3(3n+2) perturbations, n uniform in 1..12),
     synthetic tensors; every second molecule is a doublet cation (UHF, 4n+1 alpha and 4n beta
-    occupied orbitals), the others closed shells (RHF, 4n+1 doubly occupied).  Each molecule is
+    occupied orbitals), the others closed shells (RHF, 4n+1 doubly occupied).  Each system's
+    RI-JK scale gives |A v| ~ 0.3 |v| for random v, shrunk where A then has eigenvalues below
+    -0.75 (an indefinite I + A, an unstable reference state, is not a CPHF system).  Each molecule is
     solved to 1e-8 (one subspace of up to 42 cycles) on one created stream, plain FP64 with the
     response launched eagerly and mixed precision (BF16x3 operator, FP64 refinement) with the
     response replayed from HIP graphs; each leg runs once untimed first (buffers sized, products
@@ -648,6 +660,7 @@ def workload_cphf_batch(args):
     caps = launches = 0
     err = 0.0
     nuhf = fallbacks = 0
+    min_eig = []
 
     def diff(a, b):
         if isinstance(a, (list, tuple)):
@@ -665,23 +678,30 @@ def workload_cphf_batch(args):
             C = torch.from_numpy(synth.mo_coeff(nao, seed=synth.SEED_C + 16 * i)).to(dev)
             if uhf:
                 occ = [np.where(np.arange(nao) < 4 * n + 1, 1.0, 0.0), np.where(np.arange(nao) < 4 * n, 1.0, 0.0)]
-                probe = hesship.UHessRIJK(cd, 1.0, 1.0)
-                probe.make_response_preparation([C, C], occ)
-                pdrv, _, _, _, ratio = ucphf_case(probe, C, occ, nao, nprop, dev, cfg)
-                del pdrv, probe
-                obj = hesship.UHessRIJK(cd * (0.3 / ratio) ** 0.5, 1.0, 1.0)
-                obj.make_response_preparation([C, C], occ)
-                drv, _, f1, s1, _ = ucphf_case(obj, C, occ, nao, nprop, dev, cfg)
+                noc = [4 * n + 1, 4 * n]
+
+                def system(scale2):
+                    o = hesship.UHessRIJK(cd * scale2 ** 0.5, 1.0, 1.0)
+                    o.make_response_preparation([C, C], occ)
+                    return (o,) + ucphf_case(o, C, occ, nao, nprop, dev, cfg)
             else:
-                nocc = 4 * n + 1
-                occ = np.where(np.arange(nao) < nocc, 2.0, 0.0)
-                probe = hesship.RHessRIJK(cd, 1.0, 1.0)
-                probe.make_response_preparation(C, occ)
-                pdrv, _, _, _, ratio = cphf_case(probe, C, occ, nao, nocc, nprop, dev, cfg)
-                del pdrv, probe
-                obj = hesship.RHessRIJK(cd * (0.3 / ratio) ** 0.5, 1.0, 1.0)
-                obj.make_response_preparation(C, occ)
-                drv, _, f1, s1, _ = cphf_case(obj, C, occ, nao, nocc, nprop, dev, cfg)
+                noc = 4 * n + 1
+                occ = np.where(np.arange(nao) < noc, 2.0, 0.0)
+
+                def system(scale2):
+                    o = hesship.RHessRIJK(cd * scale2 ** 0.5, 1.0, 1.0)
+                    o.make_response_preparation(C, occ)
+                    return (o,) + cphf_case(o, C, occ, nao, noc, nprop, dev, cfg)
+            # |A v| ~ 0.3 |v| for random v, then shrunk where that leaves I + A indefinite
+            ratio = system(1.0)[-1]
+            scale2 = 0.3 / ratio
+            obj, drv, _, f1, s1, _ = system(scale2)
+            lo = response_spectrum(drv, nao, noc, dev, which=("SR",))[0]
+            if lo < -LOWEST_A:
+                scale2 *= LOWEST_A / -lo
+                del obj, drv
+                obj, drv, _, f1, s1, _ = system(scale2)
+            min_eig.append(round(max(lo, -LOWEST_A), 4))
             drv.set_graphs(False)
             run_cphf(drv, f1, s1, "fp64")
             ref, _, t64, st64 = run_cphf(drv, f1, s1, "fp64")
@@ -712,6 +732,7 @@ def workload_cphf_batch(args):
                  {"bound": "launch", "achieved": None, "peak": None, "unit": None, "frac": None, "traffic": None,
                   "kernel": "whole solves"}, higher=False)
     line["uhf_molecules"] = nuhf
+    line["min_eig_A"] = min_eig
     line["fp64_eager_wall_s"] = round(tot["fp64"], 4)
     line["speedup_mixed_vs_fp64"] = round(tot["fp64"] / wall, 3)
     line["krylov_cycles"] = cyc

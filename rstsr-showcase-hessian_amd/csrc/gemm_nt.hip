@@ -646,17 +646,12 @@ void launch_f64_cfg(int cfg, const NtArgs& a, hipStream_t s, DevBuf& ws) {
 // measured ceiling is 1.55x that of 16 x 16 x 32): 9.5 ms against 8.55 ms
 constexpr int NT_BF_NCFG = 3;
 
+// BF16X3 operands are always K-block tiled (every producer writes that layout)
 void launch_bf_cfg(int cfg, const NtArgs& a, hipStream_t s, DevBuf& ws) {
-  if (cfg == 2) {
-    if (a.tiled) launch_nt<2, 4, 13, 2, true, true, 25, 2, true>(a, s, ws);
-    else launch_nt<2, 4, 13, 2, true, false, 25, 1, true>(a, s, ws);  // one step in flight: no spills
-  } else if (cfg == 1) {
-    if (a.tiled) launch_nt<2, 4, 7, 4, true, true, 13, 1>(a, s, ws);
-    else launch_nt<2, 4, 7, 4, true, false, 13, 1>(a, s, ws);
-  } else {
-    if (a.tiled) launch_nt<1, 8, 13, 2, true, true>(a, s, ws);
-    else launch_nt<1, 8, 13, 2, true, false>(a, s, ws);
-  }
+  HH_REQUIRE(a.tiled || a.K == 0, "gemm_nt: BF16X3 operands must be K-block tiled");
+  if (cfg == 2) launch_nt<2, 4, 13, 2, true, true, 25, 2, true>(a, s, ws);
+  else if (cfg == 1) launch_nt<2, 4, 7, 4, true, true, 13, 1>(a, s, ws);
+  else launch_nt<1, 8, 13, 2, true, true>(a, s, ws);
 }
 
 struct NtChoice {

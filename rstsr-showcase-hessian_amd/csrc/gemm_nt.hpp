@@ -3,7 +3,7 @@
 //   C[m, n] = alpha * sum_k A[m*lda + k] * B[n*ldb + k] + beta * C[m + n*ldc]     (C column-major)
 //
 // K must be a multiple of 32 and lda, ldb multiples of 8 (16-byte rows); operands are padded
-// by their producers.  With `tiled` set (BF16X3 only) both operands are stored in K blocks of 32:
+// by their producers.  With `tiled` set (required for BF16X3) both operands are stored in K blocks of 32:
 // A[((k / 32) * M + m) * 32 + k % 32], B[((k / 32) * N + n) * 32 + k % 32] (lda, ldb unused), so
 // each K step of a workgroup reads one contiguous run of 64 bytes per row for all its rows.
 // (a_rows / b_rows replace M / N as the K-block stride for row ranges of larger tiled arrays.)
@@ -30,7 +30,7 @@ struct NtArgs {
   float* C_f32 = nullptr;  // BF16X3: FP32 output instead of C
   int64_t ldc = 0;
   double alpha = 1.0, beta = 0.0;
-  bool tiled = false;  // K-block tiled operands (BF16X3)
+  bool tiled = false;  // K-block tiled operands (BF16X3: required)
   // Tiled operands that are a row range of a larger tiled array: rows per K block of the array
   // (0 = M for A, N for B); the operand pointer then addresses its first row in K block 0.
   int64_t a_rows = 0, b_rows = 0;

@@ -48,6 +48,27 @@ def test_rijk_response_full_size_production_launch(dev, c2, precision):
     assert err < TOL[precision]
 
 
+def test_rijk_response_full_size_materialized_k(dev, c2):
+    """The BF16x3 K path with the operand T materialised (the path taken past the fused kernel's
+    limits): the autotuned NT hot product at K = naux x 416 = 998400 over all 90 perturbations,
+    three columns against the oracle."""
+    import ctypes as C
+    import hesship
+    from hesship import _lib
+    cderi, Cm, occ, bra, ref = c2
+    d_cderi = torch.from_numpy(np.asfortranarray(cderi).T.copy()).to(dev).T
+    obj = hesship.RHessRIJK(d_cderi, 1.0, 1.0)
+    obj.set_k_path("materialized")
+    obj.make_response_preparation(torch.from_numpy(Cm).to(dev), occ)
+    d_bra = hesship.fortran(torch.from_numpy(bra).to(dev))
+    got = obj.get_response_bra(d_bra, precision="bf16x3")[:, :, COLS].cpu().numpy()
+    cfg, ks = C.c_int32(), C.c_int32()
+    _lib.check(_lib.lib().hh_util_gemm_nt_bf16x3_choice(400, 3600, 2400 * 416, C.byref(cfg), C.byref(ks)))
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"materialised K, hot product configuration {cfg.value} split rule {ks.value}: rel err {err:.2e}")
+    assert err < TOL["bf16x3"]
+
+
 @pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
 def test_rks_response_c3_shape(dev, precision):
     import hesship
