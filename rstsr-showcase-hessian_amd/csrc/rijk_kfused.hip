@@ -193,6 +193,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   const u16* o_hi = p.oh + (int64_t)i_l * p.jp + 8 * h;
   const u16* o_lo = p.ol + (int64_t)i_l * p.jp + 8 * h;
   const int64_t o_stride = p.nocc * p.jp;
+  auto load_o1 = [&](int64_t P, uint4 (&oh)[NH], uint4 (&ol)[NH], int q) {
+    oh[q] = ld16(o_hi + P * o_stride + 16 * q);
+    ol[q] = ld16(o_lo + P * o_stride + 16 * q);
+  };
   auto load_o = [&](int64_t P, uint4 (&oh)[NH], uint4 (&ol)[NH]) {
 #pragma unroll
     for (int q = 0; q < NH; ++q) {
@@ -206,6 +210,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   constexpr int GPA = 2 * NMB;  // groups per array (32 NMB rows x 4 slots / 64)
   const int lane_off = (lane >> 2) * 32 + (((lane & 3) ^ ((lane >> 4) & 3)) * 8);
   constexpr int64_t TILE_LEN = (int64_t)TILE;
+  // one of the NMB copy instructions of a wave (group w + 4 k)
+  auto dma_one = [&](int64_t P, u16 (&dst)[2][TILE], int k) {
+    const int64_t base = ((int64_t)b * p.naux_e + P) * TILE_LEN + lane_off;
+    const int g = w + 4 * k;
+    const int arr = g >= GPA ? 1 : 0;
+    const int gg = g - arr * GPA;
+    const u16* src = (arr ? p.bl : p.bh) + base + gg * 512;
+    __builtin_amdgcn_global_load_lds(src, (lds_ptr)&dst[arr][gg * 512], 16, 0, 0);
+  };
   auto dma_b = [&](int64_t P, u16 (&dst)[2][TILE]) {
     const int64_t base = ((int64_t)b * p.naux_e + P) * TILE_LEN + lane_off;
 #pragma unroll
@@ -267,13 +280,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 
   // Software pipeline over P: step P contracts stage P with Q(P) (built during step P - 1) and
   // builds Q(P + 1) by pairs interleaved with the second K half of the contraction; B(P + 1) goes
-  // to the other LDS stage.  Steps alternate between the two stages.  In program order
+  // to the other LDS stage, one copy instruction per row block of the first K half.  Steps alternate between the two stages.  In program order
   // (scheduling barriers keep it): per row block the fragments of block m + 2 are requested while
   // block m's three products run.
   // O(P + 1) is in registers when step P starts (loaded one step earlier, before the previous
   // barrier's wait): two register sets, swapped between the two steps of a loop trip.
   uint4 oah[NH], oal[NH], obh[NH], obl[NH];
-  v8bf qh[2], ql[2];
+  v8bf qh[2], ql[2], rh[2], rl[2];
   load_o(P0, oah, oal);
   load_o(P0 + 1, obh, obl);
   dma_b(P0, lds0);
@@ -292,10 +305,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   }
   // step P: O(P + 1) in (uh, ul), O(P + 2) requested into (lh, ll); index P + 2 <= P1 + 1 and tile
   // P + 1 <= P1 exist (padding past every range)
+  // Q(P + 1) is split into its bf16 operand registers (the second set of two, alternating between
+  // steps) right after the row block QS, so the conversion overlaps the remaining products
+  constexpr int QS = NPAIR + 1 < NMB ? NPAIR + 1 : -1;
   auto step = [&](int64_t P, const u16 (&cur)[2][TILE], u16 (&nxt)[2][TILE], const uint4 (&uh)[NH],
-                  const uint4 (&ul)[NH], uint4 (&lh)[NH], uint4 (&ll)[NH]) {
-    load_o(P + 2, lh, ll);
-    dma_b(P + 1, nxt);
+                  const uint4 (&ul)[NH], uint4 (&lh)[NH], uint4 (&ll)[NH], const v8bf (&ch)[2],
+                  const v8bf (&cl)[2], v8bf (&nh)[2], v8bf (&nl)[2]) {
     const u16* sh = cur[0];
     const u16* sl2 = cur[1];
     v16f q;
@@ -317,13 +332,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
       for (int m = 0; m < NMB; ++m) {
         // rows past 32 nb read stale LDS and are never stored
-        acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al0, qh[kap], acc[m], 0, 0, 0);
-        acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, ql[kap], acc[m], 0, 0, 0);
-        acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, qh[kap], acc[m], 0, 0, 0);
+        acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al0, ch[kap], acc[m], 0, 0, 0);
+        acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, cl[kap], acc[m], 0, 0, 0);
+        acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah0, ch[kap], acc[m], 0, 0, 0);
+        if (kap == 1 && m == QS) qsplit(q, nh, nl);
         if (kap == 1 && m < NPAIR) {
           qmma(m, bh0, bl0, uh, ul, q);
           if (m + 1 < NPAIR) qload(m + 1, bh0, bl0);
         }
+        // the copies of B(P + 1) are spread over the first K half, one per row block: issued
+        // back to back at the start of the step they delayed the first products
+        if (kap == 0) dma_one(P + 1, nxt, m);
+        if (kap == 0 && m < NH) load_o1(P + 2, lh, ll, m);  // O(P + 2), used two steps on
         ah0 = ah1;
         al0 = al1;
         if (m + 2 < NMB) frag(m + 2, ah1, al1);
@@ -332,16 +352,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       }
     }
 #pragma unroll
+    for (int q = NMB; q < NH; ++q) load_o1(P + 2, lh, ll, q);
+#pragma unroll
     for (int pr = NMB; pr < NPAIR; ++pr) {
       qmma(pr, bh0, bl0, uh, ul, q);
       if (pr + 1 < NPAIR) qload(pr + 1, bh0, bl0);
     }
-    qsplit(q, qh, ql);  // Q(P + 1); unused after the last step
+    if (QS < 0) qsplit(q, nh, nl);  // Q(P + 1); unused after the last step
     sync_stage();
   };
   for (int64_t P = P0; P < P1; P += 2) {
-    step(P, lds0, lds1, obh, obl, oah, oal);
-    step(P + 1, lds1, lds0, oah, oal, obh, obl);
+    step(P, lds0, lds1, obh, obl, oah, oal, qh, ql, rh, rl);
+    step(P + 1, lds1, lds0, oah, oal, obh, obl, rh, rl, qh, ql);
   }
 
   if (valid) {
