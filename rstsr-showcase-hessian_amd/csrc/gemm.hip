@@ -41,6 +41,7 @@ struct GemmParams {
   const void* B2;
   int64_t lda2, ldb2, sA2_0, sA2_1, sB2_0, sB2_1;
   void* C_lo;      // non-null: write C as a bf16 hi (C) / lo (C_lo) pair of u16 arrays
+  const int64_t* krange;  // non-null: K range [krange[2 b1], krange[2 b1 + 1]) per second batch index
 };
 
 template <int PREC> struct PrecTraits;
@@ -94,8 +95,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmParams p) {
   const int64_t b0 = bidx % p.nb0, b1 = bidx / p.nb0;
   const int kz = blockIdx.z;
   const int64_t m0 = tm * BM, n0 = tn * BN;
-  const int64_t kbeg = (int64_t)kz * p.kchunk;
-  const int64_t kend = min(p.K, kbeg + p.kchunk);
+  const int64_t kbeg = (p.krange ? max((int64_t)0, p.krange[2 * b1]) : 0) + (int64_t)kz * p.kchunk;
+  const int64_t kend = min(p.krange ? min(p.K, p.krange[2 * b1 + 1]) : p.K, kbeg + p.kchunk);
 
   const TIn* __restrict__ A = static_cast<const TIn*>(p.A) + b0 * p.sA0 + b1 * p.sA1;
   const TIn* __restrict__ B = static_cast<const TIn*>(p.B) + b0 * p.sB0 + b1 * p.sB1;
@@ -351,6 +352,10 @@ void launch_cfg(const GemmArgs& g, hipStream_t stream, DevBuf& ws) {
   if (p.K1 < p.K) HH_REQUIRE(g.A2 && g.B2, "gemm: second operand pair missing");
   p.C_lo = g.C_lo;
   if (g.C_lo) HH_REQUIRE(g.ksplit == 1, "gemm: split bf16 output needs ksplit = 1");
+  p.krange = g.krange;
+  if (g.krange) HH_REQUIRE(p.K1 >= p.K && g.krange_max >= 0 && g.krange_max <= g.K, "gemm: grouped K ranges need "
+                                                                                  "one operand pair and 0 <= krange_max <= K");
+  const int64_t klen = g.krange ? std::max<int64_t>(g.krange_max, 1) : g.K;  // K length one split covers
   p.tiles_m = ceil_div(g.M, BM);
   p.tiles_n = ceil_div(g.N, BN);
   const int64_t nbatch = g.nb0 * g.nb1;
@@ -360,11 +365,11 @@ void launch_cfg(const GemmArgs& g, hipStream_t stream, DevBuf& ws) {
     // enough workgroups to cover the chip ~4x, but keep >= 8 K-steps per split
     const int64_t target = 4LL * num_cus();
     ksplit = 1;
-    while (tiles * ksplit < target && ceil_div(g.K, (int64_t)BK * ksplit * 2) >= 8) ksplit *= 2;
+    while (tiles * ksplit < target && ceil_div(klen, (int64_t)BK * ksplit * 2) >= 8) ksplit *= 2;
   }
-  int64_t kchunk = ceil_div(ceil_div(g.K, ksplit), BK) * BK;
+  int64_t kchunk = ceil_div(ceil_div(klen, ksplit), BK) * BK;
   if (kchunk <= 0) kchunk = BK;
-  ksplit = (int)ceil_div(std::max<int64_t>(g.K, 1), kchunk);
+  ksplit = (int)ceil_div(std::max<int64_t>(klen, 1), kchunk);
   p.kchunk = kchunk;
   p.ksplit = ksplit;
   if (ksplit > 1) p.partial = ws.reserve(sizeof(acc_t) * (size_t)ksplit * nbatch * g.M * g.N);

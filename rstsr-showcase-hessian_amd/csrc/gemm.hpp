@@ -34,6 +34,13 @@ struct GemmArgs {
   int64_t nb0 = 1, nb1 = 1;
   int64_t sA0 = 0, sA1 = 0, sB0 = 0, sB1 = 0, sC0 = 0, sC1 = 0;
   int ksplit = 0;  // 0: choose automatically
+  // Grouped K ranges: non-null device array [nb1][2]; batch index b1 then sums only over
+  // k0 <= k < k1 (k0 = krange[2 b1], k1 = krange[2 b1 + 1], 0 <= k0, k1 <= K) of the operands
+  // addressed as for the whole K: products that share A and B and differ in a contiguous K
+  // slice (per-atom blocks) run as one launch.  krange_max: the longest slice (sizes the K split,
+  // which then splits every slice into chunks of the same length).  No second operand pair.
+  const int64_t* krange = nullptr;
+  int64_t krange_max = 0;
   // Optional second operand pair concatenated along K: for k >= K1 the product uses
   // A2[m, k - K1] B2[k - K1, n] (same majorness as A / B; K is the total length).  K1 = -1: off.
   int64_t K1 = -1;

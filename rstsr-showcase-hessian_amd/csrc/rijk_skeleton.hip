@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "elementwise.hpp"
+#include "gemm_nt.hpp"
 #include "linalg.hpp"
 #include "rijk_skeleton.hpp"
 
@@ -36,6 +37,25 @@ void gm(int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, bool akm,
   g.C = C; g.ldc = ldc;
   g.alpha = alpha; g.beta = beta;
   g.nb0 = nb0; g.sA0 = sA; g.sB0 = sB; g.sC0 = sC;
+  gemm<double, double>(g, PREC_F64, s, ws);
+}
+
+// The same product for ng groups in one launch: group g sums over the K slice
+// [krange[2 g], krange[2 g + 1]) (device table) of the shared operands and writes C + g sC1.
+void gmg(int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, bool akm, const double* B, int64_t ldb,
+         bool bkm, double* C, int64_t ldc, double alpha, double beta, hipStream_t s, DevBuf& ws, int64_t nb0,
+         int64_t sA, int64_t sB, int64_t sC, int64_t ng, int64_t sC1, const int64_t* krange, int64_t kmax) {
+  if (M <= 0 || N <= 0 || nb0 <= 0 || ng <= 0 || K <= 0) return;
+  GemmArgs g;
+  g.M = M; g.N = N; g.K = K;
+  g.A = A; g.lda = lda; g.a_kmajor = akm;
+  g.B = B; g.ldb = ldb; g.b_kmajor = bkm;
+  g.C = C; g.ldc = ldc;
+  g.alpha = alpha; g.beta = beta;
+  g.nb0 = nb0; g.sA0 = sA; g.sB0 = sB; g.sC0 = sC;
+  g.nb1 = ng; g.sC1 = sC1;
+  g.krange = krange;
+  g.krange_max = kmax;
   gemm<double, double>(g, PREC_F64, s, ws);
 }
 
@@ -268,6 +288,21 @@ __global__ void de_out_kernel(const double* H, int64_t N, double* de) {
   de[e] = H[(t + 3 * A) + 3 * N * (s + 3 * B)];
 }
 
+// X[P, c, t, A(P)] += Y[P, c, t] for the auxiliary functions P of listed atoms (A(P) = auxatom[P])
+__global__ void scatter_rows_kernel(const double* Y, int64_t m, int64_t ncol, const int64_t* auxatom, double* X) {
+  const int64_t e = ELEM;
+  if (e >= m * ncol * 3) return;
+  const int64_t P = e % m, c = (e / m) % ncol, t = e / (m * ncol);
+  const int64_t A = auxatom[P];
+  if (A >= 0) X[P + m * (c + ncol * (t + 3 * A))] += Y[e];
+}
+
+// T[i + m k] = X[k + m i]
+__global__ void transpose_sq_kernel(const double* X, int64_t m, double* T) {
+  const int64_t e = ELEM;
+  if (e < m * m) T[e] = X[(e / m) + m * (e % m)];
+}
+
 void launch_check() { HH_HIP(hipGetLastError()); }
 
 }  // namespace
@@ -321,7 +356,8 @@ void rijk_skeleton(Rijk& r, const SkeletonInput& in, double* de, double* const* 
   }
 
   DevBuf ws, bL, bLi, bmeta, bmo[2], bC[2], bD, bbD, bd, bBo[2], bbK[2], bdKij[2], bdKP[2], bTt[2], bg[2], bF, bXJ,
-      bXK[2], bG1, bj1, bk1[2], br1[2], brpp, bz, by, bq, bH, bI, bIT, bY, bGs, bHA, bM, bMp, bMpp, bE1, bW, bout;
+      bXK[2], bG1, bj1, bk1[2], br1[2], brpp, bz, by, bq, bH, bI, bIT, bY, bGs, bHA, bM, bMp, bMpp, bE1, bW, bout,
+      bYu, bktab, bLiT;
   auto D = [](DevBuf& b, int64_t count) { return static_cast<double*>(b.reserve(sizeof(double) * std::max<int64_t>(1, count))); };
   auto zero = [&](double* p, int64_t count) { fill<double>(p, count, 0.0, st); };
 
@@ -418,18 +454,19 @@ void rijk_skeleton(Rijk& r, const SkeletonInput& in, double* de, double* const* 
   double* XJ = D(bXJ, m * 3 * N);
   zero(XJ, m * 3 * N);
   double* XK[2] = {nullptr, nullptr};
+  // X[:, c, s, B] (one block per listed atom B and component s):
+  //   rows P in B: + sum_Q ip1[P, Q, s] d[Q, c]   (one product for all rows, scattered to P's atom)
+  //   all rows:    + sum_{Q in B} ip1[Q, P, s] d[Q, c]   (the atoms as K slices of one launch)
+  const int64_t ncol_max = std::max<int64_t>(1, std::max(no[0] * no[0], ns > 1 ? no[1] * no[1] : 0));
+  int64_t kmax_aux = 0;
+  for (int64_t A = 0; A < N; ++A) kmax_aux = std::max(kmax_aux, in.auxslice[2 * A + 1] - in.auxslice[2 * A]);
+  double* Yu = D(bYu, m * ncol_max * 3);
   auto add_minus_u = [&](double* X, const double* dcols, int64_t ncol) {
-    for (int64_t B = 0; B < N; ++B) {
-      const int64_t q0 = in.auxslice[2 * B], mb = in.auxslice[2 * B + 1] - q0;
-      for (int sg = 0; sg < 3; ++sg) {
-        double* Xx = X + m * ncol * (sg + 3 * B);
-        const double* ip = in.j2c_ip1 + m * m * sg;
-        // rows P in B: + sum_Q ip1[P, Q, s] d[Q]
-        gm(mb, ncol, m, ip + q0, m, false, dcols, m, true, Xx + q0, m, 1.0, 1.0, st, ws);
-        // all rows: + sum_{Q in B} ip1[Q, P, s] d[Q]
-        gm(m, ncol, mb, ip + q0, m, true, dcols + q0, m, true, Xx, m, 1.0, 1.0, st, ws);
-      }
-    }
+    gm(m, ncol, m, in.j2c_ip1, m, false, dcols, m, true, Yu, m, 1.0, 0.0, st, ws, 3, m * m, 0, m * ncol);
+    hipLaunchKernelGGL(scatter_rows_kernel, GRID(m * ncol * 3), Yu, m, ncol, d_auxatom, X);
+    launch_check();
+    gmg(m, ncol, m, in.j2c_ip1, m, true, dcols, m, true, X, m, 1.0, 1.0, st, ws, 3, m * m, 0, m * ncol, N,
+        3 * m * ncol, d_auxsl, kmax_aux);
   };
   if (doJ) add_minus_u(XJ, d, 1);
   if (doK)
@@ -488,8 +525,47 @@ void rijk_skeleton(Rijk& r, const SkeletonInput& in, double* de, double* const* 
     if (rc != 0) throw Error(HH_ERR_INVALID, "int3c generator returned " + std::to_string(rc));
   };
 
+  // per auxiliary range, the K slices of the per-atom products: ip1 k1acc (K = (k, P): the atom's
+  // AOs), ip2 j1acc (K = P: the atom's auxiliary functions in the range), ip2 k1acc (K = (k, P))
+  std::vector<int64_t> ktab((size_t)in.nrange * 6 * N, 0), kmax_tab((size_t)in.nrange * 3, 0);
+  // listed atoms with auxiliary functions in each range: [alo, ahi) (contiguous: the slices are
+  // ordered by atom), so the ip2 products launch over those atoms only
+  std::vector<int64_t> alo((size_t)in.nrange, 0), ahi((size_t)in.nrange, 0);
   for (int64_t ir = 0; ir < in.nrange; ++ir) {
     const int64_t p0 = in.aux_ranges[2 * ir], p1 = in.aux_ranges[2 * ir + 1], nb = p1 - p0;
+    int64_t* t = ktab.data() + ir * 6 * N;
+    for (int64_t A = 0; A < N; ++A) {
+      t[2 * A] = nb * in.aoslice[2 * A];
+      t[2 * A + 1] = nb * in.aoslice[2 * A + 1];
+      const int64_t q0 = std::max(p0, in.auxslice[2 * A]), q1 = std::min(p1, in.auxslice[2 * A + 1]);
+      if (q1 > q0) {
+        if (ahi[ir] == 0) alo[ir] = A;
+        ahi[ir] = A + 1;
+        t[2 * N + 2 * A] = q0 - p0;
+        t[2 * N + 2 * A + 1] = q1 - p0;
+        t[4 * N + 2 * A] = n * (q0 - p0);
+        t[4 * N + 2 * A + 1] = n * (q1 - p0);
+      }
+      for (int k = 0; k < 3; ++k)
+        kmax_tab[3 * ir + k] = std::max(kmax_tab[3 * ir + k], t[2 * N * k + 2 * A + 1] - t[2 * N * k + 2 * A]);
+    }
+  }
+  // the final K products per spin: K = (j, P) over the atom's auxiliary functions
+  for (int s = 0; s < ns; ++s)
+    for (int64_t A = 0; A < N; ++A) {
+      ktab.push_back(no[s] * in.auxslice[2 * A]);
+      ktab.push_back(no[s] * in.auxslice[2 * A + 1]);
+    }
+  int64_t* d_ktab = static_cast<int64_t*>(bktab.reserve(sizeof(int64_t) * ktab.size()));
+  const int64_t* d_ktab_fin = d_ktab + in.nrange * 6 * N;
+  HH_HIP(hipMemcpyAsync(d_ktab, ktab.data(), sizeof(int64_t) * ktab.size(), hipMemcpyHostToDevice, st));
+  HH_HIP(hipStreamSynchronize(st));  // ktab is released before the device buffers
+
+  for (int64_t ir = 0; ir < in.nrange; ++ir) {
+    const int64_t p0 = in.aux_ranges[2 * ir], p1 = in.aux_ranges[2 * ir + 1], nb = p1 - p0;
+    const int64_t* tab_ip1 = d_ktab + ir * 6 * N;
+    const int64_t* tab_ip2j = tab_ip1 + 2 * N;
+    const int64_t* tab_ip2k = tab_ip1 + 4 * N;
     // M_P = sj d_P D - fk sk sum_s C dK_P C^T for this range
     hipLaunchKernelGGL(mtot_j_kernel, GRID(n * n * nb), Dm, d, n, nb, p0, doJ ? sj : 0.0, M);
     launch_check();
@@ -530,12 +606,10 @@ void rijk_skeleton(Rijk& r, const SkeletonInput& in, double* de, double* const* 
           launch_check();
           // r1[u, v, t] += sum_{k, P} ip1[u, k, P, t] M'[k, P, v]
           gm(n, n, n * nb, I, n, false, Mp, n * nb, true, r1[s], n, 1.0, 1.0, st, ws, 3, n * n * nb, 0, n * n);
-          // k1acc[u, v, t, A] -= sum_{k in A, P} ip1[k, u, P, t] M'[k, P, v]
-          for (int64_t A = 0; A < N; ++A) {
-            const int64_t a0 = in.aoslice[2 * A], na = in.aoslice[2 * A + 1] - a0;
-            gm(n, n, nb * na, IT + nb * a0, nb * n, true, Mpp + nb * a0, nb * n, true, k1acc[s] + n * n * 3 * A, n,
-               -1.0, 1.0, st, ws, 3, n * n * nb, 0, n * n);
-          }
+          // k1acc[u, v, t, A] -= sum_{k in A, P} ip1[k, u, P, t] M'[k, P, v]: the atoms' AO
+          // blocks (K = (k, P), P fastest) as K slices of one launch
+          gmg(n, n, nb * n, IT, nb * n, true, Mpp, nb * n, true, k1acc[s], n, -1.0, 1.0, st, ws, 3, n * n * nb, 0,
+              n * n, N, n * n * 3, tab_ip1, kmax_tab[3 * ir]);
         }
       }
     }
@@ -548,13 +622,9 @@ void rijk_skeleton(Rijk& r, const SkeletonInput& in, double* de, double* const* 
       hipLaunchKernelGGL(xj_ip2_kernel, GRID(nb * 3), Yb, n, nb, d_auxatom, XJ, m, p0);
       launch_check();
     }
-    for (int64_t A = 0; A < N; ++A) {
-      const int64_t q0 = std::max(p0, in.auxslice[2 * A]), q1 = std::min(p1, in.auxslice[2 * A + 1]);
-      if (q1 <= q0) continue;
-      if (doJ && deriv1_bra)  // j1acc[u, v, t, A] -= sum_{P in A} ip2[u, v, P, t] d_P
-        gm(n * n, 1, q1 - q0, I + n * n * (q0 - p0), n * n, false, d + q0, q1 - q0, true, j1acc + n * n * 3 * A,
-           n * n, -1.0, 1.0, st, ws, 3, n * n * nb, 0, n * n);
-    }
+    if (doJ && deriv1_bra)  // j1acc[u, v, t, A] -= sum_{P in A} ip2[u, v, P, t] d_P (atoms: K slices)
+      gmg(n * n, 1, nb, I, n * n, false, d + p0, nb, true, j1acc + n * n * 3 * alo[ir], n * n, -1.0, 1.0, st, ws, 3,
+          n * n * nb, 0, n * n, ahi[ir] - alo[ir], n * n * 3, tab_ip2j + 2 * alo[ir], kmax_tab[3 * ir + 1]);
     if (doK)
       for (int s = 0; s < ns; ++s) {
         const int64_t o = no[s];
@@ -564,12 +634,9 @@ void rijk_skeleton(Rijk& r, const SkeletonInput& in, double* de, double* const* 
         launch_check();
         if (deriv1_bra) {
           gm(n, n, o, C[s], n, false, Tt[s] + n * o * p0, n, false, Mp, n * nb, 1.0, 0.0, st, ws, nb, 0, n * o, n);
-          for (int64_t A = 0; A < N; ++A) {
-            const int64_t q0 = std::max(p0, in.auxslice[2 * A]), q1 = std::min(p1, in.auxslice[2 * A + 1]);
-            if (q1 <= q0) continue;
-            gm(n, n, n * (q1 - q0), I + n * n * (q0 - p0), n, false, Mp + n * (q0 - p0), n * nb, true,
-               k1acc[s] + n * n * 3 * A, n, -1.0, 1.0, st, ws, 3, n * n * nb, 0, n * n);
-          }
+          // k1acc[u, v, t, A] -= sum_{P in A, k} ip2[u, k, P, t] M'[k, P, v] (atoms: K slices)
+          gmg(n, n, n * nb, I, n, false, Mp, n * nb, true, k1acc[s] + n * n * 3 * alo[ir], n, -1.0, 1.0, st, ws, 3,
+              n * n * nb, 0, n * n, ahi[ir] - alo[ir], n * n * 3, tab_ip2k + 2 * alo[ir], kmax_tab[3 * ir + 2]);
         }
       }
 
@@ -601,13 +668,37 @@ void rijk_skeleton(Rijk& r, const SkeletonInput& in, double* de, double* const* 
     gm(nx, nx, m, W, m, true, W, m, true, H, nx, sj, 1.0, st, ws);
     if (deriv1_bra) gm(n * n, nx, m, Bf, n * n, false, W, m, true, j1acc, n * n, 1.0, 1.0, st, ws);
   }
-  if (doK)
+  if (doK) {
+    // with naux a multiple of 32 both products run on the NT FP64 kernel (K-contiguous operands:
+    // Li^T rows, the columns of XK and of W)
+    const bool nt = m % 32 == 0;
+    double* LiT = nullptr;
+    if (nt) {
+      LiT = D(bLiT, m * m);
+      hipLaunchKernelGGL(transpose_sq_kernel, GRID(m * m), Li, m, LiT);
+      launch_check();
+    }
     for (int s = 0; s < ns; ++s) {
       const int64_t o2 = no[s] * no[s];
       double* W = D(bW, m * o2 * nx);
-      gm(m, o2 * nx, m, Li, m, false, XK[s], m, true, W, m, 1.0, 0.0, st, ws);
-      gm(nx, nx, m * o2, W, m * o2, true, W, m * o2, true, H, nx, -fk * sk, 1.0, st, ws);
+      if (nt) {
+        NtArgs a;
+        a.M = m; a.N = o2 * nx; a.K = m;
+        a.A = LiT; a.lda = m; a.B = XK[s]; a.ldb = m;
+        a.C = W; a.ldc = m;
+        gemm_nt(a, NT_F64, st, ws);
+        NtArgs h;
+        h.M = nx; h.N = nx; h.K = m * o2;
+        h.A = W; h.lda = m * o2; h.B = W; h.ldb = m * o2;
+        h.C = H; h.ldc = nx;
+        h.alpha = -fk * sk; h.beta = 1.0;
+        gemm_nt(h, NT_F64, st, ws);
+      } else {
+        gm(m, o2 * nx, m, Li, m, false, XK[s], m, true, W, m, 1.0, 0.0, st, ws);
+        gm(nx, nx, m * o2, W, m * o2, true, W, m * o2, true, H, nx, -fk * sk, 1.0, st, ws);
+      }
     }
+  }
   // ---- second-derivative-integral and metric terms ----
   hipLaunchKernelGGL(t1t2_kernel, dim3((unsigned)(9 * N * N)), dim3(256), 0, st, rpp, z, y, q, F, in.j2c_ipip1,
                      in.j2c_ip1ip2, d_aosl, d_auxsl, n, m, N, H);
@@ -628,12 +719,10 @@ void rijk_skeleton(Rijk& r, const SkeletonInput& in, double* de, double* const* 
         gm(n, o, n, F1, n, false, mo[s], n, true, out, n, sj, 1.0, st, ws, nx, n * n, 0, n * o);
       }
       if (doK) {
-        // S[u, v, t, A] = sum_{j, P in A} T~[u, j, P] g[v, j, P, t], added to the accumulator
-        for (int64_t A = 0; A < N; ++A) {
-          const int64_t q0 = in.auxslice[2 * A], mq = in.auxslice[2 * A + 1] - q0;
-          gm(n, n, o * mq, Tt[s] + n * o * q0, n, false, g[s] + n * o * q0, n, false, k1acc[s] + n * n * 3 * A, n,
-             1.0, 1.0, st, ws, 3, 0, n * o * m, n * n);
-        }
+        // S[u, v, t, A] = sum_{j, P in A} T~[u, j, P] g[v, j, P, t], added to the accumulator (the
+        // atoms as K slices (j, P), j fastest, of one launch)
+        gmg(n, n, o * m, Tt[s], n, false, g[s], n, false, k1acc[s], n, 1.0, 1.0, st, ws, 3, 0, n * o * m, n * n, N,
+            n * n * 3, d_ktab_fin + 2 * N * s, o * kmax_aux);
         hipLaunchKernelGGL(k1_final_kernel, GRID(n * n * nx), k1acc[s], r1[s], d_aoatom, n, N, F1);
         launch_check();
         gm(n, o, n, F1, n, false, mo[s], n, true, out, n, -fk * sk, 1.0, st, ws, nx, n * n, 0, n * o);

@@ -99,6 +99,33 @@ def test_uhf_skeleton(dev):
         assert rel(d1[s].cpu().numpy(), d1_o[s]) < 1e-10
 
 
+def test_skeleton_with_naux_multiple_of_32(dev):
+    """naux = 64 (a multiple of 32): the exchange first-derivative products run on the NT FP64
+    kernel; RHF and UHF against the oracle, several auxiliary ranges."""
+    import hesship
+    model = Model(nbas=(4, 3, 5), naux_per=(22, 18, 24), seed=11)
+    assert model.naux == 64
+    I, j3c_tp, cderi, int3c = setup(dev, model)
+    C, occ = model.orbitals(4, seed=12)
+    h = hesship.RHessRIJK.from_j3c(T(j3c_tp, dev), T(I["j2c"], dev), 1.0, 0.8)
+    de, d1 = h.make_skeleton(T(C, dev), occ, int3c, T(I["j2c_ip1"], dev), T(I["j2c_ipip1"], dev),
+                             T(I["j2c_ip1ip2"], dev), ranges(model.naux, [20, 30]), model.aoslices, model.auxslices)
+    de_o, d1_o = osk.skeleton_rhf(I, C, occ, 1.0, 0.8, model.aoslices, model.auxslices)
+    assert rel(de.cpu().numpy(), de_o) < 1e-10
+    assert rel(d1.cpu().numpy(), d1_o) < 1e-10
+    Cb, _ = model.orbitals(3, seed=13)
+    occa = np.where(np.arange(model.nao) < 4, 1.0, 0.0)
+    occb = np.where(np.arange(model.nao) < 3, 1.0, 0.0)
+    hu = hesship.UHessRIJK.from_j3c(T(j3c_tp, dev), T(I["j2c"], dev), 1.0, 0.6)
+    de, d1 = hu.make_skeleton([T(C, dev), T(Cb, dev)], [occa, occb], int3c, T(I["j2c_ip1"], dev),
+                              T(I["j2c_ipip1"], dev), T(I["j2c_ip1ip2"], dev), ranges(model.naux, [25]),
+                              model.aoslices, model.auxslices)
+    de_o, d1_o = osk.skeleton_uhf(I, [C, Cb], [occa, occb], 1.0, 0.6, model.aoslices, model.auxslices)
+    assert rel(de.cpu().numpy(), de_o) < 1e-10
+    for s in range(2):
+        assert rel(d1[s].cpu().numpy(), d1_o[s]) < 1e-10
+
+
 def test_skeleton_hessian_is_translation_invariant(dev):
     """The model integrals are translation invariant, so each row of the skeleton Hessian sums to
     zero over the atoms (the reference's test pattern, tests/hessian_rks_becke/test_b3lyp.rs:64-73)."""
