@@ -30,6 +30,9 @@ class Rijk {
   const double* j2c_l_inv() const { return j2c_linv_.ptr ? j2c_linv_.as<double>() : nullptr; }
   // B [nao, nao, naux] (cderi unpacked, FP64)
   const double* bfull() const { return bfull_.as<double>(); }
+  // device workspace of the skeleton call (rijk_skeleton.hip), kept between calls so a repeated
+  // call allocates nothing; released with the handle
+  std::shared_ptr<void>& skeleton_workspace() { return skel_ws_; }
 
   void prepare_response(const double* const* mo_coeff, const double* const* mo_occ, const int64_t* nmo);
   // resp[s] = scale_j * f_s * J_ao mocc_s + scale_k * K_s  (f = 1 RHF, 0.5 UHF)
@@ -111,6 +114,7 @@ class Rijk {
   DevBuf j2c_linv_;   // [naux, naux] J^-1/2 of the metric (from_j3c, HH_J2C_EIG)
   DevBuf cderi_;      // [nao_tp, naux] f64
   DevBuf bfull_;      // [nao, nao, naux] f64 (unpacked)
+  std::shared_ptr<void> skel_ws_;
   int64_t kpad_ = 0;
   DevBuf bp64_, bp32_, bp_hi_, bp_lo_;  // packed hot-GEMM A operand
   DevBuf tp_, tp_hi_, tp_lo_;           // packed hot-GEMM B operand (one perturbation chunk)

@@ -305,6 +305,12 @@ __global__ void transpose_sq_kernel(const double* X, int64_t m, double* T) {
 
 void launch_check() { HH_HIP(hipGetLastError()); }
 
+struct SkeletonWs {
+  DevBuf ws, bL, bLi, bmeta, bmo[2], bC[2], bD, bbD, bd, bBo[2], bbK[2], bdKij[2], bdKP[2], bTt[2], bg[2], bF, bXJ,
+      bXK[2], bG1, bj1, bk1[2], br1[2], brpp, bz, by, bq, bH, bI, bIT, bY, bGs, bHA, bM, bMp, bMpp, bE1, bW, bout,
+      bYu, bktab, bLiT, bCm[2];
+};
+
 }  // namespace
 
 void rijk_skeleton(Rijk& r, const SkeletonInput& in, double* de, double* const* deriv1_bra) {
@@ -355,9 +361,27 @@ void rijk_skeleton(Rijk& r, const SkeletonInput& in, double* de, double* const* 
     if (deriv1_bra) HH_REQUIRE(deriv1_bra[s], "skeleton: null deriv1_bra");
   }
 
-  DevBuf ws, bL, bLi, bmeta, bmo[2], bC[2], bD, bbD, bd, bBo[2], bbK[2], bdKij[2], bdKP[2], bTt[2], bg[2], bF, bXJ,
-      bXK[2], bG1, bj1, bk1[2], br1[2], brpp, bz, by, bq, bH, bI, bIT, bY, bGs, bHA, bM, bMp, bMpp, bE1, bW, bout,
-      bYu, bktab, bLiT;
+  // buffers: the handle's skeleton workspace (allocated on the first call, grown as needed)
+  std::shared_ptr<void>& slot = r.skeleton_workspace();
+  if (!slot) slot = std::make_shared<SkeletonWs>();
+  SkeletonWs& W_ = *static_cast<SkeletonWs*>(slot.get());
+  DevBuf &ws = W_.ws, &bL = W_.bL, &bLi = W_.bLi, &bmeta = W_.bmeta, &bD = W_.bD, &bbD = W_.bbD, &bd = W_.bd,
+         &bF = W_.bF, &bXJ = W_.bXJ, &bG1 = W_.bG1, &bj1 = W_.bj1, &brpp = W_.brpp, &bz = W_.bz, &by = W_.by,
+         &bq = W_.bq, &bH = W_.bH, &bI = W_.bI, &bIT = W_.bIT, &bY = W_.bY, &bGs = W_.bGs, &bHA = W_.bHA,
+         &bM = W_.bM, &bMp = W_.bMp, &bMpp = W_.bMpp, &bE1 = W_.bE1, &bW = W_.bW, &bout = W_.bout, &bYu = W_.bYu,
+         &bktab = W_.bktab, &bLiT = W_.bLiT;
+  DevBuf(&bmo)[2] = W_.bmo;
+  DevBuf(&bC)[2] = W_.bC;
+  DevBuf(&bBo)[2] = W_.bBo;
+  DevBuf(&bbK)[2] = W_.bbK;
+  DevBuf(&bdKij)[2] = W_.bdKij;
+  DevBuf(&bdKP)[2] = W_.bdKP;
+  DevBuf(&bTt)[2] = W_.bTt;
+  DevBuf(&bg)[2] = W_.bg;
+  DevBuf(&bXK)[2] = W_.bXK;
+  DevBuf(&bk1)[2] = W_.bk1;
+  DevBuf(&br1)[2] = W_.br1;
+  DevBuf(&bCm)[2] = W_.bCm;
   auto D = [](DevBuf& b, int64_t count) { return static_cast<double*>(b.reserve(sizeof(double) * std::max<int64_t>(1, count))); };
   auto zero = [&](double* p, int64_t count) { fill<double>(p, count, 0.0, st); };
 
@@ -509,7 +533,6 @@ void rijk_skeleton(Rijk& r, const SkeletonInput& in, double* de, double* const* 
   double* Gs = D(bGs, n * omax * nbmax * 3);
   double* HA = D(bHA, omax * omax * nbmax * 3 * N);
   double* Cm[2] = {nullptr, nullptr};
-  DevBuf bCm[2];
   if (doK)
     for (int s = 0; s < ns; ++s) {
       Cm[s] = D(bCm[s], n * no[s] * N);
@@ -559,7 +582,6 @@ void rijk_skeleton(Rijk& r, const SkeletonInput& in, double* de, double* const* 
   int64_t* d_ktab = static_cast<int64_t*>(bktab.reserve(sizeof(int64_t) * ktab.size()));
   const int64_t* d_ktab_fin = d_ktab + in.nrange * 6 * N;
   HH_HIP(hipMemcpyAsync(d_ktab, ktab.data(), sizeof(int64_t) * ktab.size(), hipMemcpyHostToDevice, st));
-  HH_HIP(hipStreamSynchronize(st));  // ktab is released before the device buffers
 
   for (int64_t ir = 0; ir < in.nrange; ++ir) {
     const int64_t p0 = in.aux_ranges[2 * ir], p1 = in.aux_ranges[2 * ir + 1], nb = p1 - p0;
