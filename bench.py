@@ -645,7 +645,8 @@ def workload_cphf_batch(args):
     solved to 1e-8 (one subspace of up to 42 cycles) on one created stream, plain FP64 with the
     response launched eagerly and mixed precision (BF16x3 operator, FP64 refinement) with the
     response replayed from HIP graphs; each leg runs once untimed first (buffers sized, products
-    tuned, graphs captured).  The molecules are handled one at a time (a 12-carbon system holds
+    tuned, graphs captured), and both again with the resident response operator (its matrix built
+    inside the timed region).  The molecules are handled one at a time (a 12-carbon system holds
     ~15 GB of workspace)."""
     import numpy as np
     import torch
@@ -655,10 +656,10 @@ def workload_cphf_batch(args):
     ns = np.random.default_rng(64).integers(1, 13, 64)
     stream = torch.cuda.Stream(device=dev)
     cfg = dict(cphf_max_space=42)
-    tot = dict(fp64=0.0, mixed=0.0)
-    cyc = dict(fp64=0, mixed=0)
+    tot = dict(fp64=0.0, mixed=0.0, fp64_resident=0.0, mixed_resident=0.0)
+    cyc = dict(fp64=0, mixed=0, fp64_resident=0, mixed_resident=0)
     caps = launches = 0
-    err = 0.0
+    err = err_res = 0.0
     nuhf = fallbacks = 0
     min_eig = []
 
@@ -680,17 +681,27 @@ def workload_cphf_batch(args):
                 occ = [np.where(np.arange(nao) < 4 * n + 1, 1.0, 0.0), np.where(np.arange(nao) < 4 * n, 1.0, 0.0)]
                 noc = [4 * n + 1, 4 * n]
 
-                def system(scale2):
-                    o = hesship.UHessRIJK(cd * scale2 ** 0.5, 1.0, 1.0)
+                def prep(o):
                     o.make_response_preparation([C, C], occ)
+
+                def system(scale2, resident=False):
+                    o = hesship.UHessRIJK(cd * scale2 ** 0.5, 1.0, 1.0)
+                    if resident:
+                        o.set_response_path("resident")
+                    prep(o)
                     return (o,) + ucphf_case(o, C, occ, nao, nprop, dev, cfg)
             else:
                 noc = 4 * n + 1
                 occ = np.where(np.arange(nao) < noc, 2.0, 0.0)
 
-                def system(scale2):
-                    o = hesship.RHessRIJK(cd * scale2 ** 0.5, 1.0, 1.0)
+                def prep(o):
                     o.make_response_preparation(C, occ)
+
+                def system(scale2, resident=False):
+                    o = hesship.RHessRIJK(cd * scale2 ** 0.5, 1.0, 1.0)
+                    if resident:
+                        o.set_response_path("resident")
+                    prep(o)
                     return (o,) + cphf_case(o, C, occ, nao, noc, nprop, dev, cfg)
             # |A v| ~ 0.3 |v| for random v, then shrunk where that leaves I + A indefinite
             ratio = system(1.0)[-1]
@@ -702,29 +713,47 @@ def workload_cphf_batch(args):
                 del obj, drv
                 obj, drv, _, f1, s1, _ = system(scale2)
             min_eig.append(round(max(lo, -LOWEST_A), 4))
+            # every leg's timed region holds prepare_response (for the resident operator that
+            # includes building its matrix on the first response), the rhs, the solve and finalize
             drv.set_graphs(False)
-            run_cphf(drv, f1, s1, "fp64")
-            ref, _, t64, st64 = run_cphf(drv, f1, s1, "fp64")
+            run_cphf(drv, f1, s1, "fp64", lambda: prep(obj))
+            ref, _, t64, st64 = run_cphf(drv, f1, s1, "fp64", lambda: prep(obj))
             drv.set_graphs(True)
-            run_cphf(drv, f1, s1, "bf16x3")
-            got, _, tmx, stmx = run_cphf(drv, f1, s1, "bf16x3")
+            run_cphf(drv, f1, s1, "bf16x3", lambda: prep(obj))
+            got, _, tmx, stmx = run_cphf(drv, f1, s1, "bf16x3", lambda: prep(obj))
             c, l = drv.graph_stats()
+            del drv, obj
+            robj, rdrv = system(scale2, resident=True)[:2]
+            rdrv.set_graphs(False)
+            run_cphf(rdrv, f1, s1, "fp64", lambda: prep(robj))
+            _, _, tr64, str64 = run_cphf(rdrv, f1, s1, "fp64", lambda: prep(robj))
+            rdrv.set_graphs(True)
+            run_cphf(rdrv, f1, s1, "bf16x3", lambda: prep(robj))
+            rgot, _, trmx, strmx = run_cphf(rdrv, f1, s1, "bf16x3", lambda: prep(robj))
             err = max(err, diff(got, ref))
+            err_res = max(err_res, diff(rgot, ref))
             tot["fp64"] += t64
             tot["mixed"] += tmx
+            tot["fp64_resident"] += tr64
+            tot["mixed_resident"] += trmx
             cyc["fp64"] += st64["cycles"]
             cyc["mixed"] += stmx["cycles"]
-            fallbacks += stmx.get("fp64_fallback", 0)
+            cyc["fp64_resident"] += str64["cycles"]
+            cyc["mixed_resident"] += strmx["cycles"]
+            fallbacks += stmx.get("fp64_fallback", 0) + strmx.get("fp64_fallback", 0)
             caps += c
             launches += l
             log(f"[bench] molecule {i}: C{n} {'UHF' if uhf else 'RHF'} nao {nao} nprop {nprop}: FP64 {st64['cycles']} "
                 f"cycles {t64 * 1e3:.1f} ms, mixed {stmx['cycles']} cycles / {stmx['refinements']} sweeps "
-                f"{' (FP64 finish)' if stmx.get('fp64_fallback') else ''} {tmx * 1e3:.1f} ms")
-            del drv, obj, cd, C, f1, s1, ref, got
+                f"{' (FP64 finish)' if stmx.get('fp64_fallback') else ''} {tmx * 1e3:.1f} ms; resident FP64 "
+                f"{tr64 * 1e3:.1f} ms, mixed {strmx['cycles']} cycles {trmx * 1e3:.1f} ms")
+            del rdrv, robj, cd, C, f1, s1, ref, got, rgot
             torch.cuda.synchronize()
-    wall = tot["mixed"]
-    log(f"[bench] cphf batch: FP64 {tot['fp64']:.3f} s, mixed {wall:.3f} s")
-    line = _line("CPHF batch wall s (64 uneven RI-JK molecules, RHF + UHF, mixed precision, graph replay)",
+    wall = tot["mixed_resident"]
+    log(f"[bench] cphf batch: FP64 {tot['fp64']:.3f} s, mixed {tot['mixed']:.3f} s, resident FP64 "
+        f"{tot['fp64_resident']:.3f} s, resident mixed {wall:.3f} s")
+    line = _line("CPHF batch wall s (64 uneven RI-JK molecules, RHF + UHF, mixed precision, resident response, "
+                 "graph replay)",
                  round(wall, 4), "s", wall * 1e3, 1, 1, "bf16x3 operator, f64 refinement",
                  {"workload": "64 alkane-sized synthetic RI-JK CPHF systems (32 closed shells, 32 doublet cations) "
                               "solved one after the other on one stream",
@@ -734,7 +763,12 @@ def workload_cphf_batch(args):
     line["uhf_molecules"] = nuhf
     line["min_eig_A"] = min_eig
     line["fp64_eager_wall_s"] = round(tot["fp64"], 4)
-    line["speedup_mixed_vs_fp64"] = round(tot["fp64"] / wall, 3)
+    line["mixed_per_call_wall_s"] = round(tot["mixed"], 4)
+    line["fp64_resident_wall_s"] = round(tot["fp64_resident"], 4)
+    line["speedup_vs_fp64"] = {"mixed_per_call": round(tot["fp64"] / tot["mixed"], 3),
+                               "fp64_resident": round(tot["fp64"] / tot["fp64_resident"], 3),
+                               "mixed_resident": round(tot["fp64"] / wall, 3)}
+    line["max_abs_mo1_diff_mixed_resident_vs_fp64"] = err_res
     line["krylov_cycles"] = cyc
     line["graphs_captured"] = caps
     line["graph_launches"] = launches

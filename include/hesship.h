@@ -119,12 +119,13 @@ hh_status hh_rijk_set_batch(hh_rijk* h, int64_t nprop_batch);
  * K loop (hess_r.rs:144-183).  FP64 / FP32 are unaffected. */
 hh_status hh_rijk_set_k_path(hh_rijk* h, int32_t path);
 /* Response algorithm of get_response_bra: 0 (default) per call, the reference's algebra
- * (hess_r.rs:82-186); 1 resident (RHF, FP64 and BF16x3): the response matrix
+ * (hess_r.rs:82-186); 1 resident (RHF and UHF, FP64 and BF16x3): the response matrix
  * M[(mu,i),(nu,j)] = 4 scale_j sum_P X_P[mu,i] X_P[nu,j] - scale_k sum_P (B_P[mu,nu] O_P[j,i] +
- * X_P[mu,j] X_P[nu,i]) (X_P = B_P C_o, O_P = C_o^T X_P) is built on the first call after
- * prepare_response and each call is resp = M bra.  Falls back to the per-call algorithm when M
- * would need more than budget_bytes (<= 0: keep the current budget, 24 GiB by default) or for
- * UHF / FP32. */
+ * X_P[mu,j] X_P[nu,i]) (X_P = B_P C_o, O_P = C_o^T X_P; UHF: rows and columns over both spins,
+ * J weight 2 scale_j between every pair of spins, K within each spin) is built on the first call
+ * after prepare_response and each call is resp = M bra.  Falls back to the per-call algorithm
+ * when M would need more than budget_bytes (<= 0: keep the current budget, 24 GiB by default)
+ * or for FP32. */
 hh_status hh_rijk_set_response_path(hh_rijk* h, int32_t path, int64_t budget_bytes);
 hh_status hh_rijk_set_profiling(hh_rijk* h, int32_t on);
 hh_status hh_rijk_get_timing(const hh_rijk* h, double* out5);

@@ -689,7 +689,7 @@ void Rijk::get_response_bra(const double* const* bra, int64_t nprop, double* con
   const bool do_j = scale_j_ != 0.0, do_k = scale_k_ != 0.0;
   const int r0 = profiling_ ? mark() : -1;
   if (response_path == 1 && resident_fits(prec)) {
-    apply_resident(bra[0], nprop, resp[0], prec, accumulate);
+    apply_resident(bra, nprop, resp, prec, accumulate);
     if (profiling_) {
       span(T_RESPONSE, r0);
       flush_timing();

@@ -60,9 +60,9 @@ class Rijk {
   // materialised operand (T generation + NT contraction).
   int k_path = 0;
   // Response algorithm: 0 = per call (the reference's algebra, default); 1 = resident response
-  // matrix (RHF, FP64 and BF16x3; rijk_resident.hip): M [nao nocc, nao nocc] built on the first
-  // call after prepare_response and kept, each call one product with M.  Falls back to the
-  // per-call algorithm when M would exceed resident_budget bytes or for UHF / FP32.
+  // matrix (RHF and UHF, FP64 and BF16x3; rijk_resident.hip): M [R, R], R = nao sum_s nocc_s,
+  // built on the first call after prepare_response and kept, each call one product with M.
+  // Falls back to the per-call algorithm when M would exceed resident_budget bytes or for FP32.
   int response_path = 0;
   int64_t resident_budget = 24LL << 30;
   bool resident_fits(Prec prec) const;
@@ -84,7 +84,7 @@ class Rijk {
   void build_k_fused(int s, const double* bra, int64_t nprop, const unsigned short* wp, double alpha, double beta,
                      double* out);
   void build_resident(Prec prec);
-  void apply_resident(const double* bra, int64_t nprop, double* resp, Prec prec, bool accumulate);
+  void apply_resident(const double* const* bra, int64_t nprop, double* const* resp, Prec prec, bool accumulate);
   // Packed hot-GEMM operand Bp[u][P][mu_pad] (mu_pad = kpad_ = nao rounded up to 32, zeros
   // beyond nao): FP64 copy and/or bf16 hi/lo split, built once per handle.
   void ensure_packed(Prec prec);

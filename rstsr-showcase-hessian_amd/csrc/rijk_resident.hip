@@ -1,4 +1,4 @@
-// Resident RHF RI-JK response operator.
+// Resident RI-JK response operator.
 //
 // Per perturbation a the RHF response (hess_r.rs:1916-1953: resp = scale_j J_ao mocc + scale_k k_bra,
 // k_bra = -K) is linear in bra and, with X_P = B_P C_o and O_P = C_o^T B_P C_o, reads
@@ -7,16 +7,20 @@
 //   G[(mu,i),(nu,j)]  = sum_P X_P[mu,i] X_P[nu,j]          (the J term),
 //   V[(mu,i),(nu,j)]  = sum_P B_P[mu,nu] O_P[j,i]           (K: sum_P B_P bra_a O_P),
 //   G~[(mu,i),(nu,j)] = sum_P X_P[mu,j] X_P[nu,i]           (K: sum_P X_P (bra_a^T X_P)).
-// M [nao nocc, nao nocc] depends only on the orbitals: it is built once per prepare_response
-// (two products of 2 (nao nocc)^2 naux flop) and kept in HBM, after which each response call is
-// one product of 2 (nao nocc)^2 nprop flop that streams M once (HBM-bound).  The per-call work
-// falls from O(nao^2 naux nocc nprop) to O(nao^2 nocc^2 nprop); the memory is (nao nocc)^2
+// UHF (hess_u.rs:270-313: r_s = 1/2 scale_j J_ao(sum_t dm1_t) mocc_s + scale_k k_bra_s): the rows
+// and columns run over both spins, (mu, i, s) with the alpha block first, and
+//   M[s, t] = 2 scale_j G^{st} - [s == t] scale_k (V^s + G~^s),  G^{st} = sum_P X^s_P (x) X^t_P:
+// J couples the spins, K does not.
+// M [R, R] (R = nao sum_s nocc_s) depends only on the orbitals: it is built once per
+// prepare_response (products of 2 R^2 naux flop) and kept in HBM, after which each response call
+// is one product of 2 R^2 nprop flop per spin block of rows that streams M once (HBM-bound).  The
+// per-call work falls from O(nao^2 naux nocc nprop) to O(nao^2 nocc^2 nprop); the memory is R^2
 // elements (c.f. 288 GB of HBM).
 //
-// Build: G is one NT product of X arranged as Xr[(mu,i)][P] with itself (M = N = nao nocc,
-// K = naux); V needs only the packed triangles of its two symmetric index pairs,
-// Vtp[(mu<=nu), (i<=j)] = sum_P B_P[mu,nu] O_P[i,j], a quarter of the full product's flops, and is
-// unpacked over (mu, nu) before the assembly, which reads all three terms coalesced (kk fastest).
+// Build: G is one NT product of X arranged as Xr[(mu,i,s)][P] with itself (M = N = R, K = naux);
+// V^s needs only the packed triangles of its two symmetric index pairs,
+// Vtp[(mu<=nu), (i<=j)] = sum_P B_P[mu,nu] O^s_P[i,j], a quarter of the full product's flops, and
+// is unpacked over (mu, nu) before the assembly, which reads all terms coalesced (kk fastest).
 #include "elementwise.hpp"
 #include "gemm_nt.hpp"
 #include "rijk.hpp"
@@ -66,42 +70,31 @@ __global__ __launch_bounds__(256) void transpose_x_kernel(const double* __restri
   }
 }
 
-// M[(mu,i),(nu,j)] = cg G[r + R kk] - cv (Vu[(nu + nao mu) + nao^2 tp(i,j)] + G[(nu + nao i) + R (mu + nao j)])
-// with r = mu + nao i, kk = nu + nao j (Vu: the (mu, nu)-unpacked Vtp).  Output BF16: tiled hi / lo
-// [(kk / 32) R + r][32] (kk < kpad, zero past R); FP64: row-major [r][kpad].  One thread per output
-// element, kk fastest.
-template <typename T, bool BF>
-__global__ void assemble_m_kernel(const T* __restrict__ G, const T* __restrict__ Vp, int64_t nao, int64_t nocc,
-                                  int64_t kpad, double cg, double cv, unsigned short* __restrict__ hi,
-                                  unsigned short* __restrict__ lo, double* __restrict__ m64) {
-  const int64_t R = nao * nocc, n2 = nao * nao;
+// M[r][kk] (FP64 row-major [R][kpad], zero for kk >= R): r = (mu, i, s), kk = (nu, j, t) with the
+// alpha block first (Ra rows); cg G[kk + R r] - [s == t] cv (Vu_s[(nu + nao mu) + nao^2 tp(i, j)]
+// + G[(nu + nao i + off_s) + R (mu + nao j + off_s)]) (Vu_s: the (mu, nu)-unpacked Vtp of spin s).
+// One thread per output element, kk fastest.
+__global__ void assemble_m_kernel(const double* __restrict__ G, const double* __restrict__ Vu0,
+                                  const double* __restrict__ Vu1, int64_t nao, int64_t Ra, int64_t R, int64_t kpad,
+                                  double cg, double cv, double* __restrict__ m64) {
+  const int64_t n2 = nao * nao;
   const int64_t total = R * kpad;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    int64_t r, kk;
-    if (BF) {
-      const int64_t kl = e % 32, rest = e / 32;
-      r = rest % R;
-      kk = (rest / R) * 32 + kl;
-    } else {
-      kk = e % kpad;
-      r = e / kpad;
-    }
+    const int64_t kk = e % kpad, r = e / kpad;
     double v = 0.0;
     if (kk < R) {
-      const int64_t mu = r % nao, i = r / nao, nu = kk % nao, j = kk / nao;
-      const double g = (double)G[kk + R * r];
-      const double gt = (double)G[(nu + nao * i) + R * (mu + nao * j)];
-      const double vv = (double)Vp[(nu + nao * mu) + n2 * tp_index(i, j)];
-      v = cg * g - cv * (vv + gt);
+      const int s = r >= Ra, t = kk >= Ra;
+      const int64_t off_s = s ? Ra : 0, off_t = t ? Ra : 0;
+      const int64_t rl = r - off_s, kl = kk - off_t;
+      v = cg * G[kk + R * r];
+      if (s == t) {
+        const int64_t mu = rl % nao, i = rl / nao, nu = kl % nao, j = kl / nao;
+        const double gt = G[(nu + nao * i + off_s) + R * (mu + nao * j + off_s)];
+        const double vv = (s ? Vu1 : Vu0)[(nu + nao * mu) + n2 * tp_index(i, j)];
+        v -= cv * (vv + gt);
+      }
     }
-    if (BF) {
-      const float x = (float)v;
-      const __bf16 h = (__bf16)x;
-      hi[e] = __builtin_bit_cast(unsigned short, h);
-      lo[e] = __builtin_bit_cast(unsigned short, (__bf16)(x - (float)h));
-    } else {
-      m64[e] = v;
-    }
+    m64[e] = v;
   }
 }
 
@@ -127,40 +120,43 @@ __global__ void split_m_kernel(const double* __restrict__ m64, int64_t R, int64_
   }
 }
 
-// dst[(a) kpad + k] = src[k + R a] (k < R), 0 for R <= k < kpad
-__global__ void pad_rows_kernel(const double* __restrict__ src, int64_t R, int64_t kpad, int64_t n,
-                                double* __restrict__ dst) {
+// dst[a kpad + k] = bra_0[k + Ra a] (k < Ra), bra_1[(k - Ra) + Rb a] (Ra <= k < Ra + Rb), 0 up to kpad:
+// the spins' bra stacked along K
+__global__ void stack_rows_kernel(const double* __restrict__ b0, const double* __restrict__ b1, int64_t Ra, int64_t Rb,
+                                  int64_t kpad, int64_t n, double* __restrict__ dst) {
   const int64_t total = kpad * n;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t k = e % kpad, a = e / kpad;
-    dst[e] = k < R ? src[k + R * a] : 0.0;
+    dst[e] = k < Ra ? b0[k + Ra * a] : (k < Ra + Rb ? b1[(k - Ra) + Rb * a] : 0.0);
   }
 }
 
 }  // namespace
 
 bool Rijk::resident_fits(Prec prec) const {
-  if (nspin_ != 1 || !prepared_ || prec == PREC_F32) return false;
-  const double R = (double)nao_ * (double)nocc_[0];
+  if (!prepared_ || prec == PREC_F32) return false;
+  double R = 0.0;
+  for (int t = 0; t < nspin_; ++t) R += (double)nao_ * (double)nocc_[t];
   // the FP64 matrix, the two FP64 product intermediates while it is assembled, and the BF16x3 copy
   const double bytes = 3.0 * 8.0 * R * R + (prec == PREC_BF16X3 ? 4.0 * R * R : 0.0);
   return bytes <= (double)resident_budget;
 }
 
 void Rijk::build_resident(Prec prec) {
-  HH_REQUIRE(nspin_ == 1, "resident response: RHF handles only");
   HH_REQUIRE(prec == PREC_F64 || prec == PREC_BF16X3, "resident response: FP64 or BF16x3");
-  const int64_t nocc = nocc_[0], R = nao_ * nocc, kpad = ceil_div(R, 32) * 32;
+  const int64_t Ra = nao_ * nocc_[0], R = Ra + (nspin_ > 1 ? nao_ * nocc_[1] : 0), kpad = ceil_div(R, 32) * 32;
   if (!res_ready_[0]) {
     // FP64 matrix: the products in FP64 (the matrix is exact to FP64 reassociation)
-    const int64_t ldk = ceil_div(naux_, 32) * 32, ntp = nocc * (nocc + 1) / 2;
-    DevBuf G, Vp, Xr, O2t, Vtp;
+    const int64_t ldk = ceil_div(naux_, 32) * 32;
+    DevBuf G, Vp[2], Xr, O2t, Vtp;
     double* g = static_cast<double*>(G.reserve(sizeof(double) * R * R));
-    // G[(mu,i),(nu,j)] = sum_P Xr[(mu,i)][P] Xr[(nu,j)][P]: one NT product, K = naux padded to 32
+    // G[(mu,i,s),(nu,j,t)] = sum_P Xr[(mu,i,s)][P] Xr[(nu,j,t)][P]: one NT product, K = naux padded to 32
     {
       double* xr = static_cast<double*>(Xr.reserve(sizeof(double) * R * ldk));
-      hipLaunchKernelGGL(transpose_x_kernel, dim3((unsigned)ceil_div(nao_, 64), (unsigned)ceil_div(ldk, 64), (unsigned)nocc),
-                         dim3(256), 0, stream_, x_[0].as<double>(), nao_, naux_, ldk, xr);
+      for (int t = 0; t < nspin_; ++t)
+        hipLaunchKernelGGL(transpose_x_kernel,
+                           dim3((unsigned)ceil_div(nao_, 64), (unsigned)ceil_div(ldk, 64), (unsigned)nocc_[t]),
+                           dim3(256), 0, stream_, x_[t].as<double>(), nao_, naux_, ldk, xr + (t ? Ra : 0) * ldk);
       HH_HIP(hipGetLastError());
       NtArgs a;
       a.M = R; a.N = R; a.K = ldk;
@@ -169,12 +165,14 @@ void Rijk::build_resident(Prec prec) {
       a.ksplit = 1;
       gemm_nt(a, NT_F64, stream_, ws_);
     }
-    // Vtp[(mu<=nu), (i<=j)] = sum_P cderi[(mu,nu), P] O2t[P, (i,j)], then unpacked over (mu, nu)
-    double* vp = static_cast<double*>(Vp.reserve(sizeof(double) * nao_ * nao_ * ntp));
-    {
+    // per spin: Vtp[(mu<=nu), (i<=j)] = sum_P cderi[(mu,nu), P] O2t[P, (i,j)], then unpacked over (mu, nu)
+    double* vp[2] = {nullptr, nullptr};
+    for (int t = 0; t < nspin_; ++t) {
+      const int64_t nocc = nocc_[t], ntp = nocc * (nocc + 1) / 2;
+      vp[t] = static_cast<double*>(Vp[t].reserve(sizeof(double) * nao_ * nao_ * ntp));
       double* o2 = static_cast<double*>(O2t.reserve(sizeof(double) * ntp * naux_));
       double* vtp = static_cast<double*>(Vtp.reserve(sizeof(double) * nao_tp_ * ntp));
-      hipLaunchKernelGGL(pack_o_tp_kernel, dim3(rblocks(ntp * naux_)), dim3(256), 0, stream_, o_[0].as<double>(), nocc,
+      hipLaunchKernelGGL(pack_o_tp_kernel, dim3(rblocks(ntp * naux_)), dim3(256), 0, stream_, o_[t].as<double>(), nocc,
                          naux_, o2);
       HH_HIP(hipGetLastError());
       GemmArgs a;
@@ -183,11 +181,13 @@ void Rijk::build_resident(Prec prec) {
       a.B = o2; a.ldb = ntp; a.b_kmajor = false;
       a.C = vtp; a.ldc = nao_tp_;
       gemm<double, double>(a, PREC_F64, stream_, ws_);
-      unpack_triu<double, double>(vtp, nao_, nao_tp_, ntp, vp, stream_);
+      unpack_triu<double, double>(vtp, nao_, nao_tp_, ntp, vp[t], stream_);
     }
     res64_.reserve(sizeof(double) * R * kpad);
-    hipLaunchKernelGGL((assemble_m_kernel<double, false>), dim3(rblocks(R * kpad)), dim3(256), 0, stream_, g, vp, nao_,
-                       nocc, kpad, 4.0 * scale_j_, scale_k_, nullptr, nullptr, res64_.as<double>());
+    // J weight: 4 sj (RHF, doubly occupied) or 2 sj (UHF: 1/2 of J from both spins' dm1)
+    const double cg = (nspin_ == 1 ? 4.0 : 2.0) * scale_j_;
+    hipLaunchKernelGGL(assemble_m_kernel, dim3(rblocks(R * kpad)), dim3(256), 0, stream_, g, vp[0],
+                       nspin_ > 1 ? vp[1] : vp[0], nao_, Ra, R, kpad, cg, scale_k_, res64_.as<double>());
     HH_HIP(hipGetLastError());
     HH_HIP(hipStreamSynchronize(stream_));  // the intermediates are freed on return
     res_ready_[0] = true;
@@ -202,36 +202,45 @@ void Rijk::build_resident(Prec prec) {
   }
 }
 
-void Rijk::apply_resident(const double* bra, int64_t nprop, double* resp, Prec prec, bool accumulate) {
+void Rijk::apply_resident(const double* const* bra, int64_t nprop, double* const* resp, Prec prec,
+                          bool accumulate) {
   const bool bf = prec == PREC_BF16X3;
   build_resident(prec);
-  const int64_t R = nao_ * nocc_[0], kpad = ceil_div(R, 32) * 32;
-  NtArgs a;
-  a.M = R; a.N = nprop; a.K = kpad;
-  a.C = resp; a.ldc = R;
-  a.alpha = 1.0; a.beta = accumulate ? 1.0 : 0.0;
-  if (profiling_) {
-    const int e0 = next_event_++, e1 = next_event_++;
-    a.ev_begin = ev(e0);
-    a.ev_main_end = ev(e1);
-    pending_.push_back({T_K_HOT, {e0, e1}});
-  }
+  const int64_t Ra = nao_ * nocc_[0], Rb = nspin_ > 1 ? nao_ * nocc_[1] : 0, R = Ra + Rb;
+  const int64_t kpad = ceil_div(R, 32) * 32;
+  // the spins' bra stacked along K (FP64, zero-padded to kpad)
+  const size_t b64 = sizeof(double) * kpad * nprop;
+  double* b = static_cast<double*>(res_b_.reserve(b64 + (bf ? 2 * sizeof(unsigned short) * kpad * nprop : 0)));
+  hipLaunchKernelGGL(stack_rows_kernel, dim3(rblocks(kpad * nprop)), dim3(256), 0, stream_, bra[0],
+                     nspin_ > 1 ? bra[1] : bra[0], Ra, Rb, kpad, nprop, b);
+  HH_HIP(hipGetLastError());
+  unsigned short* bh = nullptr;
   if (bf) {
-    unsigned short* b = static_cast<unsigned short*>(res_b_.reserve(2 * sizeof(unsigned short) * kpad * nprop));
-    split_bf16_tiled<double>(bra, R, R, nprop, kpad, b, b + kpad * nprop, stream_);
-    a.A = res_hi_.ptr; a.A_lo = res_lo_.ptr;
-    a.B = b; a.B_lo = b + kpad * nprop;
-    a.lda = a.ldb = kpad;
-    a.tiled = true;
-    gemm_nt(a, NT_BF16X3, stream_, ws_);
-  } else {
-    double* b = static_cast<double*>(res_b_.reserve(sizeof(double) * kpad * nprop));
-    hipLaunchKernelGGL(pad_rows_kernel, dim3(rblocks(kpad * nprop)), dim3(256), 0, stream_, bra, R, kpad, nprop, b);
-    HH_HIP(hipGetLastError());
-    a.A = res64_.ptr; a.lda = kpad;
-    a.B = b; a.ldb = kpad;
-    gemm_nt(a, NT_F64, stream_, ws_);
+    bh = reinterpret_cast<unsigned short*>(reinterpret_cast<char*>(b) + b64);
+    split_bf16_tiled<double>(b, kpad, kpad, nprop, kpad, bh, bh + kpad * nprop, stream_);
   }
+  const int e0 = profiling_ ? mark() : -1;
+  // one product per spin block of rows of M
+  for (int s = 0; s < nspin_; ++s) {
+    const int64_t off = s ? Ra : 0, Rs = s ? Rb : Ra;
+    NtArgs a;
+    a.M = Rs; a.N = nprop; a.K = kpad;
+    a.C = resp[s]; a.ldc = Rs;
+    a.alpha = 1.0; a.beta = accumulate ? 1.0 : 0.0;
+    if (bf) {
+      a.A = res_hi_.as<unsigned short>() + off * 32; a.A_lo = res_lo_.as<unsigned short>() + off * 32;
+      a.a_rows = R;
+      a.B = bh; a.B_lo = bh + kpad * nprop;
+      a.lda = a.ldb = kpad;
+      a.tiled = true;
+      gemm_nt(a, NT_BF16X3, stream_, ws_);
+    } else {
+      a.A = res64_.as<double>() + off * kpad; a.lda = kpad;
+      a.B = b; a.ldb = kpad;
+      gemm_nt(a, NT_F64, stream_, ws_);
+    }
+  }
+  if (profiling_) pending_.push_back({T_K_HOT, {e0, mark()}});
 }
 
 }  // namespace hh

@@ -97,8 +97,8 @@ class _RIJKBase:
 
     def set_response_path(self, path, budget_bytes=0):
         """Response algorithm: "direct" (per call, the reference's algebra; default) or "resident"
-        (RHF, FP64 / BF16x3: the response matrix built once per prepare_response, one product
-        per call; falls back to direct beyond budget_bytes)."""
+        (RHF and UHF, FP64 / BF16x3: the response matrix built once per prepare_response, one
+        product per call and spin; falls back to direct beyond budget_bytes)."""
         check(lib().hh_rijk_set_response_path(self._h, {"direct": 0, "resident": 1}[path], int(budget_bytes)))
 
     def set_profiling(self, on=True):
