@@ -178,6 +178,12 @@ __global__ void rho1_wv_s_tiled_kernel(const double* aob, const float* akb, unsi
 // LDS; rho1, the weighted kernel and the tiled S operand follow with the arithmetic of
 // rho1_wv_s_tiled_kernel, so S is bit-identical to the unfused path's and the FP32 akb array
 // (0.5 GB per batch at 72 perturbations) never reaches HBM.
+// workgroup w runs on XCD w % 8: renumber so that consecutive indices share an XCD
+__device__ __forceinline__ int64_t ni_xcd_remap(int64_t w, int64_t nwg) {
+  const int64_t q = nwg / 8, r = nwg % 8, x = w % 8;
+  return x * q + (x < r ? x : r) + w / 8;
+}
+
 typedef float v4f_ni __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf_ni __attribute__((ext_vector_type(8)));
 
@@ -196,8 +202,12 @@ __global__ __launch_bounds__(128) void kets_rho1_s_kernel(const unsigned short* 
   __shared__ float akbs[NC][32 * LDA];
   __shared__ double wvs[32 * 8 * NV];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int64_t gt0 = (int64_t)blockIdx.x * 32;  // first point of the tile (batch-local)
-  const int64_t a0 = (int64_t)blockIdx.y * na;
+  // one-dimensional grid, perturbation tiles fastest after an XCD-aware remap: the tiles of one
+  // 32-point block run together on one XCD, so its AO values come from HBM once and from L2 after
+  const int64_t nwg = gridDim.x, nty = (nprop + na - 1) / na;
+  const int64_t lin = ni_xcd_remap(blockIdx.x, nwg);
+  const int64_t gt0 = (lin / nty) * 32;  // first point of the tile (batch-local)
+  const int64_t a0 = (lin % nty) * na;
   const int nat = (int)min((int64_t)na, nprop - a0);
   const int64_t n0 = nocc * a0;
   const int nct = (int)(nocc * nat);  // columns of this tile
@@ -542,8 +552,10 @@ Numint::Numint(const double* ao, int64_t ng, int64_t nao, int64_t ncomp, const d
 
 int64_t Numint::pick_batch(int64_t ncol) const {
   if (batch > 0) return std::min(batch, ng_);
+  // about 8 GB of per-batch intermediates: few batches, so the back-contraction's split-K
+  // partial sums are written a few times per call rather than once per small batch
   const int64_t per_point = 8 * (ncol + 64) * 4;
-  int64_t gb = (int64_t)(1LL << 30) / std::max<int64_t>(per_point, 1);
+  int64_t gb = (int64_t)(1LL << 33) / std::max<int64_t>(per_point, 1);
   gb = std::max<int64_t>(256, (gb / 256) * 256);
   return std::min(gb, ng_);
 }
@@ -636,9 +648,12 @@ void launch_kets_rho1_s_den(int nfr, dim3 grid, hipStream_t s, const unsigned sh
                             int nkb, const double* aob, const double* fxc, const double* w, unsigned short* hi,
                             unsigned short* lo, int64_t cs, int64_t gb, int64_t nocc, int64_t nprop, int na, int64_t ng,
                             int64_t g0) {
+  // grid: (point tiles, perturbation tiles) launched as one dimension (see the kernel)
+  const dim3 g1((unsigned)((int64_t)grid.x * grid.y));
+  HH_REQUIRE((int64_t)grid.x * grid.y < (1LL << 31), "numint: fused grid too large");
 #define HH_KRS(N)                                                                                                   \
   case N:                                                                                                           \
-    hipLaunchKernelGGL((kets_rho1_s_kernel<DEN, N>), grid, dim3(128), 0, s, aoth, aotl, nt, ngp, kh, kl, ncol, nkb, \
+    hipLaunchKernelGGL((kets_rho1_s_kernel<DEN, N>), g1, dim3(128), 0, s, aoth, aotl, nt, ngp, kh, kl, ncol, nkb,   \
                        aob, fxc, w, hi, lo, cs, gb, nocc, nprop, na, ng, g0);                                       \
     break;
   switch (nfr) {
