@@ -17,7 +17,8 @@ values), `dft_setup` (DFT skeleton setup passes at the same sizes, FP64), `dft_b
 setup with the Becke grid shift on an atom-grouped grid, FP64), `skeleton` (RI-JK skeleton
 Hessian and first-derivative terms on synthetic integrals, FP64), `cphf_batch` (64 uneven
 alkane-sized RI-JK CPHF solves, half of them doublet cations (UHF), on one stream: plain FP64 vs
-mixed precision with HIP-graph replay of the response).
+mixed precision with HIP-graph replay of the response), `xc` (B3LYP exc, vxc_eff and fxc_eff on a
+synthetic 543 600-point density, closed shell and spin polarised, FP64).
 """
 import argparse
 import json
@@ -884,6 +885,77 @@ def workload_skeleton(args):
                  higher=False)
 
 
+def synthetic_density(ng, nspin, seed):
+    """Densities 1e-5 .. 1e2 with gradients up to a few times rho (molecular-grid-like), [ng, 4(, 2)]."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+
+    def one():
+        r = np.empty((ng, 4))
+        r[:, 0] = 10 ** rng.uniform(-5, 2, ng)
+        r[:, 1:] = rng.normal(size=(ng, 3)) * r[:, :1] * rng.uniform(0.05, 3.0, (ng, 1))
+        return r
+    return one() if nspin == 1 else np.stack([one(), one()], -1)
+
+
+XC_BYTES = {1: 8 * (4 + 1 + 4 + 16), 2: 8 * (8 + 1 + 8 + 64)}   # rho in; exc, vxc_eff, fxc_eff out
+# FP64 operations executed per grid point by xc_eff_kernel for B3LYP (rocprofv3 --pmc
+# SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64, 64 lanes x (ADD + MUL + 2 FMA + TRANS) per dispatch / ngrid)
+XC_FLOP_PER_POINT = {1: 4690.1, 2: 7525.2}
+PEAK_FP64_VALU = 78.6
+
+
+def workload_xc(args):
+    """B3LYP effective kernels (get_rho_vxc_fxc's libxc_eval_eff loop, hess_rks.rs:205-216 and
+    hess_uks.rs:96-106) on the 543 600-point grid: one step = exc, vxc_eff, fxc_eff for the
+    closed-shell and the spin-polarised density (outputs allocated once, outside the timed region)."""
+    import torch
+    import hesship
+    from hesship.tensors import empty_f
+    from hesship.xc import _accumulate_eff, _eff_shapes
+    from oracle import xcfunc as oxf
+    ng = C3["ng"]
+    dev = torch.device("cuda:0")
+    rho = {s: torch.from_numpy(synthetic_density(ng, s, 0x5EED0007 + s)).to(dev) for s in (1, 2)}
+    fn = {1: hesship.LibXCFunctional.from_identifier("HYB_GGA_XC_B3LYP", "unpolarized"),
+          2: hesship.LibXCFunctional.from_identifier("HYB_GGA_XC_B3LYP", "polarized")}
+    outs = {s: [empty_f(sh, dev) for sh in _eff_shapes(ng, 4, s)] for s in (1, 2)}
+    log("[bench] xc B3LYP effective kernels")
+    ms = {}
+    for s in (1, 2):
+        _, ms[s] = _timed(lambda: _accumulate_eff(fn[s], rho[s], "sigma", 1.0, *outs[s], False), args.steps,
+                          args.warmup)
+    tot = ms[1] + ms[2]
+    value = 2 * ng / (tot * 1e-3) / 1e6
+    gbs = (XC_BYTES[1] + XC_BYTES[2]) * ng / (tot * 1e-3) / 1e9
+    tfl = {s: XC_FLOP_PER_POINT[s] * ng / (ms[s] * 1e-3) / 1e12 for s in (1, 2)}
+    line = _line("XC functional kernels Mpoint/s (B3LYP exc + vxc_eff + fxc_eff, RKS and UKS)", round(value, 2),
+                 "Mpoint/s", tot, args.steps, args.warmup, "f64",
+                 {"workload": "B3LYP effective kernels on a synthetic 543600-point density", "ngrid": ng,
+                  "functional": "HYB_GGA_XC_B3LYP", "spins": [1, 2]},
+                 {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
+                  "frac": round(gbs / 8000.0, 4), "traffic": None,
+                  "kernel": "xc_eff_kernel (rho in, exc/vxc_eff/fxc_eff out)"})
+    # the kernel is bound by FP64 vector arithmetic, not by HBM
+    line["compute"] = {"bound": "fp64 valu", "unit": "TFLOP/s", "peak": PEAK_FP64_VALU,
+                       "achieved_rks": round(tfl[1], 2), "achieved_uks": round(tfl[2], 2),
+                       "frac_rks": round(tfl[1] / PEAK_FP64_VALU, 3), "frac_uks": round(tfl[2] / PEAK_FP64_VALU, 3),
+                       "flop_per_point": XC_FLOP_PER_POINT}
+    line["ms_rks"] = round(ms[1], 4)
+    line["ms_uks"] = round(ms[2], 4)
+    if not args.no_cpu_baseline:
+        nsamp = 20000
+        t0 = time.perf_counter()
+        for s in (1, 2):
+            oxf.eval_eff(402, rho[s][:nsamp].cpu().numpy(), s, "sigma")
+        dt = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": round(2 * nsamp / dt / 1e6, 4), "unit": "Mpoint/s",
+                                "cores": torch.get_num_threads(), "kind": "port",
+                                "sample": f"oracle eval_eff (torch autograd FP64) on {nsamp} of {ng} points, RKS "
+                                          f"and UKS, {dt:.1f} s"}
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -895,7 +967,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cphf", action="store_true")
-    ap.add_argument("--workload", default="ri_jk", choices=["ri_jk", "numint", "dft_setup", "dft_becke", "skeleton", "cphf_batch"])
+    ap.add_argument("--workload", default="ri_jk", choices=["ri_jk", "numint", "dft_setup", "dft_becke", "skeleton", "cphf_batch", "xc"])
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run the library's 16-byte copy kernel over GIB GiB three times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes) and exit")
@@ -913,7 +985,7 @@ def main():
 
     if args.workload != "ri_jk":
         fn = {"numint": workload_numint, "dft_setup": workload_dft_setup, "dft_becke": workload_dft_becke,
-              "skeleton": workload_skeleton, "cphf_batch": workload_cphf_batch}[args.workload]
+              "skeleton": workload_skeleton, "cphf_batch": workload_cphf_batch, "xc": workload_xc}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
 

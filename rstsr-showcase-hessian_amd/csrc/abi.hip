@@ -10,6 +10,7 @@
 #include "rijk_skeleton.hpp"
 #include "rijk.hpp"
 #include "vib.hpp"
+#include "xc_functionals.hpp"
 
 namespace hh {
 namespace {
@@ -645,6 +646,37 @@ hh_status hh_xceff_transform(int32_t den_type, int32_t order, const double* rho,
   HH_ABI_BEGIN
   HH_REQUIRE(ng >= 0 && xc_val && out && (den_type == HH_DEN_RHO || rho), "null argument");
   xceff_transform(den_type, order, rho, ld_rho, xc_val, ld_xc, ng, out, (hipStream_t)stream);
+  HH_ABI_END
+}
+
+hh_status hh_xc_info(int32_t xc_id, int32_t nspin, int32_t deriv, int32_t* den_type, double* exx, int64_t* ncol) {
+  HH_ABI_BEGIN
+  bool gga = false;
+  double x = 0.0;
+  HH_REQUIRE(xc_known(xc_id, &gga, &x), "xc: unsupported functional id (LibXC ids 1, 7, 8, 106, 131, 402)");
+  HH_REQUIRE(nspin == 1 || nspin == 2, "xc: nspin must be 1 or 2");
+  HH_REQUIRE(deriv >= 0 && deriv <= 2, "xc: derivative order 0..2");
+  if (den_type) *den_type = gga ? HH_DEN_SIGMA : HH_DEN_RHO;
+  if (exx) *exx = x;
+  if (ncol) *ncol = xc_ncol(gga, nspin, deriv);
+  HH_ABI_END
+}
+
+hh_status hh_xc_eval(int32_t xc_id, int32_t nspin, int32_t deriv, const double* rho, int64_t ld_rho, int64_t ld_spin,
+                     int64_t ng, double* out, int64_t ld_out, void* stream) {
+  HH_ABI_BEGIN
+  xc_eval(xc_id, nspin, deriv, rho, ld_rho, ld_spin, ng, 1.0, false, out, ld_out, (hipStream_t)stream);
+  HH_ABI_END
+}
+
+hh_status hh_xc_eval_eff(int32_t xc_id, int32_t nspin, int32_t den_type, double scale, const double* rho,
+                         int64_t ld_rho, int64_t ld_spin, int64_t ng, int32_t accumulate, double* exc, double* vxc_eff,
+                         double* fxc_eff, void* stream) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(den_type == HH_DEN_RHO || den_type == HH_DEN_SIGMA || den_type == HH_DEN_TAU, "xc: bad den_type");
+  const int nvl = den_type == HH_DEN_RHO ? 1 : den_type == HH_DEN_SIGMA ? 4 : 5;
+  xc_eval_eff(xc_id, nspin, nvl, rho, ld_rho, ld_spin, ng, scale, accumulate != 0, exc, vxc_eff, fxc_eff,
+              (hipStream_t)stream);
   HH_ABI_END
 }
 

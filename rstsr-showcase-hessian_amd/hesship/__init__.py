@@ -4,6 +4,7 @@ Mirrors the reference's (ajz34/rstsr-showcase-hessian) Rust API names:
   RHessRIJK, UHessRIJK, get_rijk_response_bra_separated     (src/ri_jk)
   krylov_block                                               (src/util/krylov_block.rs)
   BeckeMolTables.partition                                   (src/numint_matmul/becke_partition.rs)
+  LibXCFunctional, libxc_eval_inner, libxc_eval_eff          (src/xceff/libxc_wrap.rs)
 Device tensors are torch tensors on a ROCm device, in the reference's logical shapes; the
 binding hands column-major device buffers to the C ABI.
 """
@@ -13,4 +14,6 @@ from .krylov import krylov_block, orth_block, solve, cholesky, trsm_lower  # noq
 from .cphf import RHessCPHF, UHessCPHF  # noqa: F401
 from .becke import BeckeMolTables  # noqa: F401
 from .numint import NIMatmul, libxc_to_xcfun_mapping, transform_xc, transform_xc_polarized  # noqa: F401
+from .xc import (LibXCFunctional, determine_den_type, determine_den_type_from_list,  # noqa: F401
+                 eval_xc_eff_list, libxc_eval_eff, libxc_eval_inner)
 from .rijk import RHessRIJK, UHessRIJK, get_rijk_response_bra_separated, pack_triu_tilde, unpack_triu  # noqa: F401

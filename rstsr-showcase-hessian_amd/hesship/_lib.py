@@ -135,6 +135,9 @@ SIGNATURES.update({
                                   _p, _i64, _p]),
     "hh_numint_hessian_setup_becke": (_i32, [_p, _i32, _i32, _pp, _i64, _pi64, _p, _p, _p, _p, _p, _p, _pi64, _i32,
                                              _p, _pp, _p, _pp, _pp, _pp, _pp]),
+    "hh_xc_info": (_i32, [_i32, _i32, _i32, C.POINTER(_i32), C.POINTER(_d), _pi64]),
+    "hh_xc_eval": (_i32, [_i32, _i32, _i32, _p, _i64, _i64, _i64, _p, _i64, _p]),
+    "hh_xc_eval_eff": (_i32, [_i32, _i32, _i32, _d, _p, _i64, _i64, _i64, _i32, _p, _p, _p, _p]),
 })
 
 

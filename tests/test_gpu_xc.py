@@ -1,0 +1,197 @@
+"""GPU parity: exchange-correlation functionals (hh_xc_eval, hh_xc_eval_eff) against the oracle.
+
+Densities span 1e-5 .. 1e2 with reduced gradients from small to large, as on a molecular grid.
+Tolerance (FP64): |gpu - oracle| <= 1e-9 |oracle| + 1e-11 * (largest magnitude among the same
+point's values of the same derivative order).  The absolute part covers round-off in
+cancellation-prone places (1 - zeta near 0, the LYP brace) whose conditioning is ~1e-10; every
+value is otherwise held to 1e-9 relative.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import xcfunc as X
+from tests.test_gpu_cphf import T
+
+pytestmark = pytest.mark.gpu
+
+IDS = sorted(X.GGA)
+NAMES = {v: k for k, v in X.IDS.items()}
+
+
+def dens(rng, ng, lo=-5.0, hi=2.0):
+    r = np.empty((ng, 4))
+    r[:, 0] = 10 ** rng.uniform(lo, hi, ng)
+    r[:, 1:] = rng.normal(size=(ng, 3)) * r[:, :1] * rng.uniform(0.05, 3.0, (ng, 1))
+    return r
+
+
+def case(nspin, ng=4000, seed=0):
+    rng = np.random.default_rng(seed)
+    return dens(rng, ng) if nspin == 1 else np.stack([dens(rng, ng), dens(rng, ng)], -1)
+
+
+def close(got, ref, blocks):
+    """blocks: column ranges of one derivative order; returns the worst violation ratio."""
+    worst = 0.0
+    for lo, hi in blocks:
+        g, r = got[:, lo:hi], ref[:, lo:hi]
+        sc = np.abs(r).max(1, keepdims=True)
+        tol = 1e-9 * np.abs(r) + 1e-11 * sc
+        worst = max(worst, float((np.abs(g - r) / np.maximum(tol, 1e-300)).max()))
+    return worst
+
+
+def raw_blocks(xc_id, nspin, deriv):
+    n = (2 if X.GGA[xc_id] else 1) if nspin == 1 else (5 if X.GGA[xc_id] else 2)
+    b = [(0, 1)]
+    if deriv >= 1:
+        b.append((1, 1 + n))
+    if deriv >= 2:
+        b.append((1 + n, 1 + n + n * (n + 1) // 2))
+    return b
+
+
+@pytest.mark.parametrize("nspin", [1, 2])
+@pytest.mark.parametrize("xc_id", IDS)
+def test_raw_against_oracle(dev, xc_id, nspin):
+    import hesship
+    rho = case(nspin, seed=xc_id + 10 * nspin)
+    f = hesship.LibXCFunctional.from_identifier(NAMES[xc_id], "unpolarized" if nspin == 1 else "polarized")
+    for deriv in (0, 1, 2):
+        got = hesship.libxc_eval_inner(f, T(rho, dev), deriv).cpu().numpy()
+        ref = X.eval_raw(xc_id, rho, nspin, deriv)
+        assert got.shape == ref.shape == (rho.shape[0], X.ncol(xc_id, nspin, deriv))
+        assert close(got, ref, raw_blocks(xc_id, nspin, deriv)) <= 1.0, (NAMES[xc_id], nspin, deriv)
+
+
+def _eff_close(got, ref, ng):
+    worst = 0.0
+    for g, r in zip(got, ref):
+        worst = max(worst, close(g.reshape(ng, -1), r.reshape(ng, -1), [(0, r.reshape(ng, -1).shape[1])]))
+    return worst
+
+
+@pytest.mark.parametrize("nspin", [1, 2])
+@pytest.mark.parametrize("xc_id", IDS)
+def test_eff_against_oracle(dev, xc_id, nspin):
+    import hesship
+    rho = case(nspin, ng=3000, seed=100 + xc_id + nspin)
+    f = hesship.LibXCFunctional.from_identifier(NAMES[xc_id], "unpolarized" if nspin == 1 else "polarized")
+    exc, vxc, fxc = hesship.libxc_eval_eff(f, T(rho, dev), 2)
+    den = "sigma" if X.GGA[xc_id] else "rho"
+    ref = X.eval_eff(xc_id, rho, nspin, den)
+    got = [t.cpu().numpy() for t in (exc, vxc, fxc)]
+    for g, r in zip(got, ref):
+        assert g.shape == r.shape
+    assert _eff_close(got, ref, rho.shape[0]) <= 1.0, (NAMES[xc_id], nspin)
+
+
+@pytest.mark.parametrize("nspin", [1, 2])
+def test_eff_equals_raw_through_device_transform(dev, nspin):
+    """The fused chain rule equals raw LibXC columns -> XCFun order -> hh_xceff_transform*."""
+    import hesship
+    rho = case(nspin, ng=2500, seed=7 + nspin)
+    f = hesship.LibXCFunctional.from_identifier("HYB_GGA_XC_B3LYP", "unpolarized" if nspin == 1 else "polarized")
+    d_rho = T(rho, dev)
+    raw = hesship.libxc_eval_inner(f, d_rho, 2)
+    _, vxc, fxc = hesship.libxc_eval_eff(f, d_rho, 2)
+    if nspin == 1:
+        v2 = hesship.transform_xc(d_rho, raw, "sigma", 1)
+        f2 = hesship.transform_xc(d_rho, raw, "sigma", 2)
+    else:
+        m = torch.as_tensor(hesship.libxc_to_xcfun_mapping("sigma", 2), device=dev)
+        xcf = raw[:, m]
+        v2 = hesship.transform_xc_polarized(d_rho, xcf, "sigma", 1)
+        f2 = hesship.transform_xc_polarized(d_rho, xcf, "sigma", 2)
+    ng = rho.shape[0]
+    assert _eff_close([vxc.cpu().numpy(), fxc.cpu().numpy()], [v2.cpu().numpy(), f2.cpu().numpy()], ng) <= 1.0
+
+
+@pytest.mark.parametrize("nspin", [1, 2])
+def test_functional_lists(dev, nspin):
+    """get_rho_vxc_fxc's loop: SVWN (LDA_X + LDA_C_VWN) into RHO arrays; B3LYP as one functional
+    equals its four weighted parts; an LDA term inside a SIGMA list fills the leading slice."""
+    import hesship
+    spin = "unpolarized" if nspin == 1 else "polarized"
+    F = hesship.LibXCFunctional.from_identifier
+    rho = case(nspin, ng=2000, seed=31 + nspin)
+    d_rho = T(rho, dev)
+    ng = rho.shape[0]
+    # SVWN
+    exc, vxc, fxc = hesship.eval_xc_eff_list([(1.0, F("LDA_X", spin)), (1.0, F("LDA_C_VWN", spin))], d_rho)
+    ref = [a + b for a, b in zip(X.eval_eff(1, rho, nspin, "rho"), X.eval_eff(7, rho, nspin, "rho"))]
+    assert _eff_close([t.cpu().numpy() for t in (exc, vxc, fxc)], ref, ng) <= 1.0
+    # B3LYP by parts
+    whole = hesship.eval_xc_eff_list([(1.0, F("HYB_GGA_XC_B3LYP", spin))], d_rho)
+    parts = hesship.eval_xc_eff_list([(0.08, F("LDA_X", spin)), (0.72, F("GGA_X_B88", spin)),
+                                      (0.19, F("LDA_C_VWN_RPA", spin)), (0.81, F("GGA_C_LYP", spin))], d_rho)
+    assert _eff_close([t.cpu().numpy() for t in whole], [t.cpu().numpy() for t in parts], ng) <= 1.0
+    assert F("HYB_GGA_XC_B3LYP", spin).hyb_exx_coef() == 0.2
+    # leading slice
+    _, v_l, f_l = hesship.eval_xc_eff_list([(1.0, F("LDA_C_VWN", spin)), (0.0, F("GGA_X_B88", spin))], d_rho)
+    ref_l = X.eval_eff(7, rho, nspin, "sigma")
+    assert _eff_close([v_l.cpu().numpy(), f_l.cpu().numpy()], list(ref_l[1:]), ng) <= 1.0
+
+
+def test_thresholds_and_zero_gradient(dev):
+    import hesship
+    F = hesship.LibXCFunctional.from_identifier
+    # unpolarised: zero / sub-threshold density gives zeros; zero gradient gives B88 = Slater and
+    # vsigma = -beta/2 (rho/2)^(-4/3) (the gradient-expansion coefficient)
+    r = np.zeros((5, 4))
+    r[:, 0] = [0.0, 5e-16, 0.3, 1.0, 4.0]
+    got = hesship.libxc_eval_inner(F("GGA_X_B88"), T(r, dev), 2).cpu().numpy()
+    assert np.all(got[:2] == 0.0)
+    rho = r[2:, 0]
+    np.testing.assert_allclose(got[2:, 0], -0.75 * (3 / np.pi) ** (1 / 3) * rho ** (1 / 3), rtol=1e-13)
+    np.testing.assert_allclose(got[2:, 2], -0.0042 / 2 * (rho / 2) ** (-4 / 3), rtol=1e-13)
+    # accumulate leaves sub-threshold points untouched
+    exc, vxc, fxc = hesship.eval_xc_eff_list([(1.0, F("HYB_GGA_XC_B3LYP"))], T(r, dev))
+    assert torch.all(exc[:2] == 0) and torch.all(vxc[:2] == 0) and torch.all(fxc[:2] == 0)
+    # polarised with one empty spin: exchange of the occupied spin only, LYP vanishes
+    rp = np.zeros((3, 4, 2))
+    rp[:, 0, 0] = [0.2, 1.0, 3.0]
+    rp[:, 1:, 0] = [[0.1, 0.0, 0.05], [0.3, -0.2, 0.1], [1.0, 0.5, -0.4]]
+    g_x = hesship.libxc_eval_inner(F("GGA_X_B88", "polarized"), T(rp, dev), 0).cpu().numpy()[:, 0]
+    g_u = hesship.libxc_eval_inner(F("GGA_X_B88"), T(2 * rp[:, :, 0], dev), 0).cpu().numpy()[:, 0]
+    np.testing.assert_allclose(g_x * rp[:, 0, 0], 0.5 * g_u * 2 * rp[:, 0, 0], rtol=1e-13)
+    g_c = hesship.libxc_eval_inner(F("GGA_C_LYP", "polarized"), T(rp, dev), 0).cpu().numpy()[:, 0]
+    assert np.abs(g_c).max() < 1e-12
+
+
+def test_errors(dev):
+    import hesship
+    from hesship._lib import HesshipError, lib
+    with pytest.raises(ValueError):
+        hesship.LibXCFunctional.from_identifier("MGGA_X_TPSS")
+    with pytest.raises(HesshipError):
+        hesship.LibXCFunctional(999)
+    r = T(np.ones((4, 4)), dev)
+    # GGA functional with RHO-only outputs is rejected
+    f = hesship.LibXCFunctional.from_identifier("GGA_X_B88")
+    with pytest.raises(HesshipError):
+        from hesship.xc import _accumulate_eff
+        out = torch.zeros(4, dtype=torch.float64, device=dev)
+        _accumulate_eff(f, r, "rho", 1.0, out, out, out, False)
+    # empty input is a no-op
+    assert lib().hh_xc_eval(1, 1, 2, None, 0, 0, 0, None, 0, None) == 0
+
+
+@pytest.mark.parametrize("nspin", [1, 2])
+def test_full_grid_size(dev, nspin):
+    """B3LYP at the benchmark grid size (543600 points): finite everywhere and a 2000-point
+    sample equal to the oracle."""
+    import hesship
+    ng = 543600
+    rng = np.random.default_rng(77 + nspin)
+    rho = case(nspin, ng=ng, seed=77 + nspin)
+    f = hesship.LibXCFunctional.from_identifier("HYB_GGA_XC_B3LYP", "unpolarized" if nspin == 1 else "polarized")
+    exc, vxc, fxc = hesship.libxc_eval_eff(f, T(rho, dev), 2)
+    torch.cuda.synchronize()
+    assert torch.isfinite(fxc).all() and torch.isfinite(vxc).all() and torch.isfinite(exc).all()
+    idx = np.sort(rng.choice(ng, 2000, replace=False))
+    ref = X.eval_eff(402, rho[idx], nspin, "sigma")
+    got = [t.cpu().numpy()[idx] for t in (exc, vxc, fxc)]
+    assert _eff_close(got, ref, len(idx)) <= 1.0
