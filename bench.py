@@ -728,70 +728,69 @@ def workload_cphf_batch(args):
             return max((x - y).abs().max().item() for x, y in zip(a, b))
         return (a - b).abs().max().item()
 
+    def make_mol(i, n, scale2, resident=False):
+        """(RI-JK handle, CPHF driver, f1, s1, prepare, nocc, ratio) of molecule i"""
+        nao, naux = 24 * n + 10, 84 * n + 28
+        nprop = 3 * (3 * n + 2)
+        cd = torch.from_numpy(synth.cderi(nao, naux, seed=synth.SEED_B + 16 * i).T.copy()).to(dev).T
+        C = torch.from_numpy(synth.mo_coeff(nao, seed=synth.SEED_C + 16 * i)).to(dev)
+        if i % 2 == 1:
+            occ = [np.where(np.arange(nao) < 4 * n + 1, 1.0, 0.0), np.where(np.arange(nao) < 4 * n, 1.0, 0.0)]
+            noc = [4 * n + 1, 4 * n]
+            o = hesship.UHessRIJK(cd * scale2 ** 0.5, 1.0, 1.0)
+            prep = lambda: o.make_response_preparation([C, C], occ)
+        else:
+            noc = 4 * n + 1
+            occ = np.where(np.arange(nao) < noc, 2.0, 0.0)
+            o = hesship.RHessRIJK(cd * scale2 ** 0.5, 1.0, 1.0)
+            prep = lambda: o.make_response_preparation(C, occ)
+        if resident:
+            o.set_response_path("resident")
+        prep()
+        if i % 2 == 1:
+            drv, _, f1, s1, ratio = ucphf_case(o, C, occ, nao, nprop, dev, cfg)
+        else:
+            drv, _, f1, s1, ratio = cphf_case(o, C, occ, nao, noc, nprop, dev, cfg)
+        return o, drv, f1, s1, prep, noc, ratio
+
+    scales, refs = [], []
     with torch.cuda.stream(stream):
         for i, n in enumerate(ns):
             n = int(n)
-            nao, naux = 24 * n + 10, 84 * n + 28
-            nprop = 3 * (3 * n + 2)
+            nao, nprop = 24 * n + 10, 3 * (3 * n + 2)
             uhf = i % 2 == 1
             nuhf += uhf
-            cd = torch.from_numpy(synth.cderi(nao, naux, seed=synth.SEED_B + 16 * i).T.copy()).to(dev).T
-            C = torch.from_numpy(synth.mo_coeff(nao, seed=synth.SEED_C + 16 * i)).to(dev)
-            if uhf:
-                occ = [np.where(np.arange(nao) < 4 * n + 1, 1.0, 0.0), np.where(np.arange(nao) < 4 * n, 1.0, 0.0)]
-                noc = [4 * n + 1, 4 * n]
-
-                def prep(o):
-                    o.make_response_preparation([C, C], occ)
-
-                def system(scale2, resident=False):
-                    o = hesship.UHessRIJK(cd * scale2 ** 0.5, 1.0, 1.0)
-                    if resident:
-                        o.set_response_path("resident")
-                    prep(o)
-                    return (o,) + ucphf_case(o, C, occ, nao, nprop, dev, cfg)
-            else:
-                noc = 4 * n + 1
-                occ = np.where(np.arange(nao) < noc, 2.0, 0.0)
-
-                def prep(o):
-                    o.make_response_preparation(C, occ)
-
-                def system(scale2, resident=False):
-                    o = hesship.RHessRIJK(cd * scale2 ** 0.5, 1.0, 1.0)
-                    if resident:
-                        o.set_response_path("resident")
-                    prep(o)
-                    return (o,) + cphf_case(o, C, occ, nao, noc, nprop, dev, cfg)
             # |A v| ~ 0.3 |v| for random v, then shrunk where that leaves I + A indefinite
-            ratio = system(1.0)[-1]
+            ratio = make_mol(i, n, 1.0)[-1]
             scale2 = 0.3 / ratio
-            obj, drv, _, f1, s1, _ = system(scale2)
+            obj, drv, f1, s1, prep, noc, _ = make_mol(i, n, scale2)
             lo = response_spectrum(drv, nao, noc, dev, which=("SR",))[0]
             if lo < -LOWEST_A:
                 scale2 *= LOWEST_A / -lo
                 del obj, drv
-                obj, drv, _, f1, s1, _ = system(scale2)
+                obj, drv, f1, s1, prep, noc, _ = make_mol(i, n, scale2)
             min_eig.append(round(max(lo, -LOWEST_A), 4))
+            scales.append(scale2)
             # every leg's timed region holds prepare_response (for the resident operator that
             # includes building its matrix on the first response), the rhs, the solve and finalize
             drv.set_graphs(False)
-            run_cphf(drv, f1, s1, "fp64", lambda: prep(obj))
-            ref, _, t64, st64 = run_cphf(drv, f1, s1, "fp64", lambda: prep(obj))
+            run_cphf(drv, f1, s1, "fp64", prep)
+            ref, _, t64, st64 = run_cphf(drv, f1, s1, "fp64", prep)
             drv.set_graphs(True)
-            run_cphf(drv, f1, s1, "bf16x3", lambda: prep(obj))
-            got, _, tmx, stmx = run_cphf(drv, f1, s1, "bf16x3", lambda: prep(obj))
+            run_cphf(drv, f1, s1, "bf16x3", prep)
+            got, _, tmx, stmx = run_cphf(drv, f1, s1, "bf16x3", prep)
             c, l = drv.graph_stats()
             del drv, obj
-            robj, rdrv = system(scale2, resident=True)[:2]
+            robj, rdrv, _, _, rprep, _, _ = make_mol(i, n, scale2, resident=True)
             rdrv.set_graphs(False)
-            run_cphf(rdrv, f1, s1, "fp64", lambda: prep(robj))
-            _, _, tr64, str64 = run_cphf(rdrv, f1, s1, "fp64", lambda: prep(robj))
+            run_cphf(rdrv, f1, s1, "fp64", rprep)
+            _, _, tr64, str64 = run_cphf(rdrv, f1, s1, "fp64", rprep)
             rdrv.set_graphs(True)
-            run_cphf(rdrv, f1, s1, "bf16x3", lambda: prep(robj))
-            rgot, _, trmx, strmx = run_cphf(rdrv, f1, s1, "bf16x3", lambda: prep(robj))
+            run_cphf(rdrv, f1, s1, "bf16x3", rprep)
+            rgot, _, trmx, strmx = run_cphf(rdrv, f1, s1, "bf16x3", rprep)
             err = max(err, diff(got, ref))
             err_res = max(err_res, diff(rgot, ref))
+            refs.append(ref)
             tot["fp64"] += t64
             tot["mixed"] += tmx
             tot["fp64_resident"] += tr64
@@ -807,8 +806,38 @@ def workload_cphf_batch(args):
                 f"cycles {t64 * 1e3:.1f} ms, mixed {stmx['cycles']} cycles / {stmx['refinements']} sweeps "
                 f"{' (FP64 finish)' if stmx.get('fp64_fallback') else ''} {tmx * 1e3:.1f} ms; resident FP64 "
                 f"{tr64 * 1e3:.1f} ms, mixed {strmx['cycles']} cycles {trmx * 1e3:.1f} ms")
-            del rdrv, robj, cd, C, f1, s1, ref, got, rgot
+            del rdrv, robj, f1, s1, got, rgot
             torch.cuda.synchronize()
+        # every molecule at once: all 64 systems resident and solved in lockstep (hh_cphf_batch),
+        # the timed region holding prepare_response of every handle, the right-hand sides, the
+        # resident matrices' builds, the batched solve and finalize
+        mols = [make_mol(i, int(n), scales[i], resident=True) for i, n in enumerate(ns)]
+        torch.cuda.synchronize()
+        log(f"[bench] batched leg: {len(mols)} systems created, {torch.cuda.memory_allocated() / 2**30:.1f} GiB "
+            f"held by torch")
+
+        def batched():
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for m in mols:
+                m[4]()
+            rhs = [m[1].compute_dimless_rhs(m[2], m[3]) for m in mols]
+            bt = hesship.CPHFBatch(tol=1e-8, max_cycle=42, lindep=1e-14, device=dev)
+            for m, r in zip(mols, rhs):
+                bt.add(m[1], m[0], r)
+            st = []
+            mo1 = bt.solve(stats=st)
+            outs = [m[1].finalize_cphf(m[2], m[3], x)[0] for m, x in zip(mols, mo1)]
+            torch.cuda.synchronize()
+            return time.perf_counter() - t0, outs, st, bt.info()
+
+        batched()
+        t_batch, outs, bst, binfo = batched()
+        err_batch = max(diff(a, b) for a, b in zip(outs, refs))
+        cyc["batched_fp64"] = sum(x["cycles"] for x in bst)
+        log(f"[bench] batched FP64: {t_batch:.3f} s, {binfo[1]} lockstep cycles, <= {binfo[2]} launches per cycle, "
+            f"max |mo1 - mo1_FP64| {err_batch:.2e}")
+        del mols, outs
     wall = tot["mixed_resident"]
     log(f"[bench] cphf batch: FP64 {tot['fp64']:.3f} s, mixed {tot['mixed']:.3f} s, resident FP64 "
         f"{tot['fp64_resident']:.3f} s, resident mixed {wall:.3f} s")
@@ -834,6 +863,13 @@ def workload_cphf_batch(args):
     line["graph_launches"] = launches
     line["max_abs_mo1_diff_mixed_vs_fp64"] = err
     line["mixed_solves_finished_in_fp64"] = fallbacks
+    line["batched_fp64"] = {"wall_s": round(t_batch, 4), "speedup_vs_fp64": round(tot["fp64"] / t_batch, 3),
+                            "speedup_vs_sequential_fp64_resident": round(tot["fp64_resident"] / t_batch, 3),
+                            "lockstep_cycles": binfo[1], "launches_per_cycle": binfo[2],
+                            "max_abs_mo1_diff_vs_fp64": err_batch,
+                            "note": "all 64 systems resident at once and solved in lockstep (hh_cphf_batch): "
+                                    "every Krylov step one grouped launch over the running systems, one host "
+                                    "read per cycle; FP64 throughout"}
     return line
 
 

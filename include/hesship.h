@@ -461,6 +461,35 @@ hh_status hh_ucphf_hess(hh_cphf* h, const double* const* f1mo, const double* con
 hh_status hh_ucphf_add_numint(hh_cphf* h, hh_numint* ni, int32_t den_type, const double* fxc_eff);
 
 /* ---------------------------------------------------------------------------------------- */
+/* Many CPHF systems in one batch (no reference counterpart: the reference solves one molecule  */
+/* per RHessSCF / UHessSCF, rscf.rs:481-505, each with its own krylov_block call).              */
+
+typedef struct hh_cphf_batch hh_cphf_batch;
+
+/* A batch of dimensionless CPHF solves advanced in lockstep: each step of a Krylov cycle (the
+ * response operator's three products, the projection coefficients, the update, both Gram / LDL
+ * passes) is one grouped launch over every system still running, and the host reads one small
+ * array per cycle for all systems.  Per system the solve is the reference's krylov_block
+ * (krylov_block.rs:39-197: same basis, drop rule, convergence test and projected solve) on that
+ * system's solve_dimless_cphf equations (rscf.rs:278-300, uscf.rs:278-349) in FP64, with one
+ * subspace (no hard restart: as the reference with cphf_max_space >= cphf_max_cycle). */
+hh_status hh_cphf_batch_create(double tol, int32_t max_cycle, double lindep, void* stream, hh_cphf_batch** out);
+hh_status hh_cphf_batch_destroy(hh_cphf_batch* b);
+/* Add one system: the driver (restricted or unrestricted; its orbitals and orbital energies;
+ * level_shift 0) and the RI-JK handle prepared with the same orbitals, whose resident response
+ * matrix (built here if it is not) is the system's response operator.  rhs / mo1: one device
+ * pointer per spin, [nmo[s], nocc[s], nprop], 1 <= nprop <= 128.  rhs is copied at once; mo1 is
+ * written by hh_cphf_batch_solve.  The driver and the handle must outlive the batch. */
+hh_status hh_cphf_batch_add(hh_cphf_batch* b, hh_cphf* drv, hh_rijk* rijk, const double* const* rhs, int64_t nprop,
+                            double* const* mo1);
+/* Solve every system.  stats (may be NULL): one entry per system in the order added.  Returns
+ * HH_ERR_NOT_CONVERGED if a system did not converge within max_cycle (every mo1 is still
+ * written).  Synchronous. */
+hh_status hh_cphf_batch_solve(hh_cphf_batch* b, hh_krylov_stats* stats);
+/* Systems in the batch, lockstep cycles of the last solve and the most launches one cycle made. */
+hh_status hh_cphf_batch_info(hh_cphf_batch* b, int64_t* nsys, int64_t* cycles, int64_t* launches_per_cycle);
+
+/* ---------------------------------------------------------------------------------------- */
 /* Learned diagonal CPHF preconditioner (no reference counterpart: the reference preconditions */
 /* CPHF only by the orbital-energy scaling of the dimensionless form, rscf.rs:237-263).       */
 

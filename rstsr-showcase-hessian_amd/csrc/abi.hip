@@ -4,6 +4,7 @@
 #include "gemm.hpp"
 #include "gemm_nt.hpp"
 #include "cphf.hpp"
+#include "cphf_batch.hpp"
 #include "linalg.hpp"
 #include "mlp.hpp"
 #include "numint.hpp"
@@ -114,6 +115,10 @@ struct hh_mlp {
 struct hh_cphf {
   std::unique_ptr<hh::Cphf> impl;
   std::vector<std::unique_ptr<hh::ElecResponse>> comps;
+};
+
+struct hh_cphf_batch {
+  std::unique_ptr<hh::CphfBatch> impl;
 };
 
 using namespace hh;
@@ -373,6 +378,69 @@ hh_status hh_cphf_add_rijk(hh_cphf* h, hh_rijk* rijk) {
     HH_REQUIRE(rijk->impl->nocc(t) == h->impl->nocc(t), "RI-JK component occupation differs from the driver's");
   h->comps.push_back(std::make_unique<RijkResponse>(rijk->impl.get()));
   h->impl->add_component(h->comps.back().get());
+  HH_ABI_END
+}
+
+hh_status hh_cphf_batch_create(double tol, int32_t max_cycle, double lindep, void* stream, hh_cphf_batch** out) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(out, "null output handle");
+  auto* b = new hh_cphf_batch;
+  try {
+    b->impl = std::make_unique<CphfBatch>(tol, max_cycle, lindep, (hipStream_t)stream);
+  } catch (...) {
+    delete b;
+    throw;
+  }
+  *out = b;
+  HH_ABI_END
+}
+
+hh_status hh_cphf_batch_destroy(hh_cphf_batch* b) {
+  HH_ABI_BEGIN
+  delete b;
+  HH_ABI_END
+}
+
+hh_status hh_cphf_batch_add(hh_cphf_batch* b, hh_cphf* drv, hh_rijk* rijk, const double* const* rhs, int64_t nprop,
+                            double* const* mo1) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(b && drv && rijk, "null handle");
+  b->impl->add(drv->impl.get(), rijk->impl.get(), rhs, nprop, mo1);
+  HH_ABI_END
+}
+
+hh_status hh_cphf_batch_solve(hh_cphf_batch* b, hh_krylov_stats* stats) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(b, "null handle");
+  std::vector<KrylovStats> st;
+  Error err(HH_OK, "");
+  bool failed = false;
+  try {
+    b->impl->solve(&st);
+  } catch (const Error& e) {
+    if (e.code != HH_ERR_NOT_CONVERGED) throw;
+    err = e;
+    failed = true;
+  }
+  if (stats)
+    for (size_t i = 0; i < st.size(); ++i) {
+      hh_krylov_stats k{};
+      k.cycles = st[i].cycles;
+      k.restarts = st[i].restarts;
+      k.aop_calls = st[i].aop_calls;
+      k.max_innerprod = st[i].max_innerprod;
+      stats[i] = k;
+    }
+  if (failed) throw err;
+  HH_ABI_END
+}
+
+hh_status hh_cphf_batch_info(hh_cphf_batch* b, int64_t* nsys, int64_t* cycles, int64_t* launches_per_cycle) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(b, "null handle");
+  if (nsys) *nsys = (int64_t)b->impl->size();
+  if (cycles) *cycles = b->impl->cycles();
+  if (launches_per_cycle) *launches_per_cycle = b->impl->launches_per_cycle();
   HH_ABI_END
 }
 

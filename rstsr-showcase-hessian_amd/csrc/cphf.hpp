@@ -46,6 +46,14 @@ class Cphf {
   int64_t nmo(int s = 0) const { return sp_[s].nmo; }
   int64_t nao() const { return nao_; }
   const double* mo_coeff(int s = 0) const { return sp_[s].c.as<double>(); }
+  // orbital energies of spin s on the device: occupied [nocc_s], virtual [nmo_s - nocc_s]
+  const double* eocc_dev(int s = 0) const { return sp_[s].eocc.as<double>(); }
+  const double* evir_dev(int s = 0) const { return sp_[s].evir.as<double>(); }
+  // rows of the stacked vector (sum_s nmo_s nocc_s) and the row offset of spin s's block
+  int64_t stacked_rows() const { return n_; }
+  int64_t spin_offset(int s) const { return sp_[s].off; }
+  const CphfConfig& config() const { return cfg_; }
+  size_t ncomponents() const { return comps_.size(); }
 
   // Per-spin arrays [nmo_s, nocc_s, nprop] (mo_e1: [nocc_s, nocc_s, nprop]); one pointer per spin.
   // rscf.rs:170-178 / uscf.rs:165-174

@@ -16,84 +16,21 @@
 #include "krylov.hpp"
 
 #include "elementwise.hpp"
+#include "krylov_dev.hpp"
 #include "linalg.hpp"
 
 namespace hh {
 
 namespace {
-constexpr int GB = 128;       // columns per Gram group
-constexpr int GLD = GB + 1;   // LDS leading dimension (odd: row walks hit distinct banks)
-constexpr int LDL_T = 1024;
+constexpr int GB = KR_GB;
+constexpr int LDL_T = KR_LDL_T;
 
-// LDL^T with the drop rule on the group's Gram matrix G [m, m] (column-major, lower triangle
-// read), m <= GB.  Outputs: Tm [m, m] with column c (c < kept) = row k_c of L^{-1} (k_c = c-th
-// kept pivot), so V Tm holds the kept Gram-Schmidt vectors in order; columns c >= kept are zero.
-// dout[c] = d_{k_c} (c < kept); *kout = kept.
+// LDL^T with the drop rule on the group's Gram matrix G [m, m] (every kept pivot selected):
+// V Tm holds the kept Gram-Schmidt vectors in order, dout[c] their squared norms, *kout the count.
 __global__ __launch_bounds__(LDL_T) void ldl_drop_kernel(const double* __restrict__ G, int m, double lindep,
                                                          double* __restrict__ Tm, double* __restrict__ dout,
                                                          int* __restrict__ kout) {
-  __shared__ double S[GB * GLD];
-  __shared__ double dv[GB];
-  __shared__ int kidx[GB];
-  __shared__ unsigned char kf[GB];
-  __shared__ int nkept;
-  const int tid = threadIdx.x;
-  for (int e = tid; e < m * m; e += LDL_T) {
-    const int i = e % m, j = e / m;
-    if (j <= i) S[i + GLD * j] = G[e];
-  }
-  __syncthreads();
-  for (int t = 0; t < m; ++t) {
-    const double d = S[t + GLD * t];  // final: step t touches only rows / columns > t
-    const bool keep = d > lindep;
-    const int r = m - t - 1;
-    if (keep) {
-      const double inv = 1.0 / d;
-      for (int e = tid; e < r * r; e += LDL_T) {
-        const int i = t + 1 + e % r, j = t + 1 + e / r;
-        if (j <= i) S[i + GLD * j] -= S[i + GLD * t] * S[j + GLD * t] * inv;
-      }
-      __syncthreads();
-      for (int i = t + 1 + tid; i < m; i += LDL_T) S[i + GLD * t] *= inv;
-    } else {
-      for (int i = t + 1 + tid; i < m; i += LDL_T) S[i + GLD * t] = 0.0;
-    }
-    if (tid == 0) {
-      dv[t] = d;
-      kf[t] = keep ? 1 : 0;
-    }
-    __syncthreads();
-  }
-  if (tid == 0) {
-    int k = 0;
-    for (int t = 0; t < m; ++t)
-      if (kf[t]) kidx[k++] = t;
-    nkept = k;
-    *kout = k;
-  }
-  __syncthreads();
-  const int kept = nkept;
-  // row k of L^{-1}: x = e_k; for q = k .. 1: x[s] -= L[q][s] x[q] (s < q).  One wave per row,
-  // x[s] held by lane s % 64 in slot s / 64.
-  const int lane = tid & 63, wave = tid >> 6;
-  for (int c = wave; c < m; c += LDL_T / 64) {
-    double* out = Tm + (int64_t)c * m;
-    if (c >= kept) {
-      for (int s2 = lane; s2 < m; s2 += 64) out[s2] = 0.0;
-      continue;
-    }
-    const int k = kidx[c];
-    double x0 = (lane == k) ? 1.0 : 0.0;
-    double x1 = (lane + 64 == k) ? 1.0 : 0.0;
-    for (int q = k; q >= 1; --q) {
-      const double xq = __shfl(q < 64 ? x0 : x1, q & 63, 64);
-      if (lane < q) x0 -= S[q + GLD * lane] * xq;
-      if (lane + 64 < q) x1 -= S[q + GLD * (lane + 64)] * xq;
-    }
-    if (lane < m) out[lane] = x0;
-    if (lane + 64 < m) out[lane + 64] = x1;
-    if (lane == 0) dout[c] = dv[k];
-  }
+  ldl_drop_block(G, m, lindep, -__builtin_huge_val(), Tm, dout, kout, nullptr);
 }
 
 __global__ void scale_rows_kernel(double* D, int64_t rows, int64_t cols, const double* dn) {

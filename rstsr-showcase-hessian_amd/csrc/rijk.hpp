@@ -66,6 +66,10 @@ class Rijk {
   int response_path = 0;
   int64_t resident_budget = 24LL << 30;
   bool resident_fits(Prec prec) const;
+  // The resident response matrix in FP64, row-major [R][kpad] (R = nao sum_s nocc_s, the alpha
+  // block first; kpad = R rounded up to 32, zeros past R), built now if it is not (needs
+  // prepare_response; independent of response_path and of the budget).
+  const double* resident_matrix(int64_t* R, int64_t* kpad);
 
   // Stage timing with HIP events on the handle's stream (the reference keeps wall-clock
   // `timing` vectors per component, hess_r.rs:1746).  Indices of the accumulated times (ms):

@@ -202,6 +202,15 @@ void Rijk::build_resident(Prec prec) {
   }
 }
 
+const double* Rijk::resident_matrix(int64_t* R, int64_t* kpad) {
+  HH_REQUIRE(prepared_, "resident response matrix: make_response_preparation first");
+  build_resident(PREC_F64);
+  const int64_t r = nao_ * (nocc_[0] + (nspin_ > 1 ? nocc_[1] : 0));
+  if (R) *R = r;
+  if (kpad) *kpad = ceil_div(r, 32) * 32;
+  return res64_.as<double>();
+}
+
 void Rijk::apply_resident(const double* const* bra, int64_t nprop, double* const* resp, Prec prec,
                           bool accumulate) {
   const bool bf = prec == PREC_BF16X3;

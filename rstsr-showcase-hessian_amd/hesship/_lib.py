@@ -129,6 +129,11 @@ SIGNATURES.update({
     "hh_cphf_set_graphs": (_i32, [_p, _i32]),
     "hh_cphf_graph_stats": (_i32, [_p, _pi64, _pi64]),
     "hh_cphf_finalize_reuses": (_i32, [_p, _pi64]),
+    "hh_cphf_batch_create": (_i32, [_d, _i32, _d, _p, C.POINTER(C.c_void_p)]),
+    "hh_cphf_batch_destroy": (_i32, [_p]),
+    "hh_cphf_batch_add": (_i32, [_p, _p, _p, _pp, _i64, _pp]),
+    "hh_cphf_batch_solve": (_i32, [_p, _p]),
+    "hh_cphf_batch_info": (_i32, [_p, _pi64, _pi64, _pi64]),
     "hh_becke_create": (_i32, [_i64, _p, _p, _p, C.POINTER(C.c_void_p)]),
     "hh_becke_destroy": (_i32, [_p]),
     "hh_becke_partition": (_i32, [_p, _p, _i64, _i32, _pi64, _p, _i32, _i32, _p, _p, _p, _p, _i64, _p, _p, _i64, _p,

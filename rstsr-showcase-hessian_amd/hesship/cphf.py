@@ -278,3 +278,60 @@ class UHessCPHF:
         mo1 = self.solve_dimless_cphf(rhs, precision=precision, stats=stats)
         mo1, mo_e1 = self.finalize_cphf(f1mo, s1mo, mo1)
         return self.get_cphf_hess(f1mo, s1mo, mo1, mo_e1)
+
+
+class CPHFBatch:
+    """Many dimensionless CPHF solves advanced in lockstep (no reference counterpart: the
+    reference runs one RHessSCF / UHessSCF per molecule, rscf.rs:481-505).  Each system is a
+    driver (RHessCPHF or UHessCPHF) plus the RI-JK handle prepared with its orbitals, whose
+    resident response matrix is the system's operator; each is solved by the reference's
+    krylov_block algorithm in FP64 with one subspace (no restart), every Krylov step one grouped
+    launch over all running systems."""
+
+    def __init__(self, tol=1e-8, max_cycle=42, lindep=1e-14, device=None):
+        dev = device or torch.device("cuda", torch.cuda.current_device())
+        h = C.c_void_p()
+        check(lib().hh_cphf_batch_create(tol, max_cycle, lindep, stream_of(dev), C.byref(h)))
+        self._h = h
+        self._keep = []
+        self._out = []
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                lib().hh_cphf_batch_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def add(self, drv, rijk, rhs):
+        """Add a system: rhs [nmo, nocc, ...] (restricted) or a pair of them (unrestricted).
+        Returns the index of the system."""
+        pair = isinstance(drv, UHessCPHF)
+        rl = list(rhs) if pair else [rhs]
+        flat = [fortran(t.reshape(t.shape[0], t.shape[1], -1)) for t in rl]
+        nprop = flat[0].shape[2]
+        out = [empty_f(tuple(t.shape), t.device) for t in flat]
+        check(lib().hh_cphf_batch_add(self._h, drv._h, rijk._h, ptr_array([x.data_ptr() for x in flat]), nprop,
+                                      ptr_array([x.data_ptr() for x in out])))
+        self._keep.append((drv, rijk, flat))
+        self._out.append((pair, [o.reshape(t.shape) for o, t in zip(out, rl)]))
+        return len(self._out) - 1
+
+    def solve(self, stats=None):
+        """Solve every system; returns the mo1 of each (a tensor, or a pair for unrestricted).
+        stats (list, optional) receives one dict per system."""
+        n = len(self._out)
+        st = (KrylovStats * max(n, 1))()
+        rc = lib().hh_cphf_batch_solve(self._h, C.cast(st, C.c_void_p))
+        if stats is not None:
+            stats.extend(st[i].as_dict() for i in range(n))
+        check(rc)
+        return [o if pair else o[0] for pair, o in self._out]
+
+    def info(self):
+        """(systems, lockstep cycles of the last solve, most launches in one cycle)."""
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().hh_cphf_batch_info(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
