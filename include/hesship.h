@@ -162,6 +162,54 @@ hh_status hh_rijk_skeleton(hh_rijk* h, const double* const* mo_coeff, const int6
                            const int64_t* aoslice, const int64_t* auxslice, double* de, double* const* deriv1_bra);
 
 /* ---------------------------------------------------------------------------------------- */
+/* Gaussian-basis integrals and AO values (the libcint calls of the reference's Hessian path)  */
+
+typedef struct hh_cint hh_cint;
+/* Integral kinds: int2c2e / int3c2e (hess_r.rs:1731, pure_decompose.rs:139) and the derivative
+ * integrals of the skeleton, libcint's ip = the gradient on the electron coordinate of one
+ * function: ip1 (d mu nu|P), ip2 (mu nu|d P), ipip1 (dd mu nu|P), ipvip1 (d mu d nu|P), ip1ip2
+ * (d mu nu|d P), ipip2 (mu nu|dd P) (hess_r.rs:537-540, 1058, 1356); 2-centre: PLAIN, IP1
+ * (dP|Q), IPIP1 (ddP|Q), IP1IP2 (dP|dQ) (hess_r.rs:416, 788-789). */
+#define HH_INT_PLAIN 0
+#define HH_INT_IP1 1
+#define HH_INT_IP2 2
+#define HH_INT_IPIP1 3
+#define HH_INT_IPVIP1 4
+#define HH_INT_IP1IP2 5
+#define HH_INT_IPIP2 6
+/* A basis in libcint's format (what the reference's CInt holds): atm [natm][6] (charge,
+ * ptr_coord, ...), bas [nbas][8] (atom, l, nprim, nctr, kappa, ptr_exp, ptr_coeff, -), env
+ * [nenv] (coordinates in bohr; contraction coefficients [nprim, nctr] used as given, i.e. with
+ * the radial normalisation folded in as libcint expects).  l <= 5.  cart: 0 spherical (real solid
+ * harmonics, libcint's order), 1 Cartesian.  Host arrays, copied. */
+hh_status hh_cint_create(const int32_t* atm, int32_t natm, const int32_t* bas, int32_t nbas, const double* env,
+                         int64_t nenv, int32_t cart, hh_cint** out);
+hh_status hh_cint_destroy(hh_cint* h);
+/* number of AO functions; ao_loc (HOST, may be NULL) receives nbas + 1 AO offsets of the shells */
+hh_status hh_cint_nao(const hh_cint* h, int64_t* nao, int64_t* ao_loc);
+/* eval_gto (CInt::eval_gto "deriv0".."deriv3" as NIMatmul::prepare_ao calls it, nimatmul.rs:94-99):
+ * out (device) [ng, nao, ncomp] column-major, ncomp = 1, 4, 10, 20 (value, then x, y, z, then
+ * xx, xy, xz, yy, yz, zz, then the ten third derivatives in the same order); coords (device)
+ * [ng][3]. */
+hh_status hh_cint_eval_gto(const hh_cint* h, int32_t deriv, const double* coords, int64_t ng, double* out,
+                           void* stream);
+/* int2c2e of one basis (hess_intor(aux, "int2c2e[_ip1|_ipip1|_ip1ip2]", "s1"), cint_handling.rs:20-44):
+ * out (device) [naux, naux, comp], comp 1 / 3 / 9 / 9 (9 as [3, 3], the later function's
+ * direction fastest).  Synchronous. */
+hh_status hh_cint_int2c2e(const hh_cint* aux, int32_t kind, double* out, void* stream);
+/* int3c2e over every AO pair of `mol` and the auxiliary shells [sh0, sh1) of `aux`
+ * (generator_hess_intor_j3c_by_aux, cint_handling.rs:80-92): out (device) [nao, nao, n01, comp].
+ * Synchronous. */
+hh_status hh_cint_int3c2e(const hh_cint* mol, const hh_cint* aux, int32_t kind, int32_t sh0, int32_t sh1,
+                          double* out, void* stream);
+/* An hh_int3c_deriv_fn for hh_rijk_skeleton over the engine: ctx = an hh_cint_pair made from the
+ * AO and auxiliary bases; the auxiliary ranges it is called with must be shell aligned. */
+typedef struct hh_cint_pair hh_cint_pair;
+hh_status hh_cint_pair_create(const hh_cint* mol, const hh_cint* aux, hh_cint_pair** out);
+hh_status hh_cint_pair_destroy(hh_cint_pair* p);
+int32_t hh_cint_int3c_deriv(void* ctx, int32_t kind, int64_t p0, int64_t p1, double* out, void* stream);
+
+/* ---------------------------------------------------------------------------------------- */
 /* Numerical-integration grid contractions (src/numint_matmul)                               */
 
 typedef struct hh_numint hh_numint;
@@ -488,6 +536,10 @@ hh_status hh_cphf_batch_add(hh_cphf_batch* b, hh_cphf* drv, hh_rijk* rijk, const
 hh_status hh_cphf_batch_solve(hh_cphf_batch* b, hh_krylov_stats* stats);
 /* Systems in the batch, lockstep cycles of the last solve and the most launches one cycle made. */
 hh_status hh_cphf_batch_info(hh_cphf_batch* b, int64_t* nsys, int64_t* cycles, int64_t* launches_per_cycle);
+/* Wall seconds of the last solve's phases: the resident matrices' builds, set-up with the first
+ * (right-hand side) block, the lockstep cycles, the projected solves with the solutions'
+ * assembly. */
+hh_status hh_cphf_batch_times(hh_cphf_batch* b, double* seconds4);
 
 /* ---------------------------------------------------------------------------------------- */
 /* Learned diagonal CPHF preconditioner (no reference counterpart: the reference preconditions */

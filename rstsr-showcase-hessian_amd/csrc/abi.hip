@@ -5,6 +5,7 @@
 #include "gemm_nt.hpp"
 #include "cphf.hpp"
 #include "cphf_batch.hpp"
+#include "cint.hpp"
 #include "linalg.hpp"
 #include "mlp.hpp"
 #include "numint.hpp"
@@ -115,6 +116,14 @@ struct hh_mlp {
 struct hh_cphf {
   std::unique_ptr<hh::Cphf> impl;
   std::vector<std::unique_ptr<hh::ElecResponse>> comps;
+};
+
+struct hh_cint {
+  std::unique_ptr<hh::Cint> impl;
+};
+struct hh_cint_pair {
+  const hh::Cint* mol;
+  const hh::Cint* aux;
 };
 
 struct hh_cphf_batch {
@@ -381,6 +390,89 @@ hh_status hh_cphf_add_rijk(hh_cphf* h, hh_rijk* rijk) {
   HH_ABI_END
 }
 
+hh_status hh_cint_create(const int32_t* atm, int32_t natm, const int32_t* bas, int32_t nbas, const double* env,
+                         int64_t nenv, int32_t cart, hh_cint** out) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(out, "null output handle");
+  auto* h = new hh_cint;
+  try {
+    h->impl = std::make_unique<Cint>(atm, natm, bas, nbas, env, nenv, cart != 0);
+  } catch (...) {
+    delete h;
+    throw;
+  }
+  *out = h;
+  HH_ABI_END
+}
+
+hh_status hh_cint_destroy(hh_cint* h) {
+  HH_ABI_BEGIN
+  delete h;
+  HH_ABI_END
+}
+
+hh_status hh_cint_nao(const hh_cint* h, int64_t* nao, int64_t* ao_loc) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h, "null handle");
+  if (nao) *nao = h->impl->nao();
+  if (ao_loc)
+    for (int i = 0; i <= h->impl->nbas(); ++i) ao_loc[i] = h->impl->ao_loc(i);
+  HH_ABI_END
+}
+
+hh_status hh_cint_eval_gto(const hh_cint* h, int32_t deriv, const double* coords, int64_t ng, double* out,
+                           void* stream) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h, "null handle");
+  h->impl->eval_gto(deriv, coords, ng, out, (hipStream_t)stream);
+  HH_ABI_END
+}
+
+hh_status hh_cint_int2c2e(const hh_cint* aux, int32_t kind, double* out, void* stream) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(aux, "null handle");
+  aux->impl->int2c2e(kind, 0, aux->impl->nbas(), out, (hipStream_t)stream);
+  HH_ABI_END
+}
+
+hh_status hh_cint_int3c2e(const hh_cint* mol, const hh_cint* aux, int32_t kind, int32_t sh0, int32_t sh1,
+                          double* out, void* stream) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(mol && aux, "null handle");
+  Cint::int3c2e(*mol->impl, *aux->impl, kind, sh0, sh1, out, (hipStream_t)stream);
+  HH_ABI_END
+}
+
+hh_status hh_cint_pair_create(const hh_cint* mol, const hh_cint* aux, hh_cint_pair** out) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(mol && aux && out, "null argument");
+  *out = new hh_cint_pair{mol->impl.get(), aux->impl.get()};
+  HH_ABI_END
+}
+
+hh_status hh_cint_pair_destroy(hh_cint_pair* p) {
+  HH_ABI_BEGIN
+  delete p;
+  HH_ABI_END
+}
+
+int32_t hh_cint_int3c_deriv(void* ctx, int32_t kind, int64_t p0, int64_t p1, double* out, void* stream) {
+  HH_ABI_BEGIN
+  auto* pr = static_cast<hh_cint_pair*>(ctx);
+  HH_REQUIRE(pr && pr->mol && pr->aux, "null integral pair");
+  static const int map[6] = {HH_INT_IP1, HH_INT_IP2, HH_INT_IPIP1, HH_INT_IPVIP1, HH_INT_IP1IP2, HH_INT_IPIP2};
+  HH_REQUIRE(kind >= 0 && kind < 6, "unknown skeleton integral kind");
+  // AO range -> shell range
+  int sh0 = -1, sh1 = -1;
+  for (int i = 0; i <= pr->aux->nbas(); ++i) {
+    if (pr->aux->ao_loc(i) == p0) sh0 = i;
+    if (pr->aux->ao_loc(i) == p1) sh1 = i;
+  }
+  HH_REQUIRE(sh0 >= 0 && sh1 >= sh0, "auxiliary range not shell aligned");
+  Cint::int3c2e(*pr->mol, *pr->aux, map[kind], sh0, sh1, out, (hipStream_t)stream);
+  HH_ABI_END
+}
+
 hh_status hh_cphf_batch_create(double tol, int32_t max_cycle, double lindep, void* stream, hh_cphf_batch** out) {
   HH_ABI_BEGIN
   HH_REQUIRE(out, "null output handle");
@@ -441,6 +533,13 @@ hh_status hh_cphf_batch_info(hh_cphf_batch* b, int64_t* nsys, int64_t* cycles, i
   if (nsys) *nsys = (int64_t)b->impl->size();
   if (cycles) *cycles = b->impl->cycles();
   if (launches_per_cycle) *launches_per_cycle = b->impl->launches_per_cycle();
+  HH_ABI_END
+}
+
+hh_status hh_cphf_batch_times(hh_cphf_batch* b, double* seconds4) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(b && seconds4, "null argument");
+  for (int i = 0; i < 4; ++i) seconds4[i] = b->impl->phase_seconds()[i];
   HH_ABI_END
 }
 

@@ -1,0 +1,773 @@
+// Gaussian-basis AO values and two-electron Coulomb integrals on the device (see cint.hpp).
+//
+// AO values (eval_gto): one thread per (grid point, contracted shell); the Cartesian monomial
+// x^i y^j z^k e^{-a r^2} and its derivatives factor per direction, with
+// d^n/dx^n [x^k e^{-a x^2}] = p_{k,n}(x) e^{-a x^2}, p_{k,0} = x^k,
+// p_{k,n+1} = k p_{k-1,n} - 2 a p_{k+1,n}; contracted over primitives, then taken to real solid
+// harmonics.
+//
+// Coulomb integrals (ab|c) over primitive Cartesian Gaussians by McMurchie-Davidson: the product
+// a b expanded in Hermite Gaussians at P (coefficients E^{ij}_t per direction), c at its own
+// centre, (Lambda_tuv | Lambda_tau nu phi) = (-1)^{tau+nu+phi} 2 pi^{5/2} / (p g sqrt(p + g))
+// R_{t+tau, u+nu, v+phi}(rho, P - C), the R table from the Boys function by the Hermite
+// recursion.  Derivative integrals (libcint's ip: the gradient with respect to the electron
+// coordinate acting on one function) are combinations of integrals over Cartesian functions of
+// shifted angular momentum: d/dx [x^i e^{-a x^2}] = i x^{i-1} e^{..} - 2 a x^{i+1} e^{..}.
+// 2-centre integrals (a|c) are the same with the unit function (exponent 0) as b.
+//
+// One workgroup (one wave) per contracted shell triple; every per-triple table lives in LDS.
+#include "cint.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <tuple>
+
+namespace hh {
+
+namespace {
+
+constexpr int LMAXS = Cint::LMAX;
+constexpr int C2S_TOTAL = 476;  // sum over l <= 5 of (2l + 1)(l + 1)(l + 2)/2
+__constant__ double c_c2s[C2S_TOTAL];
+
+__host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+__host__ __device__ constexpr int c2s_off(int l) {
+  return l == 0 ? 0 : l == 1 ? 1 : l == 2 ? 10 : l == 3 ? 40 : l == 4 ? 110 : 245;
+}
+// index of (lx, ly, lz) within its shell in libcint's Cartesian order (x^l, x^{l-1} y, x^{l-1} z, ...)
+__host__ __device__ constexpr int cart_index(int l, int lx, int lz) { return (l - lx) * (l - lx + 1) / 2 + lz; }
+// the s / p factors libcint puts on Cartesian functions
+__host__ __device__ constexpr double cart_fac(int l) {
+  return l == 0 ? 0.282094791773878143 : l == 1 ? 0.488602511902919921 : 1.0;
+}
+
+double binom(int n, int k) {
+  if (k < 0 || k > n) return 0.0;
+  double r = 1.0;
+  for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+  return r;
+}
+double fact(int n) {
+  double r = 1.0;
+  for (int i = 2; i <= n; ++i) r *= i;
+  return r;
+}
+
+// Real solid harmonics r^l Y_lm (unit-sphere normalised) as Cartesian polynomials: the closed
+// form of Helgaker, Jorgensen and Olsen, Molecular Electronic-Structure Theory, eqs. 6.4.47-48
+// (S_lm with norm 4 pi / (2l + 1) on the sphere), times sqrt((2l + 1) / (4 pi)).  Rows in
+// libcint's order: m = -l .. l, and x, y, z for l = 1.
+std::vector<double> c2s_host() {
+  std::vector<double> t(C2S_TOTAL, 0.0);
+  for (int l = 0; l <= LMAXS; ++l) {
+    const int nc = ncart(l);
+    const double unit = std::sqrt((2.0 * l + 1.0) / (4.0 * M_PI));
+    for (int m = -l; m <= l; ++m) {
+      const int am = std::abs(m);
+      const double nlm = 1.0 / (std::pow(2.0, am) * fact(l)) *
+                         std::sqrt(2.0 * fact(l + am) * fact(l - am) / (m == 0 ? 2.0 : 1.0));
+      int row = m + l;
+      if (l == 1) row = (m == 1) ? 0 : (m == -1) ? 1 : 2;
+      double* r = &t[c2s_off(l) + row * nc];
+      for (int tt = 0; tt <= (l - am) / 2; ++tt)
+        for (int u = 0; u <= tt; ++u)
+          for (int v2 = (m < 0 ? 1 : 0); v2 <= am; v2 += 2) {
+            // (-1)^(t + v - v_m) with v = v2 / 2, v_m = 1/2 for m < 0
+            const int sgn_exp = tt + (v2 - (m < 0 ? 1 : 0)) / 2;
+            const double c = ((sgn_exp & 1) ? -1.0 : 1.0) * std::pow(0.25, tt) * binom(l, tt) * binom(l - tt, am + tt) *
+                             binom(tt, u) * binom(am, v2);
+            const int px = 2 * tt + am - 2 * u - v2, py = 2 * u + v2, pz = l - 2 * tt - am;
+            r[cart_index(l, px, pz)] += unit * nlm * c;
+            (void)py;
+          }
+    }
+  }
+  return t;
+}
+
+void upload_c2s() {
+  static bool done = false;
+  if (done) return;
+  std::vector<double> t = c2s_host();
+  HH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_c2s), t.data(), sizeof(double) * C2S_TOTAL));
+  done = true;
+}
+
+// ---------------------------------------------------------------------------------------------
+// AO values
+
+// out[k] = p_{k, n}(x) for k <= L (powers xp[k] = x^k, k <= L + n)
+template <int L, int NMAX>
+__device__ __forceinline__ void ptab(const double* xp, int n, double a, double* out) {
+  double b[L + NMAX + 1];
+#pragma unroll
+  for (int k = 0; k <= L + NMAX; ++k) b[k] = xp[k];
+#pragma unroll
+  for (int j = 0; j < NMAX; ++j) {
+    if (j < n) {
+      double prev = 0.0;  // old b[k - 1]
+#pragma unroll
+      for (int k = 0; k < L + NMAX - j; ++k) {
+        const double cur = b[k];
+        b[k] = k * prev - 2.0 * a * b[k + 1];
+        prev = cur;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k <= L; ++k) out[k] = b[k];
+}
+
+template <int L, int D>
+__global__ __launch_bounds__(256) void eval_gto_kernel(const CShell* __restrict__ cs, const int* __restrict__ list,
+                                                       const double* __restrict__ prim,
+                                                       const double* __restrict__ coords, int64_t ng, int64_t nao,
+                                                       int cart, double* __restrict__ out) {
+  constexpr int NC = ncart(L);
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= ng) return;
+  const CShell sh = cs[list[blockIdx.y]];
+  const double x = coords[3 * g] - sh.x, y = coords[3 * g + 1] - sh.y, z = coords[3 * g + 2] - sh.z;
+  const double r2 = x * x + y * y + z * z;
+  double xp[L + D + 1], yp[L + D + 1], zp[L + D + 1];
+  xp[0] = yp[0] = zp[0] = 1.0;
+#pragma unroll
+  for (int k = 1; k <= L + D; ++k) {
+    xp[k] = xp[k - 1] * x;
+    yp[k] = yp[k - 1] * y;
+    zp[k] = zp[k - 1] * z;
+  }
+  const double* ex = prim + sh.prim_off;
+  const double* co = ex + sh.nprim;
+  int d = 0;
+#pragma unroll
+  for (int n = 0; n <= D; ++n)
+#pragma unroll
+    for (int dx = n; dx >= 0; --dx)
+#pragma unroll
+      for (int dy = n - dx; dy >= 0; --dy) {
+        const int dz = n - dx - dy;
+        double acc[NC];
+#pragma unroll
+        for (int i = 0; i < NC; ++i) acc[i] = 0.0;
+        for (int p = 0; p < sh.nprim; ++p) {
+          const double a = ex[p];
+          if (a * r2 > 60.0) continue;
+          const double e = co[p] * exp(-a * r2);
+          double px[L + 1], py[L + 1], pz[L + 1];
+          ptab<L, D>(xp, dx, a, px);
+          ptab<L, D>(yp, dy, a, py);
+          ptab<L, D>(zp, dz, a, pz);
+#pragma unroll
+          for (int lx = L; lx >= 0; --lx)
+#pragma unroll
+            for (int ly = L - lx; ly >= 0; --ly) {
+              const int lz = L - lx - ly;
+              acc[cart_index(L, lx, lz)] += e * px[lx] * py[ly] * pz[lz];
+            }
+        }
+        double* o = out + g + ng * (sh.ao0 + nao * (int64_t)d);
+        if (cart) {
+#pragma unroll
+          for (int i = 0; i < NC; ++i) o[ng * i] = cart_fac(L) * acc[i];
+        } else {
+#pragma unroll
+          for (int m = 0; m < 2 * L + 1; ++m) {
+            double v = 0.0;
+#pragma unroll
+            for (int i = 0; i < NC; ++i) v += c_c2s[c2s_off(L) + m * NC + i] * acc[i];
+            o[ng * m] = v;
+          }
+        }
+        ++d;
+      }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Coulomb integrals
+
+// Boys function F_n(T), n = 0 .. nmax: for T <= 35 the series
+// F_n(T) = e^{-T} sum_k (2T)^k / ((2n+1)(2n+3)...(2n+2k+1)) at n = nmax and the downward
+// recursion F_{n-1} = (2T F_n + e^{-T}) / (2n - 1); above, F_0 from erf and the upward recursion
+// F_{n+1} = ((2n+1) F_n - e^{-T}) / (2T) (stable there: T > nmax).
+__device__ void boys(int nmax, double T, double* F) {
+  if (T <= 35.0) {
+    const double et = exp(-T);
+    double term = 1.0 / (2 * nmax + 1), sum = term;
+    for (int k = 1; k < 400; ++k) {
+      term *= 2.0 * T / (2 * nmax + 2 * k + 1);
+      sum += term;
+      if (term < 1e-17 * sum) break;
+    }
+    F[nmax] = et * sum;
+    for (int n = nmax; n > 0; --n) F[n - 1] = (2.0 * T * F[n] + et) / (2 * n - 1);
+  } else {
+    const double et = exp(-T);
+    F[0] = 0.5 * sqrt(M_PI / T) * erf(sqrt(T));
+    for (int n = 0; n < nmax; ++n) F[n + 1] = ((2 * n + 1) * F[n] - et) / (2.0 * T);
+  }
+}
+
+__device__ __forceinline__ int tet(int n) { return n * (n + 1) * (n + 2) / 6; }
+// compact index of (t, u, v), t + u + v <= L
+__device__ __forceinline__ int ridx(int t, int u, int v, int L) {
+  const int m = L - t;
+  return tet(L + 1) - tet(L + 1 - t) + u * (m + 1) - u * (u - 1) / 2 + v;
+}
+
+struct TripleArgs {
+  const CShell* csa;  // shells of the a / c positions (3c: a, b from mol, c from aux)
+  const CShell* csb;
+  const CShell* csc;
+  const double* prima;
+  const double* primb;
+  const double* primc;
+  const int* la_list;  // contracted shells of this group
+  const int* lb_list;
+  const int* lc_list;
+  int na_list, nb_list, nc_list;
+  int la, lb, lc;      // angular momenta of the group (lb = 0 and two_c for 2-centre)
+  int two_c;
+  int nda, ndb, ndc;   // derivatives on a / b / c
+  int kind;            // HH_INT_*
+  int q0, q1;          // derivative components of this launch
+  int cart;
+  // output: element (ia, ib, ic, q) of the triple at
+  // out[(ao_a + ia) + s1 ((ao_b + ib) + s2 ((ao_c + ic - c0) + s3 q))]
+  double* out;
+  int64_t s1, s2, s3, c0;
+  // LDS layout (doubles)
+  int LA, LB, LC, LT, NR;
+  int nA, nB, nC;      // sizes of the shifted Cartesian sets
+  int loA, loB, loC;   // lowest l of each set
+  int offR, offE, offEc, offF, offI, offAcc, offSet, lds_doubles;
+};
+
+// derivative terms of libcint's ip on a Cartesian function: up to 4 (coef, shifted lx, ly, lz)
+__device__ __forceinline__ int deriv_terms(int lx, int ly, int lz, double a, int nd, int d1, int d2, double* cf,
+                                           int (*sh)[3]) {
+  int n = 1;
+  cf[0] = 1.0;
+  sh[0][0] = lx; sh[0][1] = ly; sh[0][2] = lz;
+  const int dirs[2] = {d1, d2};
+  for (int s = 0; s < nd; ++s) {
+    const int d = dirs[s];
+    double cf2[4];
+    int sh2[4][3];
+    int m = 0;
+    for (int k = 0; k < n; ++k) {
+      const int l = sh[k][d];
+      if (l > 0) {
+        cf2[m] = cf[k] * l;
+        sh2[m][0] = sh[k][0]; sh2[m][1] = sh[k][1]; sh2[m][2] = sh[k][2];
+        sh2[m][d] = l - 1;
+        ++m;
+      }
+      cf2[m] = cf[k] * (-2.0 * a);
+      sh2[m][0] = sh[k][0]; sh2[m][1] = sh[k][1]; sh2[m][2] = sh[k][2];
+      sh2[m][d] = l + 1;
+      ++m;
+    }
+    n = m;
+    for (int k = 0; k < n; ++k) {
+      cf[k] = cf2[k];
+      sh[k][0] = sh2[k][0]; sh[k][1] = sh2[k][1]; sh[k][2] = sh2[k][2];
+    }
+  }
+  return n;
+}
+
+__device__ __forceinline__ int set_index(int lo, int lx, int ly, int lz) {
+  const int l = lx + ly + lz;
+  return tet(l) - tet(lo) + cart_index(l, lx, lz);
+}
+
+__global__ __launch_bounds__(64) void int_triple_kernel(TripleArgs A) {
+  extern __shared__ double lds[];
+  const int tid = threadIdx.x;
+  const int ia_s = A.la_list[blockIdx.x % A.na_list];
+  const int ib_s = A.two_c ? -1 : A.lb_list[blockIdx.x / A.na_list];
+  const int ic_s = A.lc_list[blockIdx.y];
+  const CShell sa = A.csa[ia_s];
+  CShell sb;
+  if (A.two_c) {
+    sb.x = sa.x; sb.y = sa.y; sb.z = sa.z; sb.l = 0; sb.nprim = 1; sb.prim_off = 0; sb.ao0 = 0;
+  } else {
+    sb = A.csb[ib_s];
+  }
+  const CShell sc = A.csc[ic_s];
+  double* R0 = lds + A.offR;
+  double* R1 = R0 + A.NR;
+  double* Eab = lds + A.offE;
+  double* Ec = lds + A.offEc;
+  double* F = lds + A.offF;
+  double* I = lds + A.offI;
+  double* acc = lds + A.offAcc;
+  int* setA = reinterpret_cast<int*>(lds + A.offSet);
+  int* setB = setA + A.nA;
+  int* setC = setB + A.nB;
+  const int LA = A.LA, LB = A.LB, LC = A.LC, LT = A.LT;
+  const int ncA = ncart(A.la), ncB = ncart(A.lb), ncC = ncart(A.lc);
+  const int nq = A.q1 - A.q0;
+  const int nacc = ncA * ncB * ncC * nq;
+  // shifted Cartesian sets (packed lx | ly << 8 | lz << 16)
+  if (tid == 0) {
+    int k = 0;
+    for (int l = A.loA; l <= LA; ++l)
+      for (int lx = l; lx >= 0; --lx)
+        for (int ly = l - lx; ly >= 0; --ly) setA[k++] = lx | (ly << 8) | ((l - lx - ly) << 16);
+    k = 0;
+    for (int l = A.loB; l <= LB; ++l)
+      for (int lx = l; lx >= 0; --lx)
+        for (int ly = l - lx; ly >= 0; --ly) setB[k++] = lx | (ly << 8) | ((l - lx - ly) << 16);
+    k = 0;
+    for (int l = A.loC; l <= LC; ++l)
+      for (int lx = l; lx >= 0; --lx)
+        for (int ly = l - lx; ly >= 0; --ly) setC[k++] = lx | (ly << 8) | ((l - lx - ly) << 16);
+  }
+  for (int e = tid; e < nacc; e += 64) acc[e] = 0.0;
+  __syncthreads();
+  const double* exa = A.prima + sa.prim_off;
+  const double* exb = A.two_c ? nullptr : A.primb + sb.prim_off;
+  const double* exc = A.primc + sc.prim_off;
+  const double ABx = sa.x - sb.x, ABy = sa.y - sb.y, ABz = sa.z - sb.z;
+  const double ab2 = ABx * ABx + ABy * ABy + ABz * ABz;
+  const int nEab = (LA + 1) * (LB + 1) * (LA + LB + 1);
+  const int nEc = (LC + 1) * (LC + 1);
+  for (int pa = 0; pa < sa.nprim; ++pa)
+    for (int pb = 0; pb < sb.nprim; ++pb) {
+      const double al = exa[pa], be = A.two_c ? 0.0 : exb[pb];
+      const double ca = exa[sa.nprim + pa], cb = A.two_c ? 1.0 : exb[sb.nprim + pb];
+      const double p = al + be;
+      const double mu = al * be / p;
+      if (mu * ab2 > 40.0) continue;  // K_AB < 4e-18
+      const double Kab = exp(-mu * ab2);
+      const double Px = (al * sa.x + be * sb.x) / p, Py = (al * sa.y + be * sb.y) / p, Pz = (al * sa.z + be * sb.z) / p;
+      // E^{ij}_t per direction (threads 0..2), E^{ab} index ((i (LB+1) + j) (LA+LB+1) + t)
+      if (tid < 3) {
+        const double PA = (tid == 0 ? Px - sa.x : tid == 1 ? Py - sa.y : Pz - sa.z);
+        const double PB = (tid == 0 ? Px - sb.x : tid == 1 ? Py - sb.y : Pz - sb.z);
+        double* E = Eab + tid * nEab;
+        const int W = LA + LB + 1;
+        for (int e = 0; e < nEab; ++e) E[e] = 0.0;
+        E[0] = 1.0;
+        const double h = 0.5 / p;
+        for (int i = 0; i <= LA; ++i) {
+          if (i > 0) {  // (i, 0) from (i-1, 0)
+            const double* s = E + ((i - 1) * (LB + 1)) * W;
+            double* d = E + (i * (LB + 1)) * W;
+            for (int t = 0; t <= i; ++t)
+              d[t] = (t > 0 ? h * s[t - 1] : 0.0) + PA * s[t] + (t + 1 <= i - 1 ? (t + 1) * s[t + 1] : 0.0);
+          }
+          for (int j = 1; j <= LB; ++j) {  // (i, j) from (i, j-1)
+            const double* s = E + (i * (LB + 1) + j - 1) * W;
+            double* d = E + (i * (LB + 1) + j) * W;
+            for (int t = 0; t <= i + j; ++t)
+              d[t] = (t > 0 ? h * s[t - 1] : 0.0) + PB * s[t] + (t + 1 <= i + j - 1 ? (t + 1) * s[t + 1] : 0.0);
+          }
+        }
+      }
+      for (int pc = 0; pc < sc.nprim; ++pc) {
+        const double ga = exc[pc], cc = exc[sc.nprim + pc];
+        const double rho = p * ga / (p + ga);
+        const double PCx = Px - sc.x, PCy = Py - sc.y, PCz = Pz - sc.z;
+        const double T = rho * (PCx * PCx + PCy * PCy + PCz * PCz);
+        const double pref = 2.0 * pow(M_PI, 2.5) / (p * ga * sqrt(p + ga)) * Kab;
+        if (tid == 3) boys(LT, T, F);
+        if (tid >= 4 && tid < 7) {  // E^c per direction: E[k (LC+1) + tau]
+          const int dd = tid - 4;
+          double* E = Ec + dd * nEc;
+          for (int e = 0; e < nEc; ++e) E[e] = 0.0;
+          E[0] = 1.0;
+          const double h = 0.5 / ga;
+          for (int k = 1; k <= LC; ++k)
+            for (int t = 0; t <= k; ++t)
+              E[k * (LC + 1) + t] = (t > 0 ? h * E[(k - 1) * (LC + 1) + t - 1] : 0.0) +
+                                    (t + 1 <= k - 1 ? (t + 1) * E[(k - 1) * (LC + 1) + t + 1] : 0.0);
+        }
+        __syncthreads();
+        // R^n_tuv, n = LT .. 0 (level n holds t + u + v <= LT - n)
+        double* Rn = (LT % 2 == 0) ? R0 : R1;
+        if (tid == 0) Rn[0] = pow(-2.0 * rho, LT) * F[LT];
+        __syncthreads();
+        for (int n = LT - 1; n >= 0; --n) {
+          const double* Rp = Rn;
+          Rn = (n % 2 == 0) ? R0 : R1;
+          const int Ln = LT - n;
+          const int cnt = tet(Ln + 1);
+          const double r00 = pow(-2.0 * rho, n) * F[n];
+          for (int e = tid; e < cnt; e += 64) {
+            // decode e (compact order of sums <= Ln) into (t, u, v)
+            int t = 0, rem = e;
+            while (true) {
+              const int sz = (Ln - t + 1) * (Ln - t + 2) / 2;
+              if (rem < sz) break;
+              rem -= sz;
+              ++t;
+            }
+            int u = 0;
+            while (true) {
+              const int sz = Ln - t - u + 1;
+              if (rem < sz) break;
+              rem -= sz;
+              ++u;
+            }
+            const int v = rem;
+            double val;
+            if (t > 0)
+              val = (t > 1 ? (t - 1) * Rp[ridx(t - 2, u, v, LT)] : 0.0) + PCx * Rp[ridx(t - 1, u, v, LT)];
+            else if (u > 0)
+              val = (u > 1 ? (u - 1) * Rp[ridx(t, u - 2, v, LT)] : 0.0) + PCy * Rp[ridx(t, u - 1, v, LT)];
+            else if (v > 0)
+              val = (v > 1 ? (v - 1) * Rp[ridx(t, u, v - 2, LT)] : 0.0) + PCz * Rp[ridx(t, u, v - 1, LT)];
+            else
+              val = r00;
+            Rn[ridx(t, u, v, LT)] = val;
+          }
+          __syncthreads();
+        }
+        // primitive integrals over the shifted sets: I[(a nB + b) nC + c]
+        const int W = LA + LB + 1;
+        const int ntot = A.nA * A.nB * A.nC;
+        for (int e = tid; e < ntot; e += 64) {
+          const int ic = e % A.nC, ib = (e / A.nC) % A.nB, iA = e / (A.nC * A.nB);
+          const int a = setA[iA], b = setB[ib], c = setC[ic];
+          const int ax = a & 255, ay = (a >> 8) & 255, az = a >> 16;
+          const int bx = b & 255, by = (b >> 8) & 255, bz = b >> 16;
+          const int cx = c & 255, cy = (c >> 8) & 255, cz = c >> 16;
+          const double* Ex = Eab + (ax * (LB + 1) + bx) * W;
+          const double* Ey = Eab + nEab + (ay * (LB + 1) + by) * W;
+          const double* Ez = Eab + 2 * nEab + (az * (LB + 1) + bz) * W;
+          const double* Gx = Ec + cx * (LC + 1);
+          const double* Gy = Ec + nEc + cy * (LC + 1);
+          const double* Gz = Ec + 2 * nEc + cz * (LC + 1);
+          double s = 0.0;
+          for (int t = 0; t <= ax + bx; ++t)
+            for (int u = 0; u <= ay + by; ++u)
+              for (int v = 0; v <= az + bz; ++v) {
+                const double eab = Ex[t] * Ey[u] * Ez[v];
+                if (eab == 0.0) continue;
+                double hsum = 0.0;
+                for (int tau = cx & 1; tau <= cx; tau += 2)
+                  for (int nu = cy & 1; nu <= cy; nu += 2)
+                    for (int ph = cz & 1; ph <= cz; ph += 2) {
+                      const double g = Gx[tau] * Gy[nu] * Gz[ph];
+                      hsum += (((tau + nu + ph) & 1) ? -g : g) * R0[ridx(t + tau, u + nu, v + ph, LT)];
+                    }
+                s += eab * hsum;
+              }
+          I[e] = pref * s;
+        }
+        __syncthreads();
+        // derivative integrals of the Cartesian components, contracted into acc
+        const double wgt = ca * cb * cc;
+        for (int e = tid; e < nacc; e += 64) {
+          const int q = A.q0 + e % nq;
+          const int rest = e / nq;
+          const int jc = rest % ncC, jb = (rest / ncC) % ncB, ja = rest / (ncC * ncB);
+          // Cartesian components of the output functions
+          int lxa = 0, lya = 0, lza = 0, lxb = 0, lyb = 0, lzb = 0, lxc = 0, lyc = 0, lzc = 0;
+          {
+            int k = 0;
+            for (int lx = A.la; lx >= 0; --lx)
+              for (int ly = A.la - lx; ly >= 0; --ly, ++k)
+                if (k == ja) { lxa = lx; lya = ly; lza = A.la - lx - ly; }
+            k = 0;
+            for (int lx = A.lb; lx >= 0; --lx)
+              for (int ly = A.lb - lx; ly >= 0; --ly, ++k)
+                if (k == jb) { lxb = lx; lyb = ly; lzb = A.lb - lx - ly; }
+            k = 0;
+            for (int lx = A.lc; lx >= 0; --lx)
+              for (int ly = A.lc - lx; ly >= 0; --ly, ++k)
+                if (k == jc) { lxc = lx; lyc = ly; lzc = A.lc - lx - ly; }
+          }
+          // derivative directions per function: comp q of 3 (one derivative) or 9 = 3 t + s
+          int da1 = 0, da2 = 0, db1 = 0, dc1 = 0, dc2 = 0;
+          switch (A.kind) {
+            case HH_INT_IP1: da1 = q; break;
+            case HH_INT_IP2: dc1 = q; break;
+            case HH_INT_IPIP1: da1 = q / 3; da2 = q % 3; break;
+            case HH_INT_IPVIP1: da1 = q / 3; db1 = q % 3; break;
+            case HH_INT_IP1IP2: da1 = q / 3; dc1 = q % 3; break;
+            case HH_INT_IPIP2: dc1 = q / 3; dc2 = q % 3; break;
+            default: break;
+          }
+          double cfa[4], cfb[4], cfc[4];
+          int sha[4][3], shb[4][3], shc[4][3];
+          const int nta = deriv_terms(lxa, lya, lza, al, A.nda, da1, da2, cfa, sha);
+          const int ntb = deriv_terms(lxb, lyb, lzb, be, A.ndb, db1, 0, cfb, shb);
+          const int ntc = deriv_terms(lxc, lyc, lzc, ga, A.ndc, dc1, dc2, cfc, shc);
+          double v = 0.0;
+          for (int x = 0; x < nta; ++x) {
+            const int iA = set_index(A.loA, sha[x][0], sha[x][1], sha[x][2]);
+            for (int y = 0; y < ntb; ++y) {
+              const int iB = set_index(A.loB, shb[y][0], shb[y][1], shb[y][2]);
+              for (int z = 0; z < ntc; ++z) {
+                const int iC = set_index(A.loC, shc[z][0], shc[z][1], shc[z][2]);
+                v += cfa[x] * cfb[y] * cfc[z] * I[(iA * A.nB + iB) * A.nC + iC];
+              }
+            }
+          }
+          acc[e] += wgt * v;
+        }
+        __syncthreads();
+      }
+    }
+  // to spherical (or Cartesian with libcint's s / p factors) and out
+  const int nsa = A.cart ? ncA : 2 * A.la + 1, nsb = A.cart ? ncB : 2 * A.lb + 1, nsc = A.cart ? ncC : 2 * A.lc + 1;
+  const int nout = nsa * nsb * nsc * nq;
+  for (int e = tid; e < nout; e += 64) {
+    const int qq = e % nq;
+    const int rest = e / nq;
+    const int mc = rest % nsc, mb = (rest / nsc) % nsb, ma = rest / (nsc * nsb);
+    double v = 0.0;
+    if (A.cart) {
+      v = acc[((ma * ncB + mb) * ncC + mc) * nq + qq] * cart_fac(A.la) * (A.two_c ? 1.0 : cart_fac(A.lb)) *
+          cart_fac(A.lc);
+    } else {
+      const double* ra = c_c2s + c2s_off(A.la) + ma * ncA;
+      const double* rb = c_c2s + c2s_off(A.lb) + mb * ncB;
+      const double* rc = c_c2s + c2s_off(A.lc) + mc * ncC;
+      for (int ja = 0; ja < ncA; ++ja) {
+        if (ra[ja] == 0.0) continue;
+        for (int jb = 0; jb < ncB; ++jb) {
+          const double wb = A.two_c ? 1.0 : rb[jb];
+          if (wb == 0.0) continue;
+          for (int jc = 0; jc < ncC; ++jc)
+            v += ra[ja] * wb * rc[jc] * acc[((ja * ncB + jb) * ncC + jc) * nq + qq];
+        }
+      }
+    }
+    const int64_t ao_b = A.two_c ? 0 : sb.ao0 + mb;
+    const int64_t idx = (sa.ao0 + ma) + A.s1 * (ao_b + A.s2 * ((sc.ao0 + mc - A.c0) + A.s3 * (A.q0 + qq)));
+    A.out[idx] = v;
+  }
+}
+
+int kind_ncomp(int kind) {
+  switch (kind) {
+    case HH_INT_PLAIN: return 1;
+    case HH_INT_IP1: case HH_INT_IP2: return 3;
+    case HH_INT_IPIP1: case HH_INT_IPVIP1: case HH_INT_IP1IP2: case HH_INT_IPIP2: return 9;
+    default: throw Error(HH_ERR_INVALID, "invalid argument: unknown integral kind");
+  }
+}
+
+void kind_derivs(int kind, int* na, int* nb, int* nc) {
+  *na = *nb = *nc = 0;
+  switch (kind) {
+    case HH_INT_IP1: *na = 1; break;
+    case HH_INT_IP2: *nc = 1; break;
+    case HH_INT_IPIP1: *na = 2; break;
+    case HH_INT_IPVIP1: *na = 1; *nb = 1; break;
+    case HH_INT_IP1IP2: *na = 1; *nc = 1; break;
+    case HH_INT_IPIP2: *nc = 2; break;
+    default: break;
+  }
+}
+
+int tet_h(int n) { return n * (n + 1) * (n + 2) / 6; }
+
+// Launch the triple kernel for every (la, lb, lc) group.  a / c lists: contracted shells per
+// angular momentum; b likewise (ignored for 2-centre).
+void launch_triples(const Cint* ma, const Cint* mb, const Cint* mc, const std::vector<int>& alist_all,
+                    const std::vector<int>& blist_all, const std::vector<int>& clist_all, bool two_c, int kind,
+                    bool cart, double* out, int64_t s1, int64_t s2, int64_t s3, int64_t c0, hipStream_t s,
+                    const DevBuf& d_csa, const DevBuf& d_prima, const DevBuf& d_csb, const DevBuf& d_primb,
+                    const DevBuf& d_csc, const DevBuf& d_primc) {
+  upload_c2s();
+  const int ncomp = kind_ncomp(kind);
+  int na, nb, nc;
+  kind_derivs(kind, &na, &nb, &nc);
+  if (two_c) HH_REQUIRE(nb == 0, "2-centre integrals: no derivative on the unit function");
+  // group shells by l
+  auto group = [](const Cint* m, const std::vector<int>& list) {
+    std::map<int, std::vector<int>> g;
+    for (int i : list) g[m->cshells()[i].l].push_back(i);
+    return g;
+  };
+  auto ga = group(ma, alist_all);
+  auto gc = group(mc, clist_all);
+  std::map<int, std::vector<int>> gb;
+  if (two_c) gb[0] = {0};
+  else gb = group(mb, blist_all);
+  static int max_lds = 0;
+  if (max_lds == 0) {
+    int dev = 0;
+    HH_HIP(hipGetDevice(&dev));
+    HH_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    HH_HIP(hipFuncSetAttribute((const void*)int_triple_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+  }
+  std::vector<DevBuf*> keep;
+  for (auto& [la, al] : ga)
+    for (auto& [lb, bl] : gb)
+      for (auto& [lc, cl] : gc) {
+        TripleArgs A{};
+        A.csa = d_csa.as<CShell>(); A.prima = d_prima.as<double>();
+        A.csb = two_c ? nullptr : d_csb.as<CShell>(); A.primb = two_c ? nullptr : d_primb.as<double>();
+        A.csc = d_csc.as<CShell>(); A.primc = d_primc.as<double>();
+        A.la = la; A.lb = lb; A.lc = lc; A.two_c = two_c ? 1 : 0;
+        A.nda = na; A.ndb = nb; A.ndc = nc; A.kind = kind; A.cart = cart ? 1 : 0;
+        A.out = out; A.s1 = s1; A.s2 = s2; A.s3 = s3; A.c0 = c0;
+        A.LA = la + na; A.LB = lb + nb; A.LC = lc + nc; A.LT = A.LA + A.LB + A.LC;
+        A.loA = std::max(0, la - na); A.loB = std::max(0, lb - nb); A.loC = std::max(0, lc - nc);
+        A.nA = tet_h(A.LA + 1) - tet_h(A.loA);
+        A.nB = tet_h(A.LB + 1) - tet_h(A.loB);
+        A.nC = tet_h(A.LC + 1) - tet_h(A.loC);
+        A.NR = tet_h(A.LT + 1);
+        const int nEab = (A.LA + 1) * (A.LB + 1) * (A.LA + A.LB + 1);
+        const int nEc = (A.LC + 1) * (A.LC + 1);
+        const int ncarts = ncart(la) * ncart(lb) * ncart(lc);
+        int off = 0;
+        A.offR = off; off += 2 * A.NR;
+        A.offE = off; off += 3 * nEab;
+        A.offEc = off; off += 3 * nEc;
+        A.offF = off; off += A.LT + 1;
+        A.offI = off; off += A.nA * A.nB * A.nC;
+        A.offSet = off; off += (A.nA + A.nB + A.nC + 1) / 2;
+        const int fixed = off;
+        // derivative components per launch so that the accumulator fits
+        int qchunk = ncomp;
+        while (qchunk > 1 && (size_t)(fixed + ncarts * qchunk) * 8 > (size_t)max_lds) qchunk = (qchunk + 1) / 2;
+        HH_REQUIRE((size_t)(fixed + ncarts * qchunk) * 8 <= (size_t)max_lds,
+                   "integral engine: angular momenta too high for the device's LDS");
+        // device lists of this group
+        auto* la_d = new DevBuf; auto* lb_d = new DevBuf; auto* lc_d = new DevBuf;
+        keep.push_back(la_d); keep.push_back(lb_d); keep.push_back(lc_d);
+        la_d->reserve(sizeof(int) * al.size());
+        HH_HIP(hipMemcpyAsync(la_d->ptr, al.data(), sizeof(int) * al.size(), hipMemcpyHostToDevice, s));
+        lb_d->reserve(sizeof(int) * bl.size());
+        HH_HIP(hipMemcpyAsync(lb_d->ptr, bl.data(), sizeof(int) * bl.size(), hipMemcpyHostToDevice, s));
+        lc_d->reserve(sizeof(int) * cl.size());
+        HH_HIP(hipMemcpyAsync(lc_d->ptr, cl.data(), sizeof(int) * cl.size(), hipMemcpyHostToDevice, s));
+        A.la_list = la_d->as<int>(); A.lb_list = lb_d->as<int>(); A.lc_list = lc_d->as<int>();
+        A.na_list = (int)al.size(); A.nb_list = (int)bl.size(); A.nc_list = (int)cl.size();
+        const int64_t nab = (int64_t)al.size() * (two_c ? 1 : (int64_t)bl.size());
+        HH_REQUIRE(nab < (1LL << 31) && cl.size() < 65536, "integral engine: too many shells");
+        for (int q0 = 0; q0 < ncomp; q0 += qchunk) {
+          A.q0 = q0;
+          A.q1 = std::min(ncomp, q0 + qchunk);
+          A.offAcc = fixed;
+          A.lds_doubles = fixed + ncarts * (A.q1 - A.q0);
+          hipLaunchKernelGGL(int_triple_kernel, dim3((unsigned)nab, (unsigned)cl.size()), dim3(64),
+                             sizeof(double) * A.lds_doubles, s, A);
+          HH_HIP(hipGetLastError());
+        }
+      }
+  HH_HIP(hipStreamSynchronize(s));  // the group lists are freed on return
+  for (DevBuf* b : keep) delete b;
+}
+
+}  // namespace
+
+Cint::Cint(const int32_t* atm, int natm, const int32_t* bas, int nbas, const double* env, int64_t nenv, bool cart)
+    : cart_(cart) {
+  HH_REQUIRE(atm && bas && env && natm > 0 && nbas > 0 && nenv > 0, "cint: empty molecule");
+  bas_cs_.push_back(0);
+  bas_ao_.push_back(0);
+  for (int i = 0; i < nbas; ++i) {
+    const int32_t* b = bas + 8 * i;
+    const int atom = b[0], l = b[1], nprim = b[2], nctr = b[3], pexp = b[5], pcoef = b[6];
+    HH_REQUIRE(atom >= 0 && atom < natm, "cint: shell atom out of range");
+    HH_REQUIRE(l >= 0 && l <= LMAX, "cint: angular momentum above 5");
+    HH_REQUIRE(nprim >= 1 && nctr >= 1, "cint: empty shell");
+    HH_REQUIRE(pexp >= 0 && pexp + nprim <= nenv && pcoef >= 0 && pcoef + (int64_t)nprim * nctr <= nenv,
+               "cint: shell data outside env");
+    const int pc = atm[6 * atom + 1];
+    HH_REQUIRE(pc >= 0 && pc + 3 <= nenv, "cint: atom coordinates outside env");
+    const int ncomp = cart ? ncart(l) : 2 * l + 1;
+    for (int c = 0; c < nctr; ++c) {
+      CShell s{};
+      s.x = env[pc]; s.y = env[pc + 1]; s.z = env[pc + 2];
+      s.l = l;
+      s.nprim = nprim;
+      s.prim_off = (int)prim_h_.size();
+      s.ao0 = (int)nao_;
+      for (int p = 0; p < nprim; ++p) prim_h_.push_back(env[pexp + p]);
+      for (int p = 0; p < nprim; ++p) prim_h_.push_back(env[pcoef + p + (int64_t)nprim * c]);
+      cs_.push_back(s);
+      nao_ += ncomp;
+      lmax_ = std::max(lmax_, l);
+    }
+    bas_cs_.push_back((int)cs_.size());
+    bas_ao_.push_back(nao_);
+  }
+  d_cs_.reserve(sizeof(CShell) * cs_.size());
+  HH_HIP(hipMemcpy(d_cs_.ptr, cs_.data(), sizeof(CShell) * cs_.size(), hipMemcpyHostToDevice));
+  d_prim_.reserve(sizeof(double) * prim_h_.size());
+  HH_HIP(hipMemcpy(d_prim_.ptr, prim_h_.data(), sizeof(double) * prim_h_.size(), hipMemcpyHostToDevice));
+}
+
+void Cint::eval_gto(int deriv, const double* coords, int64_t ng, double* out, hipStream_t s) const {
+  HH_REQUIRE(deriv >= 0 && deriv <= 3, "eval_gto: deriv 0..3");
+  HH_REQUIRE(ng >= 0 && (ng == 0 || (coords && out)), "eval_gto: null pointer");
+  if (ng == 0) return;
+  upload_c2s();
+  std::vector<std::vector<int>> byl(LMAX + 1);
+  for (int i = 0; i < (int)cs_.size(); ++i) byl[cs_[i].l].push_back(i);
+  std::vector<int> flat;
+  std::vector<int> start(LMAX + 2, 0);
+  for (int l = 0; l <= LMAX; ++l) {
+    start[l] = (int)flat.size();
+    flat.insert(flat.end(), byl[l].begin(), byl[l].end());
+  }
+  start[LMAX + 1] = (int)flat.size();
+  DevBuf lst;
+  lst.reserve(sizeof(int) * flat.size());
+  HH_HIP(hipMemcpyAsync(lst.ptr, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice, s));
+  const unsigned gx = (unsigned)ceil_div(ng, 256);
+  HH_REQUIRE(ceil_div(ng, 256) < (1LL << 31), "eval_gto: too many points");
+#define HH_EVAL_L(L)                                                                                               \
+  if (!byl[L].empty()) {                                                                                           \
+    const dim3 grid(gx, (unsigned)byl[L].size());                                                                  \
+    const int* lp = lst.as<int>() + start[L];                                                                      \
+    switch (deriv) {                                                                                               \
+      case 0: hipLaunchKernelGGL((eval_gto_kernel<L, 0>), grid, dim3(256), 0, s, d_cs_.as<CShell>(), lp,           \
+                                 d_prim_.as<double>(), coords, ng, nao_, (int)cart_, out); break;                  \
+      case 1: hipLaunchKernelGGL((eval_gto_kernel<L, 1>), grid, dim3(256), 0, s, d_cs_.as<CShell>(), lp,           \
+                                 d_prim_.as<double>(), coords, ng, nao_, (int)cart_, out); break;                  \
+      case 2: hipLaunchKernelGGL((eval_gto_kernel<L, 2>), grid, dim3(256), 0, s, d_cs_.as<CShell>(), lp,           \
+                                 d_prim_.as<double>(), coords, ng, nao_, (int)cart_, out); break;                  \
+      default: hipLaunchKernelGGL((eval_gto_kernel<L, 3>), grid, dim3(256), 0, s, d_cs_.as<CShell>(), lp,          \
+                                  d_prim_.as<double>(), coords, ng, nao_, (int)cart_, out); break;                 \
+    }                                                                                                              \
+    HH_HIP(hipGetLastError());                                                                                     \
+  }
+  HH_EVAL_L(0) HH_EVAL_L(1) HH_EVAL_L(2) HH_EVAL_L(3) HH_EVAL_L(4) HH_EVAL_L(5)
+#undef HH_EVAL_L
+  HH_HIP(hipStreamSynchronize(s));  // the shell list is freed on return
+}
+
+void Cint::int2c2e(int kind, int sh0, int sh1, double* out, hipStream_t s) const {
+  HH_REQUIRE(kind == HH_INT_PLAIN || kind == HH_INT_IP1 || kind == HH_INT_IPIP1 || kind == HH_INT_IP1IP2,
+             "int2c2e: kinds PLAIN, IP1, IPIP1, IP1IP2");
+  HH_REQUIRE(out, "int2c2e: null output");
+  HH_REQUIRE(0 <= sh0 && sh0 <= sh1 && sh1 <= nbas(), "int2c2e: shell range");
+  if (sh0 == sh1) return;
+  std::vector<int> rows, cols;
+  for (int i = bas_cs_[sh0]; i < bas_cs_[sh1]; ++i) rows.push_back(i);
+  for (int i = 0; i < (int)cs_.size(); ++i) cols.push_back(i);
+  const int64_t n01 = bas_ao_[sh1] - bas_ao_[sh0];
+  // element (a, c, q) at out[(a - ao(sh0)) + n01 (c + nao q)]: the a offset is folded into the
+  // output pointer (rows start at AO ao(sh0))
+  double* o = out - bas_ao_[sh0];
+  launch_triples(this, nullptr, this, rows, {}, cols, true, kind, cart_, o, n01, 1, nao_, 0, s, d_cs_, d_prim_, d_cs_,
+                 d_prim_, d_cs_, d_prim_);
+}
+
+void Cint::int3c2e(const Cint& mol, const Cint& aux, int kind, int sh0, int sh1, double* out, hipStream_t s) {
+  kind_ncomp(kind);
+  HH_REQUIRE(out, "int3c2e: null output");
+  HH_REQUIRE(mol.cart_ == aux.cart_, "int3c2e: both bases spherical or both Cartesian");
+  HH_REQUIRE(0 <= sh0 && sh0 <= sh1 && sh1 <= aux.nbas(), "int3c2e: auxiliary shell range");
+  if (sh0 == sh1) return;
+  std::vector<int> a, c;
+  for (int i = 0; i < (int)mol.cs_.size(); ++i) a.push_back(i);
+  for (int i = aux.bas_cs_[sh0]; i < aux.bas_cs_[sh1]; ++i) c.push_back(i);
+  const int64_t n01 = aux.bas_ao_[sh1] - aux.bas_ao_[sh0];
+  launch_triples(&mol, &mol, &aux, a, a, c, false, kind, mol.cart_, out, mol.nao_, mol.nao_, n01, aux.bas_ao_[sh0], s,
+                 mol.d_cs_, mol.d_prim_, mol.d_cs_, mol.d_prim_, aux.d_cs_, aux.d_prim_);
+}
+
+}  // namespace hh

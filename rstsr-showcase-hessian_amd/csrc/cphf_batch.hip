@@ -22,6 +22,7 @@
 #include "cphf_batch.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 
 #include "elementwise.hpp"
@@ -452,6 +453,8 @@ void CphfBatch::solve(std::vector<KrylovStats>* stats) {
   const int64_t nsys = (int64_t)sys_.size();
   if (stats) stats->assign(nsys, KrylovStats{});
   if (nsys == 0) return;
+  using clk = std::chrono::steady_clock;
+  const auto t_start = clk::now();
   if (!pinned_) {
     HH_HIP(hipHostMalloc(&pinned_, ARENA_BYTES, hipHostMallocDefault));
     pinned_bytes_ = ARENA_BYTES;
@@ -459,9 +462,11 @@ void CphfBatch::solve(std::vector<KrylovStats>* stats) {
   desc_.reserve(ARENA_BYTES);
   resd_.reserve(sizeof(double) * 2 * MAX_SYSTEMS);
   const double conv = std::max(lindep_, tol_ * tol_);
-  // per-system buffers and the resident matrices
+  // the resident matrices, then the per-system buffers
+  for (Sys* s : sys_) s->m64 = s->rijk->resident_matrix(&s->R, &s->kpad);
+  HH_HIP(hipStreamSynchronize(stream_));
+  const auto t_built = clk::now();
   for (Sys* s : sys_) {
-    s->m64 = s->rijk->resident_matrix(&s->R, &s->kpad);
     s->ubra.reserve(sizeof(double) * s->R * s->nset);
     s->rbra.reserve(sizeof(double) * s->R * s->nset);
     s->xnew.reserve(sizeof(double) * s->n * s->nset);
@@ -496,6 +501,7 @@ void CphfBatch::solve(std::vector<KrylovStats>* stats) {
       if (sys_[i]->nb == 0) sys_[i]->done = true;  // zero right-hand side: x = 0
     }
   }
+  const auto t_cyc = clk::now();
   // lockstep cycles
   launches_per_cycle_ = 0;
   lockstep_cycles_ = 0;
@@ -563,6 +569,7 @@ void CphfBatch::solve(std::vector<KrylovStats>* stats) {
       }
     }
   }
+  const auto t_fin = clk::now();
   // projected solve per system: (xs^T ax + diag(inner)) c = xs^T b, x = xs c (krylov_block.rs:204-215)
   arena_off_ = 0;
   std::vector<GProb> hp, gp, xp;
@@ -623,6 +630,11 @@ void CphfBatch::solve(std::vector<KrylovStats>* stats) {
     all_ok = all_ok && (s.converged || s.nd == 0);
   }
   HH_HIP(hipStreamSynchronize(stream_));
+  const auto t_end = clk::now();
+  phase_s_[0] = std::chrono::duration<double>(t_built - t_start).count();
+  phase_s_[1] = std::chrono::duration<double>(t_cyc - t_built).count();
+  phase_s_[2] = std::chrono::duration<double>(t_fin - t_cyc).count();
+  phase_s_[3] = std::chrono::duration<double>(t_end - t_fin).count();
   if (!all_ok) {
     for (int64_t i = 0; i < nsys; ++i)
       if (!sys_[i]->converged && sys_[i]->nd > 0) {

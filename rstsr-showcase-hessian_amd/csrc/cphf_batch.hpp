@@ -51,6 +51,9 @@ class CphfBatch {
   void solve(std::vector<KrylovStats>* stats);
 
   int64_t launches_per_cycle() const { return launches_per_cycle_; }
+  // wall seconds of the last solve: resident matrix builds, set-up and first block, lockstep
+  // cycles, projected solves
+  const double* phase_seconds() const { return phase_s_; }
   int64_t cycles() const { return lockstep_cycles_; }
   size_t size() const { return sys_.size(); }
 
@@ -93,6 +96,7 @@ class CphfBatch {
   void* pinned_ = nullptr;  // this cycle's descriptors (host side), copied to desc_ at the same offsets
   size_t pinned_bytes_ = 0, arena_off_ = 0;
   int64_t launches_per_cycle_ = 0, lockstep_cycles_ = 0;
+  double phase_s_[4] = {0, 0, 0, 0};
 };
 
 }  // namespace hh

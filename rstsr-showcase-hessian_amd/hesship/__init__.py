@@ -12,6 +12,7 @@ from ._lib import PREC, HesshipError, KrylovNotConverged, lib  # noqa: F401
 from .tensors import fortran, empty_f  # noqa: F401
 from .krylov import krylov_block, orth_block, solve, cholesky, trsm_lower  # noqa: F401
 from .cphf import RHessCPHF, UHessCPHF, CPHFBatch  # noqa: F401
+from .cint import CInt, Int3cGenerator, make_env  # noqa: F401
 from .becke import BeckeMolTables  # noqa: F401
 from .numint import NIMatmul, libxc_to_xcfun_mapping, transform_xc, transform_xc_polarized  # noqa: F401
 from .xc import (LibXCFunctional, determine_den_type, determine_den_type_from_list,  # noqa: F401

@@ -129,11 +129,21 @@ SIGNATURES.update({
     "hh_cphf_set_graphs": (_i32, [_p, _i32]),
     "hh_cphf_graph_stats": (_i32, [_p, _pi64, _pi64]),
     "hh_cphf_finalize_reuses": (_i32, [_p, _pi64]),
+    "hh_cint_create": (_i32, [_p, _i32, _p, _i32, _p, _i64, _i32, C.POINTER(C.c_void_p)]),
+    "hh_cint_destroy": (_i32, [_p]),
+    "hh_cint_nao": (_i32, [_p, _pi64, _pi64]),
+    "hh_cint_eval_gto": (_i32, [_p, _i32, _p, _i64, _p, _p]),
+    "hh_cint_int2c2e": (_i32, [_p, _i32, _p, _p]),
+    "hh_cint_int3c2e": (_i32, [_p, _p, _i32, _i32, _i32, _p, _p]),
+    "hh_cint_pair_create": (_i32, [_p, _p, C.POINTER(C.c_void_p)]),
+    "hh_cint_pair_destroy": (_i32, [_p]),
+    "hh_cint_int3c_deriv": (_i32, [_p, _i32, _i64, _i64, _p, _p]),
     "hh_cphf_batch_create": (_i32, [_d, _i32, _d, _p, C.POINTER(C.c_void_p)]),
     "hh_cphf_batch_destroy": (_i32, [_p]),
     "hh_cphf_batch_add": (_i32, [_p, _p, _p, _pp, _i64, _pp]),
     "hh_cphf_batch_solve": (_i32, [_p, _p]),
     "hh_cphf_batch_info": (_i32, [_p, _pi64, _pi64, _pi64]),
+    "hh_cphf_batch_times": (_i32, [_p, C.POINTER(C.c_double)]),
     "hh_becke_create": (_i32, [_i64, _p, _p, _p, C.POINTER(C.c_void_p)]),
     "hh_becke_destroy": (_i32, [_p]),
     "hh_becke_partition": (_i32, [_p, _p, _i64, _i32, _pi64, _p, _i32, _i32, _p, _p, _p, _p, _i64, _p, _p, _i64, _p,

@@ -335,3 +335,10 @@ class CPHFBatch:
         a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
         check(lib().hh_cphf_batch_info(self._h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
+
+    def times(self):
+        """Wall seconds of the last solve: matrix builds, set-up and first block, lockstep cycles,
+        projected solves."""
+        t = (C.c_double * 4)()
+        check(lib().hh_cphf_batch_times(self._h, t))
+        return tuple(t)
