@@ -31,6 +31,17 @@
 
 namespace hh {
 
+struct NtProb {
+  int64_t M, N, K;
+  const double* A;
+  int64_t lda;
+  const double* B;
+  int64_t ldb;
+  double* C;
+  int64_t ldc;
+  int64_t tiles_m, item0;
+};
+
 namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -182,6 +193,113 @@ __global__ __launch_bounds__(256) void ggemm_kernel(const GProb* __restrict__ pr
   }
 }
 
+// The response matrix products rbra = M ubra of every system in one launch: C[m, n] =
+// sum_k A[m lda + k] B[n ldb + k] (both operands K-contiguous, K a multiple of 16, rows 16-byte
+// aligned), 208 x 128 tiles (8 waves of 13 x 1 MFMA blocks) so that a trial block of up to 128
+// columns reads each row of M once; 16-byte loads, double-buffered LDS with one barrier per K
+// step (the FP64 NT product of gemm_nt.hip, with the problem looked up per workgroup).
+
+
+constexpr int NT_BM = 208, NT_BN = 128, NT_BK = 16;
+
+__global__ __launch_bounds__(512) void gnt_f64_kernel(const NtProb* __restrict__ probs, int nprob) {
+  constexpr int FM = 13, BM = NT_BM, BN = NT_BN, BK = NT_BK;
+  constexpr int LD = BK + 2;  // doubles per LDS row (144 B)
+  constexpr int A_ELEMS = BM * LD, B_ELEMS = BN * LD;
+  constexpr int STAGE = A_ELEMS + B_ELEMS;
+  constexpr int A_CH = BM * (BK / 2), B_CH = BN * (BK / 2);  // 16-byte chunks
+  constexpr int A_PT = (A_CH + 511) / 512, B_PT = (B_CH + 511) / 512;
+  __shared__ __attribute__((aligned(16))) double smem[2 * STAGE];
+  const int64_t item = blockIdx.x;
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (probs[mid].item0 <= item) lo = mid;
+    else hi = mid - 1;
+  }
+  const NtProb p = probs[lo];
+  const int tid = threadIdx.x, lane = tid & 63, wn = tid >> 6;
+  const int64_t m0 = (item - p.item0) * BM, n0 = 0;
+  v4d acc[FM];
+#pragma unroll
+  for (int i = 0; i < FM; ++i) acc[i] = v4d{0, 0, 0, 0};
+  double2 ra[A_PT], rb[B_PT];
+  auto gload = [&](int64_t k0) {
+#pragma unroll
+    for (int e = 0; e < A_PT; ++e) {
+      const int c = tid + 512 * e;
+      double2 v = {0.0, 0.0};
+      if (c < A_CH) {
+        const int r = c / (BK / 2), kk = (c % (BK / 2)) * 2;
+        const int64_t gm = m0 + r;
+        if (gm < p.M) v = *reinterpret_cast<const double2*>(p.A + gm * p.lda + k0 + kk);
+      }
+      ra[e] = v;
+    }
+#pragma unroll
+    for (int e = 0; e < B_PT; ++e) {
+      const int c = tid + 512 * e;
+      double2 v = {0.0, 0.0};
+      if (c < B_CH) {
+        const int r = c / (BK / 2), kk = (c % (BK / 2)) * 2;
+        const int64_t gn = n0 + r;
+        if (gn < p.N) v = *reinterpret_cast<const double2*>(p.B + gn * p.ldb + k0 + kk);
+      }
+      rb[e] = v;
+    }
+  };
+  auto sstore = [&](int buf) {
+    double* sm = smem + buf * STAGE;
+#pragma unroll
+    for (int e = 0; e < A_PT; ++e) {
+      const int c = tid + 512 * e;
+      if (c < A_CH) *reinterpret_cast<double2*>(sm + (c / (BK / 2)) * LD + (c % (BK / 2)) * 2) = ra[e];
+    }
+#pragma unroll
+    for (int e = 0; e < B_PT; ++e) {
+      const int c = tid + 512 * e;
+      if (c < B_CH) *reinterpret_cast<double2*>(sm + A_ELEMS + (c / (BK / 2)) * LD + (c % (BK / 2)) * 2) = rb[e];
+    }
+  };
+  auto compute = [&](int buf) {
+    const double* sm = smem + buf * STAGE;
+#pragma unroll
+    for (int ks = 0; ks < BK / 4; ++ks) {
+      const int kc = ks * 4 + (lane >> 4);
+      const double bf = sm[A_ELEMS + (wn * 16 + (lane & 15)) * LD + kc];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        const double af = sm[(i * 16 + (lane & 15)) * LD + kc];
+        acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(af, bf, acc[i], 0, 0, 0);
+      }
+    }
+  };
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  int buf = 0;
+  for (int64_t k0 = 0; k0 < p.K; k0 += BK) {
+    const bool more = k0 + BK < p.K;
+    if (more) gload(k0 + BK);
+    compute(buf);
+    if (more) sstore(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+  // f64 result layout D[(l/16) + 4r][l%16]
+  const int64_t col = n0 + wn * 16 + (lane & 15);
+  const int64_t row0 = m0 + (lane >> 4);
+  if (col < p.N) {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gm = row0 + i * 16 + 4 * r;
+        if (gm < p.M) p.C[gm + col * p.ldc] = acc[i][r];
+      }
+  }
+}
+
 // Split-K problems: C = epilogue(sum_z partial[z]) in a fixed order; 256 elements per item.
 __global__ __launch_bounds__(256) void greduce_kernel(const GProb* __restrict__ probs, int nprob) {
   const int64_t item = blockIdx.x;
@@ -239,6 +357,9 @@ CphfBatch::CphfBatch(double tol, int max_cycle, double lindep, hipStream_t s)
 CphfBatch::~CphfBatch() {
   for (Sys* s : sys_) delete s;
   if (pinned_) (void)hipHostFree(pinned_);
+  for (hipStream_t st : lu_streams_) (void)hipStreamDestroy(st);
+  for (hipEvent_t ev : lu_events_) (void)hipEventDestroy(ev);
+  if (lu_fork_) (void)hipEventDestroy(lu_fork_);
 }
 
 void CphfBatch::add(Cphf* drv, Rijk* rijk, const double* const* rhs, int64_t nprop, double* const* mo1) {
@@ -308,6 +429,8 @@ void CphfBatch::grow(Sys& s, int64_t need) {
   std::swap(s.inner.ptr, in2.ptr);
   std::swap(s.inner.bytes, in2.bytes);
   s.cap = cap;
+  s.coeffs.reserve(sizeof(double) * cap * s.nset);  // the projection coefficients of any cycle
+  s.lu.reserve(sizeof(int64_t) * cap + 256);          // pivots and status of the projected solve
 }
 
 int CphfBatch::launch(std::vector<GProb>& probs, bool a_kmajor) {
@@ -362,7 +485,7 @@ int CphfBatch::launch(std::vector<GProb>& probs, bool a_kmajor) {
   return n;
 }
 
-void CphfBatch::add_operator(std::vector<GProb>* g1, std::vector<GProb>* g2, std::vector<GProb>* g3, Sys& s) {
+void CphfBatch::add_operator(std::vector<GProb>* g1, std::vector<NtProb>* g2, std::vector<GProb>* g3, Sys& s) {
   const double* x = s.xs.as<double>() + s.nd * s.n;
   double* ax = s.ax.as<double>() + s.nd * s.n;
   double* ubra = s.ubra.as<double>();
@@ -372,7 +495,7 @@ void CphfBatch::add_operator(std::vector<GProb>* g1, std::vector<GProb>* g2, std
     a.M = s.nao; a.N = s.nocc[t] * s.nb; a.K = s.nmo[t];
     a.A = s.C[t]; a.lda = s.nao;
     a.B = x + s.off[t]; a.ldb = s.nmo[t]; a.cib = s.nocc[t]; a.ldb2 = s.n;
-    a.C = ubra + s.roff[t]; a.ldc = s.nao; a.cic = s.nocc[t]; a.ldc2 = s.R;
+    a.C = ubra + s.roff[t]; a.ldc = s.nao; a.cic = s.nocc[t]; a.ldc2 = s.kpad;
     g1->push_back(a);
     GProb c;  // ax_t[m, (i, c)] = scale(sum_mu C_t[mu, m] rbra_t[mu, (i, c)])
     c.M = s.nmo[t]; c.N = s.nocc[t] * s.nb; c.K = s.nao;
@@ -382,12 +505,33 @@ void CphfBatch::add_operator(std::vector<GProb>* g1, std::vector<GProb>* g2, std
     c.eocc = s.eocc[t]; c.evir = s.evir[t]; c.nocc = s.nocc[t];
     g3->push_back(c);
   }
-  GProb m;  // rbra[r, c] = sum_k M[r][k] ubra[k, c]
-  m.M = s.R; m.N = s.nb; m.K = s.R;
+  NtProb m{};  // rbra[r, c] = sum_k M[r][k] ubra[k, c] (ubra rows R .. kpad are zero)
+  m.M = s.R; m.N = s.nb; m.K = s.kpad;
   m.A = s.m64; m.lda = s.kpad;
-  m.B = ubra; m.ldb = s.R;
+  m.B = ubra; m.ldb = s.kpad;
   m.C = rbra; m.ldc = s.R;
   g2->push_back(m);
+}
+
+int CphfBatch::launch_nt(std::vector<NtProb>& probs) {
+  if (probs.empty()) return 0;
+  int64_t items = 0;
+  for (NtProb& p : probs) {
+    HH_REQUIRE(p.M > 0 && p.N > 0 && p.N <= NT_BN && p.K % NT_BK == 0 && p.lda % 2 == 0 && p.ldb % 2 == 0,
+               "cphf batch: matrix product shape");
+    p.tiles_m = ceil_div(p.M, NT_BM);
+    p.item0 = items;
+    items += p.tiles_m;
+  }
+  const size_t bytes = sizeof(NtProb) * probs.size();
+  HH_REQUIRE(arena_off_ + bytes <= ARENA_BYTES, "cphf batch: too many products in one cycle");
+  memcpy(static_cast<char*>(pinned_) + arena_off_, probs.data(), bytes);
+  NtProb* dp = reinterpret_cast<NtProb*>(static_cast<char*>(desc_.ptr) + arena_off_);
+  HH_HIP(hipMemcpyAsync(dp, static_cast<char*>(pinned_) + arena_off_, bytes, hipMemcpyHostToDevice, stream_));
+  arena_off_ += (bytes + 255) / 256 * 256;
+  hipLaunchKernelGGL(gnt_f64_kernel, dim3((unsigned)items), dim3(512), 0, stream_, dp, (int)probs.size());
+  HH_HIP(hipGetLastError());
+  return 1;
 }
 
 int CphfBatch::orth(const std::vector<Sys*>& ss, const std::vector<int64_t>& nd_dst, double sel) {
@@ -463,11 +607,20 @@ void CphfBatch::solve(std::vector<KrylovStats>* stats) {
   resd_.reserve(sizeof(double) * 2 * MAX_SYSTEMS);
   const double conv = std::max(lindep_, tol_ * tol_);
   // the resident matrices, then the per-system buffers
-  for (Sys* s : sys_) s->m64 = s->rijk->resident_matrix(&s->R, &s->kpad);
-  HH_HIP(hipStreamSynchronize(stream_));
+  {
+    // largest first, with one set of build intermediates for all (freed once, at the end: with
+    // many matrices resident, freeing multi-GB buffers per build costs more than the products)
+    std::vector<Sys*> order(sys_.begin(), sys_.end());
+    for (Sys* s : order) s->R = s->nao * (s->nocc[0] + (s->nspin > 1 ? s->nocc[1] : 0));
+    std::stable_sort(order.begin(), order.end(), [](const Sys* a, const Sys* b) { return a->R > b->R; });
+    ResidentScratch scratch;
+    for (Sys* s : order) s->m64 = s->rijk->resident_matrix(&s->R, &s->kpad, &scratch);
+    HH_HIP(hipStreamSynchronize(stream_));
+  }
   const auto t_built = clk::now();
   for (Sys* s : sys_) {
-    s->ubra.reserve(sizeof(double) * s->R * s->nset);
+    s->ubra.reserve(sizeof(double) * s->kpad * s->nset);
+    fill<double>(s->ubra.as<double>(), s->kpad * s->nset, 0.0, stream_);  // rows R .. kpad stay zero
     s->rbra.reserve(sizeof(double) * s->R * s->nset);
     s->xnew.reserve(sizeof(double) * s->n * s->nset);
     s->w.reserve(sizeof(double) * s->n * s->nset);
@@ -518,15 +671,15 @@ void CphfBatch::solve(std::vector<KrylovStats>* stats) {
       grow(*run[i], nd_new[i] + run[i]->nset);
     }
     // 1-3: ax = A x on the trial block
-    std::vector<GProb> g1, g2, g3, cf, up;
+    std::vector<GProb> g1, g3, cf, up;
+    std::vector<NtProb> g2;
     for (Sys* s : run) add_operator(&g1, &g2, &g3, *s);
     launches += launch(g1, false);
-    launches += launch(g2, true);
+    launches += launch_nt(g2);
     launches += launch(g3, true);
     // 4: coefficients = (xs^T ax) / inner  [nd_new, nb]
     for (size_t i = 0; i < run.size(); ++i) {
       Sys& s = *run[i];
-      s.coeffs.reserve(sizeof(double) * nd_new[i] * s.nset);
       GProb g;
       g.M = nd_new[i]; g.N = s.nb; g.K = s.n;
       g.A = s.xs.as<double>(); g.lda = s.n;
@@ -595,7 +748,33 @@ void CphfBatch::solve(std::vector<KrylovStats>* stats) {
   }
   launch(hp, true);
   launch(gp, true);
-  for (Sys* s : solved) lu_solve(s->hbuf.as<double>(), s->nd, s->gbuf.as<double>(), s->nset, stream_, ws_);
+  // the projected systems are independent: their LU solves run side by side on created streams
+  // (each a chain of single-workgroup panel steps), with no host synchronisation until all end
+  {
+    constexpr int NLU = 8;
+    if (lu_streams_.empty()) {
+      for (int i = 0; i < NLU; ++i) {
+        hipStream_t st;
+        HH_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        lu_streams_.push_back(st);
+        hipEvent_t ev;
+        HH_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        lu_events_.push_back(ev);
+      }
+      HH_HIP(hipEventCreateWithFlags(&lu_fork_, hipEventDisableTiming));
+    }
+    HH_HIP(hipEventRecord(lu_fork_, stream_));
+    for (int i = 0; i < NLU; ++i) HH_HIP(hipStreamWaitEvent(lu_streams_[i], lu_fork_, 0));
+    for (size_t i = 0; i < solved.size(); ++i) {
+      Sys* s = solved[i];
+      lu_solve(s->hbuf.as<double>(), s->nd, s->gbuf.as<double>(), s->nset, lu_streams_[i % NLU], s->luws, s->lu.ptr,
+               false);
+    }
+    for (int i = 0; i < NLU; ++i) {
+      HH_HIP(hipEventRecord(lu_events_[i], lu_streams_[i]));
+      HH_HIP(hipStreamWaitEvent(stream_, lu_events_[i], 0));
+    }
+  }
   for (Sys* s : solved) {
     GProb x;  // x = xs c, written over xnew's storage ([n, nset])
     x.M = s->n; x.N = s->nset; x.K = s->nd;
@@ -630,6 +809,11 @@ void CphfBatch::solve(std::vector<KrylovStats>* stats) {
     all_ok = all_ok && (s.converged || s.nd == 0);
   }
   HH_HIP(hipStreamSynchronize(stream_));
+  for (Sys* s : solved) {
+    int bad = 0;
+    HH_HIP(hipMemcpy(&bad, static_cast<char*>(s->lu.ptr) + sizeof(int64_t) * s->nd, sizeof(int), hipMemcpyDeviceToHost));
+    HH_REQUIRE(bad == 0, "cphf batch: singular projected system");
+  }
   const auto t_end = clk::now();
   phase_s_[0] = std::chrono::duration<double>(t_built - t_start).count();
   phase_s_[1] = std::chrono::duration<double>(t_cyc - t_built).count();

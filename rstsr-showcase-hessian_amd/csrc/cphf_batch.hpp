@@ -37,6 +37,8 @@ struct GProb {
   double* partial = nullptr;
 };
 
+struct NtProb;
+
 class CphfBatch {
  public:
   CphfBatch(double tol, int max_cycle, double lindep, hipStream_t s);
@@ -69,7 +71,7 @@ class CphfBatch {
     Rijk* rijk = nullptr;
     double* mo1[2] = {nullptr, nullptr};
     // device state
-    DevBuf b, xs, ax, ubra, rbra, xnew, w, coeffs, gram, tm, inner, d1, hbuf, gbuf;
+    DevBuf b, xs, ax, ubra, rbra, xnew, w, coeffs, gram, tm, inner, d1, hbuf, gbuf, lu, luws;
     int64_t cap = 0;  // columns of xs / ax
     // host state
     int64_t nd = 0, nb = 0;
@@ -80,9 +82,11 @@ class CphfBatch {
   void grow(Sys& s, int64_t need);
   // the three products of the response operator for the running systems: ax = A x on columns
   // [nd, nd + nb) of xs / ax
-  void add_operator(std::vector<GProb>* g1, std::vector<GProb>* g2, std::vector<GProb>* g3, Sys& s);
+  void add_operator(std::vector<GProb>* g1, std::vector<NtProb>* g2, std::vector<GProb>* g3, Sys& s);
   // enqueue a grouped product (instance chosen by a_kmajor); returns the launches made
   int launch(std::vector<GProb>& probs, bool a_kmajor);
+  // the matrix products of the operator, one grouped NT launch
+  int launch_nt(std::vector<NtProb>& probs);
   // two Gram / LDL passes over xnew [n, nb] of every listed system; the second pass selects
   // pivots with d > sel and writes the selected vectors to xs[:, nd_dst), their norms to
   // inner[nd_dst), and per system (max d, count) to resd
@@ -96,6 +100,9 @@ class CphfBatch {
   void* pinned_ = nullptr;  // this cycle's descriptors (host side), copied to desc_ at the same offsets
   size_t pinned_bytes_ = 0, arena_off_ = 0;
   int64_t launches_per_cycle_ = 0, lockstep_cycles_ = 0;
+  std::vector<hipStream_t> lu_streams_;  // the projected solves' streams (default priority)
+  std::vector<hipEvent_t> lu_events_;
+  hipEvent_t lu_fork_ = nullptr;
   double phase_s_[4] = {0, 0, 0, 0};
 };
 

@@ -260,8 +260,10 @@ int read_info(const int* dinfo, hipStream_t s) {
 
 }  // namespace
 
-void lu_solve(double* H, int64_t nd, double* G, int64_t nrhs, hipStream_t s, DevBuf& ws, void* scratch) {
+void lu_solve(double* H, int64_t nd, double* G, int64_t nrhs, hipStream_t s, DevBuf& ws, void* scratch,
+              bool check) {
   if (nd == 0) return;
+  HH_REQUIRE(check || scratch, "lu_solve: an unchecked solve needs the caller's scratch");
   DevBuf meta;
   // pivots [nd] and the status word: the caller's scratch (no allocation) or a buffer of our own
   char* m = scratch ? static_cast<char*>(scratch) : static_cast<char*>(meta.reserve(sizeof(int64_t) * nd + 256));
@@ -303,8 +305,10 @@ void lu_solve(double* H, int64_t nd, double* G, int64_t nrhs, hipStream_t s, Dev
       }
     }
   }
-  const int bad = read_info(info, s);
-  if (bad) throw Error(HH_ERR_INVALID, "lu_solve: singular matrix (zero pivot at column " + std::to_string(bad) + ")");
+  if (check) {
+    const int bad = read_info(info, s);
+    if (bad) throw Error(HH_ERR_INVALID, "lu_solve: singular matrix (zero pivot at column " + std::to_string(bad) + ")");
+  }
   // back substitution U X = Y, blocks from the bottom
   const int64_t nblk = ceil_div(nd, NB);
   for (int64_t b = nblk - 1; b >= 0; --b) {

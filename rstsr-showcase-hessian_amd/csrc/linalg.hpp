@@ -9,7 +9,10 @@ namespace hh {
 // Replaces rstsr linalg solve_general (LAPACK getrf/getrs) as used by krylov_block.rs:214.
 // `scratch` (optional): device memory of at least 8 nd + 256 bytes for the pivots and the status
 // word; with it the solve allocates nothing (ws stays untouched).
-void lu_solve(double* H, int64_t nd, double* G, int64_t nrhs, hipStream_t s, DevBuf& ws, void* scratch = nullptr);
+// check = false: no host synchronisation; the status word (int, 0 = success, else the 1-based
+// column of a zero pivot) is left at scratch + 8 nd for the caller (scratch required).
+void lu_solve(double* H, int64_t nd, double* G, int64_t nrhs, hipStream_t s, DevBuf& ws, void* scratch = nullptr,
+              bool check = true);
 
 // The same solve without any device allocation: pivots and status in a file-scope device array
 // of LU_SCRATCH_BYTES (nd <= LU_NOALLOC_MAX_ND, else Error(HH_ERR_INVALID)).  The array is shared

@@ -11,6 +11,12 @@
 
 namespace hh {
 
+// Device intermediates of a resident-matrix build (G, X transposed, the unpacked V of both spins,
+// the packed O and V triangles); a caller building many matrices keeps one set for all of them.
+struct ResidentScratch {
+  DevBuf g, xr, vp, o2t, vtp;
+};
+
 class Rijk {
  public:
   ~Rijk();
@@ -68,8 +74,9 @@ class Rijk {
   bool resident_fits(Prec prec) const;
   // The resident response matrix in FP64, row-major [R][kpad] (R = nao sum_s nocc_s, the alpha
   // block first; kpad = R rounded up to 32, zeros past R), built now if it is not (needs
-  // prepare_response; independent of response_path and of the budget).
-  const double* resident_matrix(int64_t* R, int64_t* kpad);
+  // prepare_response; independent of response_path and of the budget).  `scratch` (optional):
+  // the build's intermediates go there and stay for the caller's next build.
+  const double* resident_matrix(int64_t* R, int64_t* kpad, ResidentScratch* scratch = nullptr);
 
   // Stage timing with HIP events on the handle's stream (the reference keeps wall-clock
   // `timing` vectors per component, hess_r.rs:1746).  Indices of the accumulated times (ms):
@@ -87,7 +94,7 @@ class Rijk {
   void ensure_mixed();
   void build_k_fused(int s, const double* bra, int64_t nprop, const unsigned short* wp, double alpha, double beta,
                      double* out);
-  void build_resident(Prec prec);
+  void build_resident(Prec prec, ResidentScratch* scratch = nullptr);
   void apply_resident(const double* const* bra, int64_t nprop, double* const* resp, Prec prec, bool accumulate);
   // Packed hot-GEMM operand Bp[u][P][mu_pad] (mu_pad = kpad_ = nao rounded up to 32, zeros
   // beyond nao): FP64 copy and/or bf16 hi/lo split, built once per handle.

@@ -142,17 +142,20 @@ bool Rijk::resident_fits(Prec prec) const {
   return bytes <= (double)resident_budget;
 }
 
-void Rijk::build_resident(Prec prec) {
+void Rijk::build_resident(Prec prec, ResidentScratch* scratch) {
   HH_REQUIRE(prec == PREC_F64 || prec == PREC_BF16X3, "resident response: FP64 or BF16x3");
   const int64_t Ra = nao_ * nocc_[0], R = Ra + (nspin_ > 1 ? nao_ * nocc_[1] : 0), kpad = ceil_div(R, 32) * 32;
   if (!res_ready_[0]) {
-    // FP64 matrix: the products in FP64 (the matrix is exact to FP64 reassociation)
+    // FP64 matrix: the products in FP64 (the matrix is exact to FP64 reassociation).  The
+    // intermediates live in the caller's scratch (kept for the next build) or in local buffers
+    // freed on return.
     const int64_t ldk = ceil_div(naux_, 32) * 32;
-    DevBuf G, Vp[2], Xr, O2t, Vtp;
-    double* g = static_cast<double*>(G.reserve(sizeof(double) * R * R));
+    ResidentScratch local;
+    ResidentScratch& w = scratch ? *scratch : local;
+    double* g = static_cast<double*>(w.g.reserve(sizeof(double) * R * R));
     // G[(mu,i,s),(nu,j,t)] = sum_P Xr[(mu,i,s)][P] Xr[(nu,j,t)][P]: one NT product, K = naux padded to 32
     {
-      double* xr = static_cast<double*>(Xr.reserve(sizeof(double) * R * ldk));
+      double* xr = static_cast<double*>(w.xr.reserve(sizeof(double) * R * ldk));
       for (int t = 0; t < nspin_; ++t)
         hipLaunchKernelGGL(transpose_x_kernel,
                            dim3((unsigned)ceil_div(nao_, 64), (unsigned)ceil_div(ldk, 64), (unsigned)nocc_[t]),
@@ -167,11 +170,14 @@ void Rijk::build_resident(Prec prec) {
     }
     // per spin: Vtp[(mu<=nu), (i<=j)] = sum_P cderi[(mu,nu), P] O2t[P, (i,j)], then unpacked over (mu, nu)
     double* vp[2] = {nullptr, nullptr};
+    const int64_t ntp0 = nocc_[0] * (nocc_[0] + 1) / 2;
+    const int64_t ntp1 = nspin_ > 1 ? nocc_[1] * (nocc_[1] + 1) / 2 : 0;
+    double* vbase = static_cast<double*>(w.vp.reserve(sizeof(double) * nao_ * nao_ * (ntp0 + ntp1)));
     for (int t = 0; t < nspin_; ++t) {
       const int64_t nocc = nocc_[t], ntp = nocc * (nocc + 1) / 2;
-      vp[t] = static_cast<double*>(Vp[t].reserve(sizeof(double) * nao_ * nao_ * ntp));
-      double* o2 = static_cast<double*>(O2t.reserve(sizeof(double) * ntp * naux_));
-      double* vtp = static_cast<double*>(Vtp.reserve(sizeof(double) * nao_tp_ * ntp));
+      vp[t] = vbase + (t ? nao_ * nao_ * ntp0 : 0);
+      double* o2 = static_cast<double*>(w.o2t.reserve(sizeof(double) * ntp * naux_));
+      double* vtp = static_cast<double*>(w.vtp.reserve(sizeof(double) * nao_tp_ * ntp));
       hipLaunchKernelGGL(pack_o_tp_kernel, dim3(rblocks(ntp * naux_)), dim3(256), 0, stream_, o_[t].as<double>(), nocc,
                          naux_, o2);
       HH_HIP(hipGetLastError());
@@ -189,7 +195,7 @@ void Rijk::build_resident(Prec prec) {
     hipLaunchKernelGGL(assemble_m_kernel, dim3(rblocks(R * kpad)), dim3(256), 0, stream_, g, vp[0],
                        nspin_ > 1 ? vp[1] : vp[0], nao_, Ra, R, kpad, cg, scale_k_, res64_.as<double>());
     HH_HIP(hipGetLastError());
-    HH_HIP(hipStreamSynchronize(stream_));  // the intermediates are freed on return
+    HH_HIP(hipStreamSynchronize(stream_));  // local intermediates are freed on return
     res_ready_[0] = true;
   }
   if (prec == PREC_BF16X3 && !res_ready_[1]) {
@@ -202,9 +208,9 @@ void Rijk::build_resident(Prec prec) {
   }
 }
 
-const double* Rijk::resident_matrix(int64_t* R, int64_t* kpad) {
+const double* Rijk::resident_matrix(int64_t* R, int64_t* kpad, ResidentScratch* scratch) {
   HH_REQUIRE(prepared_, "resident response matrix: make_response_preparation first");
-  build_resident(PREC_F64);
+  build_resident(PREC_F64, scratch);
   const int64_t r = nao_ * (nocc_[0] + (nspin_ > 1 ? nocc_[1] : 0));
   if (R) *R = r;
   if (kpad) *kpad = ceil_div(r, 32) * 32;
