@@ -979,6 +979,85 @@ XC_FLOP_PER_POINT = {1: 4690.1, 2: 7525.2}
 PEAK_FP64_VALU = 78.6
 
 
+# A def2-SVP-sized synthetic basis (O 3s2p1d, H 2s1p: 24 functions per water) and an
+# even-tempered auxiliary basis up to g (79 functions per water); no basis-set files exist here.
+SVP_LIKE = {"O": [(0, [[2266.2, 0.0053], [340.87, 0.0399], [77.36, 0.1785], [21.48, 0.4643], [6.66, 0.4431]]),
+                  (0, [[0.81, 1.0]]), (0, [[0.26, 1.0]]), (1, [[17.72, 0.043], [3.86, 0.229], [1.05, 0.509]]),
+                  (1, [[0.28, 1.0]]), (2, [[1.2, 1.0]])],
+            "H": [(0, [[13.0, 0.0197], [1.96, 0.138], [0.44, 0.478]]), (0, [[0.12, 1.0]]), (1, [[0.8, 1.0]])]}
+JK_LIKE = {"O": [(0, [[20.0, 1.0]]), (0, [[5.0, 1.0]]), (0, [[1.5, 1.0]]), (0, [[0.45, 1.0]]), (1, [[6.0, 1.0]]),
+                 (1, [[1.8, 1.0]]), (1, [[0.55, 1.0]]), (2, [[4.0, 1.0]]), (2, [[1.2, 1.0]]), (2, [[0.4, 1.0]]),
+                 (3, [[2.0, 1.0]]), (3, [[0.7, 1.0]]), (4, [[1.2, 1.0]])],
+           "H": [(0, [[3.0, 1.0]]), (0, [[0.8, 1.0]]), (0, [[0.25, 1.0]]), (1, [[1.5, 1.0]]), (1, [[0.4, 1.0]]),
+                 (2, [[0.9, 1.0]])]}
+
+
+def workload_cint(args):
+    """Integral engine and grids on a seeded (H2O)_8 cluster (O-H 0.96 A, 104.5 deg, 7 A box):
+    the 75 x 302 Becke grid (543 600 points) built and partitioned, AO values with first and
+    third derivatives on it (the numint and setup inputs), the 2-centre metric and its three
+    derivatives, and every 3-centre kind the RI-JK skeleton consumes, over all auxiliary shells.
+    One step = all of it.  Roofline: the AO evaluation is HBM-bound at best (output bytes)."""
+    import numpy as np
+    import torch
+    import hesship
+    from hesship import synth
+    from hesship.grids import Grids
+    from oracle import cint as oc
+    dev = torch.device("cuda:0")
+    z, R = synth.water_cluster(8, box_ang=7.0, seed=0x5EED0008)
+    sym = {8: "O", 1: "H"}
+    atoms = [(sym[int(c)], tuple(x)) for c, x in zip(z, R)]
+    mol = hesship.CInt.from_basis(atoms, SVP_LIKE)
+    aux = hesship.CInt.from_basis(atoms, JK_LIKE)
+    nao, naux = mol.nao, aux.nao
+    log(f"[bench] cint: (H2O)_8 nao {nao} naux {naux}")
+    kinds3 = ["plain", "ip1", "ip2", "ipip1", "ipvip1", "ip1ip2", "ipip2"]
+    kinds2 = ["plain", "ip1", "ipip1", "ip1ip2"]
+    out = {}
+
+    def t(fn):
+        return _timed(fn, args.steps, args.warmup)
+
+    g = Grids(z, R, nrad=75, nang=302, device=dev)
+    (coords, w), out["grid_ms"] = t(g.build)
+    ng = coords.shape[0]
+    _, out["ao_deriv1_ms"] = t(lambda: mol.eval_gto(coords, 1))
+    _, out["ao_deriv3_ms"] = t(lambda: mol.eval_gto(coords, 3))
+    for k in kinds2:
+        _, out[f"int2c2e_{k}_ms"] = t(lambda: aux.int2c2e(k))
+    for k in kinds3:
+        _, out[f"int3c2e_{k}_ms"] = t(lambda: mol.int3c2e(aux, k))
+    tot = sum(out.values())
+    ao_bytes = ng * nao * 20 * 8
+    gbs = ao_bytes / (out["ao_deriv3_ms"] * 1e-3) / 1e9
+    nint3 = nao * nao * naux * (1 + 6 + 54)
+    # CPU: the oracle (torch autograd restatement) on a bounded sample
+    sh = []
+    for b in mol.bas:  # one (centre, l, exps, coefs) per contraction, libcint order
+        A, l, nprim, nctr, pe, pc = (int(v) for v in (b[0], b[1], b[2], b[3], b[5], b[6]))
+        for c in range(nctr):
+            sh.append((mol.env[mol.atm[A][1]:mol.atm[A][1] + 3], l, mol.env[pe:pe + nprim],
+                       mol.env[pc + nprim * c:pc + nprim * (c + 1)]))
+    pts = coords[:64].cpu().numpy()
+    t0 = time.perf_counter()
+    oc.eval_gto(sh, pts, 3)
+    cpu_pts = 64 / (time.perf_counter() - t0)
+    line = _line("Integral engine ms per step ((H2O)_8: Becke grid 543600 points, AO values deriv 1 and 3, int2c2e x 4 "
+                 "kinds, int3c2e x 7 kinds)", round(tot, 3), "ms", tot, args.steps, args.warmup, "f64",
+                 {"workload": "synthetic (H2O)_8, def2-SVP-sized basis (nao 192), even-tempered auxiliary to g",
+                  "nao": nao, "naux": naux, "ngrid": int(ng)},
+                 {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s", "frac": round(gbs / 8000.0, 4),
+                  "traffic": None, "kernel": "eval_gto_kernel<L, 3> (AO values and 19 derivatives, output bytes)"},
+                 higher=False)
+    line["stages_ms"] = {k: round(v, 3) for k, v in out.items()}
+    line["int3c2e_values_per_s"] = round(nint3 / (sum(out[f"int3c2e_{k}_ms"] for k in kinds3) * 1e-3), 1)
+    line["cpu_oracle"] = {"ao_deriv3_points_per_s": round(cpu_pts, 1), "cores": _blas_threads(),
+                          "gpu_points_per_s": round(ng / (out["ao_deriv3_ms"] * 1e-3), 1),
+                          "note": "torch-autograd restatement of the AO values, 64 points"}
+    return line
+
+
 def workload_xc(args):
     """B3LYP effective kernels (get_rho_vxc_fxc's libxc_eval_eff loop, hess_rks.rs:205-216 and
     hess_uks.rs:96-106) on the 543 600-point grid: one step = exc, vxc_eff, fxc_eff for the
@@ -1046,7 +1125,8 @@ def main():
     ap.add_argument("--batch-scales", default=None, help="cphf_batch: JSON list of the per-molecule RI-JK scales "
                                                          "(the rijk_scale2 field of an earlier run) instead of the "
                                                          "spectrum search")
-    ap.add_argument("--workload", default="ri_jk", choices=["ri_jk", "numint", "dft_setup", "dft_becke", "skeleton", "cphf_batch", "xc"])
+    ap.add_argument("--workload", default="ri_jk", choices=["ri_jk", "numint", "dft_setup", "dft_becke", "skeleton", "cphf_batch", "xc",
+                                                            "cint"])
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run the library's 16-byte copy kernel over GIB GiB three times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes) and exit")
@@ -1064,7 +1144,8 @@ def main():
 
     if args.workload != "ri_jk":
         fn = {"numint": workload_numint, "dft_setup": workload_dft_setup, "dft_becke": workload_dft_becke,
-              "skeleton": workload_skeleton, "cphf_batch": workload_cphf_batch, "xc": workload_xc}[args.workload]
+              "skeleton": workload_skeleton, "cphf_batch": workload_cphf_batch, "xc": workload_xc,
+            "cint": workload_cint}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
 

@@ -81,3 +81,24 @@ def rijk_workload(nao=400, naux=2400, nocc=40, nprop=90):
     U = mo1(nao, nocc, nprop)
     bra = np.einsum("um,mia->uia", Cm, U, optimize=True)
     return B, Cm, occ, bra
+
+
+def water_cluster(nmol, box_ang=7.0, seed=0, min_oo_ang=2.7):
+    """Seeded synthetic (H2O)_n: O-H 0.96 A, H-O-H 104.5 deg, random orientations, oxygens
+    uniform in a cubic box of box_ang with every O-O distance >= min_oo_ang.  Returns (charges
+    [3n], coordinates [3n, 3] in bohr), each water as O, H, H."""
+    rng = np.random.default_rng(seed)
+    oxy = []
+    while len(oxy) < nmol:
+        p = rng.uniform(0.0, box_ang, 3)
+        if all(np.linalg.norm(p - q) >= min_oo_ang for q in oxy):
+            oxy.append(p)
+    half = np.deg2rad(104.5) / 2
+    h_local = 0.96 * np.array([[np.sin(half), 0.0, np.cos(half)], [-np.sin(half), 0.0, np.cos(half)]])
+    coords, charges = [], []
+    for p in oxy:
+        q, r = np.linalg.qr(rng.standard_normal((3, 3)))
+        q = q * np.sign(np.diag(r))[None, :]
+        coords += [p, p + h_local[0] @ q.T, p + h_local[1] @ q.T]
+        charges += [8, 1, 1]
+    return np.array(charges), np.array(coords) / 0.52917721092
