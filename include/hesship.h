@@ -202,6 +202,11 @@ hh_status hh_cint_int2c2e(const hh_cint* aux, int32_t kind, double* out, void* s
  * Synchronous. */
 hh_status hh_cint_int3c2e(const hh_cint* mol, const hh_cint* aux, int32_t kind, int32_t sh0, int32_t sh1,
                           double* out, void* stream);
+/* int3c2e with "s2ij" symmetry (hess_intor_cross(&[mol, mol, aux], "int3c2e", "s2ij"),
+ * hess_r.rs:1731): out (device) [nao_tp, n01], the upper triangle mu <= nu packed column-major
+ * (index nu (nu + 1) / 2 + mu), the j3c that hh_rijk_create_from_j3c takes.  Synchronous. */
+hh_status hh_cint_int3c2e_s2ij(const hh_cint* mol, const hh_cint* aux, int32_t sh0, int32_t sh1, double* out,
+                               void* stream);
 /* An hh_int3c_deriv_fn for hh_rijk_skeleton over the engine: ctx = an hh_cint_pair made from the
  * AO and auxiliary bases; the auxiliary ranges it is called with must be shell aligned. */
 typedef struct hh_cint_pair hh_cint_pair;

@@ -443,6 +443,14 @@ hh_status hh_cint_int3c2e(const hh_cint* mol, const hh_cint* aux, int32_t kind, 
   HH_ABI_END
 }
 
+hh_status hh_cint_int3c2e_s2ij(const hh_cint* mol, const hh_cint* aux, int32_t sh0, int32_t sh1, double* out,
+                               void* stream) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(mol && aux, "null handle");
+  Cint::int3c2e_s2ij(*mol->impl, *aux->impl, sh0, sh1, out, (hipStream_t)stream);
+  HH_ABI_END
+}
+
 hh_status hh_cint_pair_create(const hh_cint* mol, const hh_cint* aux, hh_cint_pair** out) {
   HH_ABI_BEGIN
   HH_REQUIRE(mol && aux && out, "null argument");

@@ -659,7 +659,35 @@ void launch_triples(const Cint* ma, const Cint* mb, const Cint* mc, const std::v
   for (DevBuf* b : keep) delete b;
 }
 
+// out[nu (nu + 1) / 2 + mu + ntp p] = full[mu + nao (nu + nao p)], mu <= nu (packed upper triangle)
+__global__ void pack_upper_kernel(const double* __restrict__ full, int64_t nao, int64_t np, double* __restrict__ out) {
+  const int64_t ntp = nao * (nao + 1) / 2, total = ntp * np;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t t = e % ntp, p = e / ntp;
+    int64_t nu = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while (nu * (nu + 1) / 2 > t) --nu;
+    while ((nu + 1) * (nu + 2) / 2 <= t) ++nu;
+    const int64_t mu = t - nu * (nu + 1) / 2;
+    out[e] = full[mu + nao * (nu + nao * p)];
+  }
+}
+
 }  // namespace
+
+void Cint::int3c2e_s2ij(const Cint& mol, const Cint& aux, int sh0, int sh1, double* out, hipStream_t s) {
+  HH_REQUIRE(out, "int3c2e: null output");
+  HH_REQUIRE(0 <= sh0 && sh0 <= sh1 && sh1 <= aux.nbas(), "int3c2e: auxiliary shell range");
+  const int64_t n01 = aux.bas_ao_[sh1] - aux.bas_ao_[sh0];
+  if (n01 == 0) return;
+  DevBuf full;
+  full.reserve(sizeof(double) * mol.nao_ * mol.nao_ * n01);
+  int3c2e(mol, aux, HH_INT_PLAIN, sh0, sh1, full.as<double>(), s);
+  const int64_t total = mol.nao_ * (mol.nao_ + 1) / 2 * n01;
+  hipLaunchKernelGGL(pack_upper_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(total, 256), 65536)), dim3(256), 0,
+                     s, full.as<double>(), mol.nao_, n01, out);
+  HH_HIP(hipGetLastError());
+  HH_HIP(hipStreamSynchronize(s));  // the full block is freed on return
+}
 
 Cint::Cint(const int32_t* atm, int natm, const int32_t* bas, int nbas, const double* env, int64_t nenv, bool cart)
     : cart_(cart) {

@@ -46,6 +46,10 @@ class Cint {
   // later function's direction fastest).
   void int2c2e(int kind, int sh0, int sh1, double* out, hipStream_t s) const;
   static void int3c2e(const Cint& mol, const Cint& aux, int kind, int sh0, int sh1, double* out, hipStream_t s);
+  // int3c2e with "s2ij" symmetry: the upper triangle (mu <= nu) packed, out [nao_tp, n01]
+  // (nao_tp = nao (nao + 1) / 2, index nu (nu + 1) / 2 + mu): the j3c of the metric
+  // decomposition (hess_r.rs:1731).
+  static void int3c2e_s2ij(const Cint& mol, const Cint& aux, int sh0, int sh1, double* out, hipStream_t s);
 
   const std::vector<CShell>& cshells() const { return cs_; }
 

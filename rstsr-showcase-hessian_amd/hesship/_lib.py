@@ -135,6 +135,7 @@ SIGNATURES.update({
     "hh_cint_eval_gto": (_i32, [_p, _i32, _p, _i64, _p, _p]),
     "hh_cint_int2c2e": (_i32, [_p, _i32, _p, _p]),
     "hh_cint_int3c2e": (_i32, [_p, _p, _i32, _i32, _i32, _p, _p]),
+    "hh_cint_int3c2e_s2ij": (_i32, [_p, _p, _i32, _i32, _p, _p]),
     "hh_cint_pair_create": (_i32, [_p, _p, C.POINTER(C.c_void_p)]),
     "hh_cint_pair_destroy": (_i32, [_p]),
     "hh_cint_int3c_deriv": (_i32, [_p, _i32, _i64, _i64, _p, _p]),

@@ -127,6 +127,16 @@ class CInt:
         return _shape(out, nc)
 
 
+    def int3c2e_s2ij(self, aux, shls=None, device=None):
+        """[nao_tp, n01]: the packed upper triangle (mu <= nu) of int3c2e ("s2ij", hess_r.rs:1731)."""
+        dev = device or torch.device("cuda", torch.cuda.current_device())
+        sh0, sh1 = shls if shls is not None else (0, aux.nbas)
+        n01 = int(aux.ao_loc[sh1] - aux.ao_loc[sh0])
+        out = empty_f((self.nao * (self.nao + 1) // 2, n01), dev)
+        check(lib().hh_cint_int3c2e_s2ij(self._h, aux._h, sh0, sh1, out.data_ptr(), stream_of(dev)))
+        return out
+
+
 def _shape(t, nc):
     if nc == 1:
         return t[..., 0]

@@ -230,3 +230,12 @@ def test_skeleton_on_engine_integrals(dev):
     assert rel(de, de_o) < 1e-10
     assert rel(d1.cpu().numpy(), d1_o) < 1e-10
     assert np.abs(de.sum(axis=3)).max() < 1e-9 * np.abs(de).max()
+
+
+def test_int3c2e_s2ij_packs_the_upper_triangle(dev):
+    mol, aux = bases()
+    full = mol.int3c2e(aux).cpu().numpy()
+    got = mol.int3c2e_s2ij(aux).cpu().numpy()
+    assert np.array_equal(got, orc.pack_triu(full))
+    part = mol.int3c2e_s2ij(aux, shls=(2, 6)).cpu().numpy()
+    assert np.array_equal(part, got[:, aux.ao_loc[2]:aux.ao_loc[6]])
