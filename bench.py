@@ -959,15 +959,19 @@ def workload_skeleton(args):
                  higher=False)
 
 
-def synthetic_density(ng, nspin, seed):
-    """Densities 1e-5 .. 1e2 with gradients up to a few times rho (molecular-grid-like), [ng, 4(, 2)]."""
+def synthetic_density(ng, nspin, seed, tau=False):
+    """Densities 1e-5 .. 1e2 with gradients up to a few times rho (molecular-grid-like), [ng, 4(, 2)];
+    with tau, [ng, 5(, 2)] and tau between tau_W and 6 tau_W plus a uniform-gas part."""
     import numpy as np
     rng = np.random.default_rng(seed)
 
     def one():
-        r = np.empty((ng, 4))
+        r = np.empty((ng, 5 if tau else 4))
         r[:, 0] = 10 ** rng.uniform(-5, 2, ng)
-        r[:, 1:] = rng.normal(size=(ng, 3)) * r[:, :1] * rng.uniform(0.05, 3.0, (ng, 1))
+        r[:, 1:4] = rng.normal(size=(ng, 3)) * r[:, :1] * rng.uniform(0.05, 3.0, (ng, 1))
+        if tau:
+            tw = (r[:, 1:4] ** 2).sum(1) / (8 * r[:, 0])
+            r[:, 4] = tw * rng.uniform(1.0, 6.0, ng) + rng.uniform(0.0, 0.5, ng) * r[:, 0] ** (5 / 3)
         return r
     return one() if nspin == 1 else np.stack([one(), one()], -1)
 
@@ -1096,6 +1100,19 @@ def workload_xc(args):
                        "flop_per_point": XC_FLOP_PER_POINT}
     line["ms_rks"] = round(ms[1], 4)
     line["ms_uks"] = round(ms[2], 4)
+    # TPSS0 (meta-GGA: rho, grad rho, tau) on a density of the same kind with tau
+    rho_t = {s: torch.from_numpy(synthetic_density(ng, s, 0x5EED0017 + s, tau=True)).to(dev) for s in (1, 2)}
+    fn_t = {1: hesship.LibXCFunctional.from_identifier("HYB_MGGA_XC_TPSS0", "unpolarized"),
+            2: hesship.LibXCFunctional.from_identifier("HYB_MGGA_XC_TPSS0", "polarized")}
+    outs_t = {s: [empty_f(sh, dev) for sh in _eff_shapes(ng, 5, s)] for s in (1, 2)}
+    log("[bench] xc TPSS0 effective kernels")
+    ms_t = {}
+    for s in (1, 2):
+        _, ms_t[s] = _timed(lambda: _accumulate_eff(fn_t[s], rho_t[s], "tau", 1.0, *outs_t[s], False), args.steps,
+                            args.warmup)
+    line["tpss0"] = {"ms_rks": round(ms_t[1], 4), "ms_uks": round(ms_t[2], 4),
+                     "mpoint_per_s": round(2 * ng / ((ms_t[1] + ms_t[2]) * 1e-3) / 1e6, 2),
+                     "note": "HYB_MGGA_XC_TPSS0 exc + vxc_eff + fxc_eff (TAU arrays), RKS and UKS, same grid size"}
     if not args.no_cpu_baseline:
         nsamp = 20000
         t0 = time.perf_counter()

@@ -356,25 +356,32 @@ hh_status hh_xceff_libxc_to_xcfun_map(int32_t den_type, int32_t deriv, int64_t* 
  * libxc_wrap.rs:10-23).  Functional ids are LibXC's: HH_XC_LDA_X 1 (Slater), HH_XC_LDA_C_VWN 7
  * (VWN5), HH_XC_LDA_C_VWN_RPA 8, HH_XC_GGA_X_B88 106, HH_XC_GGA_C_LYP 131, HH_XC_HYB_GGA_XC_B3LYP
  * 402 (0.08 Slater + 0.72 B88 + 0.19 VWN_RPA + 0.81 LYP; the 0.2 of exact exchange is the
- * caller's K scale).  Derivative order 0..2. */
+ * caller's K scale), HH_XC_MGGA_X_TPSS 202, HH_XC_MGGA_C_TPSS 231, HH_XC_HYB_MGGA_XC_TPSS0 396
+ * (0.75 TPSS exchange + TPSS correlation; 0.25 exact exchange).  Derivative order 0..2. */
 #define HH_XC_LDA_X 1
 #define HH_XC_LDA_C_VWN 7
 #define HH_XC_LDA_C_VWN_RPA 8
 #define HH_XC_GGA_X_B88 106
 #define HH_XC_GGA_C_LYP 131
 #define HH_XC_HYB_GGA_XC_B3LYP 402
-/* Host.  *den_type (HH_DEN_RHO / HH_DEN_SIGMA, determine_den_type), *exx the fraction of exact
+#define HH_XC_MGGA_X_TPSS 202
+#define HH_XC_MGGA_C_TPSS 231
+#define HH_XC_HYB_MGGA_XC_TPSS0 396
+/* Host.  *den_type (HH_DEN_RHO / HH_DEN_SIGMA / HH_DEN_TAU, determine_den_type), *exx the fraction of exact
  * exchange (xc_func.hyb_exx_coef()), *ncol the raw column count of hh_xc_eval at (nspin, deriv).
  * Any output may be NULL.  HH_ERR_INVALID for an id this library does not evaluate. */
 hh_status hh_xc_info(int32_t xc_id, int32_t nspin, int32_t deriv, int32_t* den_type, double* exx, int64_t* ncol);
 /* libxc_eval_inner: out[g + ld_out*c] = column c of LibXC's output at point g, columns
- * [zk | vrho | vsigma | v2rho2 | v2rhosigma | v2sigma2] up to `deriv` (LibXC's order: unpolarised
- * (rho, sigma); polarised vrho (a, b), vsigma (aa, ab, bb), v2rho2 (aa, ab, bb), v2rhosigma
- * (a x {aa, ab, bb}, b x ...), v2sigma2 (upper triangle, row-major)), as the reference holds them
- * after transposing LibXC's point-major polarised blocks (libxc_eval_eff_serial,
- * libxc_wrap.rs:91-120).  zk is the energy per particle.  rho [ng, nvar, nspin] with element
- * (g, x, s) at g + ld_rho*x + ld_spin*s (x = 0 the density, 1..3 its gradient; sigma is formed
- * from the gradient).  Points with total density below 1e-15 give zeros. */
+ * [zk | vrho | vsigma | vtau | v2rho2 | v2rhosigma | v2rhotau | v2sigma2 | v2sigmatau | v2tau2] up to
+ * `deriv`, the blocks the functional's family has (LibXC's order: unpolarised (rho, sigma, tau);
+ * polarised vrho (a, b), vsigma (aa, ab, bb), vtau (a, b), v2rho2 (aa, ab, bb), v2rhosigma
+ * (a x {aa, ab, bb}, b x ...), v2rhotau (a x {a, b}, b x ...), v2sigma2 (upper triangle,
+ * row-major), v2sigmatau (aa x {a, b}, ab x ..., bb x ...), v2tau2 (aa, ab, bb); no Laplacian
+ * columns), as the reference holds them after transposing LibXC's point-major polarised blocks
+ * (libxc_eval_eff_serial, libxc_wrap.rs:91-120).  zk is the energy per particle.  rho
+ * [ng, nvar, nspin] with element (g, x, s) at g + ld_rho*x + ld_spin*s (x = 0 the density, 1..3
+ * its gradient, 4 tau; sigma is formed from the gradient).  Points with total density below
+ * 1e-15 give zeros. */
 hh_status hh_xc_eval(int32_t xc_id, int32_t nspin, int32_t deriv, const double* rho, int64_t ld_rho, int64_t ld_spin,
                      int64_t ng, double* out, int64_t ld_out, void* stream);
 /* One term of get_rho_vxc_fxc's functional loop (hess_rks.rs:205-216, hess_rks_becke.rs:195-205,

@@ -32,9 +32,12 @@ import math
 import numpy as np
 import torch
 
-IDS = {"LDA_X": 1, "LDA_C_VWN": 7, "LDA_C_VWN_RPA": 8, "GGA_X_B88": 106, "GGA_C_LYP": 131, "HYB_GGA_XC_B3LYP": 402}
-GGA = {1: False, 7: False, 8: False, 106: True, 131: True, 402: True}
-EXX = {402: 0.2}
+IDS = {"LDA_X": 1, "LDA_C_VWN": 7, "LDA_C_VWN_RPA": 8, "GGA_X_B88": 106, "GGA_C_LYP": 131, "HYB_GGA_XC_B3LYP": 402,
+       "MGGA_X_TPSS": 202, "MGGA_C_TPSS": 231, "HYB_MGGA_XC_TPSS0": 396}
+# family: 0 LDA (rho), 1 GGA (rho, sigma), 2 meta-GGA (rho, sigma, tau)
+FAMILY = {1: 0, 7: 0, 8: 0, 106: 1, 131: 1, 402: 1, 202: 2, 231: 2, 396: 2}
+GGA = {k: v >= 1 for k, v in FAMILY.items()}
+EXX = {402: 0.2, 396: 0.25}
 
 DENS_THR = 1e-15
 ZETA_THR = 2.220446049250313e-16
@@ -122,7 +125,123 @@ def lyp(ra, rb, saa, sab, sbb):
     return -4.0 * a / den * ra * rb / rho - a * b * omega * brace
 
 
-def energy_density(xc_id, ra, rb, saa, sab, sbb, on_a, on_b):
+# ---- TPSS (Tao, Perdew, Staroverov, Scuseria, Phys. Rev. Lett. 91, 146401 (2003)) ----------
+TPSS_KAPPA, TPSS_B, TPSS_C, TPSS_E, TPSS_MU, TPSS_D = 0.804, 0.40, 1.59096, 1.537, 0.21951, 2.8
+K2 = (3.0 * math.pi ** 2) ** (2.0 / 3.0)
+
+
+def tpss_x_total(n, s, tau):
+    """TPSS exchange energy per volume of a spin-unpolarised density n (|grad n|^2 = s, kinetic
+    energy density tau): e_x^unif(n) F_x(p, z) with p = s / (4 (3 pi^2)^(2/3) n^(8/3)),
+    z = tau_W / tau (tau_W = s / 8n; tau below tau_W taken as tau_W), alpha = (tau - tau_W) /
+    tau_unif, q_b = 9/20 (alpha - 1) / sqrt(1 + b alpha (alpha - 1)) + 2p/3 and
+    x = {[10/81 + c z^2/(1+z^2)^2] p + 146/2025 q_b^2 - 73/405 q_b sqrt(1/2 (3z/5)^2 + 1/2 p^2)
+         + (10/81)^2 p^2 / kappa + 2 sqrt(e) 10/81 (3z/5)^2 + e mu p^3} / (1 + sqrt(e) p)^2,
+    F_x = 1 + kappa - kappa / (1 + x / kappa).  The square root is taken as p sqrt(1/2 (3r/5)^2 + 1/2)
+    with r = z / p = (3 pi^2)^(2/3) n^(5/3) / (2 tau), which has no kink at zero gradient."""
+    exunif = -0.75 * (3.0 / math.pi) ** (1.0 / 3.0) * n ** (4.0 / 3.0)
+    p = s / (4.0 * K2 * n ** (8.0 / 3.0))
+    tw = s / (8.0 * n)
+    above = tau > tw
+    taue = torch.where(above, tau, tw)
+    # z = 1 and alpha = 0 exactly (constants) where tau is taken as tau_W
+    z = torch.where(above, tw / taue, torch.ones_like(tw))
+    alpha = torch.where(above, (taue - tw) / (0.3 * K2 * n ** (5.0 / 3.0)), torch.zeros_like(tw))
+    qb = 0.45 * (alpha - 1.0) / torch.sqrt(1.0 + TPSS_B * alpha * (alpha - 1.0)) + 2.0 * p / 3.0
+    r = K2 * n ** (5.0 / 3.0) / (2.0 * taue)
+    se = math.sqrt(TPSS_E)
+    x = ((10.0 / 81.0 + TPSS_C * z ** 2 / (1.0 + z ** 2) ** 2) * p + 146.0 / 2025.0 * qb ** 2
+         - 73.0 / 405.0 * qb * p * torch.sqrt(0.5 * (0.6 * r) ** 2 + 0.5)
+         + (10.0 / 81.0) ** 2 / TPSS_KAPPA * p ** 2 + 2.0 * se * 10.0 / 81.0 * (0.6 * z) ** 2
+         + TPSS_E * TPSS_MU * p ** 3) / (1.0 + se * p) ** 2
+    return exunif * (1.0 + TPSS_KAPPA - TPSS_KAPPA / (1.0 + x / TPSS_KAPPA))
+
+
+def tpss_x_spin(r, s, t):
+    """Exchange of one spin channel by spin scaling: 1/2 e_x(2 rho_s, 4 sigma_ss, 2 tau_s)."""
+    return 0.5 * tpss_x_total(2.0 * r, 4.0 * s, 2.0 * t)
+
+
+def _opz(t, p):
+    return torch.where(t <= ZETA_THR, torch.full_like(t, ZETA_THR ** p), torch.clamp(t, min=ZETA_THR) ** p)
+
+
+PWMOD = [(0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294),
+         (0.01554535, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517),
+         (0.0168869, 0.11125, 10.357, 3.6231, 0.88026, 0.49671)]   # paramagnetic, ferromagnetic, -alpha_c
+PW_FPP = 1.709920934161365617563962776245
+
+
+def pw92_g(rs, A, a1, b1, b2, b3, b4):
+    srs = torch.sqrt(rs)
+    return -2.0 * A * (1.0 + a1 * rs) * torch.log1p(1.0 / (2.0 * A * (b1 * srs + b2 * rs + b3 * rs * srs + b4 * rs * rs)))
+
+
+def pw92_eps(rs, z, opz=None, omz=None):
+    """Perdew-Wang 1992 correlation per particle (the constants of LibXC's modified version);
+    opz, omz: 1 + zeta and 1 - zeta when the caller has them without cancellation."""
+    opz = 1.0 + z if opz is None else opz
+    omz = 1.0 - z if omz is None else omz
+    e0, e1 = pw92_g(rs, *PWMOD[0]), pw92_g(rs, *PWMOD[1])
+    ac = -pw92_g(rs, *PWMOD[2])
+    f = (_opz(opz, 4.0 / 3.0) + _opz(omz, 4.0 / 3.0) - 2.0) / (2.0 ** (4.0 / 3.0) - 2.0)
+    z4 = z ** 4
+    return e0 + ac * f * (1.0 - z4) / PW_FPP + (e1 - e0) * f * z4
+
+
+PBE_BETA = 0.06672455060314922
+PBE_GAMMA = (1.0 - math.log(2.0)) / math.pi ** 2
+
+
+def pbe_c_eps(n, z, s, opz=None, omz=None):
+    """PBE correlation per particle (Perdew, Burke, Ernzerhof 1996): PW92 + H(rs, zeta, t),
+    H = gamma phi^3 ln[1 + beta/gamma t^2 (1 + A t^2) / (1 + A t^2 + A^2 t^4)],
+    A = beta/gamma / (exp(-eps_c / (gamma phi^3)) - 1), t^2 = pi s / (16 phi^2 k_F n^2)."""
+    opz = 1.0 + z if opz is None else opz
+    omz = 1.0 - z if omz is None else omz
+    rs = (3.0 / (4.0 * math.pi)) ** (1.0 / 3.0) * n ** (-1.0 / 3.0)
+    ec = pw92_eps(rs, z, opz, omz)
+    phi = (_opz(opz, 2.0 / 3.0) + _opz(omz, 2.0 / 3.0)) / 2.0
+    kf = (3.0 * math.pi ** 2 * n) ** (1.0 / 3.0)
+    t2 = math.pi * s / (16.0 * phi ** 2 * kf * n ** 2)
+    phi3 = phi ** 3
+    A = PBE_BETA / PBE_GAMMA / torch.expm1(-ec / (PBE_GAMMA * phi3))
+    y = A * t2
+    return ec + PBE_GAMMA * phi3 * torch.log1p(PBE_BETA / PBE_GAMMA * t2 * (1.0 + y) / (1.0 + y + y * y))
+
+
+def tpss_c(ra, rb, saa, sab, sbb, ta, tb):
+    """TPSS correlation energy per volume: n eps_rev (1 + d eps_rev z^3), eps_rev = eps_PBE (1 + C z^2)
+    - (1 + C) z^2 sum_s n_s / n max(eps_PBE(n_s, 0), eps_PBE), C(zeta, xi) = (0.53 + 0.87 zeta^2
+    + 0.50 zeta^4 + 2.26 zeta^6) / {1 + xi^2 [(1+zeta)^(-4/3) + (1-zeta)^(-4/3)] / 2}^4,
+    xi = |grad zeta| / (2 (3 pi^2 n)^(1/3)), z = tau_W / tau, d = 2.8."""
+    n = ra + rb
+    z = (ra - rb) / n
+    opz, omz = 2.0 * ra / n, 2.0 * rb / n   # 1 +- zeta without cancellation
+    s = saa + 2.0 * sab + sbb
+    epbe = pbe_c_eps(n, z, s, opz, omz)
+    one = torch.ones_like(ra)
+    ea, eb = pbe_c_eps(ra, one, saa), pbe_c_eps(rb, one, sbb)
+    eta = torch.where(ea > epbe, ea, epbe)   # max, the second argument on a tie
+    etb = torch.where(eb > epbe, eb, epbe)
+    tw = s / (8.0 * n)
+    tt = ta + tb
+    zt = torch.where(tt > tw, tw / tt, torch.ones_like(tw))
+    gz2 = 4.0 / n ** 4 * (rb ** 2 * saa - 2.0 * ra * rb * sab + ra ** 2 * sbb)
+    xi2 = gz2 / (4.0 * (3.0 * math.pi ** 2 * n) ** (2.0 / 3.0))
+    c0 = 0.53 + 0.87 * z ** 2 + 0.50 * z ** 4 + 2.26 * z ** 6
+    cz = c0 / (1.0 + xi2 * (_opz(opz, -4.0 / 3.0) + _opz(omz, -4.0 / 3.0)) / 2.0) ** 4
+    # eps_PBE (1 + C z^2) - (1 + C) z^2 sum_s n_s/n eps_s, written with sum_s n_s/n = 1 so that
+    # it is zero exactly (value and derivatives) where z = 1 and eps_s = eps_PBE for both spins
+    erev = epbe * (1.0 - zt ** 2) + (1.0 + cz) * zt ** 2 * (ra / n * (epbe - eta) + rb / n * (epbe - etb))
+    # there it vanishes identically in a neighbourhood; the explicit zero keeps autograd's
+    # second-order residue of the cancelling terms (~1e-14) out of the derivatives
+    zero = (tt <= tw) & ~(ea > epbe) & ~(eb > epbe)
+    erev = torch.where(zero, torch.zeros_like(erev), erev)
+    return n * erev * (1.0 + TPSS_D * erev * zt ** 3)
+
+
+def energy_density(xc_id, ra, rb, saa, sab, sbb, on_a, on_b, ta=None, tb=None):
     """Energy per volume of functional xc_id over the polarised variables; on_s masks the
     exchange of a spin channel at or below the density threshold."""
     def slater():
@@ -143,58 +262,84 @@ def energy_density(xc_id, ra, rb, saa, sab, sbb, on_a, on_b):
         return lyp(ra, rb, saa, sab, sbb)
     if xc_id == 402:
         return 0.08 * slater() + 0.72 * b88() + 0.19 * vwn(VWN_RPA, ra, rb) + 0.81 * lyp(ra, rb, saa, sab, sbb)
+
+    def tpssx():
+        return (torch.where(on_a, tpss_x_spin(ra, saa, ta), 0.0 * ra)
+                + torch.where(on_b, tpss_x_spin(rb, sbb, tb), 0.0 * rb))
+
+    if xc_id == 202:
+        return tpssx()
+    if xc_id == 231:
+        return tpss_c(ra, rb, saa, sab, sbb, ta, tb)
+    if xc_id == 396:
+        return 0.75 * tpssx() + tpss_c(ra, rb, saa, sab, sbb, ta, tb)
     raise ValueError(f"unsupported functional id {xc_id}")
 
 
-def _variables(rho, nspin, gga):
-    """LibXC variables from rho [ng, nvar] (nspin 1) or [ng, nvar, 2] (nspin 2)."""
+def _variables(rho, nspin, fam):
+    """LibXC variables from rho [ng, nvar] (nspin 1) or [ng, nvar, 2] (nspin 2): rho(s), then
+    sigma(s) (GGA, meta-GGA), then tau(s) (meta-GGA, component 4)."""
     rho = torch.as_tensor(np.asarray(rho), dtype=torch.float64)
     if nspin == 1:
         v = [rho[:, 0].clone()]
-        if gga:
+        if fam >= 1:
             v.append(torch.clamp((rho[:, 1:4] ** 2).sum(1), min=SIGMA_FLOOR))
+        if fam >= 2:
+            v.append(rho[:, 4].clone())
         rtot = rho[:, 0]
     else:
         v = [rho[:, 0, 0].clone(), rho[:, 0, 1].clone()]
-        if gga:
+        if fam >= 1:
             ga, gb = rho[:, 1:4, 0], rho[:, 1:4, 1]
             v += [torch.clamp((ga * ga).sum(1), min=SIGMA_FLOOR), (ga * gb).sum(1),
                   torch.clamp((gb * gb).sum(1), min=SIGMA_FLOOR)]
+        if fam >= 2:
+            v += [rho[:, 4, 0].clone(), rho[:, 4, 1].clone()]
         rtot = rho[:, 0, 0] + rho[:, 0, 1]
     return v, rtot
 
 
-def _polarised_args(v, nspin, gga):
+def _polarised_args(v, nspin, fam):
+    """(ra, rb, saa, sab, sbb, on_a, on_b, ta, tb) of the polarised energy densities."""
     if nspin == 1:
         r = v[0]
         ra = rb = 0.5 * r
-        s = 0.25 * v[1] if gga else torch.zeros_like(r)
+        s = 0.25 * v[1] if fam >= 1 else torch.zeros_like(r)
+        t = 0.5 * v[2] if fam >= 2 else torch.zeros_like(r)
         on = torch.ones_like(r, dtype=torch.bool)
-        return ra, rb, s, s, s, on, on
+        return ra, rb, s, s, s, on, on, t, t
     on_a, on_b = v[0] > DENS_THR, v[1] > DENS_THR
     ra = torch.clamp(v[0], min=DENS_THR)
     rb = torch.clamp(v[1], min=DENS_THR)
-    if gga:
-        return ra, rb, v[2], v[3], v[4], on_a, on_b
     z = torch.zeros_like(ra)
-    return ra, rb, z, z, z, on_a, on_b
+    sg = (v[2], v[3], v[4]) if fam >= 1 else (z, z, z)
+    tt = (v[5], v[6]) if fam >= 2 else (z, z)
+    return (ra, rb) + sg + (on_a, on_b) + tt
 
 
 def libxc_order_pairs(nvar, nspin, gga):
-    """Second-derivative variable pairs in LibXC's column order."""
+    """Second-derivative variable pairs in LibXC's column order (polarised: v2rho2, v2rhosigma,
+    v2rhotau, v2sigma2, v2sigmatau, v2tau2 blocks, each spin-pair list in row-major order)."""
     if nspin == 1 or not gga:
         return [(i, j) for i in range(nvar) for j in range(i, nvar)]
-    return ([(0, 0), (0, 1), (1, 1)] + [(i, j) for i in range(2) for j in range(2, 5)]
-            + [(i, j) for i in range(2, 5) for j in range(i, 5)])
+    pairs = ([(0, 0), (0, 1), (1, 1)] + [(i, j) for i in range(2) for j in range(2, 5)])
+    if nvar == 7:
+        pairs += [(i, j) for i in range(2) for j in range(5, 7)]
+    pairs += [(i, j) for i in range(2, 5) for j in range(i, 5)]
+    if nvar == 7:
+        pairs += [(i, j) for i in range(2, 5) for j in range(5, 7)] + [(5, 5), (5, 6), (6, 6)]
+    return pairs
 
 
 def eval_raw(xc_id, rho, nspin, deriv):
     """LibXC output [ng, ncol] (numpy) of functional xc_id at rho."""
     gga = GGA[xc_id]
-    v, rtot = _variables(rho, nspin, gga)
+    fam = FAMILY[xc_id]
+    v, rtot = _variables(rho, nspin, fam)
     for t in v:
         t.requires_grad_(True)
-    e = energy_density(xc_id, *_polarised_args(v, nspin, gga))
+    ra, rb, saa, sab, sbb, on_a, on_b, ta, tb = _polarised_args(v, nspin, fam)
+    e = energy_density(xc_id, ra, rb, saa, sab, sbb, on_a, on_b, ta, tb)
     ok = rtot >= DENS_THR
     cols = [(e / rtot).detach()]
     if deriv >= 1:
@@ -217,7 +362,7 @@ def eval_raw(xc_id, rho, nspin, deriv):
 
 
 def ncol(xc_id, nspin, deriv):
-    n = (2 if GGA[xc_id] else 1) if nspin == 1 else (5 if GGA[xc_id] else 2)
+    n = (1 + FAMILY[xc_id]) if nspin == 1 else (2, 5, 7)[FAMILY[xc_id]]
     return 1 + (n if deriv >= 1 else 0) + (n * (n + 1) // 2 if deriv >= 2 else 0)
 
 
@@ -227,26 +372,27 @@ def eval_eff(xc_id, rho, nspin, den_list):
     Chain rule by the separately pinned transform of oracle.xceff."""
     from . import xceff
     gga = GGA[xc_id]
+    fam = FAMILY[xc_id]
     nvl = {"rho": 1, "sigma": 4, "tau": 5}[den_list]
     raw = eval_raw(xc_id, rho, nspin, 2)
     rho = np.asarray(rho)
     ng = raw.shape[0]
     exc = raw[:, 0].copy()
+    den = ("rho", "sigma", "tau")[fam]
+    nv = (1, 4, 5)[fam]
     if nspin == 1:
         vxc = np.zeros((ng, nvl))
         fxc = np.zeros((ng, nvl, nvl))
         if gga:
-            vxc[:, :4] = xceff.transform(rho[:, :4], raw, "sigma", 1)
-            fxc[:, :4, :4] = xceff.transform(rho[:, :4], raw, "sigma", 2)
+            vxc[:, :nv] = xceff.transform(rho[:, :nv], raw, den, 1)
+            fxc[:, :nv, :nv] = xceff.transform(rho[:, :nv], raw, den, 2)
         else:
             vxc[:, 0] = raw[:, 1]
             fxc[:, 0, 0] = raw[:, 2]
         return exc, vxc, fxc
     vxc = np.zeros((ng, nvl, 2))
     fxc = np.zeros((ng, nvl, 2, nvl, 2))
-    den = "sigma" if gga else "rho"
     xcf = raw[:, xceff.libxc_to_xcfun_map(den, 2)]
-    nv = 4 if gga else 1
     vxc[:, :nv] = xceff.transform_pol(rho[:, :nv], xcf, den, 1)
     fxc[:, :nv, :, :nv] = xceff.transform_pol(rho[:, :nv], xcf, den, 2)
     return exc, vxc, fxc

@@ -826,14 +826,14 @@ hh_status hh_xceff_transform(int32_t den_type, int32_t order, const double* rho,
 
 hh_status hh_xc_info(int32_t xc_id, int32_t nspin, int32_t deriv, int32_t* den_type, double* exx, int64_t* ncol) {
   HH_ABI_BEGIN
-  bool gga = false;
+  int fam = 0;
   double x = 0.0;
-  HH_REQUIRE(xc_known(xc_id, &gga, &x), "xc: unsupported functional id (LibXC ids 1, 7, 8, 106, 131, 402)");
+  HH_REQUIRE(xc_known(xc_id, &fam, &x), "xc: unsupported functional id (LibXC ids 1, 7, 8, 106, 131, 402, 202, 231, 396)");
   HH_REQUIRE(nspin == 1 || nspin == 2, "xc: nspin must be 1 or 2");
   HH_REQUIRE(deriv >= 0 && deriv <= 2, "xc: derivative order 0..2");
-  if (den_type) *den_type = gga ? HH_DEN_SIGMA : HH_DEN_RHO;
+  if (den_type) *den_type = fam == 0 ? HH_DEN_RHO : (fam == 1 ? HH_DEN_SIGMA : HH_DEN_TAU);
   if (exx) *exx = x;
-  if (ncol) *ncol = xc_ncol(gga, nspin, deriv);
+  if (ncol) *ncol = xc_ncol(fam, nspin, deriv);
   HH_ABI_END
 }
 

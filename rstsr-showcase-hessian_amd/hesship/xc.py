@@ -4,15 +4,17 @@ Mirrors the reference's use of LibXC: LibXCFunctional.from_identifier(name, spin
 determine_den_type, libxc_eval_inner (raw LibXC columns), libxc_eval_eff (exc, vxc_eff, fxc_eff),
 and the functional-list loop of get_rho_vxc_fxc (src/numint_matmul/hess_rks.rs:174-218,
 hess_uks.rs:56-108) as eval_xc_eff_list.  Functionals: LDA_X, LDA_C_VWN, LDA_C_VWN_RPA, GGA_X_B88,
-GGA_C_LYP, HYB_GGA_XC_B3LYP.  rho: [ng, nvar] unpolarised or [ng, nvar, 2] polarised (nvar 1 for
-RHO, 4 for SIGMA: density and its gradient), float64 on the device.
+GGA_C_LYP, HYB_GGA_XC_B3LYP, MGGA_X_TPSS, MGGA_C_TPSS, HYB_MGGA_XC_TPSS0.  rho: [ng, nvar]
+unpolarised or [ng, nvar, 2] polarised (nvar 1 for RHO, 4 for SIGMA: density and its gradient, 5
+for TAU: then the kinetic energy density tau), float64 on the device.
 """
 import ctypes as C
 
 from ._lib import check, lib
 from .tensors import empty_f, fortran, stream_of
 
-IDS = {"LDA_X": 1, "LDA_C_VWN": 7, "LDA_C_VWN_RPA": 8, "GGA_X_B88": 106, "GGA_C_LYP": 131, "HYB_GGA_XC_B3LYP": 402}
+IDS = {"LDA_X": 1, "LDA_C_VWN": 7, "LDA_C_VWN_RPA": 8, "GGA_X_B88": 106, "GGA_C_LYP": 131, "HYB_GGA_XC_B3LYP": 402,
+       "MGGA_X_TPSS": 202, "MGGA_C_TPSS": 231, "HYB_MGGA_XC_TPSS0": 396}
 DEN_NAMES = {0: "rho", 1: "sigma", 2: "tau"}
 DEN = {"rho": 0, "sigma": 1, "tau": 2}
 NVAR = {"rho": 1, "sigma": 4, "tau": 5}

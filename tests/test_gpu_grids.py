@@ -57,10 +57,11 @@ def test_engine_ao_functions_are_normalised_on_the_grid(dev):
     assert float(np.linalg.eigvalsh(S.cpu().numpy()).min()) > 0
 
 
-def test_becke_setup_on_engine_ao_and_grid(dev):
-    """B3LYP grid-shift setup (make_hessian_setup_becke, hess_rks_becke.rs:1106-1266) with every
-    input made here: engine AO values to third order on the partitioned grid, the device
-    functional at the grid density of a synthetic density matrix."""
+@pytest.mark.parametrize("xc,den", [("HYB_GGA_XC_B3LYP", "sigma"), ("HYB_MGGA_XC_TPSS0", "tau")])
+def test_becke_setup_on_engine_ao_and_grid(dev, xc, den):
+    """B3LYP / TPSS0 grid-shift setup (make_hessian_setup_becke, hess_rks_becke.rs:1106-1266)
+    with every input made here: engine AO values to third order on the partitioned grid, the
+    device functional at the grid density of a synthetic density matrix."""
     import hesship
     from hesship.grids import Grids, treutler_radii_adjust
     z, R, atoms = water(2, seed=3)
@@ -73,17 +74,55 @@ def test_becke_setup_on_engine_ao_and_grid(dev):
     C = np.linalg.qr(rng.standard_normal((nao, nao)))[0][:, :10] * 0.3
     D = 2.0 * C @ C.T
     ni = hesship.NIMatmul(ao, w, nbatch=512)
-    rho = ni.setup_rho("sigma", T(D, dev))
-    fn = hesship.LibXCFunctional.from_identifier("HYB_GGA_XC_B3LYP")
+    rho = ni.setup_rho(den, T(D, dev))
+    fn = hesship.LibXCFunctional.from_identifier(xc)
     exc, vxc, fxc = hesship.eval_xc_eff_list([(1.0, fn)], rho)
     aosl = mol.aoslice_by_atom()
-    got = ni.make_hessian_setup_becke("sigma", T(D, dev), aosl, exc, vxc, fxc, g.becke, coords,
+    got = ni.make_hessian_setup_becke(den, T(D, dev), aosl, exc, vxc, fxc, g.becke, coords,
                                       g.quadrature_weights, g.atm_quad_split, 3)
     de = got["de_xc_skeleton"].cpu().numpy()
     assert np.abs(de.sum(axis=3)).max() < 1e-9 * np.abs(de).max()
-    ref = odb.make_hessian_setup_becke("sigma", ao.cpu().numpy(), w.cpu().numpy(),
+    ref = odb.make_hessian_setup_becke(den, ao.cpu().numpy(), w.cpu().numpy(),
                                        g.quadrature_weights.cpu().numpy(), coords.cpu().numpy(), R,
                                        treutler_radii_adjust(z), 3, D, aosl, g.atm_quad_split,
                                        exc.cpu().numpy(), vxc.cpu().numpy(), fxc.cpu().numpy())
     for k in ("de_xc_skeleton", "vmat_deriv1_grid"):
         assert rel(got[k].cpu().numpy(), ref[k]) < 1e-10, k
+
+
+def test_tpss0_response_on_engine_ao_and_grid(dev):
+    """The RKS grid response of a TPSS0 kernel (get_rks_response_bra, TAU) on engine AO values
+    over a Becke grid: the functional's kernels against the oracle's at every point, the response
+    against the oracle's contraction of the same kernels; tau >= tau_W holds for the density of a
+    positive semidefinite D, which the check asserts."""
+    import hesship
+    from hesship.grids import Grids
+    from oracle import numint as oni
+    from oracle import xcfunc as X
+    z, R, atoms = water(2, seed=5)
+    mol = hesship.CInt.from_basis(atoms, SVP_LIKE)
+    coords, w = Grids(z, R, nrad=30, nang=110, device=dev).build()
+    ao = mol.eval_gto(coords, 1)
+    nao, nocc = mol.nao, 10
+    rng = np.random.default_rng(6)
+    C = np.linalg.qr(rng.standard_normal((nao, nao)))[0] * 0.4
+    D = 2.0 * C[:, :nocc] @ C[:, :nocc].T
+    ni = hesship.NIMatmul(ao, w, nbatch=512)
+    rho = ni.setup_rho("tau", T(D, dev))
+    r = rho.cpu().numpy()
+    keep = r[:, 0] > 1e-10
+    tw = (r[keep, 1:4] ** 2).sum(1) / (8 * r[keep, 0])
+    assert bool((r[keep, 4] >= tw * (1 - 1e-10)).all())
+    fn = hesship.LibXCFunctional.from_identifier("HYB_MGGA_XC_TPSS0")
+    exc, vxc, fxc = hesship.eval_xc_eff_list([(1.0, fn)], rho)
+    ref = X.eval_eff(396, r, 1, "tau")
+    for g_, r_ in zip((exc, vxc, fxc), ref):
+        g_ = g_.cpu().numpy().reshape(len(r), -1)
+        r_ = r_.reshape(len(r), -1)
+        sc = np.abs(r_).max(1, keepdims=True)
+        assert bool((np.abs(g_ - r_) <= 1e-8 * np.abs(r_) + 1e-10 * sc + 1e-14).all())
+    kets = rng.standard_normal((nao, nocc, 3, 2))
+    mocc = C[:, :nocc]
+    got = ni.get_rks_response_bra("tau", fxc, T(kets, dev), T(mocc, dev)).cpu().numpy()
+    want = oni.rks_response_bra("tau", fxc.cpu().numpy(), ao.cpu().numpy()[:, :, :4], w.cpu().numpy(), kets, mocc)
+    assert rel(got, want) < 1e-11

@@ -13,18 +13,23 @@ import torch
 from oracle import xceff
 from oracle import xcfunc as X
 
-IDS = sorted(X.GGA)
+IDS = sorted(X.FAMILY)
 
 
-def _rho_unpol(rng, ng, gga=True):
-    r = np.empty((ng, 4))
+def _rho_unpol(rng, ng, fam=1):
+    """[ng, 4] (rho, grad) or, for a meta-GGA, [ng, 5] with tau above the von Weizsaecker bound
+    |grad rho|^2 / 8 rho (the smooth branch of the functional)."""
+    r = np.empty((ng, 5 if fam == 2 else 4))
     r[:, 0] = rng.uniform(0.05, 3.0, ng)
-    r[:, 1:] = rng.normal(size=(ng, 3)) * r[:, :1] * rng.uniform(0.1, 1.5, (ng, 1))
+    r[:, 1:4] = rng.normal(size=(ng, 3)) * r[:, :1] * rng.uniform(0.1, 1.5, (ng, 1))
+    if fam == 2:
+        tw = (r[:, 1:4] ** 2).sum(1) / (8 * r[:, 0])
+        r[:, 4] = tw * rng.uniform(1.2, 4.0, ng) + 0.05 * r[:, 0] ** (5 / 3)
     return r
 
 
-def _rho_pol(rng, ng):
-    return np.stack([_rho_unpol(rng, ng), _rho_unpol(rng, ng)], -1)
+def _rho_pol(rng, ng, fam=1):
+    return np.stack([_rho_unpol(rng, ng, fam), _rho_unpol(rng, ng, fam)], -1)
 
 
 def _t(x):
@@ -144,7 +149,7 @@ def test_lyp_vanishes_fully_polarised():
 @pytest.mark.parametrize("xc_id", IDS)
 def test_unpolarised_is_polarised_at_equal_spins(xc_id):
     rng = np.random.default_rng(5)
-    r = _rho_unpol(rng, 20)
+    r = _rho_unpol(rng, 20, X.FAMILY[xc_id])
     rp = np.stack([r / 2, r / 2], -1)
     u = X.eval_raw(xc_id, r, 1, 1)
     p = X.eval_raw(xc_id, rp, 2, 1)
@@ -153,6 +158,9 @@ def test_unpolarised_is_polarised_at_equal_spins(xc_id):
     if X.GGA[xc_id]:
         # sigma = saa + 2 sab + sbb at equal spins: vsigma = (vs_aa + vs_ab + vs_bb) / 4
         np.testing.assert_allclose((p[:, 3] + p[:, 4] + p[:, 5]) / 4, u[:, 2], rtol=1e-10, atol=1e-14)
+    if X.FAMILY[xc_id] == 2:
+        # tau = tau_a + tau_b: vtau_a = vtau
+        np.testing.assert_allclose(p[:, 6], u[:, 3], rtol=1e-10, atol=1e-14)
 
 
 def test_b3lyp_composition():
@@ -164,16 +172,18 @@ def test_b3lyp_composition():
 
 
 def _energy_of_components(xc_id, comp, nspin):
-    """Energy density as a function of the density components (rho, grad rho) of each spin."""
-    gga = X.GGA[xc_id]
+    """Energy density as a function of the density components (rho, grad rho[, tau]) of each spin."""
+    fam = X.FAMILY[xc_id]
     if nspin == 1:
-        v = [comp[:, 0]] + ([(comp[:, 1:4] ** 2).sum(1)] if gga else [])
+        v = [comp[:, 0]] + ([(comp[:, 1:4] ** 2).sum(1)] if fam >= 1 else []) + ([comp[:, 4]] if fam == 2 else [])
     else:
         v = [comp[:, 0, 0], comp[:, 0, 1]]
-        if gga:
+        if fam >= 1:
             ga, gb = comp[:, 1:4, 0], comp[:, 1:4, 1]
             v += [(ga * ga).sum(1), (ga * gb).sum(1), (gb * gb).sum(1)]
-    return X.energy_density(xc_id, *X._polarised_args(v, nspin, gga))
+        if fam == 2:
+            v += [comp[:, 4, 0], comp[:, 4, 1]]
+    return X.energy_density(xc_id, *X._polarised_args(v, nspin, fam))
 
 
 @pytest.mark.parametrize("nspin", [1, 2])
@@ -183,8 +193,9 @@ def test_eff_against_direct_derivatives(xc_id, nspin):
     the energy density in the density components themselves: pins the raw column order."""
     rng = np.random.default_rng(17 + xc_id + nspin)
     ng = 12
-    rho = _rho_unpol(rng, ng) if nspin == 1 else _rho_pol(rng, ng)
-    exc, vxc, fxc = X.eval_eff(xc_id, rho, nspin, "sigma")
+    fam = X.FAMILY[xc_id]
+    rho = _rho_unpol(rng, ng, fam) if nspin == 1 else _rho_pol(rng, ng, fam)
+    exc, vxc, fxc = X.eval_eff(xc_id, rho, nspin, "tau" if fam == 2 else "sigma")
     comp = torch.tensor(rho, requires_grad=True)
     e = _energy_of_components(xc_id, comp, nspin)
     (g,) = torch.autograd.grad(e.sum(), comp, create_graph=True)
@@ -198,7 +209,7 @@ def test_eff_against_direct_derivatives(xc_id, nspin):
         np.testing.assert_allclose(fxc, H.detach().numpy(), rtol=1e-9, atol=1e-11)
     else:
         np.testing.assert_allclose(vxc, g.detach().numpy(), rtol=1e-10, atol=1e-12)
-        np.testing.assert_allclose(fxc.reshape(ng, 8, 8), H.detach().numpy(), rtol=1e-9, atol=1e-11)
+        np.testing.assert_allclose(fxc.reshape(ng, n, n), H.detach().numpy(), rtol=1e-9, atol=1e-11)
     np.testing.assert_allclose(exc * (rho[:, 0] if nspin == 1 else rho[:, 0].sum(-1)), e.detach().numpy(),
                                rtol=1e-13)
 
@@ -206,15 +217,15 @@ def test_eff_against_direct_derivatives(xc_id, nspin):
 @pytest.mark.parametrize("xc_id", IDS)
 def test_raw_derivatives_against_finite_differences(xc_id):
     rng = np.random.default_rng(23)
-    rho = _rho_pol(rng, 6)
-    gga = X.GGA[xc_id]
-    v, _ = X._variables(rho, 2, gga)
+    fam = X.FAMILY[xc_id]
+    rho = _rho_pol(rng, 6, fam)
+    v, _ = X._variables(rho, 2, fam)
     raw = X.eval_raw(xc_id, rho, 2, 2)
     nv = len(v)
     base = [t.clone() for t in v]
 
     def e_of(vals):
-        return X.energy_density(xc_id, *X._polarised_args(vals, 2, gga)).numpy()
+        return X.energy_density(xc_id, *X._polarised_args(vals, 2, fam)).numpy()
 
     h = 1e-5
     for i in range(nv):
