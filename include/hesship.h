@@ -600,6 +600,11 @@ hh_status hh_linalg_solve(double* H, int64_t nd, double* G, int64_t nrhs, void* 
  * GPU, so nd <= 8192 (else HH_ERR_INVALID).  The array is shared by all calls on one GPU and keeps
  * the last call's contents: calls that overlap on one GPU race.  Synchronous. */
 hh_status hh_linalg_solve_noalloc(double* H, int64_t nd, double* G, int64_t nrhs, void* stream);
+/* Symmetric eigendecomposition (rstsr eigh as decomp_j2c_eig / decomp_j2c_cd use it,
+ * pure_decompose.rs:11-20, 41-45): the lower triangle of H [nd, nd] is read; H receives the
+ * orthonormal eigenvectors (columns), w [nd] (device) the eigenvalues in ascending order.
+ * Parallel cyclic Jacobi on the device.  HH_ERR_INVALID if not finite.  Synchronous. */
+hh_status hh_linalg_eigh(double* H, int64_t nd, double* w, void* stream);
 /* Lower Cholesky in place (decomp_j2c_cd, pure_decompose.rs:107-133).  HH_ERR_INVALID if not
  * positive definite.  Synchronous. */
 hh_status hh_linalg_cholesky(double* H, int64_t nd, void* stream);

@@ -46,9 +46,7 @@ def build(jobs=None, verbose=False):
             if err.strip():
                 print(f"[{os.path.basename(s)}]\n{err}")
     tmp = OUT + ".tmp"
-    # rocSOLVER (syevd) for the eigendecomposition of the RI metric
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + [
-        "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocsolver", "-lrocblas"]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stderr}")

@@ -1106,6 +1106,15 @@ hh_status hh_linalg_solve_noalloc(double* H, int64_t nd, double* G, int64_t nrhs
   HH_ABI_END
 }
 
+hh_status hh_linalg_eigh(double* H, int64_t nd, double* w, void* stream) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(nd >= 0 && (nd == 0 || (H && w)), "invalid argument");
+  if (nd == 0) return HH_OK;
+  DevBuf ws;
+  eigh(H, nd, w, (hipStream_t)stream, ws);
+  HH_ABI_END
+}
+
 hh_status hh_linalg_cholesky(double* H, int64_t nd, void* stream) {
   HH_ABI_BEGIN
   HH_REQUIRE(nd >= 0 && (nd == 0 || H), "invalid argument");

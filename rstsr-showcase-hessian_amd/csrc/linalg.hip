@@ -1,14 +1,12 @@
-// Blocked FP64 LU (partial pivoting), Cholesky and triangular solves.
+// Blocked FP64 LU (partial pivoting), Cholesky, triangular solves and the Jacobi eigensolver.
 //
 // Right-looking blocked algorithms with panel width NB: the panel is factored by one
 // workgroup (1024 lanes) in global memory (it stays in L2), the trailing update runs on the
 // matrix-core GEMM.  Pivot choice: largest |a|, ties to the smallest row index (deterministic).
 #include "linalg.hpp"
 
+#include <algorithm>
 #include <mutex>
-
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 
 namespace hh {
 
@@ -428,26 +426,239 @@ void trsm_lower(const double* L, int64_t nd, double* X, int64_t n2, int64_t ldx,
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Symmetric eigendecomposition by the parallel cyclic Jacobi method (Golub and Van Loan,
+// Matrix Computations, sections 8.5.2-8.5.5): each sweep visits every (p, q) pair once in m - 1
+// rounds of m / 2 disjoint pairs (round-robin "tournament" ordering, m = n rounded up to even,
+// the extra index a dummy); one launch per round applies all of the round's rotations at once,
+// A' = J^T A J and V' = V J, out of place.  The rotation of pair (p, q) is the symmetric Schur
+// decomposition of [[a_pp, a_pq], [a_pq, a_qq]]: tau = (a_qq - a_pp) / 2 a_pq,
+// t = sign(tau) / (|tau| + sqrt(1 + tau^2)), c = 1 / sqrt(1 + t^2), s = t c, with J_pp = J_qq = c,
+// J_pq = s, J_qp = -s; then a'_pq = 0, a'_pp = a_pp - t a_pq, a'_qq = a_qq + t a_pq exactly.
+// Sweeps stop when the off-diagonal Frobenius norm falls below 1e-15 of the whole (quadratic
+// convergence: about one more sweep reaches round-off).
+
+namespace {
+
+struct JRot {
+  int k;     // the other index of the pair (-1: unpaired)
+  double c;  // J[i, i]
+  double s;  // J[k, i]
+  double t;  // tangent (diagonal update)
+  bool is_p;
+};
+
+__device__ __forceinline__ int rr_player(int pos, int r, int m) { return pos == 0 ? 0 : 1 + (pos - 1 + r) % (m - 1); }
+__device__ __forceinline__ int rr_pos(int x, int r, int m) {
+  if (x == 0) return 0;
+  int q = (x - 1 - r) % (m - 1);
+  if (q < 0) q += m - 1;
+  return 1 + q;
+}
+
+__device__ __forceinline__ JRot jacobi_rot(const double* __restrict__ A, int n, int m, int r, int i) {
+  JRot o{-1, 1.0, 0.0, 0.0, true};
+  const int pos = rr_pos(i, r, m);
+  const int other = rr_player(m - 1 - pos, r, m);
+  if (other >= n) return o;  // paired with the dummy index
+  const bool is_p = pos < m / 2;
+  const int p = is_p ? i : other, q = is_p ? other : i;
+  const double app = A[p + (int64_t)n * p], aqq = A[q + (int64_t)n * q], apq = A[p + (int64_t)n * q];
+  double t = 0.0;
+  if (apq != 0.0) {
+    const double tau = (aqq - app) / (2.0 * apq);
+    if (fabs(tau) > 1e150)
+      t = 0.5 / tau;
+    else
+      t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+  }
+  const double c = 1.0 / sqrt(1.0 + t * t), sn = t * c;
+  o.k = other;
+  o.c = c;
+  o.s = is_p ? -sn : sn;
+  o.t = t;
+  o.is_p = is_p;
+  return o;
+}
+
+// one round: column j per blockIdx.y, rows i over threads
+__global__ __launch_bounds__(256) void jacobi_round_kernel(const double* __restrict__ A, double* __restrict__ An,
+                                                           const double* __restrict__ V, double* __restrict__ Vn,
+                                                           int n, int m, int r) {
+  const int j = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ JRot rj;
+  if (threadIdx.x == 0) rj = jacobi_rot(A, n, m, r, j);
+  __syncthreads();
+  if (i >= n) return;
+  const JRot ri = jacobi_rot(A, n, m, r, i);
+  const int64_t N = n;
+  double a;
+  if (ri.k >= 0 && i == j) {
+    const double apq = A[(ri.is_p ? i : ri.k) + N * (ri.is_p ? ri.k : i)];
+    a = A[i + N * i] + (ri.is_p ? -ri.t : ri.t) * apq;
+  } else if (ri.k >= 0 && ri.k == j) {
+    a = 0.0;
+  } else {
+    a = ri.c * rj.c * A[i + N * j];
+    if (rj.k >= 0) a += ri.c * rj.s * A[i + N * rj.k];
+    if (ri.k >= 0) {
+      a += ri.s * rj.c * A[ri.k + N * j];
+      if (rj.k >= 0) a += ri.s * rj.s * A[ri.k + N * rj.k];
+    }
+  }
+  An[i + N * j] = a;
+  double v = rj.c * V[i + N * j];
+  if (rj.k >= 0) v += rj.s * V[i + N * rj.k];
+  Vn[i + N * j] = v;
+}
+
+// A = the symmetric matrix of H's lower triangle, V = identity
+__global__ void jacobi_init_kernel(const double* __restrict__ H, int n, double* __restrict__ A, double* __restrict__ V) {
+  const int64_t N = n;
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= N * N) return;
+  const int i = (int)(e % N), j = (int)(e / N);
+  A[e] = i >= j ? H[e] : H[j + N * i];
+  V[e] = i == j ? 1.0 : 0.0;
+}
+
+// per-block partial sums of (off-diagonal, all) squares
+__global__ __launch_bounds__(256) void jacobi_norm_kernel(const double* __restrict__ A, int n, double* __restrict__ part) {
+  const int64_t N = n;
+  double off = 0.0, all = 0.0;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < N * N; e += (int64_t)gridDim.x * blockDim.x) {
+    const double x = A[e] * A[e];
+    all += x;
+    if (e % N != e / N) off += x;
+  }
+  __shared__ double so[256], sa[256];
+  so[threadIdx.x] = off;
+  sa[threadIdx.x] = all;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      so[threadIdx.x] += so[threadIdx.x + h];
+      sa[threadIdx.x] += sa[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = so[0];
+    part[2 * blockIdx.x + 1] = sa[0];
+  }
+}
+
+__global__ void jacobi_diag_kernel(const double* __restrict__ A, int n, double* __restrict__ d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = A[i + (int64_t)n * i];
+}
+
+__global__ void jacobi_minus_identity_kernel(double* __restrict__ G, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) G[i + (int64_t)n * i] -= 1.0;
+}
+
+hipError_t copy_async(double* dst, const double* src, int64_t n, hipStream_t s) {
+  return hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, s);
+}
+
+// H[:, k] = V[:, perm[k]], w[k] = d[perm[k]]
+__global__ void jacobi_sort_kernel(const double* __restrict__ V, const double* __restrict__ d,
+                                   const int* __restrict__ perm, int n, double* __restrict__ H, double* __restrict__ w) {
+  const int64_t N = n;
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= N * N) return;
+  const int i = (int)(e % N), k = (int)(e / N);
+  H[e] = V[i + N * perm[k]];
+  if (i == 0) w[k] = d[perm[k]];
+}
+
+}  // namespace
+
 void eigh(double* H, int64_t nd, double* w, hipStream_t s, DevBuf& ws) {
-  HH_REQUIRE(nd > 0 && nd < (int64_t)1 << 31, "eigh: dimension out of range");
-  rocblas_handle handle = nullptr;
-  HH_REQUIRE(rocblas_create_handle(&handle) == rocblas_status_success, "eigh: rocblas_create_handle failed");
-  struct Guard {
-    rocblas_handle h;
-    ~Guard() { (void)rocblas_destroy_handle(h); }
-  } guard{handle};
-  HH_REQUIRE(rocblas_set_stream(handle, s) == rocblas_status_success, "eigh: rocblas_set_stream failed");
-  // workspace: the off-diagonal of the tridiagonal form (nd doubles) and the info word
-  char* base = static_cast<char*>(ws.reserve(sizeof(double) * nd + 256));
-  double* e = reinterpret_cast<double*>(base);
-  rocblas_int* info = reinterpret_cast<rocblas_int*>(base + sizeof(double) * nd);
-  const rocblas_status st = rocsolver_dsyevd(handle, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)nd, H,
-                                             (rocblas_int)nd, w, e, info);
-  HH_REQUIRE(st == rocblas_status_success, "eigh: rocsolver_dsyevd returned status " + std::to_string((int)st));
-  rocblas_int hinfo = 0;
-  HH_HIP(hipMemcpyAsync(&hinfo, info, sizeof(rocblas_int), hipMemcpyDeviceToHost, s));
+  HH_REQUIRE(nd > 0 && nd <= 32768, "eigh: dimension out of range (1 .. 32768)");
+  const int n = (int)nd;
+  const int m = n + (n & 1);
+  const int64_t nn = nd * nd;
+  constexpr int NPART = 512;
+  char* base = static_cast<char*>(ws.reserve(4 * nn * sizeof(double) + (2 * NPART + nd) * sizeof(double) +
+                                             nd * sizeof(int) + 1024));
+  double* A0 = reinterpret_cast<double*>(base);
+  double* A1 = A0 + nn;
+  double* V0 = A1 + nn;
+  double* V1 = V0 + nn;
+  double* part = V1 + nn;
+  double* d = part + 2 * NPART;
+  int* perm_d = reinterpret_cast<int*>(d + nd);
+  const unsigned nb = (unsigned)ceil_div(nn, 256);
+  hipLaunchKernelGGL(jacobi_init_kernel, dim3(nb), dim3(256), 0, s, H, n, A0, V0);
+  HH_HIP(hipGetLastError());
+  std::vector<double> hp(2 * NPART);
+  const unsigned npb = (unsigned)std::min<int64_t>(NPART, ceil_div(nn, 256));
+  bool done = n == 1;
+  for (int sweep = 0; sweep < 60 && !done; ++sweep) {
+    for (int r = 0; r < m - 1; ++r) {
+      hipLaunchKernelGGL(jacobi_round_kernel, dim3((unsigned)ceil_div(nd, 256), (unsigned)n), dim3(256), 0, s, A0, A1,
+                         V0, V1, n, m, r);
+      std::swap(A0, A1);
+      std::swap(V0, V1);
+    }
+    HH_HIP(hipGetLastError());
+    hipLaunchKernelGGL(jacobi_norm_kernel, dim3(npb), dim3(256), 0, s, A0, n, part);
+    HH_HIP(hipMemcpyAsync(hp.data(), part, 2 * npb * sizeof(double), hipMemcpyDeviceToHost, s));
+    HH_HIP(hipStreamSynchronize(s));
+    double off = 0.0, all = 0.0;
+    for (unsigned b = 0; b < npb; ++b) {
+      off += hp[2 * b];
+      all += hp[2 * b + 1];
+    }
+    if (!(off == off && all == all)) throw Error(HH_ERR_INVALID, "eigh: non-finite matrix elements");
+    done = off <= 1e-30 * all;
+  }
+  if (!done) throw Error(HH_ERR_INVALID, "eigh: Jacobi sweeps did not converge");
+  hipLaunchKernelGGL(jacobi_diag_kernel, dim3((unsigned)ceil_div(nd, 256)), dim3(256), 0, s, A0, n, d);
+  std::vector<double> hd(nd);
+  HH_HIP(hipMemcpyAsync(hd.data(), d, nd * sizeof(double), hipMemcpyDeviceToHost, s));
   HH_HIP(hipStreamSynchronize(s));
-  if (hinfo != 0) throw Error(HH_ERR_INVALID, "eigh: syevd did not converge (info " + std::to_string(hinfo) + ")");
+  std::vector<int> perm(nd);
+  for (int i = 0; i < n; ++i) perm[i] = i;
+  std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return hd[a] < hd[b]; });
+  HH_HIP(hipMemcpyAsync(perm_d, perm.data(), nd * sizeof(int), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(jacobi_sort_kernel, dim3(nb), dim3(256), 0, s, V0, d, perm_d, n, H, w);
+  HH_HIP(hipGetLastError());
+  // The rotations leave V orthonormal to O(sweeps n eps); one Newton-Schulz step of the polar
+  // factor, V <- V - V (V^T V - I) / 2, takes that to O(eps^2) terms: G = V^T V - I in A1,
+  // A0 = V - G V / 2, then V = A0.
+  DevBuf gws;
+  GemmArgs g;
+  g.M = g.N = g.K = nd;
+  g.A = H;
+  g.lda = nd;
+  g.a_kmajor = true;
+  g.B = H;
+  g.ldb = nd;
+  g.b_kmajor = true;
+  g.C = A1;
+  g.ldc = nd;
+  gemm<double, double>(g, PREC_F64, s, gws);
+  hipLaunchKernelGGL(jacobi_minus_identity_kernel, dim3((unsigned)ceil_div(nd, 256)), dim3(256), 0, s, A1, n);
+  HH_HIP(copy_async(A0, H, nn, s));
+  GemmArgs h;
+  h.M = h.N = h.K = nd;
+  h.A = H;
+  h.lda = nd;
+  h.a_kmajor = false;
+  h.B = A1;
+  h.ldb = nd;
+  h.b_kmajor = true;
+  h.C = A0;
+  h.ldc = nd;
+  h.alpha = -0.5;
+  h.beta = 1.0;
+  gemm<double, double>(h, PREC_F64, s, gws);
+  HH_HIP(copy_async(H, A0, nn, s));
+  HH_HIP(hipStreamSynchronize(s));  // perm is a host vector; gws is freed on return
 }
 
 }  // namespace hh

@@ -34,8 +34,9 @@ void trsm_lower(const double* L, int64_t nd, double* X, int64_t nrow_or_ncol, in
 // Symmetric eigendecomposition H = V diag(w) V^T in FP64 on the device (rstsr linalg eigh as
 // used by decomp_j2c_eig / decomp_j2c_cd, pure_decompose.rs:11-20, 41-45): the lower triangle
 // of H [nd, nd] is read, H is overwritten by the orthonormal eigenvectors (columns), w [nd]
-// (device) receives the eigenvalues in ascending order.  rocSOLVER syevd (tridiagonal reduction
-// and divide and conquer); throws Error(HH_ERR_INVALID) if it does not converge.
+// (device) receives the eigenvalues in ascending order.  Parallel cyclic Jacobi (one launch per
+// round of n / 2 disjoint rotations, a host read per sweep); workspace 4 nd^2 doubles; throws
+// Error(HH_ERR_INVALID) if the sweeps do not converge or the matrix is not finite.
 void eigh(double* H, int64_t nd, double* w, hipStream_t s, DevBuf& ws);
 
 }  // namespace hh

@@ -10,7 +10,7 @@ binding hands column-major device buffers to the C ABI.
 """
 from ._lib import PREC, HesshipError, KrylovNotConverged, lib  # noqa: F401
 from .tensors import fortran, empty_f  # noqa: F401
-from .krylov import krylov_block, orth_block, solve, cholesky, trsm_lower  # noqa: F401
+from .krylov import krylov_block, orth_block, solve, eigh, cholesky, trsm_lower  # noqa: F401
 from .cphf import RHessCPHF, UHessCPHF, CPHFBatch  # noqa: F401
 from .cint import CInt, Int3cGenerator, make_env  # noqa: F401
 from .becke import BeckeMolTables  # noqa: F401

@@ -68,6 +68,15 @@ def solve(h, g):
     return G.reshape(g.shape)
 
 
+def eigh(a):
+    """Symmetric eigendecomposition on the device (Jacobi): (w ascending, V columns); reads the
+    lower triangle of a."""
+    V = fortran(a).clone()
+    w = torch.empty(V.shape[0], dtype=torch.float64, device=a.device)
+    check(lib().hh_linalg_eigh(V.data_ptr(), V.shape[0], w.data_ptr(), stream_of(a.device)))
+    return w, V
+
+
 def cholesky(a):
     L = fortran(a).clone()
     check(lib().hh_linalg_cholesky(L.data_ptr(), L.shape[0], stream_of(a.device)))

@@ -142,3 +142,28 @@ def test_cholesky_and_trsm(dev, nd):
                                np.linalg.solve(L, Y.T).T, rtol=1e-9, atol=1e-9)
     with pytest.raises(hesship.HesshipError):
         hesship.cholesky(T(-S, dev))
+
+
+@pytest.mark.parametrize("nd", [1, 2, 7, 64, 151, 400])
+def test_eigh_jacobi(dev, nd):
+    """The device Jacobi eigensolver against numpy.linalg.eigh: eigenvalues to 1e-13 of the
+    spectral radius, orthonormal eigenvectors reproducing H, only the lower triangle read; a
+    spectrum with a degenerate pair and eigenvalues of both signs over 12 decades."""
+    import hesship
+    rng = np.random.default_rng(nd)
+    Q = np.linalg.qr(rng.standard_normal((nd, nd)))[0]
+    lam = np.sign(rng.standard_normal(nd)) * 10 ** rng.uniform(-10, 2, nd)
+    if nd >= 4:
+        lam[1] = lam[2]
+    H = (Q * lam) @ Q.T
+    H = 0.5 * (H + H.T)
+    Hin = H.copy()
+    Hin[np.triu_indices(nd, 1)] = 1e300  # the upper triangle is not read
+    w, V = hesship.eigh(T(Hin, dev))
+    w, V = w.cpu().numpy(), V.cpu().numpy()
+    ref = np.linalg.eigvalsh(H)
+    sc = np.abs(ref).max()
+    assert np.abs(w - ref).max() <= 1e-13 * sc
+    assert np.abs(V.T @ V - np.eye(nd)).max() < 1e-13
+    assert np.abs((V * w) @ V.T - H).max() <= 1e-13 * sc
+    assert bool(np.all(np.diff(w) >= 0))

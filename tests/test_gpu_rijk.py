@@ -188,7 +188,7 @@ def _decomp_case(policy):
 
 @pytest.mark.parametrize("policy,thr", [("eig", 1e-6), ("cd", 1e-3)])
 def test_cderi_metric_fallback_policies(dev, policy, thr):
-    """decomp_j2c_eig and the eigenvalue floor of decomp_j2c_cd on the device (rocSOLVER syevd +
+    """decomp_j2c_eig and the eigenvalue floor of decomp_j2c_cd on the device (Jacobi eigensolver +
     FP64 products) against the oracle's decompositions: the response of the resulting cderi."""
     from hesship import RHessRIJK
     j3c, j2c, C, occ, bra = _decomp_case(policy)
