@@ -177,6 +177,11 @@ typedef struct hh_cint hh_cint;
 #define HH_INT_IPVIP1 4
 #define HH_INT_IP1IP2 5
 #define HH_INT_IPIP2 6
+/* one-electron operators of hh_cint_int1e */
+#define HH_INT1E_OVLP 0
+#define HH_INT1E_KIN 1
+#define HH_INT1E_NUC 2
+#define HH_INT1E_RINV 3
 /* A basis in libcint's format (what the reference's CInt holds): atm [natm][6] (charge,
  * ptr_coord, ...), bas [nbas][8] (atom, l, nprim, nctr, kappa, ptr_exp, ptr_coeff, -), env
  * [nenv] (coordinates in bohr; contraction coefficients [nprim, nctr] used as given, i.e. with
@@ -205,6 +210,17 @@ hh_status hh_cint_int3c2e(const hh_cint* mol, const hh_cint* aux, int32_t kind, 
 /* int3c2e with "s2ij" symmetry (hess_intor_cross(&[mol, mol, aux], "int3c2e", "s2ij"),
  * hess_r.rs:1731): out (device) [nao_tp, n01], the upper triangle mu <= nu packed column-major
  * (index nu (nu + 1) / 2 + mu), the j3c that hh_rijk_create_from_j3c takes.  Synchronous. */
+/* One-electron integrals out [nao, nao, comp] (column-major), the libcint int1e family the
+ * Hessian's core and overlap parts take (hcore.rs:28-34, 47-82, 155-177; ovlp.rs:31-32, 76):
+ *   op HH_INT1E_OVLP (int1e_ovlp), HH_INT1E_KIN (int1e_kin), HH_INT1E_NUC (int1e_nuc:
+ *   sum_C -Z_C / |r - R_C|), HH_INT1E_RINV (int1e_rinv with the origin at nucleus `atom`, no
+ *   charge: with_rinv_at_nucleus);
+ *   kind HH_INT_PLAIN (comp 1), HH_INT_IP1 (comp 3: ip on the first function, e.g. int1e_ipovlp,
+ *   int1e_ipkin, int1e_ipnuc, int1e_iprinv), HH_INT_IPIP1 (comp 9: ipip on the first, int1e_ipip*),
+ *   HH_INT_IPVIP1 (comp 9 = 3 t + s, t on the first function and s on the second: int1e_ipovlpip,
+ *   int1e_ipkinip, int1e_ipnucip, int1e_iprinvip).  Nuclear charges are atm[.][0] of the
+ *   molecule's libcint arrays.  Synchronous. */
+hh_status hh_cint_int1e(const hh_cint* mol, int32_t op, int32_t kind, int32_t atom, double* out, void* stream);
 hh_status hh_cint_int3c2e_s2ij(const hh_cint* mol, const hh_cint* aux, int32_t sh0, int32_t sh1, double* out,
                                void* stream);
 /* An hh_int3c_deriv_fn for hh_rijk_skeleton over the engine: ctx = an hh_cint_pair made from the

@@ -243,3 +243,63 @@ def block2c(centres, shells, kind="plain"):
     returns [2la+1, 2lc+1, comp]."""
     sh = [(None, l, ex, co) for (l, ex, co) in shells]
     return _shell_block([centres[0], centres[0], centres[1]], sh, KINDS2[kind], True)[:, 0]
+
+
+# ---- one-electron integrals ---------------------------------------------------------------
+# s-type closed forms of unnormalised primitives e^{-a |r-A|^2}, e^{-b |r-B|^2} (p = a + b,
+# mu = ab / p, P = (aA + bB) / p):
+#   overlap  (pi / p)^(3/2) e^{-mu |AB|^2}
+#   kinetic  mu (3 - 2 mu |AB|^2) (pi / p)^(3/2) e^{-mu |AB|^2}       (<a| -1/2 nabla^2 |b>)
+#   1 / |r - C|   2 pi / p e^{-mu |AB|^2} F_0(p |P - C|^2)
+# higher angular momenta by Hobson's theorem as above (the operators do not depend on A, B).
+
+def s1e(op, A, B, C, a, b):
+    p = a + b
+    mu = a * b / p
+    AB = A - B
+    ab2 = (AB * AB).sum()
+    if op == "ovlp":
+        return (math.pi / p) ** 1.5 * torch.exp(-mu * ab2)
+    if op == "kin":
+        return mu * (3.0 - 2.0 * mu * ab2) * (math.pi / p) ** 1.5 * torch.exp(-mu * ab2)
+    if op == "rinv":
+        P = (a * A + b * B) / p
+        PC = P - C
+        return 2.0 * math.pi / p * torch.exp(-mu * ab2) * _f0(p * (PC * PC).sum())
+    raise ValueError(op)
+
+
+KINDS1 = {"plain": [], "ip1": [0], "ipip1": [0, 0], "ipvip1": [0, 1]}
+
+
+def block1e(op, centres, shells, kind="plain", C=None):
+    """One (a, b) shell pair of int1e_<op> (op "ovlp", "kin" or "rinv" with origin C):
+    centres [2][3], shells [(l, exps, coefs)] x 2 (coefficients as the engine takes them);
+    returns [2la+1, 2lb+1, comp] (comp 9 = 3 t + s, t on a)."""
+    kind_fns = KINDS1[kind]
+    ncomp = 3 ** len(kind_fns)
+    (la, exa, coa), (lb, exb, cob) = shells
+    Ta, Tb = c2s(la), c2s(lb)
+    out = np.zeros((2 * la + 1, 2 * lb + 1, ncomp))
+    Cv = torch.tensor(np.zeros(3) if C is None else np.asarray(C, dtype=np.float64), dtype=torch.float64)
+    vs = [torch.tensor(float(v), dtype=torch.float64, requires_grad=True) for f in range(2) for v in centres[f]]
+    combos = [(ea, wa, eb, wb) for ea, wa in zip(exa, coa) for eb, wb in zip(exb, cob)]
+
+    def fn():
+        Av, Bv = torch.stack(vs[0:3]), torch.stack(vs[3:6])
+        tot = 0.0
+        for ea, wa, eb, wb in combos:
+            tot = tot + wa * wb * (2 * ea) ** (-la) * (2 * eb) ** (-lb) * s1e(op, Av, Bv, Cv, ea, eb)
+        return tot
+    D = _Deriv(fn, vs)
+    sign = (-1) ** len(kind_fns)
+    for q in range(ncomp):
+        dirs = [(q // 3, q % 3)[i] if len(kind_fns) == 2 else q for i in range(len(kind_fns))]
+        extra = [3 * f + d for f, d in zip(kind_fns, dirs)]
+        for ia, (ax, ay, az) in enumerate(cart_list(la)):
+            for ib, (bx, by, bz) in enumerate(cart_list(lb)):
+                key = [0] * ax + [1] * ay + [2] * az + [3] * bx + [4] * by + [5] * bz + extra
+                v = float(D.get(key).detach()) * sign
+                if v != 0.0:
+                    out[:, :, q] += v * np.outer(Ta[:, ia], Tb[:, ib])
+    return out

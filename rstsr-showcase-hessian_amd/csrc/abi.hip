@@ -443,6 +443,22 @@ hh_status hh_cint_int3c2e(const hh_cint* mol, const hh_cint* aux, int32_t kind, 
   HH_ABI_END
 }
 
+hh_status hh_cint_int1e(const hh_cint* mol, int32_t op, int32_t kind, int32_t atom, double* out, void* stream) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(mol && out, "null argument");
+  const hipStream_t st = (hipStream_t)stream;
+  if (op == HH_INT1E_OVLP || op == HH_INT1E_KIN) {
+    mol->impl->int1e(op, kind, out, st);
+  } else if (op == HH_INT1E_NUC) {
+    mol->impl->int1e_nuc(kind, -1, out, st);
+  } else {
+    HH_REQUIRE(op == HH_INT1E_RINV, "int1e: unknown operator");
+    HH_REQUIRE(atom >= 0, "int1e: RINV needs a nucleus index");
+    mol->impl->int1e_nuc(kind, atom, out, st);
+  }
+  HH_ABI_END
+}
+
 hh_status hh_cint_int3c2e_s2ij(const hh_cint* mol, const hh_cint* aux, int32_t sh0, int32_t sh1, double* out,
                                void* stream) {
   HH_ABI_BEGIN

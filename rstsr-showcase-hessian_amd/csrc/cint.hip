@@ -672,6 +672,191 @@ __global__ void pack_upper_kernel(const double* __restrict__ full, int64_t nao, 
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// One-electron overlap and kinetic integrals <a| op |b> over contracted shells: per primitive
+// pair the 1D overlaps S_d[i][j] = int x_A^i x_B^j e^{-alpha x_A^2 - beta x_B^2} dx by the
+// Obara-Saika recursion from S[0][0] = sqrt(pi / p) e^{-mu X_AB^2}
+//   S[i+1][j] = X_PA S[i][j] + (i S[i-1][j] + j S[i][j-1]) / 2p
+//   S[i][j+1] = X_PB S[i][j] + (i S[i-1][j] + j S[i][j-1]) / 2p,
+// then derivatives of either function by d/dx [x^i e^{-a x^2}] = i x^{i-1} e^{..} - 2 a x^{i+1} e^{..}
+// and the kinetic operator as -1/2 sum_e d^2/de^2 on b.  One wave per contracted shell pair.
+
+constexpr int P_LA = LMAXS + 2;  // highest a power: l + 2 (ipip on a)
+constexpr int P_LB = LMAXS + 3;  // highest b power: l + 3 (ip on b and the kinetic d^2)
+
+struct PairArgs {
+  const CShell* cs;
+  const double* prim;
+  const int* alist;
+  const int* blist;
+  int na_list, nb_list;
+  int la, lb, op, kind, cart;
+  double* out;  // element (ia, ib, q) at out[(ao_a + ia) + nao ((ao_b + ib) + nao q)]
+  int64_t nao;
+};
+
+// 1D factor: (d/dx)^ka of x^i e^{-al x^2} against (d/dx)^kb of x^j e^{-be x^2}
+__device__ __forceinline__ double pair_1d(const double* S, int i, int j, int ka, int kb, double al, double be) {
+  // ka <= 2: at most 3 distinct powers (4 before merging); kb <= 3: at most 4 (6 before merging)
+  double ca[4], cb[5];
+  int pa[4], pb[5];
+  int na = 1, nb = 1;
+  ca[0] = 1.0; pa[0] = i;
+  cb[0] = 1.0; pb[0] = j;
+  for (int s = 0; s < ka; ++s) {
+    double c2[4];
+    int p2[4];
+    int m = 0;
+    for (int k = 0; k < na; ++k) {
+      if (pa[k] > 0) { c2[m] = ca[k] * pa[k]; p2[m] = pa[k] - 1; ++m; }
+      c2[m] = -2.0 * al * ca[k]; p2[m] = pa[k] + 1; ++m;
+    }
+    // merge equal powers (at most 3 distinct after two steps)
+    na = 0;
+    for (int k = 0; k < m; ++k) {
+      int f = -1;
+      for (int t = 0; t < na; ++t) if (pa[t] == p2[k]) f = t;
+      if (f >= 0) ca[f] += c2[k]; else { ca[na] = c2[k]; pa[na] = p2[k]; ++na; }
+    }
+  }
+  for (int s = 0; s < kb; ++s) {
+    double c2[8];
+    int p2[8];
+    int m = 0;
+    for (int k = 0; k < nb; ++k) {
+      if (pb[k] > 0) { c2[m] = cb[k] * pb[k]; p2[m] = pb[k] - 1; ++m; }
+      c2[m] = -2.0 * be * cb[k]; p2[m] = pb[k] + 1; ++m;
+    }
+    nb = 0;
+    for (int k = 0; k < m; ++k) {
+      int f = -1;
+      for (int t = 0; t < nb; ++t) if (pb[t] == p2[k]) f = t;
+      if (f >= 0) cb[f] += c2[k]; else { cb[nb] = c2[k]; pb[nb] = p2[k]; ++nb; }
+    }
+  }
+  double v = 0.0;
+  for (int x = 0; x < na; ++x)
+    for (int y = 0; y < nb; ++y) v += ca[x] * cb[y] * S[pa[x] * (P_LB + 1) + pb[y]];
+  return v;
+}
+
+__global__ __launch_bounds__(64) void int_pair_kernel(PairArgs A) {
+  __shared__ double S1[3][(P_LA + 1) * (P_LB + 1)];
+  __shared__ double acc[21 * 21 * 9];
+  const int tid = threadIdx.x;
+  const CShell sa = A.cs[A.alist[blockIdx.x % A.na_list]];
+  const CShell sb = A.cs[A.blist[blockIdx.x / A.na_list]];
+  const int ncA = ncart(A.la), ncB = ncart(A.lb);
+  const int nq = A.kind == HH_INT_PLAIN ? 1 : (A.kind == HH_INT_IP1 ? 3 : 9);
+  const int nacc = ncA * ncB * nq;
+  int nda = 0, ndb = 0;
+  if (A.kind == HH_INT_IP1) nda = 1;
+  if (A.kind == HH_INT_IPIP1) nda = 2;
+  if (A.kind == HH_INT_IPVIP1) { nda = 1; ndb = 1; }
+  const int LA = A.la + nda, LB = A.lb + ndb + (A.op == 1 ? 2 : 0);
+  for (int e = tid; e < nacc; e += 64) acc[e] = 0.0;
+  const double* exa = A.prim + sa.prim_off;
+  const double* exb = A.prim + sb.prim_off;
+  const double AB[3] = {sa.x - sb.x, sa.y - sb.y, sa.z - sb.z};
+  const double ab2 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
+  const double Ac[3] = {sa.x, sa.y, sa.z}, Bc[3] = {sb.x, sb.y, sb.z};
+  for (int pa = 0; pa < sa.nprim; ++pa)
+    for (int pb = 0; pb < sb.nprim; ++pb) {
+      const double al = exa[pa], be = exb[pb];
+      const double p = al + be, mu = al * be / p;
+      if (mu * ab2 > 40.0) continue;  // e^{-mu AB^2} < 4e-18
+      __syncthreads();
+      if (tid < 3) {
+        double* S = S1[tid];
+        const double P = (al * Ac[tid] + be * Bc[tid]) / p;
+        const double xpa = P - Ac[tid], xpb = P - Bc[tid], h = 0.5 / p;
+        S[0] = sqrt(M_PI / p) * exp(-mu * AB[tid] * AB[tid]);
+        for (int i = 0; i <= LA; ++i) {
+          if (i > 0) {
+            const double sm = i >= 2 ? S[(i - 2) * (P_LB + 1)] : 0.0;
+            S[i * (P_LB + 1)] = xpa * S[(i - 1) * (P_LB + 1)] + h * (i - 1) * sm;
+          }
+          for (int j = 1; j <= LB; ++j) {
+            const double a1 = i > 0 ? S[(i - 1) * (P_LB + 1) + j - 1] : 0.0;
+            const double b1 = j >= 2 ? S[i * (P_LB + 1) + j - 2] : 0.0;
+            S[i * (P_LB + 1) + j] = xpb * S[i * (P_LB + 1) + j - 1] + h * (i * a1 + (j - 1) * b1);
+          }
+        }
+      }
+      __syncthreads();
+      const double wgt = exa[sa.nprim + pa] * exb[sb.nprim + pb];
+      for (int e = tid; e < nacc; e += 64) {
+        const int q = e % nq;
+        const int rest = e / nq;
+        const int jb = rest % ncB, ja = rest / ncB;
+        int la3[3] = {0, 0, 0}, lb3[3] = {0, 0, 0};
+        {
+          int k = 0;
+          for (int lx = A.la; lx >= 0; --lx)
+            for (int ly = A.la - lx; ly >= 0; --ly, ++k)
+              if (k == ja) { la3[0] = lx; la3[1] = ly; la3[2] = A.la - lx - ly; }
+          k = 0;
+          for (int lx = A.lb; lx >= 0; --lx)
+            for (int ly = A.lb - lx; ly >= 0; --ly, ++k)
+              if (k == jb) { lb3[0] = lx; lb3[1] = ly; lb3[2] = A.lb - lx - ly; }
+        }
+        int ka[3] = {0, 0, 0}, kb[3] = {0, 0, 0};
+        switch (A.kind) {
+          case HH_INT_IP1: ka[q] += 1; break;
+          case HH_INT_IPIP1: ka[q / 3] += 1; ka[q % 3] += 1; break;
+          case HH_INT_IPVIP1: ka[q / 3] += 1; kb[q % 3] += 1; break;
+          default: break;
+        }
+        double v;
+        if (A.op == 0) {
+          v = 1.0;
+          for (int d = 0; d < 3; ++d) v *= pair_1d(S1[d], la3[d], lb3[d], ka[d], kb[d], al, be);
+        } else {
+          v = 0.0;
+          for (int ed = 0; ed < 3; ++ed) {
+            double t = 1.0;
+            for (int d = 0; d < 3; ++d) t *= pair_1d(S1[d], la3[d], lb3[d], ka[d], kb[d] + (d == ed ? 2 : 0), al, be);
+            v += t;
+          }
+          v *= -0.5;
+        }
+        acc[(ja * ncB + jb) * nq + q] += wgt * v;
+      }
+    }
+  __syncthreads();
+  const int nsa = A.cart ? ncA : 2 * A.la + 1, nsb = A.cart ? ncB : 2 * A.lb + 1;
+  for (int e = tid; e < nsa * nsb * nq; e += 64) {
+    const int q = e % nq;
+    const int rest = e / nq;
+    const int mb = rest % nsb, ma = rest / nsb;
+    double v = 0.0;
+    if (A.cart) {
+      v = acc[(ma * ncB + mb) * nq + q] * cart_fac(A.la) * cart_fac(A.lb);
+    } else {
+      const double* ra = c_c2s + c2s_off(A.la) + ma * ncA;
+      const double* rb = c_c2s + c2s_off(A.lb) + mb * ncB;
+      for (int ja = 0; ja < ncA; ++ja) {
+        if (ra[ja] == 0.0) continue;
+        for (int jb = 0; jb < ncB; ++jb)
+          if (rb[jb] != 0.0) v += ra[ja] * rb[jb] * acc[(ja * ncB + jb) * nq + q];
+      }
+    }
+    A.out[(sa.ao0 + ma) + A.nao * ((sb.ao0 + mb) + A.nao * (int64_t)q)] = v;
+  }
+}
+
+// out[i + n2 q] = sum_c w[c] in[i + n2 (c + nc q)], i < n2
+__global__ void weighted_sum_kernel(const double* __restrict__ in, int64_t n2, int nc, int nq,
+                                    const double* __restrict__ w, double* __restrict__ out) {
+  const int64_t total = n2 * nq;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e % n2, q = e / n2;
+    double v = 0.0;
+    for (int c = 0; c < nc; ++c) v += w[c] * in[i + n2 * (c + (int64_t)nc * q)];
+    out[e] = v;
+  }
+}
+
 }  // namespace
 
 void Cint::int3c2e_s2ij(const Cint& mol, const Cint& aux, int sh0, int sh1, double* out, hipStream_t s) {
@@ -692,6 +877,12 @@ void Cint::int3c2e_s2ij(const Cint& mol, const Cint& aux, int sh0, int sh1, doub
 Cint::Cint(const int32_t* atm, int natm, const int32_t* bas, int nbas, const double* env, int64_t nenv, bool cart)
     : cart_(cart) {
   HH_REQUIRE(atm && bas && env && natm > 0 && nbas > 0 && nenv > 0, "cint: empty molecule");
+  for (int a = 0; a < natm; ++a) {
+    const int pc = atm[6 * a + 1];
+    HH_REQUIRE(pc >= 0 && pc + 3 <= nenv, "cint: atom coordinates outside env");
+    atom_z_.push_back((double)atm[6 * a]);
+    for (int d = 0; d < 3; ++d) atom_xyz_.push_back(env[pc + d]);
+  }
   bas_cs_.push_back(0);
   bas_ao_.push_back(0);
   for (int i = 0; i < nbas; ++i) {
@@ -798,4 +989,103 @@ void Cint::int3c2e(const Cint& mol, const Cint& aux, int kind, int sh0, int sh1,
                  mol.d_cs_, mol.d_prim_, mol.d_cs_, mol.d_prim_, aux.d_cs_, aux.d_prim_);
 }
 
+
+void Cint::int1e(int op, int kind, double* out, hipStream_t s) const {
+  HH_REQUIRE(op == HH_INT1E_OVLP || op == HH_INT1E_KIN, "int1e: operator OVLP or KIN here");
+  HH_REQUIRE(kind == HH_INT_PLAIN || kind == HH_INT_IP1 || kind == HH_INT_IPIP1 || kind == HH_INT_IPVIP1,
+             "int1e: kinds PLAIN, IP1, IPIP1, IPVIP1");
+  HH_REQUIRE(out, "int1e: null output");
+  upload_c2s();
+  std::map<int, std::vector<int>> g;
+  for (int i = 0; i < (int)cs_.size(); ++i) g[cs_[i].l].push_back(i);
+  std::vector<DevBuf*> keep;
+  for (auto& [la, al] : g) {
+    auto* d = new DevBuf;
+    keep.push_back(d);
+    d->reserve(sizeof(int) * al.size());
+    HH_HIP(hipMemcpyAsync(d->ptr, al.data(), sizeof(int) * al.size(), hipMemcpyHostToDevice, s));
+  }
+  int ia = 0;
+  for (auto& [la, al] : g) {
+    int ib = 0;
+    for (auto& [lb, bl] : g) {
+      PairArgs A{};
+      A.cs = d_cs_.as<CShell>();
+      A.prim = d_prim_.as<double>();
+      A.alist = keep[ia]->as<int>();
+      A.blist = keep[ib]->as<int>();
+      A.na_list = (int)al.size();
+      A.nb_list = (int)bl.size();
+      A.la = la;
+      A.lb = lb;
+      A.op = op == HH_INT1E_KIN ? 1 : 0;
+      A.kind = kind;
+      A.cart = cart_ ? 1 : 0;
+      A.out = out;
+      A.nao = nao_;
+      const int64_t nab = (int64_t)al.size() * (int64_t)bl.size();
+      HH_REQUIRE(nab < (1LL << 31), "int1e: too many shell pairs");
+      hipLaunchKernelGGL(int_pair_kernel, dim3((unsigned)nab), dim3(64), 0, s, A);
+      HH_HIP(hipGetLastError());
+      ++ib;
+    }
+    ++ia;
+  }
+  HH_HIP(hipStreamSynchronize(s));  // the shell lists are freed on return
+  for (DevBuf* b : keep) delete b;
+}
+
+const Cint& Cint::point_charges() const {
+  if (!points_) {
+    // one s function per nucleus, c e^{-g r^2} times the s factor 1 / (2 sqrt(pi)), normalised to
+    // unit charge: c = (g / pi)^(3/2) 2 sqrt(pi); with g = 1e20 the Coulomb integral (ab|c) equals
+    // <a| 1 / |r - C| |b> to a relative 1e-16 p (p the exponent sum of the pair)
+    const double g = 1e20;
+    const int natm = (int)atom_z_.size();
+    std::vector<int32_t> atm(6 * natm, 0), bas(8 * natm, 0);
+    std::vector<double> env(20, 0.0);
+    for (int a = 0; a < natm; ++a) {
+      atm[6 * a + 1] = (int32_t)env.size();
+      env.push_back(atom_xyz_[3 * a]);
+      env.push_back(atom_xyz_[3 * a + 1]);
+      env.push_back(atom_xyz_[3 * a + 2]);
+    }
+    const int pexp = (int)env.size();
+    env.push_back(g);
+    env.push_back(pow(g / M_PI, 1.5) * 2.0 * sqrt(M_PI));
+    for (int a = 0; a < natm; ++a) {
+      int32_t* b = &bas[8 * a];
+      b[0] = a; b[1] = 0; b[2] = 1; b[3] = 1; b[5] = pexp; b[6] = pexp + 1;
+    }
+    points_ = std::make_unique<Cint>(atm.data(), natm, bas.data(), natm, env.data(), (int64_t)env.size(), cart_);
+  }
+  return *points_;
+}
+
+void Cint::int1e_nuc(int kind, int atom, double* out, hipStream_t s) const {
+  HH_REQUIRE(kind == HH_INT_PLAIN || kind == HH_INT_IP1 || kind == HH_INT_IPIP1 || kind == HH_INT_IPVIP1,
+             "int1e: kinds PLAIN, IP1, IPIP1, IPVIP1");
+  HH_REQUIRE(out, "int1e: null output");
+  const int natm = (int)atom_z_.size();
+  HH_REQUIRE(atom >= -1 && atom < natm, "int1e: nucleus index out of range");
+  const Cint& pc = point_charges();
+  if (atom >= 0) {
+    // <a| 1 / |r - R_atom| |b>: element (a, b, 0, q) of the 3-centre block of one point
+    int3c2e(*this, pc, kind, atom, atom + 1, out, s);
+    return;
+  }
+  const int nq = kind == HH_INT_PLAIN ? 1 : (kind == HH_INT_IP1 ? 3 : 9);
+  const int64_t n2 = nao_ * nao_;
+  DevBuf tmp, w;
+  tmp.reserve(sizeof(double) * n2 * natm * nq);
+  int3c2e(*this, pc, kind, 0, natm, tmp.as<double>(), s);
+  std::vector<double> mz(natm);
+  for (int a = 0; a < natm; ++a) mz[a] = -atom_z_[a];
+  w.reserve(sizeof(double) * natm);
+  HH_HIP(hipMemcpyAsync(w.ptr, mz.data(), sizeof(double) * natm, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(weighted_sum_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(n2 * nq, 256), 65536)), dim3(256),
+                     0, s, tmp.as<double>(), n2, natm, nq, w.as<double>(), out);
+  HH_HIP(hipGetLastError());
+  HH_HIP(hipStreamSynchronize(s));  // tmp and w are freed on return
+}
 }  // namespace hh

@@ -11,6 +11,7 @@
 // env coefficients as given; Cartesian functions carry the s / p factors of libcint
 // (1 / (2 sqrt(pi)), sqrt(3 / (4 pi))) and none for l >= 2.
 #pragma once
+#include <memory>
 #include <vector>
 
 #include "hh_common.hpp"
@@ -51,9 +52,24 @@ class Cint {
   // decomposition (hess_r.rs:1731).
   static void int3c2e_s2ij(const Cint& mol, const Cint& aux, int sh0, int sh1, double* out, hipStream_t s);
 
+  // One-electron integrals out [nao, nao, comp] (libcint's int1e_ovlp / int1e_kin and their
+  // ip / ipip / ipvip derivatives: comp 3 for IP1 (on a), 9 = 3 t + s for IPIP1 (t, s on a) and
+  // IPVIP1 (t on a, s on b)); op HH_INT1E_OVLP or HH_INT1E_KIN.
+  void int1e(int op, int kind, double* out, hipStream_t s) const;
+  // Nuclear attraction: atom = -1: sum_C -Z_C <a| 1 / |r - R_C| |b> (int1e_nuc and its ip / ipip /
+  // ipvip derivatives on the functions); atom >= 0: <a| 1 / |r - R_atom| |b> (int1e_rinv with the
+  // origin at that nucleus, no charge), as 3-centre Coulomb integrals against unit point charges.
+  void int1e_nuc(int kind, int atom, double* out, hipStream_t s) const;
+  int natm() const { return (int)atom_z_.size(); }
+  double atom_charge(int a) const { return atom_z_[a]; }
+  const double* atom_xyz(int a) const { return &atom_xyz_[3 * a]; }
+
   const std::vector<CShell>& cshells() const { return cs_; }
 
  private:
+  const Cint& point_charges() const;
+  std::vector<double> atom_z_, atom_xyz_;
+  mutable std::unique_ptr<Cint> points_;
   std::vector<CShell> cs_;          // contracted shells (host copy)
   std::vector<int> bas_cs_;         // first contracted shell of libcint shell i (i <= nbas)
   std::vector<int64_t> bas_ao_;     // AO offset of libcint shell i

@@ -18,6 +18,24 @@ from .tensors import empty_f, stream_of
 
 KIND = {"plain": 0, "ip1": 1, "ip2": 2, "ipip1": 3, "ipvip1": 4, "ip1ip2": 5, "ipip2": 6}
 NCOMP = {"plain": 1, "ip1": 3, "ip2": 3, "ipip1": 9, "ipvip1": 9, "ip1ip2": 9, "ipip2": 9}
+# libcint one-electron integral names -> (operator, kind) of hh_cint_int1e
+INT1E_OP = {"ovlp": 0, "kin": 1, "nuc": 2, "rinv": 3}
+INT1E = {}
+for _op in INT1E_OP:
+    INT1E[f"int1e_{_op}"] = (_op, "plain")
+    INT1E[f"int1e_ip{_op}"] = (_op, "ip1")
+    INT1E[f"int1e_ipip{_op}"] = (_op, "ipip1")
+    INT1E[f"int1e_ip{_op}ip"] = (_op, "ipvip1")
+ELEMENTS = ["X", "H", "He", "Li", "Be", "B", "C", "N", "O", "F", "Ne", "Na", "Mg", "Al", "Si", "P", "S", "Cl", "Ar"]
+
+
+def charge_of(sym):
+    """Nuclear charge of an element symbol (a label such as "H1" or "O-a" reads as its element)."""
+    base = "".join(ch for ch in str(sym) if ch.isalpha())
+    for n in (2, 1):
+        if base[:n].capitalize() in ELEMENTS:
+            return ELEMENTS.index(base[:n].capitalize())
+    return 0
 
 
 def _gint(n, a):
@@ -31,7 +49,7 @@ def make_env(atoms, basis):
     env = [0.0] * 20  # libcint reserves the first 20 slots
     atm, bas = [], []
     for ia, (sym, xyz) in enumerate(atoms):
-        atm.append([0, len(env), 1, 0, 0, 0])
+        atm.append([charge_of(sym), len(env), 1, 0, 0, 0])
         env.extend(float(v) for v in xyz)
     for ia, (sym, _) in enumerate(atoms):
         for l, rows in basis[sym]:
@@ -126,6 +144,25 @@ class CInt:
         check(lib().hh_cint_int3c2e(self._h, aux._h, KIND[kind], sh0, sh1, out.data_ptr(), stream_of(dev)))
         return _shape(out, nc)
 
+
+    def atom_charges(self):
+        return self.atm[:, 0].astype(np.float64)
+
+    def intor(self, name, rinv_atom=None, device=None):
+        """One-electron integrals by libcint name (int1e_ovlp, int1e_kin, int1e_nuc, int1e_rinv and
+        their ip / ipip / ip..ip forms): [nao, nao(, 3 | 3, 3)]; int1e_*rinv* at nucleus rinv_atom
+        (no charge: the caller scales by -Z as with_rinv_at_nucleus users do)."""
+        if name not in INT1E:
+            raise ValueError(f"unknown one-electron integral {name!r}")
+        op, kind = INT1E[name]
+        if op == "rinv" and rinv_atom is None:
+            raise ValueError("rinv integrals need rinv_atom")
+        dev = device or torch.device("cuda", torch.cuda.current_device())
+        nc = NCOMP[kind]
+        out = empty_f((self.nao, self.nao, nc), dev)
+        check(lib().hh_cint_int1e(self._h, INT1E_OP[op], KIND[kind], -1 if rinv_atom is None else int(rinv_atom),
+                                  out.data_ptr(), stream_of(dev)))
+        return _shape(out, nc)
 
     def int3c2e_s2ij(self, aux, shls=None, device=None):
         """[nao_tp, n01]: the packed upper triangle (mu <= nu) of int3c2e ("s2ij", hess_r.rs:1731)."""

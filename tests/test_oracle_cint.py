@@ -136,3 +136,76 @@ def test_golden_blocks_are_the_oracles():
         for n, (a, c) in enumerate(cb.blocks2(sc, maxl)[:2]):
             ref = oc.block2c([a[1], c[1]], [(s[2], s[3], s[4]) for s in (a, c)], kind)
             np.testing.assert_allclose(g[f"2c_{kind}_{n}"], ref, rtol=1e-13, atol=1e-15)
+
+
+# ---- one-electron integrals -----------------------------------------------------------------
+
+def _norm_shell(l, a):
+    from hesship.cint import make_env
+    atm, bas, env = make_env([("X", (0.0, 0.0, 0.0))], {"X": [(l, [[a, 1.0]])]})
+    return (l, [a], [float(env[bas[0][6]])])
+
+
+@pytest.mark.parametrize("l", [0, 1, 2])
+def test_one_electron_closed_forms(l):
+    """Normalised e^{-a r^2} r^l Y_lm: <a|a> = 1, <a| -1/2 nabla^2 |a> = (2l + 3) a / 2, and for s
+    <a| 1/r |a> = 2 sqrt(2a / pi) (origin at the centre)."""
+    a = 0.83
+    sh = _norm_shell(l, a)
+    cen = [np.zeros(3), np.zeros(3)]
+    np.testing.assert_allclose(oc.block1e("ovlp", cen, [sh, sh])[..., 0], np.eye(2 * l + 1), atol=1e-12)
+    np.testing.assert_allclose(oc.block1e("kin", cen, [sh, sh])[..., 0], (2 * l + 3) * a / 2 * np.eye(2 * l + 1),
+                               atol=1e-12)
+    if l == 0:
+        assert oc.block1e("rinv", cen, [sh, sh], C=np.zeros(3))[0, 0, 0] == pytest.approx(
+            2 * math.sqrt(2 * a / math.pi), rel=1e-12)
+
+
+def _grid(n_rad=160, order=41):
+    from scipy.integrate import lebedev_rule
+    xs, ws = lebedev_rule(order)
+    t = np.cos(np.arange(1, n_rad + 1) * np.pi / (n_rad + 1))
+    wt = np.pi / (n_rad + 1) * np.sin(np.arange(1, n_rad + 1) * np.pi / (n_rad + 1)) ** 2 / np.sqrt(1 - t ** 2)
+    r = (1 + t) / (1 - t) * 1.2
+    wr = wt * 2 * 1.2 / (1 - t) ** 2 * r ** 2
+    pts = (r[:, None, None] * xs.T[None, :, :]).reshape(-1, 3)
+    return pts, (wr[:, None] * ws[None, :]).reshape(-1)
+
+
+@pytest.mark.parametrize("kind", ["plain", "ip1", "ipip1", "ipvip1"])
+def test_one_electron_blocks_against_quadrature(kind):
+    """A d shell at the origin against a contracted p shell 1.3 bohr away: overlap, kinetic (as
+    1/2 int grad f . grad g) and 1/r at the origin, each kind from AO values and their
+    derivatives on a radial x Lebedev grid centred at the origin (an independent route)."""
+    A = np.zeros(3)
+    B = np.array([0.5, -0.9, 0.8])
+    sa = (2, [1.1], [_norm_shell(2, 1.1)[2][0]])
+    sb = (1, [0.9, 0.35], [0.6, 0.5])
+    pts, w = _grid()
+    d = 3 if kind == "ipip1" else 2
+    fa = oc.eval_gto([(A, *sa)], pts, deriv=d)   # [ng, 5, ncomp]
+    fb = oc.eval_gto([(B, *sb)], pts, deriv=2)   # [ng, 3, ncomp]
+    comp = {c: i for i, c in enumerate(oc.cart_list(0) + oc.cart_list(1) + oc.cart_list(2) + oc.cart_list(3))}
+
+    def dd(f, dirs):
+        e = [0, 0, 0]
+        for x in dirs:
+            e[x] += 1
+        return f[:, :, comp[tuple(e)]]
+    rinv = 1.0 / np.linalg.norm(pts, axis=1)
+    kf = oc.KINDS1[kind]
+    nq = 3 ** len(kf)
+    for op in ("ovlp", "kin", "rinv"):
+        ref = np.zeros((5, 3, nq))
+        for q in range(nq):
+            dirs = [(q // 3, q % 3)[i] if len(kf) == 2 else q for i in range(len(kf))]
+            da = [x for f, x in zip(kf, dirs) if f == 0]
+            db = [x for f, x in zip(kf, dirs) if f == 1]
+            if op == "kin":
+                val = sum(np.einsum("g,ga,gb->ab", 0.5 * w, dd(fa, da + [e]), dd(fb, db + [e])) for e in range(3))
+            else:
+                wt = w * (rinv if op == "rinv" else 1.0)
+                val = np.einsum("g,ga,gb->ab", wt, dd(fa, da), dd(fb, db))
+            ref[:, :, q] = val
+        got = oc.block1e(op, [A, B], [sa, sb], kind, C=np.zeros(3))
+        np.testing.assert_allclose(got, ref, atol=2e-7 * np.abs(ref).max(), err_msg=f"{op} {kind}")
