@@ -221,6 +221,37 @@ hh_status hh_cint_int3c2e(const hh_cint* mol, const hh_cint* aux, int32_t kind, 
  *   int1e_ipkinip, int1e_ipnucip, int1e_iprinvip).  Nuclear charges are atm[.][0] of the
  *   molecule's libcint arrays.  Synchronous. */
 hh_status hh_cint_int1e(const hh_cint* mol, int32_t op, int32_t kind, int32_t atom, double* out, void* stream);
+
+/* Restricted Hartree-Fock with RI-JK (density-fitted J and K over the auxiliary basis `aux`, the
+ * metric of hh_rijk_create_from_j3c_decomp with HH_J2C_CD and threshold 1e-14) on the device:
+ * the mo_coeff / mo_occ / mo_energy the reference's RHessSCF takes (rscf.rs:30-95; its tests read
+ * them from PySCF, tests/hessian_rhf/mod.rs:39-41).  nocc doubly occupied orbitals; converged when
+ * |dE| < conv_tol and the orthogonal-basis commutator max|X^T (FDS - SDF) X| < sqrt(conv_tol)
+ * (core-Hamiltonian guess, DIIS over 8 Fock matrices).  Outputs on the device: mo_coeff [nao, nao],
+ * mo_energy [nao] ascending; host: *energy (total, with nuclear repulsion), *cycles (either may be
+ * NULL).  HH_ERR_NOT_CONVERGED after max_cycle cycles; HH_ERR_INVALID for a linearly dependent
+ * basis.  Synchronous. */
+hh_status hh_scf_rhf(const hh_cint* mol, const hh_cint* aux, int32_t nocc, double conv_tol, int32_t max_cycle,
+                     double* mo_coeff, double* mo_energy, double* energy, int32_t* cycles, void* stream);
+
+/* One-electron terms of the analytic Hessian (src/hessian/nuc_repl.rs, hcore.rs, ovlp.rs) on the
+ * device, all atoms.  Hessians out [3, 3, natm, natm] (column-major; block (., ., B, A) as the
+ * reference fills it); first derivatives out [nao, nao, 3, natm].  The basis shells must be
+ * grouped by atom (aoslice_by_atom).  Synchronous. */
+/* get_nuc_repl_hess (nuc_repl.rs:14-58) */
+hh_status hh_hess_nuc_repl(const hh_cint* mol, double* out, void* stream);
+/* get_hess_hcore (hcore.rs:105-125) with generator_hcore_deriv2 (hcore.rs:15-92): dm0 [nao, nao] */
+hh_status hh_hess_hcore(const hh_cint* mol, const double* dm0, double* out, void* stream);
+/* get_hess_ovlp (ovlp.rs:24-52): dme0 [nao, nao] the energy-weighted density */
+hh_status hh_hess_ovlp(const hh_cint* mol, const double* dme0, double* out, void* stream);
+/* generator_hcore_deriv1 (hcore.rs:140-161) for every atom: out[:, :, :, A] = its [nao, nao, 3] */
+hh_status hh_hess_hcore_deriv1(const hh_cint* mol, double* out, void* stream);
+/* generator_ovlp_deriv1 (ovlp.rs:67-82) for every atom */
+hh_status hh_hess_ovlp_deriv1(const hh_cint* mol, double* out, void* stream);
+/* out [nao, nao] = sum_i w[i] C[:, i] C[:, i]^T, C [nao, nmo] on the device, w [nmo] HOST:
+ * get_dm0_restricted (w = mo_occ) and get_dme0_restricted (w = mo_occ * mo_energy). */
+hh_status hh_hess_weighted_density(const double* mo_coeff, int64_t nao, int64_t nmo, const double* w, double* out,
+                                   void* stream);
 hh_status hh_cint_int3c2e_s2ij(const hh_cint* mol, const hh_cint* aux, int32_t sh0, int32_t sh1, double* out,
                                void* stream);
 /* An hh_int3c_deriv_fn for hh_rijk_skeleton over the engine: ctx = an hh_cint_pair made from the

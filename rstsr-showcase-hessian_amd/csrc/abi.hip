@@ -6,6 +6,8 @@
 #include "cphf.hpp"
 #include "cphf_batch.hpp"
 #include "cint.hpp"
+#include "hess_core.hpp"
+#include "scf.hpp"
 #include "linalg.hpp"
 #include "mlp.hpp"
 #include "numint.hpp"
@@ -440,6 +442,59 @@ hh_status hh_cint_int3c2e(const hh_cint* mol, const hh_cint* aux, int32_t kind, 
   HH_ABI_BEGIN
   HH_REQUIRE(mol && aux, "null handle");
   Cint::int3c2e(*mol->impl, *aux->impl, kind, sh0, sh1, out, (hipStream_t)stream);
+  HH_ABI_END
+}
+
+hh_status hh_scf_rhf(const hh_cint* mol, const hh_cint* aux, int32_t nocc, double conv_tol, int32_t max_cycle,
+                     double* mo_coeff, double* mo_energy, double* energy, int32_t* cycles, void* stream) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(mol && aux, "null argument");
+  const ScfResult r = rhf_scf(*mol->impl, *aux->impl, nocc, conv_tol, max_cycle, mo_coeff, mo_energy,
+                              (hipStream_t)stream);
+  if (energy) *energy = r.energy;
+  if (cycles) *cycles = r.cycles;
+  HH_ABI_END
+}
+
+hh_status hh_hess_nuc_repl(const hh_cint* mol, double* out, void* stream) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(mol && out, "null argument");
+  nuc_repl_hess(*mol->impl, out, (hipStream_t)stream);
+  HH_ABI_END
+}
+
+hh_status hh_hess_hcore(const hh_cint* mol, const double* dm0, double* out, void* stream) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(mol && dm0 && out, "null argument");
+  hcore_hess(*mol->impl, dm0, out, (hipStream_t)stream);
+  HH_ABI_END
+}
+
+hh_status hh_hess_ovlp(const hh_cint* mol, const double* dme0, double* out, void* stream) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(mol && dme0 && out, "null argument");
+  ovlp_hess(*mol->impl, dme0, out, (hipStream_t)stream);
+  HH_ABI_END
+}
+
+hh_status hh_hess_hcore_deriv1(const hh_cint* mol, double* out, void* stream) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(mol && out, "null argument");
+  hcore_deriv1(*mol->impl, out, (hipStream_t)stream);
+  HH_ABI_END
+}
+
+hh_status hh_hess_ovlp_deriv1(const hh_cint* mol, double* out, void* stream) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(mol && out, "null argument");
+  ovlp_deriv1(*mol->impl, out, (hipStream_t)stream);
+  HH_ABI_END
+}
+
+hh_status hh_hess_weighted_density(const double* mo_coeff, int64_t nao, int64_t nmo, const double* w, double* out,
+                                   void* stream) {
+  HH_ABI_BEGIN
+  weighted_density(mo_coeff, nao, nmo, w, out, (hipStream_t)stream);
   HH_ABI_END
 }
 

@@ -912,6 +912,33 @@ Cint::Cint(const int32_t* atm, int natm, const int32_t* bas, int nbas, const dou
     bas_cs_.push_back((int)cs_.size());
     bas_ao_.push_back(nao_);
   }
+  // AO range per atom; an atom without shells gets an empty range at its position
+  atom_ao_.assign(2 * (size_t)natm, -1);
+  for (int i = 0; i < nbas; ++i) {
+    const int atom = bas[8 * i];
+    int64_t& lo = atom_ao_[2 * atom];
+    int64_t& hi = atom_ao_[2 * atom + 1];
+    if (lo < 0) {
+      lo = bas_ao_[i];
+      hi = bas_ao_[i + 1];
+    } else {
+      hi = std::max(hi, bas_ao_[i + 1]);
+      lo = std::min(lo, bas_ao_[i]);
+    }
+  }
+  {
+    int64_t prev = 0, count = 0;
+    for (int a = 0; a < natm; ++a) {
+      if (atom_ao_[2 * a] < 0) {
+        atom_ao_[2 * a] = atom_ao_[2 * a + 1] = prev;
+      } else {
+        prev = atom_ao_[2 * a + 1];
+      }
+      count += atom_ao_[2 * a + 1] - atom_ao_[2 * a];
+    }
+    // grouped by atom: the ranges tile [0, nao) without overlap
+    atom_contiguous_ = count == nao_;
+  }
   d_cs_.reserve(sizeof(CShell) * cs_.size());
   HH_HIP(hipMemcpy(d_cs_.ptr, cs_.data(), sizeof(CShell) * cs_.size(), hipMemcpyHostToDevice));
   d_prim_.reserve(sizeof(double) * prim_h_.size());
@@ -1060,6 +1087,13 @@ const Cint& Cint::point_charges() const {
     points_ = std::make_unique<Cint>(atm.data(), natm, bas.data(), natm, env.data(), (int64_t)env.size(), cart_);
   }
   return *points_;
+}
+
+void Cint::int1e_rinv_all(int kind, double* out, hipStream_t s) const {
+  HH_REQUIRE(kind == HH_INT_PLAIN || kind == HH_INT_IP1 || kind == HH_INT_IPIP1 || kind == HH_INT_IPVIP1,
+             "int1e: kinds PLAIN, IP1, IPIP1, IPVIP1");
+  HH_REQUIRE(out, "int1e: null output");
+  int3c2e(*this, point_charges(), kind, 0, natm(), out, s);
 }
 
 void Cint::int1e_nuc(int kind, int atom, double* out, hipStream_t s) const {

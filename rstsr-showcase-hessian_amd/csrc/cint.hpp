@@ -60,6 +60,12 @@ class Cint {
   // ipvip derivatives on the functions); atom >= 0: <a| 1 / |r - R_atom| |b> (int1e_rinv with the
   // origin at that nucleus, no charge), as 3-centre Coulomb integrals against unit point charges.
   void int1e_nuc(int kind, int atom, double* out, hipStream_t s) const;
+  // <a| 1 / |r - R_C| |b> for every nucleus C: out [nao, nao, natm, comp]
+  void int1e_rinv_all(int kind, double* out, hipStream_t s) const;
+  // AO range [ao0, ao1) of atom a (aoslice_by_atom; shells must be grouped by atom)
+  int64_t atom_ao0(int a) const { return atom_ao_[2 * a]; }
+  int64_t atom_ao1(int a) const { return atom_ao_[2 * a + 1]; }
+  bool atoms_contiguous() const { return atom_contiguous_; }
   int natm() const { return (int)atom_z_.size(); }
   double atom_charge(int a) const { return atom_z_[a]; }
   const double* atom_xyz(int a) const { return &atom_xyz_[3 * a]; }
@@ -69,6 +75,8 @@ class Cint {
  private:
   const Cint& point_charges() const;
   std::vector<double> atom_z_, atom_xyz_;
+  std::vector<int64_t> atom_ao_;
+  bool atom_contiguous_ = false;
   mutable std::unique_ptr<Cint> points_;
   std::vector<CShell> cs_;          // contracted shells (host copy)
   std::vector<int> bas_cs_;         // first contracted shell of libcint shell i (i <= nbas)

@@ -86,6 +86,13 @@ SIGNATURES.update({
     "hh_linalg_cholesky": (_i32, [_p, _i64, _p]),
     "hh_linalg_eigh": (_i32, [_p, _i64, _p, _p]),
     "hh_cint_int1e": (_i32, [_p, _i32, _i32, _i32, _p, _p]),
+    "hh_hess_nuc_repl": (_i32, [_p, _p, _p]),
+    "hh_scf_rhf": (_i32, [_p, _p, _i32, _d, _i32, _p, _p, _p, _p, _p]),
+    "hh_hess_hcore": (_i32, [_p, _p, _p, _p]),
+    "hh_hess_ovlp": (_i32, [_p, _p, _p, _p]),
+    "hh_hess_hcore_deriv1": (_i32, [_p, _p, _p]),
+    "hh_hess_ovlp_deriv1": (_i32, [_p, _p, _p]),
+    "hh_hess_weighted_density": (_i32, [_p, _i64, _i64, _p, _p, _p]),
     "hh_linalg_trsm_lower": (_i32, [_p, _i64, _p, _i64, _i64, _i32, _p]),
     "hh_util_copy": (_i32, [_p, _p, _i64, _p]),
     "hh_vib_harmonic_analysis": (_i32, [_p, _p, _p, _i64, _i32, _i32, _p]),

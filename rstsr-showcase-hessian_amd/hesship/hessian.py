@@ -1,0 +1,199 @@
+"""The whole RHF analytic Hessian on the device (reference: src/hessian/rscf.rs RHessSCF with
+its components: HessNucRepl, nuc_repl.rs; RHessHcore, hcore.rs; RHessOvlp, ovlp.rs; the RI-JK
+electron interaction RHessRIJK, hess_r.rs; the CPHF part) and the density-fitted RHF that
+supplies its orbitals (the reference reads them from PySCF, tests/hessian_rhf/mod.rs:39-41).
+
+    mol = CInt.from_basis(atoms, basis); aux = CInt.from_basis(atoms, aux_basis)
+    scf = RHF(mol, aux, nocc).run()
+    hess = RHessSCF(mol, aux, scf.mo_coeff, scf.mo_occ, scf.mo_energy)
+    de = hess.make_hess()             # [3, 3, natm, natm]; hess.result holds the parts
+
+Every term runs in the library (hh_* entry points); torch holds device memory and adds the five
+[3, 3, natm, natm] parts.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from ._lib import check, lib
+from .cphf import RHessCPHF
+from .rijk import RHessRIJK
+from .tensors import empty_f, fortran, stream_of
+
+
+def _dev(device):
+    return device or torch.device("cuda", torch.cuda.current_device())
+
+
+def _gemm(M, N, K, A, lda, a_kmajor, B, ldb, b_kmajor, Cm, ldc, alpha=1.0, beta=0.0, nb0=1, strides=None, dev=None):
+    st = (C.c_int64 * 6)(*strides) if strides is not None else None
+    check(lib().hh_util_gemm(M, N, K, float(alpha), A.data_ptr(), lda, int(a_kmajor), B.data_ptr(), ldb,
+                             int(b_kmajor), float(beta), Cm.data_ptr(), ldc, nb0, 1, st, 0, stream_of(dev)))
+
+
+def weighted_density(mo_coeff, w):
+    """sum_i w_i C_i C_i^T (get_dm0_restricted with w = mo_occ, get_dme0_restricted with
+    w = mo_occ mo_energy)."""
+    c = fortran(mo_coeff)
+    nao, nmo = c.shape
+    wh = np.ascontiguousarray(np.asarray(w, dtype=np.float64))
+    out = empty_f((nao, nao), c.device)
+    check(lib().hh_hess_weighted_density(c.data_ptr(), nao, nmo, wh.ctypes.data, out.data_ptr(), stream_of(c.device)))
+    return out
+
+
+class RHF:
+    """Density-fitted restricted Hartree-Fock on the device (hh_scf_rhf)."""
+
+    def __init__(self, mol, aux, nocc, conv_tol=1e-12, max_cycle=100, device=None):
+        self.mol, self.aux, self.nocc = mol, aux, int(nocc)
+        self.conv_tol, self.max_cycle = float(conv_tol), int(max_cycle)
+        self.device = _dev(device)
+        self.mo_coeff = self.mo_energy = self.mo_occ = None
+        self.e_tot = None
+        self.cycles = 0
+
+    def run(self):
+        nao = self.mol.nao
+        c = empty_f((nao, nao), self.device)
+        e = torch.empty(nao, dtype=torch.float64, device=self.device)
+        en, cy = C.c_double(), C.c_int32()
+        check(lib().hh_scf_rhf(self.mol._h, self.aux._h, self.nocc, self.conv_tol, self.max_cycle, c.data_ptr(),
+                               e.data_ptr(), C.byref(en), C.byref(cy), stream_of(self.device)))
+        self.mo_coeff, self.mo_energy = c, e
+        self.mo_occ = np.where(np.arange(nao) < self.nocc, 2.0, 0.0)
+        self.e_tot, self.cycles = en.value, cy.value
+        return self
+
+
+def nuc_repl_hess(mol, device=None):
+    """get_nuc_repl_hess (nuc_repl.rs:14-58): [3, 3, natm, natm]."""
+    dev = _dev(device)
+    natm = len(mol.atm)
+    out = empty_f((3, 3, natm, natm), dev)
+    check(lib().hh_hess_nuc_repl(mol._h, out.data_ptr(), stream_of(dev)))
+    return out
+
+
+def hcore_hess(mol, dm0):
+    """get_hess_hcore (hcore.rs:105-125)."""
+    d = fortran(dm0)
+    natm = len(mol.atm)
+    out = empty_f((3, 3, natm, natm), d.device)
+    check(lib().hh_hess_hcore(mol._h, d.data_ptr(), out.data_ptr(), stream_of(d.device)))
+    return out
+
+
+def ovlp_hess(mol, dme0):
+    """get_hess_ovlp (ovlp.rs:24-52)."""
+    d = fortran(dme0)
+    natm = len(mol.atm)
+    out = empty_f((3, 3, natm, natm), d.device)
+    check(lib().hh_hess_ovlp(mol._h, d.data_ptr(), out.data_ptr(), stream_of(d.device)))
+    return out
+
+
+def hcore_deriv1(mol, device=None):
+    """generator_hcore_deriv1 (hcore.rs:140-161) for every atom: [nao, nao, 3, natm]."""
+    dev = _dev(device)
+    out = empty_f((mol.nao, mol.nao, 3, len(mol.atm)), dev)
+    check(lib().hh_hess_hcore_deriv1(mol._h, out.data_ptr(), stream_of(dev)))
+    return out
+
+
+def ovlp_deriv1(mol, device=None):
+    """generator_ovlp_deriv1 (ovlp.rs:67-82) for every atom: [nao, nao, 3, natm]."""
+    dev = _dev(device)
+    out = empty_f((mol.nao, mol.nao, 3, len(mol.atm)), dev)
+    check(lib().hh_hess_ovlp_deriv1(mol._h, out.data_ptr(), stream_of(dev)))
+    return out
+
+
+def rijk_for(mol, aux, device=None):
+    """RHessRIJK::new_without_cderi (hess_r.rs:1751-1773): the metric of (mol, aux) with the
+    Cholesky policy and threshold 1e-14, scale_j = scale_k = 1 (RHF)."""
+    dev = _dev(device)
+    j3c = mol.int3c2e_s2ij(aux, device=dev)
+    j2c = aux.int2c2e("plain", device=dev)
+    return RHessRIJK.from_j3c(j3c, j2c, 1.0, 1.0, threshold=1e-14, policy="cd")
+
+
+class RHessSCF:
+    """RHessSCF (rscf.rs:21-506) for RHF with the RI-JK electron interaction: nuclear repulsion,
+    core Hamiltonian, overlap, RI-JK skeleton and the CPHF part, all on the device."""
+
+    def __init__(self, mol, aux, mo_coeff, mo_occ, mo_energy, config=None, rijk=None):
+        self.mol, self.aux = mol, aux
+        self.mo_coeff = fortran(mo_coeff)
+        self.device = self.mo_coeff.device
+        self.mo_occ = np.asarray(mo_occ.cpu() if isinstance(mo_occ, torch.Tensor) else mo_occ, dtype=np.float64)
+        self.mo_energy = np.asarray(mo_energy.cpu() if isinstance(mo_energy, torch.Tensor) else mo_energy,
+                                    dtype=np.float64)
+        self.config = dict(config or {})
+        self.rijk = rijk if rijk is not None else rijk_for(mol, aux, self.device)
+        self.result = {}
+
+    @property
+    def natm(self):
+        return len(self.mol.atm)
+
+    def _skeleton_rijk(self):
+        mol, aux, dev = self.mol, self.aux, self.device
+
+        def int3c(kind, p0, p1):
+            sh0 = int(np.nonzero(aux.ao_loc == p0)[0][0])
+            sh1 = int(np.nonzero(aux.ao_loc == p1)[0][0])
+            return mol.int3c2e(aux, kind, shls=(sh0, sh1), device=dev)
+        ranges = [(int(aux.ao_loc[0]), int(aux.ao_loc[-1]))]
+        v1, v2, v3 = (aux.int2c2e(k, device=dev) for k in ("ip1", "ipip1", "ip1ip2"))
+        return self.rijk.make_skeleton(self.mo_coeff, self.mo_occ, int3c, v1, v2, v3, ranges,
+                                       mol.aoslice_by_atom(), aux.aoslice_by_atom())
+
+    def compute_f1mo_s1mo(self, f1bra_el):
+        """compute_dimless_cphf_rhs's f1mo and s1mo (rscf.rs:125-164): C^T (f1bra_el + h1ao C_o)
+        and C^T s1ao C_o, [nmo, nocc, 3, natm]."""
+        c = self.mo_coeff
+        nao, nmo = c.shape
+        nocc = int((self.mo_occ > 0).sum())
+        nb = 3 * self.natm
+        h1 = hcore_deriv1(self.mol, self.device)
+        s1 = ovlp_deriv1(self.mol, self.device)
+        f1bra = fortran(f1bra_el).clone()
+        s1bra = empty_f((nao, nocc, 3, self.natm), self.device)
+        # (h1ao + ...) C_o per (x, A): batched over the 3 natm matrices
+        _gemm(nao, nocc, nao, h1, nao, False, c, nao, True, f1bra, nao, 1.0, 1.0, nb,
+              (nao * nao, 0, 0, 0, nao * nocc, 0), self.device)
+        _gemm(nao, nocc, nao, s1, nao, False, c, nao, True, s1bra, nao, 1.0, 0.0, nb,
+              (nao * nao, 0, 0, 0, nao * nocc, 0), self.device)
+        f1mo = empty_f((nmo, nocc, 3, self.natm), self.device)
+        s1mo = empty_f((nmo, nocc, 3, self.natm), self.device)
+        # C^T x: A[m, k] = C[k, m] (k-major), batched
+        _gemm(nmo, nocc, nao, c, nao, True, f1bra, nao, True, f1mo, nmo, 1.0, 0.0, nb,
+              (0, 0, nao * nocc, 0, nmo * nocc, 0), self.device)
+        _gemm(nmo, nocc, nao, c, nao, True, s1bra, nao, True, s1mo, nmo, 1.0, 0.0, nb,
+              (0, 0, nao * nocc, 0, nmo * nocc, 0), self.device)
+        return f1mo, s1mo
+
+    def make_hess(self, precision="fp64", stats=None):
+        """rscf.rs:481-505: skeleton (nuclear, core, RI-JK) + overlap + CPHF; parts in self.result."""
+        occ, en = self.mo_occ, self.mo_energy
+        dm0 = weighted_density(self.mo_coeff, occ)
+        dme0 = weighted_density(self.mo_coeff, occ * en)
+        r = self.result
+        r["de_skeleton_nuc"] = nuc_repl_hess(self.mol, self.device)
+        r["de_skeleton_hcore"] = hcore_hess(self.mol, dm0)
+        de_jk, f1bra_el = self._skeleton_rijk()
+        r["de_skeleton_rijk"] = de_jk
+        r["de_ovlp"] = ovlp_hess(self.mol, dme0)
+        f1mo, s1mo = self.compute_f1mo_s1mo(f1bra_el)
+        self.rijk.make_response_preparation(self.mo_coeff, occ)
+        cfg = {k: v for k, v in self.config.items() if k.startswith("cphf") or k == "level_shift"}
+        drv = RHessCPHF(self.mo_coeff, occ, en, cfg)
+        drv.add_el(self.rijk)
+        r["de_cphf"] = drv.make_cphf_hess(f1mo, s1mo, precision=precision, stats=stats)
+        tot = None
+        for k in ("de_skeleton_nuc", "de_skeleton_hcore", "de_skeleton_rijk", "de_ovlp", "de_cphf"):
+            tot = r[k].clone() if tot is None else tot.add_(r[k])
+        r["de_tot"] = tot
+        return tot
