@@ -1062,6 +1062,74 @@ def workload_cint(args):
     return line
 
 
+def workload_hessian(args):
+    """The whole RHF analytic Hessian of a seeded (H2O)_n cluster (default n = 8: the c3 molecule,
+    def2-SVP-sized basis, nao 192, even-tempered auxiliary basis naux 632) on the device: the
+    density-fitted SCF, then make_hess (nuclear repulsion + core + RI-JK skeleton + overlap +
+    CPHF) with the CPHF in FP64 and in the mixed BF16x3 mode with FP64 refinement; harmonic
+    frequencies of both Hessians.  One step = one make_hess."""
+    import numpy as np
+    import torch
+    import hesship
+    from hesship import synth, vib
+    dev = torch.device("cuda:0")
+    z, R = synth.water_cluster(args.nwater, box_ang=7.0 if args.nwater <= 8 else 7.0 * (args.nwater / 8) ** (1 / 3),
+                               seed=0x5EED0008)
+    sym = {8: "O", 1: "H"}
+    atoms = [(sym[int(c)], tuple(x)) for c, x in zip(z, R)]
+    mol = hesship.CInt.from_basis(atoms, SVP_LIKE)
+    aux = hesship.CInt.from_basis(atoms, JK_LIKE)
+    nocc = 5 * args.nwater
+    log(f"[bench] hessian: (H2O)_{args.nwater} nao {mol.nao} naux {aux.nao} natm {len(atoms)}")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    scf = hesship.RHF(mol, aux, nocc, conv_tol=1e-11, max_cycle=200).run()
+    scf_s = time.perf_counter() - t0
+    log(f"[bench] hessian: SCF {scf_s:.2f} s, {scf.cycles} cycles, E = {scf.e_tot:.8f}")
+    out = {}
+    for prec in ("fp64", "bf16x3"):
+        drv = hesship.RHessSCF(mol, aux, scf.mo_coeff, scf.mo_occ, scf.mo_energy,
+                               config=dict(cphf_tol=1e-9 if prec == "bf16x3" else 1e-8))
+        st = {}
+        walls = []
+        for it in range(args.warmup + args.steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            de = drv.make_hess(precision=prec, stats=st)
+            torch.cuda.synchronize()
+            if it >= args.warmup:
+                walls.append(time.perf_counter() - t0)
+        out[prec] = {"wall_s": round(min(walls), 4), "stages_s": {k: round(v, 4) for k, v in drv.timing.items()},
+                     "cphf_cycles": st.get("cycles"), "de": de.cpu().numpy()}
+        log(f"[bench] hessian {prec}: {out[prec]['wall_s']} s {out[prec]['stages_s']}")
+    natm = len(atoms)
+    mass = np.array([15.99491462 if s == "O" else 1.00782503 for s, _ in atoms])
+
+    def freqs(de):
+        H = de.transpose(2, 0, 3, 1).reshape(3 * natm, 3 * natm)
+        H = 0.5 * (H + H.T)
+        r = vib.harmonic_analysis(H, np.asarray(R).T, mass)
+        return np.where(r.imag[r.vib_indices], -1.0, 1.0) * r.omega[r.vib_indices]
+    f64, fmx = freqs(out["fp64"]["de"]), freqs(out["bf16x3"]["de"])
+    dH = float(np.abs(out["fp64"]["de"] - out["bf16x3"]["de"]).max())
+    wall = out["bf16x3"]["wall_s"]
+    line = _line("CPHF+Hessian wall s (whole RHF Hessian, mixed-precision CPHF), 1 MI355X", wall, "s",
+                 wall * 1e3, args.steps, args.warmup, "bf16x3 CPHF response with FP64 refinement; FP64 elsewhere",
+                 {"workload": f"seeded (H2O)_{args.nwater} RHF, def2-SVP-sized basis, even-tempered auxiliary",
+                  "nao": mol.nao, "naux": aux.nao, "natm": natm, "nocc": nocc, "nprop": 3 * natm}, None,
+                 higher=False)
+    line["data"] = "synthetic molecule (seeded water cluster; basis sets synthetic, none exist in this image)"
+    line["scf"] = {"wall_s": round(scf_s, 3), "cycles": scf.cycles, "e_tot": scf.e_tot}
+    line["fp64"] = {k: v for k, v in out["fp64"].items() if k != "de"}
+    line["mixed"] = {k: v for k, v in out["bf16x3"].items() if k != "de"}
+    line["speedup_vs_fp64"] = round(out["fp64"]["wall_s"] / wall, 3)
+    line["accuracy"] = {"max_abs_hessian_diff_vs_fp64": dH,
+                        "max_abs_freq_diff_cm1_vs_fp64": float(np.abs(f64 - fmx).max()) if len(f64) == len(fmx) else None,
+                        "n_vib": int(len(f64)), "lowest_cm1": float(f64.min()), "highest_cm1": float(f64.max())}
+    line["note"] = "roofline per kernel: the ri_jk, skeleton and numint workloads; no CPU leg (the oracle has no whole Hessian)"
+    return line
+
+
 def workload_xc(args):
     """B3LYP effective kernels (get_rho_vxc_fxc's libxc_eval_eff loop, hess_rks.rs:205-216 and
     hess_uks.rs:96-106) on the 543 600-point grid: one step = exc, vxc_eff, fxc_eff for the
@@ -1143,7 +1211,8 @@ def main():
                                                          "(the rijk_scale2 field of an earlier run) instead of the "
                                                          "spectrum search")
     ap.add_argument("--workload", default="ri_jk", choices=["ri_jk", "numint", "dft_setup", "dft_becke", "skeleton", "cphf_batch", "xc",
-                                                            "cint"])
+                                                            "cint", "hessian"])
+    ap.add_argument("--nwater", type=int, default=8, help="hessian: water molecules in the cluster")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run the library's 16-byte copy kernel over GIB GiB three times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes) and exit")
@@ -1162,7 +1231,7 @@ def main():
     if args.workload != "ri_jk":
         fn = {"numint": workload_numint, "dft_setup": workload_dft_setup, "dft_becke": workload_dft_becke,
               "skeleton": workload_skeleton, "cphf_batch": workload_cphf_batch, "xc": workload_xc,
-            "cint": workload_cint}[args.workload]
+            "cint": workload_cint, "hessian": workload_hessian}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
 

@@ -12,6 +12,7 @@ Every term runs in the library (hh_* entry points); torch holds device memory an
 [3, 3, natm, natm] parts.
 """
 import ctypes as C
+import time
 
 import numpy as np
 import torch
@@ -133,6 +134,7 @@ class RHessSCF:
         self.config = dict(config or {})
         self.rijk = rijk if rijk is not None else rijk_for(mol, aux, self.device)
         self.result = {}
+        self.timing = {}  # seconds per stage of the last make_hess (every stage ends synchronised)
 
     @property
     def natm(self):
@@ -178,20 +180,34 @@ class RHessSCF:
     def make_hess(self, precision="fp64", stats=None):
         """rscf.rs:481-505: skeleton (nuclear, core, RI-JK) + overlap + CPHF; parts in self.result."""
         occ, en = self.mo_occ, self.mo_energy
+        r, tm = self.result, self.timing
+        tm.clear()
+        t0 = time.perf_counter()
+
+        def lap(name):
+            nonlocal t0
+            t1 = time.perf_counter()
+            tm[name] = t1 - t0
+            t0 = t1
         dm0 = weighted_density(self.mo_coeff, occ)
         dme0 = weighted_density(self.mo_coeff, occ * en)
-        r = self.result
         r["de_skeleton_nuc"] = nuc_repl_hess(self.mol, self.device)
+        lap("nuc")
         r["de_skeleton_hcore"] = hcore_hess(self.mol, dm0)
+        lap("hcore")
         de_jk, f1bra_el = self._skeleton_rijk()
         r["de_skeleton_rijk"] = de_jk
+        lap("rijk_skeleton")
         r["de_ovlp"] = ovlp_hess(self.mol, dme0)
+        lap("ovlp")
         f1mo, s1mo = self.compute_f1mo_s1mo(f1bra_el)
+        lap("f1mo_s1mo")
         self.rijk.make_response_preparation(self.mo_coeff, occ)
         cfg = {k: v for k, v in self.config.items() if k.startswith("cphf") or k == "level_shift"}
         drv = RHessCPHF(self.mo_coeff, occ, en, cfg)
         drv.add_el(self.rijk)
         r["de_cphf"] = drv.make_cphf_hess(f1mo, s1mo, precision=precision, stats=stats)
+        lap("cphf")
         tot = None
         for k in ("de_skeleton_nuc", "de_skeleton_hcore", "de_skeleton_rijk", "de_ovlp", "de_cphf"):
             tot = r[k].clone() if tot is None else tot.add_(r[k])
