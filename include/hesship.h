@@ -398,6 +398,20 @@ hh_status hh_xceff_transform_polarized(int32_t den_type, int32_t order, const do
 hh_status hh_xceff_libxc_to_xcfun_map(int32_t den_type, int32_t deriv, int64_t* map, int64_t capacity,
                                       int64_t* len);
 
+/* Restricted Kohn-Sham with RI-JK: hh_scf_rhf's SCF with exact exchange scaled by exx (the
+ * hybrid fraction, hyb_exx_coef) and the functional list (xc_ids / xc_scales HOST arrays of
+ * nfunc LibXC ids and scales, as get_rho_vxc_fxc's list, hess_rks.rs:193-216) evaluated on the
+ * grid of `grid` (an hh_numint whose AO values carry first derivatives for SIGMA / TAU; it must
+ * run on `stream`); den_type the list's strictest family.  E = E_core + E_J - exx E_K + E_xc. */
+hh_status hh_scf_rks(const hh_cint* mol, const hh_cint* aux, hh_numint* grid, int32_t den_type, int32_t nfunc,
+                     const int32_t* xc_ids, const double* xc_scales, double exx, int32_t nocc, double conv_tol,
+                     int32_t max_cycle, double* mo_coeff, double* mo_energy, double* energy, int32_t* cycles,
+                     void* stream);
+/* The exchange-correlation potential matrix out [nao, nao] = dE_xc / dD (the Kohn-Sham Fock
+ * term; no reference counterpart on the Hessian path, which takes converged orbitals) from
+ * vxc_eff [ng, nvar] of hh_xc_eval_eff (weights not applied).  Synchronous. */
+hh_status hh_numint_vxc_matrix(hh_numint* h, int32_t den_type, const double* vxc_eff, double* out);
+
 /* Exchange-correlation functionals evaluated on the device (the values LibXC's compute_xc gives
  * libxc_eval_inner, src/xceff/libxc_wrap.rs:32-82; functional families as determine_den_type,
  * libxc_wrap.rs:10-23).  Functional ids are LibXC's: HH_XC_LDA_X 1 (Slater), HH_XC_LDA_C_VWN 7

@@ -456,6 +456,36 @@ hh_status hh_scf_rhf(const hh_cint* mol, const hh_cint* aux, int32_t nocc, doubl
   HH_ABI_END
 }
 
+hh_status hh_scf_rks(const hh_cint* mol, const hh_cint* aux, hh_numint* grid, int32_t den_type, int32_t nfunc,
+                     const int32_t* xc_ids, const double* xc_scales, double exx, int32_t nocc, double conv_tol,
+                     int32_t max_cycle, double* mo_coeff, double* mo_energy, double* energy, int32_t* cycles,
+                     void* stream) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(mol && aux && grid, "null argument");
+  HH_REQUIRE(den_type >= HH_DEN_RHO && den_type <= HH_DEN_TAU, "den_type must be RHO, SIGMA or TAU");
+  std::vector<int> ids(xc_ids, xc_ids + std::max(0, nfunc));
+  ScfXc xc;
+  xc.ni = grid->impl.get();
+  xc.den = den_type;
+  xc.nfunc = nfunc;
+  xc.ids = ids.data();
+  xc.scales = xc_scales;
+  xc.exx = exx;
+  const ScfResult r = rhf_scf(*mol->impl, *aux->impl, nocc, conv_tol, max_cycle, mo_coeff, mo_energy,
+                              (hipStream_t)stream, &xc);
+  if (energy) *energy = r.energy;
+  if (cycles) *cycles = r.cycles;
+  HH_ABI_END
+}
+
+hh_status hh_numint_vxc_matrix(hh_numint* h, int32_t den_type, const double* vxc_eff, double* out) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(h && vxc_eff && out, "null argument");
+  h->impl->vxc_matrix(den_type, vxc_eff, out);
+  HH_HIP(hipStreamSynchronize(h->impl->stream()));
+  HH_ABI_END
+}
+
 hh_status hh_hess_nuc_repl(const hh_cint* mol, double* out, void* stream) {
   HH_ABI_BEGIN
   HH_REQUIRE(mol && out, "null argument");

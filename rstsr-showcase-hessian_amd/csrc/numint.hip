@@ -825,6 +825,80 @@ void Numint::fxc_pot_bra_trans(int den, const double* fxc_eff, const double* rho
   }
 }
 
+namespace {
+// Exchange-correlation potential operand of one grid batch: S_c[g, nu] with
+// V[mu, nu] = sum_c sum_g ao_c[g, mu] S_c[g, nu] = int phi_mu v_rho phi_nu + v_t (phi_mu d_t phi_nu +
+// d_t phi_mu phi_nu) + v_tau / 2 grad phi_mu . grad phi_nu  (wv = w vxc_eff)
+template <int DEN>
+__global__ void vxc_s_kernel(const double* __restrict__ ao, int64_t ng, int64_t nao, int64_t g0, int64_t gb,
+                             const double* __restrict__ vxc, const double* __restrict__ w, double* __restrict__ S) {
+  const int64_t total = gb * nao;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t g = e % gb, nu = e / gb;
+    const int64_t gg = g0 + g;
+    const double wg = w[gg];
+    const double phi = ao[gg + ng * nu];
+    if (DEN == DEN_RHO) {
+      S[e] = wg * vxc[gg] * phi;
+    } else {
+      double d[3], wv[4];
+      for (int t = 0; t < 3; ++t) d[t] = ao[gg + ng * (nu + nao * (t + 1))];
+      for (int x = 0; x < 4; ++x) wv[x] = wg * vxc[gg + ng * x];
+      const double htau = DEN == DEN_TAU ? 0.5 * wg * vxc[gg + ng * 4] : 0.0;
+      S[e] = wv[0] * phi + wv[1] * d[0] + wv[2] * d[1] + wv[3] * d[2];
+      for (int t = 0; t < 3; ++t) S[e + gb * nao * (t + 1)] = wv[t + 1] * phi + htau * d[t];
+    }
+  }
+}
+
+// part[b] = sum over the block's points of w exc rho
+__global__ __launch_bounds__(256) void xc_energy_kernel(const double* __restrict__ w, const double* __restrict__ exc,
+                                                        const double* __restrict__ rho, int64_t ng,
+                                                        double* __restrict__ part) {
+  double v = 0.0;
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < ng; g += (int64_t)gridDim.x * blockDim.x)
+    v += w[g] * exc[g] * rho[g];
+  __shared__ double sd[256];
+  sd[threadIdx.x] = v;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) sd[threadIdx.x] += sd[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sd[0];
+}
+}  // namespace
+
+void Numint::vxc_matrix(int den, const double* vxc_eff, double* out) {
+  HH_REQUIRE(den >= DEN_RHO && den <= DEN_TAU, "numint: den_type must be RHO, SIGMA or TAU");
+  const int nc = den_ncomp(den);
+  HH_REQUIRE(ncomp_ >= nc, "numint: AO component dimension insufficient");
+  HH_REQUIRE(vxc_eff && out, "numint: null argument");
+  const int64_t gbmax = std::min<int64_t>(ng_, std::max<int64_t>(256, (1LL << 27) / (nao_ * nc)));
+  double* S = static_cast<double*>(akb_.reserve(sizeof(double) * gbmax * nao_ * nc));
+  for (int64_t g0 = 0; g0 < ng_; g0 += gbmax) {
+    const int64_t gb = std::min(gbmax, ng_ - g0);
+    HH_DEN_DISPATCH(den, vxc_s_kernel, dim3(nblk(gb * nao_)), dim3(256), 0, stream_, ao_.as<double>(), ng_, nao_, g0,
+                    gb, vxc_eff, w_.as<double>(), S);
+    back_contract(ao_.as<double>(), ng_, nao_, g0, gb, nc, S, nao_, out, 1.0, g0 > 0, PREC_F64, stream_, ws_);
+  }
+}
+
+double Numint::xc_energy(const double* exc, const double* rho) {
+  HH_REQUIRE(exc && rho, "numint: null argument");
+  constexpr int NB = 256;
+  const unsigned nb = (unsigned)std::min<int64_t>(NB, ceil_div(ng_, 256));
+  double* part = static_cast<double*>(aob2_.reserve(sizeof(double) * NB));
+  hipLaunchKernelGGL(xc_energy_kernel, dim3(nb), dim3(256), 0, stream_, w_.as<double>(), exc, rho, ng_, part);
+  HH_HIP(hipGetLastError());
+  std::vector<double> h(nb);
+  HH_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, stream_));
+  HH_HIP(hipStreamSynchronize(stream_));
+  double e = 0.0;
+  for (double v : h) e += v;
+  return e;
+}
+
 void Numint::rks_response_bra(int den, const double* fxc_eff, const double* mo1_bra, int64_t nprop,
                               const double* mocc, int64_t nocc, double* out, Prec prec, bool accumulate) {
   HH_REQUIRE(den >= DEN_RHO && den <= DEN_TAU, "numint: den_type must be RHO, SIGMA or TAU");

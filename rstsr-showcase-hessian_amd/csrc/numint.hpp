@@ -48,6 +48,13 @@ class Numint {
                         const double* const* mocc, const int64_t* nocc, double* const* out, Prec prec,
                         bool accumulate = false);
 
+  // The exchange-correlation potential matrix out [nao, nao] = dE_xc / dD of an RKS density from
+  // vxc_eff [ng, nvar] (the rho_t derivatives, weights not applied): the Fock contribution of the
+  // functional in an SCF.  Needs AO first derivatives for SIGMA / TAU.
+  void vxc_matrix(int den, const double* vxc_eff, double* out);
+  // sum_g w_g exc_g rho_g (exc [ng] the energy per particle, rho the density column)
+  double xc_energy(const double* exc, const double* rho);
+
   // DFT skeleton-Hessian setup over the grid (hess_rks.rs:71-565; hess_uks.rs:56-384), nspin 1
   // (RKS, dm0 = the closed-shell density) or 2 (UKS, one density per spin).  The AO tensor needs
   // derivatives to order 2 (RHO, 10 components) or 3 (SIGMA / TAU, 20 components).

@@ -17,7 +17,9 @@
 
 #include "gemm.hpp"
 #include "linalg.hpp"
+#include "numint.hpp"
 #include "rijk.hpp"
+#include "xc_functionals.hpp"
 
 namespace hh {
 
@@ -146,7 +148,7 @@ void mm(int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, bool ta, 
 }  // namespace
 
 ScfResult rhf_scf(const Cint& mol, const Cint& aux, int nocc, double conv_tol, int max_cycle, double* mo_coeff,
-                  double* mo_energy, hipStream_t s) {
+                  double* mo_energy, hipStream_t s, const ScfXc* xc) {
   const int64_t nao = mol.nao(), naux = aux.nao(), n2 = nao * nao;
   HH_REQUIRE(nocc > 0 && nocc <= nao, "scf: nocc out of range");
   HH_REQUIRE(conv_tol > 0.0 && max_cycle > 0, "scf: conv_tol > 0, max_cycle > 0");
@@ -166,6 +168,18 @@ ScfResult rhf_scf(const Cint& mol, const Cint& aux, int nocc, double conv_tol, i
   D.alloc(n2); J.alloc(n2); T.alloc(nao * nocc * naux); v.alloc(naux); W1.alloc(n2); W2.alloc(n2);
   Fs.alloc(NDIIS * n2); Es.alloc(NDIIS * n2); G.alloc(NDIIS * NDIIS); H.alloc((NDIIS + 1) * (NDIIS + 1));
   rhs.alloc(NDIIS + 1);
+  const double exx = xc ? xc->exx : 1.0;
+  Mat rho, exc, vxc, V;
+  int64_t ng = 0;
+  int nvl = 1;
+  if (xc) {
+    HH_REQUIRE(xc->ni && xc->nfunc >= 0 && (xc->nfunc == 0 || (xc->ids && xc->scales)), "scf: incomplete XC spec");
+    HH_REQUIRE(xc->ni->nao() == nao, "scf: the grid's AO count differs from the basis");
+    HH_REQUIRE(xc->ni->stream() == s, "scf: the grid handle runs on another stream");
+    ng = xc->ni->ng();
+    nvl = xc->den == HH_DEN_RHO ? 1 : (xc->den == HH_DEN_SIGMA ? 4 : 5);
+    rho.alloc(ng * nvl); exc.alloc(ng); vxc.alloc(ng * nvl); V.alloc(n2);
+  }
   constexpr int NPART = 256;
   part.alloc(2 * NPART);
   std::vector<double> hpart(2 * NPART);
@@ -234,10 +248,25 @@ ScfResult rhf_scf(const Cint& mol, const Cint& aux, int nocc, double conv_tol, i
     }
     hipLaunchKernelGGL(lincomb_kernel, dim3(grid_for(n2)), dim3(256), 0, s, 1.0, h.p, 1.0, J.p, n2, F.p);
     HH_HIP(hipGetLastError());
-    mm(nao, nao, nocc * naux, T.p, nao, false, T.p, nao, true, F.p, nao, -1.0, 1.0, s, ws);
-    // energy: (D . (h + F)) / 2; commutator FDS - SDF in the orthogonal basis
+    mm(nao, nao, nocc * naux, T.p, nao, false, T.p, nao, true, F.p, nao, -exx, 1.0, s, ws);
+    // energy: (D . (h + F)) / 2 over the quadratic part, plus E_xc
     hipLaunchKernelGGL(lincomb_kernel, dim3(grid_for(n2)), dim3(256), 0, s, 0.5, h.p, 0.5, F.p, n2, tmp.p);
     HH_HIP(hipGetLastError());
+    double e_xc = 0.0;
+    if (xc) {
+      // rho of D on the grid, the functional list's exc / vxc_eff, F += V_xc
+      const double* dptr = D.p;
+      xc->ni->setup_rho(xc->den, 1, &dptr, rho.p);
+      HH_HIP(hipMemsetAsync(exc.p, 0, sizeof(double) * ng, s));
+      HH_HIP(hipMemsetAsync(vxc.p, 0, sizeof(double) * ng * nvl, s));
+      for (int f = 0; f < xc->nfunc; ++f)
+        xc_eval_eff(xc->ids[f], 1, nvl, rho.p, ng, 0, ng, xc->scales[f], true, exc.p, vxc.p, nullptr, s);
+      e_xc = xc->ni->xc_energy(exc.p, rho.p);
+      xc->ni->vxc_matrix(xc->den, vxc.p, V.p);
+      hipLaunchKernelGGL(lincomb_kernel, dim3(grid_for(n2)), dim3(256), 0, s, 1.0, F.p, 1.0, V.p, n2, F.p);
+      HH_HIP(hipGetLastError());
+    }
+    // commutator FDS - SDF in the orthogonal basis
     mm(nao, nao, nao, F.p, nao, false, D.p, nao, false, W1.p, nao, 1.0, 0.0, s, ws);   // F D
     mm(nao, nao, nao, W1.p, nao, false, S.p, nao, false, W2.p, nao, 1.0, 0.0, s, ws);  // F D S
     // (F D S - S D F) X with S D F = (F D S)^T
@@ -255,7 +284,7 @@ ScfResult rhf_scf(const Cint& mol, const Cint& aux, int nocc, double conv_tol, i
       e_elec += hpart[2 * b];
       err = std::max(err, hpart[2 * b + 1]);
     }
-    const double energy = e_elec + res.e_nuc;
+    const double energy = e_elec + e_xc + res.e_nuc;
     HH_REQUIRE(std::isfinite(energy), "scf: non-finite energy");
     res.cycles = cycle;
     res.energy = energy;

@@ -18,12 +18,25 @@ struct ScfResult {
   bool converged = false;
 };
 
-// Closed shell, nocc doubly occupied orbitals; converged when |dE| < conv_tol and the
-// commutator error < sqrt(conv_tol) (DIIS over up to 8 Fock matrices, core-Hamiltonian guess).
-// mo_coeff [nao, nao] and mo_energy [nao] on the device (orbital energies ascending).
-// Throws Error(HH_ERR_INVALID) for a linearly dependent basis (overlap eigenvalue below 1e-10
-// of the largest) and Error(HH_ERR_NOT_CONVERGED) if max_cycle is reached.
+class Numint;
+// Kohn-Sham terms of a restricted SCF: the functional list (LibXC ids with scales, evaluated by
+// xc_eval_eff on the grid of `ni`, whose AO values carry first derivatives for GGA / meta-GGA) and
+// the fraction of exact exchange exx that scales K (1 for HF).
+struct ScfXc {
+  Numint* ni = nullptr;
+  int den = 0;  // HH_DEN_RHO / SIGMA / TAU: the strictest family of the list
+  int nfunc = 0;
+  const int* ids = nullptr;
+  const double* scales = nullptr;
+  double exx = 1.0;
+};
+
+// Closed shell, nocc doubly occupied orbitals (RHF, or RKS with `xc`); converged when
+// |dE| < conv_tol and the commutator error < sqrt(conv_tol) (DIIS over up to 8 Fock matrices,
+// core-Hamiltonian guess).  mo_coeff [nao, nao] and mo_energy [nao] on the device (orbital
+// energies ascending).  Throws Error(HH_ERR_INVALID) for a linearly dependent basis (overlap
+// eigenvalue below 1e-10 of the largest) and Error(HH_ERR_NOT_CONVERGED) if max_cycle is reached.
 ScfResult rhf_scf(const Cint& mol, const Cint& aux, int nocc, double conv_tol, int max_cycle, double* mo_coeff,
-                  double* mo_energy, hipStream_t s);
+                  double* mo_energy, hipStream_t s, const ScfXc* xc = nullptr);
 
 }  // namespace hh

@@ -1,14 +1,18 @@
-"""The whole RHF analytic Hessian on the device (reference: src/hessian/rscf.rs RHessSCF with
-its components: HessNucRepl, nuc_repl.rs; RHessHcore, hcore.rs; RHessOvlp, ovlp.rs; the RI-JK
-electron interaction RHessRIJK, hess_r.rs; the CPHF part) and the density-fitted RHF that
-supplies its orbitals (the reference reads them from PySCF, tests/hessian_rhf/mod.rs:39-41).
+"""The whole restricted analytic Hessian on the device (reference: src/hessian/rscf.rs RHessSCF
+with its components: HessNucRepl, nuc_repl.rs; RHessHcore, hcore.rs; RHessOvlp, ovlp.rs; the
+RI-JK electron interaction RHessRIJK, hess_r.rs; for Kohn-Sham the grid component
+RHessKSNIMatmul with the Becke grid shift, hess_rks_becke.rs; the CPHF part) and the
+density-fitted RHF / RKS that supplies its orbitals (the reference reads them from PySCF,
+tests/hessian_rhf/mod.rs:39-41, tests/hessian_rks_becke/mod.rs:39-55).
 
     mol = CInt.from_basis(atoms, basis); aux = CInt.from_basis(atoms, aux_basis)
-    scf = RHF(mol, aux, nocc).run()
-    hess = RHessSCF(mol, aux, scf.mo_coeff, scf.mo_occ, scf.mo_energy)
+    scf = RHF(mol, aux, nocc).run()                                   # or
+    scf = RKS(mol, aux, nocc, [(1.0, LibXCFunctional.from_identifier("HYB_GGA_XC_B3LYP"))],
+              Grids(charges, coords)).run()
+    hess = RHessSCF(mol, aux, scf.mo_coeff, scf.mo_occ, scf.mo_energy, xc=scf.xc, grids=scf.grids)
     de = hess.make_hess()             # [3, 3, natm, natm]; hess.result holds the parts
 
-Every term runs in the library (hh_* entry points); torch holds device memory and adds the five
+Every term runs in the library (hh_* entry points); torch holds device memory and adds the
 [3, 3, natm, natm] parts.
 """
 import ctypes as C
@@ -46,6 +50,8 @@ def weighted_density(mo_coeff, w):
 
 class RHF:
     """Density-fitted restricted Hartree-Fock on the device (hh_scf_rhf)."""
+    xc = None
+    grids = None
 
     def __init__(self, mol, aux, nocc, conv_tol=1e-12, max_cycle=100, device=None):
         self.mol, self.aux, self.nocc = mol, aux, int(nocc)
@@ -62,6 +68,46 @@ class RHF:
         en, cy = C.c_double(), C.c_int32()
         check(lib().hh_scf_rhf(self.mol._h, self.aux._h, self.nocc, self.conv_tol, self.max_cycle, c.data_ptr(),
                                e.data_ptr(), C.byref(en), C.byref(cy), stream_of(self.device)))
+        self.mo_coeff, self.mo_energy = c, e
+        self.mo_occ = np.where(np.arange(nao) < self.nocc, 2.0, 0.0)
+        self.e_tot, self.cycles = en.value, cy.value
+        return self
+
+
+def xc_list_info(xc):
+    """(den_type, exx, ids, scales) of a functional list [(scale, LibXCFunctional)]: the strictest
+    family (determine_den_type_from_list) and sum_i scale_i hyb_exx_coef_i."""
+    from .xc import determine_den_type_from_list
+    den = determine_den_type_from_list([f for _, f in xc])
+    exx = sum(float(sc) * f.hyb_exx_coef() for sc, f in xc)
+    return den, exx, [int(f.xc_id) for _, f in xc], [float(sc) for sc, _ in xc]
+
+
+class RKS(RHF):
+    """Density-fitted restricted Kohn-Sham on the device (hh_scf_rks): the functional list xc
+    [(scale, LibXCFunctional)] on the partitioned grid of `grids` (hesship.grids.Grids), exact
+    exchange scaled by the list's hybrid fraction."""
+
+    def __init__(self, mol, aux, nocc, xc, grids, conv_tol=1e-12, max_cycle=100, device=None):
+        super().__init__(mol, aux, nocc, conv_tol, max_cycle, device)
+        self.xc, self.grids = list(xc), grids
+
+    def run(self):
+        from .numint import DEN, NIMatmul
+        if self.grids.weights is None:
+            self.grids.build()
+        den, exx, ids, scales = xc_list_info(self.xc)
+        ao = self.mol.eval_gto(self.grids.coords, 0 if den == "rho" else 1)
+        ni = NIMatmul(ao, self.grids.weights)
+        nao = self.mol.nao
+        c = empty_f((nao, nao), self.device)
+        e = torch.empty(nao, dtype=torch.float64, device=self.device)
+        en, cy = C.c_double(), C.c_int32()
+        idv = (C.c_int32 * len(ids))(*ids)
+        scv = (C.c_double * len(scales))(*scales)
+        check(lib().hh_scf_rks(self.mol._h, self.aux._h, ni._h, DEN[den], len(ids), idv, scv, exx, self.nocc,
+                               self.conv_tol, self.max_cycle, c.data_ptr(), e.data_ptr(), C.byref(en), C.byref(cy),
+                               stream_of(self.device)))
         self.mo_coeff, self.mo_energy = c, e
         self.mo_occ = np.where(np.arange(nao) < self.nocc, 2.0, 0.0)
         self.e_tot, self.cycles = en.value, cy.value
@@ -111,20 +157,22 @@ def ovlp_deriv1(mol, device=None):
     return out
 
 
-def rijk_for(mol, aux, device=None):
+def rijk_for(mol, aux, device=None, scale_k=1.0):
     """RHessRIJK::new_without_cderi (hess_r.rs:1751-1773): the metric of (mol, aux) with the
-    Cholesky policy and threshold 1e-14, scale_j = scale_k = 1 (RHF)."""
+    Cholesky policy and threshold 1e-14, scale_j = 1, scale_k = 1 (RHF) or the hybrid fraction."""
     dev = _dev(device)
     j3c = mol.int3c2e_s2ij(aux, device=dev)
     j2c = aux.int2c2e("plain", device=dev)
-    return RHessRIJK.from_j3c(j3c, j2c, 1.0, 1.0, threshold=1e-14, policy="cd")
+    return RHessRIJK.from_j3c(j3c, j2c, 1.0, float(scale_k), threshold=1e-14, policy="cd")
 
 
 class RHessSCF:
-    """RHessSCF (rscf.rs:21-506) for RHF with the RI-JK electron interaction: nuclear repulsion,
-    core Hamiltonian, overlap, RI-JK skeleton and the CPHF part, all on the device."""
+    """RHessSCF (rscf.rs:21-506) for RHF or RKS with the RI-JK electron interaction: nuclear
+    repulsion, core Hamiltonian, overlap, RI-JK skeleton (K scaled by the hybrid fraction), for
+    RKS the grid component with the Becke grid shift (de_xc_skeleton, vmat_deriv1 and the CPKS
+    response, hess_rks_becke.rs), and the CPHF part, all on the device."""
 
-    def __init__(self, mol, aux, mo_coeff, mo_occ, mo_energy, config=None, rijk=None):
+    def __init__(self, mol, aux, mo_coeff, mo_occ, mo_energy, config=None, rijk=None, xc=None, grids=None):
         self.mol, self.aux = mol, aux
         self.mo_coeff = fortran(mo_coeff)
         self.device = self.mo_coeff.device
@@ -132,7 +180,11 @@ class RHessSCF:
         self.mo_energy = np.asarray(mo_energy.cpu() if isinstance(mo_energy, torch.Tensor) else mo_energy,
                                     dtype=np.float64)
         self.config = dict(config or {})
-        self.rijk = rijk if rijk is not None else rijk_for(mol, aux, self.device)
+        self.xc, self.grids = (list(xc) if xc else None), grids
+        if self.xc and grids is None:
+            raise ValueError("a Kohn-Sham Hessian needs the SCF's grids")
+        exx = xc_list_info(self.xc)[1] if self.xc else 1.0
+        self.rijk = rijk if rijk is not None else rijk_for(mol, aux, self.device, scale_k=exx)
         self.result = {}
         self.timing = {}  # seconds per stage of the last make_hess (every stage ends synchronised)
 
@@ -152,7 +204,25 @@ class RHessSCF:
         return self.rijk.make_skeleton(self.mo_coeff, self.mo_occ, int3c, v1, v2, v3, ranges,
                                        mol.aoslice_by_atom(), aux.aoslice_by_atom())
 
-    def compute_f1mo_s1mo(self, f1bra_el):
+    def _skeleton_xc(self, dm0):
+        """make_skeleton_hess and get_deriv1_ao of the Becke grid component (hess_rks_becke.rs:
+        1106-1266, 1403-1425): (de_xc_skeleton, vmat_deriv1 [nao, nao, 3, natm], the NIMatmul and
+        the transformed kernel kept for the CPKS response)."""
+        from .numint import NIMatmul
+        from .xc import eval_xc_eff_list
+        g = self.grids
+        if g.weights is None:
+            g.build()
+        den = xc_list_info(self.xc)[0]
+        ao = self.mol.eval_gto(g.coords, 3)
+        ni = NIMatmul(ao, g.weights)
+        rho = ni.setup_rho(den, dm0)
+        exc, vxc, fxc = eval_xc_eff_list(self.xc, rho)
+        out = ni.make_hessian_setup_becke(den, dm0, self.mol.aoslice_by_atom(), exc, vxc, fxc, g.becke, g.coords,
+                                          g.quadrature_weights, g.atm_quad_split, g.hardness)
+        return out["de_xc_skeleton"], out["vmat_deriv1_grid"], ni, den, fxc
+
+    def compute_f1mo_s1mo(self, f1bra_el, vmat_deriv1=None):
         """compute_dimless_cphf_rhs's f1mo and s1mo (rscf.rs:125-164): C^T (f1bra_el + h1ao C_o)
         and C^T s1ao C_o, [nmo, nocc, 3, natm]."""
         c = self.mo_coeff
@@ -163,9 +233,13 @@ class RHessSCF:
         s1 = ovlp_deriv1(self.mol, self.device)
         f1bra = fortran(f1bra_el).clone()
         s1bra = empty_f((nao, nocc, 3, self.natm), self.device)
-        # (h1ao + ...) C_o per (x, A): batched over the 3 natm matrices
+        # (h1ao + vmat_deriv1) C_o per (x, A): batched over the 3 natm matrices
         _gemm(nao, nocc, nao, h1, nao, False, c, nao, True, f1bra, nao, 1.0, 1.0, nb,
               (nao * nao, 0, 0, 0, nao * nocc, 0), self.device)
+        if vmat_deriv1 is not None:
+            v1 = fortran(vmat_deriv1)
+            _gemm(nao, nocc, nao, v1, nao, False, c, nao, True, f1bra, nao, 1.0, 1.0, nb,
+                  (nao * nao, 0, 0, 0, nao * nocc, 0), self.device)
         _gemm(nao, nocc, nao, s1, nao, False, c, nao, True, s1bra, nao, 1.0, 0.0, nb,
               (nao * nao, 0, 0, 0, nao * nocc, 0), self.device)
         f1mo = empty_f((nmo, nocc, 3, self.natm), self.device)
@@ -200,16 +274,23 @@ class RHessSCF:
         lap("rijk_skeleton")
         r["de_ovlp"] = ovlp_hess(self.mol, dme0)
         lap("ovlp")
-        f1mo, s1mo = self.compute_f1mo_s1mo(f1bra_el)
+        vmat1 = ni = None
+        if self.xc:
+            r["de_skeleton_xc"], vmat1, ni, den, fxc = self._skeleton_xc(dm0)
+            lap("xc_skeleton")
+        f1mo, s1mo = self.compute_f1mo_s1mo(f1bra_el, vmat1)
         lap("f1mo_s1mo")
         self.rijk.make_response_preparation(self.mo_coeff, occ)
         cfg = {k: v for k, v in self.config.items() if k.startswith("cphf") or k == "level_shift"}
         drv = RHessCPHF(self.mo_coeff, occ, en, cfg)
         drv.add_el(self.rijk)
+        if ni is not None:
+            drv.add_numint(ni, den, fxc)
         r["de_cphf"] = drv.make_cphf_hess(f1mo, s1mo, precision=precision, stats=stats)
         lap("cphf")
         tot = None
-        for k in ("de_skeleton_nuc", "de_skeleton_hcore", "de_skeleton_rijk", "de_ovlp", "de_cphf"):
-            tot = r[k].clone() if tot is None else tot.add_(r[k])
+        for k in ("de_skeleton_nuc", "de_skeleton_hcore", "de_skeleton_rijk", "de_skeleton_xc", "de_ovlp", "de_cphf"):
+            if k in r:
+                tot = r[k].clone() if tot is None else tot.add_(r[k])
         r["de_tot"] = tot
         return tot

@@ -113,6 +113,13 @@ class NIMatmul:
     def _spins(dm0):
         return list(dm0) if isinstance(dm0, (list, tuple)) else [dm0]
 
+    def vxc_matrix(self, den_type, vxc_eff):
+        """The exchange-correlation potential matrix [nao, nao] = dE_xc / dD from vxc_eff [ng, nvar]."""
+        v = fortran(vxc_eff)
+        out = empty_f((self.nao, self.nao), self.device)
+        check(lib().hh_numint_vxc_matrix(self._h, DEN[den_type], v.data_ptr(), out.data_ptr()))
+        return out
+
     def setup_rho(self, den_type, dm0):
         """rho [ng, nvar] (RKS, dm0 a matrix) or [ng, nvar, 2] (UKS, dm0 a pair)."""
         d = [fortran(x) for x in self._spins(dm0)]
