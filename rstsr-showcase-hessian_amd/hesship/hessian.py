@@ -260,6 +260,7 @@ class RHessSCF:
 
         def lap(name):
             nonlocal t0
+            torch.cuda.synchronize(self.device)
             t1 = time.perf_counter()
             tm[name] = t1 - t0
             t0 = t1
@@ -280,6 +281,10 @@ class RHessSCF:
             lap("xc_skeleton")
         f1mo, s1mo = self.compute_f1mo_s1mo(f1bra_el, vmat1)
         lap("f1mo_s1mo")
+        # the RI-JK response as the resident (nao nocc)^2 matrix, built once for the CPHF solve
+        # (the per-call algebra beyond the memory budget)
+        self.rijk.set_response_path(self.config.get("rijk_response", "resident"),
+                                    int(self.config.get("rijk_resident_budget", 0)))
         self.rijk.make_response_preparation(self.mo_coeff, occ)
         cfg = {k: v for k, v in self.config.items() if k.startswith("cphf") or k == "level_shift"}
         drv = RHessCPHF(self.mo_coeff, occ, en, cfg)
