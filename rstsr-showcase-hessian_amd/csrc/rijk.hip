@@ -447,8 +447,13 @@ void Rijk::ensure_packed(Prec prec) {
 void Rijk::build_k(int s, const double* bra, int64_t nprop, Prec prec, double alpha, double beta, double* out) {
   const int64_t nocc = nocc_[s];
   const int64_t nP = naux_ * nocc;  // (P, i) columns
+  if (prec == PREC_BF16X3 && !tgen_supported(nocc) && !(k_path != 2 && kfused_supported(nao_, nocc))) {
+    // neither BF16x3 K algorithm takes this many occupied orbitals (the materialised T tile holds
+    // 2 nocc <= 256 rows): the K product runs in FP32 (J stays BF16x3)
+    build_k(s, bra, nprop, PREC_F32, alpha, beta, out);
+    return;
+  }
   const bool f64 = (prec == PREC_F64), bf = (prec == PREC_BF16X3);
-  HH_REQUIRE(!bf || tgen_supported(nocc), "BF16x3 RI-K needs nocc <= 128 (use FP32 or FP64)");
   ensure_packed(prec);
   if (!f64) ensure_mixed();
   const size_t esz = f64 ? sizeof(double) : sizeof(float);

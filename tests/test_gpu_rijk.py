@@ -221,3 +221,17 @@ def test_cderi_eig_on_spd_metric_equals_cholesky(dev):
     # no threshold: a metric that is not positive definite is an error (the reference panics)
     with pytest.raises(HesshipError):
         RHessRIJK.from_j3c(T(orc.pack_triu(j3c), dev), T(-j2c, dev), 1.0, 1.0, threshold=None)
+
+
+def test_bf16x3_response_beyond_128_occupied(dev):
+    """nocc above both BF16x3 K algorithms' limits (fused: 80, materialised T tile: 2 nocc <= 256):
+    the K product runs in FP32, J in BF16x3; the response stays within the mixed tolerance."""
+    from hesship import RHessRIJK
+    nao, naux, nocc, nprop = 160, 200, 130, 2
+    rng, cderi, C, occ = system(nao, naux, nocc, seed=9)
+    bra = rng.standard_normal((nao, nocc, nprop))
+    obj = RHessRIJK(T(cderi, dev), 1.0, 1.0)
+    obj.make_response_preparation(T(C, dev), occ)
+    got = obj.get_response_bra(T(bra, dev), precision="bf16x3").cpu().numpy()
+    ref = orc.rhf_response_bra(cderi, C, occ, bra, 1.0, 1.0)
+    assert rel(got, ref) < 1e-4
