@@ -407,6 +407,14 @@ hh_status hh_scf_rks(const hh_cint* mol, const hh_cint* aux, hh_numint* grid, in
                      const int32_t* xc_ids, const double* xc_scales, double exx, int32_t nocc, double conv_tol,
                      int32_t max_cycle, double* mo_coeff, double* mo_energy, double* energy, int32_t* cycles,
                      void* stream);
+/* Unrestricted HF (grid NULL: xc arguments ignored) or KS: nalpha / nbeta singly occupied
+ * orbitals; mo_coeff[2] [nao, nao] and mo_energy[2] [nao] device arrays per spin channel (the
+ * mo_coeff / mo_occ / mo_energy pairs of UHessSCF, uscf.rs:32-77); otherwise as hh_scf_rks,
+ * with the functional evaluated spin-polarised. */
+hh_status hh_scf_uhf(const hh_cint* mol, const hh_cint* aux, hh_numint* grid, int32_t den_type, int32_t nfunc,
+                     const int32_t* xc_ids, const double* xc_scales, double exx, int32_t nalpha, int32_t nbeta,
+                     double conv_tol, int32_t max_cycle, double* const* mo_coeff, double* const* mo_energy,
+                     double* energy, int32_t* cycles, void* stream);
 /* The exchange-correlation potential matrix out [nao, nao] = dE_xc / dD (the Kohn-Sham Fock
  * term; no reference counterpart on the Hessian path, which takes converged orbitals) from
  * vxc_eff [ng, nvar] of hh_xc_eval_eff (weights not applied).  Synchronous. */

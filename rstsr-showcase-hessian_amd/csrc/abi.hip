@@ -478,6 +478,31 @@ hh_status hh_scf_rks(const hh_cint* mol, const hh_cint* aux, hh_numint* grid, in
   HH_ABI_END
 }
 
+hh_status hh_scf_uhf(const hh_cint* mol, const hh_cint* aux, hh_numint* grid, int32_t den_type, int32_t nfunc,
+                     const int32_t* xc_ids, const double* xc_scales, double exx, int32_t nalpha, int32_t nbeta,
+                     double conv_tol, int32_t max_cycle, double* const* mo_coeff, double* const* mo_energy,
+                     double* energy, int32_t* cycles, void* stream) {
+  HH_ABI_BEGIN
+  HH_REQUIRE(mol && aux && mo_coeff && mo_energy, "null argument");
+  ScfXc xc;
+  std::vector<int> ids;
+  if (grid) {
+    HH_REQUIRE(den_type >= HH_DEN_RHO && den_type <= HH_DEN_TAU, "den_type must be RHO, SIGMA or TAU");
+    ids.assign(xc_ids, xc_ids + std::max(0, nfunc));
+    xc.ni = grid->impl.get();
+    xc.den = den_type;
+    xc.nfunc = nfunc;
+    xc.ids = ids.data();
+    xc.scales = xc_scales;
+    xc.exx = exx;
+  }
+  const ScfResult r = uhf_scf(*mol->impl, *aux->impl, nalpha, nbeta, conv_tol, max_cycle, mo_coeff, mo_energy,
+                              (hipStream_t)stream, grid ? &xc : nullptr);
+  if (energy) *energy = r.energy;
+  if (cycles) *cycles = r.cycles;
+  HH_ABI_END
+}
+
 hh_status hh_numint_vxc_matrix(hh_numint* h, int32_t den_type, const double* vxc_eff, double* out) {
   HH_ABI_BEGIN
   HH_REQUIRE(h && vxc_eff && out, "null argument");

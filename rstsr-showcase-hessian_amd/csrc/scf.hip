@@ -147,12 +147,21 @@ void mm(int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, bool ta, 
 
 }  // namespace
 
-ScfResult rhf_scf(const Cint& mol, const Cint& aux, int nocc, double conv_tol, int max_cycle, double* mo_coeff,
-                  double* mo_energy, hipStream_t s, const ScfXc* xc) {
+namespace {
+
+// The SCF of one (restricted) or two (unrestricted) spin channels.  Channel s has nocc[s]
+// occupied orbitals with occupation f = 2 / nspin: D_s = f C_s,o C_s,o^T, F_s = h + J[D_a + D_b]
+// - exx T_s T_s^T (T_s,P = B_P C_s,o, so T_s T_s^T = K[D_s] / f) + V_xc,s, and
+// E = sum_s (D_s . (h + F_s^quad)) / 2 + E_xc + E_nuc with F^quad the Fock matrix without V_xc.
+ScfResult scf_run(const Cint& mol, const Cint& aux, int nspin, const int* nocc, double conv_tol, int max_cycle,
+                  double* const* mo_coeff, double* const* mo_energy, hipStream_t s, const ScfXc* xc) {
   const int64_t nao = mol.nao(), naux = aux.nao(), n2 = nao * nao;
-  HH_REQUIRE(nocc > 0 && nocc <= nao, "scf: nocc out of range");
+  HH_REQUIRE(nspin == 1 || nspin == 2, "scf: nspin must be 1 or 2");
+  for (int sp = 0; sp < nspin; ++sp)
+    HH_REQUIRE(nocc[sp] >= 0 && nocc[sp] <= nao && (nspin == 2 || nocc[sp] > 0), "scf: nocc out of range");
   HH_REQUIRE(conv_tol > 0.0 && max_cycle > 0, "scf: conv_tol > 0, max_cycle > 0");
-  HH_REQUIRE(mo_coeff && mo_energy, "scf: null output");
+  for (int sp = 0; sp < nspin; ++sp) HH_REQUIRE(mo_coeff[sp] && mo_energy[sp], "scf: null output");
+  const double focc = nspin == 1 ? 2.0 : 1.0;
   ScfResult res;
   for (int a = 0; a < mol.natm(); ++a)
     for (int b = 0; b < a; ++b) {
@@ -162,12 +171,18 @@ ScfResult rhf_scf(const Cint& mol, const Cint& aux, int nocc, double conv_tol, i
                             (ra[2] - rb[2]) * (ra[2] - rb[2]));
       res.e_nuc += mol.atom_charge(a) * mol.atom_charge(b) / d;
     }
+  const int nomax = std::max(nocc[0], nspin == 2 ? nocc[1] : 0);
   DevBuf ws, lws;
-  Mat S, h, X, sv, tmp, F, Fo, C, D, J, T, v, W1, W2, Fs, Es, G, H, rhs, part;
-  S.alloc(n2); h.alloc(n2); X.alloc(n2); sv.alloc(nao); tmp.alloc(n2); F.alloc(n2); Fo.alloc(n2); C.alloc(n2);
-  D.alloc(n2); J.alloc(n2); T.alloc(nao * nocc * naux); v.alloc(naux); W1.alloc(n2); W2.alloc(n2);
-  Fs.alloc(NDIIS * n2); Es.alloc(NDIIS * n2); G.alloc(NDIIS * NDIIS); H.alloc((NDIIS + 1) * (NDIIS + 1));
-  rhs.alloc(NDIIS + 1);
+  // per-channel matrices are stored channel after channel: F[s * n2], ...
+  Mat S, h, X, sv, tmp, F, Fo, C, D, Dt, J, T, v, W1, W2, Fs, Es, G, H, rhs, part;
+  const int64_t nsn2 = nspin * n2;
+  S.alloc(n2); h.alloc(n2); X.alloc(n2); sv.alloc(nao); tmp.alloc(nsn2); F.alloc(nsn2); Fo.alloc(n2); C.alloc(nsn2);
+  D.alloc(nsn2); Dt.alloc(n2); J.alloc(n2); T.alloc(nao * std::max(nomax, 1) * naux); v.alloc(naux); W1.alloc(n2);
+  W2.alloc(n2); Fs.alloc(NDIIS * nsn2); Es.alloc(NDIIS * nsn2); G.alloc(NDIIS * NDIIS);
+  H.alloc((NDIIS + 1) * (NDIIS + 1)); rhs.alloc(NDIIS + 1);
+  constexpr int NPART = 256;
+  part.alloc(2 * NPART);
+  std::vector<double> hpart(2 * NPART);
   const double exx = xc ? xc->exx : 1.0;
   Mat rho, exc, vxc, V;
   int64_t ng = 0;
@@ -178,11 +193,8 @@ ScfResult rhf_scf(const Cint& mol, const Cint& aux, int nocc, double conv_tol, i
     HH_REQUIRE(xc->ni->stream() == s, "scf: the grid handle runs on another stream");
     ng = xc->ni->ng();
     nvl = xc->den == HH_DEN_RHO ? 1 : (xc->den == HH_DEN_SIGMA ? 4 : 5);
-    rho.alloc(ng * nvl); exc.alloc(ng); vxc.alloc(ng * nvl); V.alloc(n2);
+    rho.alloc(ng * nvl * nspin); exc.alloc(ng); vxc.alloc(ng * nvl * nspin); V.alloc(n2);
   }
-  constexpr int NPART = 256;
-  part.alloc(2 * NPART);
-  std::vector<double> hpart(2 * NPART);
   // one-electron matrices
   mol.int1e(HH_INT1E_OVLP, HH_INT_PLAIN, S.p, s);
   mol.int1e(HH_INT1E_KIN, HH_INT_PLAIN, h.p, s);
@@ -211,80 +223,107 @@ ScfResult rhf_scf(const Cint& mol, const Cint& aux, int nocc, double conv_tol, i
     rijk = Rijk::from_j3c(j3c.p, j2c.p, nao, naux, HH_J2C_CD, 1e-14, 1.0, 1.0, 1, s);
   }
   const double* B = rijk->bfull();
-  // core-Hamiltonian guess
-  HH_HIP(hipMemcpyAsync(F.p, h.p, sizeof(double) * n2, hipMemcpyDeviceToDevice, s));
+  // diagonalise F_s (orthogonal basis) into C_s, mo_energy[s]
+  auto diagonalise = [&](int sp) {
+    double* Fp = F.p + n2 * sp;
+    mm(nao, nao, nao, X.p, nao, true, Fp, nao, false, tmp.p, nao, 1.0, 0.0, s, ws);   // X^T F
+    mm(nao, nao, nao, tmp.p, nao, false, X.p, nao, false, Fo.p, nao, 1.0, 0.0, s, ws);  // X^T F X
+    eigh(Fo.p, nao, mo_energy[sp], s, lws);
+    mm(nao, nao, nao, X.p, nao, false, Fo.p, nao, false, C.p + n2 * sp, nao, 1.0, 0.0, s, ws);  // C = X C'
+  };
+  // core-Hamiltonian guess for every channel
+  for (int sp = 0; sp < nspin; ++sp)
+    HH_HIP(hipMemcpyAsync(F.p + n2 * sp, h.p, sizeof(double) * n2, hipMemcpyDeviceToDevice, s));
   double e_prev = 0.0;
   int ndiis = 0, slot = 0;
   const unsigned npb = (unsigned)std::min<int64_t>(NPART, ceil_div(n2, 256));
   for (int cycle = 1; cycle <= max_cycle; ++cycle) {
-    // diagonalise F in the orthogonal basis
-    mm(nao, nao, nao, X.p, nao, true, F.p, nao, false, tmp.p, nao, 1.0, 0.0, s, ws);  // X^T F
-    mm(nao, nao, nao, tmp.p, nao, false, X.p, nao, false, Fo.p, nao, 1.0, 0.0, s, ws);  // X^T F X
-    eigh(Fo.p, nao, mo_energy, s, lws);
-    mm(nao, nao, nao, X.p, nao, false, Fo.p, nao, false, C.p, nao, 1.0, 0.0, s, ws);  // C = X C'
-    mm(nao, nao, nocc, C.p, nao, false, C.p, nao, true, D.p, nao, 2.0, 0.0, s, ws);  // D = 2 C_o C_o^T
-    // J = B (B^T d)
-    mm(naux, 1, n2, B, n2, true, D.p, n2, false, v.p, naux, 1.0, 0.0, s, ws);
-    mm(n2, 1, naux, B, n2, false, v.p, naux, false, J.p, n2, 1.0, 0.0, s, ws);
-    // T_P = B_P C_o (batched over P), K = 2 T T^T; F = h + J - T T^T
-    {
-      GemmArgs g;
-      g.M = nao;
-      g.N = nocc;
-      g.K = nao;
-      g.A = B;
-      g.lda = nao;
-      g.a_kmajor = false;
-      g.B = C.p;
-      g.ldb = nao;
-      g.b_kmajor = true;
-      g.C = T.p;
-      g.ldc = nao;
-      g.nb0 = naux;
-      g.sA0 = n2;
-      g.sB0 = 0;
-      g.sC0 = nao * nocc;
-      gemm<double, double>(g, PREC_F64, s, ws);
+    for (int sp = 0; sp < nspin; ++sp) {
+      diagonalise(sp);
+      double* Dp = D.p + n2 * sp;
+      if (nocc[sp] > 0)
+        mm(nao, nao, nocc[sp], C.p + n2 * sp, nao, false, C.p + n2 * sp, nao, true, Dp, nao, focc, 0.0, s, ws);
+      else
+        HH_HIP(hipMemsetAsync(Dp, 0, sizeof(double) * n2, s));
     }
-    hipLaunchKernelGGL(lincomb_kernel, dim3(grid_for(n2)), dim3(256), 0, s, 1.0, h.p, 1.0, J.p, n2, F.p);
+    // total density and J = B (B^T d)
+    if (nspin == 2)
+      hipLaunchKernelGGL(lincomb_kernel, dim3(grid_for(n2)), dim3(256), 0, s, 1.0, D.p, 1.0, D.p + n2, n2, Dt.p);
+    else
+      HH_HIP(hipMemcpyAsync(Dt.p, D.p, sizeof(double) * n2, hipMemcpyDeviceToDevice, s));
     HH_HIP(hipGetLastError());
-    mm(nao, nao, nocc * naux, T.p, nao, false, T.p, nao, true, F.p, nao, -exx, 1.0, s, ws);
-    // energy: (D . (h + F)) / 2 over the quadratic part, plus E_xc
-    hipLaunchKernelGGL(lincomb_kernel, dim3(grid_for(n2)), dim3(256), 0, s, 0.5, h.p, 0.5, F.p, n2, tmp.p);
-    HH_HIP(hipGetLastError());
+    mm(naux, 1, n2, B, n2, true, Dt.p, n2, false, v.p, naux, 1.0, 0.0, s, ws);
+    mm(n2, 1, naux, B, n2, false, v.p, naux, false, J.p, n2, 1.0, 0.0, s, ws);
+    double e_quad = 0.0, err = 0.0;
+    for (int sp = 0; sp < nspin; ++sp) {
+      double* Fp = F.p + n2 * sp;
+      hipLaunchKernelGGL(lincomb_kernel, dim3(grid_for(n2)), dim3(256), 0, s, 1.0, h.p, 1.0, J.p, n2, Fp);
+      HH_HIP(hipGetLastError());
+      if (nocc[sp] > 0) {
+        // T_P = B_P C_s,o (batched over P); F_s -= exx T T^T
+        GemmArgs g;
+        g.M = nao;
+        g.N = nocc[sp];
+        g.K = nao;
+        g.A = B;
+        g.lda = nao;
+        g.a_kmajor = false;
+        g.B = C.p + n2 * sp;
+        g.ldb = nao;
+        g.b_kmajor = true;
+        g.C = T.p;
+        g.ldc = nao;
+        g.nb0 = naux;
+        g.sA0 = n2;
+        g.sB0 = 0;
+        g.sC0 = nao * nocc[sp];
+        gemm<double, double>(g, PREC_F64, s, ws);
+        mm(nao, nao, (int64_t)nocc[sp] * naux, T.p, nao, false, T.p, nao, true, Fp, nao, -exx, 1.0, s, ws);
+      }
+      // (D_s . (h + F_s)) / 2 of the quadratic part
+      hipLaunchKernelGGL(lincomb_kernel, dim3(grid_for(n2)), dim3(256), 0, s, 0.5, h.p, 0.5, Fp, n2, tmp.p);
+      HH_HIP(hipGetLastError());
+      hipLaunchKernelGGL(dot_max_kernel, dim3(npb), dim3(256), 0, s, D.p + n2 * sp, tmp.p, tmp.p, n2, part.p);
+      HH_HIP(hipGetLastError());
+      HH_HIP(hipMemcpyAsync(hpart.data(), part.p, sizeof(double) * 2 * npb, hipMemcpyDeviceToHost, s));
+      HH_HIP(hipStreamSynchronize(s));
+      for (unsigned b = 0; b < npb; ++b) e_quad += hpart[2 * b];
+    }
     double e_xc = 0.0;
     if (xc) {
-      // rho of D on the grid, the functional list's exc / vxc_eff, F += V_xc
-      const double* dptr = D.p;
-      xc->ni->setup_rho(xc->den, 1, &dptr, rho.p);
+      // rho of the channel densities on the grid, the functional list's exc / vxc_eff, F_s += V_xc,s
+      const double* dptr[2] = {D.p, D.p + n2};
+      xc->ni->setup_rho(xc->den, nspin, dptr, rho.p);
       HH_HIP(hipMemsetAsync(exc.p, 0, sizeof(double) * ng, s));
-      HH_HIP(hipMemsetAsync(vxc.p, 0, sizeof(double) * ng * nvl, s));
+      HH_HIP(hipMemsetAsync(vxc.p, 0, sizeof(double) * ng * nvl * nspin, s));
       for (int f = 0; f < xc->nfunc; ++f)
-        xc_eval_eff(xc->ids[f], 1, nvl, rho.p, ng, 0, ng, xc->scales[f], true, exc.p, vxc.p, nullptr, s);
-      e_xc = xc->ni->xc_energy(exc.p, rho.p);
-      xc->ni->vxc_matrix(xc->den, vxc.p, V.p);
-      hipLaunchKernelGGL(lincomb_kernel, dim3(grid_for(n2)), dim3(256), 0, s, 1.0, F.p, 1.0, V.p, n2, F.p);
-      HH_HIP(hipGetLastError());
+        xc_eval_eff(xc->ids[f], nspin, nvl, rho.p, ng, ng * nvl, ng, xc->scales[f], true, exc.p, vxc.p, nullptr, s);
+      for (int sp = 0; sp < nspin; ++sp) {
+        e_xc += xc->ni->xc_energy(exc.p, rho.p + ng * nvl * sp);
+        xc->ni->vxc_matrix(xc->den, vxc.p + ng * nvl * sp, V.p);
+        hipLaunchKernelGGL(lincomb_kernel, dim3(grid_for(n2)), dim3(256), 0, s, 1.0, F.p + n2 * sp, 1.0, V.p, n2,
+                           F.p + n2 * sp);
+        HH_HIP(hipGetLastError());
+      }
     }
-    // commutator FDS - SDF in the orthogonal basis
-    mm(nao, nao, nao, F.p, nao, false, D.p, nao, false, W1.p, nao, 1.0, 0.0, s, ws);   // F D
-    mm(nao, nao, nao, W1.p, nao, false, S.p, nao, false, W2.p, nao, 1.0, 0.0, s, ws);  // F D S
-    // (F D S - S D F) X with S D F = (F D S)^T
-    mm(nao, nao, nao, W2.p, nao, true, X.p, nao, false, W1.p, nao, -1.0, 0.0, s, ws);  // -(FDS)^T X
-    mm(nao, nao, nao, W2.p, nao, false, X.p, nao, false, W1.p, nao, 1.0, 1.0, s, ws);  // (FDS - SDF) X
-    double* e_slot = Es.p + n2 * slot;
-    mm(nao, nao, nao, X.p, nao, true, W1.p, nao, false, e_slot, nao, 1.0, 0.0, s, ws);  // X^T (..) X
-    HH_HIP(hipMemcpyAsync(Fs.p + n2 * slot, F.p, sizeof(double) * n2, hipMemcpyDeviceToDevice, s));
-    hipLaunchKernelGGL(dot_max_kernel, dim3(npb), dim3(256), 0, s, D.p, tmp.p, e_slot, n2, part.p);
+    // commutators X^T (F_s D_s S - S D_s F_s) X into the DIIS slot (channels stacked)
+    double* e_slot = Es.p + nsn2 * slot;
+    for (int sp = 0; sp < nspin; ++sp) {
+      mm(nao, nao, nao, F.p + n2 * sp, nao, false, D.p + n2 * sp, nao, false, W1.p, nao, 1.0, 0.0, s, ws);  // F D
+      mm(nao, nao, nao, W1.p, nao, false, S.p, nao, false, W2.p, nao, 1.0, 0.0, s, ws);  // F D S
+      mm(nao, nao, nao, W2.p, nao, true, X.p, nao, false, W1.p, nao, -1.0, 0.0, s, ws);  // -(FDS)^T X
+      mm(nao, nao, nao, W2.p, nao, false, X.p, nao, false, W1.p, nao, 1.0, 1.0, s, ws);  // (FDS - SDF) X
+      mm(nao, nao, nao, X.p, nao, true, W1.p, nao, false, e_slot + n2 * sp, nao, 1.0, 0.0, s, ws);
+    }
+    HH_HIP(hipMemcpyAsync(Fs.p + nsn2 * slot, F.p, sizeof(double) * nsn2, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(dot_max_kernel, dim3((unsigned)std::min<int64_t>(NPART, ceil_div(nsn2, 256))), dim3(256), 0, s,
+                       e_slot, e_slot, e_slot, nsn2, part.p);
     HH_HIP(hipGetLastError());
-    HH_HIP(hipMemcpyAsync(hpart.data(), part.p, sizeof(double) * 2 * npb, hipMemcpyDeviceToHost, s));
+    const unsigned npe = (unsigned)std::min<int64_t>(NPART, ceil_div(nsn2, 256));
+    HH_HIP(hipMemcpyAsync(hpart.data(), part.p, sizeof(double) * 2 * npe, hipMemcpyDeviceToHost, s));
     HH_HIP(hipStreamSynchronize(s));
-    double e_elec = 0.0, err = 0.0;
-    for (unsigned b = 0; b < npb; ++b) {
-      e_elec += hpart[2 * b];
-      err = std::max(err, hpart[2 * b + 1]);
-    }
-    const double energy = e_elec + e_xc + res.e_nuc;
+    for (unsigned b = 0; b < npe; ++b) err = std::max(err, hpart[2 * b + 1]);
+    const double energy = e_quad + e_xc + res.e_nuc;
     HH_REQUIRE(std::isfinite(energy), "scf: non-finite energy");
     res.cycles = cycle;
     res.energy = energy;
@@ -295,32 +334,46 @@ ScfResult rhf_scf(const Cint& mol, const Cint& aux, int nocc, double conv_tol, i
       break;
     }
     e_prev = energy;
-    // DIIS extrapolation of F over the stored slots
+    // DIIS extrapolation of the stacked Fock matrices over the stored slots
     ndiis = std::min(ndiis + 1, NDIIS);
     slot = (slot + 1) % NDIIS;
     if (ndiis >= 2) {
-      hipLaunchKernelGGL(diis_gram_kernel, dim3((unsigned)(ndiis * ndiis)), dim3(256), 0, s, Es.p, n2, ndiis, G.p);
+      hipLaunchKernelGGL(diis_gram_kernel, dim3((unsigned)(ndiis * ndiis)), dim3(256), 0, s, Es.p, nsn2, ndiis, G.p);
       hipLaunchKernelGGL(diis_system_kernel, dim3(1), dim3(128), 0, s, G.p, ndiis, H.p, rhs.p);
       HH_HIP(hipGetLastError());
       lu_solve(H.p, ndiis + 1, rhs.p, 1, s, lws);
-      hipLaunchKernelGGL(diis_combine_kernel, dim3(grid_for(n2)), dim3(256), 0, s, Fs.p, rhs.p, ndiis, n2, F.p);
+      hipLaunchKernelGGL(diis_combine_kernel, dim3(grid_for(nsn2)), dim3(256), 0, s, Fs.p, rhs.p, ndiis, nsn2, F.p);
       HH_HIP(hipGetLastError());
     }
   }
-  if (res.converged) {
-    // canonical orbitals of the last Fock matrix (built from the converged density, not extrapolated)
-    mm(nao, nao, nao, X.p, nao, true, F.p, nao, false, tmp.p, nao, 1.0, 0.0, s, ws);
-    mm(nao, nao, nao, tmp.p, nao, false, X.p, nao, false, Fo.p, nao, 1.0, 0.0, s, ws);
-    eigh(Fo.p, nao, mo_energy, s, lws);
-    mm(nao, nao, nao, X.p, nao, false, Fo.p, nao, false, C.p, nao, 1.0, 0.0, s, ws);
-  }
-  HH_HIP(hipMemcpyAsync(mo_coeff, C.p, sizeof(double) * n2, hipMemcpyDeviceToDevice, s));
+  // canonical orbitals of the last Fock matrices (built from the converged densities, not
+  // extrapolated); on failure the last cycle's orbitals
+  if (res.converged)
+    for (int sp = 0; sp < nspin; ++sp) diagonalise(sp);
+  for (int sp = 0; sp < nspin; ++sp)
+    HH_HIP(hipMemcpyAsync(mo_coeff[sp], C.p + n2 * sp, sizeof(double) * n2, hipMemcpyDeviceToDevice, s));
   HH_HIP(hipStreamSynchronize(s));
   if (!res.converged)
     throw Error(HH_ERR_NOT_CONVERGED, "scf: not converged in " + std::to_string(max_cycle) + " cycles (dE " +
                                           std::to_string(res.last_delta) + ", error " +
                                           std::to_string(res.last_error) + ")");
   return res;
+}
+
+}  // namespace
+
+ScfResult rhf_scf(const Cint& mol, const Cint& aux, int nocc, double conv_tol, int max_cycle, double* mo_coeff,
+                  double* mo_energy, hipStream_t s, const ScfXc* xc) {
+  int no[1] = {nocc};
+  double* c[1] = {mo_coeff};
+  double* e[1] = {mo_energy};
+  return scf_run(mol, aux, 1, no, conv_tol, max_cycle, c, e, s, xc);
+}
+
+ScfResult uhf_scf(const Cint& mol, const Cint& aux, int nalpha, int nbeta, double conv_tol, int max_cycle,
+                  double* const* mo_coeff, double* const* mo_energy, hipStream_t s, const ScfXc* xc) {
+  int no[2] = {nalpha, nbeta};
+  return scf_run(mol, aux, 2, no, conv_tol, max_cycle, mo_coeff, mo_energy, s, xc);
 }
 
 }  // namespace hh

@@ -38,5 +38,10 @@ struct ScfXc {
 // eigenvalue below 1e-10 of the largest) and Error(HH_ERR_NOT_CONVERGED) if max_cycle is reached.
 ScfResult rhf_scf(const Cint& mol, const Cint& aux, int nocc, double conv_tol, int max_cycle, double* mo_coeff,
                   double* mo_energy, hipStream_t s, const ScfXc* xc = nullptr);
+// Unrestricted (UHF, or UKS with `xc`): nalpha / nbeta singly occupied orbitals per channel,
+// mo_coeff[s] [nao, nao] and mo_energy[s] [nao] per channel on the device (the reference's
+// UHessSCF inputs, uscf.rs:32-77); DIIS over the stacked channels.
+ScfResult uhf_scf(const Cint& mol, const Cint& aux, int nalpha, int nbeta, double conv_tol, int max_cycle,
+                  double* const* mo_coeff, double* const* mo_energy, hipStream_t s, const ScfXc* xc = nullptr);
 
 }  // namespace hh

@@ -19,4 +19,4 @@ from .xc import (LibXCFunctional, determine_den_type, determine_den_type_from_li
                  eval_xc_eff_list, libxc_eval_eff, libxc_eval_inner)
 from .rijk import RHessRIJK, UHessRIJK, get_rijk_response_bra_separated, pack_triu_tilde, unpack_triu  # noqa: F401
 from .grids import Grids  # noqa: F401
-from .hessian import RHF, RKS, RHessSCF  # noqa: F401
+from .hessian import RHF, RKS, UHF, UKS, RHessSCF, UHessSCF  # noqa: F401

@@ -90,6 +90,7 @@ SIGNATURES.update({
     "hh_scf_rhf": (_i32, [_p, _p, _i32, _d, _i32, _p, _p, _p, _p, _p]),
     "hh_scf_rks": (_i32, [_p, _p, _p, _i32, _i32, _p, _p, _d, _i32, _d, _i32, _p, _p, _p, _p, _p]),
     "hh_numint_vxc_matrix": (_i32, [_p, _i32, _p, _p]),
+    "hh_scf_uhf": (_i32, [_p, _p, _p, _i32, _i32, _p, _p, _d, _i32, _i32, _d, _i32, _p, _p, _p, _p, _p]),
     "hh_hess_hcore": (_i32, [_p, _p, _p, _p]),
     "hh_hess_ovlp": (_i32, [_p, _p, _p, _p]),
     "hh_hess_hcore_deriv1": (_i32, [_p, _p, _p]),

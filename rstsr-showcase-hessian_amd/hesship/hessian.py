@@ -22,8 +22,8 @@ import numpy as np
 import torch
 
 from ._lib import check, lib
-from .cphf import RHessCPHF
-from .rijk import RHessRIJK
+from .cphf import RHessCPHF, UHessCPHF
+from .rijk import RHessRIJK, UHessRIJK
 from .tensors import empty_f, fortran, stream_of
 
 
@@ -112,6 +112,63 @@ class RKS(RHF):
         self.mo_occ = np.where(np.arange(nao) < self.nocc, 2.0, 0.0)
         self.e_tot, self.cycles = en.value, cy.value
         return self
+
+
+class UHF:
+    """Density-fitted unrestricted Hartree-Fock on the device (hh_scf_uhf): nalpha / nbeta
+    singly occupied orbitals; mo_coeff, mo_energy, mo_occ as (alpha, beta) pairs."""
+    xc = None
+    grids = None
+
+    def __init__(self, mol, aux, nalpha, nbeta, conv_tol=1e-12, max_cycle=100, device=None):
+        self.mol, self.aux = mol, aux
+        self.nocc = (int(nalpha), int(nbeta))
+        self.conv_tol, self.max_cycle = float(conv_tol), int(max_cycle)
+        self.device = _dev(device)
+        self.mo_coeff = self.mo_energy = self.mo_occ = None
+        self.e_tot = None
+        self.cycles = 0
+
+    def _run(self, grid_h=None, den=None, ids=(), scales=(), exx=1.0):
+        from .numint import DEN
+        from ._lib import ptr_array
+        nao = self.mol.nao
+        cs = [empty_f((nao, nao), self.device) for _ in range(2)]
+        es = [torch.empty(nao, dtype=torch.float64, device=self.device) for _ in range(2)]
+        en, cy = C.c_double(), C.c_int32()
+        idv = (C.c_int32 * max(1, len(ids)))(*ids)
+        scv = (C.c_double * max(1, len(scales)))(*scales)
+        check(lib().hh_scf_uhf(self.mol._h, self.aux._h, grid_h, DEN[den] if den else 0, len(ids), idv, scv, exx,
+                               self.nocc[0], self.nocc[1], self.conv_tol, self.max_cycle,
+                               ptr_array([c.data_ptr() for c in cs]), ptr_array([e.data_ptr() for e in es]),
+                               C.byref(en), C.byref(cy), stream_of(self.device)))
+        self.mo_coeff, self.mo_energy = tuple(cs), tuple(es)
+        self.mo_occ = tuple(np.where(np.arange(nao) < n, 1.0, 0.0) for n in self.nocc)
+        self.e_tot, self.cycles = en.value, cy.value
+        return self
+
+    def run(self):
+        return self._run()
+
+
+class UKS(UHF):
+    """Density-fitted unrestricted Kohn-Sham on the device (hh_scf_uhf with a grid): xc
+    [(scale, LibXCFunctional)] with polarised functionals, on the grid of `grids`."""
+
+    def __init__(self, mol, aux, nalpha, nbeta, xc, grids, conv_tol=1e-12, max_cycle=100, device=None):
+        super().__init__(mol, aux, nalpha, nbeta, conv_tol, max_cycle, device)
+        if any(f.nspin != 2 for _, f in xc):
+            raise ValueError("UKS needs polarized functionals")
+        self.xc, self.grids = list(xc), grids
+
+    def run(self):
+        from .numint import NIMatmul
+        if self.grids.weights is None:
+            self.grids.build()
+        den, exx, ids, scales = xc_list_info(self.xc)
+        ao = self.mol.eval_gto(self.grids.coords, 0 if den == "rho" else 1)
+        ni = NIMatmul(ao, self.grids.weights)
+        return self._run(ni._h, den, ids, scales, exx)
 
 
 def nuc_repl_hess(mol, device=None):
@@ -219,7 +276,7 @@ class RHessSCF:
         rho = ni.setup_rho(den, dm0)
         exc, vxc, fxc = eval_xc_eff_list(self.xc, rho)
         out = ni.make_hessian_setup_becke(den, dm0, self.mol.aoslice_by_atom(), exc, vxc, fxc, g.becke, g.coords,
-                                          g.quadrature_weights, g.atm_quad_split, g.hardness)
+                                          g.quadrature_weights, g.atm_quad_split, g.hardness, grid_fixed_keys=False)
         return out["de_xc_skeleton"], out["vmat_deriv1_grid"], ni, den, fxc
 
     def compute_f1mo_s1mo(self, f1bra_el, vmat_deriv1=None):
@@ -288,6 +345,139 @@ class RHessSCF:
         self.rijk.make_response_preparation(self.mo_coeff, occ)
         cfg = {k: v for k, v in self.config.items() if k.startswith("cphf") or k == "level_shift"}
         drv = RHessCPHF(self.mo_coeff, occ, en, cfg)
+        drv.add_el(self.rijk)
+        if ni is not None:
+            drv.add_numint(ni, den, fxc)
+        r["de_cphf"] = drv.make_cphf_hess(f1mo, s1mo, precision=precision, stats=stats)
+        lap("cphf")
+        tot = None
+        for k in ("de_skeleton_nuc", "de_skeleton_hcore", "de_skeleton_rijk", "de_skeleton_xc", "de_ovlp", "de_cphf"):
+            if k in r:
+                tot = r[k].clone() if tot is None else tot.add_(r[k])
+        r["de_tot"] = tot
+        return tot
+
+
+def _mo_transform(mol, c, nocc, f1bra, h1=None, vmat=None, s1=None, device=None):
+    """C^T (f1bra + (h1 + vmat) C_o) and C^T s1 C_o per (x, A), [nmo, nocc, 3, natm] (f1bra is
+    updated in place with the C_o products)."""
+    nao, nmo = c.shape
+    natm = len(mol.atm)
+    nb = 3 * natm
+    st = (nao * nao, 0, 0, 0, nao * nocc, 0)
+    for m in (h1, vmat):
+        if m is not None:
+            _gemm(nao, nocc, nao, fortran(m), nao, False, c, nao, True, f1bra, nao, 1.0, 1.0, nb, st, device)
+    s1bra = empty_f((nao, nocc, 3, natm), device)
+    _gemm(nao, nocc, nao, s1, nao, False, c, nao, True, s1bra, nao, 1.0, 0.0, nb, st, device)
+    f1mo = empty_f((nmo, nocc, 3, natm), device)
+    s1mo = empty_f((nmo, nocc, 3, natm), device)
+    stt = (0, 0, nao * nocc, 0, nmo * nocc, 0)
+    _gemm(nmo, nocc, nao, c, nao, True, f1bra, nao, True, f1mo, nmo, 1.0, 0.0, nb, stt, device)
+    _gemm(nmo, nocc, nao, c, nao, True, s1bra, nao, True, s1mo, nmo, 1.0, 0.0, nb, stt, device)
+    return f1mo, s1mo
+
+
+class UHessSCF:
+    """UHessSCF (uscf.rs:32-540) for UHF or UKS with the RI-JK electron interaction: nuclear
+    repulsion, core Hamiltonian and overlap on the summed (energy-weighted) densities
+    (UHessHcore, UHessOvlp: ovlp.rs:128-131), the UHF RI-JK skeleton (hess_u.rs:107-261), for UKS
+    the grid component with the Becke grid shift (hess_uks_becke.rs), and the unrestricted CPHF
+    part; every orbital argument an (alpha, beta) pair."""
+
+    def __init__(self, mol, aux, mo_coeff, mo_occ, mo_energy, config=None, rijk=None, xc=None, grids=None):
+        self.mol, self.aux = mol, aux
+        self.mo_coeff = tuple(fortran(c) for c in mo_coeff)
+        self.device = self.mo_coeff[0].device
+        host = lambda x: np.asarray(x.cpu() if isinstance(x, torch.Tensor) else x, dtype=np.float64)
+        self.mo_occ = tuple(host(o) for o in mo_occ)
+        self.mo_energy = tuple(host(e) for e in mo_energy)
+        self.config = dict(config or {})
+        self.xc, self.grids = (list(xc) if xc else None), grids
+        if self.xc and grids is None:
+            raise ValueError("a Kohn-Sham Hessian needs the SCF's grids")
+        exx = xc_list_info(self.xc)[1] if self.xc else 1.0
+        if rijk is None:
+            j3c = mol.int3c2e_s2ij(aux, device=self.device)
+            j2c = aux.int2c2e("plain", device=self.device)
+            rijk = UHessRIJK.from_j3c(j3c, j2c, 1.0, exx, threshold=1e-14, policy="cd")
+        self.rijk = rijk
+        self.result = {}
+        self.timing = {}
+
+    @property
+    def natm(self):
+        return len(self.mol.atm)
+
+    def _total(self, weights):
+        """sum over both channels of sum_i w_i C_i C_i^T (one product over the joined columns)."""
+        c = torch.cat(self.mo_coeff, dim=1)
+        return weighted_density(c, np.concatenate(weights))
+
+    def make_hess(self, precision="fp64", stats=None):
+        """uscf.rs:514-540: skeleton (nuclear, core, RI-JK, grid) + overlap + CPHF."""
+        from .numint import NIMatmul
+        from .xc import eval_xc_eff_list
+        mol, aux, dev = self.mol, self.aux, self.device
+        occ, en = self.mo_occ, self.mo_energy
+        r, tm = self.result, self.timing
+        tm.clear()
+        t0 = time.perf_counter()
+
+        def lap(name):
+            nonlocal t0
+            torch.cuda.synchronize(dev)
+            t1 = time.perf_counter()
+            tm[name] = t1 - t0
+            t0 = t1
+        dm0 = [weighted_density(c, o) for c, o in zip(self.mo_coeff, occ)]
+        r["de_skeleton_nuc"] = nuc_repl_hess(mol, dev)
+        lap("nuc")
+        r["de_skeleton_hcore"] = hcore_hess(mol, self._total(occ))
+        lap("hcore")
+
+        def int3c(kind, p0, p1):
+            sh0 = int(np.nonzero(aux.ao_loc == p0)[0][0])
+            sh1 = int(np.nonzero(aux.ao_loc == p1)[0][0])
+            return mol.int3c2e(aux, kind, shls=(sh0, sh1), device=dev)
+        v1, v2, v3 = (aux.int2c2e(k, device=dev) for k in ("ip1", "ipip1", "ip1ip2"))
+        de_jk, f1bra = self.rijk.make_skeleton(self.mo_coeff, occ, int3c, v1, v2, v3,
+                                               [(int(aux.ao_loc[0]), int(aux.ao_loc[-1]))], mol.aoslice_by_atom(),
+                                               aux.aoslice_by_atom())
+        r["de_skeleton_rijk"] = de_jk
+        lap("rijk_skeleton")
+        r["de_ovlp"] = ovlp_hess(mol, self._total([o * e for o, e in zip(occ, en)]))
+        lap("ovlp")
+        vmat = (None, None)
+        ni = None
+        if self.xc:
+            g = self.grids
+            if g.weights is None:
+                g.build()
+            den = xc_list_info(self.xc)[0]
+            ni = NIMatmul(mol.eval_gto(g.coords, 3), g.weights)
+            rho = ni.setup_rho(den, tuple(dm0))
+            exc, vxc, fxc = eval_xc_eff_list(self.xc, rho)
+            out = ni.make_hessian_setup_becke(den, tuple(dm0), mol.aoslice_by_atom(), exc, vxc, fxc, g.becke,
+                                              g.coords, g.quadrature_weights, g.atm_quad_split, g.hardness,
+                                              grid_fixed_keys=False)
+            r["de_skeleton_xc"] = out["de_xc_skeleton"]
+            vmat = (out["vmat_deriv1_grid_a"], out["vmat_deriv1_grid_b"])
+            lap("xc_skeleton")
+        h1 = hcore_deriv1(mol, dev)
+        s1 = ovlp_deriv1(mol, dev)
+        f1mo, s1mo = [], []
+        for sp in range(2):
+            no = int((occ[sp] > 0).sum())
+            f, s_ = _mo_transform(mol, self.mo_coeff[sp], no, fortran(f1bra[sp]).clone(), h1, vmat[sp], s1, dev)
+            f1mo.append(f)
+            s1mo.append(s_)
+        lap("f1mo_s1mo")
+        self.rijk.set_response_path(self.config.get("rijk_response", "resident"),
+                                    int(self.config.get("rijk_resident_budget", 0)))
+        self.rijk.make_response_preparation(self.mo_coeff, occ)
+        cfg = {k: v for k, v in self.config.items() if k.startswith("cphf") or k == "level_shift"}
+        drv = UHessCPHF(self.mo_coeff, occ, en, cfg)
         drv.add_el(self.rijk)
         if ni is not None:
             drv.add_numint(ni, den, fxc)
