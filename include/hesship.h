@@ -760,7 +760,8 @@ hh_status hh_util_gemm_nt_choice(int64_t M, int64_t N, int64_t K, int32_t* confi
 /* The same product from FP64 operands in split-BF16 x3 (the RI-K hot product's arithmetic: A and
  * B split into bf16 hi / lo, three MFMA products, FP32 accumulation, split-K partials summed in
  * FP64; no reference counterpart).  config 0 / 1: 208 x 256 tiles as 1 x 8 or 2 x 4 waves; 2: 400 x 128
- * tiles (2 x 4 waves, swizzled LDS rows); -1: the
+ * tiles (2 x 4 waves, swizzled LDS rows); 3: 128 x 128 tiles (2 x 2 waves, swizzled LDS rows, two
+ * workgroups per CU); -1: the
  * autotuner (its choice for the shape class: hh_util_gemm_nt_bf16x3_choice).  lda, ldb >= K. */
 hh_status hh_util_gemm_nt_bf16x3(int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
                                  const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int32_t config,

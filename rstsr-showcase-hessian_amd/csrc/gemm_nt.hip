@@ -643,13 +643,19 @@ void launch_f64_cfg(int cfg, const NtArgs& a, hipStream_t s, DevBuf& ws) {
 // not kept: 400 x 128 on 2 x 2 waves of 13 (12) x 4 blocks (one wave per SIMD, accumulators in AGPRs,
 // half the LDS fragment reads per MFMA): 10.8 ms against 8.6 ms for configuration 2 on the RI-K
 // hot shape (M 400, N 3600, K 998400); 416 x 128 on 32 x 32 x 16 MFMA blocks (the instruction's
-// measured ceiling is 1.55x that of 16 x 16 x 32): 9.5 ms against 8.55 ms
-constexpr int NT_BF_NCFG = 3;
+// measured ceiling is 1.55x that of 16 x 16 x 32): 9.5 ms against 8.55 ms.  3 = 128 x 128 on
+// 2 x 2 waves of 4 x 4 blocks, swizzled rows (64 KB of LDS: two workgroups per CU, so one
+// workgroup's loads and epilogue overlap the other's products on short-K shapes).  Measured and
+// not kept for the RI-K W = bra^T X product (M 3600, N 96000, K 416; configuration 1 at 1.87 ms):
+// the W operand staged through LDS and written as 16-byte runs (2.05 ms), and a grouped tile
+// order that keeps six row tiles' panels in L2 (no change).
+constexpr int NT_BF_NCFG = 4;
 
 // BF16X3 operands are always K-block tiled (every producer writes that layout)
 void launch_bf_cfg(int cfg, const NtArgs& a, hipStream_t s, DevBuf& ws) {
   HH_REQUIRE(a.tiled || a.K == 0, "gemm_nt: BF16X3 operands must be K-block tiled");
-  if (cfg == 2) launch_nt<2, 4, 13, 2, true, true, 25, 2, true>(a, s, ws);
+  if (cfg == 3) launch_nt<2, 2, 4, 4, true, true, 8, 2, true>(a, s, ws);
+  else if (cfg == 2) launch_nt<2, 4, 13, 2, true, true, 25, 2, true>(a, s, ws);
   else if (cfg == 1) launch_nt<2, 4, 7, 4, true, true, 13, 1>(a, s, ws);
   else launch_nt<1, 8, 13, 2, true, true>(a, s, ws);
 }
