@@ -1,7 +1,8 @@
 // Density-fitted restricted Hartree-Fock on the device (see scf.hpp).
 //
 // Per cycle: F' = X^T F X diagonalised by the Jacobi eigensolver (X = U s^-1/2 from the overlap),
-// C = X C', D = 2 C_o C_o^T; Coulomb v_P = sum_{mu nu} B[mu, nu, P] D[mu, nu], J = B v (two FP64
+// warm-started in the previous cycle's eigenvectors V (V^T F' V is nearly diagonal, so the
+// quadratically convergent sweeps stop after a few), C = X V, D = 2 C_o C_o^T; Coulomb v_P = sum_{mu nu} B[mu, nu, P] D[mu, nu], J = B v (two FP64
 // matrix-vector products over the unpacked cderi B [nao, nao, naux]); exchange T_P = B_P C_o
 // (batched over P) and K = 2 T T^T with T as [nao, nocc naux] (one FP64 matrix-core product);
 // F = h + J - K / 2; E = (D . (h + F)) / 2 + E_nuc.  DIIS (Pulay) on the orthogonal-basis
@@ -174,7 +175,7 @@ ScfResult scf_run(const Cint& mol, const Cint& aux, int nspin, const int* nocc, 
   const int nomax = std::max(nocc[0], nspin == 2 ? nocc[1] : 0);
   DevBuf ws, lws;
   // per-channel matrices are stored channel after channel: F[s * n2], ...
-  Mat S, h, X, sv, tmp, F, Fo, C, D, Dt, J, T, v, W1, W2, Fs, Es, G, H, rhs, part;
+  Mat S, h, X, sv, tmp, F, Fo, C, D, Dt, J, T, v, W1, W2, Fs, Es, G, H, rhs, part, Vev;
   const int64_t nsn2 = nspin * n2;
   S.alloc(n2); h.alloc(n2); X.alloc(n2); sv.alloc(nao); tmp.alloc(nsn2); F.alloc(nsn2); Fo.alloc(n2); C.alloc(nsn2);
   D.alloc(nsn2); Dt.alloc(n2); J.alloc(n2); T.alloc(nao * std::max(nomax, 1) * naux); v.alloc(naux); W1.alloc(n2);
@@ -223,13 +224,28 @@ ScfResult scf_run(const Cint& mol, const Cint& aux, int nspin, const int* nocc, 
     rijk = Rijk::from_j3c(j3c.p, j2c.p, nao, naux, HH_J2C_CD, 1e-14, 1.0, 1.0, 1, s);
   }
   const double* B = rijk->bfull();
-  // diagonalise F_s (orthogonal basis) into C_s, mo_energy[s]
+  // diagonalise F_s (orthogonal basis) into C_s, mo_energy[s]: F' = X^T F X in the basis of the
+  // channel's last eigenvectors V_s, V_s <- V_s W with W the eigenvectors of V_s^T F' V_s
+  Vev.alloc(nsn2);
+  bool have_v[2] = {false, false};
   auto diagonalise = [&](int sp) {
     double* Fp = F.p + n2 * sp;
+    double* Vs = Vev.p + n2 * sp;
     mm(nao, nao, nao, X.p, nao, true, Fp, nao, false, tmp.p, nao, 1.0, 0.0, s, ws);   // X^T F
     mm(nao, nao, nao, tmp.p, nao, false, X.p, nao, false, Fo.p, nao, 1.0, 0.0, s, ws);  // X^T F X
+    if (have_v[sp]) {
+      mm(nao, nao, nao, Vs, nao, true, Fo.p, nao, false, tmp.p, nao, 1.0, 0.0, s, ws);   // V^T F'
+      mm(nao, nao, nao, tmp.p, nao, false, Vs, nao, false, Fo.p, nao, 1.0, 0.0, s, ws);  // V^T F' V
+    }
     eigh(Fo.p, nao, mo_energy[sp], s, lws);
-    mm(nao, nao, nao, X.p, nao, false, Fo.p, nao, false, C.p + n2 * sp, nao, 1.0, 0.0, s, ws);  // C = X C'
+    if (have_v[sp]) {
+      mm(nao, nao, nao, Vs, nao, false, Fo.p, nao, false, tmp.p, nao, 1.0, 0.0, s, ws);  // V W
+      HH_HIP(hipMemcpyAsync(Vs, tmp.p, sizeof(double) * n2, hipMemcpyDeviceToDevice, s));
+    } else {
+      HH_HIP(hipMemcpyAsync(Vs, Fo.p, sizeof(double) * n2, hipMemcpyDeviceToDevice, s));
+      have_v[sp] = true;
+    }
+    mm(nao, nao, nao, X.p, nao, false, Vs, nao, false, C.p + n2 * sp, nao, 1.0, 0.0, s, ws);  // C = X V
   };
   // core-Hamiltonian guess for every channel
   for (int sp = 0; sp < nspin; ++sp)
