@@ -1068,7 +1068,9 @@ def workload_hessian(args):
     device: the density-fitted SCF (RHF, or RKS with --xc on an --nrad x --nang Becke grid), then
     make_hess (nuclear repulsion + core + RI-JK skeleton + overlap (+ the grid component with the
     Becke grid shift) + CPHF) with the CPHF in FP64 and in the mixed BF16x3 mode with FP64
-    refinement; harmonic frequencies of both Hessians.  One step = one make_hess."""
+    refinement; harmonic frequencies of both Hessians.  With --open-shell the cation doublet
+    (H2O)_n^+ (5n alpha, 5n - 1 beta electrons) through UHF / UKS and UHessSCF.  One step = one
+    make_hess."""
     import numpy as np
     import torch
     import hesship
@@ -1081,26 +1083,31 @@ def workload_hessian(args):
     mol = hesship.CInt.from_basis(atoms, SVP_LIKE)
     aux = hesship.CInt.from_basis(atoms, JK_LIKE)
     nocc = 5 * args.nwater
-    log(f"[bench] hessian: (H2O)_{args.nwater} nao {mol.nao} naux {aux.nao} natm {len(atoms)} xc {args.xc}")
+    op = bool(getattr(args, "open_shell", False))
+    log(f"[bench] hessian: (H2O)_{args.nwater}{'+' if op else ''} nao {mol.nao} naux {aux.nao} natm {len(atoms)} "
+        f"xc {args.xc}")
     xc_names = {"none": None, "svwn": [(1.0, "LDA_X"), (1.0, "LDA_C_VWN")], "b3lyp": [(1.0, "HYB_GGA_XC_B3LYP")],
                 "tpss0": [(1.0, "HYB_MGGA_XC_TPSS0")]}[args.xc]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    occ = (nocc, nocc - 1) if op else (nocc,)
     if xc_names is None:
-        scf = hesship.RHF(mol, aux, nocc, conv_tol=1e-11, max_cycle=200).run()
+        scf = (hesship.UHF if op else hesship.RHF)(mol, aux, *occ, conv_tol=1e-11, max_cycle=200).run()
         ngrid = 0
     else:
         from hesship.grids import Grids
-        xc = [(sc, hesship.LibXCFunctional.from_identifier(n)) for sc, n in xc_names]
+        xc = [(sc, hesship.LibXCFunctional.from_identifier(n, "polarized" if op else "unpolarized"))
+              for sc, n in xc_names]
         grids = Grids(z, R, nrad=args.nrad, nang=args.nang, device=dev)
-        scf = hesship.RKS(mol, aux, nocc, xc, grids, conv_tol=1e-11, max_cycle=200).run()
+        scf = (hesship.UKS if op else hesship.RKS)(mol, aux, *occ, xc, grids, conv_tol=1e-11, max_cycle=200).run()
         ngrid = int(grids.coords.shape[0])
     scf_s = time.perf_counter() - t0
     log(f"[bench] hessian: SCF {scf_s:.2f} s, {scf.cycles} cycles, E = {scf.e_tot:.8f}")
     out = {}
     for prec in ("fp64", "bf16x3"):
-        drv = hesship.RHessSCF(mol, aux, scf.mo_coeff, scf.mo_occ, scf.mo_energy, xc=scf.xc, grids=scf.grids,
-                               config=dict(cphf_tol=1e-9 if prec == "bf16x3" else 1e-8))
+        drv = (hesship.UHessSCF if op else hesship.RHessSCF)(
+            mol, aux, scf.mo_coeff, scf.mo_occ, scf.mo_energy, xc=scf.xc, grids=scf.grids,
+            config=dict(cphf_tol=1e-9 if prec == "bf16x3" else 1e-8))
         st = {}
         walls = []
         for it in range(args.warmup + args.steps):
@@ -1124,11 +1131,14 @@ def workload_hessian(args):
     f64, fmx = freqs(out["fp64"]["de"]), freqs(out["bf16x3"]["de"])
     dH = float(np.abs(out["fp64"]["de"] - out["bf16x3"]["de"]).max())
     wall = out["bf16x3"]["wall_s"]
-    method = "RHF" if xc_names is None else f"RKS {args.xc.upper()} ({args.nrad} x {args.nang} Becke grid)"
+    u = "U" if op else "R"
+    method = f"{u}HF" if xc_names is None else f"{u}KS {args.xc.upper()} ({args.nrad} x {args.nang} Becke grid)"
     line = _line(f"CPHF+Hessian wall s (whole {method.split()[0]} Hessian, mixed-precision CPHF), 1 MI355X", wall, "s",
                  wall * 1e3, args.steps, args.warmup, "bf16x3 CPHF response with FP64 refinement; FP64 elsewhere",
-                 {"workload": f"seeded (H2O)_{args.nwater} {method}, def2-SVP-sized basis, even-tempered auxiliary",
-                  "nao": mol.nao, "naux": aux.nao, "natm": natm, "nocc": nocc, "nprop": 3 * natm, "ngrid": ngrid},
+                 {"workload": f"seeded (H2O)_{args.nwater}{'+ doublet' if op else ''} {method}, def2-SVP-sized basis, "
+                              f"even-tempered auxiliary",
+                  "nao": mol.nao, "naux": aux.nao, "natm": natm, "nocc": list(occ) if op else nocc, "nprop": 3 * natm,
+                  "ngrid": ngrid},
                  None, higher=False)
     line["data"] = "synthetic molecule (seeded water cluster; basis sets synthetic, none exist in this image)"
     line["scf"] = {"wall_s": round(scf_s, 3), "cycles": scf.cycles, "e_tot": scf.e_tot}
@@ -1229,6 +1239,8 @@ def main():
                     help="hessian: RHF (none) or RKS with this functional")
     ap.add_argument("--nrad", type=int, default=75, help="hessian: radial points per atom (RKS)")
     ap.add_argument("--nang", type=int, default=302, help="hessian: Lebedev points per shell (RKS)")
+    ap.add_argument("--open-shell", action="store_true",
+                    help="hessian: the cation doublet through UHF / UKS and the unrestricted Hessian")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run the library's 16-byte copy kernel over GIB GiB three times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes) and exit")
