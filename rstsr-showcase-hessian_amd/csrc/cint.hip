@@ -278,9 +278,13 @@ __device__ __forceinline__ int deriv_terms(int lx, int ly, int lz, double a, int
   return n;
 }
 
+// Index of (lx, ly, lz) in a shifted Cartesian set holding the l = lo, lo + 2, .. of one parity
+// (n derivatives move l by +-1 each, so only l0 + n, l0 + n - 2, .. >= 0 occur)
 __device__ __forceinline__ int set_index(int lo, int lx, int ly, int lz) {
   const int l = lx + ly + lz;
-  return tet(l) - tet(lo) + cart_index(l, lx, lz);
+  int k = 0;
+  for (int m = lo; m < l; m += 2) k += (m + 1) * (m + 2) / 2;
+  return k + cart_index(l, lx, lz);
 }
 
 __global__ __launch_bounds__(64) void int_triple_kernel(TripleArgs A) {
@@ -314,15 +318,15 @@ __global__ __launch_bounds__(64) void int_triple_kernel(TripleArgs A) {
   // shifted Cartesian sets (packed lx | ly << 8 | lz << 16)
   if (tid == 0) {
     int k = 0;
-    for (int l = A.loA; l <= LA; ++l)
+    for (int l = A.loA; l <= LA; l += 2)
       for (int lx = l; lx >= 0; --lx)
         for (int ly = l - lx; ly >= 0; --ly) setA[k++] = lx | (ly << 8) | ((l - lx - ly) << 16);
     k = 0;
-    for (int l = A.loB; l <= LB; ++l)
+    for (int l = A.loB; l <= LB; l += 2)
       for (int lx = l; lx >= 0; --lx)
         for (int ly = l - lx; ly >= 0; --ly) setB[k++] = lx | (ly << 8) | ((l - lx - ly) << 16);
     k = 0;
-    for (int l = A.loC; l <= LC; ++l)
+    for (int l = A.loC; l <= LC; l += 2)
       for (int lx = l; lx >= 0; --lx)
         for (int ly = l - lx; ly >= 0; --ly) setC[k++] = lx | (ly << 8) | ((l - lx - ly) << 16);
   }
@@ -611,10 +615,17 @@ void launch_triples(const Cint* ma, const Cint* mb, const Cint* mc, const std::v
         A.nda = na; A.ndb = nb; A.ndc = nc; A.kind = kind; A.cart = cart ? 1 : 0;
         A.out = out; A.s1 = s1; A.s2 = s2; A.s3 = s3; A.c0 = c0;
         A.LA = la + na; A.LB = lb + nb; A.LC = lc + nc; A.LT = A.LA + A.LB + A.LC;
-        A.loA = std::max(0, la - na); A.loB = std::max(0, lb - nb); A.loC = std::max(0, lc - nc);
-        A.nA = tet_h(A.LA + 1) - tet_h(A.loA);
-        A.nB = tet_h(A.LB + 1) - tet_h(A.loB);
-        A.nC = tet_h(A.LC + 1) - tet_h(A.loC);
+        // shifted sets: l = l0 + n, l0 + n - 2, .. >= 0 (the parity n derivatives reach)
+        auto lowest = [](int l0, int n) { return l0 >= n ? l0 - n : (l0 + n) % 2; };
+        auto set_size = [](int lo, int hi) {
+          int k = 0;
+          for (int l = lo; l <= hi; l += 2) k += (l + 1) * (l + 2) / 2;
+          return k;
+        };
+        A.loA = lowest(la, na); A.loB = lowest(lb, nb); A.loC = lowest(lc, nc);
+        A.nA = set_size(A.loA, A.LA);
+        A.nB = set_size(A.loB, A.LB);
+        A.nC = set_size(A.loC, A.LC);
         A.NR = tet_h(A.LT + 1);
         const int nEab = (A.LA + 1) * (A.LB + 1) * (A.LA + A.LB + 1);
         const int nEc = (A.LC + 1) * (A.LC + 1);
