@@ -330,7 +330,7 @@ __global__ __launch_bounds__(64) void int_triple_kernel(TripleArgs A) {
       for (int lx = l; lx >= 0; --lx)
         for (int ly = l - lx; ly >= 0; --ly) setC[k++] = lx | (ly << 8) | ((l - lx - ly) << 16);
   }
-  for (int e = tid; e < nacc; e += 64) acc[e] = 0.0;
+  for (int e = tid; e < A.nA * A.nB * A.nC; e += 64) I[e] = 0.0;
   __syncthreads();
   const double* exa = A.prima + sa.prim_off;
   const double* exb = A.two_c ? nullptr : A.primb + sb.prim_off;
@@ -431,7 +431,13 @@ __global__ __launch_bounds__(64) void int_triple_kernel(TripleArgs A) {
           }
           __syncthreads();
         }
-        // primitive integrals over the shifted sets: I[(a nB + b) nC + c]
+        // primitive integrals over the shifted sets, contracted into I[(a nB + b) nC + c] with the
+        // exponent powers of their derivative terms: an element of a set at angular momentum l
+        // arises from r = (l - l0 + n) / 2 raising steps, each carrying -2 alpha, so the sum over
+        // primitives weighs it by alpha^r and the derivative coefficients are applied once, after
+        // the primitive loops, with alpha = 1
+        const double wgt = ca * cb * cc * pref;
+        const double pwa[3] = {1.0, al, al * al}, pwb[3] = {1.0, be, be * be}, pwc[3] = {1.0, ga, ga * ga};
         const int W = LA + LB + 1;
         const int ntot = A.nA * A.nB * A.nC;
         for (int e = tid; e < ntot; e += 64) {
@@ -461,63 +467,64 @@ __global__ __launch_bounds__(64) void int_triple_kernel(TripleArgs A) {
                     }
                 s += eab * hsum;
               }
-          I[e] = pref * s;
-        }
-        __syncthreads();
-        // derivative integrals of the Cartesian components, contracted into acc
-        const double wgt = ca * cb * cc;
-        for (int e = tid; e < nacc; e += 64) {
-          const int q = A.q0 + e % nq;
-          const int rest = e / nq;
-          const int jc = rest % ncC, jb = (rest / ncC) % ncB, ja = rest / (ncC * ncB);
-          // Cartesian components of the output functions
-          int lxa = 0, lya = 0, lza = 0, lxb = 0, lyb = 0, lzb = 0, lxc = 0, lyc = 0, lzc = 0;
-          {
-            int k = 0;
-            for (int lx = A.la; lx >= 0; --lx)
-              for (int ly = A.la - lx; ly >= 0; --ly, ++k)
-                if (k == ja) { lxa = lx; lya = ly; lza = A.la - lx - ly; }
-            k = 0;
-            for (int lx = A.lb; lx >= 0; --lx)
-              for (int ly = A.lb - lx; ly >= 0; --ly, ++k)
-                if (k == jb) { lxb = lx; lyb = ly; lzb = A.lb - lx - ly; }
-            k = 0;
-            for (int lx = A.lc; lx >= 0; --lx)
-              for (int ly = A.lc - lx; ly >= 0; --ly, ++k)
-                if (k == jc) { lxc = lx; lyc = ly; lzc = A.lc - lx - ly; }
-          }
-          // derivative directions per function: comp q of 3 (one derivative) or 9 = 3 t + s
-          int da1 = 0, da2 = 0, db1 = 0, dc1 = 0, dc2 = 0;
-          switch (A.kind) {
-            case HH_INT_IP1: da1 = q; break;
-            case HH_INT_IP2: dc1 = q; break;
-            case HH_INT_IPIP1: da1 = q / 3; da2 = q % 3; break;
-            case HH_INT_IPVIP1: da1 = q / 3; db1 = q % 3; break;
-            case HH_INT_IP1IP2: da1 = q / 3; dc1 = q % 3; break;
-            case HH_INT_IPIP2: dc1 = q / 3; dc2 = q % 3; break;
-            default: break;
-          }
-          double cfa[4], cfb[4], cfc[4];
-          int sha[4][3], shb[4][3], shc[4][3];
-          const int nta = deriv_terms(lxa, lya, lza, al, A.nda, da1, da2, cfa, sha);
-          const int ntb = deriv_terms(lxb, lyb, lzb, be, A.ndb, db1, 0, cfb, shb);
-          const int ntc = deriv_terms(lxc, lyc, lzc, ga, A.ndc, dc1, dc2, cfc, shc);
-          double v = 0.0;
-          for (int x = 0; x < nta; ++x) {
-            const int iA = set_index(A.loA, sha[x][0], sha[x][1], sha[x][2]);
-            for (int y = 0; y < ntb; ++y) {
-              const int iB = set_index(A.loB, shb[y][0], shb[y][1], shb[y][2]);
-              for (int z = 0; z < ntc; ++z) {
-                const int iC = set_index(A.loC, shc[z][0], shc[z][1], shc[z][2]);
-                v += cfa[x] * cfb[y] * cfc[z] * I[(iA * A.nB + iB) * A.nC + iC];
-              }
-            }
-          }
-          acc[e] += wgt * v;
+          const int ra = (ax + ay + az - A.la + A.nda) >> 1, rb = (bx + by + bz - A.lb + A.ndb) >> 1;
+          const int rc = (cx + cy + cz - A.lc + A.ndc) >> 1;
+          I[e] += wgt * pwa[ra] * pwb[rb] * pwc[rc] * s;
         }
         __syncthreads();
       }
     }
+  // derivative integrals of the Cartesian components from the contracted shifted-set integrals
+  for (int e = tid; e < nacc; e += 64) {
+    const int q = A.q0 + e % nq;
+    const int rest = e / nq;
+    const int jc = rest % ncC, jb = (rest / ncC) % ncB, ja = rest / (ncC * ncB);
+    // Cartesian components of the output functions
+    int lxa = 0, lya = 0, lza = 0, lxb = 0, lyb = 0, lzb = 0, lxc = 0, lyc = 0, lzc = 0;
+    {
+      int k = 0;
+      for (int lx = A.la; lx >= 0; --lx)
+        for (int ly = A.la - lx; ly >= 0; --ly, ++k)
+          if (k == ja) { lxa = lx; lya = ly; lza = A.la - lx - ly; }
+      k = 0;
+      for (int lx = A.lb; lx >= 0; --lx)
+        for (int ly = A.lb - lx; ly >= 0; --ly, ++k)
+          if (k == jb) { lxb = lx; lyb = ly; lzb = A.lb - lx - ly; }
+      k = 0;
+      for (int lx = A.lc; lx >= 0; --lx)
+        for (int ly = A.lc - lx; ly >= 0; --ly, ++k)
+          if (k == jc) { lxc = lx; lyc = ly; lzc = A.lc - lx - ly; }
+    }
+    // derivative directions per function: comp q of 3 (one derivative) or 9 = 3 t + s
+    int da1 = 0, da2 = 0, db1 = 0, dc1 = 0, dc2 = 0;
+    switch (A.kind) {
+      case HH_INT_IP1: da1 = q; break;
+      case HH_INT_IP2: dc1 = q; break;
+      case HH_INT_IPIP1: da1 = q / 3; da2 = q % 3; break;
+      case HH_INT_IPVIP1: da1 = q / 3; db1 = q % 3; break;
+      case HH_INT_IP1IP2: da1 = q / 3; dc1 = q % 3; break;
+      case HH_INT_IPIP2: dc1 = q / 3; dc2 = q % 3; break;
+      default: break;
+    }
+    double cfa[4], cfb[4], cfc[4];
+    int sha[4][3], shb[4][3], shc[4][3];
+    const int nta = deriv_terms(lxa, lya, lza, 1.0, A.nda, da1, da2, cfa, sha);
+    const int ntb = deriv_terms(lxb, lyb, lzb, 1.0, A.ndb, db1, 0, cfb, shb);
+    const int ntc = deriv_terms(lxc, lyc, lzc, 1.0, A.ndc, dc1, dc2, cfc, shc);
+    double v = 0.0;
+    for (int x = 0; x < nta; ++x) {
+      const int iA = set_index(A.loA, sha[x][0], sha[x][1], sha[x][2]);
+      for (int y = 0; y < ntb; ++y) {
+        const int iB = set_index(A.loB, shb[y][0], shb[y][1], shb[y][2]);
+        for (int z = 0; z < ntc; ++z) {
+          const int iC = set_index(A.loC, shc[z][0], shc[z][1], shc[z][2]);
+          v += cfa[x] * cfb[y] * cfc[z] * I[(iA * A.nB + iB) * A.nC + iC];
+        }
+      }
+    }
+    acc[e] = v;
+  }
+  __syncthreads();
   // to spherical (or Cartesian with libcint's s / p factors) and out
   const int nsa = A.cart ? ncA : 2 * A.la + 1, nsb = A.cart ? ncB : 2 * A.lb + 1, nsc = A.cart ? ncC : 2 * A.lc + 1;
   const int nout = nsa * nsb * nsc * nq;
