@@ -229,6 +229,8 @@ struct TripleArgs {
   int na_list, nb_list, nc_list;
   int la, lb, lc;      // angular momenta of the group (lb = 0 and two_c for 2-centre)
   int two_c;
+  int sym_ab;          // (ab|c) = (ba|c) (same a / b shells, no derivative on a or b): only
+                       // la >= lb groups are launched, la == lb triples with a >= b, both written
   int nda, ndb, ndc;   // derivatives on a / b / c
   int kind;            // HH_INT_*
   int q0, q1;          // derivative components of this launch
@@ -293,6 +295,7 @@ __global__ __launch_bounds__(64) void int_triple_kernel(TripleArgs A) {
   const int ia_s = A.la_list[blockIdx.x % A.na_list];
   const int ib_s = A.two_c ? -1 : A.lb_list[blockIdx.x / A.na_list];
   const int ic_s = A.lc_list[blockIdx.y];
+  if (A.sym_ab && A.la == A.lb && ia_s < ib_s) return;  // its mirror image writes this block
   const CShell sa = A.csa[ia_s];
   CShell sb;
   if (A.two_c) {
@@ -551,8 +554,9 @@ __global__ __launch_bounds__(64) void int_triple_kernel(TripleArgs A) {
       }
     }
     const int64_t ao_b = A.two_c ? 0 : sb.ao0 + mb;
-    const int64_t idx = (sa.ao0 + ma) + A.s1 * (ao_b + A.s2 * ((sc.ao0 + mc - A.c0) + A.s3 * (A.q0 + qq)));
-    A.out[idx] = v;
+    const int64_t cq = A.s2 * ((sc.ao0 + mc - A.c0) + A.s3 * (A.q0 + qq));
+    A.out[(sa.ao0 + ma) + A.s1 * (ao_b + cq)] = v;
+    if (A.sym_ab && ia_s != ib_s) A.out[ao_b + A.s1 * ((sa.ao0 + ma) + cq)] = v;
   }
 }
 
@@ -611,14 +615,16 @@ void launch_triples(const Cint* ma, const Cint* mb, const Cint* mc, const std::v
     HH_HIP(hipFuncSetAttribute((const void*)int_triple_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
   }
   std::vector<DevBuf*> keep;
+  const bool sym = !two_c && ma == mb && alist_all == blist_all && na == 0 && nb == 0;
   for (auto& [la, al] : ga)
     for (auto& [lb, bl] : gb)
       for (auto& [lc, cl] : gc) {
+        if (sym && la < lb) continue;
         TripleArgs A{};
         A.csa = d_csa.as<CShell>(); A.prima = d_prima.as<double>();
         A.csb = two_c ? nullptr : d_csb.as<CShell>(); A.primb = two_c ? nullptr : d_primb.as<double>();
         A.csc = d_csc.as<CShell>(); A.primc = d_primc.as<double>();
-        A.la = la; A.lb = lb; A.lc = lc; A.two_c = two_c ? 1 : 0;
+        A.la = la; A.lb = lb; A.lc = lc; A.two_c = two_c ? 1 : 0; A.sym_ab = sym ? 1 : 0;
         A.nda = na; A.ndb = nb; A.ndc = nc; A.kind = kind; A.cart = cart ? 1 : 0;
         A.out = out; A.s1 = s1; A.s2 = s2; A.s3 = s3; A.c0 = c0;
         A.LA = la + na; A.LB = lb + nb; A.LC = lc + nc; A.LT = A.LA + A.LB + A.LC;
