@@ -229,8 +229,9 @@ struct TripleArgs {
   int na_list, nb_list, nc_list;
   int la, lb, lc;      // angular momenta of the group (lb = 0 and two_c for 2-centre)
   int two_c;
-  int sym_ab;          // (ab|c) = (ba|c) (same a / b shells, no derivative on a or b): only
-                       // la >= lb groups are launched, la == lb triples with a >= b, both written
+  int sym_ab;          // (ab|c) = (ba|c) (same a / b shells, no derivative on a or b, or one on
+                       // each with the components swapped): only la >= lb groups are launched,
+                       // la == lb triples with a >= b, both blocks written
   int nda, ndb, ndc;   // derivatives on a / b / c
   int kind;            // HH_INT_*
   int q0, q1;          // derivative components of this launch
@@ -556,7 +557,10 @@ __global__ __launch_bounds__(64) void int_triple_kernel(TripleArgs A) {
     const int64_t ao_b = A.two_c ? 0 : sb.ao0 + mb;
     const int64_t cq = A.s2 * ((sc.ao0 + mc - A.c0) + A.s3 * (A.q0 + qq));
     A.out[(sa.ao0 + ma) + A.s1 * (ao_b + cq)] = v;
-    if (A.sym_ab && ia_s != ib_s) A.out[ao_b + A.s1 * ((sa.ao0 + ma) + cq)] = v;
+    if (A.sym_ab && ia_s != ib_s) {
+      const int q = A.q0 + qq, qm = A.kind == HH_INT_IPVIP1 ? 3 * (q % 3) + q / 3 : q;
+      A.out[ao_b + A.s1 * ((sa.ao0 + ma) + A.s2 * ((sc.ao0 + mc - A.c0) + A.s3 * qm))] = v;
+    }
   }
 }
 
@@ -615,7 +619,7 @@ void launch_triples(const Cint* ma, const Cint* mb, const Cint* mc, const std::v
     HH_HIP(hipFuncSetAttribute((const void*)int_triple_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
   }
   std::vector<DevBuf*> keep;
-  const bool sym = !two_c && ma == mb && alist_all == blist_all && na == 0 && nb == 0;
+  const bool sym = !two_c && ma == mb && alist_all == blist_all && ((na == 0 && nb == 0) || kind == HH_INT_IPVIP1);
   for (auto& [la, al] : ga)
     for (auto& [lb, bl] : gb)
       for (auto& [lc, cl] : gc) {
