@@ -191,8 +191,57 @@ __global__ __launch_bounds__(256) void eval_gto_kernel(const CShell* __restrict_
 // F_n(T) = e^{-T} sum_k (2T)^k / ((2n+1)(2n+3)...(2n+2k+1)) at n = nmax and the downward
 // recursion F_{n-1} = (2T F_n + e^{-T}) / (2n - 1); above, F_0 from erf and the upward recursion
 // F_{n+1} = ((2n+1) F_n - e^{-T}) / (2T) (stable there: T > nmax).
+// Boys function table F_m(T_i), T_i = i h (i < BOYS_NT, h = 1/20, so T_i <= 35), m < BOYS_M:
+// generated on the host by the series at F_{BOYS_M - 1} and downward recursion in long double.
+// The device takes F_nmax(T) from the nearest point by the Taylor series dF_m/dT = -F_{m+1},
+// eight terms (|T - T_i| <= h/2 and F_{m+8} <= F_m: remainder below 4e-18 of F_nmax), then
+// recurses downward as before.
+constexpr int BOYS_NT = 701, BOYS_M = 40;
+constexpr double BOYS_H = 0.05;
+__device__ double g_boys[BOYS_NT * BOYS_M];
+
+std::vector<double> boys_host() {
+  std::vector<double> t((size_t)BOYS_NT * BOYS_M);
+  const int top = BOYS_M - 1;
+  for (int i = 0; i < BOYS_NT; ++i) {
+    const long double T = (long double)(i * BOYS_H);  // the double the device subtracts
+    const long double et = expl(-T);
+    long double term = 1.0L / (2 * top + 1), sum = term;
+    for (int k = 1; k < 4000; ++k) {
+      term *= 2.0L * T / (2 * top + 2 * k + 1);
+      sum += term;
+      if (term < 1e-24L * sum) break;
+    }
+    long double F = et * sum;
+    t[(size_t)i * BOYS_M + top] = (double)F;
+    for (int n = top; n > 0; --n) {
+      F = (2.0L * T * F + et) / (2 * n - 1);
+      t[(size_t)i * BOYS_M + n - 1] = (double)F;
+    }
+  }
+  return t;
+}
+
+void upload_boys() {
+  static bool done = false;
+  if (done) return;
+  std::vector<double> b = boys_host();
+  HH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(g_boys), b.data(), sizeof(double) * b.size()));
+  done = true;
+}
+
 __device__ void boys(int nmax, double T, double* F) {
-  if (T <= 35.0) {
+  if (T <= 35.0 && nmax + 8 <= BOYS_M) {
+    const int i = (int)(T * 20.0 + 0.5);
+    const double x = i * BOYS_H - T;  // -(T - T_i)
+    const double* row = g_boys + i * BOYS_M + nmax;
+    double s = row[7];
+#pragma unroll
+    for (int k = 6; k >= 0; --k) s = row[k] + s * x * (1.0 / (k + 1));
+    const double et = exp(-T);
+    F[nmax] = s;
+    for (int n = nmax; n > 0; --n) F[n - 1] = (2.0 * T * F[n] + et) / (2 * n - 1);
+  } else if (T <= 35.0) {
     const double et = exp(-T);
     double term = 1.0 / (2 * nmax + 1), sum = term;
     for (int k = 1; k < 400; ++k) {
@@ -596,6 +645,7 @@ void launch_triples(const Cint* ma, const Cint* mb, const Cint* mc, const std::v
                     const DevBuf& d_csa, const DevBuf& d_prima, const DevBuf& d_csb, const DevBuf& d_primb,
                     const DevBuf& d_csc, const DevBuf& d_primc) {
   upload_c2s();
+  upload_boys();
   const int ncomp = kind_ncomp(kind);
   int na, nb, nc;
   kind_derivs(kind, &na, &nb, &nc);
