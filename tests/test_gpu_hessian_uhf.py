@@ -4,7 +4,7 @@ the summed densities, the UHF RI-JK skeleton, for UKS the grid component with th
 shift, and the unrestricted CPHF with the RI-JK and grid responses) against central finite
 differences of the unrestricted energy itself, on the water cation (doublet, 5 alpha / 4 beta);
 the reference's UKS tests use the same functional lists (tests/hessian_uks/test_svwn.rs:14-15,
-test_b3lyp.rs).  A closed shell run through the unrestricted path must give the restricted
+test_b3lyp.rs), and TPSS0 adds the spin-polarised meta-GGA.  A closed shell run through the unrestricted path must give the restricted
 Hessian.
 
 Tolerance: h = 2e-3 bohr on energies converged to 1e-13 Eh; analytic vs finite difference within
@@ -21,7 +21,8 @@ from tests.test_gpu_hessian_rhf import AUXB, BASIS, GEOM, NOCC, SYMS
 
 pytestmark = pytest.mark.gpu
 
-FUNCTIONALS = {"hf": None, "svwn": [(1.0, "LDA_X"), (1.0, "LDA_C_VWN")], "b3lyp": [(1.0, "HYB_GGA_XC_B3LYP")]}
+FUNCTIONALS = {"hf": None, "svwn": [(1.0, "LDA_X"), (1.0, "LDA_C_VWN")], "b3lyp": [(1.0, "HYB_GGA_XC_B3LYP")],
+               "tpss0": [(1.0, "HYB_MGGA_XC_TPSS0")]}
 Z = {"O": 8, "H": 1}
 NA, NB = NOCC, NOCC - 1
 
@@ -57,7 +58,7 @@ def square(de):
     return de.transpose(2, 0, 3, 1).reshape(3 * natm, 3 * natm)
 
 
-@pytest.mark.parametrize("name", ["hf", "svwn", "b3lyp"])
+@pytest.mark.parametrize("name", ["hf", "svwn", "b3lyp", "tpss0"])
 def test_uhf_hessian_against_finite_differences(dev, name):
     s = scf(GEOM, name)
     for e, n in zip(s.mo_energy, s.nocc):
