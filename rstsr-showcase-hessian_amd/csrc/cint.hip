@@ -296,40 +296,6 @@ struct TripleArgs {
   int offR, offE, offEc, offF, offI, offAcc, offSet, lds_doubles;
 };
 
-// derivative terms of libcint's ip on a Cartesian function: up to 4 (coef, shifted lx, ly, lz)
-__device__ __forceinline__ int deriv_terms(int lx, int ly, int lz, double a, int nd, int d1, int d2, double* cf,
-                                           int (*sh)[3]) {
-  int n = 1;
-  cf[0] = 1.0;
-  sh[0][0] = lx; sh[0][1] = ly; sh[0][2] = lz;
-  const int dirs[2] = {d1, d2};
-  for (int s = 0; s < nd; ++s) {
-    const int d = dirs[s];
-    double cf2[4];
-    int sh2[4][3];
-    int m = 0;
-    for (int k = 0; k < n; ++k) {
-      const int l = sh[k][d];
-      if (l > 0) {
-        cf2[m] = cf[k] * l;
-        sh2[m][0] = sh[k][0]; sh2[m][1] = sh[k][1]; sh2[m][2] = sh[k][2];
-        sh2[m][d] = l - 1;
-        ++m;
-      }
-      cf2[m] = cf[k] * (-2.0 * a);
-      sh2[m][0] = sh[k][0]; sh2[m][1] = sh[k][1]; sh2[m][2] = sh[k][2];
-      sh2[m][d] = l + 1;
-      ++m;
-    }
-    n = m;
-    for (int k = 0; k < n; ++k) {
-      cf[k] = cf2[k];
-      sh[k][0] = sh2[k][0]; sh[k][1] = sh2[k][1]; sh[k][2] = sh2[k][2];
-    }
-  }
-  return n;
-}
-
 // Index of (lx, ly, lz) in a shifted Cartesian set holding the l = lo, lo + 2, .. of one parity
 // (n derivatives move l by +-1 each, so only l0 + n, l0 + n - 2, .. >= 0 occur)
 __device__ __forceinline__ int set_index(int lo, int lx, int ly, int lz) {
@@ -339,7 +305,54 @@ __device__ __forceinline__ int set_index(int lo, int lx, int ly, int lz) {
   return k + cart_index(l, lx, lz);
 }
 
+// Derivative terms of libcint's ip operators on a Cartesian function, ND (0..2) compile-time:
+// up to 2^ND terms (coefficient, shifted powers) in registers; a lowering step on a zero power
+// leaves a term of coefficient 0 (skipped by the consumer).  Exponents are taken as 1: the
+// contracted shifted-set integrals carry their powers.
+template <int ND>
+struct DTerms {
+  double c[1 << ND];
+  int x[1 << ND], y[1 << ND], z[1 << ND];
+};
+template <int ND>
+__device__ __forceinline__ void dterms(int lx, int ly, int lz, int d1, int d2, DTerms<ND>& t) {
+  double c[4] = {1.0, 0.0, 0.0, 0.0};
+  int x[4] = {lx, 0, 0, 0}, y[4] = {ly, 0, 0, 0}, z[4] = {lz, 0, 0, 0};
+#pragma unroll
+  for (int s = 0; s < ND; ++s) {
+    const int d = s == 0 ? d1 : d2;
+    const int n = 1 << s;
+#pragma unroll
+    for (int k = n - 1; k >= 0; --k) {
+      const int l = d == 0 ? x[k] : (d == 1 ? y[k] : z[k]);
+      const int ex = d == 0, ey = d == 1, ez = d == 2;
+      c[2 * k + 1] = -2.0 * c[k];
+      x[2 * k + 1] = x[k] + ex; y[2 * k + 1] = y[k] + ey; z[2 * k + 1] = z[k] + ez;
+      c[2 * k] = c[k] * l;
+      x[2 * k] = x[k] - ex; y[2 * k] = y[k] - ey; z[2 * k] = z[k] - ez;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < (1 << ND); ++k) {
+    t.c[k] = c[k]; t.x[k] = x[k]; t.y[k] = y[k]; t.z[k] = z[k];
+  }
+}
+
+// Cartesian powers of component k (libcint order: lx descending, then ly descending) of l
+__device__ __forceinline__ void cart_comp(int l, int k, int& lx, int& ly, int& lz) {
+  int m = 0;
+  while ((m + 1) * (m + 2) / 2 <= k) ++m;
+  lx = l - m;
+  ly = m - (k - m * (m + 1) / 2);
+  lz = m - ly;
+}
+
+template <int KIND>
 __global__ __launch_bounds__(64) void int_triple_kernel(TripleArgs A) {
+  constexpr int NDA = KIND == HH_INT_IP1 || KIND == HH_INT_IPVIP1 || KIND == HH_INT_IP1IP2 ? 1
+                      : (KIND == HH_INT_IPIP1 ? 2 : 0);
+  constexpr int NDB = KIND == HH_INT_IPVIP1 ? 1 : 0;
+  constexpr int NDC = KIND == HH_INT_IP2 || KIND == HH_INT_IP1IP2 ? 1 : (KIND == HH_INT_IPIP2 ? 2 : 0);
   extern __shared__ double lds[];
   const int tid = threadIdx.x;
   const int ia_s = A.la_list[blockIdx.x % A.na_list];
@@ -520,8 +533,8 @@ __global__ __launch_bounds__(64) void int_triple_kernel(TripleArgs A) {
                     }
                 s += eab * hsum;
               }
-          const int ra = (ax + ay + az - A.la + A.nda) >> 1, rb = (bx + by + bz - A.lb + A.ndb) >> 1;
-          const int rc = (cx + cy + cz - A.lc + A.ndc) >> 1;
+          const int ra = (ax + ay + az - A.la + NDA) >> 1, rb = (bx + by + bz - A.lb + NDB) >> 1;
+          const int rc = (cx + cy + cz - A.lc + NDC) >> 1;
           I[e] += wgt * pwa[ra] * pwb[rb] * pwc[rc] * s;
         }
         __syncthreads();
@@ -532,46 +545,32 @@ __global__ __launch_bounds__(64) void int_triple_kernel(TripleArgs A) {
     const int q = A.q0 + e % nq;
     const int rest = e / nq;
     const int jc = rest % ncC, jb = (rest / ncC) % ncB, ja = rest / (ncC * ncB);
-    // Cartesian components of the output functions
-    int lxa = 0, lya = 0, lza = 0, lxb = 0, lyb = 0, lzb = 0, lxc = 0, lyc = 0, lzc = 0;
-    {
-      int k = 0;
-      for (int lx = A.la; lx >= 0; --lx)
-        for (int ly = A.la - lx; ly >= 0; --ly, ++k)
-          if (k == ja) { lxa = lx; lya = ly; lza = A.la - lx - ly; }
-      k = 0;
-      for (int lx = A.lb; lx >= 0; --lx)
-        for (int ly = A.lb - lx; ly >= 0; --ly, ++k)
-          if (k == jb) { lxb = lx; lyb = ly; lzb = A.lb - lx - ly; }
-      k = 0;
-      for (int lx = A.lc; lx >= 0; --lx)
-        for (int ly = A.lc - lx; ly >= 0; --ly, ++k)
-          if (k == jc) { lxc = lx; lyc = ly; lzc = A.lc - lx - ly; }
-    }
+    int lxa, lya, lza, lxb = 0, lyb = 0, lzb = 0, lxc, lyc, lzc;
+    cart_comp(A.la, ja, lxa, lya, lza);
+    if (!A.two_c) cart_comp(A.lb, jb, lxb, lyb, lzb);
+    cart_comp(A.lc, jc, lxc, lyc, lzc);
     // derivative directions per function: comp q of 3 (one derivative) or 9 = 3 t + s
-    int da1 = 0, da2 = 0, db1 = 0, dc1 = 0, dc2 = 0;
-    switch (A.kind) {
-      case HH_INT_IP1: da1 = q; break;
-      case HH_INT_IP2: dc1 = q; break;
-      case HH_INT_IPIP1: da1 = q / 3; da2 = q % 3; break;
-      case HH_INT_IPVIP1: da1 = q / 3; db1 = q % 3; break;
-      case HH_INT_IP1IP2: da1 = q / 3; dc1 = q % 3; break;
-      case HH_INT_IPIP2: dc1 = q / 3; dc2 = q % 3; break;
-      default: break;
-    }
-    double cfa[4], cfb[4], cfc[4];
-    int sha[4][3], shb[4][3], shc[4][3];
-    const int nta = deriv_terms(lxa, lya, lza, 1.0, A.nda, da1, da2, cfa, sha);
-    const int ntb = deriv_terms(lxb, lyb, lzb, 1.0, A.ndb, db1, 0, cfb, shb);
-    const int ntc = deriv_terms(lxc, lyc, lzc, 1.0, A.ndc, dc1, dc2, cfc, shc);
+    const int d1 = NDA + NDB + NDC == 1 ? q : q / 3, d2 = q % 3;
+    DTerms<NDA> ta;
+    DTerms<NDB> tb;
+    DTerms<NDC> tc;
+    dterms<NDA>(lxa, lya, lza, d1, d2, ta);
+    dterms<NDB>(lxb, lyb, lzb, d2, 0, tb);
+    dterms<NDC>(lxc, lyc, lzc, KIND == HH_INT_IP1IP2 ? d2 : d1, d2, tc);
     double v = 0.0;
-    for (int x = 0; x < nta; ++x) {
-      const int iA = set_index(A.loA, sha[x][0], sha[x][1], sha[x][2]);
-      for (int y = 0; y < ntb; ++y) {
-        const int iB = set_index(A.loB, shb[y][0], shb[y][1], shb[y][2]);
-        for (int z = 0; z < ntc; ++z) {
-          const int iC = set_index(A.loC, shc[z][0], shc[z][1], shc[z][2]);
-          v += cfa[x] * cfb[y] * cfc[z] * I[(iA * A.nB + iB) * A.nC + iC];
+#pragma unroll
+    for (int x = 0; x < (1 << NDA); ++x) {
+      if (ta.c[x] == 0.0) continue;
+      const int iA = set_index(A.loA, ta.x[x], ta.y[x], ta.z[x]);
+#pragma unroll
+      for (int y = 0; y < (1 << NDB); ++y) {
+        if (tb.c[y] == 0.0) continue;
+        const int iB = set_index(A.loB, tb.x[y], tb.y[y], tb.z[y]);
+#pragma unroll
+        for (int z = 0; z < (1 << NDC); ++z) {
+          if (tc.c[z] == 0.0) continue;
+          const int iC = set_index(A.loC, tc.x[z], tc.y[z], tc.z[z]);
+          v += ta.c[x] * tb.c[y] * tc.c[z] * I[(iA * A.nB + iB) * A.nC + iC];
         }
       }
     }
@@ -607,7 +606,7 @@ __global__ __launch_bounds__(64) void int_triple_kernel(TripleArgs A) {
     const int64_t cq = A.s2 * ((sc.ao0 + mc - A.c0) + A.s3 * (A.q0 + qq));
     A.out[(sa.ao0 + ma) + A.s1 * (ao_b + cq)] = v;
     if (A.sym_ab && ia_s != ib_s) {
-      const int q = A.q0 + qq, qm = A.kind == HH_INT_IPVIP1 ? 3 * (q % 3) + q / 3 : q;
+      const int q = A.q0 + qq, qm = KIND == HH_INT_IPVIP1 ? 3 * (q % 3) + q / 3 : q;
       A.out[ao_b + A.s1 * ((sa.ao0 + ma) + A.s2 * ((sc.ao0 + mc - A.c0) + A.s3 * qm))] = v;
     }
   }
@@ -666,7 +665,11 @@ void launch_triples(const Cint* ma, const Cint* mb, const Cint* mc, const std::v
     int dev = 0;
     HH_HIP(hipGetDevice(&dev));
     HH_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    HH_HIP(hipFuncSetAttribute((const void*)int_triple_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+    const void* fns[] = {(const void*)int_triple_kernel<HH_INT_PLAIN>, (const void*)int_triple_kernel<HH_INT_IP1>,
+                         (const void*)int_triple_kernel<HH_INT_IP2>, (const void*)int_triple_kernel<HH_INT_IPIP1>,
+                         (const void*)int_triple_kernel<HH_INT_IPVIP1>, (const void*)int_triple_kernel<HH_INT_IP1IP2>,
+                         (const void*)int_triple_kernel<HH_INT_IPIP2>};
+    for (const void* f : fns) HH_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
   }
   std::vector<DevBuf*> keep;
   const bool sym = !two_c && ma == mb && alist_all == blist_all && ((na == 0 && nb == 0) || kind == HH_INT_IPVIP1);
@@ -728,8 +731,17 @@ void launch_triples(const Cint* ma, const Cint* mb, const Cint* mc, const std::v
           A.q1 = std::min(ncomp, q0 + qchunk);
           A.offAcc = fixed;
           A.lds_doubles = fixed + ncarts * (A.q1 - A.q0);
-          hipLaunchKernelGGL(int_triple_kernel, dim3((unsigned)nab, (unsigned)cl.size()), dim3(64),
-                             sizeof(double) * A.lds_doubles, s, A);
+          const dim3 grid((unsigned)nab, (unsigned)cl.size());
+          const size_t lds = sizeof(double) * A.lds_doubles;
+          switch (kind) {
+            case HH_INT_IP1: hipLaunchKernelGGL(int_triple_kernel<HH_INT_IP1>, grid, dim3(64), lds, s, A); break;
+            case HH_INT_IP2: hipLaunchKernelGGL(int_triple_kernel<HH_INT_IP2>, grid, dim3(64), lds, s, A); break;
+            case HH_INT_IPIP1: hipLaunchKernelGGL(int_triple_kernel<HH_INT_IPIP1>, grid, dim3(64), lds, s, A); break;
+            case HH_INT_IPVIP1: hipLaunchKernelGGL(int_triple_kernel<HH_INT_IPVIP1>, grid, dim3(64), lds, s, A); break;
+            case HH_INT_IP1IP2: hipLaunchKernelGGL(int_triple_kernel<HH_INT_IP1IP2>, grid, dim3(64), lds, s, A); break;
+            case HH_INT_IPIP2: hipLaunchKernelGGL(int_triple_kernel<HH_INT_IPIP2>, grid, dim3(64), lds, s, A); break;
+            default: hipLaunchKernelGGL(int_triple_kernel<HH_INT_PLAIN>, grid, dim3(64), lds, s, A);
+          }
           HH_HIP(hipGetLastError());
         }
       }
